@@ -1,0 +1,230 @@
+"""Parameter tree of the FiRE hot path: random initialisation, packing for the C ABI.
+
+The tree mirrors the reference's ``MoonLikeParams(embedding, to_orbitals, jastrow)``
+(src/sparse_wf/model/wave_function.py:54-57) with ``EmbeddingParams``
+(new_sparse_model.py:218-225) and flax-style sub-trees for the modules.  Module names
+follow flax's naming rules applied to the reference source (``beta_ee``
+new_sparse_model.py:176, ``to_orbitals_up``/``env_up`` orbitals.py:53-81, ``Linear_i`` of
+``SparseMLP`` sparse_fwd_lap.py:222, ...).
+
+Initialisers restate the reference families with NumPy's PCG64:
+
+* ``lecun_normal`` (model/utils.py:339-342): truncated normal on [-1, 1] times 1/sqrt(shape[0]);
+* flax ``Dense``/``Linear`` kernels: flax ``lecun_normal`` (variance-scaling, truncated at 2 sigma);
+* ``scale_initializer`` (utils.py:378-383), ``envelope_sigma_initializer`` (envelopes.py:28-32),
+  isotropic-envelope ``pi`` ~ N(0, 1/n_envelopes) (envelopes.py:69-74);
+* nucleus-dependent e-n filter params (new_sparse_model.py:382-397).
+
+The normalisation scales ``embedding.scales = (s0, s1)`` are set from a forward pass as
+``1/std`` (new_sparse_model.py:378, utils.py:461-468); see :func:`set_scales`.
+"""
+from __future__ import annotations
+
+from typing import Callable
+
+import numpy as np
+
+from .config import FireConfig
+
+_FLAX_TRUNC = 0.87962566103423978  # std of a unit normal truncated to [-2, 2]
+
+
+def _truncnorm(rng, lo, hi, shape):
+    out = rng.standard_normal(size=shape)
+    bad = (out < lo) | (out > hi)
+    while bad.any():
+        out[bad] = rng.standard_normal(size=int(bad.sum()))
+        bad = (out < lo) | (out > hi)
+    return out
+
+
+def lecun_normal_ref(rng, shape):
+    """model/utils.py:339-342."""
+    return (_truncnorm(rng, -1.0, 1.0, shape) / np.sqrt(shape[0])).astype(np.float32)
+
+
+def flax_lecun_normal(rng, shape):
+    fan_in = int(np.prod(shape[:-1]))
+    std = np.sqrt(1.0 / fan_in) / _FLAX_TRUNC
+    return (_truncnorm(rng, -2.0, 2.0, shape) * std).astype(np.float32)
+
+
+def scale_initializer(rng, cutoff, shape):
+    """model/utils.py:378-383."""
+    n = shape[-1]
+    max_len = min(20.0, 0.75 * cutoff)
+    scale = np.linspace(0.5 * max_len, max_len, n)
+    scale = scale * (1 + 0.1 * rng.standard_normal(size=shape))
+    return scale.astype(np.float32)
+
+
+def envelope_sigma_initializer(rng, shape, smin=1.0, smax=5.0):
+    """envelopes.py:28-32 (the stored value is the softplus pre-image)."""
+    scale = np.geomspace(smin, smax, shape[-1])
+    # jax.random.truncated_normal(key, 0.8, 1.2): a unit normal truncated to [0.8, 1.2]
+    scale = scale * _truncnorm(rng, 0.8, 1.2, shape)
+    return np.log(np.expm1(scale)).astype(np.float32)
+
+
+def _dense(rng, fan_in, fan_out, bias=True, bias_init=None):
+    p = {"kernel": flax_lecun_normal(rng, (fan_in, fan_out))}
+    if bias:
+        p["bias"] = np.zeros(fan_out, np.float32) if bias_init is None else bias_init(rng, (fan_out,))
+    return p
+
+
+def _pairwise_filter_dense(rng, n_env, w0, w1):
+    return {"Dense_0": _dense(rng, w0, w1, True), "Dense_1": _dense(rng, n_env, w1, False)}
+
+
+def init_params(cfg: FireConfig, seed: int = 0) -> dict:
+    """Random parameters in the reference layout; ``embedding.scales`` = (1, 1) until
+    :func:`set_scales` is applied."""
+    rng = np.random.default_rng(seed)
+    F = cfg.feature_dim
+    w0, w1 = cfg.pair_mlp_widths
+    M = cfg.n_nuc
+    E_en = cfg.pair_n_envelopes
+    N, D, E = cfg.n_el, cfg.n_determinants, cfg.n_envelopes
+
+    dyn = {
+        "filter": {
+            "scales": scale_initializer(rng, cfg.cutoff, (M, E_en)),
+            "kernel": lecun_normal_ref(rng, (M, 4, w0)),
+            "bias": (rng.standard_normal(size=(M, w0)) * 2.0).astype(np.float32),
+        },
+        "nuc_embedding": rng.standard_normal(size=(M, F)).astype(np.float32),
+    }
+    elec_init = {"params": {
+        "ElecNucEdge_0": {
+            "PairwiseFilter_0": _pairwise_filter_dense(rng, E_en, w0, w1),
+            "Dense_0": _dense(rng, w1, F, False),
+            "Dense_1": _dense(rng, 4, F, False),
+        },
+        "LayerNorm_0": {"scale": np.ones(F, np.float32), "bias": np.zeros(F, np.float32)},
+        "GatedLinearUnit_0": {"Dense_0": _dense(rng, F, 2 * F, False)},
+        "Dense_0": _dense(rng, F, F, True),
+        "Dense_1": _dense(rng, F, 5 * F, True),
+    }}
+
+    def edge():
+        return {"params": {
+            "beta_ee": {
+                "scale": scale_initializer(rng, cfg.cutoff, (w0,)),
+                "kernel": lecun_normal_ref(rng, (4, w0)),
+                "bias": rng.standard_normal(size=(w0,)).astype(np.float32),
+                **_pairwise_filter_dense(rng, w0, w0, w1),
+            },
+            "Dense_0": _dense(rng, w1, F, False),
+            "Dense_1": _dense(rng, 4, F, True),
+        }}
+
+    embedding = {
+        "dynamic_params_en": dyn,
+        "elec_init": elec_init,
+        "edge_same": edge(),
+        "edge_diff": edge(),
+        "updates": [{"params": {"Linear_0": _dense(rng, F, F, True), "Linear_1": _dense(rng, F, F, True)}}],
+        "scales": [np.float32(1.0), np.float32(1.0)],
+        "cutoff_param": np.float32(10.0),
+    }
+
+    def trunc_bias(r, shape):
+        return (_truncnorm(r, -2.0, 2.0, shape) * 0.01).astype(np.float32)
+
+    orb = {}
+    for s in ("up", "dn"):
+        if s == "dn" and cfg.spin_restricted:
+            continue
+        orb[f"to_orbitals_{s}"] = _dense(rng, F, N * D, True, bias_init=trunc_bias)
+        orb[f"env_{s}"] = {
+            "sigma": envelope_sigma_initializer(rng, (M, D, E)),
+            "pi": (rng.standard_normal(size=(M, D, E, N)) / np.sqrt(E)).astype(np.float32),
+        }
+    to_orbitals = {"params": orb}
+
+    jp = {}
+    if cfg.e_e_cusps == "psiformer":
+        jp["pairwise_cusps"] = {
+            "alpha_same": np.float32(1.0), "alpha_diff": np.float32(1.0),
+            "factor_same": np.float32(0.0), "factor_diff": np.float32(0.0),
+        }
+    W = cfg.mlp_width
+    if cfg.use_mlp_jastrow or cfg.use_log_jastrow:
+        widths = [W] * cfg.mlp_depth + [2]
+        mlp, fan = {}, F
+        for i, w in enumerate(widths):
+            mlp[f"Linear_{i}"] = _dense(rng, fan, w, True)
+            fan = w
+        jp["mlp"] = mlp
+        jp["mlp_scale"] = np.zeros(2, np.float32)
+        if cfg.use_log_jastrow:
+            jp["log_bias"] = np.float32(1.0)
+    if cfg.use_attention:
+        H, Dk = cfg.attention_heads, cfg.attention_dim
+        out, fan = {}, 2 * H * Dk
+        for i, w in enumerate([W] * cfg.mlp_depth + [2]):
+            out[f"Dense_{i}"] = _dense(rng, fan, w, True)
+            fan = w
+        jp["att"] = {
+            "register_keys": rng.standard_normal(size=(H, Dk)).astype(np.float32),
+            "Q_W": _dense(rng, F, H * Dk, True),
+            "V_W": _dense(rng, F, H * Dk, True),
+            "out": out,
+            "scale": np.zeros(2, np.float32),
+            "bias": np.ones(1, np.float32),
+        }
+    jastrow = {"params": jp}
+    return {"embedding": embedding, "to_orbitals": to_orbitals, "jastrow": jastrow}
+
+
+def exercise_jastrow(params: dict) -> dict:
+    """Make the Jastrow non-trivial, as the reference test fixture does by setting every
+    ``jastrow ... scale`` leaf to one (tests/utils.py:73-78); the e-e cusp factors are set
+    to the cusp-condition values noted in jastrow.py:59."""
+    jp = params["jastrow"]["params"]
+    if "mlp_scale" in jp:
+        jp["mlp_scale"] = np.ones_like(jp["mlp_scale"])
+    if "att" in jp:
+        jp["att"]["scale"] = np.ones_like(jp["att"]["scale"])
+    if "pairwise_cusps" in jp:
+        jp["pairwise_cusps"]["factor_same"] = np.float32(-0.25)
+        jp["pairwise_cusps"]["factor_diff"] = np.float32(-0.5)
+    return params
+
+
+def set_scales(params: dict, embed_stages: Callable) -> dict:
+    """``embed_stages(params) -> (h_contracted, h_mlp_unscaled_input_fn)``.
+
+    Restates ``normalize(h, None)`` (utils.py:461-468) as used at init
+    (new_sparse_model.py:378): s0 = 1/std(h after message passing); s1 = 1/std(MLP(s0 h)).
+    ``embed_stages`` must return the two arrays in that order for the given params."""
+    params["embedding"]["scales"] = [np.float32(1.0), np.float32(1.0)]
+    h_contr, _ = embed_stages(params)
+    s0 = 1.0 / np.std(np.asarray(h_contr, np.float64))
+    s0 = np.float32(s0 if np.isfinite(s0) else 1.0)
+    params["embedding"]["scales"] = [s0, np.float32(1.0)]
+    _, h_mlp = embed_stages(params)
+    s1 = 1.0 / np.std(np.asarray(h_mlp, np.float64))
+    s1 = np.float32(s1 if np.isfinite(s1) else 1.0)
+    params["embedding"]["scales"] = [s0, s1]
+    return params
+
+
+def tree_map(fn, tree):
+    if isinstance(tree, dict):
+        return {k: tree_map(fn, v) for k, v in tree.items()}
+    if isinstance(tree, (list, tuple)):
+        return type(tree)(tree_map(fn, v) for v in tree)
+    return fn(tree)
+
+
+def leaves_with_path(tree, prefix=""):
+    if isinstance(tree, dict):
+        for k in tree:
+            yield from leaves_with_path(tree[k], f"{prefix}/{k}")
+    elif isinstance(tree, (list, tuple)):
+        for i, v in enumerate(tree):
+            yield from leaves_with_path(v, f"{prefix}/{i}")
+    else:
+        yield prefix, tree
