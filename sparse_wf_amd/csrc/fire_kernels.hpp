@@ -1,0 +1,58 @@
+// Kernel launchers shared between the kernel translation units and the host API.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fire_common.hpp"
+
+namespace fire {
+
+// Row layout of a family of tangent-row groups: group g starts at row a*g + b*ptr[g]
+// (ptr = CSR pair offsets; b = 0 for uniform groups) and has
+// row_start(g+1) - row_start(g) rows; row 0 = value, row 1 = Laplacian, rows >= 2 jacobian.
+struct GroupDesc {
+  int a, b;
+  const int64_t* ptr;
+  __host__ __device__ __forceinline__ int64_t row(int64_t g) const { return a * g + (b ? b * ptr[g] : 0); }
+};
+
+struct EnParams {  // e-n edges of ElecInit (new_sparse_model.py:139-158, 382-397)
+  const float *K, *b, *sp, *hnuc;  // [M,4,w0], [M,w0], [M,E], [M,F]
+  const float *Wd, *bd, *We;       // [w0,w1], [w1], [E,w1]
+  const float *WG, *WF;            // [w1,F], [4,F]
+  const float *ln_g, *ln_b;        // [F]
+  int w0, w1, E;
+};
+
+struct EdgeParams {  // ElecElecEdges (new_sparse_model.py:161-187)
+  FilterParams filt;
+  const float *WG, *WF, *BF;  // [w1,F], [4,F], [F]
+};
+
+enum Act { ACT_SILU = 0, ACT_EXP_NEG_SCALED = 1 };
+
+// graph
+void launch_nbr_count(hipStream_t s, const float* r, int B, int N, int n_up, float cutoff, int32_t* cnt);
+void launch_nbr_fill(hipStream_t s, const float* r, int B, int N, int n_up, float cutoff, const int64_t* ptr,
+                     int32_t* nbr);
+void launch_scan(hipStream_t s, const int32_t* cnt, int G, int64_t* ptr);
+void launch_rev(hipStream_t s, const int64_t* ptr, const int32_t* nbr, int B, int N, int n_up, int32_t* rev);
+
+// embedding
+void launch_elec_init_en(hipStream_t s, const float* r, const float* R, int B, int N, int n_up, int M, float cutoff_1el,
+                         int F, EnParams P, bool tangents, float* out);
+void launch_glu(hipStream_t s, const float* Z, int G, int F, bool tangents, float* out);
+void launch_act(hipStream_t s, float* Y, int64_t ld, int ncols, int G, GroupDesc gd, int act, float act_scale);
+void launch_messages(hipStream_t s, const float* r, int B, int N, int n_up, const int64_t* ptr, const int32_t* nbr,
+                     const float* M5, EdgeParams same, EdgeParams diff, float cut, int F, bool tangents, float* out);
+void launch_scale(hipStream_t s, float* p, int64_t n, float a);
+void launch_rowflag(hipStream_t s, int G, GroupDesc gd, int64_t n_rows, uint8_t* flag);
+
+// dense contractions: C[M,N] = A[M,K] W[K,N] (+ bias on rows with flag / row % period == 0)
+void launch_gemm_f32(hipStream_t s, const float* A, int64_t lda, const float* W, const float* bias, float* C,
+                     int64_t ldc, int64_t M, int N, int K, const uint8_t* rowflag, int period);
+void launch_gemm_bf16(hipStream_t s, const float* A, int64_t lda, const uint16_t* Wt /*[N,K] bf16*/,
+                      const float* bias, float* C, int64_t ldc, int64_t M, int N, int K, const uint8_t* rowflag,
+                      int period);
+
+}  // namespace fire

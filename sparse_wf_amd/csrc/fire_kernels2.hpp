@@ -1,0 +1,56 @@
+// Launchers of the orbital / determinant / Jastrow / energy kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fire_kernels.hpp"
+
+namespace fire {
+
+// orbitals and determinants
+void launch_envelopes(hipStream_t s, const float* r, const float* R, int B, int N, int n_up, int M, int D, int E,
+                      const float* sp_up, const float* pi_up, const float* sp_dn, const float* pi_dn, bool tangents,
+                      float* out);
+void launch_orb_product(hipStream_t s, float* O, int ND, int G, GroupDesc gd, const float* env, bool tangents);
+void launch_gather_A(hipStream_t s, const float* O, int ND, GroupDesc gd, int B, int N, int n_up, int D, double* A);
+void launch_lu_inv(hipStream_t s, double* A, int n_mat, int N, double* Ainv, double* logdet, double* sign,
+                   bool want_inv);
+void launch_det_trace(hipStream_t s, const float* O, int ND, GroupDesc gd, int B, int N, int n_up, int D,
+                      const double* Ainv, double* tr);
+size_t det_lap_smem(int smax);
+hipError_t det_lap_set_smem_limit(size_t bytes);
+void launch_det_lap(hipStream_t s, const float* O, int ND, GroupDesc gd, const int64_t* ptr, const int32_t* nbr,
+                    const int32_t* rev, int B, int N, int n_up, int D, int smax, const double* Ainv, double* trip,
+                    double* grad);
+void launch_det_combine(hipStream_t s, int B, int N, int D, const double* logdet, const double* sgn, const double* tr,
+                        const double* trip, const double* grad, double* out_logpsi, double* out_sign,
+                        double* out_grad, double* out_lap, bool tangents);
+
+// Jastrow
+void launch_jmlp_last(hipStream_t s, const float* Y, int W, int G, GroupDesc gd, const float* W3, const float* b3,
+                      const float* scale, const float* zb, float* JR);
+void launch_node_sums(hipStream_t s, const float* X, int64_t ld, int ncols, GroupDesc gd, int B, int N, int n_up,
+                      double* out /*[B][2 spin][2 (value, lap)][ncols]*/);
+void launch_dep_gather(hipStream_t s, const float* X, int64_t ld, int ncols, GroupDesc gd, const int64_t* ptr,
+                       const int32_t* nbr, const int32_t* rev, int B, int N, int n_up,
+                       double* out /*[B][2 spin][N][3][ncols]*/);
+void launch_att_group(hipStream_t s, const float* QV, int H, int Dk, int G, GroupDesc gd, const float* keys,
+                      bool tangents, float* AV);
+void launch_att_readin(hipStream_t s, const double* NS, const double* DG, int B, int N, int H, int Dk, bool tangents,
+                       float* RIN);
+void launch_cusp_coulomb(hipStream_t s, const float* r, const float* R, const float* Z, int B, int N, int n_up, int M,
+                         int cusp, const float* cusp_params, double* cval, double* clap, double* cgrad, double* epot);
+
+struct AssembleArgs {
+  int B, N, use_mlp, use_log, use_att, cusp, tangents;
+  const double *det_logpsi, *det_sign, *det_grad, *det_lap;
+  const double *mlp_ns, *mlp_dg;  // node sums [B][2][2][2], dep gather [B][2][N][3][2]
+  const float* att_out;           // [B][2+3N][2] (or [B][1][2] without tangents)
+  const float *att_scale, *att_zb;
+  const double *cval, *clap, *cgrad, *epot;
+  double e_nn;
+  double *e_loc, *e_kin, *e_pot, *logpsi, *sign, *grad, *lap;
+};
+void launch_assemble(hipStream_t s, const AssembleArgs& a);
+
+}  // namespace fire

@@ -1,0 +1,342 @@
+// FiRE embedding kernels: ElecInit e-n edges + LayerNorm, GLU, activations, e-e messages.
+//
+// References (sparse_wf): ElecInit new_sparse_model.py:190-215, ElecNucEdge :139-158,
+// ElecElecEdges :161-187, contract_with_fwd_lap :95-131, ElementWise.apply_with_fwd_lap
+// sparse_fwd_lap.py:184-206, GatedLinearUnit model/utils.py:55-62, flax LayerNorm (eps 1e-6).
+//
+// Every kernel here works on tangent-row groups (fire_common.hpp).  With `tangents`
+// false only the value row is produced (groups of one row): the value path used by
+// log|psi| and the MCMC sweep.
+#include "fire_common.hpp"
+#include "fire_kernels.hpp"
+
+namespace fire {
+
+#define EN_CHUNK 64
+
+// One workgroup (blockDim = F) per electron: sum over nuclei within cutoff_1el of
+// Gamma_J * e_J with forward-Laplacian rows, then LayerNorm with forward-Laplacian.
+template <bool TAN>
+__global__ void __launch_bounds__(1024) k_elec_init_en(const float* __restrict__ r, const float* __restrict__ R, int B,
+                                                       int N, int n_up, int M, float cutoff_1el, int F, EnParams P,
+                                                       float* __restrict__ out) {
+  __shared__ float s_beta[EN_CHUNK][5][FIRE_W1MAX];
+  __shared__ float s_g[EN_CHUNK][5][4];
+  __shared__ int s_ok[EN_CHUNK];
+  __shared__ float s_red[16 * 8];
+  const int g = blockIdx.x;
+  int w, i;
+  group_inv(g, B, n_up, N - n_up, w, i);
+  const float* ri = r + ((size_t)w * N + i) * 3;
+  const float rx = ri[0], ry = ri[1], rz = ri[2];
+  const int f = threadIdx.x;
+  float acc[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+  float wg[FIRE_W1MAX], wf[4];
+  for (int k = 0; k < P.w1; ++k) wg[k] = P.WG[k * F + f];
+  for (int m = 0; m < 4; ++m) wf[m] = P.WF[m * F + f];
+
+  for (int J0 = 0; J0 < M; J0 += EN_CHUNK) {
+    const int J = J0 + threadIdx.x;
+    if (threadIdx.x < EN_CHUNK) s_ok[threadIdx.x] = 0;
+    if (threadIdx.x < EN_CHUNK && J < M) {
+      const float diff[3] = {__fsub_rn(rx, R[3 * J]), __fsub_rn(ry, R[3 * J + 1]), __fsub_rn(rz, R[3 * J + 2])};
+      const float s2 = __fadd_rn(__fadd_rn(__fmul_rn(diff[0], diff[0]), __fmul_rn(diff[1], diff[1])),
+                                 __fmul_rn(diff[2], diff[2]));
+      if (__fsqrt_rn(s2) < cutoff_1el) {
+        FilterParams fp;
+        fp.K = P.K + (size_t)J * 4 * P.w0;
+        fp.b = P.b + (size_t)J * P.w0;
+        fp.sp = P.sp + (size_t)J * P.E;
+        fp.Wd = P.Wd;
+        fp.bd = P.bd;
+        fp.We = P.We;
+        fp.w0 = P.w0;
+        fp.w1 = P.w1;
+        fp.E = P.E;
+        float beta[5][FIRE_W1MAX], gg[5][4];
+        pair_filter_fl(diff, fp, cutoff_1el, beta, gg);
+        const int slot = threadIdx.x;
+        s_ok[slot] = 1;
+        for (int rr = 0; rr < 5; ++rr) {
+          for (int k = 0; k < P.w1; ++k) s_beta[slot][rr][k] = beta[rr][k];
+          for (int m = 0; m < 4; ++m) s_g[slot][rr][m] = gg[rr][m];
+        }
+      }
+    }
+    __syncthreads();
+    // nuclei in ascending index order (the order of the reference's e-n list)
+    const int nv = min(EN_CHUNK, M - J0);
+    if (f < F) {
+      for (int t = 0; t < nv; ++t) {
+        if (!s_ok[t]) continue;
+        const int bestJ = J0 + t;
+        float G5[5], E5[5];
+#pragma unroll
+        for (int rr = 0; rr < 5; ++rr) {
+          float a = 0.f;
+          for (int k = 0; k < P.w1; ++k) a += s_beta[t][rr][k] * wg[k];
+          G5[rr] = a;
+          float b = 0.f;
+#pragma unroll
+          for (int m = 0; m < 4; ++m) b += s_g[t][rr][m] * wf[m];
+          E5[rr] = b;
+        }
+        E5[0] += P.hnuc[(size_t)bestJ * F + f];
+        acc[0] += G5[0] * E5[0];
+        if (TAN) {
+          acc[1] += G5[1] * E5[0] + G5[0] * E5[1] + 2.f * (G5[2] * E5[2] + G5[3] * E5[3] + G5[4] * E5[4]);
+#pragma unroll
+          for (int t3 = 0; t3 < 3; ++t3) acc[2 + t3] += G5[2 + t3] * E5[0] + G5[0] * E5[2 + t3];
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // LayerNorm with forward-Laplacian (mean/variance over the F features)
+  const float invF = 1.0f / (float)F;
+  float m5[5] = {acc[0], acc[1], acc[2], acc[3], acc[4]};
+  block_sum<5>(m5, s_red);
+  const float c = acc[0] - m5[0] * invF;
+  const float cl = acc[1] - m5[1] * invF;
+  const float ct[3] = {acc[2] - m5[2] * invF, acc[3] - m5[3] * invF, acc[4] - m5[4] * invF};
+  float v6[6] = {c * c, c * ct[0], c * ct[1], c * ct[2], ct[0] * ct[0] + ct[1] * ct[1] + ct[2] * ct[2], c * cl};
+  block_sum<6>(v6, s_red);
+  const float var = v6[0] * invF;
+  const float vt[3] = {2.f * v6[1] * invF, 2.f * v6[2] * invF, 2.f * v6[3] * invF};
+  const float vl = 2.f * (v6[4] + v6[5]) * invF;
+  const float ve = var + 1e-6f;
+  const float sq = sqrtf(ve);
+  const float s = 1.0f / sq;
+  const float s32 = s / ve;        // ve^-3/2
+  const float s52 = s32 / ve;      // ve^-5/2
+  const float st[3] = {-0.5f * s32 * vt[0], -0.5f * s32 * vt[1], -0.5f * s32 * vt[2]};
+  const float sl = 0.75f * s52 * (vt[0] * vt[0] + vt[1] * vt[1] + vt[2] * vt[2]) - 0.5f * s32 * vl;
+  const float gam = P.ln_g[f], bet = P.ln_b[f];
+  if (TAN) {
+    float* o = out + (size_t)g * 5 * F + f;
+    o[0] = gam * c * s + bet;
+    o[F] = gam * (cl * s + 2.f * (ct[0] * st[0] + ct[1] * st[1] + ct[2] * st[2]) + c * sl);
+    for (int t3 = 0; t3 < 3; ++t3) o[(2 + t3) * F] = gam * (ct[t3] * s + c * st[t3]);
+  } else {
+    out[(size_t)g * F + f] = gam * c * s + bet;
+  }
+}
+
+void launch_elec_init_en(hipStream_t s, const float* r, const float* R, int B, int N, int n_up, int M, float cutoff_1el,
+                         int F, EnParams P, bool tangents, float* out) {
+  const int G = B * N;
+  if (tangents)
+    k_elec_init_en<true><<<G, F, 0, s>>>(r, R, B, N, n_up, M, cutoff_1el, F, P, out);
+  else
+    k_elec_init_en<false><<<G, F, 0, s>>>(r, R, B, N, n_up, M, cutoff_1el, F, P, out);
+}
+
+// GatedLinearUnit product y = a * silu(gate) on 5-row groups; Z = [a | gate] per row.
+template <bool TAN>
+__global__ void k_glu(const float* __restrict__ Z, int G, int F, float* __restrict__ out) {
+  const int g = blockIdx.x;
+  const int f = blockIdx.y * blockDim.x + threadIdx.x;
+  if (f >= F) return;
+  const int nr = TAN ? 5 : 1;
+  const float* z = Z + (size_t)g * nr * 2 * F;
+  float* o = out + (size_t)g * nr * F;
+  const float ax = z[f], gx = z[F + f];
+  float sf, s1, s2;
+  silu3(gx, sf, s1, s2);
+  o[f] = ax * sf;
+  if (TAN) {
+    const float al = z[2 * F + f], gl = z[3 * F + f];
+    float gt[3], at[3], sum2 = 0.f, cross = 0.f;
+    for (int t = 0; t < 3; ++t) {
+      at[t] = z[(2 + t) * 2 * F + f];
+      gt[t] = z[(2 + t) * 2 * F + F + f];
+      sum2 += gt[t] * gt[t];
+    }
+    const float sl = s1 * gl + s2 * sum2;
+    for (int t = 0; t < 3; ++t) {
+      const float stt = s1 * gt[t];
+      o[(2 + t) * F + f] = at[t] * sf + ax * stt;
+      cross += at[t] * stt;
+    }
+    o[F + f] = al * sf + ax * sl + 2.f * cross;
+  }
+}
+
+void launch_glu(hipStream_t s, const float* Z, int G, int F, bool tangents, float* out) {
+  dim3 grid(G, (F + 255) / 256);
+  if (tangents)
+    k_glu<true><<<grid, 256, 0, s>>>(Z, G, F, out);
+  else
+    k_glu<false><<<grid, 256, 0, s>>>(Z, G, F, out);
+}
+
+// In-place elementwise activation with the forward-Laplacian rule on every group.
+__global__ void k_act(float* __restrict__ Y, int64_t ld, int ncols, GroupDesc gd, int act, float sc) {
+  const int g = blockIdx.x;
+  const int c = blockIdx.y * blockDim.x + threadIdx.x;
+  if (c >= ncols) return;
+  const int64_t r0 = gd.row(g), r1 = gd.row(g + 1);
+  float* y = Y + r0 * ld + c;
+  const float x = y[0];
+  float f, f1, f2;
+  if (act == ACT_SILU) {
+    silu3(x, f, f1, f2);
+  } else {  // exp(-sc * x)
+    f = expf(-sc * x);
+    f1 = -sc * f;
+    f2 = sc * sc * f;
+  }
+  y[0] = f;
+  const int64_t nr = r1 - r0;
+  if (nr == 1) return;
+  float ss = 0.f;
+  for (int64_t k = 2; k < nr; ++k) {
+    const float v = y[k * ld];
+    ss += v * v;
+    y[k * ld] = v * f1;
+  }
+  y[ld] = y[ld] * f1 + f2 * ss;
+}
+
+void launch_act(hipStream_t s, float* Y, int64_t ld, int ncols, int G, GroupDesc gd, int act, float act_scale) {
+  if (G == 0) return;
+  dim3 grid(G, (ncols + 255) / 256);
+  k_act<<<grid, 256, 0, s>>>(Y, ld, ncols, gd, act, act_scale);
+}
+
+__global__ void k_rowflag(int G, GroupDesc gd, uint8_t* flag) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < G) flag[gd.row(g)] = 1;
+}
+
+__global__ void k_zero_u8(uint8_t* p, int64_t n) {
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) p[k] = 0;
+}
+
+void launch_rowflag(hipStream_t s, int G, GroupDesc gd, int64_t n_rows, uint8_t* flag) {
+  k_zero_u8<<<1024, 256, 0, s>>>(flag, n_rows);
+  k_rowflag<<<(G + 255) / 256, 256, 0, s>>>(G, gd, flag);
+}
+
+__global__ void k_scale(float* p, int64_t n, float a) {
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) p[k] *= a;
+}
+
+void launch_scale(hipStream_t s, float* p, int64_t n, float a) { k_scale<<<1024, 256, 0, s>>>(p, n, a); }
+
+#define MSG_CHUNK 64
+
+// Message passing h_i = h0_i + sum_j Gamma_ij * silu(e_ij + m_ct[i] + m_nb[j]) with
+// forward-Laplacian rows; one workgroup (blockDim = F) per centre electron.  The pair
+// filter quantities of up to MSG_CHUNK neighbours are computed by one thread each and
+// shared through LDS; each thread then owns one feature column.
+template <bool TAN>
+__global__ void __launch_bounds__(1024) k_messages(const float* __restrict__ r, int B, int N, int n_up,
+                                                   const int64_t* __restrict__ ptr, const int32_t* __restrict__ nbr,
+                                                   const float* __restrict__ M5, EdgeParams Ps, EdgeParams Pd,
+                                                   float cut, int F, float* __restrict__ out) {
+  __shared__ float s_beta[MSG_CHUNK][5][FIRE_W1MAX];
+  __shared__ float s_g[MSG_CHUNK][5][4];
+  __shared__ int s_gj[MSG_CHUNK];
+  __shared__ int s_same[MSG_CHUNK];
+  const int g = blockIdx.x;
+  const int n_dn = N - n_up;
+  int w, i;
+  group_inv(g, B, n_up, n_dn, w, i);
+  const int f = threadIdx.x;
+  const int NR = TAN ? 5 : 1;
+  const int64_t p0 = ptr[g];
+  const int nb = (int)(ptr[g + 1] - p0);
+  const int64_t row0 = TAN ? (5 * (int64_t)g + 3 * p0) : g;
+  const float* rw = r + (size_t)w * N * 3;
+  const bool up_i = i < n_up;
+
+  // weights of this feature column for same- and opposite-spin edges
+  const int w1 = Ps.filt.w1;
+  float wgs[FIRE_W1MAX], wgd[FIRE_W1MAX], wfs[4], wfd[4];
+  for (int k = 0; k < w1; ++k) { wgs[k] = Ps.WG[k * F + f]; wgd[k] = Pd.WG[k * F + f]; }
+  for (int m = 0; m < 4; ++m) { wfs[m] = Ps.WF[m * F + f]; wfd[m] = Pd.WF[m * F + f]; }
+  const float bfs = Ps.BF[f], bfd = Pd.BF[f];
+
+  const float* m5i = M5 + (size_t)g * NR * 5 * F;
+  float acc[5], mcs[5], mcd[5];
+  for (int rr = 0; rr < NR; ++rr) {
+    acc[rr] = m5i[rr * 5 * F + 0 * F + f];
+    mcs[rr] = m5i[rr * 5 * F + 1 * F + f];
+    mcd[rr] = m5i[rr * 5 * F + 2 * F + f];
+  }
+  for (int c0 = 0; c0 < nb; c0 += MSG_CHUNK) {
+    const int cn = min(MSG_CHUNK, nb - c0);
+    if (threadIdx.x < cn) {
+      const int k = c0 + threadIdx.x;
+      const int j = nbr[p0 + k];
+      const float diff[3] = {rw[3 * i] - rw[3 * j], rw[3 * i + 1] - rw[3 * j + 1], rw[3 * i + 2] - rw[3 * j + 2]};
+      const bool same = up_i == (j < n_up);
+      float beta[5][FIRE_W1MAX], gg[5][4];
+      pair_filter_fl(diff, same ? Ps.filt : Pd.filt, cut, beta, gg);
+      for (int rr = 0; rr < 5; ++rr) {
+        for (int q = 0; q < w1; ++q) s_beta[threadIdx.x][rr][q] = beta[rr][q];
+        for (int m = 0; m < 4; ++m) s_g[threadIdx.x][rr][m] = gg[rr][m];
+      }
+      s_gj[threadIdx.x] = group_of(w, j, B, n_up, n_dn);
+      s_same[threadIdx.x] = same;
+    }
+    __syncthreads();
+    for (int kk = 0; kk < cn; ++kk) {
+      const bool same = s_same[kk];
+      const float* wg = same ? wgs : wgd;
+      const float* wf = same ? wfs : wfd;
+      const float* mc = same ? mcs : mcd;
+      const float* m5j = M5 + (size_t)s_gj[kk] * NR * 5 * F + (same ? 3 : 4) * F + f;
+      float G5[5], E5[5], mn[5];
+      for (int rr = 0; rr < NR; ++rr) {
+        float a = 0.f;
+        for (int q = 0; q < w1; ++q) a += s_beta[kk][rr][q] * wg[q];
+        G5[rr] = a;
+        float b = 0.f;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) b += s_g[kk][rr][m] * wf[m];
+        E5[rr] = b;
+        mn[rr] = m5j[rr * 5 * F];
+      }
+      E5[0] += same ? bfs : bfd;
+      const float ux = E5[0] + mc[0] + mn[0];
+      float sv, s1, s2;
+      silu3(ux, sv, s1, s2);
+      acc[0] += G5[0] * sv;
+      if (TAN) {
+        float uc[3], un[3], q2 = 0.f;
+        for (int t = 0; t < 3; ++t) {
+          uc[t] = E5[2 + t] + mc[2 + t];
+          un[t] = -E5[2 + t] + mn[2 + t];
+          q2 += uc[t] * uc[t] + un[t] * un[t];
+        }
+        const float ul = 2.f * E5[1] + mc[1] + mn[1];
+        const float vl = s1 * ul + s2 * q2;
+        float cross = 0.f;
+        float* orow = out + (row0 + 5 + 3 * (int64_t)(c0 + kk)) * F + f;
+        for (int t = 0; t < 3; ++t) {
+          const float vc = s1 * uc[t], vn = s1 * un[t];
+          acc[2 + t] += G5[2 + t] * sv + G5[0] * vc;
+          orow[t * F] = -G5[2 + t] * sv + G5[0] * vn;
+          cross += G5[2 + t] * (vc - vn);
+        }
+        acc[1] += 2.f * G5[1] * sv + G5[0] * vl + 2.f * cross;
+      }
+    }
+    __syncthreads();
+  }
+  float* o = out + row0 * F + f;
+  for (int rr = 0; rr < NR; ++rr) o[rr * F] = acc[rr];
+}
+
+void launch_messages(hipStream_t s, const float* r, int B, int N, int n_up, const int64_t* ptr, const int32_t* nbr,
+                     const float* M5, EdgeParams same, EdgeParams diff, float cut, int F, bool tangents, float* out) {
+  const int G = B * N;
+  if (tangents)
+    k_messages<true><<<G, F, 0, s>>>(r, B, N, n_up, ptr, nbr, M5, same, diff, cut, F, out);
+  else
+    k_messages<false><<<G, F, 0, s>>>(r, B, N, n_up, ptr, nbr, M5, same, diff, cut, F, out);
+}
+
+}  // namespace fire

@@ -1,0 +1,129 @@
+// Neighbour lists (CSR) for a batch of walkers.
+//
+// Follows get_pair_indices / is_neighbour (src/sparse_wf/model/new_sparse_model.py:238-263,
+// src/sparse_wf/model/graph_utils.py:179-184): distance in float32 as
+// sqrt((dx*dx + dy*dy) + dz*dz) with every operation rounded separately (no FMA
+// contraction), diagonal excluded, strict d < cutoff, neighbours in ascending order.
+//
+// One 64-lane wavefront per centre electron; a ballot over 64 candidate neighbours at a
+// time gives the count (pass 1) and each lane's output slot via a masked popcount (pass 2),
+// so the order is deterministic.  Groups (centre electrons) are numbered up-spin first,
+// walker-major (fire::group_of).
+#include "fire_common.hpp"
+#include "fire_kernels.hpp"
+
+namespace fire {
+
+__device__ __forceinline__ bool in_cutoff(const float* ri, const float* rj, float cutoff) {
+  const float dx = __fsub_rn(ri[0], rj[0]);
+  const float dy = __fsub_rn(ri[1], rj[1]);
+  const float dz = __fsub_rn(ri[2], rj[2]);
+  const float s = __fadd_rn(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)), __fmul_rn(dz, dz));
+  return __fsqrt_rn(s) < cutoff;
+}
+
+template <bool FILL>
+__global__ void __launch_bounds__(256) k_nbr(const float* __restrict__ r, int B, int N, int n_up, float cutoff,
+                                             int32_t* __restrict__ cnt, const int64_t* __restrict__ ptr,
+                                             int32_t* __restrict__ nbr) {
+  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (wave >= B * N) return;
+  const int w = wave / N, i = wave - w * N;
+  const float* rw = r + (size_t)w * N * 3;
+  const float ri[3] = {rw[3 * i], rw[3 * i + 1], rw[3 * i + 2]};
+  const int g = group_of(w, i, B, n_up, N - n_up);
+  int count = 0;
+  int64_t base = FILL ? ptr[g] : 0;
+  for (int j0 = 0; j0 < N; j0 += 64) {
+    const int j = j0 + lane;
+    bool in = false;
+    if (j < N && j != i) in = in_cutoff(ri, rw + 3 * j, cutoff);
+    const uint64_t m = __ballot(in);
+    if (FILL && in) {
+      const int before = __popcll(m & ((1ull << lane) - 1ull));
+      nbr[base + count + before] = j;
+    }
+    count += __popcll(m);
+  }
+  if (!FILL && lane == 0) cnt[g] = count;
+}
+
+// Exclusive scan of the neighbour counts in group order (single workgroup, 1024 threads).
+// ptr[g] = pairs before group g; ptr[G] = total pairs; ptr[G+1] = largest count.
+__global__ void __launch_bounds__(1024) k_scan(const int32_t* __restrict__ cnt, int G, int64_t* __restrict__ ptr) {
+  __shared__ int64_t part[1024];
+  __shared__ int32_t mx[1024];
+  const int t = threadIdx.x;
+  const int per = (G + 1023) / 1024;
+  const int lo = min(G, t * per), hi = min(G, lo + per);
+  int64_t s = 0;
+  int32_t m = 0;
+  for (int k = lo; k < hi; ++k) {
+    s += cnt[k];
+    m = max(m, cnt[k]);
+  }
+  part[t] = s;
+  mx[t] = m;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const int64_t v = t >= off ? part[t - off] : 0;
+    const int32_t vm = t >= off ? mx[t - off] : 0;
+    __syncthreads();
+    part[t] += v;
+    mx[t] = max(mx[t], vm);
+    __syncthreads();
+  }
+  int64_t run = part[t] - s;
+  for (int k = lo; k < hi; ++k) {
+    ptr[k] = run;
+    run += cnt[k];
+  }
+  if (t == 1023) {
+    ptr[G] = part[1023];
+    ptr[G + 1] = mx[1023];
+  }
+}
+
+// rev[p] for pair p = (i -> j) at slot k of i's list: the slot of i in j's list
+// (binary search in j's ascending list).
+__global__ void k_rev(const int64_t* __restrict__ ptr, const int32_t* __restrict__ nbr, int B, int N, int n_up,
+                      int32_t* __restrict__ rev) {
+  const int g = blockIdx.x;
+  int w, i;
+  group_inv(g, B, n_up, N - n_up, w, i);
+  const int64_t p0 = ptr[g], p1 = ptr[g + 1];
+  for (int64_t p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
+    const int j = nbr[p];
+    const int gj = group_of(w, j, B, n_up, N - n_up);
+    int64_t a = ptr[gj], b = ptr[gj + 1];
+    while (a < b) {
+      int64_t mid = (a + b) >> 1;
+      if (nbr[mid] < i) a = mid + 1; else b = mid;
+    }
+    rev[p] = (int32_t)(a - ptr[gj]);
+  }
+}
+
+void launch_nbr_count(hipStream_t s, const float* r, int B, int N, int n_up, float cutoff, int32_t* cnt) {
+  const int waves = B * N;
+  const int blocks = (waves * 64 + 255) / 256;
+  k_nbr<false><<<blocks, 256, 0, s>>>(r, B, N, n_up, cutoff, cnt, nullptr, nullptr);
+}
+
+void launch_nbr_fill(hipStream_t s, const float* r, int B, int N, int n_up, float cutoff, const int64_t* ptr,
+                     int32_t* nbr) {
+  const int waves = B * N;
+  const int blocks = (waves * 64 + 255) / 256;
+  k_nbr<true><<<blocks, 256, 0, s>>>(r, B, N, n_up, cutoff, nullptr, ptr, nbr);
+}
+
+void launch_scan(hipStream_t s, const int32_t* cnt, int G, int64_t* ptr) {
+  k_scan<<<1, 1024, 0, s>>>(cnt, G, ptr);
+}
+
+void launch_rev(hipStream_t s, const int64_t* ptr, const int32_t* nbr, int B, int N, int n_up, int32_t* rev) {
+  k_rev<<<B * N, 64, 0, s>>>(ptr, nbr, B, N, n_up, rev);
+}
+
+}  // namespace fire

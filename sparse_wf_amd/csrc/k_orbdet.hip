@@ -1,0 +1,401 @@
+// Orbitals x envelopes, Slater LU / inverse / log-determinant (FP64) and the sparse
+// determinant Laplacian.
+//
+// References (sparse_wf): EfficientIsotropicEnvelopes envelopes.py:55-82,
+// Orbitals.fwd_lap orbitals.py:148-174, merge_up_down / multiply_with_1el_fn
+// sparse_fwd_lap.py:129-137,322-335, slog_and_inverse model/utils.py:144-175,
+// sparse_slogdet_with_fwd_lap sparse_fwd_lap.py:294-319 (triplets :228-291).
+#include "fire_common.hpp"
+#include "fire_kernels.hpp"
+#include "fire_kernels2.hpp"
+
+namespace fire {
+
+#define ENV_COEF_MAX 1600  // (nuclei x dets x envelopes) per LDS chunk
+#define ENV_QMAX 8         // orbital columns per thread: n_el * n_det <= 8 * 256
+
+// Envelope rows of one electron: env[d,o] = sum_{n,e} pi[n,d,e,o] exp(-sp[n,d,e] |r - R_n|).
+// d/dr_t exp(-s d_n) = -s q n_t ; Laplacian = q (s^2 - 2 s / d_n).
+template <bool TAN>
+__global__ void __launch_bounds__(256) k_envelopes(const float* __restrict__ r, const float* __restrict__ R, int B,
+                                                   int N, int n_up, int M, int D, int E, const float* __restrict__ sp_up,
+                                                   const float* __restrict__ pi_up, const float* __restrict__ sp_dn,
+                                                   const float* __restrict__ pi_dn, float* __restrict__ out) {
+  __shared__ float coef[5][ENV_COEF_MAX];
+  const int g = blockIdx.x;
+  int w, i;
+  group_inv(g, B, n_up, N - n_up, w, i);
+  const float* ri = r + ((size_t)w * N + i) * 3;
+  const float* sp = i < n_up ? sp_up : sp_dn;
+  const float* pi = i < n_up ? pi_up : pi_dn;
+  const int ND = N * D;
+  const int NR = TAN ? 5 : 1;
+  const int per_nuc = D * E;
+  const int mch = max(1, ENV_COEF_MAX / per_nuc);
+  float acc[ENV_QMAX][5];
+  for (int q = 0; q < ENV_QMAX; ++q)
+    for (int rr = 0; rr < 5; ++rr) acc[q][rr] = 0.f;
+  for (int m0 = 0; m0 < M; m0 += mch) {
+    const int mn = min(mch, M - m0);
+    for (int idx = threadIdx.x; idx < mn * per_nuc; idx += blockDim.x) {
+      const int m = m0 + idx / per_nuc;
+      const float dx = ri[0] - R[3 * m], dy = ri[1] - R[3 * m + 1], dz = ri[2] - R[3 * m + 2];
+      const float dn = sqrtf(dx * dx + dy * dy + dz * dz);
+      const float s = sp[(size_t)m * per_nuc + idx % per_nuc];
+      const float q = expf(-s * dn);
+      coef[0][idx] = q;
+      if (TAN) {
+        const float inv = 1.0f / dn;
+        coef[1][idx] = q * (s * s - 2.0f * s * inv);
+        coef[2][idx] = -s * q * dx * inv;
+        coef[3][idx] = -s * q * dy * inv;
+        coef[4][idx] = -s * q * dz * inv;
+      }
+    }
+    __syncthreads();
+    for (int q = 0; q * (int)blockDim.x + (int)threadIdx.x < ND && q < ENV_QMAX; ++q) {
+      const int c = q * blockDim.x + threadIdx.x;
+      const int d = c / N, o = c - d * N;
+      for (int mm = 0; mm < mn; ++mm) {
+        const int m = m0 + mm;
+        for (int e = 0; e < E; ++e) {
+          const int li = mm * per_nuc + d * E + e;
+          const float p = pi[(((size_t)m * D + d) * E + e) * N + o];
+          for (int rr = 0; rr < NR; ++rr) acc[q][rr] += coef[rr][li] * p;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  for (int q = 0; q < ENV_QMAX; ++q) {
+    const int c = q * blockDim.x + threadIdx.x;
+    if (c >= ND) break;
+    for (int rr = 0; rr < NR; ++rr) out[((size_t)g * NR + rr) * ND + c] = acc[q][rr];
+  }
+}
+
+void launch_envelopes(hipStream_t s, const float* r, const float* R, int B, int N, int n_up, int M, int D, int E,
+                      const float* sp_up, const float* pi_up, const float* sp_dn, const float* pi_dn, bool tangents,
+                      float* out) {
+  const int G = B * N;
+  if (tangents)
+    k_envelopes<true><<<G, 256, 0, s>>>(r, R, B, N, n_up, M, D, E, sp_up, pi_up, sp_dn, pi_dn, out);
+  else
+    k_envelopes<false><<<G, 256, 0, s>>>(r, R, B, N, n_up, M, D, E, sp_up, pi_up, sp_dn, pi_dn, out);
+}
+
+// Phi = orbital rows x envelope rows (product with a one-electron function), in place.
+template <bool TAN>
+__global__ void k_orb_product(float* __restrict__ O, int ND, GroupDesc gd, const float* __restrict__ env) {
+  const int g = blockIdx.x;
+  const int c = blockIdx.y * blockDim.x + threadIdx.x;
+  if (c >= ND) return;
+  const int NR = TAN ? 5 : 1;
+  const float* e = env + (size_t)g * NR * ND + c;
+  const int64_t r0 = gd.row(g), r1 = gd.row(g + 1);
+  float* o = O + r0 * ND + c;
+  const float ox = o[0], ex = e[0];
+  o[0] = ox * ex;
+  if (!TAN) return;
+  const float ol = o[ND], el = e[ND];
+  float cross = 0.f;
+  for (int t = 0; t < 3; ++t) {
+    const float ot = o[(2 + t) * ND], et = e[(2 + t) * ND];
+    cross += ot * et;
+    o[(2 + t) * (int64_t)ND] = ox * et + ex * ot;
+  }
+  o[ND] = ol * ex + el * ox + 2.f * cross;
+  for (int64_t k = 5; k < r1 - r0; ++k) o[k * ND] *= ex;
+}
+
+void launch_orb_product(hipStream_t s, float* O, int ND, int G, GroupDesc gd, const float* env, bool tangents) {
+  dim3 grid(G, (ND + 255) / 256);
+  if (tangents)
+    k_orb_product<true><<<grid, 256, 0, s>>>(O, ND, gd, env);
+  else
+    k_orb_product<false><<<grid, 256, 0, s>>>(O, ND, gd, env);
+}
+
+// A[w,d][i][o] (FP64) from the value rows of the orbital groups.
+__global__ void k_gather_A(const float* __restrict__ O, int ND, GroupDesc gd, int B, int N, int n_up, int D,
+                           double* __restrict__ A) {
+  const int g = blockIdx.x;
+  int w, i;
+  group_inv(g, B, n_up, N - n_up, w, i);
+  const float* o = O + gd.row(g) * ND;
+  for (int c = threadIdx.x; c < ND; c += blockDim.x) {
+    const int d = c / N, col = c - d * N;
+    A[(((size_t)w * D + d) * N + i) * N + col] = (double)o[c];
+  }
+}
+
+void launch_gather_A(hipStream_t s, const float* O, int ND, GroupDesc gd, int B, int N, int n_up, int D, double* A) {
+  k_gather_A<<<B * N, 256, 0, s>>>(O, ND, gd, B, N, n_up, D, A);
+}
+
+// LU with partial pivoting (first maximum wins), log|det|, sign and the inverse, FP64.
+// One workgroup per matrix; A is overwritten by its LU factors; Ainv row-major.
+__global__ void __launch_bounds__(256) k_lu_inv(double* __restrict__ Aall, int N, double* __restrict__ Ainv_all,
+                                                double* __restrict__ logdet, double* __restrict__ sign, int want_inv) {
+  extern __shared__ unsigned char smem_raw[];
+  int* perm = reinterpret_cast<int*>(smem_raw);
+  __shared__ double s_val[4];
+  __shared__ int s_idx[4];
+  __shared__ int s_piv;
+  const int mat = blockIdx.x;
+  double* A = Aall + (size_t)mat * N * N;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  for (int k = tid; k < N; k += blockDim.x) perm[k] = k;
+  __syncthreads();
+  int swaps = 0;
+  for (int k = 0; k < N; ++k) {
+    // pivot search over rows k..N-1 of column k
+    double best = -1.0;
+    int bi = N;
+    for (int rr = k + tid; rr < N; rr += blockDim.x) {
+      const double v = fabs(A[(size_t)rr * N + k]);
+      if (v > best || (v == best && rr < bi)) { best = v; bi = rr; }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      const double ov = __shfl_xor(best, o, 64);
+      const int oi = __shfl_xor(bi, o, 64);
+      if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
+    }
+    if (lane == 0) { s_val[wid] = best; s_idx[wid] = bi; }
+    __syncthreads();
+    if (tid == 0) {
+      double b = s_val[0];
+      int ii = s_idx[0];
+      for (int q = 1; q < (int)(blockDim.x >> 6); ++q)
+        if (s_val[q] > b || (s_val[q] == b && s_idx[q] < ii)) { b = s_val[q]; ii = s_idx[q]; }
+      s_piv = ii;
+      if (ii != k) {
+        const int t = perm[k];
+        perm[k] = perm[ii];
+        perm[ii] = t;
+      }
+    }
+    __syncthreads();
+    const int p = s_piv;
+    if (p != k) {
+      ++swaps;
+      for (int c = tid; c < N; c += blockDim.x) {
+        const double t = A[(size_t)k * N + c];
+        A[(size_t)k * N + c] = A[(size_t)p * N + c];
+        A[(size_t)p * N + c] = t;
+      }
+    }
+    __syncthreads();
+    const double piv = A[(size_t)k * N + k];
+    const double ipiv = 1.0 / piv;
+    for (int rr = k + 1 + tid; rr < N; rr += blockDim.x) A[(size_t)rr * N + k] *= ipiv;
+    __syncthreads();
+    const int nt = N - k - 1;
+    for (int e = tid; e < nt * nt; e += blockDim.x) {
+      const int rr = k + 1 + e / nt, c = k + 1 + e % nt;
+      A[(size_t)rr * N + c] -= A[(size_t)rr * N + k] * A[(size_t)k * N + c];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    double ld = 0.0;
+    int neg = 0;
+    for (int k = 0; k < N; ++k) {
+      const double dk = A[(size_t)k * N + k];
+      ld += log(fabs(dk));
+      neg += dk < 0.0;
+    }
+    logdet[mat] = ld;
+    sign[mat] = ((swaps + neg) & 1) ? -1.0 : 1.0;
+  }
+  if (!want_inv) return;
+  // inverse: column c solves L U x = P e_c (thread c; Ainv row-major so stores coalesce)
+  double* X = Ainv_all + (size_t)mat * N * N;
+  for (int c = tid; c < N; c += blockDim.x) {
+    for (int k = 0; k < N; ++k) {
+      double y = (perm[k] == c) ? 1.0 : 0.0;
+      for (int j = 0; j < k; ++j) y -= A[(size_t)k * N + j] * X[(size_t)j * N + c];
+      X[(size_t)k * N + c] = y;
+    }
+    for (int k = N - 1; k >= 0; --k) {
+      double y = X[(size_t)k * N + c];
+      for (int j = k + 1; j < N; ++j) y -= A[(size_t)k * N + j] * X[(size_t)j * N + c];
+      X[(size_t)k * N + c] = y / A[(size_t)k * N + k];
+    }
+  }
+}
+
+void launch_lu_inv(hipStream_t s, double* A, int n_mat, int N, double* Ainv, double* logdet, double* sign,
+                   bool want_inv) {
+  k_lu_inv<<<n_mat, 256, N * sizeof(int), s>>>(A, N, Ainv, logdet, sign, want_inv ? 1 : 0);
+}
+
+// tr(Ainv * Lap A) for every (walker, det): sum_i sum_o lap_i[d,o] Ainv[o,i].
+__global__ void k_det_trace(const float* __restrict__ O, int ND, GroupDesc gd, int B, int N, int n_up, int D,
+                            const double* __restrict__ Ainv, double* __restrict__ tr) {
+  __shared__ double red[16];
+  const int wd = blockIdx.x;
+  const int w = wd / D, d = wd - w * D;
+  const double* X = Ainv + (size_t)wd * N * N;
+  double acc = 0.0;
+  for (int e = threadIdx.x; e < N * N; e += blockDim.x) {
+    const int i = e / N, o = e - i * N;
+    const int g = group_of(w, i, B, n_up, N - n_up);
+    acc += (double)O[(gd.row(g) + 1) * ND + d * N + o] * X[(size_t)o * N + i];
+  }
+  double v[1] = {acc};
+  block_sum<1>(v, red);
+  if (threadIdx.x == 0) tr[wd] = v[0];
+}
+
+void launch_det_trace(hipStream_t s, const float* O, int ND, GroupDesc gd, int B, int N, int n_up, int D,
+                      const double* Ainv, double* tr) {
+  k_det_trace<<<B * D, 256, 0, s>>>(O, ND, gd, B, N, n_up, D, Ainv, tr);
+}
+
+#define DL_OC 32
+
+// Triplet part of the determinant Laplacian and the determinant gradient for one
+// (walker, det, electron n): with S = {n} U N(n) and J_a the jacobian rows d Phi_a / d r_n,
+// B[a,x,b] = sum_o J_a[x,o] Ainv[o, S_b];  trip = sum B[a,x,b] B[b,x,a];  grad_x = sum_a B[a,x,a].
+__global__ void __launch_bounds__(256) k_det_lap(const float* __restrict__ O, int ND, GroupDesc gd,
+                                                 const int64_t* __restrict__ ptr, const int32_t* __restrict__ nbr,
+                                                 const int32_t* __restrict__ rev, int B, int N, int n_up, int D,
+                                                 int smax, const double* __restrict__ Ainv, double* __restrict__ trip,
+                                                 double* __restrict__ grad) {
+  extern __shared__ unsigned char smem_raw[];
+  double* Bacc = reinterpret_cast<double*>(smem_raw);              // 3 smax^2
+  double* Ac = Bacc + 3 * smax * smax;                              // smax * OC
+  float* Jc = reinterpret_cast<float*>(Ac + smax * DL_OC);          // 3 smax * OC
+  int* s_idx = reinterpret_cast<int*>(Jc + 3 * smax * DL_OC);       // smax
+  int64_t* s_row = reinterpret_cast<int64_t*>(s_idx + smax + (smax & 1));  // smax (first jac row)
+  __shared__ double red[16 * 4];
+  const int blk = blockIdx.x;
+  const int n = blk % N;
+  const int wd = blk / N;
+  const int w = wd / D, d = wd - w * D;
+  const int gn = group_of(w, n, B, n_up, N - n_up);
+  const int64_t p0 = ptr[gn];
+  const int s = 1 + (int)(ptr[gn + 1] - p0);
+  const int tid = threadIdx.x;
+  for (int a = tid; a < s; a += blockDim.x) {
+    if (a == 0) {
+      s_idx[0] = n;
+      s_row[0] = gd.row(gn) + 2;
+    } else {
+      const int i = nbr[p0 + a - 1];
+      const int gi = group_of(w, i, B, n_up, N - n_up);
+      s_idx[a] = i;
+      s_row[a] = gd.row(gi) + 5 + 3 * (int64_t)rev[p0 + a - 1];
+    }
+  }
+  const int nB = 3 * s * s;
+  for (int e = tid; e < nB; e += blockDim.x) Bacc[e] = 0.0;
+  __syncthreads();
+  const double* X = Ainv + (size_t)wd * N * N;
+  for (int o0 = 0; o0 < N; o0 += DL_OC) {
+    const int on = min(DL_OC, N - o0);
+    for (int e = tid; e < 3 * s * DL_OC; e += blockDim.x) {
+      const int ax = e / DL_OC, oc = e - ax * DL_OC;
+      const int a = ax / 3, x = ax - 3 * a;
+      Jc[e] = oc < on ? O[(s_row[a] + x) * ND + d * N + o0 + oc] : 0.f;
+    }
+    for (int e = tid; e < s * DL_OC; e += blockDim.x) {
+      const int b = e / DL_OC, oc = e - b * DL_OC;
+      Ac[e] = oc < on ? X[(size_t)(o0 + oc) * N + s_idx[b]] : 0.0;
+    }
+    __syncthreads();
+    for (int e = tid; e < nB; e += blockDim.x) {
+      const int ax = e / s, b = e - ax * s;
+      const float* jr = Jc + ax * DL_OC;
+      const double* ar = Ac + b * DL_OC;
+      double acc = 0.0;
+#pragma unroll 8
+      for (int oc = 0; oc < DL_OC; ++oc) acc += (double)jr[oc] * ar[oc];
+      Bacc[e] += acc;
+    }
+    __syncthreads();
+  }
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int e = tid; e < nB; e += blockDim.x) {
+    const int ax = e / s, b = e - ax * s;
+    const int a = ax / 3, x = ax - 3 * a;
+    v[0] += Bacc[e] * Bacc[(b * 3 + x) * s + a];
+    if (a == b) v[1 + x] += Bacc[e];
+  }
+  block_sum<4>(v, red);
+  if (tid == 0) {
+    trip[blk] = v[0];
+    for (int x = 0; x < 3; ++x) grad[(size_t)blk * 3 + x] = v[1 + x];
+  }
+}
+
+size_t det_lap_smem(int smax) {
+  return (size_t)3 * smax * smax * 8 + (size_t)smax * DL_OC * 8 + (size_t)3 * smax * DL_OC * 4 +
+         (size_t)(smax + (smax & 1)) * 4 + (size_t)smax * 8;
+}
+
+void launch_det_lap(hipStream_t s, const float* O, int ND, GroupDesc gd, const int64_t* ptr, const int32_t* nbr,
+                    const int32_t* rev, int B, int N, int n_up, int D, int smax, const double* Ainv, double* trip,
+                    double* grad) {
+  const size_t sm = det_lap_smem(smax);
+  k_det_lap<<<B * D * N, 256, sm, s>>>(O, ND, gd, ptr, nbr, rev, B, N, n_up, D, smax, Ainv, trip, grad);
+}
+
+hipError_t det_lap_set_smem_limit(size_t bytes) {
+  return hipFuncSetAttribute((const void*)k_det_lap, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+// Sum of the determinants: log|psi_det|, sign, gradient and Laplacian per walker (FP64).
+//   w_d = s_d exp(L_d - m) / sum;  grad = sum_d w_d g_d;
+//   lap = sum_d w_d (lap_d + |g_d|^2) - |grad|^2,  lap_d = tr_d - sum_n trip_{d,n}.
+__global__ void k_det_combine(int N, int D, const double* __restrict__ logdet, const double* __restrict__ sgn,
+                              const double* __restrict__ tr, const double* __restrict__ trip,
+                              const double* __restrict__ gd_, double* __restrict__ out_logpsi,
+                              double* __restrict__ out_sign, double* __restrict__ out_grad, double* __restrict__ out_lap,
+                              int tangents) {
+  __shared__ double s_w[16];
+  __shared__ double red[16 * 2];
+  const int w = blockIdx.x;
+  if (threadIdx.x == 0) {
+    double m = -1e300;
+    for (int d = 0; d < D; ++d) m = fmax(m, logdet[w * D + d]);
+    double sum = 0.0;
+    for (int d = 0; d < D; ++d) sum += sgn[w * D + d] * exp(logdet[w * D + d] - m);
+    for (int d = 0; d < D; ++d) s_w[d] = sgn[w * D + d] * exp(logdet[w * D + d] - m) / sum;
+    out_logpsi[w] = m + log(fabs(sum));
+    out_sign[w] = sum < 0 ? -1.0 : 1.0;
+  }
+  __syncthreads();
+  if (!tangents) return;
+  double v[2] = {0.0, 0.0};  // sum_d w_d (tr_d - trip_d + |g_d|^2),  |grad|^2
+  for (int e = threadIdx.x; e < 3 * N; e += blockDim.x) {
+    double gsum = 0.0;
+    for (int d = 0; d < D; ++d) {
+      const double gv = gd_[((size_t)(w * D + d) * N) * 3 + e];
+      gsum += s_w[d] * gv;
+      v[0] += s_w[d] * gv * gv;
+    }
+    out_grad[(size_t)w * 3 * N + e] = gsum;
+    v[1] += gsum * gsum;
+  }
+  for (int e = threadIdx.x; e < D * N; e += blockDim.x) {
+    const int d = e / N;
+    v[0] -= s_w[d] * trip[(size_t)(w * D + d) * N + (e - d * N)];
+  }
+  block_sum<2>(v, red);
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int d = 0; d < D; ++d) t += s_w[d] * tr[w * D + d];
+    out_lap[w] = t + v[0] - v[1];
+  }
+}
+
+void launch_det_combine(hipStream_t s, int B, int N, int D, const double* logdet, const double* sgn, const double* tr,
+                        const double* trip, const double* grad, double* out_logpsi, double* out_sign,
+                        double* out_grad, double* out_lap, bool tangents) {
+  k_det_combine<<<B, 256, 0, s>>>(N, D, logdet, sgn, tr, trip, grad, out_logpsi, out_sign, out_grad, out_lap,
+                                  tangents ? 1 : 0);
+}
+
+}  // namespace fire

@@ -228,3 +228,92 @@ def leaves_with_path(tree, prefix=""):
             yield from leaves_with_path(v, f"{prefix}/{i}")
     else:
         yield prefix, tree
+
+
+def _softplus(x):
+    x = np.asarray(x, np.float64)
+    return np.where(x > 20, x, np.log1p(np.exp(np.minimum(x, 20))))
+
+
+def _sigmoid(x):
+    return 1.0 / (1.0 + np.exp(-np.asarray(x, np.float64)))
+
+
+def _swap_perm(N: int, D: int, n_up: int) -> np.ndarray:
+    """Column permutation of _swap_spin_updn (orbitals.py:21-26): new[d,o] = old[d,(o+n_up)%N]."""
+    o = np.arange(N)
+    return np.concatenate([d * N + (o + n_up) % N for d in range(D)])
+
+
+def pack_params(cfg: FireConfig, params: dict) -> list:
+    """Flatten ``params`` into the canonical float32 tensor list of ``fire_upload_params``.
+
+    Transformations applied here (all exact re-arrangements or precomputations):
+    softplus of the filter / envelope scales; the normalisation scale s0 folded into the first
+    update-MLP weight and s1 into every first layer that consumes the embedding (orbitals,
+    MLP Jastrow, attention Q/V) -- ``h_out = s1 * MLP(s0 * h)`` (new_sparse_model.py:691-693);
+    Q and V weights concatenated; for ``spin_restricted`` the down-spin orbital / envelope
+    parameters are the up-spin ones with the columns of _swap_spin_updn permuted
+    (orbitals.py:88-107); learned cutoff = cutoff * sigmoid(cutoff_param) (:644)."""
+    f32 = lambda a: np.ascontiguousarray(np.asarray(a, np.float64).astype(np.float32))
+    ep = params["embedding"]
+    s0, s1 = (float(v) for v in ep["scales"])
+    dyn = ep["dynamic_params_en"]
+    ei = ep["elec_init"]["params"]
+    edge = ei["ElecNucEdge_0"]
+    out = [
+        dyn["filter"]["kernel"], dyn["filter"]["bias"], _softplus(dyn["filter"]["scales"]), dyn["nuc_embedding"],
+        edge["PairwiseFilter_0"]["Dense_0"]["kernel"], edge["PairwiseFilter_0"]["Dense_0"]["bias"],
+        edge["PairwiseFilter_0"]["Dense_1"]["kernel"],
+        edge["Dense_0"]["kernel"], edge["Dense_1"]["kernel"],
+        ei["LayerNorm_0"]["scale"], ei["LayerNorm_0"]["bias"],
+        ei["GatedLinearUnit_0"]["Dense_0"]["kernel"],
+        ei["Dense_0"]["kernel"], ei["Dense_0"]["bias"], ei["Dense_1"]["kernel"], ei["Dense_1"]["bias"],
+    ]
+    for name in ("edge_same", "edge_diff"):
+        q = ep[name]["params"]
+        b = q["beta_ee"]
+        out += [b["kernel"], b["bias"], _softplus(b["scale"]), b["Dense_0"]["kernel"], b["Dense_0"]["bias"],
+                b["Dense_1"]["kernel"], q["Dense_0"]["kernel"], q["Dense_1"]["kernel"], q["Dense_1"]["bias"]]
+    up = ep["updates"][0]["params"]
+    out += [np.asarray(up["Linear_0"]["kernel"], np.float64) * s0, up["Linear_0"]["bias"],
+            up["Linear_1"]["kernel"], up["Linear_1"]["bias"]]
+    op = params["to_orbitals"]["params"]
+    N, D = cfg.n_el, cfg.n_determinants
+    W_up = np.asarray(op["to_orbitals_up"]["kernel"], np.float64)
+    b_up = np.asarray(op["to_orbitals_up"]["bias"], np.float64)
+    sp_up = _softplus(op["env_up"]["sigma"])
+    pi_up = np.asarray(op["env_up"]["pi"], np.float64)
+    if cfg.spin_restricted:
+        perm = _swap_perm(N, D, cfg.n_up)
+        W_dn, b_dn = W_up[:, perm], b_up[perm]
+        sp_dn = sp_up
+        pi_dn = pi_up[..., (np.arange(N) + cfg.n_up) % N]
+    else:
+        W_dn = np.asarray(op["to_orbitals_dn"]["kernel"], np.float64)
+        b_dn = np.asarray(op["to_orbitals_dn"]["bias"], np.float64)
+        sp_dn = _softplus(op["env_dn"]["sigma"])
+        pi_dn = np.asarray(op["env_dn"]["pi"], np.float64)
+    out += [W_up * s1, b_up, W_dn * s1, b_dn, sp_up, pi_up, sp_dn, pi_dn]
+    jp = params["jastrow"]["params"]
+    if cfg.use_mlp_jastrow or cfg.use_log_jastrow:
+        mlp = jp["mlp"]
+        for l in range(cfg.mlp_depth):
+            W = np.asarray(mlp[f"Linear_{l}"]["kernel"], np.float64)
+            out += [W * s1 if l == 0 else W, mlp[f"Linear_{l}"]["bias"]]
+        last = mlp[f"Linear_{cfg.mlp_depth}"]
+        zb = [0.0, float(jp["log_bias"]) if cfg.use_log_jastrow else 0.0]
+        out += [last["kernel"], last["bias"], jp["mlp_scale"], np.asarray(zb)]
+    if cfg.use_attention:
+        a = jp["att"]
+        qv = np.concatenate([a["Q_W"]["kernel"], a["V_W"]["kernel"]], axis=1) * s1
+        out += [qv, np.concatenate([a["Q_W"]["bias"], a["V_W"]["bias"]]), a["register_keys"]]
+        for l in range(cfg.mlp_depth):
+            out += [a["out"][f"Dense_{l}"]["kernel"], a["out"][f"Dense_{l}"]["bias"]]
+        last = a["out"][f"Dense_{cfg.mlp_depth}"]
+        out += [last["kernel"], last["bias"], a["scale"], np.asarray([0.0, float(np.asarray(a["bias"]).reshape(-1)[0])])]
+    if cfg.e_e_cusps == "psiformer":
+        c = jp["pairwise_cusps"]
+        out.append(np.asarray([c["alpha_same"], c["alpha_diff"], c["factor_same"], c["factor_diff"]], np.float64))
+    out.append(np.asarray([cfg.cutoff * _sigmoid(ep["cutoff_param"]), s1]))
+    return [f32(t).reshape(-1) for t in out]

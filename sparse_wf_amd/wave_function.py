@@ -1,0 +1,243 @@
+"""Host-side mirror of the reference's wavefunction API, backed by the HIP library.
+
+``FireWaveFunction`` keeps the method names and argument meaning of
+``MoonLikeWaveFunction`` (src/sparse_wf/model/wave_function.py:67-287) and the
+``ParameterizedWaveFunction`` protocol (src/sparse_wf/api.py:161-185): ``create``, ``init``,
+``get_static_input``, ``signed``, ``__call__``, ``kinetic_energy``, ``log_psi_with_state``,
+``log_psi_low_rank_update``.  Electrons may be a single walker ``[N, 3]`` (as the reference's
+per-walker functions) or a batch ``[B, N, 3]`` (the reference's ``vmap``); results carry
+the same leading shape.  Arrays live on the GPU as torch tensors; every computation runs
+in ``libfire_amd.so``.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+from typing import Any, Mapping, NamedTuple, Optional
+
+import numpy as np
+import torch
+
+from . import _lib
+from .config import FireConfig, Molecule
+from .params import init_params, pack_params, set_scales
+
+PRECISIONS = {"fp32": _lib.FIRE_FP32, "mixed": _lib.FIRE_MIXED}
+
+
+class StaticArgs(NamedTuple):
+    """Neighbour counts of ``StaticArgs`` (new_sparse_model.py:46-80).  The HIP path uses
+    ragged CSR lists, so these are informative (capacity bookkeeping of the reference)."""
+
+    n_pairs_same: int
+    n_pairs_diff: int
+    n_triplets: int
+    n_neighbours_en: int
+    n_changed_hout: int
+    n_changed_pair_same: int
+    n_changed_pair_diff: int
+    n_pp_elecs: int
+
+
+def _stream_ptr() -> int:
+    return torch.cuda.current_stream().cuda_stream
+
+
+def _dptr(t: Optional[torch.Tensor]):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+class FireWaveFunction:
+    """FiRE (``embedding: new_sparse``) wavefunction on one MI355X."""
+
+    def __init__(self, cfg: FireConfig, device: int = 0, precision: str = "mixed"):
+        self.cfg = cfg
+        self.device = torch.device("cuda", device)
+        self.precision = precision
+        lib = _lib.load()
+        c = _lib.FireConfigC(
+            n_el=cfg.n_el, n_up=cfg.n_up, n_nuc=cfg.n_nuc, feature_dim=cfg.feature_dim,
+            filter_w0=cfg.pair_mlp_widths[0], filter_w1=cfg.pair_mlp_widths[1],
+            n_env_en=cfg.pair_n_envelopes, n_det=cfg.n_determinants, n_env_iso=cfg.n_envelopes,
+            e_e_cusp=1 if cfg.e_e_cusps == "psiformer" else 0,
+            use_mlp_jastrow=int(cfg.use_mlp_jastrow), use_log_jastrow=int(cfg.use_log_jastrow),
+            use_attention=int(cfg.use_attention), mlp_width=cfg.mlp_width, mlp_depth=cfg.mlp_depth,
+            att_heads=cfg.attention_heads, att_dim=cfg.attention_dim,
+            cutoff=cfg.cutoff, cutoff_1el=cfg.cutoff_1el,
+        )
+        R = np.ascontiguousarray(cfg.mol.R, np.float32)
+        Z = np.ascontiguousarray(cfg.mol.Z, np.float32)
+        h = ctypes.c_void_p()
+        _lib.check(lib.fire_create(ctypes.byref(c), R.ctypes.data, Z.ctypes.data, device, ctypes.byref(h)))
+        self._h = h
+        self._lib = lib
+        self._uploaded = None
+
+    # reference-style constructor ---------------------------------------------------------
+    @classmethod
+    def create(cls, mol: Molecule, embedding: Mapping[str, Any], jastrow: Mapping[str, Any],
+               envelopes: Mapping[str, Any], n_determinants: int, spin_restricted: bool, **kw):
+        """MoonLikeWaveFunction.create (wave_function.py:102-150)."""
+        cfg = FireConfig.from_model_args(mol, dict(embedding=embedding, jastrow=jastrow, envelopes=envelopes,
+                                                   n_determinants=n_determinants, spin_restricted=spin_restricted))
+        return cls(cfg, **kw)
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                self._lib.fire_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    @property
+    def R(self):
+        return self.cfg.mol.R
+
+    @property
+    def Z(self):
+        return self.cfg.mol.Z
+
+    @property
+    def n_electrons(self):
+        return self.cfg.n_el
+
+    @property
+    def n_up(self):
+        return self.cfg.n_up
+
+    # parameters ------------------------------------------------------------------------
+    def upload(self, params: dict):
+        tensors = pack_params(self.cfg, params)
+        sizes = np.asarray([t.size for t in tensors], np.int64)
+        flat = np.ascontiguousarray(np.concatenate(tensors).astype(np.float32))
+        _lib.check(self._lib.fire_upload_params(self._h, flat.ctypes.data, sizes.ctypes.data, len(tensors)))
+        self._uploaded = params
+
+    def _ensure(self, params):
+        if params is not self._uploaded:
+            self.upload(params)
+
+    def init(self, seed: int, electrons) -> dict:
+        """MoonLikeWaveFunction.init (wave_function.py:89-99): random parameters with the
+        normalisation scales set to 1/std of the embedding on ``electrons`` (one walker)."""
+        params = init_params(self.cfg, seed)
+
+        def stages(p):
+            self.upload(p)
+            hc, ho = self.embedding(p, electrons, unscaled_out=True)
+            return hc.double().cpu().numpy(), ho.double().cpu().numpy()
+
+        return set_scales(params, stages)
+
+    # inputs ----------------------------------------------------------------------------
+    def _walkers(self, electrons):
+        r = torch.as_tensor(electrons, dtype=torch.float32, device=self.device)
+        single = r.dim() == 2
+        if single:
+            r = r.unsqueeze(0)
+        if r.shape[1:] != (self.cfg.n_el, 3):
+            raise ValueError(f"electrons must be [..., {self.cfg.n_el}, 3], got {tuple(r.shape)}")
+        return r.contiguous(), single
+
+    def _prec(self, precision):
+        p = precision or self.precision
+        if p not in PRECISIONS:
+            raise ValueError(f"precision must be one of {list(PRECISIONS)}")
+        return PRECISIONS[p]
+
+    # API -------------------------------------------------------------------------------
+    def get_static_input(self, electrons, electrons_new=None, idx_changed=None, laplacian=True) -> StaticArgs:
+        """get_static_input (new_sparse_model.py:399-437), computed from the GPU neighbour lists
+        (max over walkers)."""
+        r, _ = self._walkers(electrons if electrons_new is None else electrons_new)
+        counts, nbr, ptr = self.neighbours(r)
+        B, N = r.shape[0], self.cfg.n_el
+        spin = (torch.arange(N, device=self.device) < self.cfg.n_up)
+        ctr = torch.repeat_interleave(torch.arange(B * N, device=self.device), counts.reshape(-1).long())
+        same = spin[ctr % N] == spin[nbr.long()]
+        w = ctr // N
+        n_same = torch.zeros(B, dtype=torch.long, device=self.device).index_add_(0, w, same.long())
+        n_all = counts.reshape(B, N).sum(1).long()
+        n_trip = (counts.reshape(B, N).long() + 1) ** 2
+        trip = int((n_trip.sum(1) - (2 * counts.reshape(B, N).long() + 1).sum(1)).max()) if laplacian else 0
+        return StaticArgs(int(n_same.max()), int((n_all - n_same).max()), trip, self.cfg.n_nuc, 0, 0, 0, N)
+
+    def neighbours(self, electrons):
+        """CSR neighbour lists: counts [B, N], neighbour indices [P], row pointer [B*N+1]."""
+        r, _ = self._walkers(electrons)
+        B, N = r.shape[0], self.cfg.n_el
+        counts = torch.empty((B, N), dtype=torch.int32, device=self.device)
+        npairs = ctypes.c_int64()
+        _lib.check(self._lib.fire_neighbours(self._h, _stream_ptr(), _dptr(r), B, _dptr(counts), None, 0,
+                                             ctypes.byref(npairs)))
+        nbr = torch.empty(max(npairs.value, 1), dtype=torch.int32, device=self.device)
+        _lib.check(self._lib.fire_neighbours(self._h, _stream_ptr(), _dptr(r), B, _dptr(counts), _dptr(nbr),
+                                             nbr.numel(), ctypes.byref(npairs)))
+        ptr = torch.zeros(B * N + 1, dtype=torch.int64, device=self.device)
+        ptr[1:] = torch.cumsum(counts.reshape(-1).long(), 0)
+        return counts, nbr[: npairs.value], ptr
+
+    def signed(self, params, electrons, static=None, precision=None):
+        """(sign, log|psi|) -- MoonLikeWaveFunction.signed (wave_function.py:183-190)."""
+        self._ensure(params)
+        r, single = self._walkers(electrons)
+        B = r.shape[0]
+        lp = torch.empty(B, dtype=torch.float64, device=self.device)
+        sg = torch.empty(B, dtype=torch.float64, device=self.device)
+        _lib.check(self._lib.fire_signed(self._h, _stream_ptr(), _dptr(r), B, self._prec(precision), _dptr(lp),
+                                         _dptr(sg)))
+        return (sg[0], lp[0]) if single else (sg, lp)
+
+    def __call__(self, params, electrons, static=None, precision=None):
+        return self.signed(params, electrons, static, precision)[1]
+
+    def local_energy_terms(self, params, electrons, static=None, precision=None, with_derivatives=False):
+        """dict with e_loc, e_kin, e_pot, logpsi, sign (and grad [.., N, 3], lap of log|psi|)."""
+        self._ensure(params)
+        r, single = self._walkers(electrons)
+        B, N = r.shape[0], self.cfg.n_el
+        out = {k: torch.empty(B, dtype=torch.float64, device=self.device)
+               for k in ("e_loc", "e_kin", "e_pot", "logpsi", "sign")}
+        if with_derivatives:
+            out["grad"] = torch.empty((B, N, 3), dtype=torch.float64, device=self.device)
+            out["lap"] = torch.empty(B, dtype=torch.float64, device=self.device)
+        _lib.check(self._lib.fire_local_energy(
+            self._h, _stream_ptr(), _dptr(r), B, self._prec(precision), _dptr(out["e_loc"]), _dptr(out["e_kin"]),
+            _dptr(out["e_pot"]), _dptr(out["logpsi"]), _dptr(out["sign"]), _dptr(out.get("grad")),
+            _dptr(out.get("lap"))))
+        if single:
+            out = {k: v[0] for k, v in out.items()}
+        return out
+
+    def kinetic_energy(self, params, electrons, static=None, precision=None):
+        """-1/2 (Laplacian + |grad|^2) of log|psi| -- wave_function.py:215-221."""
+        return self.local_energy_terms(params, electrons, static, precision)["e_kin"]
+
+    def embedding(self, params, electrons, precision="fp32", unscaled_out=False):
+        """(h after message passing, h_out) [B, N, F] -- NewSparseEmbedding.apply
+        (new_sparse_model.py:483-551).  ``unscaled_out`` returns the update-MLP output before
+        the scale s1 (what init's second normalize() sees, :537)."""
+        self._ensure(params)
+        r, single = self._walkers(electrons)
+        B, N, F = r.shape[0], self.cfg.n_el, self.cfg.feature_dim
+        hc = torch.empty((B, N, F), dtype=torch.float32, device=self.device)
+        ho = torch.empty((B, N, F), dtype=torch.float32, device=self.device)
+        _lib.check(self._lib.fire_embedding(self._h, _stream_ptr(), _dptr(r), B, self._prec(precision), _dptr(hc),
+                                            _dptr(ho)))
+        if unscaled_out:
+            ho = ho / float(params["embedding"]["scales"][1])
+        return (hc[0], ho[0]) if single else (hc, ho)
+
+
+def make_local_energy(wf: FireWaveFunction, energy_operator: str = "sparse", precision: Optional[str] = None):
+    """make_local_energy (hamiltonian.py:49-81) for all-electron molecules: returns
+    ``local_energy(key, params, electrons, static) -> (E_loc, static)``."""
+    if energy_operator.lower() != "sparse":
+        raise ValueError("only the sparse forward-Laplacian energy operator is provided")
+
+    def local_energy(key, params, electrons, static=None):
+        terms = wf.local_energy_terms(params, electrons, static, precision)
+        return terms["e_loc"], static
+
+    return local_energy

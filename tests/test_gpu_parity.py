@@ -1,0 +1,126 @@
+"""HIP path vs the CPU oracle on identical parameters and walkers (float64 oracle).
+
+Tolerances: neighbour lists bit-exact; FP32 kernels within the reference's own FP32
+tolerances (tests/test_fwd_lap.py: relative 1e-5 for embeddings, 1e-2 for E_kin --
+tightened here where the measured error allows); mixed precision (BF16 MFMA) within the
+looser bounds stated per test.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import fire
+from sparse_wf_amd.wave_function import FireWaveFunction
+
+from cases import make_params, small_cfg, walkers, water_cfg
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+CASES = {
+    "small": lambda: small_cfg(False),
+    "small_restricted": lambda: small_cfg(True),
+    "water": lambda: water_cfg(),
+}
+
+
+@pytest.fixture(scope="module", params=list(CASES))
+def setup(request, gpu_device):
+    cfg = CASES[request.param]()
+    prm = make_params(cfg, seed=0)
+    wf = FireWaveFunction(cfg, device=gpu_device)
+    wf.upload(prm)
+    r = walkers(cfg, 4, seed=3)
+    print(f"\n[case {request.param}] N={cfg.n_el} F={cfg.feature_dim} D={cfg.n_determinants}", flush=True)
+    return request.param, cfg, prm, wf, r
+
+
+def test_neighbours_bit_exact(setup):
+    name, cfg, prm, wf, r = setup
+    counts, nbr, ptr = wf.neighbours(r)
+    counts = counts.cpu().numpy()
+    nbr = nbr.cpu().numpy()
+    p = 0
+    for w in range(len(r)):
+        nl = fire.neighbour_lists(r[w], cfg.n_up, cfg.cutoff)
+        np.testing.assert_array_equal(counts[w], np.diff(nl.offsets))
+        np.testing.assert_array_equal(nbr[p:p + nl.n_pairs], nl.nbr)
+        p += nl.n_pairs
+    assert p == len(nbr)
+
+
+def test_neighbours_cutoff_boundary(gpu_device):
+    """Pairs at exactly the cutoff are excluded (strict <), one ulp inside is included."""
+    cfg = small_cfg(False)
+    prm = make_params(cfg, seed=0)
+    wf = FireWaveFunction(cfg, device=gpu_device)
+    wf.upload(prm)
+    r = walkers(cfg, 2, seed=5).copy()
+    c = np.float32(cfg.cutoff)
+    r[0, 1] = r[0, 0] + np.array([c, 0, 0], np.float32)
+    r[1, 1] = r[1, 0] + np.array([np.nextafter(c, np.float32(0)), 0, 0], np.float32)
+    counts, nbr, _ = wf.neighbours(r)
+    counts = counts.cpu().numpy()
+    for w in range(2):
+        nl = fire.neighbour_lists(r[w], cfg.n_up, cfg.cutoff)
+        np.testing.assert_array_equal(counts[w], np.diff(nl.offsets))
+    d0 = fire.pair_distances_f32(r[0])[0, 1]
+    d1 = fire.pair_distances_f32(r[1])[0, 1]
+    assert d0 == c and d1 < c
+
+
+@pytest.mark.parametrize("precision,tol", [("fp32", 2e-5), ("mixed", 3e-2)])
+def test_embedding(setup, precision, tol):
+    name, cfg, prm, wf, r = setup
+    pt = fire.to_torch(prm)
+    hc, ho = wf.embedding(prm, r, precision=precision)
+    for w in range(len(r)):
+        ref = fire.embedding_apply(pt, cfg, r[w]).numpy()
+        e = _rel(ho[w].cpu().numpy(), ref)
+        print(f"  embedding {precision} walker {w}: rel err {e:.2e}")
+        assert e < tol
+
+
+@pytest.mark.parametrize("precision,tol", [("fp32", 1e-4), ("mixed", 5e-2)])
+def test_signed(setup, precision, tol):
+    name, cfg, prm, wf, r = setup
+    pt = fire.to_torch(prm)
+    sg, lp = wf.signed(prm, r, precision=precision)
+    for w in range(len(r)):
+        s_ref, l_ref = fire.signed(pt, cfg, r[w])
+        d = abs(float(lp[w]) - float(l_ref))
+        print(f"  logpsi {precision} walker {w}: {float(lp[w]):.8f} vs {float(l_ref):.8f} (|d| {d:.2e})")
+        assert float(sg[w]) == float(s_ref)
+        assert d < tol * max(1.0, abs(float(l_ref)))
+
+
+@pytest.mark.parametrize("precision", ["fp32", "mixed"])
+def test_local_energy(setup, precision):
+    name, cfg, prm, wf, r = setup
+    pt = fire.to_torch(prm)
+    out = wf.local_energy_terms(prm, r, precision=precision, with_derivatives=True)
+    for w in range(len(r)):
+        sign, lp = fire.log_psi_fwd_lap(pt, cfg, r[w])
+        ek_ref = float(-0.5 * (lp.l + (lp.j * lp.j).sum()))
+        ep_ref = fire.potential_energy(r[w], cfg.mol.R, cfg.mol.Z)
+        g = out["grad"][w].cpu().numpy().reshape(-1)
+        eg = _rel(g, lp.j.numpy())
+        el = abs(float(out["lap"][w]) - float(lp.l)) / max(1.0, abs(float(lp.l)))
+        ek = abs(float(out["e_kin"][w]) - ek_ref) / max(1.0, abs(ek_ref))
+        print(f"  {precision} walker {w}: logpsi {float(out['logpsi'][w]):.6f}/{float(lp.x):.6f} "
+              f"grad rel {eg:.2e} lap {float(out['lap'][w]):.6f}/{float(lp.l):.6f} "
+              f"E_kin {float(out['e_kin'][w]):.6f}/{ek_ref:.6f} (rel {ek:.2e}) E_pot {float(out['e_pot'][w]):.6f}/{ep_ref:.6f}")
+        assert abs(float(out["e_pot"][w]) - ep_ref) < 1e-9 * max(1.0, abs(ep_ref))
+        assert float(out["sign"][w]) == float(sign)
+        if precision == "fp32":
+            assert eg < 1e-4 and el < 1e-3 and ek < 1e-3
+        else:
+            assert eg < 5e-2 and el < 5e-2 and ek < 5e-2
