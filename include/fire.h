@@ -40,8 +40,10 @@ typedef struct fire_config {
 } fire_config;
 
 enum fire_precision {
-  FIRE_FP32 = 0,  /* every embedding / orbital / Jastrow contraction in FP32 */
-  FIRE_MIXED = 1, /* BF16 MFMA with FP32 accumulation for the dense contractions */
+  FIRE_FP32 = 0,  /* every embedding / orbital / Jastrow contraction in FP32 (FP32 MFMA) */
+  FIRE_MIXED = 1, /* dense contractions as split BF16 (hi*hi + hi*lo + lo*hi) on BF16 MFMA,
+                     FP32 accumulation; element-wise rules FP32; determinants FP64 */
+  FIRE_BF16 = 2,  /* dense contractions in single BF16 on MFMA (fastest, least accurate) */
 };
 
 /* Creates a handle for molecule (R [n_nuc*3] bohr, Z [n_nuc]) on HIP device `device`.
@@ -86,6 +88,12 @@ int fire_signed(fire_handle* h, void* stream, const float* r_dev, int32_t B, int
  * the first normalize() used by init (:535, :378). */
 int fire_embedding(fire_handle* h, void* stream, const float* r_dev, int32_t B, int32_t precision,
                    float* h_contr_dev, float* h_out_dev);
+
+/* Instrumentation (no counterpart in the reference): enables / disables HIP-event timing of
+ * every dense-contraction launch of this handle.  Each call first reports what was recorded
+ * since the previous call -- total kernel milliseconds, algorithmic FLOPs (2*M*N*K per
+ * launch) and the launch count -- then resets and applies `enable`.  Synchronises. */
+int fire_profile_gemm(fire_handle* h, int32_t enable, double* total_ms, double* total_flops, int64_t* n_launches);
 
 /* Message of the last failure on this thread ("" if none). */
 const char* fire_last_error(void);

@@ -421,6 +421,57 @@ def log_psi_value_torch(p, cfg, r, nl_r32):
     return _signed_torch(p, cfg, r, nl_r32)[1]
 
 
+def _signed_torch_embedding(p, cfg, r, r32):
+    """Embedding h_out as a differentiable function of ``r`` (graph frozen at ``r32``)."""
+    return _embed_torch(p, cfg, r, r32)
+
+
+def _orbitals_torch(p, cfg, r, r32):
+    """Slater matrices [D, N, N] as a differentiable function of ``r``."""
+    h = _embed_torch(p, cfg, r, r32)
+    return _phi_torch(p, cfg, r, h)
+
+
+def _embed_torch(p, cfg, r, r32):
+    dtype = r.dtype
+    R = torch.as_tensor(np.asarray(cfg.mol.R, np.float64)).to(dtype)
+    nl = neighbour_lists(r32, cfg.n_up, cfg.cutoff)
+    h0, mcs, mcd, mns, mnd = elec_init(p, cfg, r, R, _en_mask(r32, cfg.mol.R, cfg.cutoff_1el))
+    cut = learned_cutoff(p, cfg)
+    h = h0
+    ep = p["embedding"]
+    for same, edge_p, mct, mnb in ((True, ep["edge_same"], mcs, mns), (False, ep["edge_diff"], mcd, mnd)):
+        sel = nl.same == same
+        ci, ni = torch.as_tensor(nl.ctr[sel]), torch.as_tensor(nl.nbr[sel])
+        if len(ci) == 0:
+            continue
+        gamma, e = ee_edge(edge_p, cut, r[ci] - r[ni])
+        h = h.index_add(0, ci, gamma * fl.silu(e + mct[ci] + mnb[ni]))
+    s0, s1 = ep["scales"]
+    up = ep["updates"][0]["params"]
+    x = fl.silu(fl.linear(h * s0, up["Linear_0"]["kernel"], up["Linear_0"]["bias"]))
+    x = fl.silu(fl.linear(x, up["Linear_1"]["kernel"], up["Linear_1"]["bias"]))
+    return x * s1
+
+
+def _phi_torch(p, cfg, r, h):
+    dtype = r.dtype
+    R = torch.as_tensor(np.asarray(cfg.mol.R, np.float64)).to(dtype)
+    op = p["to_orbitals"]["params"]
+    N, n_up, D = cfg.n_el, cfg.n_up, cfg.n_determinants
+    upw = op["to_orbitals_up"]
+    orb_up = h[:n_up] @ upw["kernel"] + upw["bias"]
+    env_up = envelopes_iso(op["env_up"], cfg, r[:n_up], R)
+    if cfg.spin_restricted:
+        orb_dn = _swap_updn(h[n_up:] @ upw["kernel"] + upw["bias"], N, n_up)
+        env_dn = _swap_updn(envelopes_iso(op["env_up"], cfg, r[n_up:], R), N, n_up)
+    else:
+        dn = op["to_orbitals_dn"]
+        orb_dn = h[n_up:] @ dn["kernel"] + dn["bias"]
+        env_dn = envelopes_iso(op["env_dn"], cfg, r[n_up:], R)
+    return (torch.cat([orb_up, orb_dn]) * torch.cat([env_up, env_dn])).reshape(N, D, N).permute(1, 0, 2)
+
+
 def _signed_torch(p, cfg, r, r32):
     dtype = r.dtype
     R = torch.as_tensor(np.asarray(cfg.mol.R, np.float64)).to(dtype)

@@ -12,6 +12,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libfire_amd.so"
 
 FIRE_FP32 = 0
 FIRE_MIXED = 1
+FIRE_BF16 = 2
 
 
 class FireConfigC(ctypes.Structure):
@@ -39,6 +40,8 @@ SIGNATURES = {
     "fire_local_energy": (ctypes.c_int, [_P, _P, _P, ctypes.c_int32, ctypes.c_int32, _P, _P, _P, _P, _P, _P, _P]),
     "fire_signed": (ctypes.c_int, [_P, _P, _P, ctypes.c_int32, ctypes.c_int32, _P, _P]),
     "fire_embedding": (ctypes.c_int, [_P, _P, _P, ctypes.c_int32, ctypes.c_int32, _P, _P]),
+    "fire_profile_gemm": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.POINTER(ctypes.c_double),
+                                         ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]),
     "fire_last_error": (ctypes.c_char_p, []),
 }
 

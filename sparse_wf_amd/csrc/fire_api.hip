@@ -76,7 +76,8 @@ struct fire_handle {
   // parameters
   std::vector<int64_t> expect;  // element counts in canonical order
   std::vector<float*> P;
-  std::vector<uint16_t*> Pbf;   // BF16 transposed copies of GEMM weights (null if none)
+  std::vector<uint16_t*> Pbf;   // BF16 transposed copies of GEMM weights (null if none): hi part
+  std::vector<uint16_t*> Pbf_lo;  // ... and the BF16 remainder W - hi
   std::vector<std::pair<int, int>> shape2;  // (K, N) of GEMM weights
   float cut_learned = 0.f;
   float s1 = 1.f;
@@ -92,6 +93,10 @@ struct fire_handle {
   DevBuf cval, clap, cgrad, epot;
   int chunk = 256;
   size_t smem_set = 0;
+  // optional HIP-event timing of the dense contractions (fire_profile_gemm)
+  bool gemm_timing = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> gemm_events;
+  double gemm_flops = 0.0;
 };
 
 namespace {
@@ -190,10 +195,22 @@ void gemm(Ctx& c, const float* A, int64_t lda, int wi, int bi, float* C, int64_t
   fire_handle* h = c.h;
   const float* bias = bi >= 0 ? h->P[bi] : nullptr;
   const bool bf_ok = h->Pbf[wi] != nullptr && (K % 32) == 0 && (lda % 4) == 0 && N > 16;
-  if (c.prec == FIRE_MIXED && bf_ok)
-    launch_gemm_bf16(c.s, A, lda, h->Pbf[wi], bias, C, ldc, M, N, K, rowflag, period);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (h->gemm_timing) {
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, c.s));
+  }
+  if ((c.prec == FIRE_MIXED || c.prec == FIRE_BF16) && bf_ok)
+    launch_gemm_bf16(c.s, A, lda, h->Pbf[wi], h->Pbf_lo[wi], bias, C, ldc, M, N, K, rowflag, period,
+                     c.prec == FIRE_MIXED ? 3 : 1);
   else
     launch_gemm_f32(c.s, A, lda, h->P[wi], bias, C, ldc, M, N, K, rowflag, period);
+  if (h->gemm_timing) {
+    HIPCHK(hipEventRecord(e1, c.s));
+    h->gemm_events.push_back({e0, e1});
+    h->gemm_flops += 2.0 * (double)M * N * K;
+  }
 }
 
 EnParams en_params(fire_handle* h) {
@@ -559,6 +576,8 @@ int fire_destroy(fire_handle* h) {
     if (p) (void)hipFree(p);
   for (uint16_t* p : h->Pbf)
     if (p) (void)hipFree(p);
+  for (uint16_t* p : h->Pbf_lo)
+    if (p) (void)hipFree(p);
   if (h->R) (void)hipFree(h->R);
   if (h->Z) (void)hipFree(h->Z);
   DevBuf* bufs[] = {&h->cnt, &h->ptr, &h->nbr, &h->rev, &h->rowflag, &h->T5, &h->Z2, &h->T5b, &h->M5,
@@ -584,8 +603,11 @@ int fire_upload_params(fire_handle* h, const float* packed, const int64_t* sizes
       if (p) HIPCHK(hipFree(p));
     for (uint16_t* p : h->Pbf)
       if (p) HIPCHK(hipFree(p));
+    for (uint16_t* p : h->Pbf_lo)
+      if (p) HIPCHK(hipFree(p));
     h->P.assign(n_tensors, nullptr);
     h->Pbf.assign(n_tensors, nullptr);
+    h->Pbf_lo.assign(n_tensors, nullptr);
     size_t off = 0;
     for (int k = 0; k < n_tensors; ++k) {
       HIPCHK(hipMalloc(&h->P[k], sizeof(float) * std::max<int64_t>(sizes[k], 1)));
@@ -602,16 +624,51 @@ int fire_upload_params(fire_handle* h, const float* packed, const int64_t* sizes
     }
     for (auto& t : gw) {
       const int k = t.first, K = t.second.first, Nn = t.second.second;
-      std::vector<uint16_t> tb((size_t)K * Nn);
+      std::vector<uint16_t> tb((size_t)K * Nn), tl((size_t)K * Nn);
       const float* src = packed + offs[k];
       for (int kk = 0; kk < K; ++kk)
-        for (int n = 0; n < Nn; ++n) tb[(size_t)n * K + kk] = host_f2bf(src[(size_t)kk * Nn + n]);
+        for (int n = 0; n < Nn; ++n) {
+          const float x = src[(size_t)kk * Nn + n];
+          const uint16_t hi = host_f2bf(x);
+          uint32_t u = (uint32_t)hi << 16;
+          float xh;
+          std::memcpy(&xh, &u, 4);
+          tb[(size_t)n * K + kk] = hi;
+          tl[(size_t)n * K + kk] = host_f2bf(x - xh);
+        }
       HIPCHK(hipMalloc(&h->Pbf[k], sizeof(uint16_t) * tb.size()));
       HIPCHK(hipMemcpy(h->Pbf[k], tb.data(), sizeof(uint16_t) * tb.size(), hipMemcpyHostToDevice));
+      HIPCHK(hipMalloc(&h->Pbf_lo[k], sizeof(uint16_t) * tl.size()));
+      HIPCHK(hipMemcpy(h->Pbf_lo[k], tl.data(), sizeof(uint16_t) * tl.size(), hipMemcpyHostToDevice));
     }
     h->cut_learned = packed[offs[h->iSCAL]];
     h->s1 = packed[offs[h->iSCAL] + 1];
     h->params_ok = true;
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e);
+  }
+}
+
+int fire_profile_gemm(fire_handle* h, int32_t enable, double* total_ms, double* total_flops, int64_t* n_launches) {
+  try {
+    REQUIRE(h, "null handle");
+    HIPCHK(hipSetDevice(h->device));
+    double ms = 0.0;
+    for (auto& e : h->gemm_events) {
+      HIPCHK(hipEventSynchronize(e.second));
+      float t = 0.f;
+      HIPCHK(hipEventElapsedTime(&t, e.first, e.second));
+      ms += t;
+      HIPCHK(hipEventDestroy(e.first));
+      HIPCHK(hipEventDestroy(e.second));
+    }
+    if (total_ms) *total_ms = ms;
+    if (total_flops) *total_flops = h->gemm_flops;
+    if (n_launches) *n_launches = (int64_t)h->gemm_events.size();
+    h->gemm_events.clear();
+    h->gemm_flops = 0.0;
+    h->gemm_timing = enable != 0;
     return 0;
   } catch (const std::exception& e) {
     return fail(e);
@@ -668,7 +725,7 @@ int fire_local_energy(fire_handle* h, void* stream, const float* r_dev, int32_t 
   try {
     check_ready(h);
     REQUIRE(r_dev && e_loc_dev, "null argument");
-    REQUIRE(precision == FIRE_FP32 || precision == FIRE_MIXED, "unknown precision mode");
+    REQUIRE(precision == FIRE_FP32 || precision == FIRE_MIXED || precision == FIRE_BF16, "unknown precision mode");
     HIPCHK(hipSetDevice(h->device));
     Ctx c{h, (hipStream_t)stream, precision};
     const int N = h->N;
@@ -697,7 +754,7 @@ int fire_signed(fire_handle* h, void* stream, const float* r_dev, int32_t B, int
   try {
     check_ready(h);
     REQUIRE(r_dev && logpsi_dev, "null argument");
-    REQUIRE(precision == FIRE_FP32 || precision == FIRE_MIXED, "unknown precision mode");
+    REQUIRE(precision == FIRE_FP32 || precision == FIRE_MIXED || precision == FIRE_BF16, "unknown precision mode");
     HIPCHK(hipSetDevice(h->device));
     Ctx c{h, (hipStream_t)stream, precision};
     const int N = h->N;

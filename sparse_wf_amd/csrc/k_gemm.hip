@@ -5,34 +5,58 @@
 // value row: sparse_fwd_lap.py:160-181).  Value rows are marked by a per-row flag or by
 // row % period == 0.
 //
-// MFMA kernel: 256 threads = 4 wavefronts, 128x128 output tile, each wavefront a 64x64
-// quarter made of 2x2 32x32 MFMA tiles; K staged through LDS 32 at a time.
-//   FP32 path : v_mfma_f32_32x32x2f32 (A, W in FP32)
-//   MIXED path: v_mfma_f32_32x32x8bf16_1k (A rounded to BF16 on load, W pre-rounded), FP32 accumulate.
+// MFMA kernels: 256 threads = 4 wavefronts, 128x128 output tile, each wavefront a 64x64
+// quarter made of 2x2 32x32 MFMA tiles; K staged through LDS 32 at a time with the next
+// chunk prefetched into registers while the current one is multiplied.
+//   FP32 path : v_mfma_f32_32x32x2f32, A and W in FP32.
+//   BF16 path : v_mfma_f32_32x32x16_bf16, FP32 accumulation.  NSPLIT = 3 carries each
+//               operand as hi + lo BF16 pair (x = hi + lo exactly to ~2^-17 relative) and
+//               accumulates hi*hi + hi*lo + lo*hi ("split BF16"), NSPLIT = 1 is plain BF16.
 #include "fire_common.hpp"
 #include "fire_kernels.hpp"
 
 namespace fire {
 
-typedef short short4_t __attribute__((ext_vector_type(4)));
 typedef float float16_t __attribute__((ext_vector_type(16)));
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 
 __device__ __forceinline__ uint16_t f2bf(float x) {
   uint32_t u = __float_as_uint(x);
   u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
   return (uint16_t)(u >> 16);
 }
+__device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 
 #define GT_M 128
 #define GT_N 128
 #define GT_K 32
-#define LDA_S (GT_K + 4)   // fp32 LDS row stride (floats)
-#define LDB_S (GT_K + 8)   // bf16 LDS row stride (halves)
+#define LDA_F (GT_K + 4)  // fp32 LDS row stride (floats)
+#define LDB_H (GT_K + 8)  // bf16 LDS row stride (halves): 80 bytes, keeps 16-byte alignment
 
 __device__ __forceinline__ bool is_bias_row(int64_t row, const uint8_t* rowflag, int period) {
   if (rowflag) return rowflag[row] != 0;
   if (period > 0) return (row % period) == 0;
   return false;
+}
+
+template <typename AccT>
+__device__ __forceinline__ void store_tile(const AccT (&acc)[2][2], int64_t m0, int n0, int wm, int wn, int lane,
+                                           float* __restrict__ C, int64_t ldc, int64_t M, int N,
+                                           const float* __restrict__ bias, const uint8_t* __restrict__ rowflag,
+                                           int period) {
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b) {
+      const int col = n0 + wn + b * 32 + (lane & 31);
+      if (col >= N) continue;
+      const float bcol = bias ? bias[col] : 0.f;
+      for (int v = 0; v < 16; ++v) {
+        const int64_t row = m0 + wm + a * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+        if (row >= M) continue;
+        float val = acc[a][b][v];
+        if (bias && is_bias_row(row, rowflag, period)) val += bcol;
+        C[row * ldc + col] = val;
+      }
+    }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -42,8 +66,8 @@ __global__ void __launch_bounds__(256) k_gemm_f32(const float* __restrict__ A, i
                                                   const float* __restrict__ W, const float* __restrict__ bias,
                                                   float* __restrict__ C, int64_t ldc, int64_t M, int N, int K,
                                                   const uint8_t* __restrict__ rowflag, int period) {
-  __shared__ float As[GT_M][LDA_S];
-  __shared__ float Bs[GT_N][LDA_S];  // transposed: Bs[n][k]
+  __shared__ float As[GT_M][LDA_F];
+  __shared__ float Bs[GT_N][LDA_F];  // transposed: Bs[n][k]
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   const int64_t m0 = (int64_t)blockIdx.x * GT_M;
   const int n0 = blockIdx.y * GT_N;
@@ -52,18 +76,29 @@ __global__ void __launch_bounds__(256) k_gemm_f32(const float* __restrict__ A, i
   for (int a = 0; a < 2; ++a)
     for (int b = 0; b < 2; ++b)
       for (int v = 0; v < 16; ++v) acc[a][b][v] = 0.f;
-
+  const bool vecA = (K % 4) == 0 && (lda % 4) == 0;
   for (int k0 = 0; k0 < K; k0 += GT_K) {
-    // A tile 128x32: 4096 floats, 16 per thread
-    for (int e = tid; e < GT_M * GT_K; e += 256) {
-      const int rr = e / GT_K, kk = e % GT_K;
+    // A tile 128 x 32 (4 floats per load, 4 loads per thread)
+    for (int e = tid; e < GT_M * GT_K / 4; e += 256) {
+      const int rr = e >> 3, kq = (e & 7) * 4;
       const int64_t gr = m0 + rr;
-      const int gk = k0 + kk;
-      As[rr][kk] = (gr < M && gk < K) ? A[gr * lda + gk] : 0.f;
+      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (gr < M) {
+        if (vecA && k0 + kq + 3 < K) {
+          v = *reinterpret_cast<const float4*>(A + gr * lda + k0 + kq);
+        } else {
+          const float* ap = A + gr * lda;
+          v.x = k0 + kq < K ? ap[k0 + kq] : 0.f;
+          v.y = k0 + kq + 1 < K ? ap[k0 + kq + 1] : 0.f;
+          v.z = k0 + kq + 2 < K ? ap[k0 + kq + 2] : 0.f;
+          v.w = k0 + kq + 3 < K ? ap[k0 + kq + 3] : 0.f;
+        }
+      }
+      *reinterpret_cast<float4*>(&As[rr][kq]) = v;
     }
-    // W tile 32x128 -> Bs[n][k]
+    // W tile 32 x 128 -> Bs[n][k]
     for (int e = tid; e < GT_N * GT_K; e += 256) {
-      const int kk = e / GT_N, nn = e % GT_N;
+      const int kk = e >> 7, nn = e & 127;
       const int gk = k0 + kk, gn = n0 + nn;
       Bs[nn][kk] = (gk < K && gn < N) ? W[(int64_t)gk * N + gn] : 0.f;
     }
@@ -79,30 +114,22 @@ __global__ void __launch_bounds__(256) k_gemm_f32(const float* __restrict__ A, i
     }
     __syncthreads();
   }
-  for (int a = 0; a < 2; ++a)
-    for (int b = 0; b < 2; ++b) {
-      const int col = n0 + wn + b * 32 + (lane & 31);
-      if (col >= N) continue;
-      const float bcol = bias ? bias[col] : 0.f;
-      for (int v = 0; v < 16; ++v) {
-        const int64_t row = m0 + wm + a * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
-        if (row >= M) continue;
-        float val = acc[a][b][v];
-        if (bias && is_bias_row(row, rowflag, period)) val += bcol;
-        C[row * ldc + col] = val;
-      }
-    }
+  store_tile(acc, m0, n0, wm, wn, lane, C, ldc, M, N, bias, rowflag, period);
 }
 
 // ---------------------------------------------------------------------------------------
-// BF16 MFMA GEMM, FP32 accumulation.  Wt is [N,K] BF16 (W transposed, pre-rounded).
+// BF16 / split-BF16 MFMA GEMM, FP32 accumulation.  Whi/Wlo are [N,K] BF16 (W transposed).
+// Requires K % 32 == 0 and lda % 4 == 0 (checked by the host).
 // ---------------------------------------------------------------------------------------
+template <int NSPLIT>
 __global__ void __launch_bounds__(256) k_gemm_bf16(const float* __restrict__ A, int64_t lda,
-                                                   const uint16_t* __restrict__ Wt, const float* __restrict__ bias,
-                                                   float* __restrict__ C, int64_t ldc, int64_t M, int N, int K,
-                                                   const uint8_t* __restrict__ rowflag, int period) {
-  __shared__ uint16_t As[GT_M][LDB_S];
-  __shared__ uint16_t Bs[GT_N][LDB_S];
+                                                   const uint16_t* __restrict__ Whi, const uint16_t* __restrict__ Wlo,
+                                                   const float* __restrict__ bias, float* __restrict__ C, int64_t ldc,
+                                                   int64_t M, int N, int K, const uint8_t* __restrict__ rowflag,
+                                                   int period) {
+  constexpr int NP = NSPLIT == 3 ? 2 : 1;  // hi (+ lo) planes
+  __shared__ __attribute__((aligned(16))) uint16_t As[NP][GT_M][LDB_H];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[NP][GT_N][LDB_H];
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   const int64_t m0 = (int64_t)blockIdx.x * GT_M;
   const int n0 = blockIdx.y * GT_N;
@@ -112,52 +139,88 @@ __global__ void __launch_bounds__(256) k_gemm_bf16(const float* __restrict__ A, 
     for (int b = 0; b < 2; ++b)
       for (int v = 0; v < 16; ++v) acc[a][b][v] = 0.f;
 
-  for (int k0 = 0; k0 < K; k0 += GT_K) {
-    // A tile: 128 rows x 32 k, float4 loads (K is a multiple of 32, lda of 4)
-    for (int e = tid; e < GT_M * GT_K / 4; e += 256) {
-      const int rr = e / (GT_K / 4), kq = (e % (GT_K / 4)) * 4;
-      const int64_t gr = m0 + rr;
-      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (gr < M) v = *reinterpret_cast<const float4*>(A + gr * lda + k0 + kq);
-      As[rr][kq + 0] = f2bf(v.x);
-      As[rr][kq + 1] = f2bf(v.y);
-      As[rr][kq + 2] = f2bf(v.z);
-      As[rr][kq + 3] = f2bf(v.w);
-    }
-    // W tile: 128 n x 32 k from Wt[n][k], 8-byte loads
-    for (int e = tid; e < GT_N * GT_K / 4; e += 256) {
-      const int nn = e / (GT_K / 4), kq = (e % (GT_K / 4)) * 4;
-      const int gn = n0 + nn;
-      uint2 v = make_uint2(0u, 0u);
-      if (gn < N) v = *reinterpret_cast<const uint2*>(Wt + (int64_t)gn * K + k0 + kq);
-      *reinterpret_cast<uint2*>(&Bs[nn][kq]) = v;
-    }
-    __syncthreads();
+  // per-thread load slots: A 128 rows x 32 k = 1024 float4 -> 4 per thread;
+  //                        W 128 n x 32 k = 512 x (8 bf16) -> 2 per thread per plane
+  float4 ra[4];
+  uint4 rb[NP][2];
+  auto load = [&](int k0) {
 #pragma unroll
-    for (int kk = 0; kk < GT_K; kk += 8) {
-      const int ka = kk + 4 * (lane >> 5);
-      short4_t av[2], bv[2];
-      for (int a = 0; a < 2; ++a) av[a] = *reinterpret_cast<const short4_t*>(&As[wm + a * 32 + (lane & 31)][ka]);
-      for (int b = 0; b < 2; ++b) bv[b] = *reinterpret_cast<const short4_t*>(&Bs[wn + b * 32 + (lane & 31)][ka]);
+    for (int q = 0; q < 4; ++q) {
+      const int e = tid + q * 256;
+      const int rr = e >> 3, kq = (e & 7) * 4;
+      const int64_t gr = m0 + rr;
+      ra[q] = gr < M ? *reinterpret_cast<const float4*>(A + gr * lda + k0 + kq) : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int e = tid + q * 256;
+      const int nn = e >> 2, kq = (e & 3) * 8;
+      const int gn = n0 + nn;
+      rb[0][q] = gn < N ? *reinterpret_cast<const uint4*>(Whi + (int64_t)gn * K + k0 + kq) : make_uint4(0, 0, 0, 0);
+      if (NP == 2)
+        rb[NP - 1][q] = gn < N ? *reinterpret_cast<const uint4*>(Wlo + (int64_t)gn * K + k0 + kq) : make_uint4(0, 0, 0, 0);
+    }
+  };
+  auto store = [&]() {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int e = tid + q * 256;
+      const int rr = e >> 3, kq = (e & 7) * 4;
+      const float x[4] = {ra[q].x, ra[q].y, ra[q].z, ra[q].w};
+      uint16_t hi[4], lo[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        hi[j] = f2bf(x[j]);
+        lo[j] = f2bf(x[j] - bf2f(hi[j]));
+      }
+      *reinterpret_cast<uint2*>(&As[0][rr][kq]) =
+          make_uint2((uint32_t)hi[0] | ((uint32_t)hi[1] << 16), (uint32_t)hi[2] | ((uint32_t)hi[3] << 16));
+      if (NP == 2)
+        *reinterpret_cast<uint2*>(&As[NP - 1][rr][kq]) =
+            make_uint2((uint32_t)lo[0] | ((uint32_t)lo[1] << 16), (uint32_t)lo[2] | ((uint32_t)lo[3] << 16));
+    }
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int e = tid + q * 256;
+      const int nn = e >> 2, kq = (e & 3) * 8;
+#pragma unroll
+      for (int p = 0; p < NP; ++p) *reinterpret_cast<uint4*>(&Bs[p][nn][kq]) = rb[p][q];
+    }
+  };
+
+  load(0);
+  for (int k0 = 0; k0 < K; k0 += GT_K) {
+    store();
+    __syncthreads();
+    if (k0 + GT_K < K) load(k0 + GT_K);
+#pragma unroll
+    for (int kk = 0; kk < GT_K; kk += 16) {
+      const int ka = kk + 8 * (lane >> 5);
+      bf16x8_t ah[2], bh[2], al[2], bl[2];
+#pragma unroll
+      for (int a = 0; a < 2; ++a) ah[a] = *reinterpret_cast<const bf16x8_t*>(&As[0][wm + a * 32 + (lane & 31)][ka]);
+#pragma unroll
+      for (int b = 0; b < 2; ++b) bh[b] = *reinterpret_cast<const bf16x8_t*>(&Bs[0][wn + b * 32 + (lane & 31)][ka]);
+      if (NP == 2) {
+#pragma unroll
+        for (int a = 0; a < 2; ++a) al[a] = *reinterpret_cast<const bf16x8_t*>(&As[NP - 1][wm + a * 32 + (lane & 31)][ka]);
+#pragma unroll
+        for (int b = 0; b < 2; ++b) bl[b] = *reinterpret_cast<const bf16x8_t*>(&Bs[NP - 1][wn + b * 32 + (lane & 31)][ka]);
+      }
+#pragma unroll
       for (int a = 0; a < 2; ++a)
-        for (int b = 0; b < 2; ++b)
-          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(av[a], bv[b], acc[a][b], 0, 0, 0);
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+          if (NP == 2) {
+            acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[a], bh[b], acc[a][b], 0, 0, 0);
+            acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a], bl[b], acc[a][b], 0, 0, 0);
+          }
+          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a], bh[b], acc[a][b], 0, 0, 0);
+        }
     }
     __syncthreads();
   }
-  for (int a = 0; a < 2; ++a)
-    for (int b = 0; b < 2; ++b) {
-      const int col = n0 + wn + b * 32 + (lane & 31);
-      if (col >= N) continue;
-      const float bcol = bias ? bias[col] : 0.f;
-      for (int v = 0; v < 16; ++v) {
-        const int64_t row = m0 + wm + a * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
-        if (row >= M) continue;
-        float val = acc[a][b][v];
-        if (bias && is_bias_row(row, rowflag, period)) val += bcol;
-        C[row * ldc + col] = val;
-      }
-    }
+  store_tile(acc, m0, n0, wm, wn, lane, C, ldc, M, N, bias, rowflag, period);
 }
 
 // Small-N FP32 GEMM (N <= 16): one wavefront per row, lanes across K.
@@ -192,11 +255,15 @@ void launch_gemm_f32(hipStream_t s, const float* A, int64_t lda, const float* W,
   k_gemm_f32<<<grid, 256, 0, s>>>(A, lda, W, bias, C, ldc, M, N, K, rowflag, period);
 }
 
-void launch_gemm_bf16(hipStream_t s, const float* A, int64_t lda, const uint16_t* Wt, const float* bias, float* C,
-                      int64_t ldc, int64_t M, int N, int K, const uint8_t* rowflag, int period) {
+void launch_gemm_bf16(hipStream_t s, const float* A, int64_t lda, const uint16_t* Whi, const uint16_t* Wlo,
+                      const float* bias, float* C, int64_t ldc, int64_t M, int N, int K, const uint8_t* rowflag,
+                      int period, int nsplit) {
   if (M == 0) return;
   dim3 grid((unsigned)((M + GT_M - 1) / GT_M), (N + GT_N - 1) / GT_N);
-  k_gemm_bf16<<<grid, 256, 0, s>>>(A, lda, Wt, bias, C, ldc, M, N, K, rowflag, period);
+  if (nsplit == 3)
+    k_gemm_bf16<3><<<grid, 256, 0, s>>>(A, lda, Whi, Wlo, bias, C, ldc, M, N, K, rowflag, period);
+  else
+    k_gemm_bf16<1><<<grid, 256, 0, s>>>(A, lda, Whi, Whi, bias, C, ldc, M, N, K, rowflag, period);
 }
 
 }  // namespace fire

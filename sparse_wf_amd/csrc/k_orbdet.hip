@@ -253,22 +253,30 @@ void launch_det_trace(hipStream_t s, const float* O, int ND, GroupDesc gd, int B
   k_det_trace<<<B * D, 256, 0, s>>>(O, ND, gd, B, N, n_up, D, Ainv, tr);
 }
 
-#define DL_OC 32
+#define DL_KC 32  // contraction chunk staged in LDS
+#define DL_TPW 12  // max 16x16 output tiles per wavefront (3*64 x 64 outputs / 4 waves)
+
+typedef float float4_t __attribute__((ext_vector_type(4)));
 
 // Triplet part of the determinant Laplacian and the determinant gradient for one
 // (walker, det, electron n): with S = {n} U N(n) and J_a the jacobian rows d Phi_a / d r_n,
-// B[a,x,b] = sum_o J_a[x,o] Ainv[o, S_b];  trip = sum B[a,x,b] B[b,x,a];  grad_x = sum_a B[a,x,a].
+//   B[a,x,b] = sum_o J_a[x,o] Ainv[o, S_b]       (FP32 MFMA 16x16x4, Ainv rounded from FP64)
+//   trip = sum B[a,x,b] B[b,x,a],  grad_x = sum_a B[a,x,a]   (FP64 sums).
+// Each jacobian row (i, n) belongs to exactly one set S_n, so the orbital jacobian is read
+// once.  Dynamic LDS: J chunk [RT*16][KC+4], X chunk [CT*16][KC+4], B [3s][s], indices.
 __global__ void __launch_bounds__(256) k_det_lap(const float* __restrict__ O, int ND, GroupDesc gd,
                                                  const int64_t* __restrict__ ptr, const int32_t* __restrict__ nbr,
                                                  const int32_t* __restrict__ rev, int B, int N, int n_up, int D,
                                                  int smax, const double* __restrict__ Ainv, double* __restrict__ trip,
                                                  double* __restrict__ grad) {
   extern __shared__ unsigned char smem_raw[];
-  double* Bacc = reinterpret_cast<double*>(smem_raw);              // 3 smax^2
-  double* Ac = Bacc + 3 * smax * smax;                              // smax * OC
-  float* Jc = reinterpret_cast<float*>(Ac + smax * DL_OC);          // 3 smax * OC
-  int* s_idx = reinterpret_cast<int*>(Jc + 3 * smax * DL_OC);       // smax
-  int64_t* s_row = reinterpret_cast<int64_t*>(s_idx + smax + (smax & 1));  // smax (first jac row)
+  const int RTm = (3 * smax + 15) / 16, CTm = (smax + 15) / 16;
+  const int LDK = DL_KC + 4;
+  float* Js = reinterpret_cast<float*>(smem_raw);
+  float* Xs = Js + RTm * 16 * LDK;
+  float* Bs = Xs + CTm * 16 * LDK;
+  int* s_idx = reinterpret_cast<int*>(Bs + ((3 * smax * smax + 1) & ~1));
+  int64_t* s_row = reinterpret_cast<int64_t*>(s_idx + smax + (smax & 1));
   __shared__ double red[16 * 4];
   const int blk = blockIdx.x;
   const int n = blk % N;
@@ -277,7 +285,7 @@ __global__ void __launch_bounds__(256) k_det_lap(const float* __restrict__ O, in
   const int gn = group_of(w, n, B, n_up, N - n_up);
   const int64_t p0 = ptr[gn];
   const int s = 1 + (int)(ptr[gn + 1] - p0);
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   for (int a = tid; a < s; a += blockDim.x) {
     if (a == 0) {
       s_idx[0] = n;
@@ -289,49 +297,71 @@ __global__ void __launch_bounds__(256) k_det_lap(const float* __restrict__ O, in
       s_row[a] = gd.row(gi) + 5 + 3 * (int64_t)rev[p0 + a - 1];
     }
   }
-  const int nB = 3 * s * s;
-  for (int e = tid; e < nB; e += blockDim.x) Bacc[e] = 0.0;
+  const int RT = (3 * s + 15) / 16, CT = (s + 15) / 16, NT = RT * CT;
+  float4_t acc[DL_TPW];
+  for (int t = 0; t < DL_TPW; ++t) acc[t] = (float4_t){0.f, 0.f, 0.f, 0.f};
   __syncthreads();
   const double* X = Ainv + (size_t)wd * N * N;
-  for (int o0 = 0; o0 < N; o0 += DL_OC) {
-    const int on = min(DL_OC, N - o0);
-    for (int e = tid; e < 3 * s * DL_OC; e += blockDim.x) {
-      const int ax = e / DL_OC, oc = e - ax * DL_OC;
-      const int a = ax / 3, x = ax - 3 * a;
-      Jc[e] = oc < on ? O[(s_row[a] + x) * ND + d * N + o0 + oc] : 0.f;
+  const float* Od = O + (size_t)d * N;
+  for (int o0 = 0; o0 < N; o0 += DL_KC) {
+    for (int e = tid; e < RT * 16 * DL_KC; e += blockDim.x) {
+      const int ax = e / DL_KC, k = e - ax * DL_KC;
+      float v = 0.f;
+      if (ax < 3 * s && o0 + k < N) {
+        const int a = ax / 3, x = ax - 3 * a;
+        v = Od[(s_row[a] + x) * ND + o0 + k];
+      }
+      Js[ax * LDK + k] = v;
     }
-    for (int e = tid; e < s * DL_OC; e += blockDim.x) {
-      const int b = e / DL_OC, oc = e - b * DL_OC;
-      Ac[e] = oc < on ? X[(size_t)(o0 + oc) * N + s_idx[b]] : 0.0;
+    for (int e = tid; e < CT * 16 * DL_KC; e += blockDim.x) {
+      const int k = e / (CT * 16), b = e - k * (CT * 16);
+      float v = 0.f;
+      if (b < s && o0 + k < N) v = (float)X[(size_t)(o0 + k) * N + s_idx[b]];
+      Xs[b * LDK + k] = v;
     }
     __syncthreads();
-    for (int e = tid; e < nB; e += blockDim.x) {
-      const int ax = e / s, b = e - ax * s;
-      const float* jr = Jc + ax * DL_OC;
-      const double* ar = Ac + b * DL_OC;
-      double acc = 0.0;
-#pragma unroll 8
-      for (int oc = 0; oc < DL_OC; ++oc) acc += (double)jr[oc] * ar[oc];
-      Bacc[e] += acc;
+    int q = 0;
+    for (int t = wid; t < NT; t += 4, ++q) {
+      const int rt = t / CT, ct = t - rt * CT;
+      const float* ja = Js + (rt * 16 + (lane & 15)) * LDK + (lane >> 4);
+      const float* xb = Xs + (ct * 16 + (lane & 15)) * LDK + (lane >> 4);
+#pragma unroll
+      for (int kk = 0; kk < DL_KC; kk += 4) acc[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(ja[kk], xb[kk], acc[q], 0, 0, 0);
     }
     __syncthreads();
   }
-  double v[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int e = tid; e < nB; e += blockDim.x) {
+  // scatter the 16x16 tiles into B[ax][b]
+  {
+    int q = 0;
+    for (int t = wid; t < NT; t += 4, ++q) {
+      const int rt = t / CT, ct = t - rt * CT;
+      const int b = ct * 16 + (lane & 15);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int ax = rt * 16 + 4 * (lane >> 4) + v;
+        if (ax < 3 * s && b < s) Bs[ax * s + b] = acc[q][v];
+      }
+    }
+  }
+  __syncthreads();
+  double v4[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int e = tid; e < 3 * s * s; e += blockDim.x) {
     const int ax = e / s, b = e - ax * s;
     const int a = ax / 3, x = ax - 3 * a;
-    v[0] += Bacc[e] * Bacc[(b * 3 + x) * s + a];
-    if (a == b) v[1 + x] += Bacc[e];
+    const double bv = Bs[e];
+    v4[0] += bv * (double)Bs[(3 * b + x) * s + a];
+    if (a == b) v4[1 + x] += bv;
   }
-  block_sum<4>(v, red);
+  block_sum<4>(v4, red);
   if (tid == 0) {
-    trip[blk] = v[0];
-    for (int x = 0; x < 3; ++x) grad[(size_t)blk * 3 + x] = v[1 + x];
+    trip[blk] = v4[0];
+    for (int x = 0; x < 3; ++x) grad[(size_t)blk * 3 + x] = v4[1 + x];
   }
 }
 
 size_t det_lap_smem(int smax) {
-  return (size_t)3 * smax * smax * 8 + (size_t)smax * DL_OC * 8 + (size_t)3 * smax * DL_OC * 4 +
+  const int RTm = (3 * smax + 15) / 16, CTm = (smax + 15) / 16;
+  return (size_t)(RTm * 16 + CTm * 16) * (DL_KC + 4) * 4 + (size_t)((3 * smax * smax + 1) & ~1) * 4 +
          (size_t)(smax + (smax & 1)) * 4 + (size_t)smax * 8;
 }
 

@@ -22,7 +22,7 @@ from . import _lib
 from .config import FireConfig, Molecule
 from .params import init_params, pack_params, set_scales
 
-PRECISIONS = {"fp32": _lib.FIRE_FP32, "mixed": _lib.FIRE_MIXED}
+PRECISIONS = {"fp32": _lib.FIRE_FP32, "mixed": _lib.FIRE_MIXED, "bf16": _lib.FIRE_BF16}
 
 
 class StaticArgs(NamedTuple):

@@ -65,8 +65,17 @@ def test_neighbours_cutoff_boundary(gpu_device):
     wf.upload(prm)
     r = walkers(cfg, 2, seed=5).copy()
     c = np.float32(cfg.cutoff)
-    r[0, 1] = r[0, 0] + np.array([c, 0, 0], np.float32)
-    r[1, 1] = r[1, 0] + np.array([np.nextafter(c, np.float32(0)), 0, 0], np.float32)
+    base = np.array([0.5, 0.25, -0.75], np.float32)
+    # electron 1 sits on the x axis of electron 0: find the largest offset whose float32
+    # distance is strictly below the cutoff
+    x = c
+    while True:
+        probe = np.stack([base, base + np.array([x, 0, 0], np.float32)])
+        if fire.pair_distances_f32(probe)[0, 1] < c:
+            break
+        x = np.nextafter(x, np.float32(0))
+    r[0, 0], r[0, 1] = base, base + np.array([c, 0, 0], np.float32)
+    r[1, 0], r[1, 1] = base, base + np.array([x, 0, 0], np.float32)
     counts, nbr, _ = wf.neighbours(r)
     counts = counts.cpu().numpy()
     for w in range(2):
@@ -75,9 +84,13 @@ def test_neighbours_cutoff_boundary(gpu_device):
     d0 = fire.pair_distances_f32(r[0])[0, 1]
     d1 = fire.pair_distances_f32(r[1])[0, 1]
     assert d0 == c and d1 < c
+    nl0 = fire.neighbour_lists(r[0], cfg.n_up, cfg.cutoff)
+    nl1 = fire.neighbour_lists(r[1], cfg.n_up, cfg.cutoff)
+    assert 1 not in nl0.nbr[nl0.offsets[0]:nl0.offsets[1]]
+    assert 1 in nl1.nbr[nl1.offsets[0]:nl1.offsets[1]]
 
 
-@pytest.mark.parametrize("precision,tol", [("fp32", 2e-5), ("mixed", 3e-2)])
+@pytest.mark.parametrize("precision,tol", [("fp32", 2e-5), ("mixed", 2e-4), ("bf16", 3e-2)])
 def test_embedding(setup, precision, tol):
     name, cfg, prm, wf, r = setup
     pt = fire.to_torch(prm)
@@ -89,7 +102,7 @@ def test_embedding(setup, precision, tol):
         assert e < tol
 
 
-@pytest.mark.parametrize("precision,tol", [("fp32", 1e-4), ("mixed", 5e-2)])
+@pytest.mark.parametrize("precision,tol", [("fp32", 1e-4), ("mixed", 1e-3)])
 def test_signed(setup, precision, tol):
     name, cfg, prm, wf, r = setup
     pt = fire.to_torch(prm)
@@ -123,4 +136,4 @@ def test_local_energy(setup, precision):
         if precision == "fp32":
             assert eg < 1e-4 and el < 1e-3 and ek < 1e-3
         else:
-            assert eg < 5e-2 and el < 5e-2 and ek < 5e-2
+            assert eg < 5e-3 and el < 1e-2 and ek < 1e-2
