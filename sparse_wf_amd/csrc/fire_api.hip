@@ -78,6 +78,7 @@ struct fire_handle {
   std::vector<float*> P;
   std::vector<uint16_t*> Pbf;   // BF16 transposed copies of GEMM weights (null if none): hi part
   std::vector<uint16_t*> Pbf_lo;  // ... and the BF16 remainder W - hi
+  std::vector<float*> Pt32;       // FP32 transposed copies [N,K] of GEMM weights
   std::vector<std::pair<int, int>> shape2;  // (K, N) of GEMM weights
   float cut_learned = 0.f;
   float s1 = 1.f;
@@ -86,13 +87,14 @@ struct fire_handle {
   int iEN, iEEs, iEEd, iMLP, iORB, iJM, iATT, iCUSP, iSCAL;
   // workspace
   DevBuf cnt, ptr, nbr, rev, rowflag;
-  DevBuf T5, Z2, T5b, M5, XA, XB, XC, O, ENV;
-  DevBuf A64, Ainv, ldet, dsgn, tr, trip, dgrad;
+  DevBuf T5, Z2, T5b, M5, XA, XB, XC, O, ENV, ENVC;
+  DevBuf A64, Ainv, AinvT, ldet, dsgn, tr, trip, dgrad;
   DevBuf det_lp, det_sg, det_gr, det_lap;
   DevBuf JR, mlp_ns, mlp_dg, QV, AV, att_ns, att_dg, RIN, RY1, RY2, att_out;
   DevBuf cval, clap, cgrad, epot;
   int chunk = 256;
   size_t smem_set = 0;
+  bool lu_smem_set = false;
   // optional HIP-event timing of the dense contractions (fire_profile_gemm)
   bool gemm_timing = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> gemm_events;
@@ -205,7 +207,7 @@ void gemm(Ctx& c, const float* A, int64_t lda, int wi, int bi, float* C, int64_t
     launch_gemm_bf16(c.s, A, lda, h->Pbf[wi], h->Pbf_lo[wi], bias, C, ldc, M, N, K, rowflag, period,
                      c.prec == FIRE_MIXED ? 3 : 1);
   else
-    launch_gemm_f32(c.s, A, lda, h->P[wi], bias, C, ldc, M, N, K, rowflag, period);
+    launch_gemm_f32(c.s, A, lda, h->P[wi], h->Pt32[wi], bias, C, ldc, M, N, K, rowflag, period);
   if (h->gemm_timing) {
     HIPCHK(hipEventRecord(e1, c.s));
     h->gemm_events.push_back({e0, e1});
@@ -349,8 +351,25 @@ void run_orbitals_det(Ctx& c, const float* r, int B, const Graph& gr, bool tan, 
   if (R > R_up_rows)
     gemm(c, h->XA.as<float>() + (size_t)R_up_rows * F, F, o + 2, o + 3, Ob + (size_t)R_up_rows * ND, ND,
          R - R_up_rows, ND, F, flag ? flag + R_up_rows : nullptr, period);
-  launch_envelopes(c.s, r, h->R, B, N, h->n_up, h->M, D, h->Eiso, h->P[o + 4], h->P[o + 5], h->P[o + 6], h->P[o + 7],
-                   tan, h->ENV.as<float>());
+  // envelopes: coefficients per (electron, row, nucleus, envelope), then one FP32 GEMM per
+  // (spin block, determinant) against pi stored as [det][nucleus * envelope][orbital]
+  {
+    const int ME = h->M * h->Eiso;
+    const int64_t rows_all = (int64_t)G * NR, rows_up = (int64_t)B * h->n_up * NR;
+    h->ENVC.ensure(sizeof(float) * (size_t)D * rows_all * ME);
+    launch_env_coef(c.s, r, h->R, B, N, h->n_up, h->M, D, h->Eiso, h->P[o + 4], h->P[o + 6], tan,
+                    h->ENVC.as<float>());
+    for (int sp = 0; sp < 2; ++sp) {
+      const int64_t r0 = sp == 0 ? 0 : rows_up, nr = sp == 0 ? rows_up : rows_all - rows_up;
+      if (nr == 0) continue;
+      const float* pi = h->P[o + (sp == 0 ? 5 : 7)];
+      const float* piT = h->Pt32[o + (sp == 0 ? 5 : 7)];
+      for (int d = 0; d < D; ++d)
+        launch_gemm_f32(c.s, h->ENVC.as<float>() + ((size_t)d * rows_all + r0) * ME, ME, pi + (size_t)d * ME * N,
+                        piT + (size_t)d * ME * N, nullptr, h->ENV.as<float>() + (size_t)r0 * ND + (size_t)d * N, ND,
+                        nr, N, ME, nullptr, 0);
+    }
+  }
   launch_orb_product(c.s, Ob, ND, G, gd, h->ENV.as<float>(), tan);
   const size_t nmat = (size_t)B * D;
   h->A64.ensure(sizeof(double) * nmat * N * N);
@@ -361,6 +380,11 @@ void run_orbitals_det(Ctx& c, const float* r, int B, const Graph& gr, bool tan, 
   h->det_sg.ensure(sizeof(double) * B);
   out.logpsi = h->det_lp.as<double>();
   out.sign = h->det_sg.as<double>();
+  const bool lu_lds = lu_lds_bytes(N) <= 160 * 1024;
+  if (lu_lds && lu_lds_bytes(N) > 64 * 1024 && !h->lu_smem_set) {
+    HIPCHK(lu_set_smem_limit(lu_lds_bytes(N)));
+    h->lu_smem_set = true;
+  }
   if (tan) {
     h->Ainv.ensure(sizeof(double) * nmat * N * N);
     h->tr.ensure(sizeof(double) * nmat);
@@ -368,8 +392,9 @@ void run_orbitals_det(Ctx& c, const float* r, int B, const Graph& gr, bool tan, 
     h->dgrad.ensure(sizeof(double) * nmat * N * 3);
     h->det_gr.ensure(sizeof(double) * (size_t)B * 3 * N);
     h->det_lap.ensure(sizeof(double) * B);
-    launch_lu_inv(c.s, h->A64.as<double>(), (int)nmat, N, h->Ainv.as<double>(), h->ldet.as<double>(),
-                  h->dsgn.as<double>(), true);
+    h->AinvT.ensure(sizeof(float) * nmat * N * N);
+    launch_lu_inv(c.s, h->A64.as<double>(), (int)nmat, N, h->Ainv.as<double>(), h->AinvT.as<float>(),
+                  h->ldet.as<double>(), h->dsgn.as<double>(), true, lu_lds);
     launch_det_trace(c.s, Ob, ND, gd, B, N, h->n_up, D, h->Ainv.as<double>(), h->tr.as<double>());
     const int smax = gr.max_nb + 1;
     REQUIRE(smax <= FIRE_SMAX, "an electron has more than 63 neighbours within the cutoff (determinant Laplacian limit)");
@@ -379,11 +404,12 @@ void run_orbitals_det(Ctx& c, const float* r, int B, const Graph& gr, bool tan, 
       h->smem_set = sm;
     }
     launch_det_lap(c.s, Ob, ND, gd, h->ptr.as<int64_t>(), h->nbr.as<int32_t>(), h->rev.as<int32_t>(), B, N, h->n_up, D,
-                   smax, h->Ainv.as<double>(), h->trip.as<double>(), h->dgrad.as<double>());
+                   smax, h->AinvT.as<float>(), h->trip.as<double>(), h->dgrad.as<double>());
     out.grad = h->det_gr.as<double>();
     out.lap = h->det_lap.as<double>();
   } else {
-    launch_lu_inv(c.s, h->A64.as<double>(), (int)nmat, N, nullptr, h->ldet.as<double>(), h->dsgn.as<double>(), false);
+    launch_lu_inv(c.s, h->A64.as<double>(), (int)nmat, N, nullptr, nullptr, h->ldet.as<double>(),
+                  h->dsgn.as<double>(), false, lu_lds);
     out.grad = nullptr;
     out.lap = nullptr;
   }
@@ -470,7 +496,8 @@ void run_jastrow_and_assemble(Ctx& c, const float* r, int B, bool tan, const Gro
       std::swap(cur, nxt);
     }
     const int al = ac + 3 + 2 * h->depth;
-    launch_gemm_f32(c.s, cur, W, h->P[al], h->P[al + 1], h->att_out.as<float>(), 2, RR, 2, W, nullptr, nrow);
+    launch_gemm_f32(c.s, cur, W, h->P[al], nullptr, h->P[al + 1], h->att_out.as<float>(), 2, RR, 2, W, nullptr,
+                    nrow);
     A.att_out = h->att_out.as<float>();
     A.att_scale = h->P[al + 2];
     A.att_zb = h->P[al + 3];
@@ -517,15 +544,20 @@ int fire_create(const fire_config* cfg, const float* R, const float* Z, int devi
     REQUIRE(cfg && R && Z && out, "null argument");
     const fire_config& c = *cfg;
     REQUIRE(c.n_el > 0 && c.n_up >= 0 && c.n_up <= c.n_el && c.n_nuc > 0, "invalid electron / nucleus counts");
-    REQUIRE(c.feature_dim % 64 == 0 && c.feature_dim <= 1024, "feature_dim must be a multiple of 64, at most 1024");
-    REQUIRE(c.filter_w0 > 0 && c.filter_w0 <= FIRE_W0MAX && c.filter_w1 > 0 && c.filter_w1 <= FIRE_W1MAX,
-            "pair_mlp_widths out of range (w0 <= 32, w1 <= 16)");
+    REQUIRE(c.feature_dim % 64 == 0 && c.feature_dim <= 256,
+            "feature_dim must be a multiple of 64, at most 256 (one feature column per thread of a 256-thread "
+            "workgroup in the embedding kernels)");
+    REQUIRE(!c.use_attention || (c.att_dim <= 64 && (c.att_dim & (c.att_dim - 1)) == 0 && c.att_heads * c.att_dim <= 256),
+            "attention_dim must be a power of two <= 64 and attention_heads * attention_dim <= 256");
+    REQUIRE(c.filter_w0 > 0 && c.filter_w0 <= FIRE_W0MAX &&
+                (c.filter_w1 == 4 || c.filter_w1 == 8 || c.filter_w1 == 16),
+            "pair_mlp_widths out of range (w0 <= 32; w1 zero-padded to 4, 8 or 16)");
     REQUIRE(c.n_env_en > 0 && c.n_env_en <= FIRE_EEMAX, "pair_n_envelopes out of range (<= 64)");
     REQUIRE(c.n_det > 0 && c.n_det <= 16 && c.n_el * c.n_det <= 2048,
             "n_determinants must be <= 16 and n_el * n_determinants <= 2048");
     REQUIRE(c.n_det * c.n_env_iso <= 1600, "n_determinants * n_envelopes must be <= 1600");
     REQUIRE(c.e_e_cusp == 0 || c.e_e_cusp == 1, "e_e_cusp must be 0 (none) or 1 (psiformer)");
-    REQUIRE(!c.use_attention || c.att_heads * c.att_dim <= 1024, "attention heads * dim must be <= 1024");
+
     REQUIRE(c.mlp_depth >= 1 && c.mlp_width > 0, "mlp_depth >= 1 and mlp_width > 0 required");
     HIPCHK(hipSetDevice(device));
     fire_handle* h = new fire_handle();
@@ -578,10 +610,12 @@ int fire_destroy(fire_handle* h) {
     if (p) (void)hipFree(p);
   for (uint16_t* p : h->Pbf_lo)
     if (p) (void)hipFree(p);
+  for (float* p : h->Pt32)
+    if (p) (void)hipFree(p);
   if (h->R) (void)hipFree(h->R);
   if (h->Z) (void)hipFree(h->Z);
   DevBuf* bufs[] = {&h->cnt, &h->ptr, &h->nbr, &h->rev, &h->rowflag, &h->T5, &h->Z2, &h->T5b, &h->M5,
-                    &h->XA, &h->XB, &h->XC, &h->O, &h->ENV, &h->A64, &h->Ainv, &h->ldet, &h->dsgn,
+                    &h->XA, &h->XB, &h->XC, &h->O, &h->ENV, &h->ENVC, &h->A64, &h->Ainv, &h->AinvT, &h->ldet, &h->dsgn,
                     &h->tr, &h->trip, &h->dgrad, &h->det_lp, &h->det_sg, &h->det_gr, &h->det_lap, &h->JR,
                     &h->mlp_ns, &h->mlp_dg, &h->QV, &h->AV, &h->att_ns, &h->att_dg, &h->RIN, &h->RY1,
                     &h->RY2, &h->att_out, &h->cval, &h->clap, &h->cgrad, &h->epot};
@@ -605,9 +639,12 @@ int fire_upload_params(fire_handle* h, const float* packed, const int64_t* sizes
       if (p) HIPCHK(hipFree(p));
     for (uint16_t* p : h->Pbf_lo)
       if (p) HIPCHK(hipFree(p));
+    for (float* p : h->Pt32)
+      if (p) HIPCHK(hipFree(p));
     h->P.assign(n_tensors, nullptr);
     h->Pbf.assign(n_tensors, nullptr);
     h->Pbf_lo.assign(n_tensors, nullptr);
+    h->Pt32.assign(n_tensors, nullptr);
     size_t off = 0;
     for (int k = 0; k < n_tensors; ++k) {
       HIPCHK(hipMalloc(&h->P[k], sizeof(float) * std::max<int64_t>(sizes[k], 1)));
@@ -641,6 +678,20 @@ int fire_upload_params(fire_handle* h, const float* packed, const int64_t* sizes
       HIPCHK(hipMalloc(&h->Pbf_lo[k], sizeof(uint16_t) * tl.size()));
       HIPCHK(hipMemcpy(h->Pbf_lo[k], tl.data(), sizeof(uint16_t) * tl.size(), hipMemcpyHostToDevice));
     }
+    // FP32 transposed copies [N,K] for the FP32 MFMA kernel: every GEMM weight, and the
+    // envelope weights pi per determinant ([det][ME][N] -> [det][N][ME])
+    auto upload_t = [&](int k, int nb, int K, int Nn) {
+      std::vector<float> tt((size_t)nb * K * Nn);
+      const float* src = packed + offs[k];
+      for (int b = 0; b < nb; ++b)
+        for (int kk = 0; kk < K; ++kk)
+          for (int n = 0; n < Nn; ++n)
+            tt[(size_t)b * K * Nn + (size_t)n * K + kk] = src[(size_t)b * K * Nn + (size_t)kk * Nn + n];
+      HIPCHK(hipMalloc(&h->Pt32[k], sizeof(float) * tt.size()));
+      HIPCHK(hipMemcpy(h->Pt32[k], tt.data(), sizeof(float) * tt.size(), hipMemcpyHostToDevice));
+    };
+    for (auto& t : gw) upload_t(t.first, 1, t.second.first, t.second.second);
+    for (int k : {h->iORB + 5, h->iORB + 7}) upload_t(k, h->D, h->M * h->Eiso, h->N);
     h->cut_learned = packed[offs[h->iSCAL]];
     h->s1 = packed[offs[h->iSCAL] + 1];
     h->params_ok = true;

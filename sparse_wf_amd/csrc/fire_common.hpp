@@ -41,17 +41,22 @@ __device__ __forceinline__ void silu3(float x, float& f, float& f1, float& f2) {
 // features g = [log1p d, diff * log1p(d)/d] with respect to the 3-vector `diff`.
 // Output rows follow the FL5 convention.  cut_c: cutoff of the polynomial cutoff_function
 // (model/utils.py:103-112, p = 4).
-__device__ inline void pair_filter_fl(const float diff[3], const FilterParams& P, float cut_c,
-                                      float beta[5][FIRE_W1MAX], float g[5][4]) {
+// W1 is the (zero-padded) filter width; every loop over it is unrolled so the per-thread
+// arrays stay in registers.
+template <int W1>
+__device__ inline void pair_filter_fl(const float diff[3], const FilterParams& P, float cut_c, float beta[5][W1],
+                                      float g[5][4]) {
   const float dx = diff[0], dy = diff[1], dz = diff[2];
   const float d = sqrtf(dx * dx + dy * dy + dz * dz);
   const float inv_d = 1.0f / d;
   const float n[3] = {dx * inv_d, dy * inv_d, dz * inv_d};
   const float lap_d = 2.0f * inv_d;
 
-  float dir[5][FIRE_W1MAX];
-  float rad[5][FIRE_W1MAX];
-  for (int j = 0; j < P.w1; ++j) {
+  float dir[5][W1];
+  float rad[5][W1];
+#pragma unroll
+  for (int j = 0; j < W1; ++j) {
+#pragma unroll
     for (int r = 0; r < 5; ++r) { dir[r][j] = 0.f; rad[r][j] = 0.f; }
     dir[0][j] = P.bd[j];
   }
@@ -64,8 +69,9 @@ __device__ inline void pair_filter_fl(const float diff[3], const FilterParams& P
     float f, f1, f2;
     silu3(pre, f, f1, f2);
     const float a[5] = {f, f1 * pl + f2 * (pt0 * pt0 + pt1 * pt1 + pt2 * pt2), f1 * pt0, f1 * pt1, f1 * pt2};
-    const float* wrow = P.Wd + k * P.w1;
-    for (int j = 0; j < P.w1; ++j) {
+    const float* wrow = P.Wd + k * W1;
+#pragma unroll
+    for (int j = 0; j < W1; ++j) {
       const float w = wrow[j];
 #pragma unroll
       for (int r = 0; r < 5; ++r) dir[r][j] += a[r] * w;
@@ -90,14 +96,16 @@ __device__ inline void pair_filter_fl(const float diff[3], const FilterParams& P
     const float q2 = (4.0f * x * x - 2.0f) / (sp * sp) * q;
     const float ev = q * cu, e1 = q1 * cu + q * cu1, e2 = q2 * cu + 2.0f * q1 * cu1 + q * cu2;
     const float a[5] = {ev, e2 + e1 * lap_d, e1 * n[0], e1 * n[1], e1 * n[2]};
-    const float* wrow = P.We + e * P.w1;
-    for (int j = 0; j < P.w1; ++j) {
+    const float* wrow = P.We + e * W1;
+#pragma unroll
+    for (int j = 0; j < W1; ++j) {
       const float w = wrow[j];
 #pragma unroll
       for (int r = 0; r < 5; ++r) rad[r][j] += a[r] * w;
     }
   }
-  for (int j = 0; j < P.w1; ++j) {
+#pragma unroll
+  for (int j = 0; j < W1; ++j) {
     const float dxv = dir[0][j], rx = rad[0][j];
     beta[0][j] = dxv * rx;
     beta[1][j] = dir[1][j] * rx + dxv * rad[1][j] +

@@ -49,8 +49,9 @@ void launch_scale(hipStream_t s, float* p, int64_t n, float a);
 void launch_rowflag(hipStream_t s, int G, GroupDesc gd, int64_t n_rows, uint8_t* flag);
 
 // dense contractions: C[M,N] = A[M,K] W[K,N] (+ bias on rows with flag / row % period == 0)
-void launch_gemm_f32(hipStream_t s, const float* A, int64_t lda, const float* W, const float* bias, float* C,
-                     int64_t ldc, int64_t M, int N, int K, const uint8_t* rowflag, int period);
+// W is [K,N] (used for N <= 16), Wt its transpose [N,K] (used by the MFMA kernel)
+void launch_gemm_f32(hipStream_t s, const float* A, int64_t lda, const float* W, const float* Wt, const float* bias,
+                     float* C, int64_t ldc, int64_t M, int N, int K, const uint8_t* rowflag, int period);
 void launch_gemm_bf16(hipStream_t s, const float* A, int64_t lda, const uint16_t* Whi /*[N,K] bf16*/,
                       const uint16_t* Wlo, const float* bias, float* C, int64_t ldc, int64_t M, int N, int K,
                       const uint8_t* rowflag, int period, int nsplit);

@@ -8,19 +8,20 @@
 namespace fire {
 
 // orbitals and determinants
-void launch_envelopes(hipStream_t s, const float* r, const float* R, int B, int N, int n_up, int M, int D, int E,
-                      const float* sp_up, const float* pi_up, const float* sp_dn, const float* pi_dn, bool tangents,
-                      float* out);
+void launch_env_coef(hipStream_t s, const float* r, const float* R, int B, int N, int n_up, int M, int D, int E,
+                     const float* sp_up, const float* sp_dn, bool tangents, float* coef);
 void launch_orb_product(hipStream_t s, float* O, int ND, int G, GroupDesc gd, const float* env, bool tangents);
 void launch_gather_A(hipStream_t s, const float* O, int ND, GroupDesc gd, int B, int N, int n_up, int D, double* A);
-void launch_lu_inv(hipStream_t s, double* A, int n_mat, int N, double* Ainv, double* logdet, double* sign,
-                   bool want_inv);
+void launch_lu_inv(hipStream_t s, double* A, int n_mat, int N, double* Ainv, float* AinvT32, double* logdet,
+                   double* sign, bool want_inv, bool use_lds);
+size_t lu_lds_bytes(int N);
+hipError_t lu_set_smem_limit(size_t bytes);
 void launch_det_trace(hipStream_t s, const float* O, int ND, GroupDesc gd, int B, int N, int n_up, int D,
                       const double* Ainv, double* tr);
 size_t det_lap_smem(int smax);
 hipError_t det_lap_set_smem_limit(size_t bytes);
 void launch_det_lap(hipStream_t s, const float* O, int ND, GroupDesc gd, const int64_t* ptr, const int32_t* nbr,
-                    const int32_t* rev, int B, int N, int n_up, int D, int smax, const double* Ainv, double* trip,
+                    const int32_t* rev, int B, int N, int n_up, int D, int smax, const float* AinvT, double* trip,
                     double* grad);
 void launch_det_combine(hipStream_t s, int B, int N, int D, const double* logdet, const double* sgn, const double* tr,
                         const double* trip, const double* grad, double* out_logpsi, double* out_sign,

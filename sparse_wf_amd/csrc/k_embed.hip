@@ -16,11 +16,11 @@ namespace fire {
 
 // One workgroup (blockDim = F) per electron: sum over nuclei within cutoff_1el of
 // Gamma_J * e_J with forward-Laplacian rows, then LayerNorm with forward-Laplacian.
-template <bool TAN>
-__global__ void __launch_bounds__(1024) k_elec_init_en(const float* __restrict__ r, const float* __restrict__ R, int B,
+template <bool TAN, int W1>
+__global__ void __launch_bounds__(256) k_elec_init_en(const float* __restrict__ r, const float* __restrict__ R, int B,
                                                        int N, int n_up, int M, float cutoff_1el, int F, EnParams P,
                                                        float* __restrict__ out) {
-  __shared__ float s_beta[EN_CHUNK][5][FIRE_W1MAX];
+  __shared__ float s_beta[EN_CHUNK][5][W1];
   __shared__ float s_g[EN_CHUNK][5][4];
   __shared__ int s_ok[EN_CHUNK];
   __shared__ float s_red[16 * 8];
@@ -31,8 +31,10 @@ __global__ void __launch_bounds__(1024) k_elec_init_en(const float* __restrict__
   const float rx = ri[0], ry = ri[1], rz = ri[2];
   const int f = threadIdx.x;
   float acc[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
-  float wg[FIRE_W1MAX], wf[4];
-  for (int k = 0; k < P.w1; ++k) wg[k] = P.WG[k * F + f];
+  float wg[W1], wf[4];
+#pragma unroll
+  for (int k = 0; k < W1; ++k) wg[k] = P.WG[k * F + f];
+#pragma unroll
   for (int m = 0; m < 4; ++m) wf[m] = P.WF[m * F + f];
 
   for (int J0 = 0; J0 < M; J0 += EN_CHUNK) {
@@ -53,12 +55,15 @@ __global__ void __launch_bounds__(1024) k_elec_init_en(const float* __restrict__
         fp.w0 = P.w0;
         fp.w1 = P.w1;
         fp.E = P.E;
-        float beta[5][FIRE_W1MAX], gg[5][4];
-        pair_filter_fl(diff, fp, cutoff_1el, beta, gg);
+        float beta[5][W1], gg[5][4];
+        pair_filter_fl<W1>(diff, fp, cutoff_1el, beta, gg);
         const int slot = threadIdx.x;
         s_ok[slot] = 1;
+#pragma unroll
         for (int rr = 0; rr < 5; ++rr) {
-          for (int k = 0; k < P.w1; ++k) s_beta[slot][rr][k] = beta[rr][k];
+#pragma unroll
+          for (int k = 0; k < W1; ++k) s_beta[slot][rr][k] = beta[rr][k];
+#pragma unroll
           for (int m = 0; m < 4; ++m) s_g[slot][rr][m] = gg[rr][m];
         }
       }
@@ -74,7 +79,8 @@ __global__ void __launch_bounds__(1024) k_elec_init_en(const float* __restrict__
 #pragma unroll
         for (int rr = 0; rr < 5; ++rr) {
           float a = 0.f;
-          for (int k = 0; k < P.w1; ++k) a += s_beta[t][rr][k] * wg[k];
+#pragma unroll
+          for (int k = 0; k < W1; ++k) a += s_beta[t][rr][k] * wg[k];
           G5[rr] = a;
           float b = 0.f;
 #pragma unroll
@@ -122,13 +128,21 @@ __global__ void __launch_bounds__(1024) k_elec_init_en(const float* __restrict__
   }
 }
 
+#define W1_DISPATCH(w1, ...)          \
+  switch (w1) {                       \
+    case 4: { constexpr int W1 = 4; __VA_ARGS__; } break;   \
+    case 8: { constexpr int W1 = 8; __VA_ARGS__; } break;   \
+    default: { constexpr int W1 = 16; __VA_ARGS__; } break; \
+  }
+
 void launch_elec_init_en(hipStream_t s, const float* r, const float* R, int B, int N, int n_up, int M, float cutoff_1el,
                          int F, EnParams P, bool tangents, float* out) {
   const int G = B * N;
-  if (tangents)
-    k_elec_init_en<true><<<G, F, 0, s>>>(r, R, B, N, n_up, M, cutoff_1el, F, P, out);
-  else
-    k_elec_init_en<false><<<G, F, 0, s>>>(r, R, B, N, n_up, M, cutoff_1el, F, P, out);
+  if (tangents) {
+    W1_DISPATCH(P.w1, k_elec_init_en<true, W1><<<G, F, 0, s>>>(r, R, B, N, n_up, M, cutoff_1el, F, P, out));
+  } else {
+    W1_DISPATCH(P.w1, k_elec_init_en<false, W1><<<G, F, 0, s>>>(r, R, B, N, n_up, M, cutoff_1el, F, P, out));
+  }
 }
 
 // GatedLinearUnit product y = a * silu(gate) on 5-row groups; Z = [a | gate] per row.
@@ -230,12 +244,12 @@ void launch_scale(hipStream_t s, float* p, int64_t n, float a) { k_scale<<<1024,
 // forward-Laplacian rows; one workgroup (blockDim = F) per centre electron.  The pair
 // filter quantities of up to MSG_CHUNK neighbours are computed by one thread each and
 // shared through LDS; each thread then owns one feature column.
-template <bool TAN>
-__global__ void __launch_bounds__(1024) k_messages(const float* __restrict__ r, int B, int N, int n_up,
+template <bool TAN, int W1>
+__global__ void __launch_bounds__(256) k_messages(const float* __restrict__ r, int B, int N, int n_up,
                                                    const int64_t* __restrict__ ptr, const int32_t* __restrict__ nbr,
                                                    const float* __restrict__ M5, EdgeParams Ps, EdgeParams Pd,
                                                    float cut, int F, float* __restrict__ out) {
-  __shared__ float s_beta[MSG_CHUNK][5][FIRE_W1MAX];
+  __shared__ float s_beta[MSG_CHUNK][5][W1];
   __shared__ float s_g[MSG_CHUNK][5][4];
   __shared__ int s_gj[MSG_CHUNK];
   __shared__ int s_same[MSG_CHUNK];
@@ -252,9 +266,10 @@ __global__ void __launch_bounds__(1024) k_messages(const float* __restrict__ r, 
   const bool up_i = i < n_up;
 
   // weights of this feature column for same- and opposite-spin edges
-  const int w1 = Ps.filt.w1;
-  float wgs[FIRE_W1MAX], wgd[FIRE_W1MAX], wfs[4], wfd[4];
-  for (int k = 0; k < w1; ++k) { wgs[k] = Ps.WG[k * F + f]; wgd[k] = Pd.WG[k * F + f]; }
+  float wgs[W1], wgd[W1], wfs[4], wfd[4];
+#pragma unroll
+  for (int k = 0; k < W1; ++k) { wgs[k] = Ps.WG[k * F + f]; wgd[k] = Pd.WG[k * F + f]; }
+#pragma unroll
   for (int m = 0; m < 4; ++m) { wfs[m] = Ps.WF[m * F + f]; wfd[m] = Pd.WF[m * F + f]; }
   const float bfs = Ps.BF[f], bfd = Pd.BF[f];
 
@@ -272,10 +287,13 @@ __global__ void __launch_bounds__(1024) k_messages(const float* __restrict__ r, 
       const int j = nbr[p0 + k];
       const float diff[3] = {rw[3 * i] - rw[3 * j], rw[3 * i + 1] - rw[3 * j + 1], rw[3 * i + 2] - rw[3 * j + 2]};
       const bool same = up_i == (j < n_up);
-      float beta[5][FIRE_W1MAX], gg[5][4];
-      pair_filter_fl(diff, same ? Ps.filt : Pd.filt, cut, beta, gg);
+      float beta[5][W1], gg[5][4];
+      pair_filter_fl<W1>(diff, same ? Ps.filt : Pd.filt, cut, beta, gg);
+#pragma unroll
       for (int rr = 0; rr < 5; ++rr) {
-        for (int q = 0; q < w1; ++q) s_beta[threadIdx.x][rr][q] = beta[rr][q];
+#pragma unroll
+        for (int q = 0; q < W1; ++q) s_beta[threadIdx.x][rr][q] = beta[rr][q];
+#pragma unroll
         for (int m = 0; m < 4; ++m) s_g[threadIdx.x][rr][m] = gg[rr][m];
       }
       s_gj[threadIdx.x] = group_of(w, j, B, n_up, n_dn);
@@ -284,20 +302,20 @@ __global__ void __launch_bounds__(1024) k_messages(const float* __restrict__ r, 
     __syncthreads();
     for (int kk = 0; kk < cn; ++kk) {
       const bool same = s_same[kk];
-      const float* wg = same ? wgs : wgd;
-      const float* wf = same ? wfs : wfd;
-      const float* mc = same ? mcs : mcd;
       const float* m5j = M5 + (size_t)s_gj[kk] * NR * 5 * F + (same ? 3 : 4) * F + f;
-      float G5[5], E5[5], mn[5];
+      float G5[5], E5[5], mn[5], mc[5];
+#pragma unroll
       for (int rr = 0; rr < NR; ++rr) {
         float a = 0.f;
-        for (int q = 0; q < w1; ++q) a += s_beta[kk][rr][q] * wg[q];
+#pragma unroll
+        for (int q = 0; q < W1; ++q) a += s_beta[kk][rr][q] * (same ? wgs[q] : wgd[q]);
         G5[rr] = a;
         float b = 0.f;
 #pragma unroll
-        for (int m = 0; m < 4; ++m) b += s_g[kk][rr][m] * wf[m];
+        for (int m = 0; m < 4; ++m) b += s_g[kk][rr][m] * (same ? wfs[m] : wfd[m]);
         E5[rr] = b;
         mn[rr] = m5j[rr * 5 * F];
+        mc[rr] = same ? mcs[rr] : mcd[rr];
       }
       E5[0] += same ? bfs : bfd;
       const float ux = E5[0] + mc[0] + mn[0];
@@ -333,10 +351,11 @@ __global__ void __launch_bounds__(1024) k_messages(const float* __restrict__ r, 
 void launch_messages(hipStream_t s, const float* r, int B, int N, int n_up, const int64_t* ptr, const int32_t* nbr,
                      const float* M5, EdgeParams same, EdgeParams diff, float cut, int F, bool tangents, float* out) {
   const int G = B * N;
-  if (tangents)
-    k_messages<true><<<G, F, 0, s>>>(r, B, N, n_up, ptr, nbr, M5, same, diff, cut, F, out);
-  else
-    k_messages<false><<<G, F, 0, s>>>(r, B, N, n_up, ptr, nbr, M5, same, diff, cut, F, out);
+  if (tangents) {
+    W1_DISPATCH(same.filt.w1, k_messages<true, W1><<<G, F, 0, s>>>(r, B, N, n_up, ptr, nbr, M5, same, diff, cut, F, out));
+  } else {
+    W1_DISPATCH(same.filt.w1, k_messages<false, W1><<<G, F, 0, s>>>(r, B, N, n_up, ptr, nbr, M5, same, diff, cut, F, out));
+  }
 }
 
 }  // namespace fire

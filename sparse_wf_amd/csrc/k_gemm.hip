@@ -7,10 +7,12 @@
 //
 // MFMA kernels: 256 threads = 4 wavefronts, 128x128 output tile, each wavefront a 64x64
 // quarter made of 2x2 32x32 MFMA tiles; K staged through LDS 32 at a time with the next
-// chunk prefetched into registers while the current one is multiplied.
+// chunk prefetched into registers while the current one is multiplied.  Tiles are mapped
+// so that the N-tiles of one M-tile run back to back on the same XCD (its L2 then serves the
+// repeated reads of the A rows).
 //   FP32 path : v_mfma_f32_32x32x2f32, A and W in FP32.
 //   BF16 path : v_mfma_f32_32x32x16_bf16, FP32 accumulation.  NSPLIT = 3 carries each
-//               operand as hi + lo BF16 pair (x = hi + lo exactly to ~2^-17 relative) and
+//               operand as hi + lo BF16 pair (x = hi + lo to ~2^-17 relative) and
 //               accumulates hi*hi + hi*lo + lo*hi ("split BF16"), NSPLIT = 1 is plain BF16.
 #include "fire_common.hpp"
 #include "fire_kernels.hpp"
@@ -32,11 +34,24 @@ __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint
 #define GT_K 32
 #define LDA_F (GT_K + 4)  // fp32 LDS row stride (floats)
 #define LDB_H (GT_K + 8)  // bf16 LDS row stride (halves): 80 bytes, keeps 16-byte alignment
+#define N_XCD 8
 
 __device__ __forceinline__ bool is_bias_row(int64_t row, const uint8_t* rowflag, int period) {
   if (rowflag) return rowflag[row] != 0;
   if (period > 0) return (row % period) == 0;
   return false;
+}
+
+// Linear block id -> (M tile, N tile).  Workgroups are dispatched round-robin over the
+// XCDs; tile t of the remapped order lives on XCD t / (T/8), and consecutive t share the M
+// tile, so the N tiles of an M tile are served by one L2.
+__device__ __forceinline__ void tile_of(int64_t& m_tile, int& n_tile, int nT) {
+  const int64_t bid = blockIdx.x;
+  const int64_t T = (int64_t)gridDim.x;
+  int64_t t = bid;
+  if (T % N_XCD == 0) t = (bid % N_XCD) * (T / N_XCD) + bid / N_XCD;
+  m_tile = t / nT;
+  n_tile = (int)(t - m_tile * nT);
 }
 
 template <typename AccT>
@@ -60,56 +75,75 @@ __device__ __forceinline__ void store_tile(const AccT (&acc)[2][2], int64_t m0, 
 }
 
 // ---------------------------------------------------------------------------------------
-// FP32 MFMA GEMM.  W is [K,N] FP32 row-major.
+// FP32 MFMA GEMM.  Wt is [N,K] FP32 (W transposed).  Vector path: K % 32 == 0, lda % 4 == 0.
 // ---------------------------------------------------------------------------------------
+template <bool VEC>
 __global__ void __launch_bounds__(256) k_gemm_f32(const float* __restrict__ A, int64_t lda,
-                                                  const float* __restrict__ W, const float* __restrict__ bias,
+                                                  const float* __restrict__ Wt, const float* __restrict__ bias,
                                                   float* __restrict__ C, int64_t ldc, int64_t M, int N, int K,
-                                                  const uint8_t* __restrict__ rowflag, int period) {
-  __shared__ float As[GT_M][LDA_F];
-  __shared__ float Bs[GT_N][LDA_F];  // transposed: Bs[n][k]
+                                                  const uint8_t* __restrict__ rowflag, int period, int nT) {
+  __shared__ __attribute__((aligned(16))) float As[GT_M][LDA_F];
+  __shared__ __attribute__((aligned(16))) float Bs[GT_N][LDA_F];  // Bs[n][k]
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  const int64_t m0 = (int64_t)blockIdx.x * GT_M;
-  const int n0 = blockIdx.y * GT_N;
+  int64_t mt;
+  int nt;
+  tile_of(mt, nt, nT);
+  const int64_t m0 = mt * GT_M;
+  const int n0 = nt * GT_N;
   const int wm = (wid >> 1) * 64, wn = (wid & 1) * 64;
   float16_t acc[2][2];
   for (int a = 0; a < 2; ++a)
     for (int b = 0; b < 2; ++b)
       for (int v = 0; v < 16; ++v) acc[a][b][v] = 0.f;
-  const bool vecA = (K % 4) == 0 && (lda % 4) == 0;
-  for (int k0 = 0; k0 < K; k0 += GT_K) {
-    // A tile 128 x 32 (4 floats per load, 4 loads per thread)
-    for (int e = tid; e < GT_M * GT_K / 4; e += 256) {
+  float4 ra[4], rb[4];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int e = tid + q * 256;
       const int rr = e >> 3, kq = (e & 7) * 4;
       const int64_t gr = m0 + rr;
-      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (gr < M) {
-        if (vecA && k0 + kq + 3 < K) {
-          v = *reinterpret_cast<const float4*>(A + gr * lda + k0 + kq);
-        } else {
-          const float* ap = A + gr * lda;
-          v.x = k0 + kq < K ? ap[k0 + kq] : 0.f;
-          v.y = k0 + kq + 1 < K ? ap[k0 + kq + 1] : 0.f;
-          v.z = k0 + kq + 2 < K ? ap[k0 + kq + 2] : 0.f;
-          v.w = k0 + kq + 3 < K ? ap[k0 + kq + 3] : 0.f;
+      const int gn = n0 + rr;
+      if (VEC) {
+        ra[q] = gr < M ? *reinterpret_cast<const float4*>(A + gr * lda + k0 + kq) : make_float4(0.f, 0.f, 0.f, 0.f);
+        rb[q] = gn < N ? *reinterpret_cast<const float4*>(Wt + (int64_t)gn * K + k0 + kq) : make_float4(0.f, 0.f, 0.f, 0.f);
+      } else {
+        float va[4], vb[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int gk = k0 + kq + j;
+          va[j] = (gr < M && gk < K) ? A[gr * lda + gk] : 0.f;
+          vb[j] = (gn < N && gk < K) ? Wt[(int64_t)gn * K + gk] : 0.f;
         }
+        ra[q] = make_float4(va[0], va[1], va[2], va[3]);
+        rb[q] = make_float4(vb[0], vb[1], vb[2], vb[3]);
       }
-      *reinterpret_cast<float4*>(&As[rr][kq]) = v;
     }
-    // W tile 32 x 128 -> Bs[n][k]
-    for (int e = tid; e < GT_N * GT_K; e += 256) {
-      const int kk = e >> 7, nn = e & 127;
-      const int gk = k0 + kk, gn = n0 + nn;
-      Bs[nn][kk] = (gk < K && gn < N) ? W[(int64_t)gk * N + gn] : 0.f;
+  };
+  auto store = [&]() {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int e = tid + q * 256;
+      const int rr = e >> 3, kq = (e & 7) * 4;
+      *reinterpret_cast<float4*>(&As[rr][kq]) = ra[q];
+      *reinterpret_cast<float4*>(&Bs[rr][kq]) = rb[q];
     }
+  };
+  load(0);
+  for (int k0 = 0; k0 < K; k0 += GT_K) {
+    store();
     __syncthreads();
-#pragma unroll 4
+    if (k0 + GT_K < K) load(k0 + GT_K);
+#pragma unroll
     for (int kk = 0; kk < GT_K; kk += 2) {
       const int ka = kk + (lane >> 5);
       float av[2], bv[2];
+#pragma unroll
       for (int a = 0; a < 2; ++a) av[a] = As[wm + a * 32 + (lane & 31)][ka];
+#pragma unroll
       for (int b = 0; b < 2; ++b) bv[b] = Bs[wn + b * 32 + (lane & 31)][ka];
+#pragma unroll
       for (int a = 0; a < 2; ++a)
+#pragma unroll
         for (int b = 0; b < 2; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[a], bv[b], acc[a][b], 0, 0, 0);
     }
     __syncthreads();
@@ -126,13 +160,16 @@ __global__ void __launch_bounds__(256) k_gemm_bf16(const float* __restrict__ A, 
                                                    const uint16_t* __restrict__ Whi, const uint16_t* __restrict__ Wlo,
                                                    const float* __restrict__ bias, float* __restrict__ C, int64_t ldc,
                                                    int64_t M, int N, int K, const uint8_t* __restrict__ rowflag,
-                                                   int period) {
+                                                   int period, int nT) {
   constexpr int NP = NSPLIT == 3 ? 2 : 1;  // hi (+ lo) planes
   __shared__ __attribute__((aligned(16))) uint16_t As[NP][GT_M][LDB_H];
   __shared__ __attribute__((aligned(16))) uint16_t Bs[NP][GT_N][LDB_H];
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  const int64_t m0 = (int64_t)blockIdx.x * GT_M;
-  const int n0 = blockIdx.y * GT_N;
+  int64_t mt;
+  int nt;
+  tile_of(mt, nt, nT);
+  const int64_t m0 = mt * GT_M;
+  const int n0 = nt * GT_N;
   const int wm = (wid >> 1) * 64, wn = (wid & 1) * 64;
   float16_t acc[2][2];
   for (int a = 0; a < 2; ++a)
@@ -223,7 +260,7 @@ __global__ void __launch_bounds__(256) k_gemm_bf16(const float* __restrict__ A, 
   store_tile(acc, m0, n0, wm, wn, lane, C, ldc, M, N, bias, rowflag, period);
 }
 
-// Small-N FP32 GEMM (N <= 16): one wavefront per row, lanes across K.
+// Small-N FP32 GEMM (N <= 16): one wavefront per row, lanes across K.  W is [K,N].
 __global__ void k_gemm_smalln(const float* __restrict__ A, int64_t lda, const float* __restrict__ W,
                               const float* __restrict__ bias, float* __restrict__ C, int64_t ldc, int64_t M, int N,
                               int K, const uint8_t* __restrict__ rowflag, int period) {
@@ -231,39 +268,48 @@ __global__ void k_gemm_smalln(const float* __restrict__ A, int64_t lda, const fl
   const int lane = threadIdx.x & 63;
   if (row >= M) return;
   float acc[16];
-  for (int n = 0; n < N; ++n) acc[n] = 0.f;
+  for (int n = 0; n < 16; ++n) acc[n] = 0.f;
   for (int k = lane; k < K; k += 64) {
     const float a = A[row * lda + k];
-    for (int n = 0; n < N; ++n) acc[n] += a * W[(int64_t)k * N + n];
+#pragma unroll
+    for (int n = 0; n < 16; ++n)
+      if (n < N) acc[n] += a * W[(int64_t)k * N + n];
   }
   const bool b = bias && is_bias_row(row, rowflag, period);
-  for (int n = 0; n < N; ++n) {
+#pragma unroll
+  for (int n = 0; n < 16; ++n) {
+    if (n >= N) break;
     float v = warp_sum(acc[n]);
     if (lane == 0) C[row * ldc + n] = v + (b ? bias[n] : 0.f);
   }
 }
 
-void launch_gemm_f32(hipStream_t s, const float* A, int64_t lda, const float* W, const float* bias, float* C,
-                     int64_t ldc, int64_t M, int N, int K, const uint8_t* rowflag, int period) {
+void launch_gemm_f32(hipStream_t s, const float* A, int64_t lda, const float* W, const float* Wt, const float* bias,
+                     float* C, int64_t ldc, int64_t M, int N, int K, const uint8_t* rowflag, int period) {
   if (M == 0) return;
   if (N <= 16) {
     const int64_t threads = M * 64;
     k_gemm_smalln<<<(unsigned)((threads + 255) / 256), 256, 0, s>>>(A, lda, W, bias, C, ldc, M, N, K, rowflag, period);
     return;
   }
-  dim3 grid((unsigned)((M + GT_M - 1) / GT_M), (N + GT_N - 1) / GT_N);
-  k_gemm_f32<<<grid, 256, 0, s>>>(A, lda, W, bias, C, ldc, M, N, K, rowflag, period);
+  const int nT = (N + GT_N - 1) / GT_N;
+  const int64_t blocks = ((M + GT_M - 1) / GT_M) * nT;
+  if ((K % GT_K) == 0 && (lda % 4) == 0)
+    k_gemm_f32<true><<<(unsigned)blocks, 256, 0, s>>>(A, lda, Wt, bias, C, ldc, M, N, K, rowflag, period, nT);
+  else
+    k_gemm_f32<false><<<(unsigned)blocks, 256, 0, s>>>(A, lda, Wt, bias, C, ldc, M, N, K, rowflag, period, nT);
 }
 
 void launch_gemm_bf16(hipStream_t s, const float* A, int64_t lda, const uint16_t* Whi, const uint16_t* Wlo,
                       const float* bias, float* C, int64_t ldc, int64_t M, int N, int K, const uint8_t* rowflag,
                       int period, int nsplit) {
   if (M == 0) return;
-  dim3 grid((unsigned)((M + GT_M - 1) / GT_M), (N + GT_N - 1) / GT_N);
+  const int nT = (N + GT_N - 1) / GT_N;
+  const int64_t blocks = ((M + GT_M - 1) / GT_M) * nT;
   if (nsplit == 3)
-    k_gemm_bf16<3><<<grid, 256, 0, s>>>(A, lda, Whi, Wlo, bias, C, ldc, M, N, K, rowflag, period);
+    k_gemm_bf16<3><<<(unsigned)blocks, 256, 0, s>>>(A, lda, Whi, Wlo, bias, C, ldc, M, N, K, rowflag, period, nT);
   else
-    k_gemm_bf16<1><<<grid, 256, 0, s>>>(A, lda, Whi, Whi, bias, C, ldc, M, N, K, rowflag, period);
+    k_gemm_bf16<1><<<(unsigned)blocks, 256, 0, s>>>(A, lda, Whi, Whi, bias, C, ldc, M, N, K, rowflag, period, nT);
 }
 
 }  // namespace fire

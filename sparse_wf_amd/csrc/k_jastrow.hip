@@ -118,43 +118,53 @@ void launch_dep_gather(hipStream_t s, const float* X, int64_t ld, int ncols, Gro
 
 // Attention Jastrow per group: a = sum_k q[h,k] keys[h,k]; att = exp(-a / sqrt(Dk)) and
 // vals = att * v with forward-Laplacian rows.  QV rows = [q (H*Dk) | v (H*Dk)];
-// AV rows = [att (H) | vals (H*Dk)].
+// AV rows = [att (H) | vals (H*Dk)].  Thread (h, k) owns column h*Dk + k; the head sum is
+// a shuffle reduction over the Dk consecutive lanes of the head (Dk a power of two <= 64),
+// so every row is read once with coalesced loads.
 template <bool TAN>
-__global__ void k_att_group(const float* __restrict__ QV, int H, int Dk, GroupDesc gd,
-                            const float* __restrict__ keys, float* __restrict__ AV) {
+__global__ void __launch_bounds__(256) k_att_group(const float* __restrict__ QV, int H, int Dk, GroupDesc gd,
+                                                   const float* __restrict__ keys, float* __restrict__ AV) {
   const int g = blockIdx.x;
   const int HD = H * Dk;
   const int col = threadIdx.x;  // (h, k)
-  if (col >= HD) return;
-  const int h = col / Dk, k = col - h * Dk;
+  const bool live = col < HD;
+  const int h = live ? col / Dk : 0, k = live ? col - h * Dk : 1;
   const float sc = rsqrtf((float)Dk);
+  const float key = live ? keys[col] : 0.f;
   const int64_t r0 = gd.row(g), r1 = TAN ? gd.row(g + 1) : r0 + 1;
   const int ldq = 2 * HD, lda = H + HD;
-  auto att_pre = [&](int64_t rr) {
-    float a = 0.f;
-    for (int q = 0; q < Dk; ++q) a += QV[rr * ldq + h * Dk + q] * keys[h * Dk + q];
-    return a;
+  auto head_sum = [&](float p) {
+    for (int o = 1; o < Dk; o <<= 1) p += __shfl_xor(p, o, 64);
+    return p;
   };
-  const float ax = att_pre(r0);
+  auto pre = [&](int64_t rr) { return head_sum(live ? QV[rr * ldq + col] * key : 0.f); };
+  const float ax = pre(r0);
   const float f = expf(-sc * ax), f1 = -sc * f, f2 = sc * sc * f;
-  const float vx = QV[r0 * ldq + HD + col];
-  AV[r0 * lda + HD * 0 + H + col] = f * vx;
-  if (k == 0) AV[r0 * lda + h] = f;
+  const float vx = live ? QV[r0 * ldq + HD + col] : 0.f;
+  if (live) {
+    AV[r0 * lda + H + col] = f * vx;
+    if (k == 0) AV[r0 * lda + h] = f;
+  }
   if (!TAN) return;
+  const float a_l = pre(r0 + 1);
   float ss = 0.f, cross = 0.f;
   for (int64_t rr = r0 + 2; rr < r1; ++rr) {
-    const float a_t = att_pre(rr);
+    const float a_t = pre(rr);
     const float at = a_t * f1;
     ss += a_t * a_t;
-    const float vt = QV[rr * ldq + HD + col];
-    cross += at * vt;
-    AV[rr * lda + H + col] = at * vx + f * vt;
-    if (k == 0) AV[rr * lda + h] = at;
+    if (live) {
+      const float vt = QV[rr * ldq + HD + col];
+      cross += at * vt;
+      AV[rr * lda + H + col] = at * vx + f * vt;
+      if (k == 0) AV[rr * lda + h] = at;
+    }
   }
-  const float al = att_pre(r0 + 1) * f1 + f2 * ss;
-  const float vl = QV[(r0 + 1) * ldq + HD + col];
-  AV[(r0 + 1) * lda + H + col] = al * vx + f * vl + 2.f * cross;
-  if (k == 0) AV[(r0 + 1) * lda + h] = al;
+  if (live) {
+    const float al = a_l * f1 + f2 * ss;
+    const float vl = QV[(r0 + 1) * ldq + HD + col];
+    AV[(r0 + 1) * lda + H + col] = al * vx + f * vl + 2.f * cross;
+    if (k == 0) AV[(r0 + 1) * lda + h] = al;
+  }
 }
 
 void launch_att_group(hipStream_t s, const float* QV, int H, int Dk, int G, GroupDesc gd, const float* keys,

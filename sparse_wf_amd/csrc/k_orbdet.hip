@@ -11,77 +11,48 @@
 
 namespace fire {
 
-#define ENV_COEF_MAX 1600  // (nuclei x dets x envelopes) per LDS chunk
-#define ENV_QMAX 8         // orbital columns per thread: n_el * n_det <= 8 * 256
-
-// Envelope rows of one electron: env[d,o] = sum_{n,e} pi[n,d,e,o] exp(-sp[n,d,e] |r - R_n|).
-// d/dr_t exp(-s d_n) = -s q n_t ; Laplacian = q (s^2 - 2 s / d_n).
+// Envelope coefficients of one electron for EfficientIsotropicEnvelopes:
+//   env[d,o] = sum_{n,e} pi[n,d,e,o] q_{n,d,e},  q = exp(-sp[n,d,e] |r - R_n|),
+//   d/dr_t q = -sp q n_t,  Laplacian q = q (sp^2 - 2 sp / |r - R_n|).
+// Written as coef[d][g*NR + row][n*E + e] so that the contraction with pi (stored as
+// [d][n*E + e][o]) is one FP32 GEMM per (spin block, determinant).
 template <bool TAN>
-__global__ void __launch_bounds__(256) k_envelopes(const float* __restrict__ r, const float* __restrict__ R, int B,
-                                                   int N, int n_up, int M, int D, int E, const float* __restrict__ sp_up,
-                                                   const float* __restrict__ pi_up, const float* __restrict__ sp_dn,
-                                                   const float* __restrict__ pi_dn, float* __restrict__ out) {
-  __shared__ float coef[5][ENV_COEF_MAX];
+__global__ void k_env_coef(const float* __restrict__ r, const float* __restrict__ R, int B, int N, int n_up, int M,
+                           int D, int E, const float* __restrict__ sp_up, const float* __restrict__ sp_dn,
+                           float* __restrict__ coef) {
   const int g = blockIdx.x;
   int w, i;
   group_inv(g, B, n_up, N - n_up, w, i);
   const float* ri = r + ((size_t)w * N + i) * 3;
   const float* sp = i < n_up ? sp_up : sp_dn;
-  const float* pi = i < n_up ? pi_up : pi_dn;
-  const int ND = N * D;
   const int NR = TAN ? 5 : 1;
-  const int per_nuc = D * E;
-  const int mch = max(1, ENV_COEF_MAX / per_nuc);
-  float acc[ENV_QMAX][5];
-  for (int q = 0; q < ENV_QMAX; ++q)
-    for (int rr = 0; rr < 5; ++rr) acc[q][rr] = 0.f;
-  for (int m0 = 0; m0 < M; m0 += mch) {
-    const int mn = min(mch, M - m0);
-    for (int idx = threadIdx.x; idx < mn * per_nuc; idx += blockDim.x) {
-      const int m = m0 + idx / per_nuc;
-      const float dx = ri[0] - R[3 * m], dy = ri[1] - R[3 * m + 1], dz = ri[2] - R[3 * m + 2];
-      const float dn = sqrtf(dx * dx + dy * dy + dz * dz);
-      const float s = sp[(size_t)m * per_nuc + idx % per_nuc];
-      const float q = expf(-s * dn);
-      coef[0][idx] = q;
-      if (TAN) {
-        const float inv = 1.0f / dn;
-        coef[1][idx] = q * (s * s - 2.0f * s * inv);
-        coef[2][idx] = -s * q * dx * inv;
-        coef[3][idx] = -s * q * dy * inv;
-        coef[4][idx] = -s * q * dz * inv;
-      }
+  const int ME = M * E;
+  const size_t G = (size_t)B * N;
+  for (int idx = threadIdx.x; idx < M * D * E; idx += blockDim.x) {
+    const int m = idx / (D * E), de = idx - m * (D * E), d = de / E, e = de - d * E;
+    const float dx = ri[0] - R[3 * m], dy = ri[1] - R[3 * m + 1], dz = ri[2] - R[3 * m + 2];
+    const float dn = sqrtf(dx * dx + dy * dy + dz * dz);
+    const float s = sp[idx];
+    const float q = expf(-s * dn);
+    float* c = coef + (size_t)d * G * NR * ME + (size_t)g * NR * ME + m * E + e;
+    c[0] = q;
+    if (TAN) {
+      const float inv = 1.0f / dn;
+      c[ME] = q * (s * s - 2.0f * s * inv);
+      c[2 * ME] = -s * q * dx * inv;
+      c[3 * ME] = -s * q * dy * inv;
+      c[4 * ME] = -s * q * dz * inv;
     }
-    __syncthreads();
-    for (int q = 0; q * (int)blockDim.x + (int)threadIdx.x < ND && q < ENV_QMAX; ++q) {
-      const int c = q * blockDim.x + threadIdx.x;
-      const int d = c / N, o = c - d * N;
-      for (int mm = 0; mm < mn; ++mm) {
-        const int m = m0 + mm;
-        for (int e = 0; e < E; ++e) {
-          const int li = mm * per_nuc + d * E + e;
-          const float p = pi[(((size_t)m * D + d) * E + e) * N + o];
-          for (int rr = 0; rr < NR; ++rr) acc[q][rr] += coef[rr][li] * p;
-        }
-      }
-    }
-    __syncthreads();
-  }
-  for (int q = 0; q < ENV_QMAX; ++q) {
-    const int c = q * blockDim.x + threadIdx.x;
-    if (c >= ND) break;
-    for (int rr = 0; rr < NR; ++rr) out[((size_t)g * NR + rr) * ND + c] = acc[q][rr];
   }
 }
 
-void launch_envelopes(hipStream_t s, const float* r, const float* R, int B, int N, int n_up, int M, int D, int E,
-                      const float* sp_up, const float* pi_up, const float* sp_dn, const float* pi_dn, bool tangents,
-                      float* out) {
+void launch_env_coef(hipStream_t s, const float* r, const float* R, int B, int N, int n_up, int M, int D, int E,
+                     const float* sp_up, const float* sp_dn, bool tangents, float* coef) {
   const int G = B * N;
   if (tangents)
-    k_envelopes<true><<<G, 256, 0, s>>>(r, R, B, N, n_up, M, D, E, sp_up, pi_up, sp_dn, pi_dn, out);
+    k_env_coef<true><<<G, 256, 0, s>>>(r, R, B, N, n_up, M, D, E, sp_up, sp_dn, coef);
   else
-    k_envelopes<false><<<G, 256, 0, s>>>(r, R, B, N, n_up, M, D, E, sp_up, pi_up, sp_dn, pi_dn, out);
+    k_env_coef<false><<<G, 256, 0, s>>>(r, R, B, N, n_up, M, D, E, sp_up, sp_dn, coef);
 }
 
 // Phi = orbital rows x envelope rows (product with a one-electron function), in place.
@@ -225,9 +196,137 @@ __global__ void __launch_bounds__(256) k_lu_inv(double* __restrict__ Aall, int N
   }
 }
 
-void launch_lu_inv(hipStream_t s, double* A, int n_mat, int N, double* Ainv, double* logdet, double* sign,
-                   bool want_inv) {
-  k_lu_inv<<<n_mat, 256, N * sizeof(int), s>>>(A, N, Ainv, logdet, sign, want_inv ? 1 : 0);
+// Gauss-Jordan inversion with partial pivoting (first maximum wins), in place, FP64, with
+// the matrix resident in LDS (N*N doubles + pivots; N <= 141 with gfx950's 160 KB).
+// Step k: pivot search in column k, row swap, scale the pivot row, eliminate column k from
+// every other row (all (i, j) updates in parallel).  log|det| = sum log|pivot|,
+// sign = (-1)^(#swaps + #negative pivots) -- the same as LU with partial pivoting
+// (slog_and_inverse, model/utils.py:144-175).  The column swaps at the end undo the row
+// permutation.  Writes Ainv (row-major FP64) and AinvT32 (transposed, FP32) when requested.
+__global__ void __launch_bounds__(512) k_gj_inv_lds(const double* __restrict__ Aall, int N,
+                                                    double* __restrict__ Ainv_all, float* __restrict__ AinvT32,
+                                                    double* __restrict__ logdet, double* __restrict__ sign,
+                                                    int want_inv) {
+  extern __shared__ unsigned char smem_raw[];
+  double* A = reinterpret_cast<double*>(smem_raw);
+  double* colk = A + (size_t)N * N;  // column k before elimination
+  int* piv = reinterpret_cast<int*>(colk + N);
+  __shared__ double s_val[8];
+  __shared__ int s_idx[8];
+  __shared__ int s_p;
+  __shared__ double s_pivot;
+  const int mat = blockIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6, nthr = blockDim.x;
+  const double* Ag = Aall + (size_t)mat * N * N;
+  for (int e = tid; e < N * N; e += nthr) A[e] = Ag[e];
+  __syncthreads();
+  double ld = 0.0;
+  int flips = 0;
+  for (int k = 0; k < N; ++k) {
+    double best = -1.0;
+    int bi = N;
+    for (int rr = k + tid; rr < N; rr += nthr) {
+      const double v = fabs(A[rr * N + k]);
+      if (v > best || (v == best && rr < bi)) { best = v; bi = rr; }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      const double ov = __shfl_xor(best, o, 64);
+      const int oi = __shfl_xor(bi, o, 64);
+      if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
+    }
+    if (lane == 0) { s_val[wid] = best; s_idx[wid] = bi; }
+    __syncthreads();
+    if (tid == 0) {
+      double b = s_val[0];
+      int ii = s_idx[0];
+      for (int q = 1; q < nthr >> 6; ++q)
+        if (s_val[q] > b || (s_val[q] == b && s_idx[q] < ii)) { b = s_val[q]; ii = s_idx[q]; }
+      s_p = ii;
+      piv[k] = ii;
+    }
+    __syncthreads();
+    const int p = s_p;
+    if (p != k) {
+      for (int cc = tid; cc < N; cc += nthr) {
+        const double t = A[k * N + cc];
+        A[k * N + cc] = A[p * N + cc];
+        A[p * N + cc] = t;
+      }
+      __syncthreads();
+    }
+    if (tid == 0) s_pivot = A[k * N + k];
+    __syncthreads();
+    const double pv = s_pivot;
+    if (tid == 0) {
+      ld += log(fabs(pv));
+      flips += (p != k) + (pv < 0.0);
+    }
+    const double ip = 1.0 / pv;
+    // scale the pivot row (column k becomes 1/pivot); keep column k for the elimination
+    for (int cc = tid; cc < N; cc += nthr) A[k * N + cc] = (cc == k) ? ip : A[k * N + cc] * ip;
+    for (int rr = tid; rr < N; rr += nthr) colk[rr] = A[rr * N + k];
+    __syncthreads();
+    // eliminate column k from the other rows: row i -= A[i][k] * row k; column k -> -A[i][k]/pivot
+    for (int e = tid; e < N * N; e += nthr) {
+      const int i = e / N, j = e - i * N;
+      if (i == k) continue;
+      const double fk = colk[i];
+      if (j == k)
+        A[i * N + k] = -fk * ip;
+      else
+        A[i * N + j] -= fk * A[k * N + j];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    logdet[mat] = ld;
+    sign[mat] = (flips & 1) ? -1.0 : 1.0;
+  }
+  if (!want_inv) return;
+  // undo the row permutation as column swaps, last pivot first
+  for (int k = N - 1; k >= 0; --k) {
+    const int p = piv[k];
+    if (p != k)
+      for (int rr = tid; rr < N; rr += nthr) {
+        const double t = A[rr * N + k];
+        A[rr * N + k] = A[rr * N + p];
+        A[rr * N + p] = t;
+      }
+    __syncthreads();
+  }
+  double* X = Ainv_all + (size_t)mat * N * N;
+  float* XT = AinvT32 + (size_t)mat * N * N;
+  for (int e = tid; e < N * N; e += nthr) {
+    X[e] = A[e];
+    const int i = e / N, j = e - i * N;
+    XT[(size_t)j * N + i] = (float)A[e];
+  }
+}
+
+size_t lu_lds_bytes(int N) { return (size_t)N * N * sizeof(double) + (size_t)N * (sizeof(double) + sizeof(int)); }
+
+hipError_t lu_set_smem_limit(size_t bytes) {
+  return hipFuncSetAttribute((const void*)k_gj_inv_lds, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+__global__ void k_transpose_f32(const double* __restrict__ X, int N, float* __restrict__ XT) {
+  const size_t base = (size_t)blockIdx.x * N * N;
+  for (int e = threadIdx.x; e < N * N; e += blockDim.x) {
+    const int i = e / N, j = e - i * N;
+    XT[base + (size_t)j * N + i] = (float)X[base + e];
+  }
+}
+
+// Inverse, log|det| and sign of n_mat matrices.  LDS Gauss-Jordan when the matrix fits
+// (use_lds), else LU in global memory followed by an FP32 transposed copy.
+void launch_lu_inv(hipStream_t s, double* A, int n_mat, int N, double* Ainv, float* AinvT32, double* logdet,
+                   double* sign, bool want_inv, bool use_lds) {
+  if (use_lds) {
+    k_gj_inv_lds<<<n_mat, 512, lu_lds_bytes(N), s>>>(A, N, Ainv, AinvT32, logdet, sign, want_inv ? 1 : 0);
+  } else {
+    k_lu_inv<<<n_mat, 256, N * sizeof(int), s>>>(A, N, Ainv, logdet, sign, want_inv ? 1 : 0);
+    if (want_inv) k_transpose_f32<<<n_mat, 256, 0, s>>>(Ainv, N, AinvT32);
+  }
 }
 
 // tr(Ainv * Lap A) for every (walker, det): sum_i sum_o lap_i[d,o] Ainv[o,i].
@@ -267,7 +366,7 @@ typedef float float4_t __attribute__((ext_vector_type(4)));
 __global__ void __launch_bounds__(256) k_det_lap(const float* __restrict__ O, int ND, GroupDesc gd,
                                                  const int64_t* __restrict__ ptr, const int32_t* __restrict__ nbr,
                                                  const int32_t* __restrict__ rev, int B, int N, int n_up, int D,
-                                                 int smax, const double* __restrict__ Ainv, double* __restrict__ trip,
+                                                 int smax, const float* __restrict__ AinvT, double* __restrict__ trip,
                                                  double* __restrict__ grad) {
   extern __shared__ unsigned char smem_raw[];
   const int RTm = (3 * smax + 15) / 16, CTm = (smax + 15) / 16;
@@ -276,7 +375,8 @@ __global__ void __launch_bounds__(256) k_det_lap(const float* __restrict__ O, in
   float* Xs = Js + RTm * 16 * LDK;
   float* Bs = Xs + CTm * 16 * LDK;
   int* s_idx = reinterpret_cast<int*>(Bs + ((3 * smax * smax + 1) & ~1));
-  int64_t* s_row = reinterpret_cast<int64_t*>(s_idx + smax + (smax & 1));
+  int64_t* s_row = reinterpret_cast<int64_t*>(s_idx + smax + (smax & 1));  // first jacobian row of a
+  int64_t* s_rp = s_row + smax;                                             // element offset of row (a, x)
   __shared__ double red[16 * 4];
   const int blk = blockIdx.x;
   const int n = blk % N;
@@ -301,45 +401,54 @@ __global__ void __launch_bounds__(256) k_det_lap(const float* __restrict__ O, in
   float4_t acc[DL_TPW];
   for (int t = 0; t < DL_TPW; ++t) acc[t] = (float4_t){0.f, 0.f, 0.f, 0.f};
   __syncthreads();
-  const double* X = Ainv + (size_t)wd * N * N;
-  const float* Od = O + (size_t)d * N;
+  for (int ax = tid; ax < 3 * s; ax += blockDim.x) {
+    const int a = ax / 3, x = ax - 3 * a;
+    s_rp[ax] = (s_row[a] + x) * ND + (int64_t)d * N;
+  }
+  __syncthreads();
+  const float* XT = AinvT + (size_t)wd * N * N;
   for (int o0 = 0; o0 < N; o0 += DL_KC) {
+    // 32 consecutive threads read 32 consecutive orbitals of one row (DL_KC == 32)
     for (int e = tid; e < RT * 16 * DL_KC; e += blockDim.x) {
-      const int ax = e / DL_KC, k = e - ax * DL_KC;
+      const int ax = e >> 5, k = e & 31;
       float v = 0.f;
-      if (ax < 3 * s && o0 + k < N) {
-        const int a = ax / 3, x = ax - 3 * a;
-        v = Od[(s_row[a] + x) * ND + o0 + k];
-      }
+      if (ax < 3 * s && o0 + k < N) v = O[s_rp[ax] + o0 + k];
       Js[ax * LDK + k] = v;
     }
     for (int e = tid; e < CT * 16 * DL_KC; e += blockDim.x) {
-      const int k = e / (CT * 16), b = e - k * (CT * 16);
+      const int b = e >> 5, k = e & 31;
       float v = 0.f;
-      if (b < s && o0 + k < N) v = (float)X[(size_t)(o0 + k) * N + s_idx[b]];
+      if (b < s && o0 + k < N) v = XT[(size_t)s_idx[b] * N + o0 + k];
       Xs[b * LDK + k] = v;
     }
     __syncthreads();
-    int q = 0;
-    for (int t = wid; t < NT; t += 4, ++q) {
-      const int rt = t / CT, ct = t - rt * CT;
-      const float* ja = Js + (rt * 16 + (lane & 15)) * LDK + (lane >> 4);
-      const float* xb = Xs + (ct * 16 + (lane & 15)) * LDK + (lane >> 4);
 #pragma unroll
-      for (int kk = 0; kk < DL_KC; kk += 4) acc[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(ja[kk], xb[kk], acc[q], 0, 0, 0);
+    for (int q = 0; q < DL_TPW; ++q) {
+      const int t = wid + 4 * q;
+      if (t < NT) {
+        const int rt = t / CT, ct = t - rt * CT;
+        const float* ja = Js + (rt * 16 + (lane & 15)) * LDK + (lane >> 4);
+        const float* xb = Xs + (ct * 16 + (lane & 15)) * LDK + (lane >> 4);
+#pragma unroll
+        for (int kk = 0; kk < DL_KC; kk += 4)
+          acc[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(ja[kk], xb[kk], acc[q], 0, 0, 0);
+      }
     }
     __syncthreads();
   }
   // scatter the 16x16 tiles into B[ax][b]
   {
-    int q = 0;
-    for (int t = wid; t < NT; t += 4, ++q) {
-      const int rt = t / CT, ct = t - rt * CT;
-      const int b = ct * 16 + (lane & 15);
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int ax = rt * 16 + 4 * (lane >> 4) + v;
-        if (ax < 3 * s && b < s) Bs[ax * s + b] = acc[q][v];
+    for (int q = 0; q < DL_TPW; ++q) {
+      const int t = wid + 4 * q;
+      if (t < NT) {
+        const int rt = t / CT, ct = t - rt * CT;
+        const int b = ct * 16 + (lane & 15);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int ax = rt * 16 + 4 * (lane >> 4) + v;
+          if (ax < 3 * s && b < s) Bs[ax * s + b] = acc[q][v];
+        }
       }
     }
   }
@@ -362,14 +471,14 @@ __global__ void __launch_bounds__(256) k_det_lap(const float* __restrict__ O, in
 size_t det_lap_smem(int smax) {
   const int RTm = (3 * smax + 15) / 16, CTm = (smax + 15) / 16;
   return (size_t)(RTm * 16 + CTm * 16) * (DL_KC + 4) * 4 + (size_t)((3 * smax * smax + 1) & ~1) * 4 +
-         (size_t)(smax + (smax & 1)) * 4 + (size_t)smax * 8;
+         (size_t)(smax + (smax & 1)) * 4 + (size_t)smax * 8 + (size_t)3 * smax * 8;
 }
 
 void launch_det_lap(hipStream_t s, const float* O, int ND, GroupDesc gd, const int64_t* ptr, const int32_t* nbr,
-                    const int32_t* rev, int B, int N, int n_up, int D, int smax, const double* Ainv, double* trip,
+                    const int32_t* rev, int B, int N, int n_up, int D, int smax, const float* AinvT, double* trip,
                     double* grad) {
   const size_t sm = det_lap_smem(smax);
-  k_det_lap<<<B * D * N, 256, sm, s>>>(O, ND, gd, ptr, nbr, rev, B, N, n_up, D, smax, Ainv, trip, grad);
+  k_det_lap<<<B * D * N, 256, sm, s>>>(O, ND, gd, ptr, nbr, rev, B, N, n_up, D, smax, AinvT, trip, grad);
 }
 
 hipError_t det_lap_set_smem_limit(size_t bytes) {

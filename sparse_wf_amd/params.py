@@ -245,6 +245,29 @@ def _swap_perm(N: int, D: int, n_up: int) -> np.ndarray:
     return np.concatenate([d * N + (o + n_up) % N for d in range(D)])
 
 
+def padded_w1(w1: int) -> int:
+    """Filter width the kernels are compiled for (4, 8 or 16); narrower filters are zero-padded,
+    which adds exact zeros to every contraction."""
+    for w in (4, 8, 16):
+        if w1 <= w:
+            return w
+    raise ValueError(f"pair_mlp_widths[1] = {w1} > 16 is not supported")
+
+
+def _pad_cols(a, n):
+    a = np.asarray(a, np.float64)
+    out = np.zeros(a.shape[:-1] + (n,))
+    out[..., : a.shape[-1]] = a
+    return out
+
+
+def _pad_rows(a, n):
+    a = np.asarray(a, np.float64)
+    out = np.zeros((n,) + a.shape[1:])
+    out[: a.shape[0]] = a
+    return out
+
+
 def pack_params(cfg: FireConfig, params: dict) -> list:
     """Flatten ``params`` into the canonical float32 tensor list of ``fire_upload_params``.
 
@@ -256,6 +279,7 @@ def pack_params(cfg: FireConfig, params: dict) -> list:
     parameters are the up-spin ones with the columns of _swap_spin_updn permuted
     (orbitals.py:88-107); learned cutoff = cutoff * sigmoid(cutoff_param) (:644)."""
     f32 = lambda a: np.ascontiguousarray(np.asarray(a, np.float64).astype(np.float32))
+    W1p = padded_w1(cfg.pair_mlp_widths[1])
     ep = params["embedding"]
     s0, s1 = (float(v) for v in ep["scales"])
     dyn = ep["dynamic_params_en"]
@@ -263,9 +287,10 @@ def pack_params(cfg: FireConfig, params: dict) -> list:
     edge = ei["ElecNucEdge_0"]
     out = [
         dyn["filter"]["kernel"], dyn["filter"]["bias"], _softplus(dyn["filter"]["scales"]), dyn["nuc_embedding"],
-        edge["PairwiseFilter_0"]["Dense_0"]["kernel"], edge["PairwiseFilter_0"]["Dense_0"]["bias"],
-        edge["PairwiseFilter_0"]["Dense_1"]["kernel"],
-        edge["Dense_0"]["kernel"], edge["Dense_1"]["kernel"],
+        _pad_cols(edge["PairwiseFilter_0"]["Dense_0"]["kernel"], W1p),
+        _pad_cols(edge["PairwiseFilter_0"]["Dense_0"]["bias"], W1p),
+        _pad_cols(edge["PairwiseFilter_0"]["Dense_1"]["kernel"], W1p),
+        _pad_rows(edge["Dense_0"]["kernel"], W1p), edge["Dense_1"]["kernel"],
         ei["LayerNorm_0"]["scale"], ei["LayerNorm_0"]["bias"],
         ei["GatedLinearUnit_0"]["Dense_0"]["kernel"],
         ei["Dense_0"]["kernel"], ei["Dense_0"]["bias"], ei["Dense_1"]["kernel"], ei["Dense_1"]["bias"],
@@ -273,8 +298,9 @@ def pack_params(cfg: FireConfig, params: dict) -> list:
     for name in ("edge_same", "edge_diff"):
         q = ep[name]["params"]
         b = q["beta_ee"]
-        out += [b["kernel"], b["bias"], _softplus(b["scale"]), b["Dense_0"]["kernel"], b["Dense_0"]["bias"],
-                b["Dense_1"]["kernel"], q["Dense_0"]["kernel"], q["Dense_1"]["kernel"], q["Dense_1"]["bias"]]
+        out += [b["kernel"], b["bias"], _softplus(b["scale"]), _pad_cols(b["Dense_0"]["kernel"], W1p),
+                _pad_cols(b["Dense_0"]["bias"], W1p), _pad_cols(b["Dense_1"]["kernel"], W1p),
+                _pad_rows(q["Dense_0"]["kernel"], W1p), q["Dense_1"]["kernel"], q["Dense_1"]["bias"]]
     up = ep["updates"][0]["params"]
     out += [np.asarray(up["Linear_0"]["kernel"], np.float64) * s0, up["Linear_0"]["bias"],
             up["Linear_1"]["kernel"], up["Linear_1"]["bias"]]
@@ -294,6 +320,9 @@ def pack_params(cfg: FireConfig, params: dict) -> list:
         b_dn = np.asarray(op["to_orbitals_dn"]["bias"], np.float64)
         sp_dn = _softplus(op["env_dn"]["sigma"])
         pi_dn = np.asarray(op["env_dn"]["pi"], np.float64)
+    # pi as [det][nucleus * envelope][orbital] (one GEMM operand per determinant)
+    pi_up = np.transpose(pi_up, (1, 0, 2, 3))
+    pi_dn = np.transpose(pi_dn, (1, 0, 2, 3))
     out += [W_up * s1, b_up, W_dn * s1, b_dn, sp_up, pi_up, sp_dn, pi_dn]
     jp = params["jastrow"]["params"]
     if cfg.use_mlp_jastrow or cfg.use_log_jastrow:

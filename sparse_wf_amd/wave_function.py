@@ -20,7 +20,7 @@ import torch
 
 from . import _lib
 from .config import FireConfig, Molecule
-from .params import init_params, pack_params, set_scales
+from .params import init_params, pack_params, padded_w1, set_scales
 
 PRECISIONS = {"fp32": _lib.FIRE_FP32, "mixed": _lib.FIRE_MIXED, "bf16": _lib.FIRE_BF16}
 
@@ -57,7 +57,7 @@ class FireWaveFunction:
         lib = _lib.load()
         c = _lib.FireConfigC(
             n_el=cfg.n_el, n_up=cfg.n_up, n_nuc=cfg.n_nuc, feature_dim=cfg.feature_dim,
-            filter_w0=cfg.pair_mlp_widths[0], filter_w1=cfg.pair_mlp_widths[1],
+            filter_w0=cfg.pair_mlp_widths[0], filter_w1=padded_w1(cfg.pair_mlp_widths[1]),
             n_env_en=cfg.pair_n_envelopes, n_det=cfg.n_determinants, n_env_iso=cfg.n_envelopes,
             e_e_cusp=1 if cfg.e_e_cusps == "psiformer" else 0,
             use_mlp_jastrow=int(cfg.use_mlp_jastrow), use_log_jastrow=int(cfg.use_log_jastrow),
