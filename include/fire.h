@@ -89,6 +89,58 @@ int fire_signed(fire_handle* h, void* stream, const float* r_dev, int32_t B, int
 int fire_embedding(fire_handle* h, void* stream, const float* r_dev, int32_t B, int32_t precision,
                    float* h_contr_dev, float* h_out_dev);
 
+/* ---- resident walker state and single-electron low-rank updates -------------------------
+ * The handle keeps one walker state: positions, the h0/message rows, embeddings, raw
+ * orbitals and envelopes, FP64 Slater matrices with their inverses and log-determinants,
+ * and the per-electron Jastrow rows of B walkers.  All entry points below are
+ * stream-ordered on `stream`; calls that overlap on one handle race. */
+
+/* Evaluates B walkers r_dev [B, n_el, 3] (value path) and keeps their state; logpsi_dev /
+ * sign_dev (optional, double [B]) receive log|psi| and sign.  `precision` applies to every
+ * later update of this state.
+ * Replaces MoonLikeWaveFunction.log_psi_with_state (src/sparse_wf/model/wave_function.py:226-249). */
+int fire_state_init(fire_handle* h, void* stream, const float* r_dev, int32_t B, int32_t precision,
+                    double* logpsi_dev, double* sign_dev);
+
+/* Proposal: electron k of every walker moved to rk_dev [B, 3].  Embeddings of the electrons
+ * within the cutoff of k before or after the move, the orbital rows of those electrons, the
+ * envelope row of k, the Jastrow and the determinants (Woodbury, FP64) are updated; the
+ * state is unchanged until fire_state_commit.  logpsi_dev / sign_dev (optional, [B])
+ * receive log|psi| and sign of the proposals.
+ * Replaces MoonLikeWaveFunction.log_psi_low_rank_update (wave_function.py:251-287) with
+ * NewSparseEmbedding.low_rank_update (new_sparse_model.py:553-629), Orbitals.low_rank_update
+ * (orbitals.py:121-146), woodburry_update (model/utils.py:216-244) and
+ * Jastrow.low_rank_update (jastrow.py:309-361). */
+int fire_state_propose(fire_handle* h, void* stream, int32_t k, const float* rk_dev, double* logpsi_dev,
+                       double* sign_dev);
+
+/* Applies the pending proposal to the walkers with accept_dev[w] != 0 (int32 [B]).
+ * Replaces the jnp.where(accept, new, old) selection of the state in mcmc_steps_low_rank
+ * (src/sparse_wf/mcmc.py:240-244). */
+int fire_state_commit(fire_handle* h, void* stream, const int32_t* accept_dev);
+
+/* n_steps Metropolis steps on the state: step s moves electron s mod n_el of every walker by
+ * width * N(0, 1) per coordinate and accepts when 2 (log|psi'| - log|psi|) > log u,
+ * u ~ U(0, 1).  Random numbers are Philox4x32-10 keyed by (seed, walker) with counter s, so
+ * a run is reproducible and independent of how steps are split over calls (pass the running
+ * step index as step0).  n_accept_dev (optional, int32 [B]) receives the accepted moves of
+ * each walker in this call.  Synchronises once per step (row counts of the update).
+ * Replaces mcmc_steps_low_rank with proposal_single_electron (src/sparse_wf/mcmc.py:100-111,
+ * 196-275). */
+int fire_mcmc_steps(fire_handle* h, void* stream, int32_t n_steps, float width, uint64_t seed, int64_t step0,
+                    int32_t* n_accept_dev);
+
+/* Copies the state's positions (float [B, n_el, 3]), log|psi| and sign (double [B]) out; any
+ * may be NULL.  A pending proposal is dropped.  (No counterpart: the reference returns the
+ * electrons from mcmc_steps_low_rank, mcmc.py:260-275.) */
+int fire_state_get(fire_handle* h, void* stream, float* r_dev, double* logpsi_dev, double* sign_dev);
+
+/* Counters of the state's proposals since fire_state_init: walkers, proposals (walker-moves),
+ * sum and maximum of the number of changed electrons per move (the reference's static
+ * statistics of low-rank updates, mcmc.py:225-231).  Host outputs; any may be NULL. */
+int fire_state_stats(fire_handle* h, int32_t* n_walkers, int64_t* n_proposals, int64_t* sum_changed,
+                     int64_t* max_changed);
+
 /* Instrumentation (no counterpart in the reference): enables / disables HIP-event timing of
  * every dense-contraction launch of this handle.  Each call first reports what was recorded
  * since the previous call -- total kernel milliseconds, algorithmic FLOPs (2*M*N*K per

@@ -42,6 +42,13 @@ SIGNATURES = {
     "fire_embedding": (ctypes.c_int, [_P, _P, _P, ctypes.c_int32, ctypes.c_int32, _P, _P]),
     "fire_profile_gemm": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.POINTER(ctypes.c_double),
                                          ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]),
+    "fire_state_init": (ctypes.c_int, [_P, _P, _P, ctypes.c_int32, ctypes.c_int32, _P, _P]),
+    "fire_state_propose": (ctypes.c_int, [_P, _P, ctypes.c_int32, _P, _P, _P]),
+    "fire_state_commit": (ctypes.c_int, [_P, _P, _P]),
+    "fire_mcmc_steps": (ctypes.c_int, [_P, _P, ctypes.c_int32, ctypes.c_float, ctypes.c_uint64, ctypes.c_int64, _P]),
+    "fire_state_get": (ctypes.c_int, [_P, _P, _P, _P, _P]),
+    "fire_state_stats": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int64),
+                                        ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]),
     "fire_last_error": (ctypes.c_char_p, []),
 }
 

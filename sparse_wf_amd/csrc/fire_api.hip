@@ -12,6 +12,7 @@
 #include "../../include/fire.h"
 #include "fire_kernels.hpp"
 #include "fire_kernels2.hpp"
+#include "fire_mcmc.hpp"
 
 namespace {
 
@@ -99,6 +100,38 @@ struct fire_handle {
   bool gemm_timing = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> gemm_events;
   double gemm_flops = 0.0;
+  // Resident walker state for single-electron low-rank updates (fire_state_*, fire_mcmc_steps).
+  // Per-electron rows are in group order over all B walkers (up-spin electrons first).
+  struct State {
+    int B = 0, prec = 0;
+    bool ready = false, pending = false;
+    int k = -1, T = 0, T_up = 0, amax = 0;
+    int64_t n_proposals = 0, sum_changed = 0, max_changed = 0;
+    // state: positions [B,N,3]; h0/message rows [G,5F]; embedding [G,F]; raw orbitals and
+    // envelopes [G,N*D]; Slater matrices and inverses [B,D,N,N] FP64; log|det|, sign [B,D];
+    // MLP-Jastrow rows [G,2]; attention rows [G,H+H*Dk]; log psi, sign [B]
+    DevBuf r, M5, hh, orb, env, A, Ainv, ld, sg, jr, av, logpsi, sign;
+    // proposal: positions, moved electron, uniform variate, affected sets and row lists
+    DevBuf p_r, p_rk, p_u, aff, cu, cd, offu, offd, tail, list, rowof, klist;
+    // moved-electron rows (ElecInit, messages, envelope)
+    DevBuf kT5, kZ2, kT5b, p_M5k, kcoef, p_envk;
+    // compact rows of the affected electrons
+    DevBuf pcnt, pptr, pnbr, p_hc, p_x, p_h, p_y, p_orb, p_jr, p_qv, p_av;
+    // determinant and Jastrow of the proposal
+    DevBuf phin, sinv, ldS, sgS, jsum, rin, ry1, ry2, att_out, cval, clap, cgrad, epot;
+    DevBuf p_ld, p_sg, p_logpsi, p_sign, accept;
+    size_t wb_smem_set = 0;
+    void release() {
+      DevBuf* b[] = {&r,     &M5,    &hh,    &orb,     &env,   &A,      &Ainv,  &ld,    &sg,     &jr,    &av,
+                     &logpsi, &sign, &p_r,   &p_rk,    &p_u,   &aff,    &cu,    &cd,    &offu,   &offd,  &tail,
+                     &list,  &rowof, &klist, &kT5,     &kZ2,   &kT5b,   &p_M5k, &kcoef, &p_envk, &pcnt,  &pptr,
+                     &pnbr,  &p_hc,  &p_x,   &p_h,     &p_y,   &p_orb,  &p_jr,  &p_qv,  &p_av,   &phin,  &sinv,
+                     &ldS,   &sgS,   &jsum,  &rin,     &ry1,   &ry2,    &att_out, &cval, &clap,  &cgrad, &epot,
+                     &p_ld,  &p_sg,  &p_logpsi, &p_sign, &accept};
+      for (DevBuf* x : b) x->release();
+      ready = pending = false;
+    }
+  } st;
 };
 
 namespace {
@@ -326,8 +359,15 @@ struct DetOut {
   double *logpsi, *sign, *grad, *lap;
 };
 
+// Copies kept by the state initialisation (value rows only): raw orbital rows, envelope
+// rows, Slater matrices, inverses, log|det| and signs.
+struct Keep {
+  float *orb, *env;
+  double *A, *Ainv, *ld, *sg;
+};
+
 void run_orbitals_det(Ctx& c, const float* r, int B, const Graph& gr, bool tan, const GroupDesc& gd, int64_t R,
-                      DetOut& out) {
+                      DetOut& out, const Keep* keep = nullptr) {
   fire_handle* h = c.h;
   const int N = h->N, F = h->F, G = B * N, ND = h->ND, D = h->D;
   const int NR = tan ? 5 : 1;
@@ -351,6 +391,7 @@ void run_orbitals_det(Ctx& c, const float* r, int B, const Graph& gr, bool tan, 
   if (R > R_up_rows)
     gemm(c, h->XA.as<float>() + (size_t)R_up_rows * F, F, o + 2, o + 3, Ob + (size_t)R_up_rows * ND, ND,
          R - R_up_rows, ND, F, flag ? flag + R_up_rows : nullptr, period);
+  if (keep) HIPCHK(hipMemcpyAsync(keep->orb, Ob, sizeof(float) * (size_t)R * ND, hipMemcpyDeviceToDevice, c.s));
   // envelopes: coefficients per (electron, row, nucleus, envelope), then one FP32 GEMM per
   // (spin block, determinant) against pi stored as [det][nucleus * envelope][orbital]
   {
@@ -370,12 +411,16 @@ void run_orbitals_det(Ctx& c, const float* r, int B, const Graph& gr, bool tan, 
                         nr, N, ME, nullptr, 0);
     }
   }
+  if (keep)
+    HIPCHK(hipMemcpyAsync(keep->env, h->ENV.p, sizeof(float) * (size_t)G * ND, hipMemcpyDeviceToDevice, c.s));
   launch_orb_product(c.s, Ob, ND, G, gd, h->ENV.as<float>(), tan);
   const size_t nmat = (size_t)B * D;
   h->A64.ensure(sizeof(double) * nmat * N * N);
   h->ldet.ensure(sizeof(double) * nmat);
   h->dsgn.ensure(sizeof(double) * nmat);
   launch_gather_A(c.s, Ob, ND, gd, B, N, h->n_up, D, h->A64.as<double>());
+  if (keep)
+    HIPCHK(hipMemcpyAsync(keep->A, h->A64.p, sizeof(double) * nmat * N * N, hipMemcpyDeviceToDevice, c.s));
   h->det_lp.ensure(sizeof(double) * B);
   h->det_sg.ensure(sizeof(double) * B);
   out.logpsi = h->det_lp.as<double>();
@@ -408,14 +453,18 @@ void run_orbitals_det(Ctx& c, const float* r, int B, const Graph& gr, bool tan, 
     out.grad = h->det_gr.as<double>();
     out.lap = h->det_lap.as<double>();
   } else {
-    launch_lu_inv(c.s, h->A64.as<double>(), (int)nmat, N, nullptr, nullptr, h->ldet.as<double>(),
-                  h->dsgn.as<double>(), false, lu_lds);
+    launch_lu_inv(c.s, h->A64.as<double>(), (int)nmat, N, keep ? keep->Ainv : nullptr, nullptr, h->ldet.as<double>(),
+                  h->dsgn.as<double>(), keep != nullptr, lu_lds);
     out.grad = nullptr;
     out.lap = nullptr;
   }
   launch_det_combine(c.s, B, N, D, h->ldet.as<double>(), h->dsgn.as<double>(), tan ? h->tr.as<double>() : nullptr,
                      tan ? h->trip.as<double>() : nullptr, tan ? h->dgrad.as<double>() : nullptr, out.logpsi, out.sign,
                      out.grad, out.lap, tan);
+  if (keep) {
+    HIPCHK(hipMemcpyAsync(keep->ld, h->ldet.p, sizeof(double) * nmat, hipMemcpyDeviceToDevice, c.s));
+    HIPCHK(hipMemcpyAsync(keep->sg, h->dsgn.p, sizeof(double) * nmat, hipMemcpyDeviceToDevice, c.s));
+  }
 }
 
 void run_jastrow_and_assemble(Ctx& c, const float* r, int B, bool tan, const GroupDesc& gd, int64_t R,
@@ -523,6 +572,276 @@ void run_jastrow_and_assemble(Ctx& c, const float* r, int B, bool tan, const Gro
   launch_assemble(c.s, A);
 }
 
+// ---------------------------------------------------------------------------------------
+// Resident walker state and single-electron low-rank proposals
+// (log_psi_with_state wave_function.py:226-249, log_psi_low_rank_update :251-287,
+// mcmc_steps_low_rank mcmc.py:196-275).
+// ---------------------------------------------------------------------------------------
+
+// Value pass over all B walkers at once (one group order for the whole state) keeping every
+// per-electron quantity a later single-electron update reuses.
+void state_init(Ctx& c, const float* r_dev, int B) {
+  fire_handle* h = c.h;
+  auto& S = h->st;
+  const int N = h->N, F = h->F, G = B * N, ND = h->ND, D = h->D;
+  S.ready = S.pending = false;
+  S.B = B;
+  S.prec = c.prec;
+  S.n_proposals = S.sum_changed = S.max_changed = 0;
+  S.r.ensure(sizeof(float) * (size_t)G * 3);
+  S.p_r.ensure(sizeof(float) * (size_t)G * 3);
+  HIPCHK(hipMemcpyAsync(S.r.p, r_dev, sizeof(float) * (size_t)G * 3, hipMemcpyDeviceToDevice, c.s));
+  HIPCHK(hipMemcpyAsync(S.p_r.p, r_dev, sizeof(float) * (size_t)G * 3, hipMemcpyDeviceToDevice, c.s));
+  Graph g = build_graph(c, S.r.as<float>(), B);
+  GroupDesc gd;
+  int64_t R;
+  run_embedding(c, S.r.as<float>(), B, g, false, gd, R);
+  S.M5.ensure(sizeof(float) * (size_t)G * 5 * F);
+  S.hh.ensure(sizeof(float) * (size_t)G * F);
+  HIPCHK(hipMemcpyAsync(S.M5.p, h->M5.p, sizeof(float) * (size_t)G * 5 * F, hipMemcpyDeviceToDevice, c.s));
+  HIPCHK(hipMemcpyAsync(S.hh.p, h->XA.p, sizeof(float) * (size_t)G * F, hipMemcpyDeviceToDevice, c.s));
+  const size_t nmat = (size_t)B * D;
+  S.orb.ensure(sizeof(float) * (size_t)G * ND);
+  S.env.ensure(sizeof(float) * (size_t)G * ND);
+  S.A.ensure(sizeof(double) * nmat * N * N);
+  S.Ainv.ensure(sizeof(double) * nmat * N * N);
+  S.ld.ensure(sizeof(double) * nmat);
+  S.sg.ensure(sizeof(double) * nmat);
+  Keep keep{S.orb.as<float>(), S.env.as<float>(), S.A.as<double>(), S.Ainv.as<double>(), S.ld.as<double>(),
+            S.sg.as<double>()};
+  DetOut det;
+  run_orbitals_det(c, S.r.as<float>(), B, g, false, gd, R, det, &keep);
+  S.logpsi.ensure(sizeof(double) * B);
+  S.sign.ensure(sizeof(double) * B);
+  run_jastrow_and_assemble(c, S.r.as<float>(), B, false, gd, R, det, nullptr, nullptr, nullptr, S.logpsi.as<double>(),
+                           S.sign.as<double>(), nullptr, nullptr);
+  if (h->has_mlp) {
+    S.jr.ensure(sizeof(float) * (size_t)G * 2);
+    HIPCHK(hipMemcpyAsync(S.jr.p, h->JR.p, sizeof(float) * (size_t)G * 2, hipMemcpyDeviceToDevice, c.s));
+  }
+  if (h->has_att) {
+    const size_t nc = (size_t)h->H + h->HD;
+    S.av.ensure(sizeof(float) * (size_t)G * nc);
+    HIPCHK(hipMemcpyAsync(S.av.p, h->AV.p, sizeof(float) * (size_t)G * nc, hipMemcpyDeviceToDevice, c.s));
+  }
+  S.accept.ensure(sizeof(int32_t) * B);
+  S.ready = true;
+}
+
+void state_commit(Ctx& c);
+
+// Put the positions of a pending, uncommitted proposal back to the state's.
+void state_drop_pending(Ctx& c) {
+  auto& S = c.h->st;
+  if (!S.pending) return;
+  launch_fill_i32(c.s, S.accept.as<int32_t>(), S.B, 0);
+  state_commit(c);
+}
+
+// Proposal for electron k of every walker: mode 0 Gaussian step (width, seed, step),
+// mode 1 new positions rk_in [B,3].  Leaves log psi / sign of the proposals in p_logpsi /
+// p_sign and, in mode 0, the Metropolis decisions in accept.
+void state_propose(Ctx& c, int k, int mode, const float* rk_in, float width, uint64_t seed, uint64_t step,
+                   int32_t* n_acc) {
+  fire_handle* h = c.h;
+  auto& S = h->st;
+  REQUIRE(S.ready, "no walker state (call fire_state_init first)");
+  REQUIRE(k >= 0 && k < h->N, "electron index out of range");
+  state_drop_pending(c);
+  const int B = S.B, N = h->N, F = h->F, ND = h->ND, D = h->D, W = h->W, nu = h->n_up;
+  const int H = h->H, Dk = h->Dk, HD = h->HD, nc = H + HD;
+  Ctx cc{h, c.s, S.prec};
+  S.p_rk.ensure(sizeof(float) * B * 3);
+  S.p_u.ensure(sizeof(double) * B);
+  S.aff.ensure(sizeof(int32_t) * (size_t)B * N);
+  for (DevBuf* b : {&S.cu, &S.cd, &S.offu, &S.offd}) b->ensure(sizeof(int32_t) * B);
+  S.tail.ensure(sizeof(int32_t) * 4);
+  S.list.ensure(sizeof(int2) * (size_t)B * N);
+  S.rowof.ensure(sizeof(int32_t) * (size_t)B * N);
+  S.klist.ensure(sizeof(int2) * B);
+  launch_mc_propose(c.s, S.r.as<float>(), B, N, k, width, seed, step, mode, rk_in, S.p_r.as<float>(),
+                    S.p_rk.as<float>(), S.p_u.as<double>());
+  launch_mc_affected(c.s, S.r.as<float>(), S.p_r.as<float>(), B, N, nu, k, h->cfg.cutoff, S.aff.as<int32_t>(),
+                     S.cu.as<int32_t>(), S.cd.as<int32_t>());
+  launch_mc_scan(c.s, S.cu.as<int32_t>(), S.cd.as<int32_t>(), B, S.offu.as<int32_t>(), S.offd.as<int32_t>(),
+                 S.tail.as<int32_t>());
+  launch_mc_rowlist(c.s, S.aff.as<int32_t>(), S.cu.as<int32_t>(), S.cd.as<int32_t>(), S.offu.as<int32_t>(),
+                    S.offd.as<int32_t>(), S.tail.as<int32_t>(), B, N, nu, S.list.as<int2>(), S.rowof.as<int32_t>());
+  int32_t tail[3];
+  HIPCHK(hipMemcpyAsync(tail, S.tail.p, sizeof(tail), hipMemcpyDeviceToHost, c.s));
+  HIPCHK(hipStreamSynchronize(c.s));
+  const int T_up = tail[0], T = tail[0] + tail[1], amax = tail[2];
+  S.k = k;
+  S.T = T;
+  S.T_up = T_up;
+  S.amax = amax;
+  S.n_proposals += B;
+  S.sum_changed += T;
+  S.max_changed = std::max<int64_t>(S.max_changed, amax);
+  const size_t pw = wb_prep_smem(N, amax), uw = wb_update_smem(N, amax);
+  REQUIRE(pw <= 160 * 1024 && uw <= 160 * 1024,
+          "too many electrons changed by one move (" + std::to_string(amax) +
+              ") for the LDS-resident Woodbury update at this electron count");
+  if (std::max(pw, uw) > 64 * 1024 && std::max(pw, uw) > S.wb_smem_set) {
+    HIPCHK(wb_set_smem_limit(160 * 1024, 160 * 1024));
+    S.wb_smem_set = 160 * 1024;
+  }
+
+  // h0 and messages of the moved electron (ElecInit on one electron per walker)
+  const int b = h->iEN;
+  S.kT5.ensure(sizeof(float) * (size_t)B * F);
+  S.kT5b.ensure(sizeof(float) * (size_t)B * F);
+  S.kZ2.ensure(sizeof(float) * (size_t)B * 2 * F);
+  S.p_M5k.ensure(sizeof(float) * (size_t)B * 5 * F);
+  launch_mc_klist(c.s, B, k, S.klist.as<int2>());
+  launch_elec_init_en(c.s, S.p_r.as<float>(), h->R, B, N, nu, h->M, h->cfg.cutoff_1el, F, en_params(h), false,
+                      S.kT5.as<float>(), S.klist.as<int2>(), B);
+  gemm(cc, S.kT5.as<float>(), F, b + 11, -1, S.kZ2.as<float>(), 2 * F, B, 2 * F, F, nullptr, 0);
+  launch_glu(c.s, S.kZ2.as<float>(), B, F, false, S.kT5b.as<float>());
+  gemm(cc, S.kT5b.as<float>(), F, b + 12, b + 13, S.kT5.as<float>(), F, B, F, F, nullptr, 1);
+  const GroupDesc g1{1, 0, nullptr};
+  launch_act(c.s, S.kT5.as<float>(), F, F, B, g1, ACT_SILU, 0.f);
+  gemm(cc, S.kT5.as<float>(), F, b + 14, b + 15, S.p_M5k.as<float>(), 5 * F, B, 5 * F, F, nullptr, 1);
+
+  // neighbour lists of the affected centres at the new positions, messages, update MLP
+  S.pcnt.ensure(sizeof(int32_t) * T);
+  S.pptr.ensure(sizeof(int64_t) * (T + 2));
+  S.pnbr.ensure(sizeof(int32_t) * (size_t)T * std::max(N - 1, 1));
+  launch_nbr_count(c.s, S.p_r.as<float>(), B, N, nu, h->cfg.cutoff, S.pcnt.as<int32_t>(), S.list.as<int2>(), T);
+  launch_scan(c.s, S.pcnt.as<int32_t>(), T, S.pptr.as<int64_t>());
+  launch_nbr_fill(c.s, S.p_r.as<float>(), B, N, nu, h->cfg.cutoff, S.pptr.as<int64_t>(), S.pnbr.as<int32_t>(),
+                  S.list.as<int2>(), T);
+  const int XW = std::max(F, std::max(W, 2 * HD));
+  S.p_hc.ensure(sizeof(float) * (size_t)T * F);
+  S.p_x.ensure(sizeof(float) * (size_t)T * XW);
+  S.p_y.ensure(sizeof(float) * (size_t)T * XW);
+  S.p_h.ensure(sizeof(float) * (size_t)T * F);
+  launch_messages(c.s, S.p_r.as<float>(), B, N, nu, S.pptr.as<int64_t>(), S.pnbr.as<int32_t>(), S.M5.as<float>(),
+                  edge_params(h, h->iEEs), edge_params(h, h->iEEd), h->cut_learned, F, false, S.p_hc.as<float>(),
+                  S.list.as<int2>(), T, k, S.p_M5k.as<float>());
+  const int m = h->iMLP;
+  gemm(cc, S.p_hc.as<float>(), F, m + 0, m + 1, S.p_x.as<float>(), F, T, F, F, nullptr, 1);
+  launch_act(c.s, S.p_x.as<float>(), F, F, T, g1, ACT_SILU, 0.f);
+  gemm(cc, S.p_x.as<float>(), F, m + 2, m + 3, S.p_h.as<float>(), F, T, F, F, nullptr, 1);
+  launch_act(c.s, S.p_h.as<float>(), F, F, T, g1, ACT_SILU, 0.f);
+
+  // orbital rows of the affected electrons (up-spin rows first), envelope row of electron k
+  const int o = h->iORB;
+  S.p_orb.ensure(sizeof(float) * (size_t)T * ND);
+  if (T_up > 0) gemm(cc, S.p_h.as<float>(), F, o + 0, o + 1, S.p_orb.as<float>(), ND, T_up, ND, F, nullptr, 1);
+  if (T > T_up)
+    gemm(cc, S.p_h.as<float>() + (size_t)T_up * F, F, o + 2, o + 3, S.p_orb.as<float>() + (size_t)T_up * ND, ND,
+         T - T_up, ND, F, nullptr, 1);
+  const int ME = h->M * h->Eiso;
+  S.kcoef.ensure(sizeof(float) * (size_t)D * B * ME);
+  S.p_envk.ensure(sizeof(float) * (size_t)B * ND);
+  launch_env_coef(c.s, S.p_r.as<float>(), h->R, B, N, nu, h->M, D, h->Eiso, h->P[o + 4], h->P[o + 6], false,
+                  S.kcoef.as<float>(), S.klist.as<int2>(), B);
+  {
+    const int pi_i = o + (k < nu ? 5 : 7);
+    for (int d = 0; d < D; ++d)
+      launch_gemm_f32(c.s, S.kcoef.as<float>() + (size_t)d * B * ME, ME, h->P[pi_i] + (size_t)d * ME * N,
+                      h->Pt32[pi_i] + (size_t)d * ME * N, nullptr, S.p_envk.as<float>() + (size_t)d * N, ND, B, N, ME,
+                      nullptr, 0);
+  }
+
+  // determinant ratios (Woodbury, FP64)
+  S.phin.ensure(sizeof(double) * (size_t)B * D * std::max(amax, 1) * N);
+  S.sinv.ensure(sizeof(double) * (size_t)B * D * std::max(amax, 1) * std::max(amax, 1));
+  S.ldS.ensure(sizeof(double) * (size_t)B * D);
+  S.sgS.ensure(sizeof(double) * (size_t)B * D);
+  launch_wb_prep(c.s, S.p_orb.as<float>(), S.p_envk.as<float>(), S.env.as<float>(), S.A.as<double>(),
+                 S.Ainv.as<double>(), S.aff.as<int32_t>(), S.cu.as<int32_t>(), S.cd.as<int32_t>(),
+                 S.rowof.as<int32_t>(), B, N, nu, D, k, amax, S.phin.as<double>(), S.sinv.as<double>(),
+                 S.ldS.as<double>(), S.sgS.as<double>());
+
+  // Jastrow rows of the affected electrons and the per-walker sums
+  S.jsum.ensure(sizeof(double) * (size_t)B * 2);
+  if (h->has_mlp) {
+    S.p_jr.ensure(sizeof(float) * (size_t)T * 2);
+    const int j = h->iJM;
+    float* cur = S.p_x.as<float>();
+    float* nxt = S.p_y.as<float>();
+    gemm(cc, S.p_h.as<float>(), F, j + 0, j + 1, cur, W, T, W, F, nullptr, 1);
+    launch_act(c.s, cur, W, W, T, g1, ACT_SILU, 0.f);
+    for (int l = 1; l < h->depth; ++l) {
+      gemm(cc, cur, W, j + 2 * l, j + 2 * l + 1, nxt, W, T, W, W, nullptr, 1);
+      launch_act(c.s, nxt, W, W, T, g1, ACT_SILU, 0.f);
+      std::swap(cur, nxt);
+    }
+    const int jl = j + 2 * h->depth;
+    launch_jmlp_last(c.s, cur, W, T, g1, h->P[jl], h->P[jl + 1], h->P[jl + 2], h->P[jl + 3], S.p_jr.as<float>());
+  }
+  const float* att_scale = nullptr;
+  const float* att_zb = nullptr;
+  if (h->has_att) {
+    const int ac = h->iATT;
+    S.p_qv.ensure(sizeof(float) * (size_t)T * 2 * HD);
+    S.p_av.ensure(sizeof(float) * (size_t)T * nc);
+    gemm(cc, S.p_h.as<float>(), F, ac + 0, ac + 1, S.p_qv.as<float>(), 2 * HD, T, 2 * HD, F, nullptr, 1);
+    launch_att_group(c.s, S.p_qv.as<float>(), H, Dk, T, g1, h->P[ac + 2], false, S.p_av.as<float>());
+  }
+  S.rin.ensure(sizeof(float) * (size_t)B * 2 * std::max(HD, 1));
+  if (h->has_mlp || h->has_att)
+    launch_mc_jsum(c.s, S.jr.as<float>(), S.p_jr.as<float>(), S.av.as<float>(), S.p_av.as<float>(),
+                   S.rowof.as<int32_t>(), B, N, nu, H, Dk, h->has_mlp, h->has_att, S.jsum.as<double>(),
+                   S.rin.as<float>());
+  S.att_out.ensure(sizeof(float) * (size_t)B * 2);
+  if (h->has_att) {
+    const int ac = h->iATT;
+    S.ry1.ensure(sizeof(float) * (size_t)B * W);
+    S.ry2.ensure(sizeof(float) * (size_t)B * W);
+    float* cur = S.ry1.as<float>();
+    float* nxt = S.ry2.as<float>();
+    gemm(cc, S.rin.as<float>(), 2 * HD, ac + 3, ac + 4, cur, W, B, W, 2 * HD, nullptr, 1);
+    launch_act(c.s, cur, W, W, B, g1, ACT_SILU, 0.f);
+    for (int l = 1; l < h->depth; ++l) {
+      gemm(cc, cur, W, ac + 3 + 2 * l, ac + 4 + 2 * l, nxt, W, B, W, W, nullptr, 1);
+      launch_act(c.s, nxt, W, W, B, g1, ACT_SILU, 0.f);
+      std::swap(cur, nxt);
+    }
+    const int al = ac + 3 + 2 * h->depth;
+    launch_gemm_f32(c.s, cur, W, h->P[al], nullptr, h->P[al + 1], S.att_out.as<float>(), 2, B, 2, W, nullptr, 1);
+    att_scale = h->P[al + 2];
+    att_zb = h->P[al + 3];
+  }
+  // e-e cusp of the new positions (recomputed in full, as the reference does)
+  S.cval.ensure(sizeof(double) * (size_t)B * N);
+  S.clap.ensure(sizeof(double) * (size_t)B * N);
+  S.cgrad.ensure(sizeof(double) * (size_t)B * N * 3);
+  S.epot.ensure(sizeof(double) * (size_t)B * N);
+  if (h->cusp)
+    launch_cusp_coulomb(c.s, S.p_r.as<float>(), h->R, h->Z, B, N, nu, h->M, h->cusp, h->P[h->iCUSP],
+                        S.cval.as<double>(), S.clap.as<double>(), S.cgrad.as<double>(), S.epot.as<double>());
+  for (DevBuf* x : {&S.p_ld, &S.p_sg}) x->ensure(sizeof(double) * (size_t)B * D);
+  for (DevBuf* x : {&S.p_logpsi, &S.p_sign}) x->ensure(sizeof(double) * B);
+  launch_mc_accept(c.s, B, D, h->cfg.use_mlp_jastrow, h->cfg.use_log_jastrow, h->has_att, h->cusp, S.ld.as<double>(),
+                   S.sg.as<double>(), S.ldS.as<double>(), S.sgS.as<double>(), S.jsum.as<double>(),
+                   S.att_out.as<float>(), att_scale, att_zb, S.cval.as<double>(), N, S.logpsi.as<double>(),
+                   S.p_u.as<double>(), mode, S.p_ld.as<double>(), S.p_sg.as<double>(), S.p_logpsi.as<double>(),
+                   S.p_sign.as<double>(), S.accept.as<int32_t>(), n_acc);
+  S.pending = true;
+}
+
+// Apply the pending proposal to the walkers whose accept flag is set.
+void state_commit(Ctx& c) {
+  fire_handle* h = c.h;
+  auto& S = h->st;
+  REQUIRE(S.pending, "no pending proposal to commit");
+  const int B = S.B, N = h->N, nu = h->n_up, D = h->D;
+  const int nc = h->H + h->HD;
+  launch_wb_update(c.s, S.accept.as<int32_t>(), S.aff.as<int32_t>(), S.cu.as<int32_t>(), S.cd.as<int32_t>(), B, N, D,
+                   S.amax, S.phin.as<double>(), S.sinv.as<double>(), S.A.as<double>(), S.Ainv.as<double>());
+  launch_mc_commit(c.s, S.accept.as<int32_t>(), S.list.as<int2>(), S.T, B, N, nu, D, h->F, h->ND, nc, S.k,
+                   S.p_h.as<float>(), S.p_orb.as<float>(), S.p_jr.as<float>(), S.p_av.as<float>(), S.p_rk.as<float>(),
+                   S.p_M5k.as<float>(), S.p_envk.as<float>(), S.p_ld.as<double>(), S.p_sg.as<double>(),
+                   S.p_logpsi.as<double>(), S.p_sign.as<double>(), S.hh.as<float>(), S.orb.as<float>(),
+                   h->has_mlp ? S.jr.as<float>() : nullptr, h->has_att ? S.av.as<float>() : nullptr,
+                   S.r.as<float>(), S.M5.as<float>(), S.env.as<float>(), S.ld.as<double>(), S.sg.as<double>(),
+                   S.logpsi.as<double>(), S.sign.as<double>(), S.p_r.as<float>());
+  S.pending = false;
+}
+
 int fail(const std::exception& e) {
   g_last_error = e.what();
   return 1;
@@ -620,6 +939,7 @@ int fire_destroy(fire_handle* h) {
                     &h->mlp_ns, &h->mlp_dg, &h->QV, &h->AV, &h->att_ns, &h->att_dg, &h->RIN, &h->RY1,
                     &h->RY2, &h->att_out, &h->cval, &h->clap, &h->cgrad, &h->epot};
   for (DevBuf* b : bufs) b->release();
+  h->st.release();
   delete h;
   return 0;
 }
@@ -867,6 +1187,115 @@ int fire_embedding(fire_handle* h, void* stream, const float* r_dev, int32_t B, 
       }
     }
     HIPCHK(hipStreamSynchronize(c.s));
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e);
+  }
+}
+
+int fire_state_init(fire_handle* h, void* stream, const float* r_dev, int32_t B, int32_t precision, double* logpsi_dev,
+                    double* sign_dev) {
+  try {
+    check_ready(h);
+    REQUIRE(r_dev, "null argument");
+    REQUIRE(B > 0, "batch must be positive");
+    REQUIRE(precision == FIRE_FP32 || precision == FIRE_MIXED || precision == FIRE_BF16, "unknown precision mode");
+    HIPCHK(hipSetDevice(h->device));
+    Ctx c{h, (hipStream_t)stream, precision};
+    state_init(c, r_dev, B);
+    if (logpsi_dev)
+      HIPCHK(hipMemcpyAsync(logpsi_dev, h->st.logpsi.p, sizeof(double) * B, hipMemcpyDeviceToDevice, c.s));
+    if (sign_dev) HIPCHK(hipMemcpyAsync(sign_dev, h->st.sign.p, sizeof(double) * B, hipMemcpyDeviceToDevice, c.s));
+    HIPCHK(hipGetLastError());
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e);
+  }
+}
+
+int fire_state_propose(fire_handle* h, void* stream, int32_t k, const float* rk_dev, double* logpsi_dev,
+                       double* sign_dev) {
+  try {
+    check_ready(h);
+    REQUIRE(rk_dev, "null argument");
+    HIPCHK(hipSetDevice(h->device));
+    Ctx c{h, (hipStream_t)stream, h->st.prec};
+    state_propose(c, k, 1, rk_dev, 0.f, 0, 0, nullptr);
+    const int B = h->st.B;
+    if (logpsi_dev)
+      HIPCHK(hipMemcpyAsync(logpsi_dev, h->st.p_logpsi.p, sizeof(double) * B, hipMemcpyDeviceToDevice, c.s));
+    if (sign_dev) HIPCHK(hipMemcpyAsync(sign_dev, h->st.p_sign.p, sizeof(double) * B, hipMemcpyDeviceToDevice, c.s));
+    HIPCHK(hipGetLastError());
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e);
+  }
+}
+
+int fire_state_commit(fire_handle* h, void* stream, const int32_t* accept_dev) {
+  try {
+    check_ready(h);
+    REQUIRE(accept_dev, "null argument");
+    HIPCHK(hipSetDevice(h->device));
+    Ctx c{h, (hipStream_t)stream, h->st.prec};
+    REQUIRE(h->st.pending, "no pending proposal to commit");
+    HIPCHK(hipMemcpyAsync(h->st.accept.p, accept_dev, sizeof(int32_t) * h->st.B, hipMemcpyDeviceToDevice, c.s));
+    state_commit(c);
+    HIPCHK(hipGetLastError());
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e);
+  }
+}
+
+int fire_mcmc_steps(fire_handle* h, void* stream, int32_t n_steps, float width, uint64_t seed, int64_t step0,
+                    int32_t* n_accept_dev) {
+  try {
+    check_ready(h);
+    REQUIRE(n_steps >= 0, "negative step count");
+    REQUIRE(step0 >= 0, "negative step index");
+    HIPCHK(hipSetDevice(h->device));
+    Ctx c{h, (hipStream_t)stream, h->st.prec};
+    REQUIRE(h->st.ready, "no walker state (call fire_state_init first)");
+    if (n_accept_dev) launch_fill_i32(c.s, n_accept_dev, h->st.B, 0);
+    for (int64_t s = step0; s < step0 + n_steps; ++s) {
+      state_propose(c, (int)(s % h->N), 0, nullptr, width, seed, (uint64_t)s, n_accept_dev);
+      state_commit(c);
+    }
+    HIPCHK(hipGetLastError());
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e);
+  }
+}
+
+int fire_state_get(fire_handle* h, void* stream, float* r_dev, double* logpsi_dev, double* sign_dev) {
+  try {
+    REQUIRE(h, "null handle");
+    REQUIRE(h->st.ready, "no walker state (call fire_state_init first)");
+    HIPCHK(hipSetDevice(h->device));
+    Ctx c{h, (hipStream_t)stream, h->st.prec};
+    state_drop_pending(c);
+    const int B = h->st.B;
+    if (r_dev)
+      HIPCHK(hipMemcpyAsync(r_dev, h->st.r.p, sizeof(float) * (size_t)B * h->N * 3, hipMemcpyDeviceToDevice, c.s));
+    if (logpsi_dev)
+      HIPCHK(hipMemcpyAsync(logpsi_dev, h->st.logpsi.p, sizeof(double) * B, hipMemcpyDeviceToDevice, c.s));
+    if (sign_dev) HIPCHK(hipMemcpyAsync(sign_dev, h->st.sign.p, sizeof(double) * B, hipMemcpyDeviceToDevice, c.s));
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e);
+  }
+}
+
+int fire_state_stats(fire_handle* h, int32_t* n_walkers, int64_t* n_proposals, int64_t* sum_changed,
+                     int64_t* max_changed) {
+  try {
+    REQUIRE(h, "null handle");
+    if (n_walkers) *n_walkers = h->st.ready ? h->st.B : 0;
+    if (n_proposals) *n_proposals = h->st.n_proposals;
+    if (sum_changed) *sum_changed = h->st.sum_changed;
+    if (max_changed) *max_changed = h->st.max_changed;
     return 0;
   } catch (const std::exception& e) {
     return fail(e);

@@ -32,19 +32,24 @@ struct EdgeParams {  // ElecElecEdges (new_sparse_model.py:161-187)
 enum Act { ACT_SILU = 0, ACT_EXP_NEG_SCALED = 1 };
 
 // graph
-void launch_nbr_count(hipStream_t s, const float* r, int B, int N, int n_up, float cutoff, int32_t* cnt);
+void launch_nbr_count(hipStream_t s, const float* r, int B, int N, int n_up, float cutoff, int32_t* cnt,
+                      const int2* list = nullptr, int n_list = 0);
 void launch_nbr_fill(hipStream_t s, const float* r, int B, int N, int n_up, float cutoff, const int64_t* ptr,
-                     int32_t* nbr);
+                     int32_t* nbr, const int2* list = nullptr, int n_list = 0);
 void launch_scan(hipStream_t s, const int32_t* cnt, int G, int64_t* ptr);
 void launch_rev(hipStream_t s, const int64_t* ptr, const int32_t* nbr, int B, int N, int n_up, int32_t* rev);
 
 // embedding
+// list (optional): (walker, electron) pairs to process, output row = list position
 void launch_elec_init_en(hipStream_t s, const float* r, const float* R, int B, int N, int n_up, int M, float cutoff_1el,
-                         int F, EnParams P, bool tangents, float* out);
+                         int F, EnParams P, bool tangents, float* out, const int2* list = nullptr, int n_list = 0);
 void launch_glu(hipStream_t s, const float* Z, int G, int F, bool tangents, float* out);
 void launch_act(hipStream_t s, float* Y, int64_t ld, int ncols, int G, GroupDesc gd, int act, float act_scale);
+// list mode: centres list[0..n_list) with CSR ptr/nbr over list positions; the message
+// rows of electron `kmove` of each walker are read from M5k[walker] (value rows only)
 void launch_messages(hipStream_t s, const float* r, int B, int N, int n_up, const int64_t* ptr, const int32_t* nbr,
-                     const float* M5, EdgeParams same, EdgeParams diff, float cut, int F, bool tangents, float* out);
+                     const float* M5, EdgeParams same, EdgeParams diff, float cut, int F, bool tangents, float* out,
+                     const int2* list = nullptr, int n_list = 0, int kmove = -1, const float* M5k = nullptr);
 void launch_scale(hipStream_t s, float* p, int64_t n, float a);
 void launch_rowflag(hipStream_t s, int G, GroupDesc gd, int64_t n_rows, uint8_t* flag);
 

@@ -9,7 +9,8 @@ namespace fire {
 
 // orbitals and determinants
 void launch_env_coef(hipStream_t s, const float* r, const float* R, int B, int N, int n_up, int M, int D, int E,
-                     const float* sp_up, const float* sp_dn, bool tangents, float* coef);
+                     const float* sp_up, const float* sp_dn, bool tangents, float* coef, const int2* list = nullptr,
+                     int n_list = 0);
 void launch_orb_product(hipStream_t s, float* O, int ND, int G, GroupDesc gd, const float* env, bool tangents);
 void launch_gather_A(hipStream_t s, const float* O, int ND, GroupDesc gd, int B, int N, int n_up, int D, double* A);
 void launch_lu_inv(hipStream_t s, double* A, int n_mat, int N, double* Ainv, float* AinvT32, double* logdet,

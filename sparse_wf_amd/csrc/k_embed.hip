@@ -19,14 +19,19 @@ namespace fire {
 template <bool TAN, int W1>
 __global__ void __launch_bounds__(256) k_elec_init_en(const float* __restrict__ r, const float* __restrict__ R, int B,
                                                        int N, int n_up, int M, float cutoff_1el, int F, EnParams P,
-                                                       float* __restrict__ out) {
+                                                       float* __restrict__ out, const int2* __restrict__ list) {
   __shared__ float s_beta[EN_CHUNK][5][W1];
   __shared__ float s_g[EN_CHUNK][5][4];
   __shared__ int s_ok[EN_CHUNK];
   __shared__ float s_red[16 * 8];
   const int g = blockIdx.x;
   int w, i;
-  group_inv(g, B, n_up, N - n_up, w, i);
+  if (list) {
+    w = list[g].x;
+    i = list[g].y;
+  } else {
+    group_inv(g, B, n_up, N - n_up, w, i);
+  }
   const float* ri = r + ((size_t)w * N + i) * 3;
   const float rx = ri[0], ry = ri[1], rz = ri[2];
   const int f = threadIdx.x;
@@ -136,12 +141,13 @@ __global__ void __launch_bounds__(256) k_elec_init_en(const float* __restrict__ 
   }
 
 void launch_elec_init_en(hipStream_t s, const float* r, const float* R, int B, int N, int n_up, int M, float cutoff_1el,
-                         int F, EnParams P, bool tangents, float* out) {
-  const int G = B * N;
+                         int F, EnParams P, bool tangents, float* out, const int2* list, int n_list) {
+  const int G = list ? n_list : B * N;
+  if (G == 0) return;
   if (tangents) {
-    W1_DISPATCH(P.w1, k_elec_init_en<true, W1><<<G, F, 0, s>>>(r, R, B, N, n_up, M, cutoff_1el, F, P, out));
+    W1_DISPATCH(P.w1, k_elec_init_en<true, W1><<<G, F, 0, s>>>(r, R, B, N, n_up, M, cutoff_1el, F, P, out, list));
   } else {
-    W1_DISPATCH(P.w1, k_elec_init_en<false, W1><<<G, F, 0, s>>>(r, R, B, N, n_up, M, cutoff_1el, F, P, out));
+    W1_DISPATCH(P.w1, k_elec_init_en<false, W1><<<G, F, 0, s>>>(r, R, B, N, n_up, M, cutoff_1el, F, P, out, list));
   }
 }
 
@@ -248,15 +254,24 @@ template <bool TAN, int W1>
 __global__ void __launch_bounds__(256) k_messages(const float* __restrict__ r, int B, int N, int n_up,
                                                    const int64_t* __restrict__ ptr, const int32_t* __restrict__ nbr,
                                                    const float* __restrict__ M5, EdgeParams Ps, EdgeParams Pd,
-                                                   float cut, int F, float* __restrict__ out) {
+                                                   float cut, int F, float* __restrict__ out,
+                                                   const int2* __restrict__ list, int kmove,
+                                                   const float* __restrict__ M5k) {
+  // list mode (value rows only): centres are list[g] = (walker, electron); the h0/message
+  // rows of electron `kmove` of every walker come from M5k[walker] instead of M5.
   __shared__ float s_beta[MSG_CHUNK][5][W1];
   __shared__ float s_g[MSG_CHUNK][5][4];
-  __shared__ int s_gj[MSG_CHUNK];
+  __shared__ const float* s_src[MSG_CHUNK];
   __shared__ int s_same[MSG_CHUNK];
   const int g = blockIdx.x;
   const int n_dn = N - n_up;
   int w, i;
-  group_inv(g, B, n_up, n_dn, w, i);
+  if (list) {
+    w = list[g].x;
+    i = list[g].y;
+  } else {
+    group_inv(g, B, n_up, n_dn, w, i);
+  }
   const int f = threadIdx.x;
   const int NR = TAN ? 5 : 1;
   const int64_t p0 = ptr[g];
@@ -264,6 +279,10 @@ __global__ void __launch_bounds__(256) k_messages(const float* __restrict__ r, i
   const int64_t row0 = TAN ? (5 * (int64_t)g + 3 * p0) : g;
   const float* rw = r + (size_t)w * N * 3;
   const bool up_i = i < n_up;
+  auto m5_of = [&](int e) -> const float* {
+    if (list && e == kmove) return M5k + (size_t)w * NR * 5 * F;
+    return M5 + (size_t)group_of(w, e, B, n_up, n_dn) * NR * 5 * F;
+  };
 
   // weights of this feature column for same- and opposite-spin edges
   float wgs[W1], wgd[W1], wfs[4], wfd[4];
@@ -273,7 +292,7 @@ __global__ void __launch_bounds__(256) k_messages(const float* __restrict__ r, i
   for (int m = 0; m < 4; ++m) { wfs[m] = Ps.WF[m * F + f]; wfd[m] = Pd.WF[m * F + f]; }
   const float bfs = Ps.BF[f], bfd = Pd.BF[f];
 
-  const float* m5i = M5 + (size_t)g * NR * 5 * F;
+  const float* m5i = m5_of(i);
   float acc[5], mcs[5], mcd[5];
   for (int rr = 0; rr < NR; ++rr) {
     acc[rr] = m5i[rr * 5 * F + 0 * F + f];
@@ -296,13 +315,13 @@ __global__ void __launch_bounds__(256) k_messages(const float* __restrict__ r, i
 #pragma unroll
         for (int m = 0; m < 4; ++m) s_g[threadIdx.x][rr][m] = gg[rr][m];
       }
-      s_gj[threadIdx.x] = group_of(w, j, B, n_up, n_dn);
+      s_src[threadIdx.x] = m5_of(j);
       s_same[threadIdx.x] = same;
     }
     __syncthreads();
     for (int kk = 0; kk < cn; ++kk) {
       const bool same = s_same[kk];
-      const float* m5j = M5 + (size_t)s_gj[kk] * NR * 5 * F + (same ? 3 : 4) * F + f;
+      const float* m5j = s_src[kk] + (same ? 3 : 4) * F + f;
       float G5[5], E5[5], mn[5], mc[5];
 #pragma unroll
       for (int rr = 0; rr < NR; ++rr) {
@@ -349,12 +368,16 @@ __global__ void __launch_bounds__(256) k_messages(const float* __restrict__ r, i
 }
 
 void launch_messages(hipStream_t s, const float* r, int B, int N, int n_up, const int64_t* ptr, const int32_t* nbr,
-                     const float* M5, EdgeParams same, EdgeParams diff, float cut, int F, bool tangents, float* out) {
-  const int G = B * N;
+                     const float* M5, EdgeParams same, EdgeParams diff, float cut, int F, bool tangents, float* out,
+                     const int2* list, int n_list, int kmove, const float* M5k) {
+  const int G = list ? n_list : B * N;
+  if (G == 0) return;
   if (tangents) {
-    W1_DISPATCH(same.filt.w1, k_messages<true, W1><<<G, F, 0, s>>>(r, B, N, n_up, ptr, nbr, M5, same, diff, cut, F, out));
+    W1_DISPATCH(same.filt.w1, k_messages<true, W1><<<G, F, 0, s>>>(r, B, N, n_up, ptr, nbr, M5, same, diff, cut, F,
+                                                                   out, list, kmove, M5k));
   } else {
-    W1_DISPATCH(same.filt.w1, k_messages<false, W1><<<G, F, 0, s>>>(r, B, N, n_up, ptr, nbr, M5, same, diff, cut, F, out));
+    W1_DISPATCH(same.filt.w1, k_messages<false, W1><<<G, F, 0, s>>>(r, B, N, n_up, ptr, nbr, M5, same, diff, cut, F,
+                                                                    out, list, kmove, M5k));
   }
 }
 

@@ -22,17 +22,28 @@ __device__ __forceinline__ bool in_cutoff(const float* ri, const float* rj, floa
   return __fsqrt_rn(s) < cutoff;
 }
 
+// `list` (optional): centres as (walker, electron) pairs, output index = list position;
+// otherwise every electron of every walker in group order.
 template <bool FILL>
 __global__ void __launch_bounds__(256) k_nbr(const float* __restrict__ r, int B, int N, int n_up, float cutoff,
                                              int32_t* __restrict__ cnt, const int64_t* __restrict__ ptr,
-                                             int32_t* __restrict__ nbr) {
+                                             int32_t* __restrict__ nbr, const int2* __restrict__ list, int n_list) {
   const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
-  if (wave >= B * N) return;
-  const int w = wave / N, i = wave - w * N;
+  const int n_centres = list ? n_list : B * N;
+  if (wave >= n_centres) return;
+  int w, i, g;
+  if (list) {
+    w = list[wave].x;
+    i = list[wave].y;
+    g = wave;
+  } else {
+    w = wave / N;
+    i = wave - w * N;
+    g = group_of(w, i, B, n_up, N - n_up);
+  }
   const float* rw = r + (size_t)w * N * 3;
   const float ri[3] = {rw[3 * i], rw[3 * i + 1], rw[3 * i + 2]};
-  const int g = group_of(w, i, B, n_up, N - n_up);
   int count = 0;
   int64_t base = FILL ? ptr[g] : 0;
   for (int j0 = 0; j0 < N; j0 += 64) {
@@ -105,17 +116,20 @@ __global__ void k_rev(const int64_t* __restrict__ ptr, const int32_t* __restrict
   }
 }
 
-void launch_nbr_count(hipStream_t s, const float* r, int B, int N, int n_up, float cutoff, int32_t* cnt) {
-  const int waves = B * N;
+void launch_nbr_count(hipStream_t s, const float* r, int B, int N, int n_up, float cutoff, int32_t* cnt,
+                      const int2* list, int n_list) {
+  const int waves = list ? n_list : B * N;
+  if (waves == 0) return;
   const int blocks = (waves * 64 + 255) / 256;
-  k_nbr<false><<<blocks, 256, 0, s>>>(r, B, N, n_up, cutoff, cnt, nullptr, nullptr);
+  k_nbr<false><<<blocks, 256, 0, s>>>(r, B, N, n_up, cutoff, cnt, nullptr, nullptr, list, n_list);
 }
 
 void launch_nbr_fill(hipStream_t s, const float* r, int B, int N, int n_up, float cutoff, const int64_t* ptr,
-                     int32_t* nbr) {
-  const int waves = B * N;
+                     int32_t* nbr, const int2* list, int n_list) {
+  const int waves = list ? n_list : B * N;
+  if (waves == 0) return;
   const int blocks = (waves * 64 + 255) / 256;
-  k_nbr<true><<<blocks, 256, 0, s>>>(r, B, N, n_up, cutoff, nullptr, ptr, nbr);
+  k_nbr<true><<<blocks, 256, 0, s>>>(r, B, N, n_up, cutoff, nullptr, ptr, nbr, list, n_list);
 }
 
 void launch_scan(hipStream_t s, const int32_t* cnt, int G, int64_t* ptr) {

@@ -19,15 +19,20 @@ namespace fire {
 template <bool TAN>
 __global__ void k_env_coef(const float* __restrict__ r, const float* __restrict__ R, int B, int N, int n_up, int M,
                            int D, int E, const float* __restrict__ sp_up, const float* __restrict__ sp_dn,
-                           float* __restrict__ coef) {
+                           float* __restrict__ coef, const int2* __restrict__ list, int n_list) {
   const int g = blockIdx.x;
   int w, i;
-  group_inv(g, B, n_up, N - n_up, w, i);
+  if (list) {
+    w = list[g].x;
+    i = list[g].y;
+  } else {
+    group_inv(g, B, n_up, N - n_up, w, i);
+  }
   const float* ri = r + ((size_t)w * N + i) * 3;
   const float* sp = i < n_up ? sp_up : sp_dn;
   const int NR = TAN ? 5 : 1;
   const int ME = M * E;
-  const size_t G = (size_t)B * N;
+  const size_t G = list ? (size_t)n_list : (size_t)B * N;
   for (int idx = threadIdx.x; idx < M * D * E; idx += blockDim.x) {
     const int m = idx / (D * E), de = idx - m * (D * E), d = de / E, e = de - d * E;
     const float dx = ri[0] - R[3 * m], dy = ri[1] - R[3 * m + 1], dz = ri[2] - R[3 * m + 2];
@@ -47,12 +52,14 @@ __global__ void k_env_coef(const float* __restrict__ r, const float* __restrict_
 }
 
 void launch_env_coef(hipStream_t s, const float* r, const float* R, int B, int N, int n_up, int M, int D, int E,
-                     const float* sp_up, const float* sp_dn, bool tangents, float* coef) {
-  const int G = B * N;
+                     const float* sp_up, const float* sp_dn, bool tangents, float* coef, const int2* list,
+                     int n_list) {
+  const int G = list ? n_list : B * N;
+  if (G == 0) return;
   if (tangents)
-    k_env_coef<true><<<G, 256, 0, s>>>(r, R, B, N, n_up, M, D, E, sp_up, sp_dn, coef);
+    k_env_coef<true><<<G, 256, 0, s>>>(r, R, B, N, n_up, M, D, E, sp_up, sp_dn, coef, list, n_list);
   else
-    k_env_coef<false><<<G, 256, 0, s>>>(r, R, B, N, n_up, M, D, E, sp_up, sp_dn, coef);
+    k_env_coef<false><<<G, 256, 0, s>>>(r, R, B, N, n_up, M, D, E, sp_up, sp_dn, coef, list, n_list);
 }
 
 // Phi = orbital rows x envelope rows (product with a one-electron function), in place.
@@ -295,9 +302,10 @@ __global__ void __launch_bounds__(512) k_gj_inv_lds(const double* __restrict__ A
     __syncthreads();
   }
   double* X = Ainv_all + (size_t)mat * N * N;
+  for (int e = tid; e < N * N; e += nthr) X[e] = A[e];
+  if (!AinvT32) return;
   float* XT = AinvT32 + (size_t)mat * N * N;
   for (int e = tid; e < N * N; e += nthr) {
-    X[e] = A[e];
     const int i = e / N, j = e - i * N;
     XT[(size_t)j * N + i] = (float)A[e];
   }
@@ -325,7 +333,7 @@ void launch_lu_inv(hipStream_t s, double* A, int n_mat, int N, double* Ainv, flo
     k_gj_inv_lds<<<n_mat, 512, lu_lds_bytes(N), s>>>(A, N, Ainv, AinvT32, logdet, sign, want_inv ? 1 : 0);
   } else {
     k_lu_inv<<<n_mat, 256, N * sizeof(int), s>>>(A, N, Ainv, logdet, sign, want_inv ? 1 : 0);
-    if (want_inv) k_transpose_f32<<<n_mat, 256, 0, s>>>(Ainv, N, AinvT32);
+    if (want_inv && AinvT32) k_transpose_f32<<<n_mat, 256, 0, s>>>(Ainv, N, AinvT32);
   }
 }
 

@@ -229,6 +229,135 @@ class FireWaveFunction:
             ho = ho / float(params["embedding"]["scales"][1])
         return (hc[0], ho[0]) if single else (hc, ho)
 
+    # resident state and low-rank updates ------------------------------------------------
+    def log_psi_with_state(self, params, electrons, static=None, precision=None):
+        """((sign, log|psi|), state) -- MoonLikeWaveFunction.log_psi_with_state
+        (wave_function.py:226-249).  The state stays resident on the GPU inside this wave
+        function (one state per instance); the returned ``WalkerState`` names it."""
+        self._ensure(params)
+        r, single = self._walkers(electrons)
+        B = r.shape[0]
+        lp = torch.empty(B, dtype=torch.float64, device=self.device)
+        sg = torch.empty(B, dtype=torch.float64, device=self.device)
+        _lib.check(self._lib.fire_state_init(self._h, _stream_ptr(), _dptr(r), B, self._prec(precision), _dptr(lp),
+                                             _dptr(sg)))
+        self._gen = getattr(self, "_gen", 0) + 1
+        self._committed_gen, self._pending_gen = self._gen, None
+        st = WalkerState(self, self._gen, B, single, pending=False)
+        return ((sg[0], lp[0]) if single else (sg, lp)), st
+
+    def _use_state(self, params, state: "WalkerState"):
+        if params is not self._uploaded:
+            raise ValueError("parameters changed since log_psi_with_state: the resident state belongs to the old ones")
+        if state.wf is not self:
+            raise ValueError("state belongs to another wave function")
+        if state.pending:
+            if state.gen != self._pending_gen:
+                raise ValueError("stale proposal state")
+            self.commit(state)  # the proposal's state is taken as the new state
+        elif state.gen != self._committed_gen:
+            raise ValueError("stale state: a later update was committed")
+
+    def log_psi_low_rank_update(self, params, electrons, changed_electrons, static=None, state=None):
+        """((sign, log|psi|), proposal state) after moving electron ``changed_electrons`` of every
+        walker to its position in ``electrons`` -- MoonLikeWaveFunction.log_psi_low_rank_update
+        (wave_function.py:251-287).  The resident state is not modified until the returned
+        proposal is committed: by ``commit(proposal, accept)``, or in full when the proposal is
+        passed as ``state`` to the next update (the reference's functional semantics)."""
+        if state is None:
+            raise ValueError("state from log_psi_with_state is required")
+        self._use_state(params, state)
+        r, single = self._walkers(electrons)
+        idx = np.asarray(changed_electrons).reshape(-1)
+        if idx.size != 1:
+            raise ValueError("one changed electron per update (single-electron moves)")
+        k = int(idx[0])
+        if r.shape[0] != state.n_walkers:
+            raise ValueError("walker count differs from the state's")
+        rk = r[:, k, :].contiguous()
+        B = r.shape[0]
+        lp = torch.empty(B, dtype=torch.float64, device=self.device)
+        sg = torch.empty(B, dtype=torch.float64, device=self.device)
+        _lib.check(self._lib.fire_state_propose(self._h, _stream_ptr(), k, _dptr(rk), _dptr(lp), _dptr(sg)))
+        self._gen += 1
+        self._pending_gen = self._gen
+        st = WalkerState(self, self._gen, B, state.single, pending=True)
+        return ((sg[0], lp[0]) if single else (sg, lp)), st
+
+    def commit(self, proposal: "WalkerState", accept=None):
+        """Apply a proposal to the walkers where ``accept`` is true (all when None) -- the
+        jnp.where(accept, new, old) state selection of mcmc_steps_low_rank (mcmc.py:240-244)."""
+        if not proposal.pending or proposal.gen != self._pending_gen:
+            raise ValueError("not the pending proposal")
+        B = proposal.n_walkers
+        if accept is None:
+            acc = torch.ones(B, dtype=torch.int32, device=self.device)
+        else:
+            acc = torch.as_tensor(accept, device=self.device).reshape(-1).to(torch.int32).contiguous()
+            if acc.numel() != B:
+                raise ValueError("accept must have one entry per walker")
+        _lib.check(self._lib.fire_state_commit(self._h, _stream_ptr(), _dptr(acc)))
+        self._committed_gen, self._pending_gen = proposal.gen, None
+        proposal.pending = False
+
+    def state_values(self):
+        """(electrons [B, N, 3], sign [B], log|psi| [B]) of the committed resident state."""
+        B = self._state_walkers()
+        r = torch.empty((B, self.cfg.n_el, 3), dtype=torch.float32, device=self.device)
+        lp = torch.empty(B, dtype=torch.float64, device=self.device)
+        sg = torch.empty(B, dtype=torch.float64, device=self.device)
+        _lib.check(self._lib.fire_state_get(self._h, _stream_ptr(), _dptr(r), _dptr(lp), _dptr(sg)))
+        self._pending_gen = None
+        return r, sg, lp
+
+    def _state_walkers(self) -> int:
+        nw = ctypes.c_int32()
+        _lib.check(self._lib.fire_state_stats(self._h, ctypes.byref(nw), None, None, None))
+        if nw.value == 0:
+            raise ValueError("no resident state (call log_psi_with_state first)")
+        return nw.value
+
+    def state_stats(self) -> dict:
+        """Counters of the state's single-electron updates: proposals and the mean / maximum
+        number of electrons whose embedding changed per move."""
+        nw, npr, s, mx = ctypes.c_int32(), ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        _lib.check(self._lib.fire_state_stats(self._h, ctypes.byref(nw), ctypes.byref(npr), ctypes.byref(s),
+                                              ctypes.byref(mx)))
+        return dict(n_walkers=nw.value, n_proposals=npr.value,
+                    mean_changed=(s.value / npr.value) if npr.value else 0.0, max_changed=mx.value)
+
+    def mcmc_steps_low_rank(self, params, electrons, steps: int, width: float, seed: int = 0, step0: int = 0,
+                            precision=None, keep_state: bool = False):
+        """mcmc_steps_low_rank with proposal_single_electron (mcmc.py:100-111, 196-275):
+        ``steps`` Metropolis steps, step s moving electron s mod n_el of every walker.
+        Returns (electrons, stats) with stats["pmove"] the acceptance rate.  With
+        ``keep_state`` the resident state of an earlier call (same walkers) is continued
+        instead of being rebuilt from ``electrons``."""
+        self._ensure(params)
+        if not keep_state or getattr(self, "_committed_gen", None) is None:
+            self.log_psi_with_state(params, electrons, precision=precision)
+        elif self._pending_gen is not None:
+            self._pending_gen = None  # a pending proposal is dropped by the next step
+        B = self._state_walkers()
+        nacc = torch.zeros(B, dtype=torch.int32, device=self.device)
+        _lib.check(self._lib.fire_mcmc_steps(self._h, _stream_ptr(), int(steps), float(width), int(seed) & (2**64 - 1),
+                                             int(step0), _dptr(nacc)))
+        r, sg, lp = self.state_values()
+        pmove = float(nacc.double().mean() / steps) if steps else 0.0
+        return r, dict(pmove=pmove, n_accept=nacc, logpsi=lp, sign=sg, **self.state_stats())
+
+
+@dataclass
+class WalkerState:
+    """Names the GPU-resident state of a ``FireWaveFunction`` (committed, or a pending
+    proposal) -- the reference's model state pytree (wave_function.py:226-287)."""
+
+    wf: FireWaveFunction
+    gen: int
+    n_walkers: int
+    single: bool
+    pending: bool
+
 
 def make_local_energy(wf: FireWaveFunction, energy_operator: str = "sparse", precision: Optional[str] = None):
     """make_local_energy (hamiltonian.py:49-81) for all-electron molecules: returns
