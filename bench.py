@@ -1,18 +1,25 @@
 """Benchmark: FiRE local-energy walkers/s on one MI355X.
 
 Workload (default): zigzag alkane C16H34, 130 electrons (65/65), 1024 walkers, cutoff 5 bohr,
-F = 256, 4 determinants, default Jastrow (MLP 256x2 + 16x16 attention, psiformer cusp).
-A step = one sparse forward-Laplacian local-energy evaluation (kinetic + Coulomb) of the
-whole batch, walkers resident in HBM.  The mixed-precision path is the headline value; the
-plain-FP32 kernels are timed the same way beside it.
+F = 256, 4 determinants, default Jastrow (MLP 256x2 + 16x16 attention, psiformer cusp),
+random-init parameters in the reference layout.
 
-Prints one JSON line (see the field list at the bottom).
+1. Sampling: the walkers are equilibrated by single-electron Metropolis sweeps (Gaussian
+   steps of 0.3 bohr, FP64 Woodbury low-rank determinant updates, mixed precision) on the
+   GPU; acceptance, throughput and the drift of the resident log psi are reported.
+2. Timed step = one sparse forward-Laplacian local-energy evaluation (kinetic + Coulomb) of
+   the whole batch, walkers resident in HBM.  The mixed-precision path is the headline value;
+   the plain-FP32 kernels (and the faster two-plane split) are timed the same way beside it.
+3. Energy error of each path against the FP64 oracle on a sample of the equilibrated
+   walkers; the oracle in FP32 timed on the host cores as the CPU baseline.
+
+Prints one JSON line (fields at the bottom of main()).
 """
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
-import math
 import os
 import sys
 import time
@@ -24,6 +31,10 @@ sys.path.insert(0, ROOT)
 
 METRIC = "local-energy walkers/s at 130 e-, mixed vs plain FP32 at <=1 mHa; p99 latency"
 PEAK = {"bf16": 2500.0, "fp32": 157.3, "fp64": 78.6, "hbm_gbs": 8000.0}  # MI355X dense peaks (TFLOP/s, GB/s)
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def algorithmic_flops(cfg, counts_per_walker):
@@ -51,6 +62,12 @@ def algorithmic_flops(cfg, counts_per_walker):
     return float(sum(terms.values())), {k: float(v) for k, v in terms.items()}, dict(P=P, T=T, R=R)
 
 
+def err_stats(d):
+    d = np.asarray(d, np.float64) * 1e3
+    return {"bias": float(np.mean(d)), "rms": float(np.sqrt(np.mean(d ** 2))), "median_abs": float(np.median(np.abs(d))),
+            "max_abs": float(np.max(np.abs(d)))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -58,9 +75,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="c16h34")
     ap.add_argument("--walkers", type=int, default=None)
-    ap.add_argument("--cpu-sample", type=int, default=1, help="walkers timed on the CPU oracle (FP32)")
-    ap.add_argument("--bias-sample", type=int, default=8, help="walkers checked against the FP64 oracle")
+    ap.add_argument("--sweeps", type=int, default=50, help="Metropolis sweeps (n_el single-electron steps each)")
+    ap.add_argument("--width", type=float, default=0.3, help="Gaussian step (bohr)")
+    ap.add_argument("--cpu-sample", type=int, default=64, help="walkers timed on the CPU oracle (FP32)")
+    ap.add_argument("--bias-sample", type=int, default=64, help="walkers checked against the FP64 oracle")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a device copy of this many MiB (calibrates HBM byte counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (walkers would shard "
@@ -68,6 +89,16 @@ def main():
         sys.exit(2)
 
     import torch
+
+    if args.copy_probe:
+        n = args.copy_probe * (1 << 20) // 4
+        src = torch.randn(n, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": 4 * n, "copy_probe_bytes_written": 4 * n, "launches": 3}))
+        return
 
     from sparse_wf_amd import systems
     from sparse_wf_amd import params as P
@@ -78,22 +109,55 @@ def main():
     mol, over, n_walk, sigma = systems.workload(args.workload)
     B = args.walkers or n_walk
     cfg = FireConfig(mol=mol, **over)
-    r_np = systems.init_electrons(mol, B, stddev=sigma, seed=0)
+    N = cfg.n_el
+    r0 = systems.init_electrons(mol, B, stddev=sigma, seed=0)
     wf = FireWaveFunction(cfg, device=0)
-    params = wf.init(0, r_np[0])
+    params = wf.init(0, r0[0])
     P.exercise_jastrow(params)
     wf.upload(params)
-    r = torch.as_tensor(r_np, device="cuda")
+
+    # 1. equilibrate with the GPU sampler (mixed precision state)
+    mcmc = None
+    r = torch.as_tensor(r0, device="cuda")
+    if args.sweeps > 0:
+        wf.log_psi_with_state(params, r, precision="mixed")
+        n_acc = torch.zeros(B, dtype=torch.int32, device="cuda")
+        t_mc = 0.0
+        for s in range(args.sweeps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            r, st = wf.mcmc_steps_low_rank(params, r, N, width=args.width, seed=1, step0=s * N, keep_state=True)
+            torch.cuda.synchronize()
+            t_mc += time.perf_counter() - t0
+            n_acc += st["n_accept"]
+            if s == 0 or (s + 1) % 10 == 0:
+                log(f"sweep {s + 1}/{args.sweeps}: pmove {st['pmove']:.3f}, {t_mc:.1f} s")
+        st = wf.state_stats()
+        lp_state = wf.state_values()[2]
+        lp_fresh = wf.signed(params, r, precision="mixed")[1]
+        mcmc = {
+            "sweeps": args.sweeps, "steps": args.sweeps * N, "width_bohr": args.width,
+            "walker_moves_per_s": B * N * args.sweeps / t_mc, "ms_per_step": 1e3 * t_mc / (N * args.sweeps),
+            "pmove": float(n_acc.double().mean()) / (N * args.sweeps),
+            "changed_electrons_per_move": {"mean": st["mean_changed"], "max": st["max_changed"]},
+            "logpsi_drift_max": float((lp_state - lp_fresh).abs().max()),
+            "what": "single-electron Metropolis steps, FP64 Woodbury updates of the resident state, mixed precision",
+        }
+
     counts, _, _ = wf.neighbours(r)
     mean_counts = counts.double().mean(0).cpu().numpy()
     flops_w, flop_terms, sizes = algorithmic_flops(cfg, mean_counts)
 
+    def _prof(enable):
+        ms, fl, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_int64()
+        _lib.check(wf._lib.fire_profile_gemm(wf._h, int(enable), ctypes.byref(ms), ctypes.byref(fl), ctypes.byref(n)))
+        return ms.value, fl.value, n.value
+
+    # 2. timed local-energy steps
     def run(prec, steps, warmup, profile=False):
         for _ in range(warmup):
             wf.local_energy_terms(params, r, precision=prec)
         torch.cuda.synchronize()
-        if profile:
-            _prof(True)
         lat = []
         t0 = time.perf_counter()
         for _ in range(steps):
@@ -101,40 +165,47 @@ def main():
             out = wf.local_energy_terms(params, r, precision=prec)
             torch.cuda.synchronize()
             lat.append(time.perf_counter() - ts)
-        torch.cuda.synchronize()
         dt = time.perf_counter() - t0
-        prof = _prof(False) if profile else None
+        prof = None
+        if profile:  # one more step with HIP events around every dense-contraction launch
+            _prof(True)
+            wf.local_energy_terms(params, r, precision=prec)
+            torch.cuda.synchronize()
+            prof = _prof(False)
         return out, dt, lat, prof
 
-    import ctypes
-
-    def _prof(enable):
-        ms, fl, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_int64()
-        _lib.check(wf._lib.fire_profile_gemm(wf._h, int(enable), ctypes.byref(ms), ctypes.byref(fl), ctypes.byref(n)))
-        return ms.value, fl.value, n.value
-
+    log("timing mixed")
     out_m, dt_m, lat_m, prof_m = run("mixed", args.steps, args.warmup, profile=True)
-    out_f, dt_f, lat_f, _ = run("fp32", max(1, args.steps // 2), 1)
+    log("timing fp32")
+    out_f, dt_f, lat_f, _ = run("fp32", args.steps, 1)
+    log("timing mixed_fast")
+    out_x, dt_x, _, _ = run("mixed_fast", max(1, args.steps // 2), 1)
     wps_m = B * args.steps / dt_m
-    wps_f = B * max(1, args.steps // 2) / dt_f
+    wps_f = B * args.steps / dt_f
+    wps_x = B * max(1, args.steps // 2) / dt_x
 
-    # dominant kernel family: the MFMA contractions (HIP events on the launch stream)
     g_ms, g_flops, g_n = prof_m
     g_achieved = (g_flops / (g_ms * 1e-3)) / 1e12 if g_ms > 0 else None
     roofline = {
         "bound": "mfma",
-        "kernel": "k_gemm_bf16 (all dense contractions of one step, BF16 MFMA)",
+        "kernel": "k_gemm_bf16<6> (every dense contraction of one step; BF16 MFMA on three-plane split operands)",
         "achieved": g_achieved,
         "peak": PEAK["bf16"],
         "unit": "TFLOP/s",
         "frac": (g_achieved / PEAK["bf16"]) if g_achieved else None,
         "traffic": None,
-        "launches_timed": g_n,
+        "avg_launch_ms": g_ms / g_n if g_n else None,
+        "launches_per_step": g_n,
+        "algorithmic_flop_per_step": g_flops,
+        "note": "achieved = algorithmic 2*M*N*K per launch (the three-plane split issues 6 MFMAs per product, "
+                "i.e. 6x these flops on the matrix cores) over the HIP-event duration of the launches on "
+                "their stream; whole-step rate in whole_step_tflops",
         "whole_step_tflops": flops_w * wps_m / 1e12,
     }
 
     e_m = out_m["e_loc"].cpu().numpy()
     e_f = out_f["e_loc"].cpu().numpy()
+    e_x = out_x["e_loc"].cpu().numpy()
     result = {
         "metric": METRIC,
         "value": wps_m,
@@ -147,45 +218,50 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "bf16",
-        "data": "synthetic (seeded walkers, random-init params in the reference layout)",
+        "data": "synthetic (seeded walkers equilibrated by the GPU sampler, random-init params in the reference layout)",
         "config": {"workload": f"{mol.name} N={cfg.n_el} walkers={B} cutoff={cfg.cutoff} F={cfg.feature_dim} "
                                f"D={cfg.n_determinants}", "global_batch": B, "parallelism": "none (1 GPU)"},
         "fp32_walkers_per_s": wps_f,
         "mixed_over_fp32": wps_m / wps_f,
+        "mixed_fast_walkers_per_s": wps_x,
         "latency_ms": {"p50": 1e3 * float(np.percentile(lat_m, 50)), "p99": 1e3 * float(np.percentile(lat_m, 99)),
-                       "what": "one local-energy call of the whole batch"},
-        "mixed_minus_fp32_mean_mHa": 1e3 * float(np.mean(e_m - e_f)),
+                       "what": "one local-energy call of the whole batch (mixed)"},
+        "mixed_minus_fp32_mHa": err_stats(e_m - e_f),
+        "mixed_fast_minus_fp32_mHa": err_stats(e_x - e_f),
         "algorithmic_gflop_per_walker": flops_w / 1e9,
+        "gflop_terms_per_walker": {k: v / 1e9 for k, v in flop_terms.items()},
         "graph": {"pairs_per_walker": sizes["P"], "triplets_per_walker": sizes["T"], "rows_per_walker": sizes["R"]},
         "roofline": roofline,
+        "mcmc": mcmc,
     }
 
     if not args.no_cpu:
         # CPU baseline leg: the oracle (FP32) timed on a bounded sample of the same walkers,
-        # and the FP64 oracle as the reference for the energy bias of both GPU paths.
+        # and the FP64 oracle as the reference for the energy error of the GPU paths.
         from oracle import fire
 
         torch.set_num_threads(len(os.sched_getaffinity(0)))
+        r_np = r.cpu().numpy()
+        log(f"CPU baseline on {args.cpu_sample} walkers")
         p32 = fire.to_torch(params, torch.float32)
+        fire.local_energy(p32, cfg, r_np[0])
         t0 = time.perf_counter()
         for w in range(args.cpu_sample):
             fire.local_energy(p32, cfg, r_np[w])
         dt_cpu = time.perf_counter() - t0
         result["cpu_baseline"] = {"value": args.cpu_sample / dt_cpu, "unit": "walkers/s",
                                   "cores": torch.get_num_threads(), "kind": "port",
-                                  "sample": f"{args.cpu_sample} walker(s) of the same batch, oracle in FP32 "
-                                            f"(torch CPU), {dt_cpu:.1f} s"}
+                                  "sample": f"{args.cpu_sample} walkers of the same batch, oracle in FP32 "
+                                            f"(torch CPU, sparse forward Laplacian), {dt_cpu:.1f} s"}
         if args.bias_sample > 0:
+            log(f"FP64 oracle on {args.bias_sample} walkers")
             p64 = fire.to_torch(params, torch.float64)
             ref = np.array([fire.local_energy(p64, cfg, r_np[w])[0] for w in range(args.bias_sample)])
-            dm = e_m[: args.bias_sample] - ref
-            dfp = e_f[: args.bias_sample] - ref
+            n = args.bias_sample
             result["energy_error_vs_fp64_mHa"] = {
-                "walkers": args.bias_sample,
-                "mixed": {"bias": 1e3 * float(np.mean(dm)), "rms": 1e3 * float(np.sqrt(np.mean(dm ** 2))),
-                          "max": 1e3 * float(np.max(np.abs(dm)))},
-                "fp32": {"bias": 1e3 * float(np.mean(dfp)), "rms": 1e3 * float(np.sqrt(np.mean(dfp ** 2))),
-                         "max": 1e3 * float(np.max(np.abs(dfp)))},
+                "walkers": n, "mixed": err_stats(e_m[:n] - ref), "fp32": err_stats(e_f[:n] - ref),
+                "mixed_fast": err_stats(e_x[:n] - ref),
+                "gate": "|bias| <= 1 mHa",
             }
     print(json.dumps(result))
 

@@ -41,9 +41,14 @@ typedef struct fire_config {
 
 enum fire_precision {
   FIRE_FP32 = 0,  /* every embedding / orbital / Jastrow contraction in FP32 (FP32 MFMA) */
-  FIRE_MIXED = 1, /* dense contractions as split BF16 (hi*hi + hi*lo + lo*hi) on BF16 MFMA,
-                     FP32 accumulation; element-wise rules FP32; determinants FP64 */
+  FIRE_MIXED = 1, /* dense contractions on BF16 MFMA with FP32 accumulation, each operand
+                     carried as three BF16 planes (hi + mid + lo) and the six products down
+                     to 2^-16 relative accumulated (FP32-level products); element-wise
+                     forward-Laplacian rules, envelopes and Jastrow sums FP32; Slater
+                     determinants, their inverses, updates and energy sums FP64 */
   FIRE_BF16 = 2,  /* dense contractions in single BF16 on MFMA (fastest, least accurate) */
+  FIRE_MIXED_FAST = 3, /* as FIRE_MIXED with two BF16 planes (hi + lo) and three products
+                          (hi*hi + hi*lo + lo*hi): ~2^-17 relative per product */
 };
 
 /* Creates a handle for molecule (R [n_nuc*3] bohr, Z [n_nuc]) on HIP device `device`.

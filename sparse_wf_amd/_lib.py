@@ -13,6 +13,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libfire_amd.so"
 FIRE_FP32 = 0
 FIRE_MIXED = 1
 FIRE_BF16 = 2
+FIRE_MIXED_FAST = 3
 
 
 class FireConfigC(ctypes.Structure):

@@ -78,7 +78,8 @@ struct fire_handle {
   std::vector<int64_t> expect;  // element counts in canonical order
   std::vector<float*> P;
   std::vector<uint16_t*> Pbf;   // BF16 transposed copies of GEMM weights (null if none): hi part
-  std::vector<uint16_t*> Pbf_lo;  // ... and the BF16 remainder W - hi
+  std::vector<uint16_t*> Pbf_lo;  // ... the BF16 remainder W - hi (mid plane)
+  std::vector<uint16_t*> Pbf_lo2;  // ... and the BF16 remainder W - hi - mid (lo plane)
   std::vector<float*> Pt32;       // FP32 transposed copies [N,K] of GEMM weights
   std::vector<std::pair<int, int>> shape2;  // (K, N) of GEMM weights
   float cut_learned = 0.f;
@@ -120,7 +121,7 @@ struct fire_handle {
     // determinant and Jastrow of the proposal
     DevBuf phin, sinv, ldS, sgS, jsum, rin, ry1, ry2, att_out, cval, clap, cgrad, epot;
     DevBuf p_ld, p_sg, p_logpsi, p_sign, accept;
-    size_t wb_smem_set = 0;
+    size_t wb_smem_prep = 0, wb_smem_upd = 0;
     void release() {
       DevBuf* b[] = {&r,     &M5,    &hh,    &orb,     &env,   &A,      &Ainv,  &ld,    &sg,     &jr,    &av,
                      &logpsi, &sign, &p_r,   &p_rk,    &p_u,   &aff,    &cu,    &cd,    &offu,   &offd,  &tail,
@@ -236,9 +237,9 @@ void gemm(Ctx& c, const float* A, int64_t lda, int wi, int bi, float* C, int64_t
     HIPCHK(hipEventCreate(&e1));
     HIPCHK(hipEventRecord(e0, c.s));
   }
-  if ((c.prec == FIRE_MIXED || c.prec == FIRE_BF16) && bf_ok)
-    launch_gemm_bf16(c.s, A, lda, h->Pbf[wi], h->Pbf_lo[wi], bias, C, ldc, M, N, K, rowflag, period,
-                     c.prec == FIRE_MIXED ? 3 : 1);
+  if ((c.prec == FIRE_MIXED || c.prec == FIRE_MIXED_FAST || c.prec == FIRE_BF16) && bf_ok)
+    launch_gemm_bf16(c.s, A, lda, h->Pbf[wi], h->Pbf_lo[wi], h->Pbf_lo2[wi], bias, C, ldc, M, N, K, rowflag, period,
+                     c.prec == FIRE_MIXED ? 6 : (c.prec == FIRE_MIXED_FAST ? 3 : 1));
   else
     launch_gemm_f32(c.s, A, lda, h->P[wi], h->Pt32[wi], bias, C, ldc, M, N, K, rowflag, period);
   if (h->gemm_timing) {
@@ -678,13 +679,15 @@ void state_propose(Ctx& c, int k, int mode, const float* rk_in, float width, uin
   S.n_proposals += B;
   S.sum_changed += T;
   S.max_changed = std::max<int64_t>(S.max_changed, amax);
+  // dynamic LDS of the Woodbury kernels (their static LDS, < 128 bytes, comes on top)
   const size_t pw = wb_prep_smem(N, amax), uw = wb_update_smem(N, amax);
-  REQUIRE(pw <= 160 * 1024 && uw <= 160 * 1024,
+  REQUIRE(pw <= 160 * 1024 - 512 && uw <= 160 * 1024 - 512,
           "too many electrons changed by one move (" + std::to_string(amax) +
               ") for the LDS-resident Woodbury update at this electron count");
-  if (std::max(pw, uw) > 64 * 1024 && std::max(pw, uw) > S.wb_smem_set) {
-    HIPCHK(wb_set_smem_limit(160 * 1024, 160 * 1024));
-    S.wb_smem_set = 160 * 1024;
+  if ((pw > 64 * 1024 || uw > 64 * 1024) && (pw > S.wb_smem_prep || uw > S.wb_smem_upd)) {
+    S.wb_smem_prep = std::max(S.wb_smem_prep, pw);
+    S.wb_smem_upd = std::max(S.wb_smem_upd, uw);
+    HIPCHK(wb_set_smem_limit(S.wb_smem_prep, S.wb_smem_upd));
   }
 
   // h0 and messages of the moved electron (ElecInit on one electron per walker)
@@ -847,6 +850,8 @@ int fail(const std::exception& e) {
   return 1;
 }
 
+bool valid_precision(int p) { return p == FIRE_FP32 || p == FIRE_MIXED || p == FIRE_BF16 || p == FIRE_MIXED_FAST; }
+
 void check_ready(fire_handle* h) {
   REQUIRE(h != nullptr, "null handle");
   REQUIRE(h->params_ok, "parameters not uploaded (call fire_upload_params first)");
@@ -929,6 +934,8 @@ int fire_destroy(fire_handle* h) {
     if (p) (void)hipFree(p);
   for (uint16_t* p : h->Pbf_lo)
     if (p) (void)hipFree(p);
+  for (uint16_t* p : h->Pbf_lo2)
+    if (p) (void)hipFree(p);
   for (float* p : h->Pt32)
     if (p) (void)hipFree(p);
   if (h->R) (void)hipFree(h->R);
@@ -959,11 +966,14 @@ int fire_upload_params(fire_handle* h, const float* packed, const int64_t* sizes
       if (p) HIPCHK(hipFree(p));
     for (uint16_t* p : h->Pbf_lo)
       if (p) HIPCHK(hipFree(p));
+    for (uint16_t* p : h->Pbf_lo2)
+      if (p) HIPCHK(hipFree(p));
     for (float* p : h->Pt32)
       if (p) HIPCHK(hipFree(p));
     h->P.assign(n_tensors, nullptr);
     h->Pbf.assign(n_tensors, nullptr);
     h->Pbf_lo.assign(n_tensors, nullptr);
+    h->Pbf_lo2.assign(n_tensors, nullptr);
     h->Pt32.assign(n_tensors, nullptr);
     size_t off = 0;
     for (int k = 0; k < n_tensors; ++k) {
@@ -981,22 +991,31 @@ int fire_upload_params(fire_handle* h, const float* packed, const int64_t* sizes
     }
     for (auto& t : gw) {
       const int k = t.first, K = t.second.first, Nn = t.second.second;
-      std::vector<uint16_t> tb((size_t)K * Nn), tl((size_t)K * Nn);
+      // three BF16 planes, transposed to [N,K]: x = hi + mid + lo (remainders exact in FP32)
+      std::vector<uint16_t> pl[3];
+      for (auto& v : pl) v.resize((size_t)K * Nn);
       const float* src = packed + offs[k];
+      auto bf2f_h = [](uint16_t b) {
+        uint32_t u = (uint32_t)b << 16;
+        float f;
+        std::memcpy(&f, &u, 4);
+        return f;
+      };
       for (int kk = 0; kk < K; ++kk)
         for (int n = 0; n < Nn; ++n) {
           const float x = src[(size_t)kk * Nn + n];
           const uint16_t hi = host_f2bf(x);
-          uint32_t u = (uint32_t)hi << 16;
-          float xh;
-          std::memcpy(&xh, &u, 4);
-          tb[(size_t)n * K + kk] = hi;
-          tl[(size_t)n * K + kk] = host_f2bf(x - xh);
+          const float r1 = x - bf2f_h(hi);
+          const uint16_t mid = host_f2bf(r1);
+          pl[0][(size_t)n * K + kk] = hi;
+          pl[1][(size_t)n * K + kk] = mid;
+          pl[2][(size_t)n * K + kk] = host_f2bf(r1 - bf2f_h(mid));
         }
-      HIPCHK(hipMalloc(&h->Pbf[k], sizeof(uint16_t) * tb.size()));
-      HIPCHK(hipMemcpy(h->Pbf[k], tb.data(), sizeof(uint16_t) * tb.size(), hipMemcpyHostToDevice));
-      HIPCHK(hipMalloc(&h->Pbf_lo[k], sizeof(uint16_t) * tl.size()));
-      HIPCHK(hipMemcpy(h->Pbf_lo[k], tl.data(), sizeof(uint16_t) * tl.size(), hipMemcpyHostToDevice));
+      uint16_t** dst[3] = {&h->Pbf[k], &h->Pbf_lo[k], &h->Pbf_lo2[k]};
+      for (int q = 0; q < 3; ++q) {
+        HIPCHK(hipMalloc(dst[q], sizeof(uint16_t) * pl[q].size()));
+        HIPCHK(hipMemcpy(*dst[q], pl[q].data(), sizeof(uint16_t) * pl[q].size(), hipMemcpyHostToDevice));
+      }
     }
     // FP32 transposed copies [N,K] for the FP32 MFMA kernel: every GEMM weight, and the
     // envelope weights pi per determinant ([det][ME][N] -> [det][N][ME])
@@ -1096,7 +1115,7 @@ int fire_local_energy(fire_handle* h, void* stream, const float* r_dev, int32_t 
   try {
     check_ready(h);
     REQUIRE(r_dev && e_loc_dev, "null argument");
-    REQUIRE(precision == FIRE_FP32 || precision == FIRE_MIXED || precision == FIRE_BF16, "unknown precision mode");
+    REQUIRE(valid_precision(precision), "unknown precision mode");
     HIPCHK(hipSetDevice(h->device));
     Ctx c{h, (hipStream_t)stream, precision};
     const int N = h->N;
@@ -1125,7 +1144,7 @@ int fire_signed(fire_handle* h, void* stream, const float* r_dev, int32_t B, int
   try {
     check_ready(h);
     REQUIRE(r_dev && logpsi_dev, "null argument");
-    REQUIRE(precision == FIRE_FP32 || precision == FIRE_MIXED || precision == FIRE_BF16, "unknown precision mode");
+    REQUIRE(valid_precision(precision), "unknown precision mode");
     HIPCHK(hipSetDevice(h->device));
     Ctx c{h, (hipStream_t)stream, precision};
     const int N = h->N;
@@ -1199,7 +1218,7 @@ int fire_state_init(fire_handle* h, void* stream, const float* r_dev, int32_t B,
     check_ready(h);
     REQUIRE(r_dev, "null argument");
     REQUIRE(B > 0, "batch must be positive");
-    REQUIRE(precision == FIRE_FP32 || precision == FIRE_MIXED || precision == FIRE_BF16, "unknown precision mode");
+    REQUIRE(valid_precision(precision), "unknown precision mode");
     HIPCHK(hipSetDevice(h->device));
     Ctx c{h, (hipStream_t)stream, precision};
     state_init(c, r_dev, B);

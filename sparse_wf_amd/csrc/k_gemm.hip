@@ -13,7 +13,9 @@
 //   FP32 path : v_mfma_f32_32x32x2f32, A and W in FP32.
 //   BF16 path : v_mfma_f32_32x32x16_bf16, FP32 accumulation.  NSPLIT = 3 carries each
 //               operand as hi + lo BF16 pair (x = hi + lo to ~2^-17 relative) and
-//               accumulates hi*hi + hi*lo + lo*hi ("split BF16"), NSPLIT = 1 is plain BF16.
+//               accumulates hi*hi + hi*lo + lo*hi ("split BF16 x3"); NSPLIT = 6 carries
+//               hi + mid + lo (x to ~2^-26) and accumulates the six products whose order is
+//               at most 2^-16 relative (x6, FP32-level products); NSPLIT = 1 is plain BF16.
 #include "fire_common.hpp"
 #include "fire_kernels.hpp"
 
@@ -155,13 +157,13 @@ __global__ void __launch_bounds__(256) k_gemm_f32(const float* __restrict__ A, i
 // BF16 / split-BF16 MFMA GEMM, FP32 accumulation.  Whi/Wlo are [N,K] BF16 (W transposed).
 // Requires K % 32 == 0 and lda % 4 == 0 (checked by the host).
 // ---------------------------------------------------------------------------------------
-template <int NSPLIT>
+template <int NSPLIT, bool SEPACC>
 __global__ void __launch_bounds__(256) k_gemm_bf16(const float* __restrict__ A, int64_t lda,
                                                    const uint16_t* __restrict__ Whi, const uint16_t* __restrict__ Wlo,
-                                                   const float* __restrict__ bias, float* __restrict__ C, int64_t ldc,
-                                                   int64_t M, int N, int K, const uint8_t* __restrict__ rowflag,
-                                                   int period, int nT) {
-  constexpr int NP = NSPLIT == 3 ? 2 : 1;  // hi (+ lo) planes
+                                                   const uint16_t* __restrict__ Wlo2, const float* __restrict__ bias,
+                                                   float* __restrict__ C, int64_t ldc, int64_t M, int N, int K,
+                                                   const uint8_t* __restrict__ rowflag, int period, int nT) {
+  constexpr int NP = NSPLIT == 6 ? 3 : (NSPLIT == 3 ? 2 : 1);  // hi (+ mid/lo) planes
   __shared__ __attribute__((aligned(16))) uint16_t As[NP][GT_M][LDB_H];
   __shared__ __attribute__((aligned(16))) uint16_t Bs[NP][GT_N][LDB_H];
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
@@ -171,10 +173,17 @@ __global__ void __launch_bounds__(256) k_gemm_bf16(const float* __restrict__ A, 
   const int64_t m0 = mt * GT_M;
   const int n0 = nt * GT_N;
   const int wm = (wid >> 1) * 64, wn = (wid & 1) * 64;
-  float16_t acc[2][2];
+  // acc: hi*hi products; with SEPACC, accs takes the cross products of the split planes so
+  // that they are not rounded against the (2^8 times larger) running hi*hi sum (about 3x
+  // smaller error on ill-conditioned rows, at twice the accumulator registers)
+  constexpr bool SEP = SEPACC && NP >= 2;
+  float16_t acc[2][2], accs[2][SEP ? 2 : 1];
   for (int a = 0; a < 2; ++a)
     for (int b = 0; b < 2; ++b)
-      for (int v = 0; v < 16; ++v) acc[a][b][v] = 0.f;
+      for (int v = 0; v < 16; ++v) {
+        acc[a][b][v] = 0.f;
+        if (SEP) accs[a][SEP ? b : 0][v] = 0.f;
+      }
 
   // per-thread load slots: A 128 rows x 32 k = 1024 float4 -> 4 per thread;
   //                        W 128 n x 32 k = 512 x (8 bf16) -> 2 per thread per plane
@@ -194,8 +203,10 @@ __global__ void __launch_bounds__(256) k_gemm_bf16(const float* __restrict__ A, 
       const int nn = e >> 2, kq = (e & 3) * 8;
       const int gn = n0 + nn;
       rb[0][q] = gn < N ? *reinterpret_cast<const uint4*>(Whi + (int64_t)gn * K + k0 + kq) : make_uint4(0, 0, 0, 0);
-      if (NP == 2)
-        rb[NP - 1][q] = gn < N ? *reinterpret_cast<const uint4*>(Wlo + (int64_t)gn * K + k0 + kq) : make_uint4(0, 0, 0, 0);
+      if (NP >= 2)
+        rb[1][q] = gn < N ? *reinterpret_cast<const uint4*>(Wlo + (int64_t)gn * K + k0 + kq) : make_uint4(0, 0, 0, 0);
+      if (NP == 3)
+        rb[2][q] = gn < N ? *reinterpret_cast<const uint4*>(Wlo2 + (int64_t)gn * K + k0 + kq) : make_uint4(0, 0, 0, 0);
     }
   };
   auto store = [&]() {
@@ -204,17 +215,19 @@ __global__ void __launch_bounds__(256) k_gemm_bf16(const float* __restrict__ A, 
       const int e = tid + q * 256;
       const int rr = e >> 3, kq = (e & 7) * 4;
       const float x[4] = {ra[q].x, ra[q].y, ra[q].z, ra[q].w};
-      uint16_t hi[4], lo[4];
+      uint16_t pl[3][4];
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        hi[j] = f2bf(x[j]);
-        lo[j] = f2bf(x[j] - bf2f(hi[j]));
+        // x = hi + mid + lo; each remainder is exact in FP32
+        pl[0][j] = f2bf(x[j]);
+        const float r1 = x[j] - bf2f(pl[0][j]);
+        pl[1][j] = f2bf(r1);
+        pl[2][j] = f2bf(r1 - bf2f(pl[1][j]));
       }
-      *reinterpret_cast<uint2*>(&As[0][rr][kq]) =
-          make_uint2((uint32_t)hi[0] | ((uint32_t)hi[1] << 16), (uint32_t)hi[2] | ((uint32_t)hi[3] << 16));
-      if (NP == 2)
-        *reinterpret_cast<uint2*>(&As[NP - 1][rr][kq]) =
-            make_uint2((uint32_t)lo[0] | ((uint32_t)lo[1] << 16), (uint32_t)lo[2] | ((uint32_t)lo[3] << 16));
+#pragma unroll
+      for (int p = 0; p < NP; ++p)
+        *reinterpret_cast<uint2*>(&As[p][rr][kq]) = make_uint2((uint32_t)pl[p][0] | ((uint32_t)pl[p][1] << 16),
+                                                               (uint32_t)pl[p][2] | ((uint32_t)pl[p][3] << 16));
     }
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
@@ -233,30 +246,37 @@ __global__ void __launch_bounds__(256) k_gemm_bf16(const float* __restrict__ A, 
 #pragma unroll
     for (int kk = 0; kk < GT_K; kk += 16) {
       const int ka = kk + 8 * (lane >> 5);
-      bf16x8_t ah[2], bh[2], al[2], bl[2];
+      bf16x8_t av[NP][2], bv[NP][2];
 #pragma unroll
-      for (int a = 0; a < 2; ++a) ah[a] = *reinterpret_cast<const bf16x8_t*>(&As[0][wm + a * 32 + (lane & 31)][ka]);
+      for (int p = 0; p < NP; ++p) {
 #pragma unroll
-      for (int b = 0; b < 2; ++b) bh[b] = *reinterpret_cast<const bf16x8_t*>(&Bs[0][wn + b * 32 + (lane & 31)][ka]);
-      if (NP == 2) {
+        for (int a = 0; a < 2; ++a) av[p][a] = *reinterpret_cast<const bf16x8_t*>(&As[p][wm + a * 32 + (lane & 31)][ka]);
 #pragma unroll
-        for (int a = 0; a < 2; ++a) al[a] = *reinterpret_cast<const bf16x8_t*>(&As[NP - 1][wm + a * 32 + (lane & 31)][ka]);
-#pragma unroll
-        for (int b = 0; b < 2; ++b) bl[b] = *reinterpret_cast<const bf16x8_t*>(&Bs[NP - 1][wn + b * 32 + (lane & 31)][ka]);
+        for (int b = 0; b < 2; ++b) bv[p][b] = *reinterpret_cast<const bf16x8_t*>(&Bs[p][wn + b * 32 + (lane & 31)][ka]);
       }
 #pragma unroll
       for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b) {
-          if (NP == 2) {
-            acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[a], bh[b], acc[a][b], 0, 0, 0);
-            acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a], bl[b], acc[a][b], 0, 0, 0);
+          // smallest products first
+          float16_t& as = SEP ? accs[a][SEP ? b : 0] : acc[a][b];
+          if (NP == 3) {
+            as = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[1][a], bv[1][b], as, 0, 0, 0);
+            as = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[2][a], bv[0][b], as, 0, 0, 0);
+            as = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[0][a], bv[2][b], as, 0, 0, 0);
           }
-          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a], bh[b], acc[a][b], 0, 0, 0);
+          if (NP >= 2) {
+            as = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[1][a], bv[0][b], as, 0, 0, 0);
+            as = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[0][a], bv[1][b], as, 0, 0, 0);
+          }
+          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[0][a], bv[0][b], acc[a][b], 0, 0, 0);
         }
     }
     __syncthreads();
   }
+  if (SEP)
+    for (int a = 0; a < 2; ++a)
+      for (int b = 0; b < 2; ++b) acc[a][b] += accs[a][SEP ? b : 0];
   store_tile(acc, m0, n0, wm, wn, lane, C, ldc, M, N, bias, rowflag, period);
 }
 
@@ -301,15 +321,19 @@ void launch_gemm_f32(hipStream_t s, const float* A, int64_t lda, const float* W,
 }
 
 void launch_gemm_bf16(hipStream_t s, const float* A, int64_t lda, const uint16_t* Whi, const uint16_t* Wlo,
-                      const float* bias, float* C, int64_t ldc, int64_t M, int N, int K, const uint8_t* rowflag,
-                      int period, int nsplit) {
+                      const uint16_t* Wlo2, const float* bias, float* C, int64_t ldc, int64_t M, int N, int K,
+                      const uint8_t* rowflag, int period, int nsplit) {
   if (M == 0) return;
   const int nT = (N + GT_N - 1) / GT_N;
   const int64_t blocks = ((M + GT_M - 1) / GT_M) * nT;
-  if (nsplit == 3)
-    k_gemm_bf16<3><<<(unsigned)blocks, 256, 0, s>>>(A, lda, Whi, Wlo, bias, C, ldc, M, N, K, rowflag, period, nT);
+  if (nsplit == 6)
+    k_gemm_bf16<6, false><<<(unsigned)blocks, 256, 0, s>>>(A, lda, Whi, Wlo, Wlo2, bias, C, ldc, M, N, K, rowflag, period,
+                                                    nT);
+  else if (nsplit == 3)
+    k_gemm_bf16<3, false><<<(unsigned)blocks, 256, 0, s>>>(A, lda, Whi, Wlo, Wlo, bias, C, ldc, M, N, K, rowflag, period, nT);
   else
-    k_gemm_bf16<1><<<(unsigned)blocks, 256, 0, s>>>(A, lda, Whi, Whi, bias, C, ldc, M, N, K, rowflag, period, nT);
+    k_gemm_bf16<1, false><<<(unsigned)blocks, 256, 0, s>>>(A, lda, Whi, Whi, Whi, bias, C, ldc, M, N, K, rowflag, period,
+                                                    nT);
 }
 
 }  // namespace fire

@@ -22,7 +22,10 @@ from . import _lib
 from .config import FireConfig, Molecule
 from .params import init_params, pack_params, padded_w1, set_scales
 
-PRECISIONS = {"fp32": _lib.FIRE_FP32, "mixed": _lib.FIRE_MIXED, "bf16": _lib.FIRE_BF16}
+# "mixed": BF16 MFMA on three-plane split operands (FP32-level products, the default);
+# "mixed_fast": two planes (~2^-17 per product); "bf16": one plane; "fp32": FP32 MFMA.
+PRECISIONS = {"fp32": _lib.FIRE_FP32, "mixed": _lib.FIRE_MIXED, "bf16": _lib.FIRE_BF16,
+              "mixed_fast": _lib.FIRE_MIXED_FAST}
 
 
 class StaticArgs(NamedTuple):

@@ -90,7 +90,7 @@ def test_neighbours_cutoff_boundary(gpu_device):
     assert 1 in nl1.nbr[nl1.offsets[0]:nl1.offsets[1]]
 
 
-@pytest.mark.parametrize("precision,tol", [("fp32", 2e-5), ("mixed", 2e-4), ("bf16", 3e-2)])
+@pytest.mark.parametrize("precision,tol", [("fp32", 2e-5), ("mixed", 2e-5), ("mixed_fast", 2e-4), ("bf16", 3e-2)])
 def test_embedding(setup, precision, tol):
     name, cfg, prm, wf, r = setup
     pt = fire.to_torch(prm)
@@ -102,7 +102,7 @@ def test_embedding(setup, precision, tol):
         assert e < tol
 
 
-@pytest.mark.parametrize("precision,tol", [("fp32", 1e-4), ("mixed", 1e-3)])
+@pytest.mark.parametrize("precision,tol", [("fp32", 1e-4), ("mixed", 1e-4), ("mixed_fast", 1e-3)])
 def test_signed(setup, precision, tol):
     name, cfg, prm, wf, r = setup
     pt = fire.to_torch(prm)
@@ -115,7 +115,7 @@ def test_signed(setup, precision, tol):
         assert d < tol * max(1.0, abs(float(l_ref)))
 
 
-@pytest.mark.parametrize("precision", ["fp32", "mixed"])
+@pytest.mark.parametrize("precision", ["fp32", "mixed", "mixed_fast"])
 def test_local_energy(setup, precision):
     name, cfg, prm, wf, r = setup
     pt = fire.to_torch(prm)
@@ -133,7 +133,11 @@ def test_local_energy(setup, precision):
               f"E_kin {float(out['e_kin'][w]):.6f}/{ek_ref:.6f} (rel {ek:.2e}) E_pot {float(out['e_pot'][w]):.6f}/{ep_ref:.6f}")
         assert abs(float(out["e_pot"][w]) - ep_ref) < 1e-9 * max(1.0, abs(ep_ref))
         assert float(out["sign"][w]) == float(sign)
+        # mixed: FP32-level products, but the BF16 MFMA accumulation is looser than FP32's on
+        # ill-conditioned walkers (a near-node water walker with Laplacian ~ -4e4 reaches 2.4e-4)
         if precision == "fp32":
             assert eg < 1e-4 and el < 1e-3 and ek < 1e-3
+        elif precision == "mixed":
+            assert eg < 5e-4 and el < 2e-3 and ek < 2e-3
         else:
             assert eg < 5e-3 and el < 1e-2 and ek < 1e-2
