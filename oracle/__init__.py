@@ -9,9 +9,9 @@ is measured or shipped.  Nothing under ``sparse_wf_amd/`` imports it.
 Modules
 -------
 ``fwdlap``   dense forward-Laplacian arrays (the role folx plays in the reference).
-``fire``     value path, sparse forward-Laplacian path, energies, low-rank update, MCMC.
+``fire``     value path, sparse forward-Laplacian path, energies, low-rank update.
 
-Pinning (see tests/test_oracle_*.py):
+Pinning (tests/test_oracle.py):
 
 * forward-Laplacian results vs a dense ``torch.func`` Hessian trace of the value path
   (float64, small N) -- the reference's own test strategy (tests/test_fwd_lap.py:57-117);

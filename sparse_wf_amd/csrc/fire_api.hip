@@ -120,7 +120,7 @@ struct fire_handle {
     DevBuf pcnt, pptr, pnbr, p_hc, p_x, p_h, p_y, p_orb, p_jr, p_qv, p_av;
     // determinant and Jastrow of the proposal
     DevBuf phin, sinv, ldS, sgS, jsum, rin, ry1, ry2, att_out, cval, clap, cgrad, epot;
-    DevBuf p_ld, p_sg, p_logpsi, p_sign, accept;
+    DevBuf p_ld, p_sg, p_logpsi, p_sign, accept, wbscr;
     size_t wb_smem_prep = 0, wb_smem_upd = 0;
     void release() {
       DevBuf* b[] = {&r,     &M5,    &hh,    &orb,     &env,   &A,      &Ainv,  &ld,    &sg,     &jr,    &av,
@@ -128,7 +128,7 @@ struct fire_handle {
                      &list,  &rowof, &klist, &kT5,     &kZ2,   &kT5b,   &p_M5k, &kcoef, &p_envk, &pcnt,  &pptr,
                      &pnbr,  &p_hc,  &p_x,   &p_h,     &p_y,   &p_orb,  &p_jr,  &p_qv,  &p_av,   &phin,  &sinv,
                      &ldS,   &sgS,   &jsum,  &rin,     &ry1,   &ry2,    &att_out, &cval, &clap,  &cgrad, &epot,
-                     &p_ld,  &p_sg,  &p_logpsi, &p_sign, &accept};
+                     &p_ld,  &p_sg,  &p_logpsi, &p_sign, &accept, &wbscr};
       for (DevBuf* x : b) x->release();
       ready = pending = false;
     }
@@ -753,6 +753,7 @@ void state_propose(Ctx& c, int k, int mode, const float* rk_in, float width, uin
   S.sinv.ensure(sizeof(double) * (size_t)B * D * std::max(amax, 1) * std::max(amax, 1));
   S.ldS.ensure(sizeof(double) * (size_t)B * D);
   S.sgS.ensure(sizeof(double) * (size_t)B * D);
+  S.wbscr.ensure(std::max<size_t>((size_t)B * D * wb_update_scratch(N, amax), 8));
   launch_wb_prep(c.s, S.p_orb.as<float>(), S.p_envk.as<float>(), S.env.as<float>(), S.A.as<double>(),
                  S.Ainv.as<double>(), S.aff.as<int32_t>(), S.cu.as<int32_t>(), S.cd.as<int32_t>(),
                  S.rowof.as<int32_t>(), B, N, nu, D, k, amax, S.phin.as<double>(), S.sinv.as<double>(),
@@ -834,7 +835,8 @@ void state_commit(Ctx& c) {
   const int B = S.B, N = h->N, nu = h->n_up, D = h->D;
   const int nc = h->H + h->HD;
   launch_wb_update(c.s, S.accept.as<int32_t>(), S.aff.as<int32_t>(), S.cu.as<int32_t>(), S.cd.as<int32_t>(), B, N, D,
-                   S.amax, S.phin.as<double>(), S.sinv.as<double>(), S.A.as<double>(), S.Ainv.as<double>());
+                   S.amax, S.phin.as<double>(), S.sinv.as<double>(), S.A.as<double>(), S.Ainv.as<double>(),
+                   S.wbscr.as<double>());
   launch_mc_commit(c.s, S.accept.as<int32_t>(), S.list.as<int2>(), S.T, B, N, nu, D, h->F, h->ND, nc, S.k,
                    S.p_h.as<float>(), S.p_orb.as<float>(), S.p_jr.as<float>(), S.p_av.as<float>(), S.p_rk.as<float>(),
                    S.p_M5k.as<float>(), S.p_envk.as<float>(), S.p_ld.as<double>(), S.p_sg.as<double>(),

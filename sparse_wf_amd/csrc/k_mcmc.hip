@@ -10,6 +10,8 @@
 // buffers ("p_*", rows compacted over the affected electrons of all walkers, up-spin rows
 // first) and committed to the state only for accepted walkers, so a rejected walker's
 // state is never touched.  Determinant work is FP64.
+#include <algorithm>
+
 #include "fire_common.hpp"
 #include "fire_kernels2.hpp"
 #include "fire_mcmc.hpp"
@@ -193,7 +195,8 @@ void launch_mc_klist(hipStream_t s, int B, int k, int2* list) { k_mc_klist<<<(B 
 
 // Woodbury preparation per (walker, det): new Slater rows Phi_new of the affected
 // electrons, S = I + V Ainv[:, aff] with V = Phi_new - A[aff], log|det S|, sign, S^-1.
-// Dynamic LDS: P = Ainv[:, aff] [N][amax] + S [amax][amax] + pivots.
+// Writes V = Phi_new - A[aff] (not Phi_new) to `phin`.  Dynamic LDS: P = Ainv[:, aff] [N][amax],
+// S [amax][amax] and pivots.
 __global__ void __launch_bounds__(256) k_wb_prep(const float* __restrict__ p_orb, const float* __restrict__ p_envk,
                                                  const float* __restrict__ st_env, const double* __restrict__ st_A,
                                                  const double* __restrict__ st_Ainv, const int32_t* __restrict__ aff,
@@ -203,7 +206,7 @@ __global__ void __launch_bounds__(256) k_wb_prep(const float* __restrict__ p_orb
                                                  double* __restrict__ sinv, double* __restrict__ ldS,
                                                  double* __restrict__ sgS) {
   extern __shared__ unsigned char smem_raw[];
-  double* P = reinterpret_cast<double*>(smem_raw);  // [N][amax]
+  double* P = reinterpret_cast<double*>(smem_raw);  // [N][amax]  Ainv[:, aff]
   double* S = P + (size_t)N * amax;                  // [amax][amax]
   int* piv = reinterpret_cast<int*>(S + (size_t)amax * amax);
   __shared__ double s_val[4];
@@ -225,19 +228,23 @@ __global__ void __launch_bounds__(256) k_wb_prep(const float* __restrict__ p_orb
     const float env = (i == k) ? p_envk[(size_t)w * ND + d * N + o]
                                : st_env[(size_t)group_of(w, i, B, n_up, N - n_up) * ND + d * N + o];
     // product rounded in FP32 exactly as the full evaluation forms the Slater matrix
-    PH[(size_t)j * N + o] = (double)__fmul_rn(p_orb[(size_t)t * ND + d * N + o], env);
+    const double ph = (double)__fmul_rn(p_orb[(size_t)t * ND + d * N + o], env);
+    PH[(size_t)j * N + o] = ph - Aw[(size_t)i * N + o];  // V = Phi_new - A[aff] (A rows restored by the update)
     P[(size_t)o * amax + j] = Ai[(size_t)o * N + i];
   }
   __syncthreads();
-  // S = I + V P
+  // S = I + V P: one row j of V per wavefront pass, P from LDS
   for (int e = tid; e < a * a; e += blockDim.x) {
     const int j = e / a, l = e - j * a;
-    const int i = aff[(size_t)w * N + j];
-    const double* ph = PH + (size_t)j * N;
-    const double* ar = Aw + (size_t)i * N;
-    double acc = (j == l) ? 1.0 : 0.0;
-    for (int o = 0; o < N; ++o) acc += (ph[o] - ar[o]) * P[(size_t)o * amax + l];
-    S[j * amax + l] = acc;
+    const double* vr = PH + (size_t)j * N;
+    double acc0 = 0.0, acc1 = 0.0;
+    int o = 0;
+    for (; o + 2 <= N; o += 2) {
+      acc0 += vr[o] * P[(size_t)o * amax + l];
+      acc1 += vr[o + 1] * P[(size_t)(o + 1) * amax + l];
+    }
+    if (o < N) acc0 += vr[o] * P[(size_t)o * amax + l];
+    S[j * amax + l] = (j == l ? 1.0 : 0.0) + (acc0 + acc1);
   }
   __syncthreads();
   // Gauss-Jordan inverse of S with partial pivoting; log|det S| and sign
@@ -333,11 +340,86 @@ void launch_wb_prep(hipStream_t s, const float* p_orb, const float* p_envk, cons
 hipError_t wb_set_smem_limit(size_t prep, size_t upd);
 
 // Woodbury inverse update for accepted walkers (woodburry_update, utils.py:243):
-//   Ainv <- Ainv - (Ainv[:, aff] S^-1) ((Phi_new - A[aff]) Ainv);  then A[aff] <- Phi_new.
-// Dynamic LDS: X = Ainv[:, aff] S^-1 [N][amax] and Q = V Ainv [amax][N].
+//   Ainv <- Ainv - (Ainv[:, aff] S^-1) (V Ainv);  then A[aff] <- A[aff] + V,
+// with V = Phi_new - A[aff] as left in `phin` by k_wb_prep.
+//
+// Thread c owns column c of Ainv; q = (V Ainv)[:, c] is held in AMAX registers.  V^T and
+// X = Ainv[:, aff] S^-1 are staged per (walker, det) in global scratch (row o of both is
+// read by every thread of the block: broadcast loads served by the caches), so the kernel
+// needs only S^-1 in LDS and several blocks share a CU.  Ainv is streamed twice, coalesced.
+template <int AMAX>
+__global__ void __launch_bounds__(256) k_wb_update_cols(const int32_t* __restrict__ accept,
+                                                        const int32_t* __restrict__ aff, const int32_t* __restrict__ cu,
+                                                        const int32_t* __restrict__ cd, int N, int D, int amax,
+                                                        const double* __restrict__ vmat,
+                                                        const double* __restrict__ sinv, double* __restrict__ st_A,
+                                                        double* __restrict__ st_Ainv, double* __restrict__ scr) {
+  __shared__ double SL[AMAX * AMAX];
+  __shared__ int s_af[AMAX];
+  const int wd = blockIdx.x;
+  const int w = wd / D;
+  if (!accept[w]) return;
+  const int a = cu[w] + cd[w];
+  const int tid = threadIdx.x;
+  double* Ai = st_Ainv + (size_t)wd * N * N;
+  double* Aw = st_A + (size_t)wd * N * N;
+  const double* V = vmat + (size_t)wd * amax * N;
+  const double* SI = sinv + (size_t)wd * amax * amax;
+  double* VT = scr + (size_t)wd * 2 * N * AMAX;  // [N][AMAX]
+  double* X = VT + (size_t)N * AMAX;             // [N][AMAX]
+  for (int j = tid; j < a; j += blockDim.x) s_af[j] = aff[(size_t)w * N + j];
+  for (int e = tid; e < AMAX * AMAX; e += blockDim.x) {
+    const int l = e / AMAX, j = e - l * AMAX;
+    SL[e] = (l < a && j < a) ? SI[l * amax + j] : 0.0;
+  }
+  __syncthreads();
+  for (int e = tid; e < N * AMAX; e += blockDim.x) {
+    const int o = e / AMAX, j = e - o * AMAX;
+    VT[e] = j < a ? V[(size_t)j * N + o] : 0.0;
+  }
+  for (int o = tid; o < N; o += blockDim.x) {
+    double acc[AMAX];
+#pragma unroll
+    for (int j = 0; j < AMAX; ++j) acc[j] = 0.0;
+    const double* ar = Ai + (size_t)o * N;
+    for (int l = 0; l < a; ++l) {
+      const double p = ar[s_af[l]];
+#pragma unroll
+      for (int j = 0; j < AMAX; ++j) acc[j] += p * SL[l * AMAX + j];
+    }
+#pragma unroll
+    for (int j = 0; j < AMAX; ++j) X[(size_t)o * AMAX + j] = j < a ? acc[j] : 0.0;
+  }
+  __threadfence_block();
+  __syncthreads();
+  for (int c = tid; c < N; c += blockDim.x) {
+    double q[AMAX];
+#pragma unroll
+    for (int j = 0; j < AMAX; ++j) q[j] = 0.0;
+    for (int o = 0; o < N; ++o) {
+      const double x = Ai[(size_t)o * N + c];
+      const double* v = VT + (size_t)o * AMAX;
+#pragma unroll
+      for (int j = 0; j < AMAX; ++j) q[j] += v[j] * x;
+    }
+    for (int o = 0; o < N; ++o) {
+      const double* xr = X + (size_t)o * AMAX;
+      double acc = 0.0;
+#pragma unroll
+      for (int j = 0; j < AMAX; ++j) acc += xr[j] * q[j];
+      Ai[(size_t)o * N + c] -= acc;
+    }
+  }
+  for (int e = tid; e < a * N; e += blockDim.x) {
+    const int j = e / N, o = e - j * N;
+    Aw[(size_t)s_af[j] * N + o] += V[(size_t)j * N + o];
+  }
+}
+
+// Generic form (any number of changed rows): X and Q = V Ainv in dynamic LDS.
 __global__ void __launch_bounds__(256) k_wb_update(const int32_t* __restrict__ accept, const int32_t* __restrict__ aff,
                                                    const int32_t* __restrict__ cu, const int32_t* __restrict__ cd,
-                                                   int N, int D, int amax, const double* __restrict__ phin,
+                                                   int N, int D, int amax, const double* __restrict__ vmat,
                                                    const double* __restrict__ sinv, double* __restrict__ st_A,
                                                    double* __restrict__ st_Ainv) {
   extern __shared__ unsigned char smem_raw[];
@@ -350,7 +432,7 @@ __global__ void __launch_bounds__(256) k_wb_update(const int32_t* __restrict__ a
   const int tid = threadIdx.x;
   double* Ai = st_Ainv + (size_t)wd * N * N;
   double* Aw = st_A + (size_t)wd * N * N;
-  const double* PH = phin + (size_t)wd * amax * N;
+  const double* V = vmat + (size_t)wd * amax * N;
   const double* SI = sinv + (size_t)wd * amax * amax;
   const int32_t* af = aff + (size_t)w * N;
   for (int e = tid; e < N * a; e += blockDim.x) {
@@ -361,9 +443,8 @@ __global__ void __launch_bounds__(256) k_wb_update(const int32_t* __restrict__ a
   }
   for (int e = tid; e < a * N; e += blockDim.x) {
     const int j = e / N, c = e - j * N;
-    const int i = af[j];
     double acc = 0.0;
-    for (int o = 0; o < N; ++o) acc += (PH[(size_t)j * N + o] - Aw[(size_t)i * N + o]) * Ai[(size_t)o * N + c];
+    for (int o = 0; o < N; ++o) acc += V[(size_t)j * N + o] * Ai[(size_t)o * N + c];
     Q[(size_t)j * N + c] = acc;
   }
   __syncthreads();
@@ -375,21 +456,33 @@ __global__ void __launch_bounds__(256) k_wb_update(const int32_t* __restrict__ a
   }
   for (int e = tid; e < a * N; e += blockDim.x) {
     const int j = e / N, o = e - j * N;
-    Aw[(size_t)af[j] * N + o] = PH[(size_t)j * N + o];
+    Aw[(size_t)af[j] * N + o] += V[(size_t)j * N + o];
   }
 }
 
-size_t wb_update_smem(int N, int amax) { return (size_t)2 * N * amax * 8; }
+size_t wb_update_smem(int N, int amax) { return amax <= 64 ? 0 : (size_t)2 * N * amax * 8; }
+size_t wb_update_scratch(int N, int amax) { return amax <= 64 ? (size_t)2 * N * 64 * 8 : 0; }
 
 void launch_wb_update(hipStream_t s, const int32_t* accept, const int32_t* aff, const int32_t* cu, const int32_t* cd,
-                      int B, int N, int D, int amax, const double* phin, const double* sinv, double* st_A,
-                      double* st_Ainv) {
-  k_wb_update<<<B * D, 256, wb_update_smem(N, amax), s>>>(accept, aff, cu, cd, N, D, amax, phin, sinv, st_A, st_Ainv);
+                      int B, int N, int D, int amax, const double* vmat, const double* sinv, double* st_A,
+                      double* st_Ainv, double* scratch) {
+  const int threads = std::min(256, ((N + 63) / 64) * 64);
+  if (amax <= 16)
+    k_wb_update_cols<16><<<B * D, threads, 0, s>>>(accept, aff, cu, cd, N, D, amax, vmat, sinv, st_A, st_Ainv, scratch);
+  else if (amax <= 32)
+    k_wb_update_cols<32><<<B * D, threads, 0, s>>>(accept, aff, cu, cd, N, D, amax, vmat, sinv, st_A, st_Ainv, scratch);
+  else if (amax <= 48)
+    k_wb_update_cols<48><<<B * D, threads, 0, s>>>(accept, aff, cu, cd, N, D, amax, vmat, sinv, st_A, st_Ainv, scratch);
+  else if (amax <= 64)
+    k_wb_update_cols<64><<<B * D, threads, 0, s>>>(accept, aff, cu, cd, N, D, amax, vmat, sinv, st_A, st_Ainv, scratch);
+  else
+    k_wb_update<<<B * D, 256, wb_update_smem(N, amax), s>>>(accept, aff, cu, cd, N, D, amax, vmat, sinv, st_A,
+                                                            st_Ainv);
 }
 
 hipError_t wb_set_smem_limit(size_t prep, size_t upd) {
   hipError_t e = hipFuncSetAttribute((const void*)k_wb_prep, hipFuncAttributeMaxDynamicSharedMemorySize, (int)prep);
-  if (e != hipSuccess) return e;
+  if (e != hipSuccess || upd == 0) return e;
   return hipFuncSetAttribute((const void*)k_wb_update, hipFuncAttributeMaxDynamicSharedMemorySize, (int)upd);
 }
 
