@@ -203,6 +203,15 @@ def main():
         "whole_step_tflops": flops_w * wps_m / 1e12,
     }
 
+    # tile shapes the autotuner chose for the BF16 contractions of this run
+    cap = 256
+    table = (ctypes.c_int32 * (7 * cap))()
+    n_tab = ctypes.c_int32()
+    _lib.check(wf._lib.fire_autotune(wf._h, -1, table, cap, ctypes.byref(n_tab)))
+    roofline["autotuned_tiles"] = [
+        {"split": table[7 * i], "N": table[7 * i + 1], "K": table[7 * i + 2], "log2_M": table[7 * i + 3],
+         "tile": f"{table[7 * i + 5]}x{table[7 * i + 6]}"} for i in range(min(n_tab.value, cap))]
+
     e_m = out_m["e_loc"].cpu().numpy()
     e_f = out_f["e_loc"].cpu().numpy()
     e_x = out_x["e_loc"].cpu().numpy()

@@ -146,6 +146,15 @@ int fire_state_get(fire_handle* h, void* stream, float* r_dev, double* logpsi_de
 int fire_state_stats(fire_handle* h, int32_t* n_walkers, int64_t* n_proposals, int64_t* sum_changed,
                      int64_t* max_changed);
 
+/* Tile-shape autotuner of the dense contractions (no counterpart in the reference; the
+ * reference leaves kernel selection to XLA).  On by default: the first BF16 contraction of
+ * each (split, N, K, floor(log2 M)) shape times every tile shape on its real operands
+ * (synchronising) and keeps the fastest for the life of the handle.  enable: 1 on, 0 off
+ * (tile 128 x 128), 2 on and forget earlier choices, -1 unchanged.  table (optional, int32
+ * [cap][7]) receives the current choices as (split, N, K, log2 M, variant, BM, BN);
+ * *n_entries the number of choices. */
+int fire_autotune(fire_handle* h, int32_t enable, int32_t* table, int32_t cap, int32_t* n_entries);
+
 /* Instrumentation (no counterpart in the reference): enables / disables HIP-event timing of
  * every dense-contraction launch of this handle.  Each call first reports what was recorded
  * since the previous call -- total kernel milliseconds, algorithmic FLOPs (2*M*N*K per

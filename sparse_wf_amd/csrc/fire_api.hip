@@ -7,6 +7,8 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <map>
+#include <tuple>
 #include <vector>
 
 #include "../../include/fire.h"
@@ -101,6 +103,11 @@ struct fire_handle {
   bool gemm_timing = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> gemm_events;
   double gemm_flops = 0.0;
+  // Autotuned tile shape of the BF16 contraction per (split, N, K, floor(log2 M)): the first
+  // call of a shape times every candidate on the real operands and keeps the fastest.
+  bool autotune = true;
+  std::map<std::tuple<int, int, int, int>, int> gemm_tile;
+  std::map<std::tuple<int, int, int, int>, std::vector<float>> gemm_tile_ms;
   // Resident walker state for single-electron low-rank updates (fire_state_*, fire_mcmc_steps).
   // Per-electron rows are in group order over all B walkers (up-spin electrons first).
   struct State {
@@ -231,15 +238,49 @@ void gemm(Ctx& c, const float* A, int64_t lda, int wi, int bi, float* C, int64_t
   fire_handle* h = c.h;
   const float* bias = bi >= 0 ? h->P[bi] : nullptr;
   const bool bf_ok = h->Pbf[wi] != nullptr && (K % 32) == 0 && (lda % 4) == 0 && N > 16;
+  const bool use_bf = (c.prec == FIRE_MIXED || c.prec == FIRE_MIXED_FAST || c.prec == FIRE_BF16) && bf_ok;
+  const int ns = c.prec == FIRE_MIXED ? 6 : (c.prec == FIRE_MIXED_FAST ? 3 : 1);
+  int variant = 0;
+  if (use_bf && h->autotune) {
+    int lg = 0;
+    while ((int64_t(1) << (lg + 1)) <= M) ++lg;
+    const auto key = std::make_tuple(ns, N, K, lg);
+    auto it = h->gemm_tile.find(key);
+    if (it != h->gemm_tile.end()) {
+      variant = it->second;
+    } else {
+      // time every tile shape twice on these operands (the second launch counts)
+      const int nv = gemm_bf16_variants();
+      std::vector<float> ms(nv, 0.f);
+      hipEvent_t t0, t1;
+      HIPCHK(hipEventCreate(&t0));
+      HIPCHK(hipEventCreate(&t1));
+      for (int v = 0; v < nv; ++v) {
+        launch_gemm_bf16(c.s, A, lda, h->Pbf[wi], h->Pbf_lo[wi], h->Pbf_lo2[wi], bias, C, ldc, M, N, K, rowflag,
+                         period, ns, v);
+        HIPCHK(hipEventRecord(t0, c.s));
+        launch_gemm_bf16(c.s, A, lda, h->Pbf[wi], h->Pbf_lo[wi], h->Pbf_lo2[wi], bias, C, ldc, M, N, K, rowflag,
+                         period, ns, v);
+        HIPCHK(hipEventRecord(t1, c.s));
+        HIPCHK(hipEventSynchronize(t1));
+        HIPCHK(hipEventElapsedTime(&ms[v], t0, t1));
+        if (ms[v] < ms[variant]) variant = v;
+      }
+      HIPCHK(hipEventDestroy(t0));
+      HIPCHK(hipEventDestroy(t1));
+      h->gemm_tile[key] = variant;
+      h->gemm_tile_ms[key] = ms;
+    }
+  }
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (h->gemm_timing) {
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
     HIPCHK(hipEventRecord(e0, c.s));
   }
-  if ((c.prec == FIRE_MIXED || c.prec == FIRE_MIXED_FAST || c.prec == FIRE_BF16) && bf_ok)
-    launch_gemm_bf16(c.s, A, lda, h->Pbf[wi], h->Pbf_lo[wi], h->Pbf_lo2[wi], bias, C, ldc, M, N, K, rowflag, period,
-                     c.prec == FIRE_MIXED ? 6 : (c.prec == FIRE_MIXED_FAST ? 3 : 1));
+  if (use_bf)
+    launch_gemm_bf16(c.s, A, lda, h->Pbf[wi], h->Pbf_lo[wi], h->Pbf_lo2[wi], bias, C, ldc, M, N, K, rowflag, period, ns,
+                     variant);
   else
     launch_gemm_f32(c.s, A, lda, h->P[wi], h->Pt32[wi], bias, C, ldc, M, N, K, rowflag, period);
   if (h->gemm_timing) {
@@ -1303,6 +1344,37 @@ int fire_state_get(fire_handle* h, void* stream, float* r_dev, double* logpsi_de
     if (logpsi_dev)
       HIPCHK(hipMemcpyAsync(logpsi_dev, h->st.logpsi.p, sizeof(double) * B, hipMemcpyDeviceToDevice, c.s));
     if (sign_dev) HIPCHK(hipMemcpyAsync(sign_dev, h->st.sign.p, sizeof(double) * B, hipMemcpyDeviceToDevice, c.s));
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e);
+  }
+}
+
+int fire_autotune(fire_handle* h, int32_t enable, int32_t* table, int32_t cap, int32_t* n_entries) {
+  try {
+    REQUIRE(h, "null handle");
+    int n = 0;
+    for (auto& kv : h->gemm_tile) {
+      if (table && n < cap) {
+        int bm, bn;
+        gemm_bf16_tile(kv.second, &bm, &bn);
+        int32_t* e = table + 7 * n;
+        e[0] = std::get<0>(kv.first);
+        e[1] = std::get<1>(kv.first);
+        e[2] = std::get<2>(kv.first);
+        e[3] = std::get<3>(kv.first);
+        e[4] = kv.second;
+        e[5] = bm;
+        e[6] = bn;
+      }
+      ++n;
+    }
+    if (n_entries) *n_entries = n;
+    if (enable >= 0) h->autotune = enable != 0;
+    if (enable == 2) {
+      h->gemm_tile.clear();
+      h->gemm_tile_ms.clear();
+    }
     return 0;
   } catch (const std::exception& e) {
     return fail(e);

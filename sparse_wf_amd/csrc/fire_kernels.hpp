@@ -58,8 +58,11 @@ void launch_rowflag(hipStream_t s, int G, GroupDesc gd, int64_t n_rows, uint8_t*
 void launch_gemm_f32(hipStream_t s, const float* A, int64_t lda, const float* W, const float* Wt, const float* bias,
                      float* C, int64_t ldc, int64_t M, int N, int K, const uint8_t* rowflag, int period);
 // nsplit 1: plain BF16; 3: W = hi + lo planes, 3 products; 6: W = hi + mid + lo, 6 products
+// variant: tile shape index in [0, gemm_bf16_variants()) (0 = 128 x 128)
 void launch_gemm_bf16(hipStream_t s, const float* A, int64_t lda, const uint16_t* Whi /*[N,K] bf16*/,
                       const uint16_t* Wlo, const uint16_t* Wlo2, const float* bias, float* C, int64_t ldc, int64_t M,
-                      int N, int K, const uint8_t* rowflag, int period, int nsplit);
+                      int N, int K, const uint8_t* rowflag, int period, int nsplit, int variant = 0);
+int gemm_bf16_variants();
+void gemm_bf16_tile(int variant, int* bm, int* bn);
 
 }  // namespace fire

@@ -154,65 +154,66 @@ __global__ void __launch_bounds__(256) k_gemm_f32(const float* __restrict__ A, i
 }
 
 // ---------------------------------------------------------------------------------------
-// BF16 / split-BF16 MFMA GEMM, FP32 accumulation.  Whi/Wlo are [N,K] BF16 (W transposed).
-// Requires K % 32 == 0 and lda % 4 == 0 (checked by the host).
+// BF16 / split-BF16 MFMA GEMM, FP32 accumulation.  W planes are [N,K] BF16 (W transposed):
+// NP = 1 plain BF16; NP = 2 hi + lo with the products hi*hi + hi*lo + lo*hi; NP = 3
+// hi + mid + lo with the six products of order >= 2^-18 relative (smallest first).  A is
+// split the same way while it is staged into LDS.  Output tile BM x BN, one 64 x 64
+// quarter (2 x 2 MFMA 32x32x16 tiles) per wavefront; K staged 32 at a time with the next
+// chunk prefetched into registers.  Requires K % 32 == 0 and lda % 4 == 0.
 // ---------------------------------------------------------------------------------------
-template <int NSPLIT, bool SEPACC>
-__global__ void __launch_bounds__(256) k_gemm_bf16(const float* __restrict__ A, int64_t lda,
-                                                   const uint16_t* __restrict__ Whi, const uint16_t* __restrict__ Wlo,
-                                                   const uint16_t* __restrict__ Wlo2, const float* __restrict__ bias,
-                                                   float* __restrict__ C, int64_t ldc, int64_t M, int N, int K,
-                                                   const uint8_t* __restrict__ rowflag, int period, int nT) {
-  constexpr int NP = NSPLIT == 6 ? 3 : (NSPLIT == 3 ? 2 : 1);  // hi (+ mid/lo) planes
-  __shared__ __attribute__((aligned(16))) uint16_t As[NP][GT_M][LDB_H];
-  __shared__ __attribute__((aligned(16))) uint16_t Bs[NP][GT_N][LDB_H];
+template <int NP, int BM, int BN>
+__global__ void __launch_bounds__(BM * BN / 64) k_gemm_bf16(const float* __restrict__ A, int64_t lda,
+                                                           const uint16_t* __restrict__ Wp0,
+                                                           const uint16_t* __restrict__ Wp1,
+                                                           const uint16_t* __restrict__ Wp2,
+                                                           const float* __restrict__ bias, float* __restrict__ C,
+                                                           int64_t ldc, int64_t M, int N, int K,
+                                                           const uint8_t* __restrict__ rowflag, int period, int nT) {
+  constexpr int NWN = BN / 64;
+  constexpr int NT = (BM / 64) * NWN * 64;  // threads
+  constexpr int A_PER = BM * 8 / NT;        // float4 of the A chunk per thread
+  constexpr int B_PER = BN * 4 / NT;        // uint4 of each W plane chunk per thread
+  static_assert(A_PER >= 1 && B_PER >= 1, "tile too small for its thread count");
+  extern __shared__ __attribute__((aligned(16))) unsigned char gemm_lds[];
+  uint16_t(*As)[BM][LDB_H] = reinterpret_cast<uint16_t(*)[BM][LDB_H]>(gemm_lds);
+  uint16_t(*Bs)[BN][LDB_H] = reinterpret_cast<uint16_t(*)[BN][LDB_H]>(gemm_lds + (size_t)NP * BM * LDB_H * 2);
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   int64_t mt;
   int nt;
   tile_of(mt, nt, nT);
-  const int64_t m0 = mt * GT_M;
-  const int n0 = nt * GT_N;
-  const int wm = (wid >> 1) * 64, wn = (wid & 1) * 64;
-  // acc: hi*hi products; with SEPACC, accs takes the cross products of the split planes so
-  // that they are not rounded against the (2^8 times larger) running hi*hi sum (about 3x
-  // smaller error on ill-conditioned rows, at twice the accumulator registers)
-  constexpr bool SEP = SEPACC && NP >= 2;
-  float16_t acc[2][2], accs[2][SEP ? 2 : 1];
+  const int64_t m0 = mt * BM;
+  const int n0 = nt * BN;
+  const int wm = (wid / NWN) * 64, wn = (wid % NWN) * 64;
+  float16_t acc[2][2];
   for (int a = 0; a < 2; ++a)
     for (int b = 0; b < 2; ++b)
-      for (int v = 0; v < 16; ++v) {
-        acc[a][b][v] = 0.f;
-        if (SEP) accs[a][SEP ? b : 0][v] = 0.f;
-      }
+      for (int v = 0; v < 16; ++v) acc[a][b][v] = 0.f;
+  const uint16_t* Wpl[3] = {Wp0, Wp1, Wp2};
 
-  // per-thread load slots: A 128 rows x 32 k = 1024 float4 -> 4 per thread;
-  //                        W 128 n x 32 k = 512 x (8 bf16) -> 2 per thread per plane
-  float4 ra[4];
-  uint4 rb[NP][2];
+  float4 ra[A_PER];
+  uint4 rb[NP][B_PER];
   auto load = [&](int k0) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int e = tid + q * 256;
+    for (int q = 0; q < A_PER; ++q) {
+      const int e = tid + q * NT;
       const int rr = e >> 3, kq = (e & 7) * 4;
       const int64_t gr = m0 + rr;
       ra[q] = gr < M ? *reinterpret_cast<const float4*>(A + gr * lda + k0 + kq) : make_float4(0.f, 0.f, 0.f, 0.f);
     }
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
-      const int e = tid + q * 256;
+    for (int q = 0; q < B_PER; ++q) {
+      const int e = tid + q * NT;
       const int nn = e >> 2, kq = (e & 3) * 8;
       const int gn = n0 + nn;
-      rb[0][q] = gn < N ? *reinterpret_cast<const uint4*>(Whi + (int64_t)gn * K + k0 + kq) : make_uint4(0, 0, 0, 0);
-      if (NP >= 2)
-        rb[1][q] = gn < N ? *reinterpret_cast<const uint4*>(Wlo + (int64_t)gn * K + k0 + kq) : make_uint4(0, 0, 0, 0);
-      if (NP == 3)
-        rb[2][q] = gn < N ? *reinterpret_cast<const uint4*>(Wlo2 + (int64_t)gn * K + k0 + kq) : make_uint4(0, 0, 0, 0);
+#pragma unroll
+      for (int p = 0; p < NP; ++p)
+        rb[p][q] = gn < N ? *reinterpret_cast<const uint4*>(Wpl[p] + (int64_t)gn * K + k0 + kq) : make_uint4(0, 0, 0, 0);
     }
   };
   auto store = [&]() {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int e = tid + q * 256;
+    for (int q = 0; q < A_PER; ++q) {
+      const int e = tid + q * NT;
       const int rr = e >> 3, kq = (e & 7) * 4;
       const float x[4] = {ra[q].x, ra[q].y, ra[q].z, ra[q].w};
       uint16_t pl[3][4];
@@ -230,8 +231,8 @@ __global__ void __launch_bounds__(256) k_gemm_bf16(const float* __restrict__ A, 
                                                                (uint32_t)pl[p][2] | ((uint32_t)pl[p][3] << 16));
     }
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
-      const int e = tid + q * 256;
+    for (int q = 0; q < B_PER; ++q) {
+      const int e = tid + q * NT;
       const int nn = e >> 2, kq = (e & 3) * 8;
 #pragma unroll
       for (int p = 0; p < NP; ++p) *reinterpret_cast<uint4*>(&Bs[p][nn][kq]) = rb[p][q];
@@ -258,25 +259,20 @@ __global__ void __launch_bounds__(256) k_gemm_bf16(const float* __restrict__ A, 
       for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b) {
-          // smallest products first
-          float16_t& as = SEP ? accs[a][SEP ? b : 0] : acc[a][b];
           if (NP == 3) {
-            as = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[1][a], bv[1][b], as, 0, 0, 0);
-            as = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[2][a], bv[0][b], as, 0, 0, 0);
-            as = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[0][a], bv[2][b], as, 0, 0, 0);
+            acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[1][a], bv[1][b], acc[a][b], 0, 0, 0);
+            acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[2][a], bv[0][b], acc[a][b], 0, 0, 0);
+            acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[0][a], bv[2][b], acc[a][b], 0, 0, 0);
           }
           if (NP >= 2) {
-            as = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[1][a], bv[0][b], as, 0, 0, 0);
-            as = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[0][a], bv[1][b], as, 0, 0, 0);
+            acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[1][a], bv[0][b], acc[a][b], 0, 0, 0);
+            acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[0][a], bv[1][b], acc[a][b], 0, 0, 0);
           }
           acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[0][a], bv[0][b], acc[a][b], 0, 0, 0);
         }
     }
     __syncthreads();
   }
-  if (SEP)
-    for (int a = 0; a < 2; ++a)
-      for (int b = 0; b < 2; ++b) acc[a][b] += accs[a][SEP ? b : 0];
   store_tile(acc, m0, n0, wm, wn, lane, C, ldc, M, N, bias, rowflag, period);
 }
 
@@ -320,20 +316,62 @@ void launch_gemm_f32(hipStream_t s, const float* A, int64_t lda, const float* W,
     k_gemm_f32<false><<<(unsigned)blocks, 256, 0, s>>>(A, lda, Wt, bias, C, ldc, M, N, K, rowflag, period, nT);
 }
 
+// Tile shapes offered to the autotuner (index = variant).
+struct Bf16Tile {
+  int bm, bn;
+};
+static const Bf16Tile kBf16Tiles[] = {{128, 128}, {128, 256}, {256, 128}, {64, 256}, {256, 64}, {64, 128}, {128, 64}};
+
+int gemm_bf16_variants() { return (int)(sizeof(kBf16Tiles) / sizeof(kBf16Tiles[0])); }
+void gemm_bf16_tile(int variant, int* bm, int* bn) {
+  const Bf16Tile t = kBf16Tiles[(variant >= 0 && variant < gemm_bf16_variants()) ? variant : 0];
+  *bm = t.bm;
+  *bn = t.bn;
+}
+
+template <int NP, int BM, int BN>
+static void launch_tile(hipStream_t s, const float* A, int64_t lda, const uint16_t* const* W, const float* bias,
+                        float* C, int64_t ldc, int64_t M, int N, int K, const uint8_t* rowflag, int period) {
+  static bool attr_set = false;  // LDS above 64 KB must be enabled once per kernel
+  const size_t lds = (size_t)NP * (BM + BN) * LDB_H * 2;
+  if (!attr_set) {
+    if (lds > 64 * 1024)
+      (void)hipFuncSetAttribute((const void*)k_gemm_bf16<NP, BM, BN>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds);
+    attr_set = true;
+  }
+  const int nT = (N + BN - 1) / BN;
+  const int64_t blocks = ((M + BM - 1) / BM) * nT;
+  k_gemm_bf16<NP, BM, BN><<<(unsigned)blocks, BM * BN / 64, lds, s>>>(A, lda, W[0], W[1], W[2], bias, C, ldc, M, N, K,
+                                                                     rowflag, period, nT);
+}
+
+template <int NP>
+static void launch_np(hipStream_t s, int variant, const float* A, int64_t lda, const uint16_t* const* W,
+                      const float* bias, float* C, int64_t ldc, int64_t M, int N, int K, const uint8_t* rowflag,
+                      int period) {
+  switch (variant) {
+    case 1: launch_tile<NP, 128, 256>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period); break;
+    case 2: launch_tile<NP, 256, 128>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period); break;
+    case 3: launch_tile<NP, 64, 256>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period); break;
+    case 4: launch_tile<NP, 256, 64>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period); break;
+    case 5: launch_tile<NP, 64, 128>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period); break;
+    case 6: launch_tile<NP, 128, 64>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period); break;
+    default: launch_tile<NP, 128, 128>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period); break;
+  }
+}
+
 void launch_gemm_bf16(hipStream_t s, const float* A, int64_t lda, const uint16_t* Whi, const uint16_t* Wlo,
                       const uint16_t* Wlo2, const float* bias, float* C, int64_t ldc, int64_t M, int N, int K,
-                      const uint8_t* rowflag, int period, int nsplit) {
+                      const uint8_t* rowflag, int period, int nsplit, int variant) {
   if (M == 0) return;
-  const int nT = (N + GT_N - 1) / GT_N;
-  const int64_t blocks = ((M + GT_M - 1) / GT_M) * nT;
+  const uint16_t* W[3] = {Whi, Wlo ? Wlo : Whi, Wlo2 ? Wlo2 : Whi};
   if (nsplit == 6)
-    k_gemm_bf16<6, false><<<(unsigned)blocks, 256, 0, s>>>(A, lda, Whi, Wlo, Wlo2, bias, C, ldc, M, N, K, rowflag, period,
-                                                    nT);
+    launch_np<3>(s, variant, A, lda, W, bias, C, ldc, M, N, K, rowflag, period);
   else if (nsplit == 3)
-    k_gemm_bf16<3, false><<<(unsigned)blocks, 256, 0, s>>>(A, lda, Whi, Wlo, Wlo, bias, C, ldc, M, N, K, rowflag, period, nT);
+    launch_np<2>(s, variant, A, lda, W, bias, C, ldc, M, N, K, rowflag, period);
   else
-    k_gemm_bf16<1, false><<<(unsigned)blocks, 256, 0, s>>>(A, lda, Whi, Whi, Whi, bias, C, ldc, M, N, K, rowflag, period,
-                                                    nT);
+    launch_np<1>(s, variant, A, lda, W, bias, C, ldc, M, N, K, rowflag, period);
 }
 
 }  // namespace fire
