@@ -77,9 +77,14 @@ def main():
     ap.add_argument("--walkers", type=int, default=None)
     ap.add_argument("--sweeps", type=int, default=50, help="Metropolis sweeps (n_el single-electron steps each)")
     ap.add_argument("--width", type=float, default=0.3, help="Gaussian step (bohr)")
-    ap.add_argument("--cpu-sample", type=int, default=64, help="walkers timed on the CPU oracle (FP32)")
+    ap.add_argument("--cpu-sample", type=int, default=192, help="walkers timed on the CPU oracle (FP32)")
     ap.add_argument("--bias-sample", type=int, default=64, help="walkers checked against the FP64 oracle")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=16, help="requests of the serving leg (0: skip)")
+    ap.add_argument("--serve-qps", type=float, default=4.0, help="Poisson arrival rate of the serving leg")
+    ap.add_argument("--serve-nel", default="20,400", help="electron-count range of the serving requests")
+    ap.add_argument("--serve-walkers", default="128,1024", help="walker range of the serving requests")
+    ap.add_argument("--serve-sweeps", default="1,2", help="sweep range of the serving requests")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device copy of this many MiB (calibrates HBM byte counters) and exit")
     args = ap.parse_args()
@@ -243,6 +248,31 @@ def main():
         "roofline": roofline,
         "mcmc": mcmc,
     }
+
+    if args.serve_requests > 0:
+        # serving leg: seeded Poisson stream of random H/C chains, continuous batching,
+        # the same stream once in mixed precision and once in FP32
+        from sparse_wf_amd.serving import Server, latency_summary, poisson_requests
+
+        rng2 = lambda s: tuple(int(x) for x in s.split(","))
+        serving = {"workload": f"{args.serve_requests} requests, Poisson {args.serve_qps}/s, random H/C chains "
+                               f"N={args.serve_nel}, walkers={args.serve_walkers}, sweeps={args.serve_sweeps}, "
+                               "F=256, D=4, cutoff 5; result = local energy after the sweeps"}
+        for prec in ("mixed", "fp32"):
+            log(f"serving leg ({prec}): preparing {args.serve_requests} requests")
+            reqs = poisson_requests(args.serve_requests, args.serve_qps, seed=5, n_el=rng2(args.serve_nel),
+                                    walkers=rng2(args.serve_walkers), sweeps=rng2(args.serve_sweeps),
+                                    precision=prec)
+            srv = Server(device=0)
+            t0 = time.perf_counter()
+            res = srv.run(reqs)
+            log(f"serving leg ({prec}): {time.perf_counter() - t0:.1f} s")
+            s = latency_summary(res)
+            s["max_concurrent"] = srv.max_concurrent
+            serving[prec] = s
+        result["serving"] = serving
+        result["latency_ms"]["serving_p50"] = 1e3 * serving["mixed"]["p50_s"]
+        result["latency_ms"]["serving_p99"] = 1e3 * serving["mixed"]["p99_s"]
 
     if not args.no_cpu:
         # CPU baseline leg: the oracle (FP32) timed on a bounded sample of the same walkers,

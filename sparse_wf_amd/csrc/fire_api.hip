@@ -893,6 +893,17 @@ int fail(const std::exception& e) {
   return 1;
 }
 
+// Walkers per chunk of the forward-Laplacian path: up to 256, fewer when the tangent-row
+// workspace of one walker (about 5N + 3N * min(N - 1, 40) rows of the widest contraction)
+// would put a single buffer above 12 GB.
+int tangent_chunk(const fire_handle* h) {
+  const int64_t N = h->N;
+  const int64_t rows = 5 * N + 3 * N * std::min<int64_t>(N - 1, 40);
+  const int64_t width = std::max<int64_t>(std::max<int64_t>(h->F, h->ND), std::max<int64_t>(h->W, 2 * h->HD));
+  const int64_t per_walker = rows * width * 4;
+  return (int)std::max<int64_t>(8, std::min<int64_t>(h->chunk, (int64_t)12000000000LL / per_walker));
+}
+
 bool valid_precision(int p) { return p == FIRE_FP32 || p == FIRE_MIXED || p == FIRE_BF16 || p == FIRE_MIXED_FAST; }
 
 void check_ready(fire_handle* h) {
@@ -1162,8 +1173,9 @@ int fire_local_energy(fire_handle* h, void* stream, const float* r_dev, int32_t 
     HIPCHK(hipSetDevice(h->device));
     Ctx c{h, (hipStream_t)stream, precision};
     const int N = h->N;
-    for (int w0 = 0; w0 < B; w0 += h->chunk) {
-      const int Bc = std::min(h->chunk, B - w0);
+    const int chunk = tangent_chunk(h);
+    for (int w0 = 0; w0 < B; w0 += chunk) {
+      const int Bc = std::min(chunk, B - w0);
       const float* r = r_dev + (size_t)w0 * N * 3;
       Graph g = build_graph(c, r, Bc);
       GroupDesc gd;

@@ -195,8 +195,9 @@ void launch_mc_klist(hipStream_t s, int B, int k, int2* list) { k_mc_klist<<<(B 
 
 // Woodbury preparation per (walker, det): new Slater rows Phi_new of the affected
 // electrons, S = I + V Ainv[:, aff] with V = Phi_new - A[aff], log|det S|, sign, S^-1.
-// Writes V = Phi_new - A[aff] (not Phi_new) to `phin`.  Dynamic LDS: P = Ainv[:, aff] [N][amax],
-// S [amax][amax] and pivots.
+// Writes V = Phi_new - A[aff] (not Phi_new) to `phin`.  Dynamic LDS: S [amax][amax], a pivot
+// column, pivots and (PLDS) P = Ainv[:, aff] [N][amax]; without PLDS, P is read from Ainv.
+template <bool PLDS>
 __global__ void __launch_bounds__(256) k_wb_prep(const float* __restrict__ p_orb, const float* __restrict__ p_envk,
                                                  const float* __restrict__ st_env, const double* __restrict__ st_A,
                                                  const double* __restrict__ st_Ainv, const int32_t* __restrict__ aff,
@@ -206,9 +207,11 @@ __global__ void __launch_bounds__(256) k_wb_prep(const float* __restrict__ p_orb
                                                  double* __restrict__ sinv, double* __restrict__ ldS,
                                                  double* __restrict__ sgS) {
   extern __shared__ unsigned char smem_raw[];
-  double* P = reinterpret_cast<double*>(smem_raw);  // [N][amax]  Ainv[:, aff]
-  double* S = P + (size_t)N * amax;                  // [amax][amax]
-  int* piv = reinterpret_cast<int*>(S + (size_t)amax * amax);
+  double* S = reinterpret_cast<double*>(smem_raw);  // [amax][amax]
+  double* colc = S + (size_t)amax * amax;            // [amax]
+  int* piv = reinterpret_cast<int*>(colc + amax);    // [amax]
+  double* P = reinterpret_cast<double*>(smem_raw + (size_t)amax * amax * 8 + (size_t)amax * 8 +
+                                        (((size_t)amax * 4 + 7) & ~(size_t)7));  // [N][amax] (PLDS)
   __shared__ double s_val[4];
   __shared__ int s_idx[4];
   __shared__ int s_p;
@@ -230,20 +233,22 @@ __global__ void __launch_bounds__(256) k_wb_prep(const float* __restrict__ p_orb
     // product rounded in FP32 exactly as the full evaluation forms the Slater matrix
     const double ph = (double)__fmul_rn(p_orb[(size_t)t * ND + d * N + o], env);
     PH[(size_t)j * N + o] = ph - Aw[(size_t)i * N + o];  // V = Phi_new - A[aff] (A rows restored by the update)
-    P[(size_t)o * amax + j] = Ai[(size_t)o * N + i];
+    if (PLDS) P[(size_t)o * amax + j] = Ai[(size_t)o * N + i];
   }
   __syncthreads();
   // S = I + V P: one row j of V per wavefront pass, P from LDS
   for (int e = tid; e < a * a; e += blockDim.x) {
     const int j = e / a, l = e - j * a;
     const double* vr = PH + (size_t)j * N;
+    const int cl = aff[(size_t)w * N + l];
+    auto pv = [&](int o) { return PLDS ? P[(size_t)o * amax + l] : Ai[(size_t)o * N + cl]; };
     double acc0 = 0.0, acc1 = 0.0;
     int o = 0;
     for (; o + 2 <= N; o += 2) {
-      acc0 += vr[o] * P[(size_t)o * amax + l];
-      acc1 += vr[o + 1] * P[(size_t)(o + 1) * amax + l];
+      acc0 += vr[o] * pv(o);
+      acc1 += vr[o + 1] * pv(o + 1);
     }
-    if (o < N) acc0 += vr[o] * P[(size_t)o * amax + l];
+    if (o < N) acc0 += vr[o] * pv(o);
     S[j * amax + l] = (j == l ? 1.0 : 0.0) + (acc0 + acc1);
   }
   __syncthreads();
@@ -290,7 +295,6 @@ __global__ void __launch_bounds__(256) k_wb_prep(const float* __restrict__ p_orb
     __syncthreads();
     for (int cc = tid; cc < a; cc += blockDim.x) S[c * amax + cc] = (cc == c) ? ip : S[c * amax + cc] * ip;
     __syncthreads();
-    double* colc = P;  // reuse: P is no longer needed after S was formed
     for (int rr = tid; rr < a; rr += blockDim.x) colc[rr] = S[rr * amax + c];
     __syncthreads();
     for (int e = tid; e < a * a; e += blockDim.x) {
@@ -325,16 +329,28 @@ __global__ void __launch_bounds__(256) k_wb_prep(const float* __restrict__ p_orb
   }
 }
 
+static size_t wb_prep_fixed(int amax) {
+  return (size_t)amax * amax * 8 + (size_t)amax * 8 + (((size_t)amax * 4 + 7) & ~(size_t)7);
+}
+static const size_t kLdsCap = 160 * 1024 - 512;  // dynamic LDS usable next to the kernels' static LDS
+
+// Dynamic LDS of the preparation: with P staged when it fits, else without.
 size_t wb_prep_smem(int N, int amax) {
-  return (size_t)N * amax * 8 + (size_t)amax * amax * 8 + (size_t)amax * 4 + 8;
+  const size_t with_p = wb_prep_fixed(amax) + (size_t)N * amax * 8;
+  return with_p <= kLdsCap ? with_p : wb_prep_fixed(amax);
 }
 
 void launch_wb_prep(hipStream_t s, const float* p_orb, const float* p_envk, const float* st_env, const double* st_A,
                     const double* st_Ainv, const int32_t* aff, const int32_t* cu, const int32_t* cd,
                     const int32_t* rowof, int B, int N, int n_up, int D, int k, int amax, double* phin, double* sinv,
                     double* ldS, double* sgS) {
-  k_wb_prep<<<B * D, 256, wb_prep_smem(N, amax), s>>>(p_orb, p_envk, st_env, st_A, st_Ainv, aff, cu, cd, rowof, B, N,
-                                                       n_up, D, k, amax, phin, sinv, ldS, sgS);
+  const size_t sm = wb_prep_smem(N, amax);
+  if (sm > wb_prep_fixed(amax))
+    k_wb_prep<true><<<B * D, 256, sm, s>>>(p_orb, p_envk, st_env, st_A, st_Ainv, aff, cu, cd, rowof, B, N, n_up, D, k,
+                                           amax, phin, sinv, ldS, sgS);
+  else
+    k_wb_prep<false><<<B * D, 256, sm, s>>>(p_orb, p_envk, st_env, st_A, st_Ainv, aff, cu, cd, rowof, B, N, n_up, D,
+                                            k, amax, phin, sinv, ldS, sgS);
 }
 
 hipError_t wb_set_smem_limit(size_t prep, size_t upd);
@@ -481,7 +497,9 @@ void launch_wb_update(hipStream_t s, const int32_t* accept, const int32_t* aff, 
 }
 
 hipError_t wb_set_smem_limit(size_t prep, size_t upd) {
-  hipError_t e = hipFuncSetAttribute((const void*)k_wb_prep, hipFuncAttributeMaxDynamicSharedMemorySize, (int)prep);
+  hipError_t e = hipFuncSetAttribute((const void*)k_wb_prep<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)prep);
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute((const void*)k_wb_prep<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)prep);
   if (e != hipSuccess || upd == 0) return e;
   return hipFuncSetAttribute((const void*)k_wb_update, hipFuncAttributeMaxDynamicSharedMemorySize, (int)upd);
 }
