@@ -23,6 +23,18 @@ namespace fire {
 
 typedef float float16_t __attribute__((ext_vector_type(16)));
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+typedef float float2v_t __attribute__((ext_vector_type(2)));
+
+// Split-plane product terms (plane of A, plane of B), smallest first: for NP = 3 the six
+// products of order >= 2^-18 relative, for NP = 2 the three of order >= 2^-9, for NP = 1 one.
+__host__ __device__ constexpr int n_products(int np) { return np == 3 ? 6 : (np == 2 ? 3 : 1); }
+__host__ __device__ constexpr int prod_a(int np, int t) {
+  return np == 3 ? (t == 0 ? 1 : t == 1 ? 2 : t == 2 ? 0 : t == 3 ? 1 : 0) : (np == 2 ? (t == 0 ? 1 : 0) : 0);
+}
+__host__ __device__ constexpr int prod_b(int np, int t) {
+  return np == 3 ? (t == 0 ? 1 : t == 1 ? 0 : t == 2 ? 2 : t == 3 ? 0 : t == 4 ? 1 : 0) : (np == 2 ? (t == 1 ? 1 : 0) : 0);
+}
 
 __device__ __forceinline__ uint16_t f2bf(float x) {
   uint32_t u = __float_as_uint(x);
@@ -190,16 +202,20 @@ __global__ void __launch_bounds__(BM * BN / 64) k_gemm_bf16(const float* __restr
       for (int v = 0; v < 16; ++v) acc[a][b][v] = 0.f;
   const uint16_t* Wpl[3] = {Wp0, Wp1, Wp2};
 
-  float4 ra[A_PER];
+  // A (HBM) is prefetched two chunks ahead through two register sets, the weight planes
+  // (L2-resident) one chunk ahead: about twice the A bytes in flight of a single buffer.
+  float4 ra0[A_PER], ra1[A_PER];
   uint4 rb[NP][B_PER];
-  auto load = [&](int k0) {
+  auto load_a = [&](float4(&dst)[A_PER], int k0) {
 #pragma unroll
     for (int q = 0; q < A_PER; ++q) {
       const int e = tid + q * NT;
       const int rr = e >> 3, kq = (e & 7) * 4;
       const int64_t gr = m0 + rr;
-      ra[q] = gr < M ? *reinterpret_cast<const float4*>(A + gr * lda + k0 + kq) : make_float4(0.f, 0.f, 0.f, 0.f);
+      dst[q] = gr < M ? *reinterpret_cast<const float4*>(A + gr * lda + k0 + kq) : make_float4(0.f, 0.f, 0.f, 0.f);
     }
+  };
+  auto load_b = [&](int k0) {
 #pragma unroll
     for (int q = 0; q < B_PER; ++q) {
       const int e = tid + q * NT;
@@ -210,25 +226,25 @@ __global__ void __launch_bounds__(BM * BN / 64) k_gemm_bf16(const float* __restr
         rb[p][q] = gn < N ? *reinterpret_cast<const uint4*>(Wpl[p] + (int64_t)gn * K + k0 + kq) : make_uint4(0, 0, 0, 0);
     }
   };
-  auto store = [&]() {
+  auto store = [&](const float4(&src)[A_PER]) {
 #pragma unroll
     for (int q = 0; q < A_PER; ++q) {
       const int e = tid + q * NT;
       const int rr = e >> 3, kq = (e & 7) * 4;
-      const float x[4] = {ra[q].x, ra[q].y, ra[q].z, ra[q].w};
-      uint16_t pl[3][4];
+      // x = hi + mid + lo with hardware round-to-nearest conversions (v_cvt_pk_bf16_f32);
+      // each remainder is exact in FP32
+      float2v_t x01 = {src[q].x, src[q].y}, x23 = {src[q].z, src[q].w};
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        // x = hi + mid + lo; each remainder is exact in FP32
-        pl[0][j] = f2bf(x[j]);
-        const float r1 = x[j] - bf2f(pl[0][j]);
-        pl[1][j] = f2bf(r1);
-        pl[2][j] = f2bf(r1 - bf2f(pl[1][j]));
+      for (int p = 0; p < NP; ++p) {
+        const bf16x2_t h01 = __builtin_convertvector(x01, bf16x2_t);
+        const bf16x2_t h23 = __builtin_convertvector(x23, bf16x2_t);
+        *reinterpret_cast<uint2*>(&As[p][rr][kq]) =
+            make_uint2(__builtin_bit_cast(uint32_t, h01), __builtin_bit_cast(uint32_t, h23));
+        if (p + 1 < NP) {
+          x01 = x01 - __builtin_convertvector(h01, float2v_t);
+          x23 = x23 - __builtin_convertvector(h23, float2v_t);
+        }
       }
-#pragma unroll
-      for (int p = 0; p < NP; ++p)
-        *reinterpret_cast<uint2*>(&As[p][rr][kq]) = make_uint2((uint32_t)pl[p][0] | ((uint32_t)pl[p][1] << 16),
-                                                               (uint32_t)pl[p][2] | ((uint32_t)pl[p][3] << 16));
     }
 #pragma unroll
     for (int q = 0; q < B_PER; ++q) {
@@ -238,12 +254,7 @@ __global__ void __launch_bounds__(BM * BN / 64) k_gemm_bf16(const float* __restr
       for (int p = 0; p < NP; ++p) *reinterpret_cast<uint4*>(&Bs[p][nn][kq]) = rb[p][q];
     }
   };
-
-  load(0);
-  for (int k0 = 0; k0 < K; k0 += GT_K) {
-    store();
-    __syncthreads();
-    if (k0 + GT_K < K) load(k0 + GT_K);
+  auto compute = [&]() {
 #pragma unroll
     for (int kk = 0; kk < GT_K; kk += 16) {
       const int ka = kk + 8 * (lane >> 5);
@@ -255,22 +266,35 @@ __global__ void __launch_bounds__(BM * BN / 64) k_gemm_bf16(const float* __restr
 #pragma unroll
         for (int b = 0; b < 2; ++b) bv[p][b] = *reinterpret_cast<const bf16x8_t*>(&Bs[p][wn + b * 32 + (lane & 31)][ka]);
       }
+      // product-major order: four independent accumulators between dependent MFMAs
 #pragma unroll
-      for (int a = 0; a < 2; ++a)
+      for (int t = 0; t < n_products(NP); ++t)
 #pragma unroll
-        for (int b = 0; b < 2; ++b) {
-          if (NP == 3) {
-            acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[1][a], bv[1][b], acc[a][b], 0, 0, 0);
-            acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[2][a], bv[0][b], acc[a][b], 0, 0, 0);
-            acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[0][a], bv[2][b], acc[a][b], 0, 0, 0);
-          }
-          if (NP >= 2) {
-            acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[1][a], bv[0][b], acc[a][b], 0, 0, 0);
-            acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[0][a], bv[1][b], acc[a][b], 0, 0, 0);
-          }
-          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[0][a], bv[0][b], acc[a][b], 0, 0, 0);
-        }
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int b = 0; b < 2; ++b)
+            acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[prod_a(NP, t)][a], bv[prod_b(NP, t)][b],
+                                                                acc[a][b], 0, 0, 0);
     }
+  };
+
+  const int nch = K / GT_K;
+  load_a(ra0, 0);
+  load_b(0);
+  if (nch > 1) load_a(ra1, GT_K);
+  for (int c = 0; c < nch; c += 2) {
+    store(ra0);  // chunk c
+    __syncthreads();
+    if (c + 2 < nch) load_a(ra0, (c + 2) * GT_K);
+    if (c + 1 < nch) load_b((c + 1) * GT_K);
+    compute();
+    __syncthreads();
+    if (c + 1 >= nch) break;
+    store(ra1);  // chunk c + 1
+    __syncthreads();
+    if (c + 3 < nch) load_a(ra1, (c + 3) * GT_K);
+    if (c + 2 < nch) load_b((c + 2) * GT_K);
+    compute();
     __syncthreads();
   }
   store_tile(acc, m0, n0, wm, wn, lane, C, ldc, M, N, bias, rowflag, period);

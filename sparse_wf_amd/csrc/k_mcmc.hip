@@ -297,14 +297,10 @@ __global__ void __launch_bounds__(256) k_wb_prep(const float* __restrict__ p_orb
     __syncthreads();
     for (int rr = tid; rr < a; rr += blockDim.x) colc[rr] = S[rr * amax + c];
     __syncthreads();
-    for (int e = tid; e < a * a; e += blockDim.x) {
-      const int i = e / a, j = e - i * a;
+    for (int i = wid; i < a; i += blockDim.x >> 6) {
       if (i == c) continue;
       const double fk = colc[i];
-      if (j == c)
-        S[i * amax + c] = -fk * ip;
-      else
-        S[i * amax + j] -= fk * S[c * amax + j];
+      for (int j = lane; j < a; j += 64) S[i * amax + j] = (j == c) ? -fk * ip : S[i * amax + j] - fk * S[c * amax + j];
     }
     __syncthreads();
   }

@@ -168,10 +168,10 @@ __global__ void __launch_bounds__(256) k_lu_inv(double* __restrict__ Aall, int N
     const double ipiv = 1.0 / piv;
     for (int rr = k + 1 + tid; rr < N; rr += blockDim.x) A[(size_t)rr * N + k] *= ipiv;
     __syncthreads();
-    const int nt = N - k - 1;
-    for (int e = tid; e < nt * nt; e += blockDim.x) {
-      const int rr = k + 1 + e / nt, c = k + 1 + e % nt;
-      A[(size_t)rr * N + c] -= A[(size_t)rr * N + k] * A[(size_t)k * N + c];
+    // trailing update: one row per wavefront pass, columns across lanes
+    for (int rr = k + 1 + wid; rr < N; rr += blockDim.x >> 6) {
+      const double l = A[(size_t)rr * N + k];
+      for (int c = k + 1 + lane; c < N; c += 64) A[(size_t)rr * N + c] -= l * A[(size_t)k * N + c];
     }
     __syncthreads();
   }
@@ -274,14 +274,13 @@ __global__ void __launch_bounds__(512) k_gj_inv_lds(const double* __restrict__ A
     for (int rr = tid; rr < N; rr += nthr) colk[rr] = A[rr * N + k];
     __syncthreads();
     // eliminate column k from the other rows: row i -= A[i][k] * row k; column k -> -A[i][k]/pivot
-    for (int e = tid; e < N * N; e += nthr) {
-      const int i = e / N, j = e - i * N;
+    // (one row per wavefront pass, columns across lanes: no index division in the loop)
+    for (int i = wid; i < N; i += nthr >> 6) {
       if (i == k) continue;
       const double fk = colk[i];
-      if (j == k)
-        A[i * N + k] = -fk * ip;
-      else
-        A[i * N + j] -= fk * A[k * N + j];
+      double* ri = A + (size_t)i * N;
+      const double* rk = A + (size_t)k * N;
+      for (int j = lane; j < N; j += 64) ri[j] = (j == k) ? -fk * ip : ri[j] - fk * rk[j];
     }
     __syncthreads();
   }
@@ -305,10 +304,8 @@ __global__ void __launch_bounds__(512) k_gj_inv_lds(const double* __restrict__ A
   for (int e = tid; e < N * N; e += nthr) X[e] = A[e];
   if (!AinvT32) return;
   float* XT = AinvT32 + (size_t)mat * N * N;
-  for (int e = tid; e < N * N; e += nthr) {
-    const int i = e / N, j = e - i * N;
-    XT[(size_t)j * N + i] = (float)A[e];
-  }
+  for (int j = wid; j < N; j += nthr >> 6)
+    for (int i = lane; i < N; i += 64) XT[(size_t)j * N + i] = (float)A[(size_t)i * N + j];
 }
 
 size_t lu_lds_bytes(int N) { return (size_t)N * N * sizeof(double) + (size_t)N * (sizeof(double) + sizeof(int)); }
@@ -371,6 +368,10 @@ typedef float float4_t __attribute__((ext_vector_type(4)));
 //   trip = sum B[a,x,b] B[b,x,a],  grad_x = sum_a B[a,x,a]   (FP64 sums).
 // Each jacobian row (i, n) belongs to exactly one set S_n, so the orbital jacobian is read
 // once.  Dynamic LDS: J chunk [RT*16][KC+4], X chunk [CT*16][KC+4], B [3s][s], indices.
+// VW: width of the vector loads of jacobian and inverse rows (4, 2 or 1 floats; N % VW == 0
+// keeps every row start aligned).  The next orbital chunk is prefetched into registers while
+// the current one is multiplied.
+template <int VW>
 __global__ void __launch_bounds__(256) k_det_lap(const float* __restrict__ O, int ND, GroupDesc gd,
                                                  const int64_t* __restrict__ ptr, const int32_t* __restrict__ nbr,
                                                  const int32_t* __restrict__ rev, int B, int N, int n_up, int D,
@@ -415,21 +416,47 @@ __global__ void __launch_bounds__(256) k_det_lap(const float* __restrict__ O, in
   }
   __syncthreads();
   const float* XT = AinvT + (size_t)wd * N * N;
+  typedef float vec_t __attribute__((ext_vector_type(VW)));
+  constexpr int UPR = DL_KC / VW;                                   // vector units per chunk row
+  constexpr int JPT = ((FIRE_SMAX * 3 + 15) / 16 * 16 * UPR + 255) / 256;  // J units per thread
+  constexpr int XPT = ((FIRE_SMAX + 15) / 16 * 16 * UPR + 255) / 256;      // X units per thread
+  const int nJ = RT * 16 * UPR, nX = CT * 16 * UPR;
+  vec_t rj[JPT], rx[XPT];
+  auto load = [&](int o0) {
+#pragma unroll
+    for (int q = 0; q < JPT; ++q) {
+      const int e = tid + q * 256;
+      const int ax = e / UPR, k = (e % UPR) * VW;
+      vec_t v = (vec_t)(0.f);
+      if (e < nJ && ax < 3 * s && o0 + k < N) v = *reinterpret_cast<const vec_t*>(O + s_rp[ax] + o0 + k);
+      rj[q] = v;
+    }
+#pragma unroll
+    for (int q = 0; q < XPT; ++q) {
+      const int e = tid + q * 256;
+      const int b = e / UPR, k = (e % UPR) * VW;
+      vec_t v = (vec_t)(0.f);
+      if (e < nX && b < s && o0 + k < N) v = *reinterpret_cast<const vec_t*>(XT + (size_t)s_idx[b] * N + o0 + k);
+      rx[q] = v;
+    }
+  };
+  auto store = [&]() {
+#pragma unroll
+    for (int q = 0; q < JPT; ++q) {
+      const int e = tid + q * 256;
+      if (e < nJ) *reinterpret_cast<vec_t*>(Js + (e / UPR) * LDK + (e % UPR) * VW) = rj[q];
+    }
+#pragma unroll
+    for (int q = 0; q < XPT; ++q) {
+      const int e = tid + q * 256;
+      if (e < nX) *reinterpret_cast<vec_t*>(Xs + (e / UPR) * LDK + (e % UPR) * VW) = rx[q];
+    }
+  };
+  load(0);
   for (int o0 = 0; o0 < N; o0 += DL_KC) {
-    // 32 consecutive threads read 32 consecutive orbitals of one row (DL_KC == 32)
-    for (int e = tid; e < RT * 16 * DL_KC; e += blockDim.x) {
-      const int ax = e >> 5, k = e & 31;
-      float v = 0.f;
-      if (ax < 3 * s && o0 + k < N) v = O[s_rp[ax] + o0 + k];
-      Js[ax * LDK + k] = v;
-    }
-    for (int e = tid; e < CT * 16 * DL_KC; e += blockDim.x) {
-      const int b = e >> 5, k = e & 31;
-      float v = 0.f;
-      if (b < s && o0 + k < N) v = XT[(size_t)s_idx[b] * N + o0 + k];
-      Xs[b * LDK + k] = v;
-    }
+    store();
     __syncthreads();
+    if (o0 + DL_KC < N) load(o0 + DL_KC);
 #pragma unroll
     for (int q = 0; q < DL_TPW; ++q) {
       const int t = wid + 4 * q;
@@ -486,11 +513,21 @@ void launch_det_lap(hipStream_t s, const float* O, int ND, GroupDesc gd, const i
                     const int32_t* rev, int B, int N, int n_up, int D, int smax, const float* AinvT, double* trip,
                     double* grad) {
   const size_t sm = det_lap_smem(smax);
-  k_det_lap<<<B * D * N, 256, sm, s>>>(O, ND, gd, ptr, nbr, rev, B, N, n_up, D, smax, AinvT, trip, grad);
+  if (N % 4 == 0)
+    k_det_lap<4><<<B * D * N, 256, sm, s>>>(O, ND, gd, ptr, nbr, rev, B, N, n_up, D, smax, AinvT, trip, grad);
+  else if (N % 2 == 0)
+    k_det_lap<2><<<B * D * N, 256, sm, s>>>(O, ND, gd, ptr, nbr, rev, B, N, n_up, D, smax, AinvT, trip, grad);
+  else
+    k_det_lap<1><<<B * D * N, 256, sm, s>>>(O, ND, gd, ptr, nbr, rev, B, N, n_up, D, smax, AinvT, trip, grad);
 }
 
 hipError_t det_lap_set_smem_limit(size_t bytes) {
-  return hipFuncSetAttribute((const void*)k_det_lap, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  hipError_t e = hipFuncSetAttribute((const void*)k_det_lap<4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute((const void*)k_det_lap<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute((const void*)k_det_lap<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  return e;
 }
 
 // Sum of the determinants: log|psi_det|, sign, gradient and Laplacian per walker (FP64).

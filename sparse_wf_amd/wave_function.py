@@ -110,11 +110,16 @@ class FireWaveFunction:
         return self.cfg.n_up
 
     # parameters ------------------------------------------------------------------------
-    def upload(self, params: dict):
-        tensors = pack_params(self.cfg, params)
-        sizes = np.asarray([t.size for t in tensors], np.int64)
-        flat = np.ascontiguousarray(np.concatenate(tensors).astype(np.float32))
-        _lib.check(self._lib.fire_upload_params(self._h, flat.ctypes.data, sizes.ctypes.data, len(tensors)))
+    def upload(self, params):
+        """Upload a parameter tree (reference layout) or an already packed ``PackedParams``."""
+        if isinstance(params, PackedParams):
+            flat = np.ascontiguousarray(params.flat, np.float32)
+            sizes = np.ascontiguousarray(params.sizes, np.int64)
+        else:
+            tensors = pack_params(self.cfg, params)
+            sizes = np.asarray([t.size for t in tensors], np.int64)
+            flat = np.ascontiguousarray(np.concatenate(tensors).astype(np.float32))
+        _lib.check(self._lib.fire_upload_params(self._h, flat.ctypes.data, sizes.ctypes.data, len(sizes)))
         self._uploaded = params
 
     def _ensure(self, params):
@@ -348,6 +353,15 @@ class FireWaveFunction:
         r, sg, lp = self.state_values()
         pmove = float(nacc.double().mean() / steps) if steps else 0.0
         return r, dict(pmove=pmove, n_accept=nacc, logpsi=lp, sign=sg, **self.state_stats())
+
+
+@dataclass(eq=False)
+class PackedParams:
+    """Parameters already in the canonical packed form of ``fire_upload_params``: float32
+    tensors back to back (``flat``) and their element counts (``sizes``)."""
+
+    flat: np.ndarray
+    sizes: np.ndarray
 
 
 @dataclass
