@@ -308,6 +308,149 @@ __global__ void __launch_bounds__(512) k_gj_inv_lds(const double* __restrict__ A
     for (int i = lane; i < N; i += 64) XT[(size_t)j * N + i] = (float)A[(size_t)i * N + j];
 }
 
+// Register-resident Gauss-Jordan inverse with implicit partial pivoting, FP64.  One block of
+// NW wavefronts per matrix; element (i, j) lives in a register of thread (i % NW, j % 64) at
+// [i / NW][j / 64], so the matrix never touches LDS.  Step k picks the unused row p with the
+// largest |A[p][k]| (ties: lowest row), broadcasts row p and column k through LDS and updates
+// every element in registers (two barriers per step).  Rows are not swapped: the stored result
+// S satisfies Ainv[q(r)][piv(c)] = S[r][c] with piv(k) = pivot row of step k and q its
+// inverse; det(A) = sign(piv) * prod(pivots).
+template <int NW, int RPT, int CPT>
+__global__ void __launch_bounds__(NW * 64) k_gj_inv_reg(const double* __restrict__ Aall, int N,
+                                                        double* __restrict__ Ainv_all, float* __restrict__ AinvT32,
+                                                        double* __restrict__ logdet, double* __restrict__ sign,
+                                                        int want_inv) {
+  __shared__ double s_row[CPT * 64];
+  __shared__ double s_col[RPT * NW];
+  __shared__ double s_val[NW];
+  __shared__ int s_idx[NW];
+  __shared__ int s_piv[RPT * NW];
+  __shared__ int s_inv[RPT * NW];
+  const int mat = blockIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const double* Ag = Aall + (size_t)mat * N * N;
+  double S[RPT][CPT];
+#pragma unroll
+  for (int m = 0; m < RPT; ++m)
+#pragma unroll
+    for (int c = 0; c < CPT; ++c) {
+      const int i = wid + NW * m, j = lane + 64 * c;
+      S[m][c] = (i < N && j < N) ? Ag[(size_t)i * N + j] : 0.0;
+    }
+  uint32_t used = 0;  // bit m: row wid + NW * m was a pivot (uniform in the wavefront)
+  double ld = 0.0;
+  int neg = 0;
+  for (int k = 0; k < N; ++k) {
+    const int lk = k & 63, ck = k >> 6;
+    if (lane == lk) {
+      double best = -1.0;
+      int bi = N;
+#pragma unroll
+      for (int m = 0; m < RPT; ++m) {
+        const int i = wid + NW * m;
+        double x = 0.0;
+#pragma unroll
+        for (int c = 0; c < CPT; ++c)
+          if (c == ck) x = S[m][c];
+        const double v = fabs(x);
+        if (i < N && !((used >> m) & 1u) && (v > best || (v == best && i < bi))) {
+          best = v;
+          bi = i;
+        }
+      }
+      s_val[wid] = best;
+      s_idx[wid] = bi;
+    }
+    __syncthreads();
+    double best = s_val[0];
+    int p = s_idx[0];
+#pragma unroll
+    for (int q = 1; q < NW; ++q) {
+      const double v = s_val[q];
+      const int iq = s_idx[q];
+      if (v > best || (v == best && iq < p)) {
+        best = v;
+        p = iq;
+      }
+    }
+    const int mp = p / NW;
+    if (wid == p - NW * mp) {
+#pragma unroll
+      for (int c = 0; c < CPT; ++c) {
+        double x = 0.0;
+#pragma unroll
+        for (int m = 0; m < RPT; ++m)
+          if (m == mp) x = S[m][c];
+        s_row[lane + 64 * c] = x;
+      }
+      used |= 1u << mp;
+    }
+    if (lane == lk) {
+#pragma unroll
+      for (int m = 0; m < RPT; ++m) {
+        double x = 0.0;
+#pragma unroll
+        for (int c = 0; c < CPT; ++c)
+          if (c == ck) x = S[m][c];
+        s_col[wid + NW * m] = x;
+      }
+    }
+    if (tid == 0) s_piv[k] = p;
+    __syncthreads();
+    const double pv = s_row[k];
+    const double ip = 1.0 / pv;
+    if (tid == 0) {
+      ld += log(fabs(pv));
+      neg += pv < 0.0;
+    }
+    double rs[CPT];
+#pragma unroll
+    for (int c = 0; c < CPT; ++c) rs[c] = s_row[lane + 64 * c] * ip;
+#pragma unroll
+    for (int m = 0; m < RPT; ++m) {
+      const int i = wid + NW * m;
+      if (i == p) {
+#pragma unroll
+        for (int c = 0; c < CPT; ++c) S[m][c] = (lane + 64 * c == k) ? ip : S[m][c] * ip;
+      } else {
+        const double fk = s_col[i < N ? i : 0];
+#pragma unroll
+        for (int c = 0; c < CPT; ++c) S[m][c] = (lane + 64 * c == k) ? -fk * ip : S[m][c] - fk * rs[c];
+      }
+    }
+  }
+  __syncthreads();
+  for (int k = tid; k < N; k += NW * 64) s_inv[s_piv[k]] = k;
+  __syncthreads();
+  if (tid == 0) {
+    // parity of k -> piv(k): N minus the number of cycles (visited marks kept in s_inv's sign bit)
+    int cycles = 0;
+    for (int s0 = 0; s0 < N; ++s0) {
+      if (s_inv[s0] < 0) continue;
+      ++cycles;
+      for (int t = s0; s_inv[t] >= 0; t = s_piv[t]) s_inv[t] = -1 - s_inv[t];
+    }
+    for (int t = 0; t < N; ++t) s_inv[t] = -1 - s_inv[t];
+    logdet[mat] = ld;
+    sign[mat] = (((N - cycles) + neg) & 1) ? -1.0 : 1.0;
+  }
+  if (!want_inv) return;
+  __syncthreads();
+  double* X = Ainv_all + (size_t)mat * N * N;
+  float* XT = AinvT32 ? AinvT32 + (size_t)mat * N * N : nullptr;
+#pragma unroll
+  for (int m = 0; m < RPT; ++m)
+#pragma unroll
+    for (int c = 0; c < CPT; ++c) {
+      const int r = wid + NW * m, cc = lane + 64 * c;
+      if (r < N && cc < N) {
+        const int i = s_inv[r], j = s_piv[cc];
+        X[(size_t)i * N + j] = S[m][c];
+        if (XT) XT[(size_t)j * N + i] = (float)S[m][c];
+      }
+    }
+}
+
 size_t lu_lds_bytes(int N) { return (size_t)N * N * sizeof(double) + (size_t)N * (sizeof(double) + sizeof(int)); }
 
 hipError_t lu_set_smem_limit(size_t bytes) {
@@ -326,7 +469,11 @@ __global__ void k_transpose_f32(const double* __restrict__ X, int N, float* __re
 // (use_lds), else LU in global memory followed by an FP32 transposed copy.
 void launch_lu_inv(hipStream_t s, double* A, int n_mat, int N, double* Ainv, float* AinvT32, double* logdet,
                    double* sign, bool want_inv, bool use_lds) {
-  if (use_lds) {
+  if (N <= 144) {
+    k_gj_inv_reg<16, 9, 3><<<n_mat, 1024, 0, s>>>(A, N, Ainv, AinvT32, logdet, sign, want_inv ? 1 : 0);
+  } else if (N <= 192) {
+    k_gj_inv_reg<16, 12, 3><<<n_mat, 1024, 0, s>>>(A, N, Ainv, AinvT32, logdet, sign, want_inv ? 1 : 0);
+  } else if (use_lds) {
     k_gj_inv_lds<<<n_mat, 512, lu_lds_bytes(N), s>>>(A, N, Ainv, AinvT32, logdet, sign, want_inv ? 1 : 0);
   } else {
     k_lu_inv<<<n_mat, 256, N * sizeof(int), s>>>(A, N, Ainv, logdet, sign, want_inv ? 1 : 0);
