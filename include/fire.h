@@ -41,11 +41,12 @@ typedef struct fire_config {
 
 enum fire_precision {
   FIRE_FP32 = 0,  /* every embedding / orbital / Jastrow contraction in FP32 (FP32 MFMA) */
-  FIRE_MIXED = 1, /* dense contractions on BF16 MFMA with FP32 accumulation, each operand
-                     carried as three BF16 planes (hi + mid + lo) and the six products down
-                     to 2^-16 relative accumulated (FP32-level products); element-wise
-                     forward-Laplacian rules, envelopes and Jastrow sums FP32; Slater
-                     determinants, their inverses, updates and energy sums FP64 */
+  FIRE_MIXED = 1, /* dense contractions on BF16 MFMA with FP32 accumulation: embedding and
+                     orbital contractions carry each operand as three BF16 planes (hi + mid +
+                     lo) and accumulate the six products down to 2^-16 relative (FP32-level
+                     products); Jastrow contractions use two planes (see fire_precision_plan);
+                     element-wise forward-Laplacian rules, envelopes and Jastrow sums FP32;
+                     Slater determinants, their inverses, updates and energy sums FP64 */
   FIRE_BF16 = 2,  /* dense contractions in single BF16 on MFMA (fastest, least accurate) */
   FIRE_MIXED_FAST = 3, /* as FIRE_MIXED with two BF16 planes (hi + lo) and three products
                           (hi*hi + hi*lo + lo*hi): ~2^-17 relative per product */
@@ -145,6 +146,13 @@ int fire_state_get(fire_handle* h, void* stream, float* r_dev, double* logpsi_de
  * statistics of low-rank updates, mcmc.py:225-231).  Host outputs; any may be NULL. */
 int fire_state_stats(fire_handle* h, int32_t* n_walkers, int64_t* n_proposals, int64_t* sum_changed,
                      int64_t* max_changed);
+
+/* Precision plan of FIRE_MIXED (no counterpart in the reference): BF16 planes per operand for
+ * each contraction class, in the order ElecInit MLP, update MLP, orbitals, MLP Jastrow,
+ * attention Q/V, attention readout.  3 accumulates the six products down to 2^-16 relative
+ * (FP32-level), 2 the three products of a hi + lo split, 1 plain BF16.  Default {3,3,3,2,2,2}.  Sets the first
+ * n classes from planes (n may be 0); current (optional, int32 [6]) receives the plan. */
+int fire_precision_plan(fire_handle* h, const int32_t* planes, int32_t n, int32_t* current);
 
 /* Tile-shape autotuner of the dense contractions (no counterpart in the reference; the
  * reference leaves kernel selection to XLA).  On by default: the first BF16 contraction of

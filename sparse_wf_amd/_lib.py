@@ -51,6 +51,7 @@ SIGNATURES = {
     "fire_state_stats": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int64),
                                         ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]),
     "fire_autotune": (ctypes.c_int, [_P, ctypes.c_int32, _P, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]),
+    "fire_precision_plan": (ctypes.c_int, [_P, _P, ctypes.c_int32, _P]),
     "fire_last_error": (ctypes.c_char_p, []),
 }
 

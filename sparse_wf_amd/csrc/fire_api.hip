@@ -106,6 +106,13 @@ struct fire_handle {
   // Autotuned tile shape of the BF16 contraction per (split, N, K, floor(log2 M)): the first
   // call of a shape times every candidate on the real operands and keeps the fastest.
   bool autotune = true;
+  // BF16 planes per operand used by FIRE_MIXED for each contraction class (ElecInit MLP,
+  // update MLP, orbitals, MLP Jastrow, attention Q/V, attention readout): 3 = six products
+  // (FP32-level), 2 = three products, 1 = plain.  Default: 3 for the embedding and orbital
+  // classes, 2 for the Jastrow classes (on equilibrated C16H34 walkers the Jastrow classes at
+  // 2 planes leave the FP64 energy error unchanged; any embedding/orbital class at 2 planes
+  // raises its rms from 0.4 to 3-4 mHa)
+  int planes[6] = {3, 3, 3, 2, 2, 2};
   std::map<std::tuple<int, int, int, int>, int> gemm_tile;
   std::map<std::tuple<int, int, int, int>, std::vector<float>> gemm_tile_ms;
   // Resident walker state for single-electron low-rank updates (fire_state_*, fire_mcmc_steps).
@@ -232,6 +239,15 @@ struct Ctx {
   int prec;
 };
 
+// Contraction class of weight tensor wi (index into fire_handle::planes).
+int gemm_class(const fire_handle* h, int wi) {
+  if (wi < h->iEEs) return 0;
+  if (wi < h->iORB) return 1;
+  if (wi < h->iJM) return 2;
+  if (wi < h->iATT) return 3;
+  return wi == h->iATT ? 4 : 5;
+}
+
 // C = A @ W(tensor wi) + bias(tensor bi, optional) on value rows
 void gemm(Ctx& c, const float* A, int64_t lda, int wi, int bi, float* C, int64_t ldc, int64_t M, int N, int K,
           const uint8_t* rowflag, int period) {
@@ -239,7 +255,11 @@ void gemm(Ctx& c, const float* A, int64_t lda, int wi, int bi, float* C, int64_t
   const float* bias = bi >= 0 ? h->P[bi] : nullptr;
   const bool bf_ok = h->Pbf[wi] != nullptr && (K % 32) == 0 && (lda % 4) == 0 && N > 16;
   const bool use_bf = (c.prec == FIRE_MIXED || c.prec == FIRE_MIXED_FAST || c.prec == FIRE_BF16) && bf_ok;
-  const int ns = c.prec == FIRE_MIXED ? 6 : (c.prec == FIRE_MIXED_FAST ? 3 : 1);
+  int ns = c.prec == FIRE_MIXED_FAST ? 3 : 1;
+  if (c.prec == FIRE_MIXED) {
+    const int np = h->planes[gemm_class(h, wi)];
+    ns = np == 3 ? 6 : (np == 2 ? 3 : 1);
+  }
   int variant = 0;
   if (use_bf && h->autotune) {
     int lg = 0;
@@ -1387,6 +1407,21 @@ int fire_autotune(fire_handle* h, int32_t enable, int32_t* table, int32_t cap, i
       h->gemm_tile.clear();
       h->gemm_tile_ms.clear();
     }
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e);
+  }
+}
+
+int fire_precision_plan(fire_handle* h, const int32_t* planes, int32_t n, int32_t* current) {
+  try {
+    REQUIRE(h, "null handle");
+    REQUIRE(n >= 0 && n <= 6, "at most 6 contraction classes");
+    for (int k = 0; k < n; ++k)
+      REQUIRE(planes[k] >= 1 && planes[k] <= 3, "planes per operand must be 1, 2 or 3");
+    for (int k = 0; k < n; ++k) h->planes[k] = planes[k];
+    if (current)
+      for (int k = 0; k < 6; ++k) current[k] = h->planes[k];
     return 0;
   } catch (const std::exception& e) {
     return fail(e);

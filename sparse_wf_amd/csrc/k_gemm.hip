@@ -340,11 +340,136 @@ void launch_gemm_f32(hipStream_t s, const float* A, int64_t lda, const float* W,
     k_gemm_f32<false><<<(unsigned)blocks, 256, 0, s>>>(A, lda, Wt, bias, C, ldc, M, N, K, rowflag, period, nT);
 }
 
+// Double-buffered form: two LDS stages of KC (16 or 32) k each; chunk c is multiplied from
+// stage c % 2 while chunk c + 1 (already in registers) is split into stage (c + 1) % 2 and
+// chunk c + 2 is loaded: one barrier per chunk.  With KC = 16 a 128 x 128 block needs 74 KB
+// (NP = 3), so two blocks share a CU.
+template <int NP, int BM, int BN, int KC>
+__global__ void __launch_bounds__(BM * BN / 64) k_gemm_bf16_db(const float* __restrict__ A, int64_t lda,
+                                                              const uint16_t* __restrict__ Wp0,
+                                                              const uint16_t* __restrict__ Wp1,
+                                                              const uint16_t* __restrict__ Wp2,
+                                                              const float* __restrict__ bias, float* __restrict__ C,
+                                                              int64_t ldc, int64_t M, int N, int K,
+                                                              const uint8_t* __restrict__ rowflag, int period, int nT) {
+  constexpr int NWN = BN / 64;
+  constexpr int NT = (BM / 64) * NWN * 64;
+  constexpr int LDH = KC + 8;                // halves per LDS row (16-byte aligned, conflict-free reads)
+  constexpr int A_UNITS = BM * (KC / 4), B_UNITS = BN * (KC / 8);  // float4 / uint4 per chunk
+  constexpr int A_PER = (A_UNITS + NT - 1) / NT;                    // per thread (guarded)
+  constexpr int B_PER = (B_UNITS + NT - 1) / NT;
+  constexpr size_t STAGE = (size_t)NP * (BM + BN) * LDH;  // halves per stage
+  extern __shared__ __attribute__((aligned(16))) unsigned char gemm_lds[];
+  uint16_t* lds = reinterpret_cast<uint16_t*>(gemm_lds);
+  auto As = [&](int st, int p, int r) { return lds + st * STAGE + ((size_t)p * BM + r) * LDH; };
+  auto Bs = [&](int st, int p, int r) { return lds + st * STAGE + (size_t)NP * BM * LDH + ((size_t)p * BN + r) * LDH; };
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  int64_t mt;
+  int nt;
+  tile_of(mt, nt, nT);
+  const int64_t m0 = mt * BM;
+  const int n0 = nt * BN;
+  const int wm = (wid / NWN) * 64, wn = (wid % NWN) * 64;
+  float16_t acc[2][2];
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b)
+      for (int v = 0; v < 16; ++v) acc[a][b][v] = 0.f;
+  const uint16_t* Wpl[3] = {Wp0, Wp1, Wp2};
+  float4 ra[A_PER];
+  uint4 rb[NP][B_PER];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < A_PER; ++q) {
+      const int e = tid + q * NT;
+      const int rr = e / (KC / 4), kq = (e % (KC / 4)) * 4;
+      const int64_t gr = m0 + rr;
+      ra[q] = (e < A_UNITS && gr < M) ? *reinterpret_cast<const float4*>(A + gr * lda + k0 + kq)
+                                      : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+#pragma unroll
+    for (int q = 0; q < B_PER; ++q) {
+      const int e = tid + q * NT;
+      const int nn = e / (KC / 8), kq = (e % (KC / 8)) * 8;
+      const int gn = n0 + nn;
+#pragma unroll
+      for (int p = 0; p < NP; ++p)
+        rb[p][q] = (e < B_UNITS && gn < N) ? *reinterpret_cast<const uint4*>(Wpl[p] + (int64_t)gn * K + k0 + kq)
+                                           : make_uint4(0, 0, 0, 0);
+    }
+  };
+  auto store = [&](int st) {
+#pragma unroll
+    for (int q = 0; q < A_PER; ++q) {
+      const int e = tid + q * NT;
+      if (A_UNITS % NT != 0 && e >= A_UNITS) continue;
+      const int rr = e / (KC / 4), kq = (e % (KC / 4)) * 4;
+      float2v_t x01 = {ra[q].x, ra[q].y}, x23 = {ra[q].z, ra[q].w};
+#pragma unroll
+      for (int p = 0; p < NP; ++p) {
+        const bf16x2_t h01 = __builtin_convertvector(x01, bf16x2_t);
+        const bf16x2_t h23 = __builtin_convertvector(x23, bf16x2_t);
+        *reinterpret_cast<uint2*>(As(st, p, rr) + kq) =
+            make_uint2(__builtin_bit_cast(uint32_t, h01), __builtin_bit_cast(uint32_t, h23));
+        if (p + 1 < NP) {
+          x01 = x01 - __builtin_convertvector(h01, float2v_t);
+          x23 = x23 - __builtin_convertvector(h23, float2v_t);
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < B_PER; ++q) {
+      const int e = tid + q * NT;
+      if (B_UNITS % NT != 0 && e >= B_UNITS) continue;
+      const int nn = e / (KC / 8), kq = (e % (KC / 8)) * 8;
+#pragma unroll
+      for (int p = 0; p < NP; ++p) *reinterpret_cast<uint4*>(Bs(st, p, nn) + kq) = rb[p][q];
+    }
+  };
+  auto compute = [&](int st) {
+#pragma unroll
+    for (int kk = 0; kk < KC; kk += 16) {
+      const int ka = kk + 8 * (lane >> 5);
+      bf16x8_t av[NP][2], bv[NP][2];
+#pragma unroll
+      for (int p = 0; p < NP; ++p) {
+#pragma unroll
+        for (int a = 0; a < 2; ++a) av[p][a] = *reinterpret_cast<const bf16x8_t*>(As(st, p, wm + a * 32 + (lane & 31)) + ka);
+#pragma unroll
+        for (int b = 0; b < 2; ++b) bv[p][b] = *reinterpret_cast<const bf16x8_t*>(Bs(st, p, wn + b * 32 + (lane & 31)) + ka);
+      }
+#pragma unroll
+      for (int t = 0; t < n_products(NP); ++t)
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int b = 0; b < 2; ++b)
+            acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[prod_a(NP, t)][a], bv[prod_b(NP, t)][b],
+                                                                acc[a][b], 0, 0, 0);
+    }
+  };
+  const int nch = K / KC;
+  load(0);
+  store(0);
+  if (nch > 1) load(KC);
+  __syncthreads();
+  for (int c = 0; c < nch; ++c) {
+    compute(c & 1);
+    if (c + 1 < nch) {
+      store((c + 1) & 1);
+      if (c + 2 < nch) load((c + 2) * KC);
+    }
+    __syncthreads();
+  }
+  store_tile(acc, m0, n0, wm, wn, lane, C, ldc, M, N, bias, rowflag, period);
+}
+
 // Tile shapes offered to the autotuner (index = variant).
 struct Bf16Tile {
   int bm, bn;
 };
-static const Bf16Tile kBf16Tiles[] = {{128, 128}, {128, 256}, {256, 128}, {64, 256}, {256, 64}, {64, 128}, {128, 64}};
+// variants 0-6: single LDS stage, K chunk 32; variants 7-10: two LDS stages (kc = 16 or 32)
+static const Bf16Tile kBf16Tiles[] = {{128, 128}, {128, 256}, {256, 128}, {64, 256}, {256, 64}, {64, 128}, {128, 64},
+                                      {128, 128}, {128, 128}, {256, 128}, {128, 256}};
 
 int gemm_bf16_variants() { return (int)(sizeof(kBf16Tiles) / sizeof(kBf16Tiles[0])); }
 void gemm_bf16_tile(int variant, int* bm, int* bn) {
@@ -370,11 +495,32 @@ static void launch_tile(hipStream_t s, const float* A, int64_t lda, const uint16
                                                                      rowflag, period, nT);
 }
 
+template <int NP, int BM, int BN, int KC>
+static void launch_tile_db(hipStream_t s, const float* A, int64_t lda, const uint16_t* const* W, const float* bias,
+                           float* C, int64_t ldc, int64_t M, int N, int K, const uint8_t* rowflag, int period) {
+  static bool attr_set = false;
+  const size_t lds = (size_t)2 * NP * (BM + BN) * (KC + 8) * 2;
+  if (!attr_set) {
+    if (lds > 64 * 1024)
+      (void)hipFuncSetAttribute((const void*)k_gemm_bf16_db<NP, BM, BN, KC>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    attr_set = true;
+  }
+  const int nT = (N + BN - 1) / BN;
+  const int64_t blocks = ((M + BM - 1) / BM) * nT;
+  k_gemm_bf16_db<NP, BM, BN, KC><<<(unsigned)blocks, BM * BN / 64, lds, s>>>(A, lda, W[0], W[1], W[2], bias, C, ldc,
+                                                                           M, N, K, rowflag, period, nT);
+}
+
 template <int NP>
 static void launch_np(hipStream_t s, int variant, const float* A, int64_t lda, const uint16_t* const* W,
                       const float* bias, float* C, int64_t ldc, int64_t M, int N, int K, const uint8_t* rowflag,
                       int period) {
   switch (variant) {
+    case 7: launch_tile_db<NP, 128, 128, 16>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period); break;
+    case 8: launch_tile_db<NP, 128, 128, 32>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period); break;
+    case 9: launch_tile_db<NP, 256, 128, 16>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period); break;
+    case 10: launch_tile_db<NP, 128, 256, 16>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period); break;
     case 1: launch_tile<NP, 128, 256>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period); break;
     case 2: launch_tile<NP, 256, 128>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period); break;
     case 3: launch_tile<NP, 64, 256>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period); break;
