@@ -130,11 +130,20 @@ int fire_state_commit(fire_handle* h, void* stream, const int32_t* accept_dev);
  * u ~ U(0, 1).  Random numbers are Philox4x32-10 keyed by (seed, walker) with counter s, so
  * a run is reproducible and independent of how steps are split over calls (pass the running
  * step index as step0).  n_accept_dev (optional, int32 [B]) receives the accepted moves of
- * each walker in this call.  Synchronises once per step (row counts of the update).
+ * each walker in this call.  Every proposal runs at static capacities (at most min(n_el, 64)
+ * changed electrons per move) with row counts kept on the device, so the call is
+ * asynchronous (it synchronises only when it captures a new sweep graph).  A move that
+ * changes more electrons than the capacity invalidates the state; fire_state_get reports it
+ * as an error.
  * Replaces mcmc_steps_low_rank with proposal_single_electron (src/sparse_wf/mcmc.py:100-111,
  * 196-275). */
 int fire_mcmc_steps(fire_handle* h, void* stream, int32_t n_steps, float width, uint64_t seed, int64_t step0,
                     int32_t* n_accept_dev);
+
+/* Options of fire_mcmc_steps (no counterpart in the reference): use_graphs = 1 (default)
+ * replays whole sweeps (n_el steps starting at a multiple of n_el) from one hipGraph captured
+ * on an internal stream, 0 issues every step's kernels one by one.  Results are identical. */
+int fire_mcmc_options(fire_handle* h, int32_t use_graphs);
 
 /* Copies the state's positions (float [B, n_el, 3]), log|psi| and sign (double [B]) out; any
  * may be NULL.  A pending proposal is dropped.  (No counterpart: the reference returns the

@@ -52,6 +52,7 @@ SIGNATURES = {
                                         ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]),
     "fire_autotune": (ctypes.c_int, [_P, ctypes.c_int32, _P, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]),
     "fire_precision_plan": (ctypes.c_int, [_P, _P, ctypes.c_int32, _P]),
+    "fire_mcmc_options": (ctypes.c_int, [_P, ctypes.c_int32]),
     "fire_last_error": (ctypes.c_char_p, []),
 }
 

@@ -106,6 +106,7 @@ struct fire_handle {
   // Autotuned tile shape of the BF16 contraction per (split, N, K, floor(log2 M)): the first
   // call of a shape times every candidate on the real operands and keeps the fastest.
   bool autotune = true;
+  bool use_graphs = true;  // whole Metropolis sweeps replayed from a captured hipGraph
   // BF16 planes per operand used by FIRE_MIXED for each contraction class (ElecInit MLP,
   // update MLP, orbitals, MLP Jastrow, attention Q/V, attention readout): 3 = six products
   // (FP32-level), 2 = three products, 1 = plain.  Default: 3 for the embedding and orbital
@@ -120,8 +121,21 @@ struct fire_handle {
   struct State {
     int B = 0, prec = 0;
     bool ready = false, pending = false;
-    int k = -1, T = 0, T_up = 0, amax = 0;
-    int64_t n_proposals = 0, sum_changed = 0, max_changed = 0;
+    int k = -1;
+    // static capacities of a proposal: cap changed electrons per walker, tcap = B * cap rows
+    int cap = 0;
+    int64_t tcap = 0;
+    int64_t n_proposals = 0;
+    // device: tail = {T_up, T_dn, max changed, T} of the last proposal, stats = {sum of T, max
+    // changed} since init, overflow flag, step counter of the Gaussian proposals, accepted
+    // moves per walker of the current fire_mcmc_steps call
+    DevBuf dstats, doverflow, dstep, nacc;
+    // hipGraph of one sweep (n_el steps, electron k = step within the sweep) on `gstream`
+    hipGraphExec_t graph = nullptr;
+    hipStream_t gstream = nullptr;
+    float g_width = 0.f;
+    uint64_t g_seed = 0;
+    bool g_count = false;
     // state: positions [B,N,3]; h0/message rows [G,5F]; embedding [G,F]; raw orbitals and
     // envelopes [G,N*D]; Slater matrices and inverses [B,D,N,N] FP64; log|det|, sign [B,D];
     // MLP-Jastrow rows [G,2]; attention rows [G,H+H*Dk]; log psi, sign [B]
@@ -142,7 +156,11 @@ struct fire_handle {
                      &list,  &rowof, &klist, &kT5,     &kZ2,   &kT5b,   &p_M5k, &kcoef, &p_envk, &pcnt,  &pptr,
                      &pnbr,  &p_hc,  &p_x,   &p_h,     &p_y,   &p_orb,  &p_jr,  &p_qv,  &p_av,   &phin,  &sinv,
                      &ldS,   &sgS,   &jsum,  &rin,     &ry1,   &ry2,    &att_out, &cval, &clap,  &cgrad, &epot,
-                     &p_ld,  &p_sg,  &p_logpsi, &p_sign, &accept, &wbscr};
+                     &p_ld,  &p_sg,  &p_logpsi, &p_sign, &accept, &wbscr, &dstats, &doverflow, &dstep, &nacc};
+      if (graph) (void)hipGraphExecDestroy(graph);
+      graph = nullptr;
+      if (gstream) (void)hipStreamDestroy(gstream);
+      gstream = nullptr;
       for (DevBuf* x : b) x->release();
       ready = pending = false;
     }
@@ -237,6 +255,7 @@ struct Ctx {
   fire_handle* h;
   hipStream_t s;
   int prec;
+  bool fixed = false;  // no autotuning or event timing (stream capture, device-sized launches)
 };
 
 // Contraction class of weight tensor wi (index into fire_handle::planes).
@@ -250,7 +269,7 @@ int gemm_class(const fire_handle* h, int wi) {
 
 // C = A @ W(tensor wi) + bias(tensor bi, optional) on value rows
 void gemm(Ctx& c, const float* A, int64_t lda, int wi, int bi, float* C, int64_t ldc, int64_t M, int N, int K,
-          const uint8_t* rowflag, int period) {
+          const uint8_t* rowflag, int period, DevRows dr = DevRows()) {
   fire_handle* h = c.h;
   const float* bias = bi >= 0 ? h->P[bi] : nullptr;
   const bool bf_ok = h->Pbf[wi] != nullptr && (K % 32) == 0 && (lda % 4) == 0 && N > 16;
@@ -268,7 +287,7 @@ void gemm(Ctx& c, const float* A, int64_t lda, int wi, int bi, float* C, int64_t
     auto it = h->gemm_tile.find(key);
     if (it != h->gemm_tile.end()) {
       variant = it->second;
-    } else {
+    } else if (!c.fixed) {
       // time every tile shape twice on these operands (the second launch counts)
       const int nv = gemm_bf16_variants();
       std::vector<float> ms(nv, 0.f);
@@ -293,17 +312,18 @@ void gemm(Ctx& c, const float* A, int64_t lda, int wi, int bi, float* C, int64_t
     }
   }
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  if (h->gemm_timing) {
+  const bool timed = h->gemm_timing && !c.fixed;
+  if (timed) {
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
     HIPCHK(hipEventRecord(e0, c.s));
   }
   if (use_bf)
     launch_gemm_bf16(c.s, A, lda, h->Pbf[wi], h->Pbf_lo[wi], h->Pbf_lo2[wi], bias, C, ldc, M, N, K, rowflag, period, ns,
-                     variant);
+                     variant, dr);
   else
-    launch_gemm_f32(c.s, A, lda, h->P[wi], h->Pt32[wi], bias, C, ldc, M, N, K, rowflag, period);
-  if (h->gemm_timing) {
+    launch_gemm_f32(c.s, A, lda, h->P[wi], h->Pt32[wi], bias, C, ldc, M, N, K, rowflag, period, dr);
+  if (timed) {
     HIPCHK(hipEventRecord(e1, c.s));
     h->gemm_events.push_back({e0, e1});
     h->gemm_flops += 2.0 * (double)M * N * K;
@@ -645,11 +665,16 @@ void run_jastrow_and_assemble(Ctx& c, const float* r, int B, bool tan, const Gro
 void state_init(Ctx& c, const float* r_dev, int B) {
   fire_handle* h = c.h;
   auto& S = h->st;
-  const int N = h->N, F = h->F, G = B * N, ND = h->ND, D = h->D;
+  const int N = h->N, F = h->F, G = B * N, ND = h->ND, D = h->D, W = h->W, HD = h->HD;
+  const int nc = h->H + HD;
   S.ready = S.pending = false;
+  if (S.graph) {
+    HIPCHK(hipGraphExecDestroy(S.graph));
+    S.graph = nullptr;
+  }
   S.B = B;
   S.prec = c.prec;
-  S.n_proposals = S.sum_changed = S.max_changed = 0;
+  S.n_proposals = 0;
   S.r.ensure(sizeof(float) * (size_t)G * 3);
   S.p_r.ensure(sizeof(float) * (size_t)G * 3);
   HIPCHK(hipMemcpyAsync(S.r.p, r_dev, sizeof(float) * (size_t)G * 3, hipMemcpyDeviceToDevice, c.s));
@@ -682,11 +707,73 @@ void state_init(Ctx& c, const float* r_dev, int B) {
     HIPCHK(hipMemcpyAsync(S.jr.p, h->JR.p, sizeof(float) * (size_t)G * 2, hipMemcpyDeviceToDevice, c.s));
   }
   if (h->has_att) {
-    const size_t nc = (size_t)h->H + h->HD;
     S.av.ensure(sizeof(float) * (size_t)G * nc);
     HIPCHK(hipMemcpyAsync(S.av.p, h->AV.p, sizeof(float) * (size_t)G * nc, hipMemcpyDeviceToDevice, c.s));
   }
+  // proposal buffers at their static capacities (no allocation while proposing, so a sweep
+  // can be captured into a graph): at most cap changed electrons per walker and move
+  S.cap = std::min(N, 64);
+  S.tcap = (int64_t)B * S.cap;
+  const int64_t T = S.tcap;
+  const int ME = h->M * h->Eiso, XW = std::max(F, std::max(W, 2 * HD)), amax = S.cap;
   S.accept.ensure(sizeof(int32_t) * B);
+  S.nacc.ensure(sizeof(int32_t) * B);
+  S.p_rk.ensure(sizeof(float) * B * 3);
+  S.p_u.ensure(sizeof(double) * B);
+  S.aff.ensure(sizeof(int32_t) * (size_t)B * N);
+  for (DevBuf* b : {&S.cu, &S.cd, &S.offu, &S.offd}) b->ensure(sizeof(int32_t) * B);
+  S.tail.ensure(sizeof(int32_t) * 4);
+  S.dstats.ensure(sizeof(int64_t) * 2);
+  S.doverflow.ensure(sizeof(int32_t));
+  S.dstep.ensure(sizeof(int64_t));
+  S.list.ensure(sizeof(int2) * (size_t)T);
+  S.rowof.ensure(sizeof(int32_t) * (size_t)B * N);
+  S.klist.ensure(sizeof(int2) * B);
+  S.kT5.ensure(sizeof(float) * (size_t)B * F);
+  S.kT5b.ensure(sizeof(float) * (size_t)B * F);
+  S.kZ2.ensure(sizeof(float) * (size_t)B * 2 * F);
+  S.p_M5k.ensure(sizeof(float) * (size_t)B * 5 * F);
+  S.pcnt.ensure(sizeof(int32_t) * (size_t)T);
+  S.pptr.ensure(sizeof(int64_t) * (size_t)(T + 2));
+  S.pnbr.ensure(sizeof(int32_t) * (size_t)T * std::max(N - 1, 1));
+  S.p_hc.ensure(sizeof(float) * (size_t)T * F);
+  S.p_x.ensure(sizeof(float) * (size_t)T * XW);
+  S.p_y.ensure(sizeof(float) * (size_t)T * XW);
+  S.p_h.ensure(sizeof(float) * (size_t)T * F);
+  S.p_orb.ensure(sizeof(float) * (size_t)T * ND);
+  S.kcoef.ensure(sizeof(float) * (size_t)D * B * ME);
+  S.p_envk.ensure(sizeof(float) * (size_t)B * ND);
+  S.phin.ensure(sizeof(double) * (size_t)B * D * amax * N);
+  S.sinv.ensure(sizeof(double) * (size_t)B * D * amax * amax);
+  S.ldS.ensure(sizeof(double) * (size_t)B * D);
+  S.sgS.ensure(sizeof(double) * (size_t)B * D);
+  S.wbscr.ensure(std::max<size_t>((size_t)B * D * wb_update_scratch(N, amax), 8));
+  S.jsum.ensure(sizeof(double) * (size_t)B * 2);
+  S.p_jr.ensure(sizeof(float) * (size_t)T * 2);
+  S.p_qv.ensure(sizeof(float) * (size_t)T * 2 * std::max(HD, 1));
+  S.p_av.ensure(sizeof(float) * (size_t)T * nc);
+  S.rin.ensure(sizeof(float) * (size_t)B * 2 * std::max(HD, 1));
+  S.ry1.ensure(sizeof(float) * (size_t)B * W);
+  S.ry2.ensure(sizeof(float) * (size_t)B * W);
+  S.att_out.ensure(sizeof(float) * (size_t)B * 2);
+  S.cval.ensure(sizeof(double) * (size_t)B * N);
+  S.clap.ensure(sizeof(double) * (size_t)B * N);
+  S.cgrad.ensure(sizeof(double) * (size_t)B * N * 3);
+  S.epot.ensure(sizeof(double) * (size_t)B * N);
+  for (DevBuf* x : {&S.p_ld, &S.p_sg}) x->ensure(sizeof(double) * (size_t)B * D);
+  for (DevBuf* x : {&S.p_logpsi, &S.p_sign}) x->ensure(sizeof(double) * B);
+  // dynamic LDS of the Woodbury kernels at the capacity (their static LDS comes on top)
+  const size_t pw = wb_prep_smem(N, amax), uw = wb_update_smem(N, amax);
+  REQUIRE(pw <= 160 * 1024 - 512 && uw <= 160 * 1024 - 512,
+          "the LDS-resident Woodbury update does not fit this electron count");
+  if ((pw > 64 * 1024 || uw > 64 * 1024) && (pw > S.wb_smem_prep || uw > S.wb_smem_upd)) {
+    S.wb_smem_prep = std::max(S.wb_smem_prep, pw);
+    S.wb_smem_upd = std::max(S.wb_smem_upd, uw);
+    HIPCHK(wb_set_smem_limit(S.wb_smem_prep, S.wb_smem_upd));
+  }
+  launch_fill_i32(c.s, S.doverflow.as<int32_t>(), 1, 0);
+  launch_fill_i32(c.s, reinterpret_cast<int32_t*>(S.dstats.p), 4, 0);
+  launch_set_i64(c.s, S.dstep.as<int64_t>(), 0);
   S.ready = true;
 }
 
@@ -700,9 +787,11 @@ void state_drop_pending(Ctx& c) {
   state_commit(c);
 }
 
-// Proposal for electron k of every walker: mode 0 Gaussian step (width, seed, step),
-// mode 1 new positions rk_in [B,3].  Leaves log psi / sign of the proposals in p_logpsi /
-// p_sign and, in mode 0, the Metropolis decisions in accept.
+// Proposal for electron k of every walker: mode 0 Gaussian step (width, seed, step index
+// *dstep + step), mode 1 new positions rk_in [B,3].  Leaves log psi / sign of the proposals in
+// p_logpsi / p_sign and, in mode 0, the Metropolis decisions in accept (n_acc counts them).
+// Every launch is sized by the static capacities and reads the actual row counts from the
+// device, so the sequence has no host synchronisation and can be captured into a graph.
 void state_propose(Ctx& c, int k, int mode, const float* rk_in, float width, uint64_t seed, uint64_t step,
                    int32_t* n_acc) {
   fire_handle* h = c.h;
@@ -711,52 +800,27 @@ void state_propose(Ctx& c, int k, int mode, const float* rk_in, float width, uin
   REQUIRE(k >= 0 && k < h->N, "electron index out of range");
   state_drop_pending(c);
   const int B = S.B, N = h->N, F = h->F, ND = h->ND, D = h->D, W = h->W, nu = h->n_up;
-  const int H = h->H, Dk = h->Dk, HD = h->HD, nc = H + HD;
-  Ctx cc{h, c.s, S.prec};
-  S.p_rk.ensure(sizeof(float) * B * 3);
-  S.p_u.ensure(sizeof(double) * B);
-  S.aff.ensure(sizeof(int32_t) * (size_t)B * N);
-  for (DevBuf* b : {&S.cu, &S.cd, &S.offu, &S.offd}) b->ensure(sizeof(int32_t) * B);
-  S.tail.ensure(sizeof(int32_t) * 4);
-  S.list.ensure(sizeof(int2) * (size_t)B * N);
-  S.rowof.ensure(sizeof(int32_t) * (size_t)B * N);
-  S.klist.ensure(sizeof(int2) * B);
-  launch_mc_propose(c.s, S.r.as<float>(), B, N, k, width, seed, step, mode, rk_in, S.p_r.as<float>(),
-                    S.p_rk.as<float>(), S.p_u.as<double>());
-  launch_mc_affected(c.s, S.r.as<float>(), S.p_r.as<float>(), B, N, nu, k, h->cfg.cutoff, S.aff.as<int32_t>(),
-                     S.cu.as<int32_t>(), S.cd.as<int32_t>());
-  launch_mc_scan(c.s, S.cu.as<int32_t>(), S.cd.as<int32_t>(), B, S.offu.as<int32_t>(), S.offd.as<int32_t>(),
-                 S.tail.as<int32_t>());
+  const int H = h->H, Dk = h->Dk, HD = h->HD;
+  const int64_t T = S.tcap;
+  const int amax = S.cap;
+  Ctx cc{h, c.s, S.prec, true};
+  int32_t* tail = S.tail.as<int32_t>();
+  const int32_t* dT_up = tail + 0;
+  const int32_t* dT_dn = tail + 1;
+  const int32_t* dT = tail + 3;
+  launch_mc_propose(c.s, S.r.as<float>(), B, N, k, width, seed, step, S.dstep.as<int64_t>(), mode, rk_in,
+                    S.p_r.as<float>(), S.p_rk.as<float>(), S.p_u.as<double>());
+  launch_mc_affected(c.s, S.r.as<float>(), S.p_r.as<float>(), B, N, nu, k, h->cfg.cutoff, S.cap, S.aff.as<int32_t>(),
+                     S.cu.as<int32_t>(), S.cd.as<int32_t>(), S.doverflow.as<int32_t>());
+  launch_mc_scan(c.s, S.cu.as<int32_t>(), S.cd.as<int32_t>(), B, S.offu.as<int32_t>(), S.offd.as<int32_t>(), tail,
+                 S.dstats.as<int64_t>());
   launch_mc_rowlist(c.s, S.aff.as<int32_t>(), S.cu.as<int32_t>(), S.cd.as<int32_t>(), S.offu.as<int32_t>(),
-                    S.offd.as<int32_t>(), S.tail.as<int32_t>(), B, N, nu, S.list.as<int2>(), S.rowof.as<int32_t>());
-  int32_t tail[3];
-  HIPCHK(hipMemcpyAsync(tail, S.tail.p, sizeof(tail), hipMemcpyDeviceToHost, c.s));
-  HIPCHK(hipStreamSynchronize(c.s));
-  const int T_up = tail[0], T = tail[0] + tail[1], amax = tail[2];
+                    S.offd.as<int32_t>(), tail, B, N, nu, S.list.as<int2>(), S.rowof.as<int32_t>());
   S.k = k;
-  S.T = T;
-  S.T_up = T_up;
-  S.amax = amax;
   S.n_proposals += B;
-  S.sum_changed += T;
-  S.max_changed = std::max<int64_t>(S.max_changed, amax);
-  // dynamic LDS of the Woodbury kernels (their static LDS, < 128 bytes, comes on top)
-  const size_t pw = wb_prep_smem(N, amax), uw = wb_update_smem(N, amax);
-  REQUIRE(pw <= 160 * 1024 - 512 && uw <= 160 * 1024 - 512,
-          "too many electrons changed by one move (" + std::to_string(amax) +
-              ") for the LDS-resident Woodbury update at this electron count");
-  if ((pw > 64 * 1024 || uw > 64 * 1024) && (pw > S.wb_smem_prep || uw > S.wb_smem_upd)) {
-    S.wb_smem_prep = std::max(S.wb_smem_prep, pw);
-    S.wb_smem_upd = std::max(S.wb_smem_upd, uw);
-    HIPCHK(wb_set_smem_limit(S.wb_smem_prep, S.wb_smem_upd));
-  }
 
   // h0 and messages of the moved electron (ElecInit on one electron per walker)
   const int b = h->iEN;
-  S.kT5.ensure(sizeof(float) * (size_t)B * F);
-  S.kT5b.ensure(sizeof(float) * (size_t)B * F);
-  S.kZ2.ensure(sizeof(float) * (size_t)B * 2 * F);
-  S.p_M5k.ensure(sizeof(float) * (size_t)B * 5 * F);
   launch_mc_klist(c.s, B, k, S.klist.as<int2>());
   launch_elec_init_en(c.s, S.p_r.as<float>(), h->R, B, N, nu, h->M, h->cfg.cutoff_1el, F, en_params(h), false,
                       S.kT5.as<float>(), S.klist.as<int2>(), B);
@@ -768,37 +832,26 @@ void state_propose(Ctx& c, int k, int mode, const float* rk_in, float width, uin
   gemm(cc, S.kT5.as<float>(), F, b + 14, b + 15, S.p_M5k.as<float>(), 5 * F, B, 5 * F, F, nullptr, 1);
 
   // neighbour lists of the affected centres at the new positions, messages, update MLP
-  S.pcnt.ensure(sizeof(int32_t) * T);
-  S.pptr.ensure(sizeof(int64_t) * (T + 2));
-  S.pnbr.ensure(sizeof(int32_t) * (size_t)T * std::max(N - 1, 1));
-  launch_nbr_count(c.s, S.p_r.as<float>(), B, N, nu, h->cfg.cutoff, S.pcnt.as<int32_t>(), S.list.as<int2>(), T);
-  launch_scan(c.s, S.pcnt.as<int32_t>(), T, S.pptr.as<int64_t>());
+  const DevRows rT{dT, nullptr};
+  launch_nbr_count(c.s, S.p_r.as<float>(), B, N, nu, h->cfg.cutoff, S.pcnt.as<int32_t>(), S.list.as<int2>(), (int)T,
+                   dT);
+  launch_scan(c.s, S.pcnt.as<int32_t>(), (int)T, S.pptr.as<int64_t>());
   launch_nbr_fill(c.s, S.p_r.as<float>(), B, N, nu, h->cfg.cutoff, S.pptr.as<int64_t>(), S.pnbr.as<int32_t>(),
-                  S.list.as<int2>(), T);
-  const int XW = std::max(F, std::max(W, 2 * HD));
-  S.p_hc.ensure(sizeof(float) * (size_t)T * F);
-  S.p_x.ensure(sizeof(float) * (size_t)T * XW);
-  S.p_y.ensure(sizeof(float) * (size_t)T * XW);
-  S.p_h.ensure(sizeof(float) * (size_t)T * F);
+                  S.list.as<int2>(), (int)T, dT);
   launch_messages(c.s, S.p_r.as<float>(), B, N, nu, S.pptr.as<int64_t>(), S.pnbr.as<int32_t>(), S.M5.as<float>(),
                   edge_params(h, h->iEEs), edge_params(h, h->iEEd), h->cut_learned, F, false, S.p_hc.as<float>(),
-                  S.list.as<int2>(), T, k, S.p_M5k.as<float>());
+                  S.list.as<int2>(), (int)T, k, S.p_M5k.as<float>(), dT);
   const int m = h->iMLP;
-  gemm(cc, S.p_hc.as<float>(), F, m + 0, m + 1, S.p_x.as<float>(), F, T, F, F, nullptr, 1);
-  launch_act(c.s, S.p_x.as<float>(), F, F, T, g1, ACT_SILU, 0.f);
-  gemm(cc, S.p_x.as<float>(), F, m + 2, m + 3, S.p_h.as<float>(), F, T, F, F, nullptr, 1);
-  launch_act(c.s, S.p_h.as<float>(), F, F, T, g1, ACT_SILU, 0.f);
+  gemm(cc, S.p_hc.as<float>(), F, m + 0, m + 1, S.p_x.as<float>(), F, T, F, F, nullptr, 1, rT);
+  launch_act(c.s, S.p_x.as<float>(), F, F, (int)T, g1, ACT_SILU, 0.f, dT);
+  gemm(cc, S.p_x.as<float>(), F, m + 2, m + 3, S.p_h.as<float>(), F, T, F, F, nullptr, 1, rT);
+  launch_act(c.s, S.p_h.as<float>(), F, F, (int)T, g1, ACT_SILU, 0.f, dT);
 
   // orbital rows of the affected electrons (up-spin rows first), envelope row of electron k
   const int o = h->iORB;
-  S.p_orb.ensure(sizeof(float) * (size_t)T * ND);
-  if (T_up > 0) gemm(cc, S.p_h.as<float>(), F, o + 0, o + 1, S.p_orb.as<float>(), ND, T_up, ND, F, nullptr, 1);
-  if (T > T_up)
-    gemm(cc, S.p_h.as<float>() + (size_t)T_up * F, F, o + 2, o + 3, S.p_orb.as<float>() + (size_t)T_up * ND, ND,
-         T - T_up, ND, F, nullptr, 1);
+  gemm(cc, S.p_h.as<float>(), F, o + 0, o + 1, S.p_orb.as<float>(), ND, T, ND, F, nullptr, 1, DevRows{dT_up, nullptr});
+  gemm(cc, S.p_h.as<float>(), F, o + 2, o + 3, S.p_orb.as<float>(), ND, T, ND, F, nullptr, 1, DevRows{dT_dn, dT_up});
   const int ME = h->M * h->Eiso;
-  S.kcoef.ensure(sizeof(float) * (size_t)D * B * ME);
-  S.p_envk.ensure(sizeof(float) * (size_t)B * ND);
   launch_env_coef(c.s, S.p_r.as<float>(), h->R, B, N, nu, h->M, D, h->Eiso, h->P[o + 4], h->P[o + 6], false,
                   S.kcoef.as<float>(), S.klist.as<int2>(), B);
   {
@@ -810,52 +863,40 @@ void state_propose(Ctx& c, int k, int mode, const float* rk_in, float width, uin
   }
 
   // determinant ratios (Woodbury, FP64)
-  S.phin.ensure(sizeof(double) * (size_t)B * D * std::max(amax, 1) * N);
-  S.sinv.ensure(sizeof(double) * (size_t)B * D * std::max(amax, 1) * std::max(amax, 1));
-  S.ldS.ensure(sizeof(double) * (size_t)B * D);
-  S.sgS.ensure(sizeof(double) * (size_t)B * D);
-  S.wbscr.ensure(std::max<size_t>((size_t)B * D * wb_update_scratch(N, amax), 8));
   launch_wb_prep(c.s, S.p_orb.as<float>(), S.p_envk.as<float>(), S.env.as<float>(), S.A.as<double>(),
                  S.Ainv.as<double>(), S.aff.as<int32_t>(), S.cu.as<int32_t>(), S.cd.as<int32_t>(),
                  S.rowof.as<int32_t>(), B, N, nu, D, k, amax, S.phin.as<double>(), S.sinv.as<double>(),
                  S.ldS.as<double>(), S.sgS.as<double>());
 
   // Jastrow rows of the affected electrons and the per-walker sums
-  S.jsum.ensure(sizeof(double) * (size_t)B * 2);
   if (h->has_mlp) {
-    S.p_jr.ensure(sizeof(float) * (size_t)T * 2);
     const int j = h->iJM;
     float* cur = S.p_x.as<float>();
     float* nxt = S.p_y.as<float>();
-    gemm(cc, S.p_h.as<float>(), F, j + 0, j + 1, cur, W, T, W, F, nullptr, 1);
-    launch_act(c.s, cur, W, W, T, g1, ACT_SILU, 0.f);
+    gemm(cc, S.p_h.as<float>(), F, j + 0, j + 1, cur, W, T, W, F, nullptr, 1, rT);
+    launch_act(c.s, cur, W, W, (int)T, g1, ACT_SILU, 0.f, dT);
     for (int l = 1; l < h->depth; ++l) {
-      gemm(cc, cur, W, j + 2 * l, j + 2 * l + 1, nxt, W, T, W, W, nullptr, 1);
-      launch_act(c.s, nxt, W, W, T, g1, ACT_SILU, 0.f);
+      gemm(cc, cur, W, j + 2 * l, j + 2 * l + 1, nxt, W, T, W, W, nullptr, 1, rT);
+      launch_act(c.s, nxt, W, W, (int)T, g1, ACT_SILU, 0.f, dT);
       std::swap(cur, nxt);
     }
     const int jl = j + 2 * h->depth;
-    launch_jmlp_last(c.s, cur, W, T, g1, h->P[jl], h->P[jl + 1], h->P[jl + 2], h->P[jl + 3], S.p_jr.as<float>());
+    launch_jmlp_last(c.s, cur, W, (int)T, g1, h->P[jl], h->P[jl + 1], h->P[jl + 2], h->P[jl + 3], S.p_jr.as<float>(),
+                     dT);
   }
   const float* att_scale = nullptr;
   const float* att_zb = nullptr;
   if (h->has_att) {
     const int ac = h->iATT;
-    S.p_qv.ensure(sizeof(float) * (size_t)T * 2 * HD);
-    S.p_av.ensure(sizeof(float) * (size_t)T * nc);
-    gemm(cc, S.p_h.as<float>(), F, ac + 0, ac + 1, S.p_qv.as<float>(), 2 * HD, T, 2 * HD, F, nullptr, 1);
-    launch_att_group(c.s, S.p_qv.as<float>(), H, Dk, T, g1, h->P[ac + 2], false, S.p_av.as<float>());
+    gemm(cc, S.p_h.as<float>(), F, ac + 0, ac + 1, S.p_qv.as<float>(), 2 * HD, T, 2 * HD, F, nullptr, 1, rT);
+    launch_att_group(c.s, S.p_qv.as<float>(), H, Dk, (int)T, g1, h->P[ac + 2], false, S.p_av.as<float>(), dT);
   }
-  S.rin.ensure(sizeof(float) * (size_t)B * 2 * std::max(HD, 1));
   if (h->has_mlp || h->has_att)
     launch_mc_jsum(c.s, S.jr.as<float>(), S.p_jr.as<float>(), S.av.as<float>(), S.p_av.as<float>(),
                    S.rowof.as<int32_t>(), B, N, nu, H, Dk, h->has_mlp, h->has_att, S.jsum.as<double>(),
                    S.rin.as<float>());
-  S.att_out.ensure(sizeof(float) * (size_t)B * 2);
   if (h->has_att) {
     const int ac = h->iATT;
-    S.ry1.ensure(sizeof(float) * (size_t)B * W);
-    S.ry2.ensure(sizeof(float) * (size_t)B * W);
     float* cur = S.ry1.as<float>();
     float* nxt = S.ry2.as<float>();
     gemm(cc, S.rin.as<float>(), 2 * HD, ac + 3, ac + 4, cur, W, B, W, 2 * HD, nullptr, 1);
@@ -871,15 +912,9 @@ void state_propose(Ctx& c, int k, int mode, const float* rk_in, float width, uin
     att_zb = h->P[al + 3];
   }
   // e-e cusp of the new positions (recomputed in full, as the reference does)
-  S.cval.ensure(sizeof(double) * (size_t)B * N);
-  S.clap.ensure(sizeof(double) * (size_t)B * N);
-  S.cgrad.ensure(sizeof(double) * (size_t)B * N * 3);
-  S.epot.ensure(sizeof(double) * (size_t)B * N);
   if (h->cusp)
     launch_cusp_coulomb(c.s, S.p_r.as<float>(), h->R, h->Z, B, N, nu, h->M, h->cusp, h->P[h->iCUSP],
                         S.cval.as<double>(), S.clap.as<double>(), S.cgrad.as<double>(), S.epot.as<double>());
-  for (DevBuf* x : {&S.p_ld, &S.p_sg}) x->ensure(sizeof(double) * (size_t)B * D);
-  for (DevBuf* x : {&S.p_logpsi, &S.p_sign}) x->ensure(sizeof(double) * B);
   launch_mc_accept(c.s, B, D, h->cfg.use_mlp_jastrow, h->cfg.use_log_jastrow, h->has_att, h->cusp, S.ld.as<double>(),
                    S.sg.as<double>(), S.ldS.as<double>(), S.sgS.as<double>(), S.jsum.as<double>(),
                    S.att_out.as<float>(), att_scale, att_zb, S.cval.as<double>(), N, S.logpsi.as<double>(),
@@ -896,16 +931,93 @@ void state_commit(Ctx& c) {
   const int B = S.B, N = h->N, nu = h->n_up, D = h->D;
   const int nc = h->H + h->HD;
   launch_wb_update(c.s, S.accept.as<int32_t>(), S.aff.as<int32_t>(), S.cu.as<int32_t>(), S.cd.as<int32_t>(), B, N, D,
-                   S.amax, S.phin.as<double>(), S.sinv.as<double>(), S.A.as<double>(), S.Ainv.as<double>(),
+                   S.cap, S.phin.as<double>(), S.sinv.as<double>(), S.A.as<double>(), S.Ainv.as<double>(),
                    S.wbscr.as<double>());
-  launch_mc_commit(c.s, S.accept.as<int32_t>(), S.list.as<int2>(), S.T, B, N, nu, D, h->F, h->ND, nc, S.k,
-                   S.p_h.as<float>(), S.p_orb.as<float>(), S.p_jr.as<float>(), S.p_av.as<float>(), S.p_rk.as<float>(),
-                   S.p_M5k.as<float>(), S.p_envk.as<float>(), S.p_ld.as<double>(), S.p_sg.as<double>(),
-                   S.p_logpsi.as<double>(), S.p_sign.as<double>(), S.hh.as<float>(), S.orb.as<float>(),
-                   h->has_mlp ? S.jr.as<float>() : nullptr, h->has_att ? S.av.as<float>() : nullptr,
-                   S.r.as<float>(), S.M5.as<float>(), S.env.as<float>(), S.ld.as<double>(), S.sg.as<double>(),
-                   S.logpsi.as<double>(), S.sign.as<double>(), S.p_r.as<float>());
+  launch_mc_commit(c.s, S.accept.as<int32_t>(), S.list.as<int2>(), (int)S.tcap, S.tail.as<int32_t>() + 3, B, N, nu, D,
+                   h->F, h->ND, nc, S.k, S.p_h.as<float>(), S.p_orb.as<float>(), S.p_jr.as<float>(),
+                   S.p_av.as<float>(), S.p_rk.as<float>(), S.p_M5k.as<float>(), S.p_envk.as<float>(),
+                   S.p_ld.as<double>(), S.p_sg.as<double>(), S.p_logpsi.as<double>(), S.p_sign.as<double>(),
+                   S.hh.as<float>(), S.orb.as<float>(), h->has_mlp ? S.jr.as<float>() : nullptr,
+                   h->has_att ? S.av.as<float>() : nullptr, S.r.as<float>(), S.M5.as<float>(), S.env.as<float>(),
+                   S.ld.as<double>(), S.sg.as<double>(), S.logpsi.as<double>(), S.sign.as<double>(),
+                   S.p_r.as<float>());
   S.pending = false;
+}
+
+// Raise if a proposal since init changed more electrons than the static capacity (the
+// affected set was truncated and the state is invalid).  Synchronises the stream.
+void state_check_overflow(Ctx& c) {
+  auto& S = c.h->st;
+  int32_t ov = 0;
+  HIPCHK(hipMemcpyAsync(&ov, S.doverflow.p, sizeof(int32_t), hipMemcpyDeviceToHost, c.s));
+  HIPCHK(hipStreamSynchronize(c.s));
+  REQUIRE(ov == 0, "a move changed more than " + std::to_string(S.cap) +
+                       " electrons (static capacity of the low-rank update); the walker state is invalid");
+}
+
+// n_steps Metropolis steps from step0.  Whole sweeps starting at a multiple of n_el run as
+// replays of one captured hipGraph (the step counter advances on the device); other steps
+// are issued one by one.
+void state_steps(Ctx& c, int64_t n_steps, float width, uint64_t seed, int64_t step0, bool use_graph) {
+  fire_handle* h = c.h;
+  auto& S = h->st;
+  const int N = h->N;
+  state_drop_pending(c);
+  launch_set_i64(c.s, S.dstep.as<int64_t>(), 0);
+  int64_t s = step0;
+  const int64_t end = step0 + n_steps;
+  // leading partial sweep
+  while (s < end && (!use_graph || s % N != 0)) {
+    state_propose(c, (int)(s % N), 0, nullptr, width, seed, (uint64_t)s, S.nacc.as<int32_t>());
+    state_commit(c);
+    ++s;
+  }
+  const int64_t n_sweeps = (end - s) / N;
+  if (use_graph && n_sweeps > 0) {
+    // (re)capture when the sweep parameters change; nodes take the step counter from the device
+    if (!S.graph || S.g_width != width || S.g_seed != seed) {
+      if (S.graph) HIPCHK(hipGraphExecDestroy(S.graph));
+      S.graph = nullptr;
+      if (!S.gstream) HIPCHK(hipStreamCreateWithFlags(&S.gstream, hipStreamNonBlocking));
+      // one eager zero-width proposal first (dropped, the state is unchanged), so that every
+      // kernel attribute is set before the capture starts
+      state_propose(c, 0, 0, nullptr, 0.f, seed, 0, nullptr);
+      state_drop_pending(c);
+      S.n_proposals -= S.B;
+      Ctx ce{h, S.gstream, c.prec};
+      HIPCHK(hipStreamSynchronize(c.s));
+      HIPCHK(hipStreamBeginCapture(S.gstream, hipStreamCaptureModeRelaxed));
+      for (int kk = 0; kk < N; ++kk) {
+        state_propose(ce, kk, 0, nullptr, width, seed, (uint64_t)kk, S.nacc.as<int32_t>());
+        state_commit(ce);
+      }
+      launch_add_i64(S.gstream, S.dstep.as<int64_t>(), N);
+      hipGraph_t gr = nullptr;
+      HIPCHK(hipStreamEndCapture(S.gstream, &gr));
+      HIPCHK(hipGraphInstantiate(&S.graph, gr, nullptr, nullptr, 0));
+      HIPCHK(hipGraphDestroy(gr));
+      S.g_width = width;
+      S.g_seed = seed;
+    }
+    // the graph's proposals use step = *dstep + k with *dstep = first step of the sweep
+    launch_set_i64(c.s, S.dstep.as<int64_t>(), s);
+    hipEvent_t ev;
+    HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    HIPCHK(hipEventRecord(ev, c.s));
+    HIPCHK(hipStreamWaitEvent(S.gstream, ev, 0));
+    for (int64_t q = 0; q < n_sweeps; ++q) HIPCHK(hipGraphLaunch(S.graph, S.gstream));
+    HIPCHK(hipEventRecord(ev, S.gstream));
+    HIPCHK(hipStreamWaitEvent(c.s, ev, 0));
+    HIPCHK(hipEventDestroy(ev));
+    S.n_proposals += (int64_t)S.B * N * n_sweeps;
+    s += n_sweeps * N;
+    launch_set_i64(c.s, S.dstep.as<int64_t>(), 0);
+  }
+  while (s < end) {
+    state_propose(c, (int)(s % N), 0, nullptr, width, seed, (uint64_t)s, S.nacc.as<int32_t>());
+    state_commit(c);
+    ++s;
+  }
 }
 
 int fail(const std::exception& e) {
@@ -1105,6 +1217,11 @@ int fire_upload_params(fire_handle* h, const float* packed, const int64_t* sizes
     };
     for (auto& t : gw) upload_t(t.first, 1, t.second.first, t.second.second);
     for (int k : {h->iORB + 5, h->iORB + 7}) upload_t(k, h->D, h->M * h->Eiso, h->N);
+    // a captured sweep graph holds the old parameter pointers
+    if (h->st.graph) {
+      HIPCHK(hipGraphExecDestroy(h->st.graph));
+      h->st.graph = nullptr;
+    }
     h->cut_learned = packed[offs[h->iSCAL]];
     h->s1 = packed[offs[h->iSCAL] + 1];
     h->params_ok = true;
@@ -1351,11 +1468,10 @@ int fire_mcmc_steps(fire_handle* h, void* stream, int32_t n_steps, float width, 
     HIPCHK(hipSetDevice(h->device));
     Ctx c{h, (hipStream_t)stream, h->st.prec};
     REQUIRE(h->st.ready, "no walker state (call fire_state_init first)");
-    if (n_accept_dev) launch_fill_i32(c.s, n_accept_dev, h->st.B, 0);
-    for (int64_t s = step0; s < step0 + n_steps; ++s) {
-      state_propose(c, (int)(s % h->N), 0, nullptr, width, seed, (uint64_t)s, n_accept_dev);
-      state_commit(c);
-    }
+    launch_fill_i32(c.s, h->st.nacc.as<int32_t>(), h->st.B, 0);
+    state_steps(c, n_steps, width, seed, step0, h->use_graphs);
+    if (n_accept_dev)
+      HIPCHK(hipMemcpyAsync(n_accept_dev, h->st.nacc.p, sizeof(int32_t) * h->st.B, hipMemcpyDeviceToDevice, c.s));
     HIPCHK(hipGetLastError());
     return 0;
   } catch (const std::exception& e) {
@@ -1370,6 +1486,7 @@ int fire_state_get(fire_handle* h, void* stream, float* r_dev, double* logpsi_de
     HIPCHK(hipSetDevice(h->device));
     Ctx c{h, (hipStream_t)stream, h->st.prec};
     state_drop_pending(c);
+    state_check_overflow(c);
     const int B = h->st.B;
     if (r_dev)
       HIPCHK(hipMemcpyAsync(r_dev, h->st.r.p, sizeof(float) * (size_t)B * h->N * 3, hipMemcpyDeviceToDevice, c.s));
@@ -1413,6 +1530,16 @@ int fire_autotune(fire_handle* h, int32_t enable, int32_t* table, int32_t cap, i
   }
 }
 
+int fire_mcmc_options(fire_handle* h, int32_t use_graphs) {
+  try {
+    REQUIRE(h, "null handle");
+    h->use_graphs = use_graphs != 0;
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e);
+  }
+}
+
 int fire_precision_plan(fire_handle* h, const int32_t* planes, int32_t n, int32_t* current) {
   try {
     REQUIRE(h, "null handle");
@@ -1420,6 +1547,10 @@ int fire_precision_plan(fire_handle* h, const int32_t* planes, int32_t n, int32_
     for (int k = 0; k < n; ++k)
       REQUIRE(planes[k] >= 1 && planes[k] <= 3, "planes per operand must be 1, 2 or 3");
     for (int k = 0; k < n; ++k) h->planes[k] = planes[k];
+    if (n > 0 && h->st.graph) {  // the captured sweep holds the previous contraction choices
+      HIPCHK(hipGraphExecDestroy(h->st.graph));
+      h->st.graph = nullptr;
+    }
     if (current)
       for (int k = 0; k < 6; ++k) current[k] = h->planes[k];
     return 0;
@@ -1434,8 +1565,14 @@ int fire_state_stats(fire_handle* h, int32_t* n_walkers, int64_t* n_proposals, i
     REQUIRE(h, "null handle");
     if (n_walkers) *n_walkers = h->st.ready ? h->st.B : 0;
     if (n_proposals) *n_proposals = h->st.n_proposals;
-    if (sum_changed) *sum_changed = h->st.sum_changed;
-    if (max_changed) *max_changed = h->st.max_changed;
+    int64_t st[2] = {0, 0};
+    if (h->st.ready && (sum_changed || max_changed)) {
+      HIPCHK(hipSetDevice(h->device));
+      HIPCHK(hipDeviceSynchronize());
+      HIPCHK(hipMemcpy(st, h->st.dstats.p, sizeof(st), hipMemcpyDeviceToHost));
+    }
+    if (sum_changed) *sum_changed = st[0];
+    if (max_changed) *max_changed = st[1];
     return 0;
   } catch (const std::exception& e) {
     return fail(e);

@@ -32,10 +32,11 @@ struct EdgeParams {  // ElecElecEdges (new_sparse_model.py:161-187)
 enum Act { ACT_SILU = 0, ACT_EXP_NEG_SCALED = 1 };
 
 // graph
+// list mode: n_list = capacity of the list; dn (optional, device) = entries actually in use
 void launch_nbr_count(hipStream_t s, const float* r, int B, int N, int n_up, float cutoff, int32_t* cnt,
-                      const int2* list = nullptr, int n_list = 0);
+                      const int2* list = nullptr, int n_list = 0, const int32_t* dn = nullptr);
 void launch_nbr_fill(hipStream_t s, const float* r, int B, int N, int n_up, float cutoff, const int64_t* ptr,
-                     int32_t* nbr, const int2* list = nullptr, int n_list = 0);
+                     int32_t* nbr, const int2* list = nullptr, int n_list = 0, const int32_t* dn = nullptr);
 void launch_scan(hipStream_t s, const int32_t* cnt, int G, int64_t* ptr);
 void launch_rev(hipStream_t s, const int64_t* ptr, const int32_t* nbr, int B, int N, int n_up, int32_t* rev);
 
@@ -44,24 +45,36 @@ void launch_rev(hipStream_t s, const int64_t* ptr, const int32_t* nbr, int B, in
 void launch_elec_init_en(hipStream_t s, const float* r, const float* R, int B, int N, int n_up, int M, float cutoff_1el,
                          int F, EnParams P, bool tangents, float* out, const int2* list = nullptr, int n_list = 0);
 void launch_glu(hipStream_t s, const float* Z, int G, int F, bool tangents, float* out);
-void launch_act(hipStream_t s, float* Y, int64_t ld, int ncols, int G, GroupDesc gd, int act, float act_scale);
+// dG (optional, device): groups actually in use (G is the launch capacity)
+void launch_act(hipStream_t s, float* Y, int64_t ld, int ncols, int G, GroupDesc gd, int act, float act_scale,
+                const int32_t* dG = nullptr);
 // list mode: centres list[0..n_list) with CSR ptr/nbr over list positions; the message
 // rows of electron `kmove` of each walker are read from M5k[walker] (value rows only)
 void launch_messages(hipStream_t s, const float* r, int B, int N, int n_up, const int64_t* ptr, const int32_t* nbr,
                      const float* M5, EdgeParams same, EdgeParams diff, float cut, int F, bool tangents, float* out,
-                     const int2* list = nullptr, int n_list = 0, int kmove = -1, const float* M5k = nullptr);
+                     const int2* list = nullptr, int n_list = 0, int kmove = -1, const float* M5k = nullptr,
+                     const int32_t* dn = nullptr);
 void launch_scale(hipStream_t s, float* p, int64_t n, float a);
 void launch_rowflag(hipStream_t s, int G, GroupDesc gd, int64_t n_rows, uint8_t* flag);
+
+// Optional device-side row range of a contraction (for launches sized by a capacity, e.g. under
+// stream capture): rows [*offset, *offset + min(M, *count)) of A / C / rowflag; null = 0 / M.
+struct DevRows {
+  const int32_t* count = nullptr;
+  const int32_t* offset = nullptr;
+};
 
 // dense contractions: C[M,N] = A[M,K] W[K,N] (+ bias on rows with flag / row % period == 0)
 // W is [K,N] (used for N <= 16), Wt its transpose [N,K] (used by the MFMA kernel)
 void launch_gemm_f32(hipStream_t s, const float* A, int64_t lda, const float* W, const float* Wt, const float* bias,
-                     float* C, int64_t ldc, int64_t M, int N, int K, const uint8_t* rowflag, int period);
+                     float* C, int64_t ldc, int64_t M, int N, int K, const uint8_t* rowflag, int period,
+                     DevRows dr = DevRows());
 // nsplit 1: plain BF16; 3: W = hi + lo planes, 3 products; 6: W = hi + mid + lo, 6 products
 // variant: tile shape index in [0, gemm_bf16_variants()) (0 = 128 x 128)
 void launch_gemm_bf16(hipStream_t s, const float* A, int64_t lda, const uint16_t* Whi /*[N,K] bf16*/,
                       const uint16_t* Wlo, const uint16_t* Wlo2, const float* bias, float* C, int64_t ldc, int64_t M,
-                      int N, int K, const uint8_t* rowflag, int period, int nsplit, int variant = 0);
+                      int N, int K, const uint8_t* rowflag, int period, int nsplit, int variant = 0,
+                      DevRows dr = DevRows());
 int gemm_bf16_variants();
 void gemm_bf16_tile(int variant, int* bm, int* bn);
 

@@ -30,14 +30,14 @@ void launch_det_combine(hipStream_t s, int B, int N, int D, const double* logdet
 
 // Jastrow
 void launch_jmlp_last(hipStream_t s, const float* Y, int W, int G, GroupDesc gd, const float* W3, const float* b3,
-                      const float* scale, const float* zb, float* JR);
+                      const float* scale, const float* zb, float* JR, const int32_t* dG = nullptr);
 void launch_node_sums(hipStream_t s, const float* X, int64_t ld, int ncols, GroupDesc gd, int B, int N, int n_up,
                       double* out /*[B][2 spin][2 (value, lap)][ncols]*/);
 void launch_dep_gather(hipStream_t s, const float* X, int64_t ld, int ncols, GroupDesc gd, const int64_t* ptr,
                        const int32_t* nbr, const int32_t* rev, int B, int N, int n_up,
                        double* out /*[B][2 spin][N][3][ncols]*/);
 void launch_att_group(hipStream_t s, const float* QV, int H, int Dk, int G, GroupDesc gd, const float* keys,
-                      bool tangents, float* AV);
+                      bool tangents, float* AV, const int32_t* dG = nullptr);
 void launch_att_readin(hipStream_t s, const double* NS, const double* DG, int B, int N, int H, int Dk, bool tangents,
                        float* RIN);
 void launch_cusp_coulomb(hipStream_t s, const float* r, const float* R, const float* Z, int B, int N, int n_up, int M,

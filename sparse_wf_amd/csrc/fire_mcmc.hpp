@@ -5,12 +5,15 @@
 
 namespace fire {
 
+// step index of a Gaussian proposal = *dstep (if non-null) + step
 void launch_mc_propose(hipStream_t s, const float* st_r, int B, int N, int k, float width, uint64_t seed, uint64_t step,
-                       int mode, const float* rk_in, float* p_r, float* p_rk, double* p_u);
+                       const int64_t* dstep, int mode, const float* rk_in, float* p_r, float* p_rk, double* p_u);
+void launch_add_i64(hipStream_t s, int64_t* p, int64_t v);
+void launch_set_i64(hipStream_t s, int64_t* p, int64_t v);
 void launch_mc_affected(hipStream_t s, const float* st_r, const float* p_r, int B, int N, int n_up, int k, float cutoff,
-                        int32_t* aff, int32_t* cnt_up, int32_t* cnt_dn);
+                        int cap, int32_t* aff, int32_t* cnt_up, int32_t* cnt_dn, int32_t* overflow);
 void launch_mc_scan(hipStream_t s, const int32_t* cu, const int32_t* cd, int B, int32_t* off_up, int32_t* off_dn,
-                    int32_t* tail);
+                    int32_t* tail, int64_t* stats);
 void launch_mc_rowlist(hipStream_t s, const int32_t* aff, const int32_t* cu, const int32_t* cd, const int32_t* off_up,
                        const int32_t* off_dn, const int32_t* tail, int B, int N, int n_up, int2* list, int32_t* rowof);
 void launch_mc_klist(hipStream_t s, int B, int k, int2* list);
@@ -35,7 +38,8 @@ void launch_mc_accept(hipStream_t s, int B, int D, int use_mlp, int use_log, int
                       const float* att_out, const float* att_scale, const float* att_zb, const double* cval, int N,
                       const double* st_logpsi, const double* p_u, int mode, double* p_ld, double* p_sg,
                       double* p_logpsi, double* p_sign, int32_t* accept, int32_t* n_acc);
-void launch_mc_commit(hipStream_t s, const int32_t* accept, const int2* list, int T, int B, int N, int n_up, int D,
+void launch_mc_commit(hipStream_t s, const int32_t* accept, const int2* list, int T, const int32_t* dT, int B, int N,
+                      int n_up, int D,
                       int F, int ND, int nav, int k, const float* p_h, const float* p_orb, const float* p_jr,
                       const float* p_av, const float* p_rk, const float* p_M5k, const float* p_envk, const double* p_ld,
                       const double* p_sg, const double* p_logpsi, const double* p_sign, float* st_h, float* st_orb,

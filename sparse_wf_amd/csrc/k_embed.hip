@@ -191,8 +191,10 @@ void launch_glu(hipStream_t s, const float* Z, int G, int F, bool tangents, floa
 }
 
 // In-place elementwise activation with the forward-Laplacian rule on every group.
-__global__ void k_act(float* __restrict__ Y, int64_t ld, int ncols, GroupDesc gd, int act, float sc) {
+__global__ void k_act(float* __restrict__ Y, int64_t ld, int ncols, GroupDesc gd, int act, float sc,
+                      const int32_t* __restrict__ dG) {
   const int g = blockIdx.x;
+  if (dG && g >= *dG) return;
   const int c = blockIdx.y * blockDim.x + threadIdx.x;
   if (c >= ncols) return;
   const int64_t r0 = gd.row(g), r1 = gd.row(g + 1);
@@ -218,10 +220,11 @@ __global__ void k_act(float* __restrict__ Y, int64_t ld, int ncols, GroupDesc gd
   y[ld] = y[ld] * f1 + f2 * ss;
 }
 
-void launch_act(hipStream_t s, float* Y, int64_t ld, int ncols, int G, GroupDesc gd, int act, float act_scale) {
+void launch_act(hipStream_t s, float* Y, int64_t ld, int ncols, int G, GroupDesc gd, int act, float act_scale,
+                const int32_t* dG) {
   if (G == 0) return;
   dim3 grid(G, (ncols + 255) / 256);
-  k_act<<<grid, 256, 0, s>>>(Y, ld, ncols, gd, act, act_scale);
+  k_act<<<grid, 256, 0, s>>>(Y, ld, ncols, gd, act, act_scale, dG);
 }
 
 __global__ void k_rowflag(int G, GroupDesc gd, uint8_t* flag) {
@@ -256,9 +259,11 @@ __global__ void __launch_bounds__(256) k_messages(const float* __restrict__ r, i
                                                    const float* __restrict__ M5, EdgeParams Ps, EdgeParams Pd,
                                                    float cut, int F, float* __restrict__ out,
                                                    const int2* __restrict__ list, int kmove,
-                                                   const float* __restrict__ M5k) {
+                                                   const float* __restrict__ M5k, const int32_t* __restrict__ dn) {
   // list mode (value rows only): centres are list[g] = (walker, electron); the h0/message
-  // rows of electron `kmove` of every walker come from M5k[walker] instead of M5.
+  // rows of electron `kmove` of every walker come from M5k[walker] instead of M5; dn
+  // (optional, device) is the number of list entries in use.
+  if (list && dn && (int)blockIdx.x >= *dn) return;
   __shared__ float s_beta[MSG_CHUNK][5][W1];
   __shared__ float s_g[MSG_CHUNK][5][4];
   __shared__ const float* s_src[MSG_CHUNK];
@@ -369,15 +374,15 @@ __global__ void __launch_bounds__(256) k_messages(const float* __restrict__ r, i
 
 void launch_messages(hipStream_t s, const float* r, int B, int N, int n_up, const int64_t* ptr, const int32_t* nbr,
                      const float* M5, EdgeParams same, EdgeParams diff, float cut, int F, bool tangents, float* out,
-                     const int2* list, int n_list, int kmove, const float* M5k) {
+                     const int2* list, int n_list, int kmove, const float* M5k, const int32_t* dn) {
   const int G = list ? n_list : B * N;
   if (G == 0) return;
   if (tangents) {
     W1_DISPATCH(same.filt.w1, k_messages<true, W1><<<G, F, 0, s>>>(r, B, N, n_up, ptr, nbr, M5, same, diff, cut, F,
-                                                                   out, list, kmove, M5k));
+                                                                   out, list, kmove, M5k, dn));
   } else {
     W1_DISPATCH(same.filt.w1, k_messages<false, W1><<<G, F, 0, s>>>(r, B, N, n_up, ptr, nbr, M5, same, diff, cut, F,
-                                                                    out, list, kmove, M5k));
+                                                                    out, list, kmove, M5k, dn));
   }
 }
 

@@ -95,13 +95,22 @@ template <bool VEC>
 __global__ void __launch_bounds__(256) k_gemm_f32(const float* __restrict__ A, int64_t lda,
                                                   const float* __restrict__ Wt, const float* __restrict__ bias,
                                                   float* __restrict__ C, int64_t ldc, int64_t M, int N, int K,
-                                                  const uint8_t* __restrict__ rowflag, int period, int nT) {
+                                                  const uint8_t* __restrict__ rowflag, int period, int nT,
+                                                  DevRows dr) {
   __shared__ __attribute__((aligned(16))) float As[GT_M][LDA_F];
   __shared__ __attribute__((aligned(16))) float Bs[GT_N][LDA_F];  // Bs[n][k]
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   int64_t mt;
   int nt;
   tile_of(mt, nt, nT);
+  if (dr.offset) {
+    const int64_t o = *dr.offset;
+    A += o * lda;
+    C += o * ldc;
+    if (rowflag) rowflag += o;
+  }
+  if (dr.count) M = M < (int64_t)*dr.count ? M : (int64_t)*dr.count;
+  if (mt * GT_M >= M) return;
   const int64_t m0 = mt * GT_M;
   const int n0 = nt * GT_N;
   const int wm = (wid >> 1) * 64, wn = (wid & 1) * 64;
@@ -180,7 +189,8 @@ __global__ void __launch_bounds__(BM * BN / 64) k_gemm_bf16(const float* __restr
                                                            const uint16_t* __restrict__ Wp2,
                                                            const float* __restrict__ bias, float* __restrict__ C,
                                                            int64_t ldc, int64_t M, int N, int K,
-                                                           const uint8_t* __restrict__ rowflag, int period, int nT) {
+                                                           const uint8_t* __restrict__ rowflag, int period, int nT,
+                                                           DevRows dr) {
   constexpr int NWN = BN / 64;
   constexpr int NT = (BM / 64) * NWN * 64;  // threads
   constexpr int A_PER = BM * 8 / NT;        // float4 of the A chunk per thread
@@ -193,6 +203,14 @@ __global__ void __launch_bounds__(BM * BN / 64) k_gemm_bf16(const float* __restr
   int64_t mt;
   int nt;
   tile_of(mt, nt, nT);
+  if (dr.offset) {
+    const int64_t o = *dr.offset;
+    A += o * lda;
+    C += o * ldc;
+    if (rowflag) rowflag += o;
+  }
+  if (dr.count) M = M < (int64_t)*dr.count ? M : (int64_t)*dr.count;
+  if (mt * BM >= M) return;
   const int64_t m0 = mt * BM;
   const int n0 = nt * BN;
   const int wm = (wid / NWN) * 64, wn = (wid % NWN) * 64;
@@ -303,7 +321,14 @@ __global__ void __launch_bounds__(BM * BN / 64) k_gemm_bf16(const float* __restr
 // Small-N FP32 GEMM (N <= 16): one wavefront per row, lanes across K.  W is [K,N].
 __global__ void k_gemm_smalln(const float* __restrict__ A, int64_t lda, const float* __restrict__ W,
                               const float* __restrict__ bias, float* __restrict__ C, int64_t ldc, int64_t M, int N,
-                              int K, const uint8_t* __restrict__ rowflag, int period) {
+                              int K, const uint8_t* __restrict__ rowflag, int period, DevRows dr) {
+  if (dr.offset) {
+    const int64_t o = *dr.offset;
+    A += o * lda;
+    C += o * ldc;
+    if (rowflag) rowflag += o;
+  }
+  if (dr.count) M = M < (int64_t)*dr.count ? M : (int64_t)*dr.count;
   const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
   if (row >= M) return;
@@ -325,19 +350,20 @@ __global__ void k_gemm_smalln(const float* __restrict__ A, int64_t lda, const fl
 }
 
 void launch_gemm_f32(hipStream_t s, const float* A, int64_t lda, const float* W, const float* Wt, const float* bias,
-                     float* C, int64_t ldc, int64_t M, int N, int K, const uint8_t* rowflag, int period) {
+                     float* C, int64_t ldc, int64_t M, int N, int K, const uint8_t* rowflag, int period, DevRows dr) {
   if (M == 0) return;
   if (N <= 16) {
     const int64_t threads = M * 64;
-    k_gemm_smalln<<<(unsigned)((threads + 255) / 256), 256, 0, s>>>(A, lda, W, bias, C, ldc, M, N, K, rowflag, period);
+    k_gemm_smalln<<<(unsigned)((threads + 255) / 256), 256, 0, s>>>(A, lda, W, bias, C, ldc, M, N, K, rowflag, period,
+                                                                      dr);
     return;
   }
   const int nT = (N + GT_N - 1) / GT_N;
   const int64_t blocks = ((M + GT_M - 1) / GT_M) * nT;
   if ((K % GT_K) == 0 && (lda % 4) == 0)
-    k_gemm_f32<true><<<(unsigned)blocks, 256, 0, s>>>(A, lda, Wt, bias, C, ldc, M, N, K, rowflag, period, nT);
+    k_gemm_f32<true><<<(unsigned)blocks, 256, 0, s>>>(A, lda, Wt, bias, C, ldc, M, N, K, rowflag, period, nT, dr);
   else
-    k_gemm_f32<false><<<(unsigned)blocks, 256, 0, s>>>(A, lda, Wt, bias, C, ldc, M, N, K, rowflag, period, nT);
+    k_gemm_f32<false><<<(unsigned)blocks, 256, 0, s>>>(A, lda, Wt, bias, C, ldc, M, N, K, rowflag, period, nT, dr);
 }
 
 // Double-buffered form: two LDS stages of KC (16 or 32) k each; chunk c is multiplied from
@@ -351,7 +377,8 @@ __global__ void __launch_bounds__(BM * BN / 64) k_gemm_bf16_db(const float* __re
                                                               const uint16_t* __restrict__ Wp2,
                                                               const float* __restrict__ bias, float* __restrict__ C,
                                                               int64_t ldc, int64_t M, int N, int K,
-                                                              const uint8_t* __restrict__ rowflag, int period, int nT) {
+                                                              const uint8_t* __restrict__ rowflag, int period, int nT,
+                                                              DevRows dr) {
   constexpr int NWN = BN / 64;
   constexpr int NT = (BM / 64) * NWN * 64;
   constexpr int LDH = KC + 8;                // halves per LDS row (16-byte aligned, conflict-free reads)
@@ -367,6 +394,14 @@ __global__ void __launch_bounds__(BM * BN / 64) k_gemm_bf16_db(const float* __re
   int64_t mt;
   int nt;
   tile_of(mt, nt, nT);
+  if (dr.offset) {
+    const int64_t o = *dr.offset;
+    A += o * lda;
+    C += o * ldc;
+    if (rowflag) rowflag += o;
+  }
+  if (dr.count) M = M < (int64_t)*dr.count ? M : (int64_t)*dr.count;
+  if (mt * BM >= M) return;
   const int64_t m0 = mt * BM;
   const int n0 = nt * BN;
   const int wm = (wid / NWN) * 64, wn = (wid % NWN) * 64;
@@ -480,7 +515,8 @@ void gemm_bf16_tile(int variant, int* bm, int* bn) {
 
 template <int NP, int BM, int BN>
 static void launch_tile(hipStream_t s, const float* A, int64_t lda, const uint16_t* const* W, const float* bias,
-                        float* C, int64_t ldc, int64_t M, int N, int K, const uint8_t* rowflag, int period) {
+                        float* C, int64_t ldc, int64_t M, int N, int K, const uint8_t* rowflag, int period,
+                        DevRows dr) {
   static bool attr_set = false;  // LDS above 64 KB must be enabled once per kernel
   const size_t lds = (size_t)NP * (BM + BN) * LDB_H * 2;
   if (!attr_set) {
@@ -492,12 +528,13 @@ static void launch_tile(hipStream_t s, const float* A, int64_t lda, const uint16
   const int nT = (N + BN - 1) / BN;
   const int64_t blocks = ((M + BM - 1) / BM) * nT;
   k_gemm_bf16<NP, BM, BN><<<(unsigned)blocks, BM * BN / 64, lds, s>>>(A, lda, W[0], W[1], W[2], bias, C, ldc, M, N, K,
-                                                                     rowflag, period, nT);
+                                                                     rowflag, period, nT, dr);
 }
 
 template <int NP, int BM, int BN, int KC>
 static void launch_tile_db(hipStream_t s, const float* A, int64_t lda, const uint16_t* const* W, const float* bias,
-                           float* C, int64_t ldc, int64_t M, int N, int K, const uint8_t* rowflag, int period) {
+                           float* C, int64_t ldc, int64_t M, int N, int K, const uint8_t* rowflag, int period,
+                           DevRows dr) {
   static bool attr_set = false;
   const size_t lds = (size_t)2 * NP * (BM + BN) * (KC + 8) * 2;
   if (!attr_set) {
@@ -509,39 +546,39 @@ static void launch_tile_db(hipStream_t s, const float* A, int64_t lda, const uin
   const int nT = (N + BN - 1) / BN;
   const int64_t blocks = ((M + BM - 1) / BM) * nT;
   k_gemm_bf16_db<NP, BM, BN, KC><<<(unsigned)blocks, BM * BN / 64, lds, s>>>(A, lda, W[0], W[1], W[2], bias, C, ldc,
-                                                                           M, N, K, rowflag, period, nT);
+                                                                           M, N, K, rowflag, period, nT, dr);
 }
 
 template <int NP>
 static void launch_np(hipStream_t s, int variant, const float* A, int64_t lda, const uint16_t* const* W,
                       const float* bias, float* C, int64_t ldc, int64_t M, int N, int K, const uint8_t* rowflag,
-                      int period) {
+                      int period, DevRows dr) {
   switch (variant) {
-    case 7: launch_tile_db<NP, 128, 128, 16>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period); break;
-    case 8: launch_tile_db<NP, 128, 128, 32>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period); break;
-    case 9: launch_tile_db<NP, 256, 128, 16>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period); break;
-    case 10: launch_tile_db<NP, 128, 256, 16>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period); break;
-    case 1: launch_tile<NP, 128, 256>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period); break;
-    case 2: launch_tile<NP, 256, 128>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period); break;
-    case 3: launch_tile<NP, 64, 256>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period); break;
-    case 4: launch_tile<NP, 256, 64>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period); break;
-    case 5: launch_tile<NP, 64, 128>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period); break;
-    case 6: launch_tile<NP, 128, 64>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period); break;
-    default: launch_tile<NP, 128, 128>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period); break;
+    case 7: launch_tile_db<NP, 128, 128, 16>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
+    case 8: launch_tile_db<NP, 128, 128, 32>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
+    case 9: launch_tile_db<NP, 256, 128, 16>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
+    case 10: launch_tile_db<NP, 128, 256, 16>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
+    case 1: launch_tile<NP, 128, 256>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
+    case 2: launch_tile<NP, 256, 128>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
+    case 3: launch_tile<NP, 64, 256>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
+    case 4: launch_tile<NP, 256, 64>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
+    case 5: launch_tile<NP, 64, 128>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
+    case 6: launch_tile<NP, 128, 64>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
+    default: launch_tile<NP, 128, 128>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
   }
 }
 
 void launch_gemm_bf16(hipStream_t s, const float* A, int64_t lda, const uint16_t* Whi, const uint16_t* Wlo,
                       const uint16_t* Wlo2, const float* bias, float* C, int64_t ldc, int64_t M, int N, int K,
-                      const uint8_t* rowflag, int period, int nsplit, int variant) {
+                      const uint8_t* rowflag, int period, int nsplit, int variant, DevRows dr) {
   if (M == 0) return;
   const uint16_t* W[3] = {Whi, Wlo ? Wlo : Whi, Wlo2 ? Wlo2 : Whi};
   if (nsplit == 6)
-    launch_np<3>(s, variant, A, lda, W, bias, C, ldc, M, N, K, rowflag, period);
+    launch_np<3>(s, variant, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr);
   else if (nsplit == 3)
-    launch_np<2>(s, variant, A, lda, W, bias, C, ldc, M, N, K, rowflag, period);
+    launch_np<2>(s, variant, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr);
   else
-    launch_np<1>(s, variant, A, lda, W, bias, C, ldc, M, N, K, rowflag, period);
+    launch_np<1>(s, variant, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr);
 }
 
 }  // namespace fire

@@ -27,11 +27,16 @@ __device__ __forceinline__ bool in_cutoff(const float* ri, const float* rj, floa
 template <bool FILL>
 __global__ void __launch_bounds__(256) k_nbr(const float* __restrict__ r, int B, int N, int n_up, float cutoff,
                                              int32_t* __restrict__ cnt, const int64_t* __restrict__ ptr,
-                                             int32_t* __restrict__ nbr, const int2* __restrict__ list, int n_list) {
+                                             int32_t* __restrict__ nbr, const int2* __restrict__ list, int n_list,
+                                             const int32_t* __restrict__ dn) {
   const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
   const int n_centres = list ? n_list : B * N;
   if (wave >= n_centres) return;
+  if (list && dn && wave >= *dn) {  // beyond the device-side list length: empty row
+    if (!FILL && lane == 0) cnt[wave] = 0;
+    return;
+  }
   int w, i, g;
   if (list) {
     w = list[wave].x;
@@ -117,19 +122,19 @@ __global__ void k_rev(const int64_t* __restrict__ ptr, const int32_t* __restrict
 }
 
 void launch_nbr_count(hipStream_t s, const float* r, int B, int N, int n_up, float cutoff, int32_t* cnt,
-                      const int2* list, int n_list) {
+                      const int2* list, int n_list, const int32_t* dn) {
   const int waves = list ? n_list : B * N;
   if (waves == 0) return;
   const int blocks = (waves * 64 + 255) / 256;
-  k_nbr<false><<<blocks, 256, 0, s>>>(r, B, N, n_up, cutoff, cnt, nullptr, nullptr, list, n_list);
+  k_nbr<false><<<blocks, 256, 0, s>>>(r, B, N, n_up, cutoff, cnt, nullptr, nullptr, list, n_list, dn);
 }
 
 void launch_nbr_fill(hipStream_t s, const float* r, int B, int N, int n_up, float cutoff, const int64_t* ptr,
-                     int32_t* nbr, const int2* list, int n_list) {
+                     int32_t* nbr, const int2* list, int n_list, const int32_t* dn) {
   const int waves = list ? n_list : B * N;
   if (waves == 0) return;
   const int blocks = (waves * 64 + 255) / 256;
-  k_nbr<true><<<blocks, 256, 0, s>>>(r, B, N, n_up, cutoff, nullptr, ptr, nbr, list, n_list);
+  k_nbr<true><<<blocks, 256, 0, s>>>(r, B, N, n_up, cutoff, nullptr, ptr, nbr, list, n_list, dn);
 }
 
 void launch_scan(hipStream_t s, const int32_t* cnt, int G, int64_t* ptr) {

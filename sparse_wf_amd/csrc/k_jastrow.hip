@@ -14,8 +14,9 @@ namespace fire {
 __global__ void __launch_bounds__(256) k_jmlp_last(const float* __restrict__ Y, int W, GroupDesc gd,
                                                    const float* __restrict__ W3, const float* __restrict__ b3,
                                                    const float* __restrict__ scale, const float* __restrict__ zb,
-                                                   float* __restrict__ JR) {
+                                                   float* __restrict__ JR, const int32_t* __restrict__ dG) {
   const int g = blockIdx.x;
+  if (dG && g >= *dG) return;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int64_t r0 = gd.row(g), r1 = gd.row(g + 1);
   for (int64_t rr = r0 + wid; rr < r1; rr += blockDim.x >> 6) {
@@ -54,8 +55,9 @@ __global__ void __launch_bounds__(256) k_jmlp_last(const float* __restrict__ Y, 
 }
 
 void launch_jmlp_last(hipStream_t s, const float* Y, int W, int G, GroupDesc gd, const float* W3, const float* b3,
-                      const float* scale, const float* zb, float* JR) {
-  k_jmlp_last<<<G, 256, 0, s>>>(Y, W, gd, W3, b3, scale, zb, JR);
+                      const float* scale, const float* zb, float* JR, const int32_t* dG) {
+  if (G == 0) return;
+  k_jmlp_last<<<G, 256, 0, s>>>(Y, W, gd, W3, b3, scale, zb, JR, dG);
 }
 
 // Per walker and spin: sum over centre electrons of the value row and the Laplacian row.
@@ -123,8 +125,10 @@ void launch_dep_gather(hipStream_t s, const float* X, int64_t ld, int ncols, Gro
 // so every row is read once with coalesced loads.
 template <bool TAN>
 __global__ void __launch_bounds__(256) k_att_group(const float* __restrict__ QV, int H, int Dk, GroupDesc gd,
-                                                   const float* __restrict__ keys, float* __restrict__ AV) {
+                                                   const float* __restrict__ keys, float* __restrict__ AV,
+                                                   const int32_t* __restrict__ dG) {
   const int g = blockIdx.x;
+  if (dG && g >= *dG) return;
   const int HD = H * Dk;
   const int col = threadIdx.x;  // (h, k)
   const bool live = col < HD;
@@ -168,12 +172,13 @@ __global__ void __launch_bounds__(256) k_att_group(const float* __restrict__ QV,
 }
 
 void launch_att_group(hipStream_t s, const float* QV, int H, int Dk, int G, GroupDesc gd, const float* keys,
-                      bool tangents, float* AV) {
+                      bool tangents, float* AV, const int32_t* dG) {
+  if (G == 0) return;
   const int threads = ((H * Dk + 63) / 64) * 64;
   if (tangents)
-    k_att_group<true><<<G, threads, 0, s>>>(QV, H, Dk, gd, keys, AV);
+    k_att_group<true><<<G, threads, 0, s>>>(QV, H, Dk, gd, keys, AV, dG);
   else
-    k_att_group<false><<<G, threads, 0, s>>>(QV, H, Dk, gd, keys, AV);
+    k_att_group<false><<<G, threads, 0, s>>>(QV, H, Dk, gd, keys, AV, dG);
 }
 
 // Readout input u[s,h,k] = vals[s,h,k] / norm[s,h] with dense forward-Laplacian over the

@@ -53,14 +53,16 @@ __device__ inline void step_randoms(uint64_t seed, int w, uint64_t step, double 
 // Proposal: p_r = state positions with electron k moved.  mode 0: Gaussian step of width
 // `width` (proposal_single_electron); mode 1: new position given in rk_in [B,3].
 __global__ void k_mc_propose(const float* __restrict__ st_r, int B, int N, int k, float width, uint64_t seed,
-                             uint64_t step, int mode, const float* __restrict__ rk_in, float* __restrict__ p_r,
-                             float* __restrict__ p_rk, double* __restrict__ p_u) {
+                             uint64_t step, const int64_t* __restrict__ dstep, int mode,
+                             const float* __restrict__ rk_in, float* __restrict__ p_r, float* __restrict__ p_rk,
+                             double* __restrict__ p_u) {
   const int w = blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= B) return;
   float rk[3];
   if (mode == 0) {
     double n3[3], u;
-    step_randoms(seed, w, step, n3, u);
+    // step index = device base (advanced by the sweep itself under graph replay) + offset
+    step_randoms(seed, w, (dstep ? (uint64_t)*dstep : 0) + step, n3, u);
     for (int c = 0; c < 3; ++c) rk[c] = st_r[((size_t)w * N + k) * 3 + c] + (float)(n3[c] * (double)width);
     p_u[w] = u;
   } else {
@@ -74,9 +76,18 @@ __global__ void k_mc_propose(const float* __restrict__ st_r, int B, int N, int k
 }
 
 void launch_mc_propose(hipStream_t s, const float* st_r, int B, int N, int k, float width, uint64_t seed, uint64_t step,
-                       int mode, const float* rk_in, float* p_r, float* p_rk, double* p_u) {
-  k_mc_propose<<<(B + 63) / 64, 64, 0, s>>>(st_r, B, N, k, width, seed, step, mode, rk_in, p_r, p_rk, p_u);
+                       const int64_t* dstep, int mode, const float* rk_in, float* p_r, float* p_rk, double* p_u) {
+  k_mc_propose<<<(B + 63) / 64, 64, 0, s>>>(st_r, B, N, k, width, seed, step, dstep, mode, rk_in, p_r, p_rk, p_u);
 }
+
+__global__ void k_add_i64(int64_t* p, int64_t v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) p[0] += v;
+}
+__global__ void k_set_i64(int64_t* p, int64_t v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) p[0] = v;
+}
+void launch_add_i64(hipStream_t s, int64_t* p, int64_t v) { k_add_i64<<<1, 64, 0, s>>>(p, v); }
+void launch_set_i64(hipStream_t s, int64_t* p, int64_t v) { k_set_i64<<<1, 64, 0, s>>>(p, v); }
 
 __device__ __forceinline__ bool near32(const float* a, const float* b, float cutoff) {
   const float dx = __fsub_rn(a[0], b[0]), dy = __fsub_rn(a[1], b[1]), dz = __fsub_rn(a[2], b[2]);
@@ -88,8 +99,8 @@ __device__ __forceinline__ bool near32(const float* a, const float* b, float cut
 // is a neighbour of k before or after the move (get_low_rank_pair_indices :449-453),
 // ascending; counts split by spin.
 __global__ void k_mc_affected(const float* __restrict__ st_r, const float* __restrict__ p_r, int B, int N, int n_up,
-                              int k, float cutoff, int32_t* __restrict__ aff, int32_t* __restrict__ cnt_up,
-                              int32_t* __restrict__ cnt_dn) {
+                              int k, float cutoff, int cap, int32_t* __restrict__ aff, int32_t* __restrict__ cnt_up,
+                              int32_t* __restrict__ cnt_dn, int32_t* __restrict__ overflow) {
   const int w = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
   if (w >= B) return;
@@ -107,20 +118,30 @@ __global__ void k_mc_affected(const float* __restrict__ st_r, const float* __res
     count += __popcll(m);
   }
   if (lane == 0) {
+    // static capacity: at most `cap` changed electrons per move (an excess is flagged and
+    // truncated so that every buffer access stays in bounds; the step is then invalid)
+    int cd = count - cu;
+    if (count > cap) {
+      overflow[0] = 1;
+      cu = min(cu, cap);
+      cd = min(cd, cap - cu);
+    }
     cnt_up[w] = cu;
-    cnt_dn[w] = count - cu;
+    cnt_dn[w] = cd;
   }
 }
 
 void launch_mc_affected(hipStream_t s, const float* st_r, const float* p_r, int B, int N, int n_up, int k, float cutoff,
-                        int32_t* aff, int32_t* cnt_up, int32_t* cnt_dn) {
-  k_mc_affected<<<(B * 64 + 255) / 256, 256, 0, s>>>(st_r, p_r, B, N, n_up, k, cutoff, aff, cnt_up, cnt_dn);
+                        int cap, int32_t* aff, int32_t* cnt_up, int32_t* cnt_dn, int32_t* overflow) {
+  k_mc_affected<<<(B * 64 + 255) / 256, 256, 0, s>>>(st_r, p_r, B, N, n_up, k, cutoff, cap, aff, cnt_up, cnt_dn,
+                                                     overflow);
 }
 
-// Exclusive scans of the per-walker up/down counts; tail = {T_up, T_dn, max count}.
+// Exclusive scans of the per-walker up/down counts; tail = {T_up, T_dn, max count, T};
+// stats = {sum over steps of T, max over steps of the max count}.
 __global__ void __launch_bounds__(1024) k_mc_scan(const int32_t* __restrict__ cu, const int32_t* __restrict__ cd, int B,
                                                   int32_t* __restrict__ off_up, int32_t* __restrict__ off_dn,
-                                                  int32_t* __restrict__ tail) {
+                                                  int32_t* __restrict__ tail, int64_t* __restrict__ stats) {
   __shared__ int32_t su[1024], sd[1024], sm[1024];
   const int t = threadIdx.x;
   const int per = (B + 1023) / 1024;
@@ -154,12 +175,15 @@ __global__ void __launch_bounds__(1024) k_mc_scan(const int32_t* __restrict__ cu
     tail[0] = su[1023];
     tail[1] = sd[1023];
     tail[2] = sm[1023];
+    tail[3] = su[1023] + sd[1023];
+    stats[0] += su[1023] + sd[1023];
+    stats[1] = max(stats[1], (int64_t)sm[1023]);
   }
 }
 
 void launch_mc_scan(hipStream_t s, const int32_t* cu, const int32_t* cd, int B, int32_t* off_up, int32_t* off_dn,
-                    int32_t* tail) {
-  k_mc_scan<<<1, 1024, 0, s>>>(cu, cd, B, off_up, off_dn, tail);
+                    int32_t* tail, int64_t* stats) {
+  k_mc_scan<<<1, 1024, 0, s>>>(cu, cd, B, off_up, off_dn, tail, stats);
 }
 
 // Compact row list of the affected electrons: up-spin rows of all walkers first
@@ -328,12 +352,11 @@ __global__ void __launch_bounds__(256) k_wb_prep(const float* __restrict__ p_orb
 static size_t wb_prep_fixed(int amax) {
   return (size_t)amax * amax * 8 + (size_t)amax * 8 + (((size_t)amax * 4 + 7) & ~(size_t)7);
 }
-static const size_t kLdsCap = 160 * 1024 - 512;  // dynamic LDS usable next to the kernels' static LDS
-
-// Dynamic LDS of the preparation: with P staged when it fits, else without.
+// Dynamic LDS of the preparation: P staged when that keeps the block within 64 KB (several
+// blocks per CU), else P is read from Ainv.
 size_t wb_prep_smem(int N, int amax) {
   const size_t with_p = wb_prep_fixed(amax) + (size_t)N * amax * 8;
-  return with_p <= kLdsCap ? with_p : wb_prep_fixed(amax);
+  return with_p <= 64 * 1024 ? with_p : wb_prep_fixed(amax);
 }
 
 void launch_wb_prep(hipStream_t s, const float* p_orb, const float* p_envk, const float* st_env, const double* st_A,
@@ -389,6 +412,8 @@ __global__ void __launch_bounds__(256) k_wb_update_cols(const int32_t* __restric
     const int o = e / AMAX, j = e - o * AMAX;
     VT[e] = j < a ? V[(size_t)j * N + o] : 0.0;
   }
+  // the j loops below run in chunks of 16 and skip chunks beyond this walker's count a
+  // (uniform in the block), so the cost follows a, not the capacity AMAX
   for (int o = tid; o < N; o += blockDim.x) {
     double acc[AMAX];
 #pragma unroll
@@ -397,7 +422,11 @@ __global__ void __launch_bounds__(256) k_wb_update_cols(const int32_t* __restric
     for (int l = 0; l < a; ++l) {
       const double p = ar[s_af[l]];
 #pragma unroll
-      for (int j = 0; j < AMAX; ++j) acc[j] += p * SL[l * AMAX + j];
+      for (int ch = 0; ch < AMAX / 16; ++ch)
+        if (ch * 16 < a) {
+#pragma unroll
+          for (int u = 0; u < 16; ++u) acc[ch * 16 + u] += p * SL[l * AMAX + ch * 16 + u];
+        }
     }
 #pragma unroll
     for (int j = 0; j < AMAX; ++j) X[(size_t)o * AMAX + j] = j < a ? acc[j] : 0.0;
@@ -412,13 +441,21 @@ __global__ void __launch_bounds__(256) k_wb_update_cols(const int32_t* __restric
       const double x = Ai[(size_t)o * N + c];
       const double* v = VT + (size_t)o * AMAX;
 #pragma unroll
-      for (int j = 0; j < AMAX; ++j) q[j] += v[j] * x;
+      for (int ch = 0; ch < AMAX / 16; ++ch)
+        if (ch * 16 < a) {
+#pragma unroll
+          for (int u = 0; u < 16; ++u) q[ch * 16 + u] += v[ch * 16 + u] * x;
+        }
     }
     for (int o = 0; o < N; ++o) {
       const double* xr = X + (size_t)o * AMAX;
       double acc = 0.0;
 #pragma unroll
-      for (int j = 0; j < AMAX; ++j) acc += xr[j] * q[j];
+      for (int ch = 0; ch < AMAX / 16; ++ch)
+        if (ch * 16 < a) {
+#pragma unroll
+          for (int u = 0; u < 16; ++u) acc += xr[ch * 16 + u] * q[ch * 16 + u];
+        }
       Ai[(size_t)o * N + c] -= acc;
     }
   }
@@ -602,12 +639,13 @@ void launch_mc_accept(hipStream_t s, int B, int D, int use_mlp, int use_log, int
 }
 
 // Commit the compact per-row proposal buffers of accepted walkers into the state.
-__global__ void k_mc_commit_rows(const int32_t* __restrict__ accept, const int2* __restrict__ list, int B, int N,
-                                 int n_up, int F, int ND, int nav, const float* __restrict__ p_h,
-                                 const float* __restrict__ p_orb, const float* __restrict__ p_jr,
-                                 const float* __restrict__ p_av, float* __restrict__ st_h, float* __restrict__ st_orb,
-                                 float* __restrict__ st_jr, float* __restrict__ st_av) {
+__global__ void k_mc_commit_rows(const int32_t* __restrict__ accept, const int2* __restrict__ list,
+                                 const int32_t* __restrict__ dT, int B, int N, int n_up, int F, int ND, int nav,
+                                 const float* __restrict__ p_h, const float* __restrict__ p_orb,
+                                 const float* __restrict__ p_jr, const float* __restrict__ p_av, float* __restrict__ st_h,
+                                 float* __restrict__ st_orb, float* __restrict__ st_jr, float* __restrict__ st_av) {
   const int t = blockIdx.x;
+  if (dT && t >= *dT) return;
   const int w = list[t].x, i = list[t].y;
   if (!accept[w]) return;
   const size_t g = group_of(w, i, B, n_up, N - n_up);
@@ -647,15 +685,16 @@ __global__ void k_mc_commit_walker(const int32_t* __restrict__ accept, int B, in
   if (threadIdx.x < 3) p_r[((size_t)w * N + k) * 3 + threadIdx.x] = rk[threadIdx.x];
 }
 
-void launch_mc_commit(hipStream_t s, const int32_t* accept, const int2* list, int T, int B, int N, int n_up, int D,
+void launch_mc_commit(hipStream_t s, const int32_t* accept, const int2* list, int T, const int32_t* dT, int B, int N,
+                      int n_up, int D,
                       int F, int ND, int nav, int k, const float* p_h, const float* p_orb, const float* p_jr,
                       const float* p_av, const float* p_rk, const float* p_M5k, const float* p_envk, const double* p_ld,
                       const double* p_sg, const double* p_logpsi, const double* p_sign, float* st_h, float* st_orb,
                       float* st_jr, float* st_av, float* st_r, float* st_M5, float* st_env, double* st_ld,
                       double* st_sg, double* st_logpsi, double* st_sign, float* p_r) {
   if (T > 0)
-    k_mc_commit_rows<<<T, 256, 0, s>>>(accept, list, B, N, n_up, F, ND, nav, p_h, p_orb, p_jr, p_av, st_h, st_orb,
-                                       st_jr, st_av);
+    k_mc_commit_rows<<<T, 256, 0, s>>>(accept, list, dT, B, N, n_up, F, ND, nav, p_h, p_orb, p_jr, p_av, st_h,
+                                       st_orb, st_jr, st_av);
   k_mc_commit_walker<<<B, 256, 0, s>>>(accept, B, N, n_up, D, 5 * F, ND, k, p_rk, p_M5k, p_envk, p_ld, p_sg, p_logpsi,
                                        p_sign, st_r, st_M5, st_env, st_ld, st_sg, st_logpsi, st_sign, p_r);
 }

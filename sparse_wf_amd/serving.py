@@ -100,6 +100,7 @@ class Server:
             with torch.cuda.device(self.device), torch.cuda.stream(stream):
                 wf = FireWaveFunction(req.cfg, device=self.device, precision=req.precision)
                 wf._lib.fire_autotune(wf._h, 0, None, 0, None)  # fixed tiles: no timing syncs
+                wf.mcmc_options(False)  # steps issued eagerly: no stream capture from worker threads
                 wf.upload(req.params)
                 r = torch.as_tensor(req.walkers, device=f"cuda:{self.device}")
                 wf.log_psi_with_state(req.params, r, precision=req.precision)

@@ -325,6 +325,11 @@ class FireWaveFunction:
             raise ValueError("no resident state (call log_psi_with_state first)")
         return nw.value
 
+    def mcmc_options(self, use_graphs: bool = True):
+        """Replay whole sweeps from a captured hipGraph (default) or issue every step eagerly;
+        both give identical results."""
+        _lib.check(self._lib.fire_mcmc_options(self._h, int(bool(use_graphs))))
+
     def state_stats(self) -> dict:
         """Counters of the state's single-electron updates: proposals and the mean / maximum
         number of electrons whose embedding changed per move."""

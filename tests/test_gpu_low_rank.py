@@ -200,3 +200,34 @@ def test_alkane_updates_and_sweep(gpu_device):
     sg_f, lp_f = wf.signed(prm, r_end, precision="fp32")
     _close(stm["logpsi"], lp_f, 1e-8)
     assert 0.0 < stm["pmove"] < 1.0
+
+
+def test_sweep_graph_matches_eager(setup):
+    """Whole sweeps replayed from the captured hipGraph give bit-identical walkers, log psi
+    and acceptance counts to the same steps issued eagerly, over several replays with eager
+    work in between, and a run split at a sweep boundary equals one call."""
+    name, cfg, prm, wf, r = setup
+    r = walkers(cfg, 16, seed=31)
+    N = cfg.n_el
+    results = {}
+    for use_graphs in (False, True):
+        wf.mcmc_options(use_graphs)
+        out = []
+        (_, _), st = wf.log_psi_with_state(prm, r, precision="mixed")
+        for rep in range(3):
+            rr, s = wf.mcmc_steps_low_rank(prm, r, 2 * N, width=0.3, seed=77, step0=2 * N * rep, keep_state=True)
+            wf.signed(prm, rr, precision="fp32")  # eager work between replays
+            out.append((rr.clone(), s["logpsi"].clone(), s["n_accept"].clone()))
+        results[use_graphs] = out
+    wf.mcmc_options(True)
+    for (ra, la, na), (rb, lb, nb) in zip(results[False], results[True]):
+        assert torch.equal(ra, rb)
+        assert torch.equal(la, lb)
+        assert torch.equal(na, nb)
+    # split at a sweep boundary plus a partial sweep: same as one call
+    wf.log_psi_with_state(prm, r, precision="mixed")
+    r1, _ = wf.mcmc_steps_low_rank(prm, r, N + 3, width=0.3, seed=5, keep_state=True)
+    wf.log_psi_with_state(prm, r, precision="mixed")
+    r2, _ = wf.mcmc_steps_low_rank(prm, r, N, width=0.3, seed=5, keep_state=True)
+    r2, _ = wf.mcmc_steps_low_rank(prm, r2, 3, width=0.3, seed=5, step0=N, keep_state=True)
+    assert torch.equal(r1, r2)
