@@ -147,14 +147,13 @@ struct fire_handle {
     // compact rows of the affected electrons
     DevBuf pcnt, pptr, pnbr, p_hc, p_x, p_h, p_y, p_orb, p_jr, p_qv, p_av;
     // determinant and Jastrow of the proposal
-    DevBuf phin, sinv, ldS, sgS, jsum, rin, ry1, ry2, att_out, cval, clap, cgrad, epot;
+    DevBuf phin, sbuf, ldS, sgS, jsum, rin, ry1, ry2, att_out, cval, clap, cgrad, epot;
     DevBuf p_ld, p_sg, p_logpsi, p_sign, accept, wbscr;
-    size_t wb_smem_prep = 0, wb_smem_upd = 0;
     void release() {
       DevBuf* b[] = {&r,     &M5,    &hh,    &orb,     &env,   &A,      &Ainv,  &ld,    &sg,     &jr,    &av,
                      &logpsi, &sign, &p_r,   &p_rk,    &p_u,   &aff,    &cu,    &cd,    &offu,   &offd,  &tail,
                      &list,  &rowof, &klist, &kT5,     &kZ2,   &kT5b,   &p_M5k, &kcoef, &p_envk, &pcnt,  &pptr,
-                     &pnbr,  &p_hc,  &p_x,   &p_h,     &p_y,   &p_orb,  &p_jr,  &p_qv,  &p_av,   &phin,  &sinv,
+                     &pnbr,  &p_hc,  &p_x,   &p_h,     &p_y,   &p_orb,  &p_jr,  &p_qv,  &p_av,   &phin,  &sbuf,
                      &ldS,   &sgS,   &jsum,  &rin,     &ry1,   &ry2,    &att_out, &cval, &clap,  &cgrad, &epot,
                      &p_ld,  &p_sg,  &p_logpsi, &p_sign, &accept, &wbscr, &dstats, &doverflow, &dstep, &nacc};
       if (graph) (void)hipGraphExecDestroy(graph);
@@ -743,11 +742,11 @@ void state_init(Ctx& c, const float* r_dev, int B) {
   S.p_orb.ensure(sizeof(float) * (size_t)T * ND);
   S.kcoef.ensure(sizeof(float) * (size_t)D * B * ME);
   S.p_envk.ensure(sizeof(float) * (size_t)B * ND);
-  S.phin.ensure(sizeof(double) * (size_t)B * D * amax * N);
-  S.sinv.ensure(sizeof(double) * (size_t)B * D * amax * amax);
+  S.phin.ensure(sizeof(double) * (size_t)B * D * wb_vs_doubles(N));
+  S.sbuf.ensure(sizeof(double) * (size_t)B * D * 64 * 64);
   S.ldS.ensure(sizeof(double) * (size_t)B * D);
   S.sgS.ensure(sizeof(double) * (size_t)B * D);
-  S.wbscr.ensure(std::max<size_t>((size_t)B * D * wb_update_scratch(N, amax), 8));
+  S.wbscr.ensure((size_t)B * D * wb_update_scratch(N));
   S.jsum.ensure(sizeof(double) * (size_t)B * 2);
   S.p_jr.ensure(sizeof(float) * (size_t)T * 2);
   S.p_qv.ensure(sizeof(float) * (size_t)T * 2 * std::max(HD, 1));
@@ -762,15 +761,6 @@ void state_init(Ctx& c, const float* r_dev, int B) {
   S.epot.ensure(sizeof(double) * (size_t)B * N);
   for (DevBuf* x : {&S.p_ld, &S.p_sg}) x->ensure(sizeof(double) * (size_t)B * D);
   for (DevBuf* x : {&S.p_logpsi, &S.p_sign}) x->ensure(sizeof(double) * B);
-  // dynamic LDS of the Woodbury kernels at the capacity (their static LDS comes on top)
-  const size_t pw = wb_prep_smem(N, amax), uw = wb_update_smem(N, amax);
-  REQUIRE(pw <= 160 * 1024 - 512 && uw <= 160 * 1024 - 512,
-          "the LDS-resident Woodbury update does not fit this electron count");
-  if ((pw > 64 * 1024 || uw > 64 * 1024) && (pw > S.wb_smem_prep || uw > S.wb_smem_upd)) {
-    S.wb_smem_prep = std::max(S.wb_smem_prep, pw);
-    S.wb_smem_upd = std::max(S.wb_smem_upd, uw);
-    HIPCHK(wb_set_smem_limit(S.wb_smem_prep, S.wb_smem_upd));
-  }
   launch_fill_i32(c.s, S.doverflow.as<int32_t>(), 1, 0);
   launch_fill_i32(c.s, reinterpret_cast<int32_t*>(S.dstats.p), 4, 0);
   launch_set_i64(c.s, S.dstep.as<int64_t>(), 0);
@@ -835,7 +825,7 @@ void state_propose(Ctx& c, int k, int mode, const float* rk_in, float width, uin
   const DevRows rT{dT, nullptr};
   launch_nbr_count(c.s, S.p_r.as<float>(), B, N, nu, h->cfg.cutoff, S.pcnt.as<int32_t>(), S.list.as<int2>(), (int)T,
                    dT);
-  launch_scan(c.s, S.pcnt.as<int32_t>(), (int)T, S.pptr.as<int64_t>());
+  launch_scan(c.s, S.pcnt.as<int32_t>(), (int)T, S.pptr.as<int64_t>(), dT);
   launch_nbr_fill(c.s, S.p_r.as<float>(), B, N, nu, h->cfg.cutoff, S.pptr.as<int64_t>(), S.pnbr.as<int32_t>(),
                   S.list.as<int2>(), (int)T, dT);
   launch_messages(c.s, S.p_r.as<float>(), B, N, nu, S.pptr.as<int64_t>(), S.pnbr.as<int32_t>(), S.M5.as<float>(),
@@ -865,8 +855,8 @@ void state_propose(Ctx& c, int k, int mode, const float* rk_in, float width, uin
   // determinant ratios (Woodbury, FP64)
   launch_wb_prep(c.s, S.p_orb.as<float>(), S.p_envk.as<float>(), S.env.as<float>(), S.A.as<double>(),
                  S.Ainv.as<double>(), S.aff.as<int32_t>(), S.cu.as<int32_t>(), S.cd.as<int32_t>(),
-                 S.rowof.as<int32_t>(), B, N, nu, D, k, amax, S.phin.as<double>(), S.sinv.as<double>(),
-                 S.ldS.as<double>(), S.sgS.as<double>());
+                 S.rowof.as<int32_t>(), B, N, nu, D, k, S.phin.as<double>(), S.sbuf.as<double>(), S.ldS.as<double>(),
+                 S.sgS.as<double>());
 
   // Jastrow rows of the affected electrons and the per-walker sums
   if (h->has_mlp) {
@@ -931,8 +921,8 @@ void state_commit(Ctx& c) {
   const int B = S.B, N = h->N, nu = h->n_up, D = h->D;
   const int nc = h->H + h->HD;
   launch_wb_update(c.s, S.accept.as<int32_t>(), S.aff.as<int32_t>(), S.cu.as<int32_t>(), S.cd.as<int32_t>(), B, N, D,
-                   S.cap, S.phin.as<double>(), S.sinv.as<double>(), S.A.as<double>(), S.Ainv.as<double>(),
-                   S.wbscr.as<double>());
+                   S.phin.as<double>(), S.sbuf.as<double>(), S.wbscr.as<double>(), S.A.as<double>(),
+                   S.Ainv.as<double>());
   launch_mc_commit(c.s, S.accept.as<int32_t>(), S.list.as<int2>(), (int)S.tcap, S.tail.as<int32_t>() + 3, B, N, nu, D,
                    h->F, h->ND, nc, S.k, S.p_h.as<float>(), S.p_orb.as<float>(), S.p_jr.as<float>(),
                    S.p_av.as<float>(), S.p_rk.as<float>(), S.p_M5k.as<float>(), S.p_envk.as<float>(),

@@ -42,22 +42,23 @@ __device__ __forceinline__ void silu3(float x, float& f, float& f1, float& f2) {
 // Output rows follow the FL5 convention.  cut_c: cutoff of the polynomial cutoff_function
 // (model/utils.py:103-112, p = 4).
 // W1 is the (zero-padded) filter width; every loop over it is unrolled so the per-thread
-// arrays stay in registers.
-template <int W1>
-__device__ inline void pair_filter_fl(const float diff[3], const FilterParams& P, float cut_c, float beta[5][W1],
-                                      float g[5][4]) {
+// arrays stay in registers.  NRF = 1 computes the value row only (value-only evaluation).
+template <int W1, int NRF = 5>
+__device__ inline void pair_filter_fl(const float diff[3], const FilterParams& P, float cut_c, float beta[NRF][W1],
+                                      float g[NRF][4]) {
+  static_assert(NRF == 1 || NRF == 5, "value row only, or all forward-Laplacian rows");
   const float dx = diff[0], dy = diff[1], dz = diff[2];
   const float d = sqrtf(dx * dx + dy * dy + dz * dz);
   const float inv_d = 1.0f / d;
   const float n[3] = {dx * inv_d, dy * inv_d, dz * inv_d};
   const float lap_d = 2.0f * inv_d;
 
-  float dir[5][W1];
-  float rad[5][W1];
+  float dir[NRF][W1];
+  float rad[NRF][W1];
 #pragma unroll
   for (int j = 0; j < W1; ++j) {
 #pragma unroll
-    for (int r = 0; r < 5; ++r) { dir[r][j] = 0.f; rad[r][j] = 0.f; }
+    for (int r = 0; r < NRF; ++r) { dir[r][j] = 0.f; rad[r][j] = 0.f; }
     dir[0][j] = P.bd[j];
   }
   // directional MLP: silu([d, diff] @ K + b) @ Wd + bd
@@ -74,7 +75,7 @@ __device__ inline void pair_filter_fl(const float diff[3], const FilterParams& P
     for (int j = 0; j < W1; ++j) {
       const float w = wrow[j];
 #pragma unroll
-      for (int r = 0; r < 5; ++r) dir[r][j] += a[r] * w;
+      for (int r = 0; r < NRF; ++r) dir[r][j] += a[r] * w;
     }
   }
   // radial envelopes exp(-(d/sp)^2) * cutoff(d / cut_c), then Dense (no bias)
@@ -101,33 +102,37 @@ __device__ inline void pair_filter_fl(const float diff[3], const FilterParams& P
     for (int j = 0; j < W1; ++j) {
       const float w = wrow[j];
 #pragma unroll
-      for (int r = 0; r < 5; ++r) rad[r][j] += a[r] * w;
+      for (int r = 0; r < NRF; ++r) rad[r][j] += a[r] * w;
     }
   }
 #pragma unroll
   for (int j = 0; j < W1; ++j) {
     const float dxv = dir[0][j], rx = rad[0][j];
     beta[0][j] = dxv * rx;
-    beta[1][j] = dir[1][j] * rx + dxv * rad[1][j] +
-                 2.0f * (dir[2][j] * rad[2][j] + dir[3][j] * rad[3][j] + dir[4][j] * rad[4][j]);
-    beta[2][j] = dir[2][j] * rx + dxv * rad[2][j];
-    beta[3][j] = dir[3][j] * rx + dxv * rad[3][j];
-    beta[4][j] = dir[4][j] * rx + dxv * rad[4][j];
+    if constexpr (NRF == 5) {
+      beta[1][j] = dir[1][j] * rx + dxv * rad[1][j] +
+                   2.0f * (dir[2][j] * rad[2][j] + dir[3][j] * rad[3][j] + dir[4][j] * rad[4][j]);
+      beta[2][j] = dir[2][j] * rx + dxv * rad[2][j];
+      beta[3][j] = dir[3][j] * rx + dxv * rad[3][j];
+      beta[4][j] = dir[4][j] * rx + dxv * rad[4][j];
+    }
   }
   // log-scaled features
   const float L = log1pf(d), L1 = 1.0f / (1.0f + d), L2 = -L1 * L1;
-  g[0][0] = L;
-  g[1][0] = L2 + L1 * lap_d;
-  g[2][0] = L1 * n[0];
-  g[3][0] = L1 * n[1];
-  g[4][0] = L1 * n[2];
   const float w = L * inv_d;
-  const float w1 = L1 * inv_d - L * inv_d * inv_d;
-  const float w2 = L2 * inv_d - 2.0f * L1 * inv_d * inv_d + 2.0f * L * inv_d * inv_d * inv_d;
-  for (int c = 0; c < 3; ++c) {
-    g[0][1 + c] = diff[c] * w;
-    g[1][1 + c] = 2.0f * w1 * n[c] + diff[c] * (w2 + w1 * lap_d);
-    for (int t = 0; t < 3; ++t) g[2 + t][1 + c] = (t == c ? w : 0.0f) + diff[c] * w1 * n[t];
+  g[0][0] = L;
+  for (int c = 0; c < 3; ++c) g[0][1 + c] = diff[c] * w;
+  if constexpr (NRF == 5) {
+    g[1][0] = L2 + L1 * lap_d;
+    g[2][0] = L1 * n[0];
+    g[3][0] = L1 * n[1];
+    g[4][0] = L1 * n[2];
+    const float w1 = L1 * inv_d - L * inv_d * inv_d;
+    const float w2 = L2 * inv_d - 2.0f * L1 * inv_d * inv_d + 2.0f * L * inv_d * inv_d * inv_d;
+    for (int c = 0; c < 3; ++c) {
+      g[1][1 + c] = 2.0f * w1 * n[c] + diff[c] * (w2 + w1 * lap_d);
+      for (int t = 0; t < 3; ++t) g[2 + t][1 + c] = (t == c ? w : 0.0f) + diff[c] * w1 * n[t];
+    }
   }
 }
 

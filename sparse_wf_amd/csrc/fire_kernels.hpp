@@ -37,7 +37,7 @@ void launch_nbr_count(hipStream_t s, const float* r, int B, int N, int n_up, flo
                       const int2* list = nullptr, int n_list = 0, const int32_t* dn = nullptr);
 void launch_nbr_fill(hipStream_t s, const float* r, int B, int N, int n_up, float cutoff, const int64_t* ptr,
                      int32_t* nbr, const int2* list = nullptr, int n_list = 0, const int32_t* dn = nullptr);
-void launch_scan(hipStream_t s, const int32_t* cnt, int G, int64_t* ptr);
+void launch_scan(hipStream_t s, const int32_t* cnt, int G, int64_t* ptr, const int32_t* dn = nullptr);
 void launch_rev(hipStream_t s, const int64_t* ptr, const int32_t* nbr, int B, int N, int n_up, int32_t* rev);
 
 // embedding

@@ -18,17 +18,19 @@ void launch_mc_rowlist(hipStream_t s, const int32_t* aff, const int32_t* cu, con
                        const int32_t* off_dn, const int32_t* tail, int B, int N, int n_up, int2* list, int32_t* rowof);
 void launch_mc_klist(hipStream_t s, int B, int k, int2* list);
 
-size_t wb_prep_smem(int N, int amax);
-size_t wb_update_smem(int N, int amax);     // dynamic LDS of the update (0 when amax <= 64)
-size_t wb_update_scratch(int N, int amax);  // global scratch per (walker, det) of the update
-hipError_t wb_set_smem_limit(size_t prep, size_t upd);
+// V operand tiles per (walker, det), doubles: 4 row tiles x ceil(N/4) k steps x 64
+__host__ __device__ inline size_t wb_vs_doubles(int N) { return (size_t)4 * 64 * ((N + 3) / 4); }
+// scratch bytes per (walker, det): X operand tiles, then Q operand tiles (generic update)
+__host__ __device__ inline size_t wb_update_scratch(int N) {
+  return (size_t)2 * ((N + 15) / 16) * 16 * 64 * sizeof(double);
+}
 void launch_wb_prep(hipStream_t s, const float* p_orb, const float* p_envk, const float* st_env, const double* st_A,
                     const double* st_Ainv, const int32_t* aff, const int32_t* cu, const int32_t* cd,
-                    const int32_t* rowof, int B, int N, int n_up, int D, int k, int amax, double* phin, double* sinv,
-                    double* ldS, double* sgS);
+                    const int32_t* rowof, int B, int N, int n_up, int D, int k, double* vs, double* sbuf, double* ldS,
+                    double* sgS);
 void launch_wb_update(hipStream_t s, const int32_t* accept, const int32_t* aff, const int32_t* cu, const int32_t* cd,
-                      int B, int N, int D, int amax, const double* vmat, const double* sinv, double* st_A,
-                      double* st_Ainv, double* scratch);
+                      int B, int N, int D, const double* vs, const double* sbuf, double* scratch, double* st_A,
+                      double* st_Ainv);
 
 void launch_mc_jsum(hipStream_t s, const float* st_jr, const float* p_jr, const float* st_av, const float* p_av,
                     const int32_t* rowof, int B, int N, int n_up, int H, int Dk, int has_mlp, int has_att,

@@ -20,8 +20,9 @@ template <bool TAN, int W1>
 __global__ void __launch_bounds__(256) k_elec_init_en(const float* __restrict__ r, const float* __restrict__ R, int B,
                                                        int N, int n_up, int M, float cutoff_1el, int F, EnParams P,
                                                        float* __restrict__ out, const int2* __restrict__ list) {
-  __shared__ float s_beta[EN_CHUNK][5][W1];
-  __shared__ float s_g[EN_CHUNK][5][4];
+  constexpr int NRF = TAN ? 5 : 1;  // forward-Laplacian rows staged per nucleus
+  __shared__ float s_beta[EN_CHUNK][NRF][W1];
+  __shared__ float s_g[EN_CHUNK][NRF][4];
   __shared__ int s_ok[EN_CHUNK];
   __shared__ float s_red[16 * 8];
   const int g = blockIdx.x;
@@ -60,12 +61,12 @@ __global__ void __launch_bounds__(256) k_elec_init_en(const float* __restrict__ 
         fp.w0 = P.w0;
         fp.w1 = P.w1;
         fp.E = P.E;
-        float beta[5][W1], gg[5][4];
-        pair_filter_fl<W1>(diff, fp, cutoff_1el, beta, gg);
+        float beta[NRF][W1], gg[NRF][4];
+        pair_filter_fl<W1, NRF>(diff, fp, cutoff_1el, beta, gg);
         const int slot = threadIdx.x;
         s_ok[slot] = 1;
 #pragma unroll
-        for (int rr = 0; rr < 5; ++rr) {
+        for (int rr = 0; rr < NRF; ++rr) {
 #pragma unroll
           for (int k = 0; k < W1; ++k) s_beta[slot][rr][k] = beta[rr][k];
 #pragma unroll
@@ -82,7 +83,7 @@ __global__ void __launch_bounds__(256) k_elec_init_en(const float* __restrict__ 
         const int bestJ = J0 + t;
         float G5[5], E5[5];
 #pragma unroll
-        for (int rr = 0; rr < 5; ++rr) {
+        for (int rr = 0; rr < NRF; ++rr) {
           float a = 0.f;
 #pragma unroll
           for (int k = 0; k < W1; ++k) a += s_beta[t][rr][k] * wg[k];
@@ -264,8 +265,9 @@ __global__ void __launch_bounds__(256) k_messages(const float* __restrict__ r, i
   // rows of electron `kmove` of every walker come from M5k[walker] instead of M5; dn
   // (optional, device) is the number of list entries in use.
   if (list && dn && (int)blockIdx.x >= *dn) return;
-  __shared__ float s_beta[MSG_CHUNK][5][W1];
-  __shared__ float s_g[MSG_CHUNK][5][4];
+  constexpr int NRF = TAN ? 5 : 1;  // forward-Laplacian rows staged per neighbour
+  __shared__ float s_beta[MSG_CHUNK][NRF][W1];
+  __shared__ float s_g[MSG_CHUNK][NRF][4];
   __shared__ const float* s_src[MSG_CHUNK];
   __shared__ int s_same[MSG_CHUNK];
   const int g = blockIdx.x;
@@ -311,10 +313,10 @@ __global__ void __launch_bounds__(256) k_messages(const float* __restrict__ r, i
       const int j = nbr[p0 + k];
       const float diff[3] = {rw[3 * i] - rw[3 * j], rw[3 * i + 1] - rw[3 * j + 1], rw[3 * i + 2] - rw[3 * j + 2]};
       const bool same = up_i == (j < n_up);
-      float beta[5][W1], gg[5][4];
-      pair_filter_fl<W1>(diff, same ? Ps.filt : Pd.filt, cut, beta, gg);
+      float beta[NRF][W1], gg[NRF][4];
+      pair_filter_fl<W1, NRF>(diff, same ? Ps.filt : Pd.filt, cut, beta, gg);
 #pragma unroll
-      for (int rr = 0; rr < 5; ++rr) {
+      for (int rr = 0; rr < NRF; ++rr) {
 #pragma unroll
         for (int q = 0; q < W1; ++q) s_beta[threadIdx.x][rr][q] = beta[rr][q];
 #pragma unroll
@@ -324,46 +326,60 @@ __global__ void __launch_bounds__(256) k_messages(const float* __restrict__ r, i
       s_same[threadIdx.x] = same;
     }
     __syncthreads();
-    for (int kk = 0; kk < cn; ++kk) {
-      const bool same = s_same[kk];
-      const float* m5j = s_src[kk] + (same ? 3 : 4) * F + f;
-      float G5[5], E5[5], mn[5], mc[5];
+    // neighbours in groups of U: the message rows of the whole group are loaded first
+    constexpr int U = TAN ? 2 : 8;
+    for (int k0 = 0; k0 < cn; k0 += U) {
+      float mnu[U][TAN ? 5 : 1];
 #pragma unroll
-      for (int rr = 0; rr < NR; ++rr) {
-        float a = 0.f;
+      for (int u = 0; u < U; ++u)
+        if (k0 + u < cn) {
+          const float* m5j = s_src[k0 + u] + (s_same[k0 + u] ? 3 : 4) * F + f;
 #pragma unroll
-        for (int q = 0; q < W1; ++q) a += s_beta[kk][rr][q] * (same ? wgs[q] : wgd[q]);
-        G5[rr] = a;
-        float b = 0.f;
-#pragma unroll
-        for (int m = 0; m < 4; ++m) b += s_g[kk][rr][m] * (same ? wfs[m] : wfd[m]);
-        E5[rr] = b;
-        mn[rr] = m5j[rr * 5 * F];
-        mc[rr] = same ? mcs[rr] : mcd[rr];
-      }
-      E5[0] += same ? bfs : bfd;
-      const float ux = E5[0] + mc[0] + mn[0];
-      float sv, s1, s2;
-      silu3(ux, sv, s1, s2);
-      acc[0] += G5[0] * sv;
-      if (TAN) {
-        float uc[3], un[3], q2 = 0.f;
-        for (int t = 0; t < 3; ++t) {
-          uc[t] = E5[2 + t] + mc[2 + t];
-          un[t] = -E5[2 + t] + mn[2 + t];
-          q2 += uc[t] * uc[t] + un[t] * un[t];
+          for (int rr = 0; rr < (TAN ? 5 : 1); ++rr) mnu[u][rr] = m5j[rr * 5 * F];
         }
-        const float ul = 2.f * E5[1] + mc[1] + mn[1];
-        const float vl = s1 * ul + s2 * q2;
-        float cross = 0.f;
-        float* orow = out + (row0 + 5 + 3 * (int64_t)(c0 + kk)) * F + f;
-        for (int t = 0; t < 3; ++t) {
-          const float vc = s1 * uc[t], vn = s1 * un[t];
-          acc[2 + t] += G5[2 + t] * sv + G5[0] * vc;
-          orow[t * F] = -G5[2 + t] * sv + G5[0] * vn;
-          cross += G5[2 + t] * (vc - vn);
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int kk = k0 + u;
+        if (kk >= cn) break;
+        const bool same = s_same[kk];
+        float G5[5], E5[5], mn[5], mc[5];
+#pragma unroll
+        for (int rr = 0; rr < NR; ++rr) {
+          float a = 0.f;
+#pragma unroll
+          for (int q = 0; q < W1; ++q) a += s_beta[kk][rr][q] * (same ? wgs[q] : wgd[q]);
+          G5[rr] = a;
+          float b = 0.f;
+#pragma unroll
+          for (int m = 0; m < 4; ++m) b += s_g[kk][rr][m] * (same ? wfs[m] : wfd[m]);
+          E5[rr] = b;
+          mn[rr] = mnu[u][rr];
+          mc[rr] = same ? mcs[rr] : mcd[rr];
         }
-        acc[1] += 2.f * G5[1] * sv + G5[0] * vl + 2.f * cross;
+        E5[0] += same ? bfs : bfd;
+        const float ux = E5[0] + mc[0] + mn[0];
+        float sv, s1, s2;
+        silu3(ux, sv, s1, s2);
+        acc[0] += G5[0] * sv;
+        if (TAN) {
+          float uc[3], un[3], q2 = 0.f;
+          for (int t = 0; t < 3; ++t) {
+            uc[t] = E5[2 + t] + mc[2 + t];
+            un[t] = -E5[2 + t] + mn[2 + t];
+            q2 += uc[t] * uc[t] + un[t] * un[t];
+          }
+          const float ul = 2.f * E5[1] + mc[1] + mn[1];
+          const float vl = s1 * ul + s2 * q2;
+          float cross = 0.f;
+          float* orow = out + (row0 + 5 + 3 * (int64_t)(c0 + kk)) * F + f;
+          for (int t = 0; t < 3; ++t) {
+            const float vc = s1 * uc[t], vn = s1 * un[t];
+            acc[2 + t] += G5[2 + t] * sv + G5[0] * vc;
+            orow[t * F] = -G5[2 + t] * sv + G5[0] * vn;
+            cross += G5[2 + t] * (vc - vn);
+          }
+          acc[1] += 2.f * G5[1] * sv + G5[0] * vl + 2.f * cross;
+        }
       }
     }
     __syncthreads();

@@ -67,37 +67,61 @@ __global__ void __launch_bounds__(256) k_nbr(const float* __restrict__ r, int B,
 
 // Exclusive scan of the neighbour counts in group order (single workgroup, 1024 threads).
 // ptr[g] = pairs before group g; ptr[G] = total pairs; ptr[G+1] = largest count.
-__global__ void __launch_bounds__(1024) k_scan(const int32_t* __restrict__ cnt, int G, int64_t* __restrict__ ptr) {
-  __shared__ int64_t part[1024];
-  __shared__ int32_t mx[1024];
-  const int t = threadIdx.x;
-  const int per = (G + 1023) / 1024;
-  const int lo = min(G, t * per), hi = min(G, lo + per);
-  int64_t s = 0;
-  int32_t m = 0;
-  for (int k = lo; k < hi; ++k) {
-    s += cnt[k];
-    m = max(m, cnt[k]);
-  }
-  part[t] = s;
-  mx[t] = m;
-  __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    const int64_t v = t >= off ? part[t - off] : 0;
-    const int32_t vm = t >= off ? mx[t - off] : 0;
+// Exclusive prefix sum of the neighbour counts: ptr[0..G) offsets, ptr[G] total,
+// ptr[G+1] largest count.  One block walks the counts in tiles of 4096 (4 consecutive per
+// thread, coalesced): wave prefix sums by shuffles, one LDS pass over the 16 wave totals.
+// With dn (device count) only the first *dn counts are read; later entries are zero by
+// construction and get ptr = total.
+__global__ void __launch_bounds__(1024) k_scan(const int32_t* __restrict__ cnt, int G, int64_t* __restrict__ ptr,
+                                               const int32_t* __restrict__ dn) {
+  __shared__ long s_wsum[16];
+  __shared__ int32_t s_wmax[16];
+  const int n = dn ? min(G, *dn) : G;
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  long carry = 0;
+  int32_t mx_all = 0;
+  for (int base = 0; base < n; base += 4096) {
+    const int k0 = base + 4 * t;
+    int32_t c[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) c[u] = k0 + u < n ? cnt[k0 + u] : 0;
+    const long sum = (long)c[0] + c[1] + c[2] + c[3];
+    int32_t m = max(max(c[0], c[1]), max(c[2], c[3]));
+    long incl = sum;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const long y = __shfl_up(incl, off, 64);
+      if (lane >= off) incl += y;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off, 64));
+    if (lane == 63) s_wsum[wv] = incl;
+    if (lane == 0) s_wmax[wv] = m;
     __syncthreads();
-    part[t] += v;
-    mx[t] = max(mx[t], vm);
-    __syncthreads();
+    long before = 0, tot = 0;
+    int32_t mx = 0;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const long v = s_wsum[q];
+      before += q < wv ? v : 0;
+      tot += v;
+      mx = max(mx, s_wmax[q]);
+    }
+    long run = carry + before + incl - sum;
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (k0 + u < n) {
+        ptr[k0 + u] = run;
+        run += c[u];
+      }
+    carry += tot;
+    mx_all = max(mx_all, mx);
+    __syncthreads();  // s_wsum is rewritten by the next tile
   }
-  int64_t run = part[t] - s;
-  for (int k = lo; k < hi; ++k) {
-    ptr[k] = run;
-    run += cnt[k];
-  }
-  if (t == 1023) {
-    ptr[G] = part[1023];
-    ptr[G + 1] = mx[1023];
+  for (int k = n + t; k < G; k += 1024) ptr[k] = carry;
+  if (t == 0) {
+    ptr[G] = carry;
+    ptr[G + 1] = mx_all;
   }
 }
 
@@ -137,8 +161,8 @@ void launch_nbr_fill(hipStream_t s, const float* r, int B, int N, int n_up, floa
   k_nbr<true><<<blocks, 256, 0, s>>>(r, B, N, n_up, cutoff, nullptr, ptr, nbr, list, n_list, dn);
 }
 
-void launch_scan(hipStream_t s, const int32_t* cnt, int G, int64_t* ptr) {
-  k_scan<<<1, 1024, 0, s>>>(cnt, G, ptr);
+void launch_scan(hipStream_t s, const int32_t* cnt, int G, int64_t* ptr, const int32_t* dn) {
+  k_scan<<<1, 1024, 0, s>>>(cnt, G, ptr, dn);
 }
 
 void launch_rev(hipStream_t s, const int64_t* ptr, const int32_t* nbr, int B, int N, int n_up, int32_t* rev) {

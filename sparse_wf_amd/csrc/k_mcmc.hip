@@ -217,324 +217,478 @@ __global__ void k_mc_klist(int B, int k, int2* __restrict__ list) {
 
 void launch_mc_klist(hipStream_t s, int B, int k, int2* list) { k_mc_klist<<<(B + 255) / 256, 256, 0, s>>>(B, k, list); }
 
-// Woodbury preparation per (walker, det): new Slater rows Phi_new of the affected
-// electrons, S = I + V Ainv[:, aff] with V = Phi_new - A[aff], log|det S|, sign, S^-1.
-// Writes V = Phi_new - A[aff] (not Phi_new) to `phin`.  Dynamic LDS: S [amax][amax], a pivot
-// column, pivots and (PLDS) P = Ainv[:, aff] [N][amax]; without PLDS, P is read from Ainv.
-template <bool PLDS>
-__global__ void __launch_bounds__(256) k_wb_prep(const float* __restrict__ p_orb, const float* __restrict__ p_envk,
-                                                 const float* __restrict__ st_env, const double* __restrict__ st_A,
-                                                 const double* __restrict__ st_Ainv, const int32_t* __restrict__ aff,
-                                                 const int32_t* __restrict__ cu, const int32_t* __restrict__ cd,
-                                                 const int32_t* __restrict__ rowof, int B, int N, int n_up, int D,
-                                                 int k, int amax, double* __restrict__ phin,
-                                                 double* __restrict__ sinv, double* __restrict__ ldS,
-                                                 double* __restrict__ sgS) {
-  extern __shared__ unsigned char smem_raw[];
-  double* S = reinterpret_cast<double*>(smem_raw);  // [amax][amax]
-  double* colc = S + (size_t)amax * amax;            // [amax]
-  int* piv = reinterpret_cast<int*>(colc + amax);    // [amax]
-  double* P = reinterpret_cast<double*>(smem_raw + (size_t)amax * amax * 8 + (size_t)amax * 8 +
-                                        (((size_t)amax * 4 + 7) & ~(size_t)7));  // [N][amax] (PLDS)
-  __shared__ double s_val[4];
-  __shared__ int s_idx[4];
-  __shared__ int s_p;
+// ---------------------------------------------------------------------------------------
+// FP64 Woodbury kernels on the matrix cores.
+//
+// v_mfma_f64_16x16x4f64 computes D = A B + C with A 16x4, B 4x16, C and D 16x16.  Lane l
+// holds A[l%16][l/16] and B[l/16][l%16]; register i of lane l holds D[4i + l/16][l%16]
+// (measured on gfx950).  At most kWbCap electrons change per move (the state's capacity),
+// so every a x a matrix fits four 16-wide tiles.
+//
+// Per (walker, det) the update reads, in 16x4 operand tiles of 64 doubles (one coalesced
+// access per tile):
+//   vs[(t * KS + ks) * 64 + l] = V[16t + l%16][4ks + l/16],  V = Phi_new - A[aff] (a x N),
+//     written by k_wb_s;
+//   xs[(to * NKA + ks) * 64 + l] = X[16to + l%16][4ks + l/16],  X = Ainv[:, aff] S^-1 (N x a),
+//     written by k_wb_x;
+// with KS = ceil(N / 4), NKA = 4 ceil(a / 16) and zeros outside the matrices.
+// ---------------------------------------------------------------------------------------
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ dbl4 mfma_f64(double a, double b, dbl4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+constexpr int kWbCap = 64;        // changed electrons per move handled here
+constexpr int kSld = kWbCap + 1;  // LDS row stride of S
+
+// value of x in lane `src` (wave-uniform) for every lane
+__device__ __forceinline__ double bcast_lane(double x, int src) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(x), src);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(x), src);
+  return __hiloint2double(hi, lo);
+}
+
+// wave-wide maximum of a 32-bit key (all 64 lanes active): DPP butterflies within each row
+// of 16 lanes, then the four row maxima by readlane; the result is wave-uniform
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false));   // quad_perm 1,0,3,2
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false));   // quad_perm 2,3,0,1
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x124, 0xF, 0xF, false));  // row_ror 4
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x128, 0xF, 0xF, false));  // row_ror 8
+  const uint32_t r0 = __builtin_amdgcn_readlane((int)v, 0), r1 = __builtin_amdgcn_readlane((int)v, 16);
+  const uint32_t r2 = __builtin_amdgcn_readlane((int)v, 32), r3 = __builtin_amdgcn_readlane((int)v, 48);
+  return max(max(r0, r1), max(r2, r3));
+}
+
+// Woodbury preparation, first kernel, per (walker, det): V = Phi_new - A[aff] of the
+// affected electrons and S = I + V P, P = Ainv[:, aff], on the matrix cores.  One wave per
+// 16x16 tile (t, u) of S with K over all electrons (the u = 0 waves also write V in operand
+// tiles); S goes to `sbuf` (64 x 64 per (walker, det), row-major).
+__global__ void __launch_bounds__(256) k_wb_s(const float* __restrict__ p_orb, const float* __restrict__ p_envk,
+                                              const float* __restrict__ st_env, const double* __restrict__ st_A,
+                                              const double* __restrict__ st_Ainv, const int32_t* __restrict__ aff,
+                                              const int32_t* __restrict__ cu, const int32_t* __restrict__ cd,
+                                              const int32_t* __restrict__ rowof, int B, int N, int n_up, int D, int k,
+                                              double* __restrict__ vs, double* __restrict__ sbuf) {
+  constexpr int CH = 8;  // k steps loaded ahead
   const int wd = blockIdx.x;
   const int w = wd / D, d = wd - w * D;
   const int a = cu[w] + cd[w];
-  const int ND = N * D;
-  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int nT = (a + 15) >> 4;
+  const int ND = N * D, KS = (N + 3) >> 2;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, m = lane & 15, g = lane >> 4;
   const double* Ai = st_Ainv + (size_t)wd * N * N;
   const double* Aw = st_A + (size_t)wd * N * N;
-  double* PH = phin + (size_t)wd * amax * N;
-  // Phi_new rows (FP64) and P = Ainv[:, aff]
-  for (int e = tid; e < a * N; e += blockDim.x) {
-    const int j = e / N, o = e - j * N;
-    const int i = aff[(size_t)w * N + j];
-    const int t = rowof[(size_t)w * N + i];
-    const float env = (i == k) ? p_envk[(size_t)w * ND + d * N + o]
-                               : st_env[(size_t)group_of(w, i, B, n_up, N - n_up) * ND + d * N + o];
-    // product rounded in FP32 exactly as the full evaluation forms the Slater matrix
-    const double ph = (double)__fmul_rn(p_orb[(size_t)t * ND + d * N + o], env);
-    PH[(size_t)j * N + o] = ph - Aw[(size_t)i * N + o];  // V = Phi_new - A[aff] (A rows restored by the update)
-    if (PLDS) P[(size_t)o * amax + j] = Ai[(size_t)o * N + i];
-  }
-  __syncthreads();
-  // S = I + V P: one row j of V per wavefront pass, P from LDS
-  for (int e = tid; e < a * a; e += blockDim.x) {
-    const int j = e / a, l = e - j * a;
-    const double* vr = PH + (size_t)j * N;
-    const int cl = aff[(size_t)w * N + l];
-    auto pv = [&](int o) { return PLDS ? P[(size_t)o * amax + l] : Ai[(size_t)o * N + cl]; };
-    double acc0 = 0.0, acc1 = 0.0;
-    int o = 0;
-    for (; o + 2 <= N; o += 2) {
-      acc0 += vr[o] * pv(o);
-      acc1 += vr[o + 1] * pv(o + 1);
-    }
-    if (o < N) acc0 += vr[o] * pv(o);
-    S[j * amax + l] = (j == l ? 1.0 : 0.0) + (acc0 + acc1);
-  }
-  __syncthreads();
-  // Gauss-Jordan inverse of S with partial pivoting; log|det S| and sign
-  double ld = 0.0;
-  int flips = 0;
-  for (int c = 0; c < a; ++c) {
-    double best = -1.0;
-    int bi = a;
-    for (int rr = c + tid; rr < a; rr += blockDim.x) {
-      const double v = fabs(S[rr * amax + c]);
-      if (v > best || (v == best && rr < bi)) { best = v; bi = rr; }
-    }
-    for (int o = 32; o > 0; o >>= 1) {
-      const double ov = __shfl_xor(best, o, 64);
-      const int oi = __shfl_xor(bi, o, 64);
-      if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
-    }
-    if (lane == 0) { s_val[wid] = best; s_idx[wid] = bi; }
-    __syncthreads();
-    if (tid == 0) {
-      double b = s_val[0];
-      int ii = s_idx[0];
-      for (int q = 1; q < 4; ++q)
-        if (s_val[q] > b || (s_val[q] == b && s_idx[q] < ii)) { b = s_val[q]; ii = s_idx[q]; }
-      s_p = ii;
-      piv[c] = ii;
-    }
-    __syncthreads();
-    const int p = s_p;
-    if (p != c)
-      for (int cc = tid; cc < a; cc += blockDim.x) {
-        const double t = S[c * amax + cc];
-        S[c * amax + cc] = S[p * amax + cc];
-        S[p * amax + cc] = t;
+  double* VS = vs + (size_t)wd * wb_vs_doubles(N);
+  double* SB = sbuf + (size_t)wd * kWbCap * kWbCap;
+  const int32_t* af = aff + (size_t)w * N;
+  for (int tt = wv; tt < nT * nT; tt += 4) {
+    const int t = tt / nT, u = tt - t * nT;
+    const int j = 16 * t + m, l = 16 * u + m;
+    const bool vj = j < a, vl = l < a;
+    const int ij = vj ? af[j] : 0;
+    const float* orb = p_orb + (size_t)(vj ? rowof[(size_t)w * N + ij] : 0) * ND + (size_t)d * N;
+    const float* env = (ij == k ? p_envk + (size_t)w * ND : st_env + (size_t)group_of(w, ij, B, n_up, N - n_up) * ND) +
+                       (size_t)d * N;
+    const double* arow = Aw + (size_t)ij * N;
+    const int cl = vl ? af[l] : 0;
+    dbl4 acc = dbl4{0.0, 0.0, 0.0, 0.0};
+    for (int ks0 = 0; ks0 < KS; ks0 += CH) {
+      float fo[CH], fe[CH];
+      double fa[CH], fb[CH];
+#pragma unroll
+      for (int q = 0; q < CH; ++q) {
+        const int o = 4 * (ks0 + q) + g;  // o >= N for k steps past the end
+        const bool oj = vj && o < N;
+        fo[q] = oj ? orb[o] : 0.0f;
+        fe[q] = oj ? env[o] : 0.0f;
+        fa[q] = oj ? arow[o] : 0.0;
+        fb[q] = (vl && o < N) ? Ai[(size_t)o * N + cl] : 0.0;
       }
-    __syncthreads();
-    const double pv = S[c * amax + c];
-    if (tid == 0) {
-      ld += log(fabs(pv));
-      flips += (p != c) + (pv < 0.0);
-    }
-    const double ip = 1.0 / pv;
-    __syncthreads();
-    for (int cc = tid; cc < a; cc += blockDim.x) S[c * amax + cc] = (cc == c) ? ip : S[c * amax + cc] * ip;
-    __syncthreads();
-    for (int rr = tid; rr < a; rr += blockDim.x) colc[rr] = S[rr * amax + c];
-    __syncthreads();
-    for (int i = wid; i < a; i += blockDim.x >> 6) {
-      if (i == c) continue;
-      const double fk = colc[i];
-      for (int j = lane; j < a; j += 64) S[i * amax + j] = (j == c) ? -fk * ip : S[i * amax + j] - fk * S[c * amax + j];
-    }
-    __syncthreads();
-  }
-  for (int c = a - 1; c >= 0; --c) {
-    const int p = piv[c];
-    if (p != c)
-      for (int rr = tid; rr < a; rr += blockDim.x) {
-        const double t = S[rr * amax + c];
-        S[rr * amax + c] = S[rr * amax + p];
-        S[rr * amax + p] = t;
+#pragma unroll
+      for (int q = 0; q < CH; ++q) {
+        const int ks = ks0 + q;
+        const int o = 4 * ks + g;
+        // product rounded in FP32 exactly as the full evaluation forms the Slater matrix
+        const double v = (vj && o < N) ? (double)__fmul_rn(fo[q], fe[q]) - fa[q] : 0.0;
+        if (u == 0 && ks < KS) VS[((size_t)t * KS + ks) * 64 + lane] = v;
+        acc = mfma_f64(v, fb[q], acc);
       }
-    __syncthreads();
-  }
-  double* SI = sinv + (size_t)wd * amax * amax;
-  for (int e = tid; e < a * a; e += blockDim.x) {
-    const int j = e / a, l = e - j * a;
-    SI[j * amax + l] = S[j * amax + l];
-  }
-  if (tid == 0) {
-    ldS[wd] = ld;
-    sgS[wd] = (flips & 1) ? -1.0 : 1.0;
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int r = 16 * t + 4 * i + g;
+      if (r < a && vl) SB[r * kWbCap + l] = acc[i] + (r == l ? 1.0 : 0.0);
+    }
   }
 }
 
-static size_t wb_prep_fixed(int amax) {
-  return (size_t)amax * amax * 8 + (size_t)amax * 8 + (((size_t)amax * 4 + 7) & ~(size_t)7);
-}
-// Dynamic LDS of the preparation: P staged when that keeps the block within 64 KB (several
-// blocks per CU), else P is read from Ainv.
-size_t wb_prep_smem(int N, int amax) {
-  const size_t with_p = wb_prep_fixed(amax) + (size_t)N * amax * 8;
-  return with_p <= 64 * 1024 ? with_p : wb_prep_fixed(amax);
+// Woodbury preparation, second kernel: S^-1 (in place in `sbuf`), log|det S| and its sign
+// by Gauss-Jordan elimination with partial pivoting, one wavefront per (walker, det) with
+// S in registers: element e = lane + 64 kk of the a x a matrix (row e / a, column e % a) is
+// val[kk], its indices packed in idx[kk] = column | row << 8.  Pivot rows are chosen
+// without swaps: step c picks the unused row p with the largest |S[p][c]| (|S| rounded to
+// FP32 with its 7 lowest mantissa bits replaced by the row, so the lowest row wins among
+// near-ties), scales it and eliminates column c from every other row; the pivot column and
+// the scaled pivot row are exchanged through LDS.  The result in place is (P S)^-1 with row
+// c of P S in row piv[c], so S^-1[i][piv[c]] = stored[piv[i]][c] and det S = sign(piv)
+// prod(pivots).  A launch with KM slots handles the walkers with ceil(a^2 / 64) <= KM
+// (KM = 16, a <= 32) or > 16 (KM = 64).
+template <int KM>
+__global__ void __launch_bounds__(64) k_wb_gj(const int32_t* __restrict__ cu, const int32_t* __restrict__ cd, int D,
+                                              double* __restrict__ sbuf, double* __restrict__ ldS,
+                                              double* __restrict__ sgS) {
+  // entry kWbCap of s_f / s_pr is a sink for the lanes whose element is not exchanged, so
+  // the passes below run without branches
+  __shared__ double s_f[kWbCap + 1];
+  __shared__ double s_pr[kWbCap + 1];
+  __shared__ int s_piv[kWbCap];
+  __shared__ int s_qinv[kWbCap];
+  constexpr int kSink = kWbCap | (kWbCap << 8);
+  const int wd = blockIdx.x;
+  const int w = wd / D;
+  const int a = cu[w] + cd[w];
+  const int a2 = a * a, nk = (a2 + 63) >> 6;
+  if (nk > KM || (KM > 16 && nk <= 16)) return;
+  const int lane = threadIdx.x;
+  const uint32_t ma = ((1u << 20) + a - 1) / a;  // e / a == (e * ma) >> 20 for e < 4096
+  double* SB = sbuf + (size_t)wd * kWbCap * kWbCap;
+  double val[KM];
+  int idx[KM];
+#pragma unroll
+  for (int kk = 0; kk < KM; ++kk) {
+    const int e = lane + 64 * kk;
+    const int r = (int)(__umul24((uint32_t)e, ma) >> 20), cc = e - __mul24(r, a);
+    const bool in = kk < nk && e < a2;
+    idx[kk] = in ? (cc | (r << 8)) : kSink;
+    val[kk] = in ? SB[r * kWbCap + cc] : 0.0;
+  }
+  bool used = false;  // lane r: row r has been a pivot row
+  double mant = 1.0;
+  int ex2 = 0, nneg = 0;
+  for (int c = 0; c < a; ++c) {
+#pragma unroll
+    for (int kk = 0; kk < KM; ++kk)
+      if (kk < nk) s_f[(idx[kk] & 255) == c ? idx[kk] >> 8 : kWbCap] = val[kk];
+    const double fr = lane < a ? s_f[lane] : 0.0;  // S[lane][c]
+    const uint32_t fb = __float_as_uint((float)fabs(fr));
+    const uint32_t key = (lane < a && !used) ? ((fb & ~127u) | (uint32_t)(127 - lane)) : 0u;
+    const int p = 127 - (int)(wave_max_u32(key) & 127u);
+    used = used || lane == p;
+    if (lane == 0) s_piv[c] = p;
+    const double pv = bcast_lane(fr, p);
+    nneg += pv < 0.0;
+    int ee;
+    mant = frexp(mant * fabs(pv), &ee);  // log|det| accumulated as mantissa and exponent
+    ex2 += ee;
+    const double ip = 1.0 / pv;
+#pragma unroll
+    for (int kk = 0; kk < KM; ++kk)
+      if (kk < nk) {
+        const int cc = idx[kk] & 255;
+        s_pr[(idx[kk] >> 8) == p ? cc : kWbCap] = cc == c ? ip : val[kk] * ip;
+      }
+#pragma unroll
+    for (int kk = 0; kk < KM; ++kk)
+      if (kk < nk) {
+        const int r = idx[kk] >> 8, cc = idx[kk] & 255;
+        const double f = s_f[r], pr = s_pr[cc];
+        const double y = cc == c ? -f * ip : val[kk] - f * pr;
+        val[kk] = r == p ? pr : y;
+      }
+  }
+  int inv = 0;
+  if (lane < a) {
+    const int pk = s_piv[lane];
+    s_qinv[pk] = lane;
+    for (int q = lane + 1; q < a; ++q) inv += s_piv[q] < pk;  // inversions of the permutation
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) inv += __shfl_xor(inv, off, 64);
+#pragma unroll
+  for (int kk = 0; kk < KM; ++kk)
+    if (kk < nk && idx[kk] != kSink) SB[s_qinv[idx[kk] >> 8] * kWbCap + s_piv[idx[kk] & 255]] = val[kk];
+  if (lane == 0) {
+    ldS[wd] = log(mant) + (double)ex2 * 0.6931471805599453;
+    sgS[wd] = ((inv + nneg) & 1) ? -1.0 : 1.0;
+  }
 }
 
 void launch_wb_prep(hipStream_t s, const float* p_orb, const float* p_envk, const float* st_env, const double* st_A,
                     const double* st_Ainv, const int32_t* aff, const int32_t* cu, const int32_t* cd,
-                    const int32_t* rowof, int B, int N, int n_up, int D, int k, int amax, double* phin, double* sinv,
-                    double* ldS, double* sgS) {
-  const size_t sm = wb_prep_smem(N, amax);
-  if (sm > wb_prep_fixed(amax))
-    k_wb_prep<true><<<B * D, 256, sm, s>>>(p_orb, p_envk, st_env, st_A, st_Ainv, aff, cu, cd, rowof, B, N, n_up, D, k,
-                                           amax, phin, sinv, ldS, sgS);
-  else
-    k_wb_prep<false><<<B * D, 256, sm, s>>>(p_orb, p_envk, st_env, st_A, st_Ainv, aff, cu, cd, rowof, B, N, n_up, D,
-                                            k, amax, phin, sinv, ldS, sgS);
+                    const int32_t* rowof, int B, int N, int n_up, int D, int k, double* vs, double* sbuf, double* ldS,
+                    double* sgS) {
+  k_wb_s<<<B * D, 256, 0, s>>>(p_orb, p_envk, st_env, st_A, st_Ainv, aff, cu, cd, rowof, B, N, n_up, D, k, vs, sbuf);
+  k_wb_gj<16><<<B * D, 64, 0, s>>>(cu, cd, D, sbuf, ldS, sgS);
+  if (N > 32) k_wb_gj<64><<<B * D, 64, 0, s>>>(cu, cd, D, sbuf, ldS, sgS);
 }
 
-hipError_t wb_set_smem_limit(size_t prep, size_t upd);
+// X^T = S^-T P^T (X = Ainv[:, aff] S^-1) for accepted walkers, in A-operand tiles of the
+// update product; the four waves take the 16-row tiles of X in turn.
+template <int NTA>
+__device__ __forceinline__ void wb_x_body(int a, int N, const int32_t* __restrict__ af, const double* __restrict__ SB,
+                                          const double* __restrict__ Ai, double* __restrict__ XS) {
+  constexpr int NKA = 4 * NTA;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, m = lane & 15, g = lane >> 4;
+  const int NT = (N + 15) >> 4;
+  for (int to = wv; to < NT; to += 4) {
+    const int o = 16 * to + m;
+    dbl4 acc[NTA];
+#pragma unroll
+    for (int t = 0; t < NTA; ++t) acc[t] = dbl4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int ks0 = 0; ks0 < NKA; ks0 += 4) {
+      double bb[4], sv[4][NTA];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int kk = 4 * (ks0 + u) + g;
+        const bool kv = kk < a;
+        bb[u] = (kv && o < N) ? Ai[(size_t)o * N + af[kk]] : 0.0;
+#pragma unroll
+        for (int t = 0; t < NTA; ++t) sv[u][t] = (kv && 16 * t + m < a) ? SB[kk * kWbCap + 16 * t + m] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int t = 0; t < NTA; ++t) acc[t] = mfma_f64(sv[u][t], bb[u], acc[t]);
+    }
+    // register i of lane l: X[16to + m][16t + 4i + g], the A operand of k step 4t + i
+#pragma unroll
+    for (int t = 0; t < NTA; ++t)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) XS[((size_t)to * NKA + 4 * t + i) * 64 + lane] = acc[t][i];
+  }
+}
 
-// Woodbury inverse update for accepted walkers (woodburry_update, utils.py:243):
-//   Ainv <- Ainv - (Ainv[:, aff] S^-1) (V Ainv);  then A[aff] <- A[aff] + V,
-// with V = Phi_new - A[aff] as left in `phin` by k_wb_prep.
-//
-// Thread c owns column c of Ainv; q = (V Ainv)[:, c] is held in AMAX registers.  V^T and
-// X = Ainv[:, aff] S^-1 are staged per (walker, det) in global scratch (row o of both is
-// read by every thread of the block: broadcast loads served by the caches), so the kernel
-// needs only S^-1 in LDS and several blocks share a CU.  Ainv is streamed twice, coalesced.
-template <int AMAX>
-__global__ void __launch_bounds__(256) k_wb_update_cols(const int32_t* __restrict__ accept,
-                                                        const int32_t* __restrict__ aff, const int32_t* __restrict__ cu,
-                                                        const int32_t* __restrict__ cd, int N, int D, int amax,
-                                                        const double* __restrict__ vmat,
-                                                        const double* __restrict__ sinv, double* __restrict__ st_A,
-                                                        double* __restrict__ st_Ainv, double* __restrict__ scr) {
-  __shared__ double SL[AMAX * AMAX];
-  __shared__ int s_af[AMAX];
+__global__ void __launch_bounds__(256) k_wb_x(const int32_t* __restrict__ accept, const int32_t* __restrict__ aff,
+                                              const int32_t* __restrict__ cu, const int32_t* __restrict__ cd, int N,
+                                              int D, const double* __restrict__ st_Ainv,
+                                              const double* __restrict__ sbuf, double* __restrict__ xs) {
   const int wd = blockIdx.x;
   const int w = wd / D;
   if (!accept[w]) return;
   const int a = cu[w] + cd[w];
-  const int tid = threadIdx.x;
+  const double* Ai = st_Ainv + (size_t)wd * N * N;
+  const double* SB = sbuf + (size_t)wd * kWbCap * kWbCap;
+  double* XS = xs + (size_t)wd * (wb_update_scratch(N) / sizeof(double));
+  const int32_t* af = aff + (size_t)w * N;
+  switch ((a + 15) >> 4) {
+    case 1: wb_x_body<1>(a, N, af, SB, Ai, XS); break;
+    case 2: wb_x_body<2>(a, N, af, SB, Ai, XS); break;
+    case 3: wb_x_body<3>(a, N, af, SB, Ai, XS); break;
+    default: wb_x_body<4>(a, N, af, SB, Ai, XS); break;
+  }
+}
+
+// Woodbury inverse update for accepted walkers (woodburry_update, utils.py:243):
+//   Ainv <- Ainv - X Q,  Q = V Ainv;  then A[aff] <- A[aff] + V,
+// with V and X in operand tiles as left by k_wb_s and k_wb_x.
+//
+// Register-tile form (N <= 16 NT): one 128-thread block per 16-wide column tile tc of Ainv;
+// wave h holds the row tiles [h NT0, min(NT, (h + 1) NT0)) of the column tile in registers,
+// register kk = 4 tt + i being Ainv[16 (h NT0 + tt) + 4i + l/16][16tc + l%16].  These are
+// the B operands of the wave's part of Q[:, tc] = V Ainv[:, tc] and, unchanged, register i
+// of the accumulator of output tile (h NT0 + tt, tc).  The two partial Q are added in LDS in
+// wave order; the D registers of Q are the B operands of X Q.  Ainv is read once and
+// written once and Q never leaves the block.  Blocks are ordered so that the column tiles
+// of one (walker, det) run on one XCD (blocks go to XCDs round-robin by index) and share its
+// L2 for the V and X tiles.
+template <int NT, int NTA>
+__device__ __forceinline__ void wb_update_tile(int a, int N, int tc, const int32_t* __restrict__ aff_w,
+                                               const double* __restrict__ VS, const double* __restrict__ XS,
+                                               double* __restrict__ Ai, double* __restrict__ Aw, dbl4* sq) {
+  constexpr int NKA = 4 * NTA, NT0 = (NT + 1) / 2, KH = 4 * NT0;
+  constexpr int CH = 4 / NTA > 0 ? 4 / NTA : 1;  // k steps of V per chunk
+  const int KS = (N + 3) >> 2;
+  const int lane = threadIdx.x & 63, h = threadIdx.x >> 6, m = lane & 15, g = lane >> 4;
+  const int t_lo = h * NT0, nt = h ? NT - NT0 : NT0;  // this wave's row tiles
+  const int c = 16 * tc + m;
+  const bool cv = c < N;
+  double mreg[KH];
+#pragma unroll
+  for (int kk = 0; kk < KH; ++kk) {
+    const int row = 16 * t_lo + 4 * kk + g;
+    mreg[kk] = (kk < 4 * nt && row < N && cv) ? Ai[(size_t)row * N + c] : 0.0;
+  }
+  dbl4 q[NTA];
+#pragma unroll
+  for (int t = 0; t < NTA; ++t) q[t] = dbl4{0.0, 0.0, 0.0, 0.0};
+  // partial Q over this wave's rows; V chunks double-buffered
+  auto load_v = [&](int kk0, double (&vv)[CH][NTA]) {
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+      const int ks = 4 * t_lo + kk0 + u;
+#pragma unroll
+      for (int t = 0; t < NTA; ++t)
+        vv[u][t] = (kk0 + u < 4 * nt && ks < KS) ? VS[((size_t)t * KS + ks) * 64 + lane] : 0.0;
+    }
+  };
+  double va[CH][NTA], vb[CH][NTA];
+  load_v(0, va);
+#pragma unroll
+  for (int kk0 = 0; kk0 < KH; kk0 += 2 * CH) {
+    if (kk0 + CH < KH) load_v(kk0 + CH, vb);
+#pragma unroll
+    for (int u = 0; u < CH; ++u)
+#pragma unroll
+      for (int t = 0; t < NTA; ++t) q[t] = mfma_f64(va[u][t], mreg[kk0 + u], q[t]);
+    if (kk0 + CH < KH) {
+      if (kk0 + 2 * CH < KH) load_v(kk0 + 2 * CH, va);
+#pragma unroll
+      for (int u = 0; u < CH; ++u)
+#pragma unroll
+        for (int t = 0; t < NTA; ++t) q[t] = mfma_f64(vb[u][t], mreg[kk0 + CH + u], q[t]);
+    }
+  }
+  // Q = partial(wave 0) + partial(wave 1)
+  if (h == 1)
+#pragma unroll
+    for (int t = 0; t < NTA; ++t) sq[t * 64 + lane] = q[t];
+  __syncthreads();
+  if (h == 0)
+#pragma unroll
+    for (int t = 0; t < NTA; ++t) {
+      q[t] += sq[t * 64 + lane];
+      sq[t * 64 + lane] = q[t];
+    }
+  __syncthreads();
+  if (h == 1)
+#pragma unroll
+    for (int t = 0; t < NTA; ++t) q[t] = sq[t * 64 + lane];
+  // Ainv[rows of this wave, tc] -= X Q; X operands of the next tile loaded ahead
+  auto load_x = [&](int tt, double (&xa)[NKA]) {
+    const int to = t_lo + tt;
+#pragma unroll
+    for (int ks = 0; ks < NKA; ++ks) xa[ks] = tt < nt ? -XS[((size_t)to * NKA + ks) * 64 + lane] : 0.0;
+  };
+  double xa[NKA], xb[NKA];
+  load_x(0, xa);
+#pragma unroll
+  for (int tt = 0; tt < NT0; ++tt) {
+    double(&xc)[NKA] = (tt & 1) ? xb : xa;
+    double(&xn)[NKA] = (tt & 1) ? xa : xb;
+    if (tt + 1 < NT0) load_x(tt + 1, xn);
+    if (tt < nt) {
+      dbl4 acc = dbl4{mreg[4 * tt], mreg[4 * tt + 1], mreg[4 * tt + 2], mreg[4 * tt + 3]};
+#pragma unroll
+      for (int t = 0; t < NTA; ++t)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) acc = mfma_f64(xc[4 * t + i], q[t][i], acc);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int r = 16 * (t_lo + tt) + 4 * i + g;
+        if (r < N && cv) Ai[(size_t)r * N + c] = acc[i];
+      }
+    }
+  }
+  // A[aff] += V on this tile's columns
+  for (int e = threadIdx.x; e < a * 16; e += 128) {
+    const int j = e >> 4, o = 16 * tc + (e & 15);
+    if (o < N)
+      Aw[(size_t)aff_w[j] * N + o] += VS[((size_t)(j >> 4) * KS + (o >> 2)) * 64 + 16 * (o & 3) + (j & 15)];
+  }
+}
+
+// One launch per class NTA = ceil(a / 16): the block returns unless its walker is in the
+// class, so each class gets its own register allocation.
+template <int NT, int NTA>
+__global__ void __launch_bounds__(128) k_wb_update_reg(const int32_t* __restrict__ accept,
+                                                       const int32_t* __restrict__ aff, const int32_t* __restrict__ cu,
+                                                       const int32_t* __restrict__ cd, int N, int D, int n_wd,
+                                                       const double* __restrict__ vs, const double* __restrict__ xs,
+                                                       double* __restrict__ st_A, double* __restrict__ st_Ainv) {
+  __shared__ dbl4 sq[NTA * 64];
+  const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+  const int wd = (slot / NT) * 8 + xcd, tc = slot - (slot / NT) * NT;
+  if (wd >= n_wd) return;
+  const int w = wd / D;
+  if (!accept[w]) return;
+  const int a = cu[w] + cd[w];
+  if (((a + 15) >> 4) != NTA && !(NTA == 4 && a > 64)) return;
+  wb_update_tile<NT, NTA>(a, N, tc, aff + (size_t)w * N, vs + (size_t)wd * wb_vs_doubles(N),
+                          xs + (size_t)wd * (wb_update_scratch(N) / sizeof(double)), st_Ainv + (size_t)wd * N * N,
+                          st_A + (size_t)wd * N * N, sq);
+}
+
+// Generic form (any N): one block per (walker, det).  Q = V Ainv goes through global scratch
+// (the second half of the per-(walker, det) scratch) in B-operand tiles; wavefronts own column
+// tiles for Q and row tiles for the update.
+__global__ void __launch_bounds__(256) k_wb_update(const int32_t* __restrict__ accept, const int32_t* __restrict__ aff,
+                                                   const int32_t* __restrict__ cu, const int32_t* __restrict__ cd,
+                                                   int N, int D, const double* __restrict__ vs,
+                                                   double* __restrict__ xs, double* __restrict__ st_A,
+                                                   double* __restrict__ st_Ainv) {
+  const int wd = blockIdx.x;
+  const int w = wd / D;
+  if (!accept[w]) return;
+  const int a = cu[w] + cd[w];
+  const int nT = (a + 15) >> 4, nK = 4 * nT;
+  const int NT = (N + 15) >> 4, KS = (N + 3) >> 2;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nw = blockDim.x >> 6, m = lane & 15, g = lane >> 4;
   double* Ai = st_Ainv + (size_t)wd * N * N;
   double* Aw = st_A + (size_t)wd * N * N;
-  const double* V = vmat + (size_t)wd * amax * N;
-  const double* SI = sinv + (size_t)wd * amax * amax;
-  double* VT = scr + (size_t)wd * 2 * N * AMAX;  // [N][AMAX]
-  double* X = VT + (size_t)N * AMAX;             // [N][AMAX]
-  for (int j = tid; j < a; j += blockDim.x) s_af[j] = aff[(size_t)w * N + j];
-  for (int e = tid; e < AMAX * AMAX; e += blockDim.x) {
-    const int l = e / AMAX, j = e - l * AMAX;
-    SL[e] = (l < a && j < a) ? SI[l * amax + j] : 0.0;
-  }
-  __syncthreads();
-  for (int e = tid; e < N * AMAX; e += blockDim.x) {
-    const int o = e / AMAX, j = e - o * AMAX;
-    VT[e] = j < a ? V[(size_t)j * N + o] : 0.0;
-  }
-  // the j loops below run in chunks of 16 and skip chunks beyond this walker's count a
-  // (uniform in the block), so the cost follows a, not the capacity AMAX
-  for (int o = tid; o < N; o += blockDim.x) {
-    double acc[AMAX];
-#pragma unroll
-    for (int j = 0; j < AMAX; ++j) acc[j] = 0.0;
-    const double* ar = Ai + (size_t)o * N;
-    for (int l = 0; l < a; ++l) {
-      const double p = ar[s_af[l]];
-#pragma unroll
-      for (int ch = 0; ch < AMAX / 16; ++ch)
-        if (ch * 16 < a) {
-#pragma unroll
-          for (int u = 0; u < 16; ++u) acc[ch * 16 + u] += p * SL[l * AMAX + ch * 16 + u];
-        }
+  const double* VS = vs + (size_t)wd * wb_vs_doubles(N);
+  const double* XS = xs + (size_t)wd * (wb_update_scratch(N) / sizeof(double));
+  double* QS = xs + (size_t)wd * (wb_update_scratch(N) / sizeof(double)) + (size_t)NT * 16 * 64;
+  // Q = V Ainv; register i of lane l is Q[16t + 4i + g][16tc + m], the B operand of k step
+  // 4t + i in the same lane
+  for (int tc = wv; tc < NT; tc += nw) {
+    dbl4 acc[4];
+    for (int t = 0; t < 4; ++t) acc[t] = dbl4{0.0, 0.0, 0.0, 0.0};
+    const int c = 16 * tc + m;
+    for (int ks = 0; ks < KS; ++ks) {
+      const int o = 4 * ks + g;
+      const double b = (o < N && c < N) ? Ai[(size_t)o * N + c] : 0.0;
+      for (int t = 0; t < nT; ++t) acc[t] = mfma_f64(VS[((size_t)t * KS + ks) * 64 + lane], b, acc[t]);
     }
-#pragma unroll
-    for (int j = 0; j < AMAX; ++j) X[(size_t)o * AMAX + j] = j < a ? acc[j] : 0.0;
+    for (int t = 0; t < nT; ++t)
+      for (int i = 0; i < 4; ++i) QS[((size_t)tc * nK + 4 * t + i) * 64 + lane] = acc[t][i];
   }
   __threadfence_block();
   __syncthreads();
-  for (int c = tid; c < N; c += blockDim.x) {
-    double q[AMAX];
-#pragma unroll
-    for (int j = 0; j < AMAX; ++j) q[j] = 0.0;
-    for (int o = 0; o < N; ++o) {
-      const double x = Ai[(size_t)o * N + c];
-      const double* v = VT + (size_t)o * AMAX;
-#pragma unroll
-      for (int ch = 0; ch < AMAX / 16; ++ch)
-        if (ch * 16 < a) {
-#pragma unroll
-          for (int u = 0; u < 16; ++u) q[ch * 16 + u] += v[ch * 16 + u] * x;
-        }
-    }
-    for (int o = 0; o < N; ++o) {
-      const double* xr = X + (size_t)o * AMAX;
-      double acc = 0.0;
-#pragma unroll
-      for (int ch = 0; ch < AMAX / 16; ++ch)
-        if (ch * 16 < a) {
-#pragma unroll
-          for (int u = 0; u < 16; ++u) acc += xr[ch * 16 + u] * q[ch * 16 + u];
-        }
-      Ai[(size_t)o * N + c] -= acc;
+  // Ainv <- Ainv - X Q, one 16x16 tile at a time
+  for (int to = wv; to < NT; to += nw) {
+    const int r0 = 16 * to + g;  // register i holds row r0 + 4i
+    for (int tc = 0; tc < NT; ++tc) {
+      const int c = 16 * tc + m;
+      dbl4 acc;
+      for (int i = 0; i < 4; ++i) acc[i] = (r0 + 4 * i < N && c < N) ? Ai[(size_t)(r0 + 4 * i) * N + c] : 0.0;
+      for (int ks = 0; ks < nK; ++ks)
+        acc = mfma_f64(-XS[((size_t)to * nK + ks) * 64 + lane], QS[((size_t)tc * nK + ks) * 64 + lane], acc);
+      for (int i = 0; i < 4; ++i)
+        if (r0 + 4 * i < N && c < N) Ai[(size_t)(r0 + 4 * i) * N + c] = acc[i];
     }
   }
   for (int e = tid; e < a * N; e += blockDim.x) {
     const int j = e / N, o = e - j * N;
-    Aw[(size_t)s_af[j] * N + o] += V[(size_t)j * N + o];
+    Aw[(size_t)aff[(size_t)w * N + j] * N + o] +=
+        VS[((size_t)(j >> 4) * KS + (o >> 2)) * 64 + 16 * (o & 3) + (j & 15)];
   }
 }
-
-// Generic form (any number of changed rows): X and Q = V Ainv in dynamic LDS.
-__global__ void __launch_bounds__(256) k_wb_update(const int32_t* __restrict__ accept, const int32_t* __restrict__ aff,
-                                                   const int32_t* __restrict__ cu, const int32_t* __restrict__ cd,
-                                                   int N, int D, int amax, const double* __restrict__ vmat,
-                                                   const double* __restrict__ sinv, double* __restrict__ st_A,
-                                                   double* __restrict__ st_Ainv) {
-  extern __shared__ unsigned char smem_raw[];
-  double* X = reinterpret_cast<double*>(smem_raw);  // [N][amax]
-  double* Q = X + (size_t)N * amax;                  // [amax][N]
-  const int wd = blockIdx.x;
-  const int w = wd / D;
-  if (!accept[w]) return;
-  const int a = cu[w] + cd[w];
-  const int tid = threadIdx.x;
-  double* Ai = st_Ainv + (size_t)wd * N * N;
-  double* Aw = st_A + (size_t)wd * N * N;
-  const double* V = vmat + (size_t)wd * amax * N;
-  const double* SI = sinv + (size_t)wd * amax * amax;
-  const int32_t* af = aff + (size_t)w * N;
-  for (int e = tid; e < N * a; e += blockDim.x) {
-    const int o = e / a, j = e - o * a;
-    double acc = 0.0;
-    for (int l = 0; l < a; ++l) acc += Ai[(size_t)o * N + af[l]] * SI[l * amax + j];
-    X[(size_t)o * amax + j] = acc;
-  }
-  for (int e = tid; e < a * N; e += blockDim.x) {
-    const int j = e / N, c = e - j * N;
-    double acc = 0.0;
-    for (int o = 0; o < N; ++o) acc += V[(size_t)j * N + o] * Ai[(size_t)o * N + c];
-    Q[(size_t)j * N + c] = acc;
-  }
-  __syncthreads();
-  for (int e = tid; e < N * N; e += blockDim.x) {
-    const int o = e / N, c = e - o * N;
-    double acc = 0.0;
-    for (int j = 0; j < a; ++j) acc += X[(size_t)o * amax + j] * Q[(size_t)j * N + c];
-    Ai[e] -= acc;
-  }
-  for (int e = tid; e < a * N; e += blockDim.x) {
-    const int j = e / N, o = e - j * N;
-    Aw[(size_t)af[j] * N + o] += V[(size_t)j * N + o];
-  }
-}
-
-size_t wb_update_smem(int N, int amax) { return amax <= 64 ? 0 : (size_t)2 * N * amax * 8; }
-size_t wb_update_scratch(int N, int amax) { return amax <= 64 ? (size_t)2 * N * 64 * 8 : 0; }
 
 void launch_wb_update(hipStream_t s, const int32_t* accept, const int32_t* aff, const int32_t* cu, const int32_t* cd,
-                      int B, int N, int D, int amax, const double* vmat, const double* sinv, double* st_A,
-                      double* st_Ainv, double* scratch) {
-  const int threads = std::min(256, ((N + 63) / 64) * 64);
-  if (amax <= 16)
-    k_wb_update_cols<16><<<B * D, threads, 0, s>>>(accept, aff, cu, cd, N, D, amax, vmat, sinv, st_A, st_Ainv, scratch);
-  else if (amax <= 32)
-    k_wb_update_cols<32><<<B * D, threads, 0, s>>>(accept, aff, cu, cd, N, D, amax, vmat, sinv, st_A, st_Ainv, scratch);
-  else if (amax <= 48)
-    k_wb_update_cols<48><<<B * D, threads, 0, s>>>(accept, aff, cu, cd, N, D, amax, vmat, sinv, st_A, st_Ainv, scratch);
-  else if (amax <= 64)
-    k_wb_update_cols<64><<<B * D, threads, 0, s>>>(accept, aff, cu, cd, N, D, amax, vmat, sinv, st_A, st_Ainv, scratch);
-  else
-    k_wb_update<<<B * D, 256, wb_update_smem(N, amax), s>>>(accept, aff, cu, cd, N, D, amax, vmat, sinv, st_A,
-                                                            st_Ainv);
-}
-
-hipError_t wb_set_smem_limit(size_t prep, size_t upd) {
-  hipError_t e = hipFuncSetAttribute((const void*)k_wb_prep<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)prep);
-  if (e == hipSuccess)
-    e = hipFuncSetAttribute((const void*)k_wb_prep<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)prep);
-  if (e != hipSuccess || upd == 0) return e;
-  return hipFuncSetAttribute((const void*)k_wb_update, hipFuncAttributeMaxDynamicSharedMemorySize, (int)upd);
+                      int B, int N, int D, const double* vs, const double* sbuf, double* scratch, double* st_A,
+                      double* st_Ainv) {
+  k_wb_x<<<B * D, 256, 0, s>>>(accept, aff, cu, cd, N, D, st_Ainv, sbuf, scratch);
+  const int NT = (N + 15) / 16;
+  const int G = (B * D + 7) / 8 * 8 * NT;
+#define WB_REG(n)                                                                                         \
+  case n:                                                                                                 \
+    k_wb_update_reg<n, 1><<<G, 128, 0, s>>>(accept, aff, cu, cd, N, D, B * D, vs, scratch, st_A, st_Ainv); \
+    k_wb_update_reg<n, 2><<<G, 128, 0, s>>>(accept, aff, cu, cd, N, D, B * D, vs, scratch, st_A, st_Ainv); \
+    if (N > 32) {                                                                                         \
+      k_wb_update_reg<n, 3><<<G, 128, 0, s>>>(accept, aff, cu, cd, N, D, B * D, vs, scratch, st_A, st_Ainv); \
+      k_wb_update_reg<n, 4><<<G, 128, 0, s>>>(accept, aff, cu, cd, N, D, B * D, vs, scratch, st_A, st_Ainv); \
+    }                                                                                                     \
+    return;
+  switch (NT) {
+    WB_REG(1) WB_REG(2) WB_REG(3) WB_REG(4) WB_REG(5) WB_REG(6)
+    WB_REG(7) WB_REG(8) WB_REG(9) WB_REG(10) WB_REG(11) WB_REG(12)
+    default: break;
+  }
+#undef WB_REG
+  k_wb_update<<<B * D, 256, 0, s>>>(accept, aff, cu, cd, N, D, vs, scratch, st_A, st_Ainv);
 }
 
 // Per walker: Jastrow sums with the affected rows replaced by the proposal rows
