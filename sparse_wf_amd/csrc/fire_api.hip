@@ -282,6 +282,7 @@ void gemm(Ctx& c, const float* A, int64_t lda, int wi, int bi, float* C, int64_t
   if (use_bf && h->autotune) {
     int lg = 0;
     while ((int64_t(1) << (lg + 1)) <= M) ++lg;
+    if (dr.count) lg += 32;  // rows counted on the device (M is the capacity): tuned apart
     const auto key = std::make_tuple(ns, N, K, lg);
     auto it = h->gemm_tile.find(key);
     if (it != h->gemm_tile.end()) {
@@ -295,10 +296,10 @@ void gemm(Ctx& c, const float* A, int64_t lda, int wi, int bi, float* C, int64_t
       HIPCHK(hipEventCreate(&t1));
       for (int v = 0; v < nv; ++v) {
         launch_gemm_bf16(c.s, A, lda, h->Pbf[wi], h->Pbf_lo[wi], h->Pbf_lo2[wi], bias, C, ldc, M, N, K, rowflag,
-                         period, ns, v);
+                         period, ns, v, dr);
         HIPCHK(hipEventRecord(t0, c.s));
         launch_gemm_bf16(c.s, A, lda, h->Pbf[wi], h->Pbf_lo[wi], h->Pbf_lo2[wi], bias, C, ldc, M, N, K, rowflag,
-                         period, ns, v);
+                         period, ns, v, dr);
         HIPCHK(hipEventRecord(t1, c.s));
         HIPCHK(hipEventSynchronize(t1));
         HIPCHK(hipEventElapsedTime(&ms[v], t0, t1));
@@ -484,12 +485,9 @@ void run_orbitals_det(Ctx& c, const float* r, int B, const Graph& gr, bool tan, 
     for (int sp = 0; sp < 2; ++sp) {
       const int64_t r0 = sp == 0 ? 0 : rows_up, nr = sp == 0 ? rows_up : rows_all - rows_up;
       if (nr == 0) continue;
-      const float* pi = h->P[o + (sp == 0 ? 5 : 7)];
       const float* piT = h->Pt32[o + (sp == 0 ? 5 : 7)];
-      for (int d = 0; d < D; ++d)
-        launch_gemm_f32(c.s, h->ENVC.as<float>() + ((size_t)d * rows_all + r0) * ME, ME, pi + (size_t)d * ME * N,
-                        piT + (size_t)d * ME * N, nullptr, h->ENV.as<float>() + (size_t)r0 * ND + (size_t)d * N, ND,
-                        nr, N, ME, nullptr, 0);
+      launch_gemm_f32_batched(c.s, h->ENVC.as<float>() + (size_t)r0 * ME, ME, rows_all * ME, piT, (int64_t)ME * N,
+                              h->ENV.as<float>() + (size_t)r0 * ND, ND, N, nr, N, ME, D);
     }
   }
   if (keep)
@@ -793,7 +791,7 @@ void state_propose(Ctx& c, int k, int mode, const float* rk_in, float width, uin
   const int H = h->H, Dk = h->Dk, HD = h->HD;
   const int64_t T = S.tcap;
   const int amax = S.cap;
-  Ctx cc{h, c.s, S.prec, true};
+  Ctx cc{h, c.s, S.prec, c.fixed};
   int32_t* tail = S.tail.as<int32_t>();
   const int32_t* dT_up = tail + 0;
   const int32_t* dT_dn = tail + 1;
@@ -846,10 +844,8 @@ void state_propose(Ctx& c, int k, int mode, const float* rk_in, float width, uin
                   S.kcoef.as<float>(), S.klist.as<int2>(), B);
   {
     const int pi_i = o + (k < nu ? 5 : 7);
-    for (int d = 0; d < D; ++d)
-      launch_gemm_f32(c.s, S.kcoef.as<float>() + (size_t)d * B * ME, ME, h->P[pi_i] + (size_t)d * ME * N,
-                      h->Pt32[pi_i] + (size_t)d * ME * N, nullptr, S.p_envk.as<float>() + (size_t)d * N, ND, B, N, ME,
-                      nullptr, 0);
+    launch_gemm_f32_batched(c.s, S.kcoef.as<float>(), ME, (int64_t)B * ME, h->Pt32[pi_i], (int64_t)ME * N,
+                            S.p_envk.as<float>(), ND, N, B, N, ME, D);
   }
 
   // determinant ratios (Woodbury, FP64)
@@ -974,7 +970,7 @@ void state_steps(Ctx& c, int64_t n_steps, float width, uint64_t seed, int64_t st
       state_propose(c, 0, 0, nullptr, 0.f, seed, 0, nullptr);
       state_drop_pending(c);
       S.n_proposals -= S.B;
-      Ctx ce{h, S.gstream, c.prec};
+      Ctx ce{h, S.gstream, c.prec, true};  // captured: no tuning or event timing
       HIPCHK(hipStreamSynchronize(c.s));
       HIPCHK(hipStreamBeginCapture(S.gstream, hipStreamCaptureModeRelaxed));
       for (int kk = 0; kk < N; ++kk) {

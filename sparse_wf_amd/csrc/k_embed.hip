@@ -270,6 +270,7 @@ __global__ void __launch_bounds__(256) k_messages(const float* __restrict__ r, i
   __shared__ float s_g[MSG_CHUNK][NRF][4];
   __shared__ const float* s_src[MSG_CHUNK];
   __shared__ int s_same[MSG_CHUNK];
+  __shared__ int s_nup;  // up-spin neighbours in the chunk (they come first)
   const int g = blockIdx.x;
   const int n_dn = N - n_up;
   int w, i;
@@ -325,7 +326,51 @@ __global__ void __launch_bounds__(256) k_messages(const float* __restrict__ r, i
       s_src[threadIdx.x] = m5_of(j);
       s_same[threadIdx.x] = same;
     }
+    if (!TAN && threadIdx.x < 64) {
+      const bool upj = threadIdx.x < cn && nbr[p0 + c0 + threadIdx.x] < n_up;
+      const uint64_t bal = __ballot(upj);
+      if (threadIdx.x == 0) s_nup = __popcll(bal);
+    }
     __syncthreads();
+    if constexpr (!TAN) {
+      // Neighbours ascend in j and up-spin electrons come first, so the chunk is a run of
+      // up-spin neighbours followed by a run of down-spin ones; each run uses one weight
+      // set.  Message rows are loaded 8 neighbours ahead.  sigmoid from the hardware exp2
+      // and reciprocal (a few ulp).
+      const int nup = s_nup;
+      for (int run = 0; run < 2; ++run) {
+        const int lo = run ? nup : 0, hi = run ? cn : nup;
+        const bool same = up_i == (run == 0);
+        float wg[W1], wf[4];
+#pragma unroll
+        for (int q = 0; q < W1; ++q) wg[q] = same ? wgs[q] : wgd[q];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) wf[m] = same ? wfs[m] : wfd[m];
+        const float bf = same ? bfs : bfd, mc = same ? mcs[0] : mcd[0];
+        const int off = (same ? 3 : 4) * F + f;
+        for (int k0 = lo; k0 < hi; k0 += 8) {
+          float mn[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) mn[u] = k0 + u < hi ? s_src[k0 + u][off] : 0.f;
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            const int kk = k0 + u;
+            if (kk < hi) {
+              float g = 0.f, e = 0.f;
+#pragma unroll
+              for (int q = 0; q < W1; ++q) g += s_beta[kk][0][q] * wg[q];
+#pragma unroll
+              for (int m = 0; m < 4; ++m) e += s_g[kk][0][m] * wf[m];
+              const float ux = e + bf + mc + mn[u];
+              const float sg = __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-1.4426950408889634f * ux));
+              acc[0] += g * (ux * sg);
+            }
+          }
+        }
+      }
+      __syncthreads();
+      continue;
+    }
     // neighbours in groups of U: the message rows of the whole group are loaded first
     constexpr int U = TAN ? 2 : 8;
     for (int k0 = 0; k0 < cn; k0 += U) {

@@ -96,10 +96,14 @@ __global__ void __launch_bounds__(256) k_gemm_f32(const float* __restrict__ A, i
                                                   const float* __restrict__ Wt, const float* __restrict__ bias,
                                                   float* __restrict__ C, int64_t ldc, int64_t M, int N, int K,
                                                   const uint8_t* __restrict__ rowflag, int period, int nT,
-                                                  DevRows dr) {
+                                                  DevRows dr, int64_t sA, int64_t sW, int64_t sC) {
   __shared__ __attribute__((aligned(16))) float As[GT_M][LDA_F];
   __shared__ __attribute__((aligned(16))) float Bs[GT_N][LDA_F];  // Bs[n][k]
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  // batch entry blockIdx.z: operands offset by the batch strides
+  A += blockIdx.z * sA;
+  Wt += blockIdx.z * sW;
+  C += blockIdx.z * sC;
   int64_t mt;
   int nt;
   tile_of(mt, nt, nT);
@@ -361,9 +365,24 @@ void launch_gemm_f32(hipStream_t s, const float* A, int64_t lda, const float* W,
   const int nT = (N + GT_N - 1) / GT_N;
   const int64_t blocks = ((M + GT_M - 1) / GT_M) * nT;
   if ((K % GT_K) == 0 && (lda % 4) == 0)
-    k_gemm_f32<true><<<(unsigned)blocks, 256, 0, s>>>(A, lda, Wt, bias, C, ldc, M, N, K, rowflag, period, nT, dr);
+    k_gemm_f32<true><<<(unsigned)blocks, 256, 0, s>>>(A, lda, Wt, bias, C, ldc, M, N, K, rowflag, period, nT, dr, 0, 0,
+                                                      0);
   else
-    k_gemm_f32<false><<<(unsigned)blocks, 256, 0, s>>>(A, lda, Wt, bias, C, ldc, M, N, K, rowflag, period, nT, dr);
+    k_gemm_f32<false><<<(unsigned)blocks, 256, 0, s>>>(A, lda, Wt, bias, C, ldc, M, N, K, rowflag, period, nT, dr, 0,
+                                                       0, 0);
+}
+
+// nb independent products C_b = A_b W_b (b < nb; operands at strides sA, sW (of Wt), sC) in
+// one launch; every product is computed exactly as by launch_gemm_f32 (N > 16).
+void launch_gemm_f32_batched(hipStream_t s, const float* A, int64_t lda, int64_t sA, const float* Wt, int64_t sW,
+                             float* C, int64_t ldc, int64_t sC, int64_t M, int N, int K, int nb) {
+  if (M == 0 || nb == 0) return;
+  const int nT = (N + GT_N - 1) / GT_N;
+  const dim3 grid((unsigned)(((M + GT_M - 1) / GT_M) * nT), 1, (unsigned)nb);
+  if ((K % GT_K) == 0 && (lda % 4) == 0)
+    k_gemm_f32<true><<<grid, 256, 0, s>>>(A, lda, Wt, nullptr, C, ldc, M, N, K, nullptr, 0, nT, DevRows(), sA, sW, sC);
+  else
+    k_gemm_f32<false><<<grid, 256, 0, s>>>(A, lda, Wt, nullptr, C, ldc, M, N, K, nullptr, 0, nT, DevRows(), sA, sW, sC);
 }
 
 // Double-buffered form: two LDS stages of KC (16 or 32) k each; chunk c is multiplied from

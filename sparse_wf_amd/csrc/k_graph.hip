@@ -65,8 +65,6 @@ __global__ void __launch_bounds__(256) k_nbr(const float* __restrict__ r, int B,
   if (!FILL && lane == 0) cnt[g] = count;
 }
 
-// Exclusive scan of the neighbour counts in group order (single workgroup, 1024 threads).
-// ptr[g] = pairs before group g; ptr[G] = total pairs; ptr[G+1] = largest count.
 // Exclusive prefix sum of the neighbour counts: ptr[0..G) offsets, ptr[G] total,
 // ptr[G+1] largest count.  One block walks the counts in tiles of 4096 (4 consecutive per
 // thread, coalesced): wave prefix sums by shuffles, one LDS pass over the 16 wave totals.

@@ -325,20 +325,21 @@ __global__ void __launch_bounds__(256) k_wb_s(const float* __restrict__ p_orb, c
 }
 
 // Woodbury preparation, second kernel: S^-1 (in place in `sbuf`), log|det S| and its sign
-// by Gauss-Jordan elimination with partial pivoting, one wavefront per (walker, det) with
-// S in registers: element e = lane + 64 kk of the a x a matrix (row e / a, column e % a) is
-// val[kk], its indices packed in idx[kk] = column | row << 8.  Pivot rows are chosen
-// without swaps: step c picks the unused row p with the largest |S[p][c]| (|S| rounded to
-// FP32 with its 7 lowest mantissa bits replaced by the row, so the lowest row wins among
-// near-ties), scales it and eliminates column c from every other row; the pivot column and
-// the scaled pivot row are exchanged through LDS.  The result in place is (P S)^-1 with row
-// c of P S in row piv[c], so S^-1[i][piv[c]] = stored[piv[i]][c] and det S = sign(piv)
-// prod(pivots).  A launch with KM slots handles the walkers with ceil(a^2 / 64) <= KM
-// (KM = 16, a <= 32) or > 16 (KM = 64).
-template <int KM>
-__global__ void __launch_bounds__(64) k_wb_gj(const int32_t* __restrict__ cu, const int32_t* __restrict__ cd, int D,
-                                              double* __restrict__ sbuf, double* __restrict__ ldS,
-                                              double* __restrict__ sgS) {
+// by Gauss-Jordan elimination with partial pivoting, one block of NW wavefronts per
+// (walker, det) with S in registers: element e = tid + 64 NW kk of the a x a matrix (row
+// e / a, column e % a) is val[kk], its indices packed in idx[kk] = column | row << 8.
+// Pivot rows are chosen without swaps: step c picks the unused row p with the largest
+// |S[p][c]| (|S| rounded to FP32 with its 7 lowest mantissa bits replaced by the row, so
+// the lowest row wins among near-ties; every wave finds the same p), scales it and
+// eliminates column c from every other row; the pivot column and the scaled pivot row are
+// exchanged through LDS.  The result in place is (P S)^-1 with row c of P S in row piv[c],
+// so S^-1[i][piv[c]] = stored[piv[i]][c] and det S = sign(piv) prod(pivots).  NW = 1
+// handles a <= 32, NW = 4 the larger a.
+template <int NW>
+__global__ void __launch_bounds__(64 * NW) k_wb_gj(const int32_t* __restrict__ cu, const int32_t* __restrict__ cd,
+                                                   int D, double* __restrict__ sbuf, double* __restrict__ ldS,
+                                                   double* __restrict__ sgS) {
+  constexpr int KM = 16;  // slots per lane: 64 NW KM >= a^2
   // entry kWbCap of s_f / s_pr is a sink for the lanes whose element is not exchanged, so
   // the passes below run without branches
   __shared__ double s_f[kWbCap + 1];
@@ -349,16 +350,16 @@ __global__ void __launch_bounds__(64) k_wb_gj(const int32_t* __restrict__ cu, co
   const int wd = blockIdx.x;
   const int w = wd / D;
   const int a = cu[w] + cd[w];
-  const int a2 = a * a, nk = (a2 + 63) >> 6;
-  if (nk > KM || (KM > 16 && nk <= 16)) return;
-  const int lane = threadIdx.x;
+  if (NW == 1 ? a > 32 : a <= 32) return;
+  const int a2 = a * a, nk = (a2 + 64 * NW - 1) / (64 * NW);
+  const int tid = threadIdx.x, lane = tid & 63;
   const uint32_t ma = ((1u << 20) + a - 1) / a;  // e / a == (e * ma) >> 20 for e < 4096
   double* SB = sbuf + (size_t)wd * kWbCap * kWbCap;
   double val[KM];
   int idx[KM];
 #pragma unroll
   for (int kk = 0; kk < KM; ++kk) {
-    const int e = lane + 64 * kk;
+    const int e = tid + 64 * NW * kk;
     const int r = (int)(__umul24((uint32_t)e, ma) >> 20), cc = e - __mul24(r, a);
     const bool in = kk < nk && e < a2;
     idx[kk] = in ? (cc | (r << 8)) : kSink;
@@ -371,12 +372,13 @@ __global__ void __launch_bounds__(64) k_wb_gj(const int32_t* __restrict__ cu, co
 #pragma unroll
     for (int kk = 0; kk < KM; ++kk)
       if (kk < nk) s_f[(idx[kk] & 255) == c ? idx[kk] >> 8 : kWbCap] = val[kk];
+    if (NW > 1) __syncthreads();
     const double fr = lane < a ? s_f[lane] : 0.0;  // S[lane][c]
     const uint32_t fb = __float_as_uint((float)fabs(fr));
     const uint32_t key = (lane < a && !used) ? ((fb & ~127u) | (uint32_t)(127 - lane)) : 0u;
     const int p = 127 - (int)(wave_max_u32(key) & 127u);
     used = used || lane == p;
-    if (lane == 0) s_piv[c] = p;
+    if (tid == 0) s_piv[c] = p;
     const double pv = bcast_lane(fr, p);
     nneg += pv < 0.0;
     int ee;
@@ -389,6 +391,7 @@ __global__ void __launch_bounds__(64) k_wb_gj(const int32_t* __restrict__ cu, co
         const int cc = idx[kk] & 255;
         s_pr[(idx[kk] >> 8) == p ? cc : kWbCap] = cc == c ? ip : val[kk] * ip;
       }
+    if (NW > 1) __syncthreads();
 #pragma unroll
     for (int kk = 0; kk < KM; ++kk)
       if (kk < nk) {
@@ -397,19 +400,21 @@ __global__ void __launch_bounds__(64) k_wb_gj(const int32_t* __restrict__ cu, co
         const double y = cc == c ? -f * ip : val[kk] - f * pr;
         val[kk] = r == p ? pr : y;
       }
+    if (NW > 1) __syncthreads();
   }
   int inv = 0;
-  if (lane < a) {
-    const int pk = s_piv[lane];
-    s_qinv[pk] = lane;
-    for (int q = lane + 1; q < a; ++q) inv += s_piv[q] < pk;  // inversions of the permutation
+  if (tid < a) {
+    const int pk = s_piv[tid];
+    s_qinv[pk] = tid;
+    for (int q = tid + 1; q < a; ++q) inv += s_piv[q] < pk;  // inversions of the permutation
   }
+  if (NW > 1) __syncthreads();
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) inv += __shfl_xor(inv, off, 64);
 #pragma unroll
   for (int kk = 0; kk < KM; ++kk)
     if (kk < nk && idx[kk] != kSink) SB[s_qinv[idx[kk] >> 8] * kWbCap + s_piv[idx[kk] & 255]] = val[kk];
-  if (lane == 0) {
+  if (tid == 0) {
     ldS[wd] = log(mant) + (double)ex2 * 0.6931471805599453;
     sgS[wd] = ((inv + nneg) & 1) ? -1.0 : 1.0;
   }
@@ -420,8 +425,8 @@ void launch_wb_prep(hipStream_t s, const float* p_orb, const float* p_envk, cons
                     const int32_t* rowof, int B, int N, int n_up, int D, int k, double* vs, double* sbuf, double* ldS,
                     double* sgS) {
   k_wb_s<<<B * D, 256, 0, s>>>(p_orb, p_envk, st_env, st_A, st_Ainv, aff, cu, cd, rowof, B, N, n_up, D, k, vs, sbuf);
-  k_wb_gj<16><<<B * D, 64, 0, s>>>(cu, cd, D, sbuf, ldS, sgS);
-  if (N > 32) k_wb_gj<64><<<B * D, 64, 0, s>>>(cu, cd, D, sbuf, ldS, sgS);
+  k_wb_gj<1><<<B * D, 64, 0, s>>>(cu, cd, D, sbuf, ldS, sgS);
+  if (N > 32) k_wb_gj<4><<<B * D, 256, 0, s>>>(cu, cd, D, sbuf, ldS, sgS);
 }
 
 // X^T = S^-T P^T (X = Ainv[:, aff] S^-1) for accepted walkers, in A-operand tiles of the

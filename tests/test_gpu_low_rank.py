@@ -202,6 +202,42 @@ def test_alkane_updates_and_sweep(gpu_device):
     assert 0.0 < stm["pmove"] < 1.0
 
 
+def test_many_changed_electrons(gpu_device):
+    """64 hydrogen atoms on a 4 x 4 x 4 cube of side 4.8 bohr with an 8 bohr cutoff: a move
+    changes most of the 64 electrons, which runs the four-wave Gauss-Jordan elimination and
+    the 3- and 4-tile classes of the inverse update (the capacity is all 64 electrons, so
+    nothing can overflow).  Updates against fresh evaluations, then a sweep."""
+    import itertools
+
+    from sparse_wf_amd import systems
+    from sparse_wf_amd.config import FireConfig
+
+    R = 1.6 * np.array(list(itertools.product(range(4), repeat=3)), np.float64)
+    mol = systems.make_molecule(R, np.ones(len(R), np.int64), name="h64_cube")
+    cfg = FireConfig(mol=mol, cutoff=8.0, feature_dim=64, n_determinants=2)
+    wf = FireWaveFunction(cfg, device=gpu_device)
+    r = systems.init_electrons(mol, 8, stddev=0.5, seed=6)
+    prm = wf.init(0, r[0])
+    wf.upload(prm)
+    (_, _), state = wf.log_psi_with_state(prm, r, precision="fp32")
+    cur = r.copy()
+    rng = np.random.default_rng(8)
+    for k in (0, 17, 30, 45, 59):
+        cur = cur.copy()
+        cur[:, k] += rng.normal(scale=0.3, size=(len(cur), 3)).astype(np.float32)
+        (sg_u, lp_u), state = wf.log_psi_low_rank_update(prm, cur, np.array([k]), state=state)
+        sg_f, lp_f = wf.signed(prm, cur, precision="fp32")
+        _close(lp_u, lp_f, 1e-8)
+        assert torch.equal(sg_u.cpu(), sg_f.cpu())
+    st = wf.state_stats()
+    print(f"  mean changed {st['mean_changed']:.1f} max {st['max_changed']}", flush=True)
+    assert st["mean_changed"] > 32 and st["max_changed"] > 48
+    r_end, stm = wf.mcmc_steps_low_rank(prm, r, cfg.n_el, width=0.3, seed=3, precision="fp32")
+    sg_f, lp_f = wf.signed(prm, r_end, precision="fp32")
+    _close(stm["logpsi"], lp_f, 1e-8)
+    assert 0.0 < stm["pmove"] < 1.0
+
+
 def test_sweep_graph_matches_eager(setup):
     """Whole sweeps replayed from the captured hipGraph give bit-identical walkers, log psi
     and acceptance counts to the same steps issued eagerly, over several replays with eager
