@@ -147,13 +147,13 @@ struct fire_handle {
     // compact rows of the affected electrons
     DevBuf pcnt, pptr, pnbr, p_hc, p_x, p_h, p_y, p_orb, p_jr, p_qv, p_av;
     // determinant and Jastrow of the proposal
-    DevBuf phin, sbuf, ldS, sgS, jsum, rin, ry1, ry2, att_out, cval, clap, cgrad, epot;
+    DevBuf phin, sbuf, ldS, sgS, ccusp, cdelta, jsum, rin, ry1, ry2, att_out, cval, clap, cgrad, epot;
     DevBuf p_ld, p_sg, p_logpsi, p_sign, accept, wbscr;
     void release() {
       DevBuf* b[] = {&r,     &M5,    &hh,    &orb,     &env,   &A,      &Ainv,  &ld,    &sg,     &jr,    &av,
                      &logpsi, &sign, &p_r,   &p_rk,    &p_u,   &aff,    &cu,    &cd,    &offu,   &offd,  &tail,
                      &list,  &rowof, &klist, &kT5,     &kZ2,   &kT5b,   &p_M5k, &kcoef, &p_envk, &pcnt,  &pptr,
-                     &pnbr,  &p_hc,  &p_x,   &p_h,     &p_y,   &p_orb,  &p_jr,  &p_qv,  &p_av,   &phin,  &sbuf,
+                     &pnbr,  &p_hc,  &p_x,   &p_h,     &p_y,   &p_orb,  &p_jr,  &p_qv,  &p_av,   &phin,  &sbuf, &ccusp, &cdelta,
                      &ldS,   &sgS,   &jsum,  &rin,     &ry1,   &ry2,    &att_out, &cval, &clap,  &cgrad, &epot,
                      &p_ld,  &p_sg,  &p_logpsi, &p_sign, &accept, &wbscr, &dstats, &doverflow, &dstep, &nacc};
       if (graph) (void)hipGraphExecDestroy(graph);
@@ -757,6 +757,14 @@ void state_init(Ctx& c, const float* r_dev, int B) {
   S.clap.ensure(sizeof(double) * (size_t)B * N);
   S.cgrad.ensure(sizeof(double) * (size_t)B * N * 3);
   S.epot.ensure(sizeof(double) * (size_t)B * N);
+  S.ccusp.ensure(sizeof(double) * (size_t)B);
+  S.cdelta.ensure(sizeof(double) * (size_t)B);
+  if (h->cusp) {
+    // the walkers' e-e cusp totals; proposals add the change of the moved electron's pairs
+    launch_cusp_coulomb(c.s, S.r.as<float>(), h->R, h->Z, B, N, h->n_up, h->M, h->cusp, h->P[h->iCUSP],
+                        S.cval.as<double>(), S.clap.as<double>(), S.cgrad.as<double>(), S.epot.as<double>());
+    launch_mc_cusp_total(c.s, S.cval.as<double>(), B, N, S.ccusp.as<double>());
+  }
   for (DevBuf* x : {&S.p_ld, &S.p_sg}) x->ensure(sizeof(double) * (size_t)B * D);
   for (DevBuf* x : {&S.p_logpsi, &S.p_sign}) x->ensure(sizeof(double) * B);
   launch_fill_i32(c.s, S.doverflow.as<int32_t>(), 1, 0);
@@ -897,13 +905,14 @@ void state_propose(Ctx& c, int k, int mode, const float* rk_in, float width, uin
     att_scale = h->P[al + 2];
     att_zb = h->P[al + 3];
   }
-  // e-e cusp of the new positions (recomputed in full, as the reference does)
+  // e-e cusp of the new positions: the state's total plus the change of the pairs (k, j)
+  // (the reference recomputes the whole pair sum; equal up to FP64 rounding)
   if (h->cusp)
-    launch_cusp_coulomb(c.s, S.p_r.as<float>(), h->R, h->Z, B, N, nu, h->M, h->cusp, h->P[h->iCUSP],
-                        S.cval.as<double>(), S.clap.as<double>(), S.cgrad.as<double>(), S.epot.as<double>());
+    launch_mc_cusp_delta(c.s, S.r.as<float>(), S.p_r.as<float>(), B, N, nu, k, h->P[h->iCUSP], S.cdelta.as<double>());
   launch_mc_accept(c.s, B, D, h->cfg.use_mlp_jastrow, h->cfg.use_log_jastrow, h->has_att, h->cusp, S.ld.as<double>(),
                    S.sg.as<double>(), S.ldS.as<double>(), S.sgS.as<double>(), S.jsum.as<double>(),
-                   S.att_out.as<float>(), att_scale, att_zb, S.cval.as<double>(), N, S.logpsi.as<double>(),
+                   S.att_out.as<float>(), att_scale, att_zb, S.ccusp.as<double>(), S.cdelta.as<double>(),
+                   S.logpsi.as<double>(),
                    S.p_u.as<double>(), mode, S.p_ld.as<double>(), S.p_sg.as<double>(), S.p_logpsi.as<double>(),
                    S.p_sign.as<double>(), S.accept.as<int32_t>(), n_acc);
   S.pending = true;
@@ -926,7 +935,7 @@ void state_commit(Ctx& c) {
                    S.hh.as<float>(), S.orb.as<float>(), h->has_mlp ? S.jr.as<float>() : nullptr,
                    h->has_att ? S.av.as<float>() : nullptr, S.r.as<float>(), S.M5.as<float>(), S.env.as<float>(),
                    S.ld.as<double>(), S.sg.as<double>(), S.logpsi.as<double>(), S.sign.as<double>(),
-                   S.p_r.as<float>());
+                   S.p_r.as<float>(), S.cdelta.as<double>(), h->cusp ? S.ccusp.as<double>() : nullptr);
   S.pending = false;
 }
 

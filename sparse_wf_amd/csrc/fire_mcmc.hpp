@@ -37,16 +37,21 @@ void launch_mc_jsum(hipStream_t s, const float* st_jr, const float* p_jr, const 
                     double* jsum, float* rin);
 void launch_mc_accept(hipStream_t s, int B, int D, int use_mlp, int use_log, int use_att, int cusp, const double* st_ld,
                       const double* st_sg, const double* ldS, const double* sgS, const double* jsum,
-                      const float* att_out, const float* att_scale, const float* att_zb, const double* cval, int N,
-                      const double* st_logpsi, const double* p_u, int mode, double* p_ld, double* p_sg,
-                      double* p_logpsi, double* p_sign, int32_t* accept, int32_t* n_acc);
+                      const float* att_out, const float* att_scale, const float* att_zb, const double* ccusp,
+                      const double* cdelta, const double* st_logpsi, const double* p_u, int mode, double* p_ld,
+                      double* p_sg, double* p_logpsi, double* p_sign, int32_t* accept, int32_t* n_acc);
+// change of the e-e cusp Jastrow when electron k moves (st_r -> p_r); per-walker cusp totals
+void launch_mc_cusp_delta(hipStream_t s, const float* st_r, const float* p_r, int B, int N, int n_up, int k,
+                          const float* cusp_params, double* cdelta);
+void launch_mc_cusp_total(hipStream_t s, const double* cval, int B, int N, double* ccusp);
 void launch_mc_commit(hipStream_t s, const int32_t* accept, const int2* list, int T, const int32_t* dT, int B, int N,
                       int n_up, int D,
                       int F, int ND, int nav, int k, const float* p_h, const float* p_orb, const float* p_jr,
                       const float* p_av, const float* p_rk, const float* p_M5k, const float* p_envk, const double* p_ld,
                       const double* p_sg, const double* p_logpsi, const double* p_sign, float* st_h, float* st_orb,
                       float* st_jr, float* st_av, float* st_r, float* st_M5, float* st_env, double* st_ld,
-                      double* st_sg, double* st_logpsi, double* st_sign, float* p_r);
+                      double* st_sg, double* st_logpsi, double* st_sign, float* p_r, const double* cdelta,
+                      double* ccusp);
 void launch_fill_i32(hipStream_t s, int32_t* p, int n, int v);
 
 }  // namespace fire

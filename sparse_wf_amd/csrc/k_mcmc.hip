@@ -699,36 +699,105 @@ void launch_wb_update(hipStream_t s, const int32_t* accept, const int32_t* aff, 
 // Per walker: Jastrow sums with the affected rows replaced by the proposal rows
 // (MLP Jastrow components, attention normaliser/values per spin) and the attention readout
 // input vals/norm.  Rows are state rows (group order) unless rowof >= 0.
-__global__ void k_mc_jsum(const float* __restrict__ st_jr, const float* __restrict__ p_jr,
-                          const float* __restrict__ st_av, const float* __restrict__ p_av,
-                          const int32_t* __restrict__ rowof, int B, int N, int n_up, int H, int Dk, int has_mlp,
-                          int has_att, double* __restrict__ jsum, float* __restrict__ rin) {
+__global__ void __launch_bounds__(256) k_mc_jsum(const float* __restrict__ st_jr, const float* __restrict__ p_jr,
+                                                 const float* __restrict__ st_av, const float* __restrict__ p_av,
+                                                 const int32_t* __restrict__ rowof, int B, int N, int n_up, int H,
+                                                 int Dk, int has_mlp, int has_att, double* __restrict__ jsum,
+                                                 float* __restrict__ rin) {
   const int w = blockIdx.x;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int HD = H * Dk, nc = H + HD;
-  if (has_mlp && threadIdx.x < 2) {
-    // per-spin sums in electron order, then their sum (the order of the full evaluation)
-    const int c = threadIdx.x;
-    double acc[2] = {0.0, 0.0};
-    for (int i = 0; i < N; ++i) {
-      const int t = rowof[(size_t)w * N + i];
-      acc[i < n_up ? 0 : 1] +=
-          t >= 0 ? p_jr[(size_t)t * 2 + c] : st_jr[(size_t)group_of(w, i, B, n_up, N - n_up) * 2 + c];
+  auto row_of = [&](int i, const float* st, const float* pp, int ld) {
+    const int t = rowof[(size_t)w * N + i];
+    return t >= 0 ? pp + (size_t)t * ld : st + (size_t)group_of(w, i, B, n_up, N - n_up) * ld;
+  };
+  if (has_mlp && wv == 0) {
+    // per-spin sums, then their sum (the grouping of the full evaluation); lanes take the
+    // electrons and the partial sums are added by a fixed shuffle tree
+    double a[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+    for (int i = lane; i < N; i += 64) {
+      const float* row = row_of(i, st_jr, p_jr, 2);
+      const int sp = i < n_up ? 0 : 1;
+      a[sp][0] += row[0];
+      a[sp][1] += row[1];
     }
-    jsum[(size_t)w * 2 + c] = acc[0] + acc[1];
+#pragma unroll
+    for (int sp = 0; sp < 2; ++sp)
+#pragma unroll
+      for (int c = 0; c < 2; ++c) a[sp][c] = warp_sum(a[sp][c]);
+    if (lane < 2) jsum[(size_t)w * 2 + lane] = a[0][lane] + a[1][lane];
   }
   if (!has_att) return;
-  for (int col = threadIdx.x; col < 2 * HD; col += blockDim.x) {
-    const int sp = col / HD, hk = col - sp * HD, h = hk / Dk;
-    const int i0 = sp == 0 ? 0 : n_up, i1 = sp == 0 ? n_up : N;
+  // attention read-in sums: one thread per column, electrons in order; the rows of a chunk
+  // of 128 electrons are looked up once into LDS
+  __shared__ const float* s_row[128];
+  const int ncol = 2 * HD;
+  for (int col0 = 0; col0 < ncol; col0 += blockDim.x) {
+    const int col = col0 + (int)threadIdx.x;
+    const int sp = col < HD ? 0 : 1, hk = col - sp * HD, h = hk / Dk;
     double nsum = 0.0, vsum = 0.0;
-    for (int i = i0; i < i1; ++i) {
-      const int t = rowof[(size_t)w * N + i];
-      const float* row = t >= 0 ? p_av + (size_t)t * nc : st_av + (size_t)group_of(w, i, B, n_up, N - n_up) * nc;
-      nsum += row[h];
-      vsum += row[H + hk];
+    for (int i0 = 0; i0 < N; i0 += 128) {
+      const int ni = min(128, N - i0);
+      __syncthreads();
+      if ((int)threadIdx.x < ni) s_row[threadIdx.x] = row_of(i0 + threadIdx.x, st_av, p_av, nc);
+      __syncthreads();
+      if (col < ncol) {
+        const int lo = max(i0, sp ? n_up : 0) - i0, hi = min(i0 + ni, sp ? N : n_up) - i0;
+#pragma unroll 8
+        for (int ii = lo; ii < hi; ++ii) {
+          const float* row = s_row[ii];
+          nsum += row[h];
+          vsum += row[H + hk];
+        }
+      }
     }
-    rin[(size_t)w * 2 * HD + col] = (float)(vsum * (1.0 / nsum));
+    if (col < ncol) rin[(size_t)w * ncol + col] = (float)(vsum * (1.0 / nsum));
   }
+}
+
+// Change of the e-e cusp Jastrow sum_{i<j} fa al^2 / (al + d_ij) (FP64) when electron k of
+// walker w moves from st_r to p_r: only the pairs (k, j) change.  One wave per walker,
+// lanes over j, fixed shuffle-tree reduction.
+__global__ void __launch_bounds__(64) k_mc_cusp_delta(const float* __restrict__ st_r, const float* __restrict__ p_r,
+                                                      int N, int n_up, int k, const float* __restrict__ cp,
+                                                      double* __restrict__ cdelta) {
+  const int w = blockIdx.x, lane = threadIdx.x;
+  const float* ro = st_r + (size_t)w * N * 3;
+  const float* rn = p_r + (size_t)w * N * 3;
+  const double xo = ro[3 * k], yo = ro[3 * k + 1], zo = ro[3 * k + 2];
+  const double xn = rn[3 * k], yn = rn[3 * k + 1], zn = rn[3 * k + 2];
+  double acc = 0.0;
+  for (int j = lane; j < N; j += 64) {
+    if (j == k) continue;
+    const double xj = ro[3 * j], yj = ro[3 * j + 1], zj = ro[3 * j + 2];
+    const bool same = (k < n_up) == (j < n_up);
+    const double al = same ? cp[0] : cp[1];
+    const double fa = same ? cp[2] : cp[3];
+    const double d_o = sqrt((xo - xj) * (xo - xj) + (yo - yj) * (yo - yj) + (zo - zj) * (zo - zj));
+    const double d_n = sqrt((xn - xj) * (xn - xj) + (yn - yj) * (yn - yj) + (zn - zj) * (zn - zj));
+    acc += fa * al * al * (1.0 / (al + d_n) - 1.0 / (al + d_o));
+  }
+  acc = warp_sum(acc);
+  if (lane == 0) cdelta[w] = acc;
+}
+
+void launch_mc_cusp_delta(hipStream_t s, const float* st_r, const float* p_r, int B, int N, int n_up, int k,
+                          const float* cusp_params, double* cdelta) {
+  k_mc_cusp_delta<<<B, 64, 0, s>>>(st_r, p_r, N, n_up, k, cusp_params, cdelta);
+}
+
+// Per walker: cusp total sum_i cval[i] (the pair sums of the full evaluation).
+__global__ void __launch_bounds__(64) k_mc_cusp_total(const double* __restrict__ cval, int N,
+                                                      double* __restrict__ ccusp) {
+  const int w = blockIdx.x;
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < N; i += 64) acc += cval[(size_t)w * N + i];
+  acc = warp_sum(acc);
+  if (threadIdx.x == 0) ccusp[w] = acc;
+}
+
+void launch_mc_cusp_total(hipStream_t s, const double* cval, int B, int N, double* ccusp) {
+  k_mc_cusp_total<<<B, 64, 0, s>>>(cval, N, ccusp);
 }
 
 void launch_mc_jsum(hipStream_t s, const float* st_jr, const float* p_jr, const float* st_av, const float* p_av,
@@ -745,7 +814,8 @@ __global__ void k_mc_accept(int B, int D, int use_mlp, int use_log, int use_att,
                             const double* __restrict__ ldS, const double* __restrict__ sgS,
                             const double* __restrict__ jsum, const float* __restrict__ att_out,
                             const float* __restrict__ att_scale, const float* __restrict__ att_zb,
-                            const double* __restrict__ cval, int N, const double* __restrict__ st_logpsi,
+                            const double* __restrict__ ccusp, const double* __restrict__ cdelta,
+                            const double* __restrict__ st_logpsi,
                             const double* __restrict__ p_u, int mode, double* __restrict__ p_ld,
                             double* __restrict__ p_sg, double* __restrict__ p_logpsi, double* __restrict__ p_sign,
                             int32_t* __restrict__ accept, int32_t* __restrict__ n_acc) {
@@ -774,11 +844,7 @@ __global__ void k_mc_accept(int B, int D, int use_mlp, int use_log, int use_att,
     logpsi += log(fabs(J[1]));
     sign *= J[1] < 0 ? -1.0 : 1.0;
   }
-  if (cusp) {
-    double cv = 0.0;
-    for (int i = 0; i < N; ++i) cv += cval[(size_t)w * N + i];
-    logpsi += cv;
-  }
+  if (cusp) logpsi += ccusp[w] + cdelta[w];
   p_logpsi[w] = logpsi;
   p_sign[w] = sign;
   int acc = 0;
@@ -789,12 +855,12 @@ __global__ void k_mc_accept(int B, int D, int use_mlp, int use_log, int use_att,
 
 void launch_mc_accept(hipStream_t s, int B, int D, int use_mlp, int use_log, int use_att, int cusp, const double* st_ld,
                       const double* st_sg, const double* ldS, const double* sgS, const double* jsum,
-                      const float* att_out, const float* att_scale, const float* att_zb, const double* cval, int N,
-                      const double* st_logpsi, const double* p_u, int mode, double* p_ld, double* p_sg,
-                      double* p_logpsi, double* p_sign, int32_t* accept, int32_t* n_acc) {
+                      const float* att_out, const float* att_scale, const float* att_zb, const double* ccusp,
+                      const double* cdelta, const double* st_logpsi, const double* p_u, int mode, double* p_ld,
+                      double* p_sg, double* p_logpsi, double* p_sign, int32_t* accept, int32_t* n_acc) {
   k_mc_accept<<<(B + 63) / 64, 64, 0, s>>>(B, D, use_mlp, use_log, use_att, cusp, st_ld, st_sg, ldS, sgS, jsum, att_out,
-                                           att_scale, att_zb, cval, N, st_logpsi, p_u, mode, p_ld, p_sg, p_logpsi,
-                                           p_sign, accept, n_acc);
+                                           att_scale, att_zb, ccusp, cdelta, st_logpsi, p_u, mode, p_ld, p_sg,
+                                           p_logpsi, p_sign, accept, n_acc);
 }
 
 // Commit the compact per-row proposal buffers of accepted walkers into the state.
@@ -824,7 +890,8 @@ __global__ void k_mc_commit_walker(const int32_t* __restrict__ accept, int B, in
                                    const double* __restrict__ p_sign, float* __restrict__ st_r,
                                    float* __restrict__ st_M5, float* __restrict__ st_env, double* __restrict__ st_ld,
                                    double* __restrict__ st_sg, double* __restrict__ st_logpsi,
-                                   double* __restrict__ st_sign, float* __restrict__ p_r) {
+                                   double* __restrict__ st_sign, float* __restrict__ p_r,
+                                   const double* __restrict__ cdelta, double* __restrict__ ccusp) {
   const int w = blockIdx.x;
   float* rk = st_r + ((size_t)w * N + k) * 3;
   if (accept[w]) {
@@ -839,6 +906,7 @@ __global__ void k_mc_commit_walker(const int32_t* __restrict__ accept, int B, in
     if (threadIdx.x == 0) {
       st_logpsi[w] = p_logpsi[w];
       st_sign[w] = p_sign[w];
+      if (ccusp) ccusp[w] += cdelta[w];
     }
   }
   if (threadIdx.x < 3) p_r[((size_t)w * N + k) * 3 + threadIdx.x] = rk[threadIdx.x];
@@ -850,12 +918,14 @@ void launch_mc_commit(hipStream_t s, const int32_t* accept, const int2* list, in
                       const float* p_av, const float* p_rk, const float* p_M5k, const float* p_envk, const double* p_ld,
                       const double* p_sg, const double* p_logpsi, const double* p_sign, float* st_h, float* st_orb,
                       float* st_jr, float* st_av, float* st_r, float* st_M5, float* st_env, double* st_ld,
-                      double* st_sg, double* st_logpsi, double* st_sign, float* p_r) {
+                      double* st_sg, double* st_logpsi, double* st_sign, float* p_r, const double* cdelta,
+                      double* ccusp) {
   if (T > 0)
     k_mc_commit_rows<<<T, 256, 0, s>>>(accept, list, dT, B, N, n_up, F, ND, nav, p_h, p_orb, p_jr, p_av, st_h,
                                        st_orb, st_jr, st_av);
   k_mc_commit_walker<<<B, 256, 0, s>>>(accept, B, N, n_up, D, 5 * F, ND, k, p_rk, p_M5k, p_envk, p_ld, p_sg, p_logpsi,
-                                       p_sign, st_r, st_M5, st_env, st_ld, st_sg, st_logpsi, st_sign, p_r);
+                                       p_sign, st_r, st_M5, st_env, st_ld, st_sg, st_logpsi, st_sign, p_r, cdelta,
+                                       ccusp);
 }
 
 // fill kernel for counters (usable under stream capture, unlike hipMemsetAsync)
