@@ -80,6 +80,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=192, help="walkers timed on the CPU oracle (FP32)")
     ap.add_argument("--bias-sample", type=int, default=64, help="walkers checked against the FP64 oracle")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--ragged", type=int, default=32, help="geometries of the ragged-batch leg (0: skip)")
+    ap.add_argument("--ragged-sweeps", type=int, default=2, help="timed Metropolis sweeps of the ragged leg")
     ap.add_argument("--serve-requests", type=int, default=16, help="requests of the serving leg (0: skip)")
     ap.add_argument("--serve-qps", type=float, default=4.0, help="Poisson arrival rate of the serving leg")
     ap.add_argument("--serve-nel", default="20,400", help="electron-count range of the serving requests")
@@ -132,7 +134,10 @@ def main():
         for s in range(args.sweeps):
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            r, st = wf.mcmc_steps_low_rank(params, r, N, width=args.width, seed=1, step0=s * N, keep_state=True)
+            # the resident state is rebuilt every 10 sweeps (inside the timed region): long chains
+            # of FP64 low-rank updates drift on the few ill-conditioned walkers otherwise
+            r, st = wf.mcmc_steps_low_rank(params, r, N, width=args.width, seed=1, step0=s * N,
+                                           keep_state=(s % 10 != 0))
             torch.cuda.synchronize()
             if s == 0:
                 t_first = time.perf_counter() - t0  # includes capturing the sweep graph
@@ -156,7 +161,7 @@ def main():
             "logpsi_drift_max": float((lp_state - lp_fresh).abs().max()),
             "what": "single-electron Metropolis steps, FP64 Woodbury updates of the resident state, mixed precision; "
                     "each sweep replayed from one captured hipGraph (the first sweep, which captures it, is timed "
-                    "separately)",
+                    "separately); the state is rebuilt from the positions every 10 sweeps (timed)",
         }
 
     counts, _, _ = wf.neighbours(r)
@@ -258,6 +263,33 @@ def main():
         "roofline": roofline,
         "mcmc": mcmc,
     }
+
+    if args.ragged > 0:
+        # ragged-batch leg: seeded random H/C/N/O chains of 10-200 electrons, 256 walkers each,
+        # every geometry's sweep replayed from its own captured graph on its own stream (all in
+        # flight at once), against the same steps issued eagerly one geometry after the other
+        from sparse_wf_amd.ragged import random_geometries, run_ragged
+
+        log(f"ragged leg: {args.ragged} geometries")
+        geoms = random_geometries(args.ragged, seed=3, n_el=(10, 200), walkers=256, feature_dim=256)
+        g_run = run_ragged(geoms, args.ragged_sweeps, precision="mixed", use_graphs=True)
+        e_run = run_ragged(geoms, args.ragged_sweeps, precision="mixed", use_graphs=False)
+        same = all(np.array_equal(a.cpu().numpy(), b.cpu().numpy())
+                   for a, b in zip(g_run["walkers"], e_run["walkers"]))
+        result["ragged"] = {
+            "workload": f"{args.ragged} random H/C/N/O chains, electrons uniform in [10, 200], 256 walkers each, "
+                        f"F=256, D=4, cutoff 5, {args.ragged_sweeps} Metropolis sweeps (sigma 0.3 bohr), mixed",
+            "electrons": g_run["electrons"],
+            "walker_moves_per_s": g_run["walker_moves_per_s"], "seconds": g_run["seconds"],
+            "eager_walker_moves_per_s": e_run["walker_moves_per_s"], "eager_seconds": e_run["seconds"],
+            "graph_over_eager": g_run["walker_moves_per_s"] / e_run["walker_moves_per_s"],
+            "pmove_mean": float(np.mean(g_run["pmove"])),
+            "walkers_identical_graph_vs_eager": bool(same),
+        }
+        log(f"ragged leg: {g_run['walker_moves_per_s']:.3g} moves/s (graphs), "
+            f"{e_run['walker_moves_per_s']:.3g} (eager)")
+        del g_run, e_run, geoms
+        torch.cuda.empty_cache()
 
     if args.serve_requests > 0:
         # serving leg: seeded Poisson stream of random H/C chains, continuous batching,

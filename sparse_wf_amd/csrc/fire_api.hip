@@ -135,6 +135,7 @@ struct fire_handle {
     hipStream_t gstream = nullptr;
     float g_width = 0.f;
     uint64_t g_seed = 0;
+    uint64_t g_sig = 0;  // state signature at capture
     bool g_count = false;
     // state: positions [B,N,3]; h0/message rows [G,5F]; embedding [G,F]; raw orbitals and
     // envelopes [G,N*D]; Slater matrices and inverses [B,D,N,N] FP64; log|det|, sign [B,D];
@@ -149,6 +150,23 @@ struct fire_handle {
     // determinant and Jastrow of the proposal
     DevBuf phin, sbuf, ldS, sgS, ccusp, cdelta, jsum, rin, ry1, ry2, att_out, cval, clap, cgrad, epot;
     DevBuf p_ld, p_sg, p_logpsi, p_sign, accept, wbscr;
+    // state buffers, walker count and precision: a captured sweep stays valid while these hold
+    uint64_t signature() {
+      uint64_t hsh = 1469598103934665603ull;
+      auto mix = [&](uint64_t v) { hsh = (hsh ^ v) * 1099511628211ull; };
+      for (DevBuf* x : bufs()) mix(reinterpret_cast<uint64_t>(x->p));
+      mix((uint64_t)B);
+      mix((uint64_t)prec);
+      return hsh;
+    }
+    std::vector<DevBuf*> bufs() {
+      return {&r,     &M5,    &hh,    &orb,     &env,   &A,      &Ainv,  &ld,    &sg,     &jr,    &av,
+              &logpsi, &sign, &p_r,   &p_rk,    &p_u,   &aff,    &cu,    &cd,    &offu,   &offd,  &tail,
+              &list,  &rowof, &klist, &kT5,     &kZ2,   &kT5b,   &p_M5k, &kcoef, &p_envk, &pcnt,  &pptr,
+              &pnbr,  &p_hc,  &p_x,   &p_h,     &p_y,   &p_orb,  &p_jr,  &p_qv,  &p_av,   &phin,  &sbuf, &ccusp, &cdelta,
+              &ldS,   &sgS,   &jsum,  &rin,     &ry1,   &ry2,    &att_out, &cval, &clap,  &cgrad, &epot,
+              &p_ld,  &p_sg,  &p_logpsi, &p_sign, &accept, &wbscr, &dstats, &doverflow, &dstep, &nacc};
+    }
     void release() {
       DevBuf* b[] = {&r,     &M5,    &hh,    &orb,     &env,   &A,      &Ainv,  &ld,    &sg,     &jr,    &av,
                      &logpsi, &sign, &p_r,   &p_rk,    &p_u,   &aff,    &cu,    &cd,    &offu,   &offd,  &tail,
@@ -664,11 +682,7 @@ void state_init(Ctx& c, const float* r_dev, int B) {
   auto& S = h->st;
   const int N = h->N, F = h->F, G = B * N, ND = h->ND, D = h->D, W = h->W, HD = h->HD;
   const int nc = h->H + HD;
-  S.ready = S.pending = false;
-  if (S.graph) {
-    HIPCHK(hipGraphExecDestroy(S.graph));
-    S.graph = nullptr;
-  }
+  S.ready = S.pending = false;  // a captured sweep is kept if the buffers stay in place
   S.B = B;
   S.prec = c.prec;
   S.n_proposals = 0;
@@ -970,7 +984,7 @@ void state_steps(Ctx& c, int64_t n_steps, float width, uint64_t seed, int64_t st
   const int64_t n_sweeps = (end - s) / N;
   if (use_graph && n_sweeps > 0) {
     // (re)capture when the sweep parameters change; nodes take the step counter from the device
-    if (!S.graph || S.g_width != width || S.g_seed != seed) {
+    if (!S.graph || S.g_width != width || S.g_seed != seed || S.g_sig != S.signature()) {
       if (S.graph) HIPCHK(hipGraphExecDestroy(S.graph));
       S.graph = nullptr;
       if (!S.gstream) HIPCHK(hipStreamCreateWithFlags(&S.gstream, hipStreamNonBlocking));
@@ -993,6 +1007,7 @@ void state_steps(Ctx& c, int64_t n_steps, float width, uint64_t seed, int64_t st
       HIPCHK(hipGraphDestroy(gr));
       S.g_width = width;
       S.g_seed = seed;
+      S.g_sig = S.signature();
     }
     // the graph's proposals use step = *dstep + k with *dstep = first step of the sweep
     launch_set_i64(c.s, S.dstep.as<int64_t>(), s);

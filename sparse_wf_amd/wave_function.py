@@ -340,12 +340,15 @@ class FireWaveFunction:
                     mean_changed=(s.value / npr.value) if npr.value else 0.0, max_changed=mx.value)
 
     def mcmc_steps_low_rank(self, params, electrons, steps: int, width: float, seed: int = 0, step0: int = 0,
-                            precision=None, keep_state: bool = False):
+                            precision=None, keep_state: bool = False, refresh_sweeps: int = 0):
         """mcmc_steps_low_rank with proposal_single_electron (mcmc.py:100-111, 196-275):
         ``steps`` Metropolis steps, step s moving electron s mod n_el of every walker.
         Returns (electrons, stats) with stats["pmove"] the acceptance rate.  With
         ``keep_state`` the resident state of an earlier call (same walkers) is continued
-        instead of being rebuilt from ``electrons``."""
+        instead of being rebuilt from ``electrons`` (the reference rebuilds it at the start
+        of every call).  ``refresh_sweeps`` > 0 rebuilds the state from the current positions
+        every that many sweeps within the call: the FP64 low-rank updates accumulate rounding
+        in the inverse, which long chains on ill-conditioned walkers make visible."""
         self._ensure(params)
         if not keep_state or getattr(self, "_committed_gen", None) is None:
             self.log_psi_with_state(params, electrons, precision=precision)
@@ -353,8 +356,15 @@ class FireWaveFunction:
             self._pending_gen = None  # a pending proposal is dropped by the next step
         B = self._state_walkers()
         nacc = torch.zeros(B, dtype=torch.int32, device=self.device)
-        _lib.check(self._lib.fire_mcmc_steps(self._h, _stream_ptr(), int(steps), float(width), int(seed) & (2**64 - 1),
-                                             int(step0), _dptr(nacc)))
+        block = int(steps) if refresh_sweeps <= 0 else int(refresh_sweeps) * self.cfg.n_el
+        done = 0
+        while done < steps:
+            if done > 0:
+                self.log_psi_with_state(params, self.state_values()[0], precision=precision)
+            n = min(block, int(steps) - done)
+            _lib.check(self._lib.fire_mcmc_steps(self._h, _stream_ptr(), n, float(width), int(seed) & (2**64 - 1),
+                                                 int(step0) + done, _dptr(nacc)))
+            done += n
         r, sg, lp = self.state_values()
         pmove = float(nacc.double().mean() / steps) if steps else 0.0
         return r, dict(pmove=pmove, n_accept=nacc, logpsi=lp, sign=sg, **self.state_stats())
