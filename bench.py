@@ -208,7 +208,8 @@ def main():
     g_achieved = (g_flops / (g_ms * 1e-3)) / 1e12 if g_ms > 0 else None
     roofline = {
         "bound": "mfma",
-        "kernel": "k_gemm_bf16<6> (every dense contraction of one step; BF16 MFMA on three-plane split operands)",
+        "kernel": "k_gemm_bf16 / k_gemm_bf16_db (every dense contraction of one mixed step: embedding and orbital "
+                  "contractions on three BF16 planes, Jastrow and attention on two; autotuned tile shapes)",
         "achieved": g_achieved,
         "peak": PEAK["bf16"],
         "unit": "TFLOP/s",
@@ -217,9 +218,10 @@ def main():
         "avg_launch_ms": g_ms / g_n if g_n else None,
         "launches_per_step": g_n,
         "algorithmic_flop_per_step": g_flops,
-        "note": "achieved = algorithmic 2*M*N*K per launch (the three-plane split issues 6 MFMAs per product, "
-                "i.e. 6x these flops on the matrix cores) over the HIP-event duration of the launches on "
-                "their stream; whole-step rate in whole_step_tflops",
+        "note": "achieved = algorithmic 2*M*N*K per launch over the HIP-event duration of the launches on "
+                "their stream; the matrix cores execute 6x (three planes) or 3x (two planes) these flops; "
+                "traffic (HBM bytes per launch) is measured separately with rocprofv3 --pmc; whole-step "
+                "rate in whole_step_tflops",
         "whole_step_tflops": flops_w * wps_m / 1e12,
     }
 
