@@ -212,8 +212,20 @@ __global__ void k_act(float* __restrict__ Y, int64_t ld, int ncols, GroupDesc gd
   y[0] = f;
   const int64_t nr = r1 - r0;
   if (nr == 1) return;
+  // jacobian rows 8 at a time: all loads before the stores (the sum keeps the row order)
   float ss = 0.f;
-  for (int64_t k = 2; k < nr; ++k) {
+  int64_t k = 2;
+  for (; k + 8 <= nr; k += 8) {
+    float v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = y[(k + u) * ld];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      ss += v[u] * v[u];
+      y[(k + u) * ld] = v[u] * f1;
+    }
+  }
+  for (; k < nr; ++k) {
     const float v = y[k * ld];
     ss += v * v;
     y[k * ld] = v * f1;

@@ -152,17 +152,28 @@ __global__ void __launch_bounds__(256) k_att_group(const float* __restrict__ QV,
   if (!TAN) return;
   const float a_l = pre(r0 + 1);
   float ss = 0.f, cross = 0.f;
-  for (int64_t rr = r0 + 2; rr < r1; ++rr) {
-    const float a_t = pre(rr);
+  auto row = [&](int64_t rr, float a_t, float vt) {
     const float at = a_t * f1;
     ss += a_t * a_t;
     if (live) {
-      const float vt = QV[rr * ldq + HD + col];
       cross += at * vt;
       AV[rr * lda + H + col] = at * vx + f * vt;
       if (k == 0) AV[rr * lda + h] = at;
     }
+  };
+  // jacobian rows 4 at a time: the rows' Q and V values are loaded before the stores
+  int64_t rr = r0 + 2;
+  for (; rr + 4 <= r1; rr += 4) {
+    float q[4], v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      q[u] = live ? QV[(rr + u) * ldq + col] : 0.f;
+      v[u] = live ? QV[(rr + u) * ldq + HD + col] : 0.f;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) row(rr + u, head_sum(q[u] * key), v[u]);
   }
+  for (; rr < r1; ++rr) row(rr, pre(rr), live ? QV[rr * ldq + HD + col] : 0.f);
   if (live) {
     const float al = a_l * f1 + f2 * ss;
     const float vl = QV[(r0 + 1) * ldq + HD + col];

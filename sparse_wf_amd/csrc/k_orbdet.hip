@@ -83,7 +83,17 @@ __global__ void k_orb_product(float* __restrict__ O, int ND, GroupDesc gd, const
     o[(2 + t) * (int64_t)ND] = ox * et + ex * ot;
   }
   o[ND] = ol * ex + el * ox + 2.f * cross;
-  for (int64_t k = 5; k < r1 - r0; ++k) o[k * ND] *= ex;
+  // neighbour rows 8 at a time: all loads before the stores
+  const int64_t nr = r1 - r0;
+  int64_t k = 5;
+  for (; k + 8 <= nr; k += 8) {
+    float v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = o[(k + u) * ND];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) o[(k + u) * ND] = v[u] * ex;
+  }
+  for (; k < nr; ++k) o[k * ND] *= ex;
 }
 
 void launch_orb_product(hipStream_t s, float* O, int ND, int G, GroupDesc gd, const float* env, bool tangents) {
