@@ -389,7 +389,7 @@ void launch_gemm_f32_batched(hipStream_t s, const float* A, int64_t lda, int64_t
 // stage c % 2 while chunk c + 1 (already in registers) is split into stage (c + 1) % 2 and
 // chunk c + 2 is loaded: one barrier per chunk.  With KC = 16 a 128 x 128 block needs 74 KB
 // (NP = 3), so two blocks share a CU.
-template <int NP, int BM, int BN, int KC>
+template <int NP, int BM, int BN, int KC, int PD>
 __global__ void __launch_bounds__(BM * BN / 64) k_gemm_bf16_db(const float* __restrict__ A, int64_t lda,
                                                               const uint16_t* __restrict__ Wp0,
                                                               const uint16_t* __restrict__ Wp1,
@@ -429,15 +429,16 @@ __global__ void __launch_bounds__(BM * BN / 64) k_gemm_bf16_db(const float* __re
     for (int b = 0; b < 2; ++b)
       for (int v = 0; v < 16; ++v) acc[a][b][v] = 0.f;
   const uint16_t* Wpl[3] = {Wp0, Wp1, Wp2};
-  float4 ra[A_PER];
-  uint4 rb[NP][B_PER];
-  auto load = [&](int k0) {
+  // PD register buffers: chunk j >= 1 waits in buffer (j - 1) % PD until it is split into LDS
+  float4 ra[PD][A_PER];
+  uint4 rb[PD][NP][B_PER];
+  auto load = [&](int k0, float4(&xa)[A_PER], uint4(&xb)[NP][B_PER]) {
 #pragma unroll
     for (int q = 0; q < A_PER; ++q) {
       const int e = tid + q * NT;
       const int rr = e / (KC / 4), kq = (e % (KC / 4)) * 4;
       const int64_t gr = m0 + rr;
-      ra[q] = (e < A_UNITS && gr < M) ? *reinterpret_cast<const float4*>(A + gr * lda + k0 + kq)
+      xa[q] = (e < A_UNITS && gr < M) ? *reinterpret_cast<const float4*>(A + gr * lda + k0 + kq)
                                       : make_float4(0.f, 0.f, 0.f, 0.f);
     }
 #pragma unroll
@@ -447,11 +448,11 @@ __global__ void __launch_bounds__(BM * BN / 64) k_gemm_bf16_db(const float* __re
       const int gn = n0 + nn;
 #pragma unroll
       for (int p = 0; p < NP; ++p)
-        rb[p][q] = (e < B_UNITS && gn < N) ? *reinterpret_cast<const uint4*>(Wpl[p] + (int64_t)gn * K + k0 + kq)
+        xb[p][q] = (e < B_UNITS && gn < N) ? *reinterpret_cast<const uint4*>(Wpl[p] + (int64_t)gn * K + k0 + kq)
                                            : make_uint4(0, 0, 0, 0);
     }
   };
-  auto store = [&](int st) {
+  auto store = [&](int st, const float4(&ra)[A_PER], const uint4(&rb)[NP][B_PER]) {
 #pragma unroll
     for (int q = 0; q < A_PER; ++q) {
       const int e = tid + q * NT;
@@ -502,17 +503,38 @@ __global__ void __launch_bounds__(BM * BN / 64) k_gemm_bf16_db(const float* __re
     }
   };
   const int nch = K / KC;
-  load(0);
-  store(0);
-  if (nch > 1) load(KC);
-  __syncthreads();
-  for (int c = 0; c < nch; ++c) {
-    compute(c & 1);
-    if (c + 1 < nch) {
-      store((c + 1) & 1);
-      if (c + 2 < nch) load((c + 2) * KC);
-    }
+  load(0, ra[0], rb[0]);
+  store(0, ra[0], rb[0]);
+  if (nch > 1) load(KC, ra[0], rb[0]);
+  if constexpr (PD == 1) {
     __syncthreads();
+    for (int c = 0; c < nch; ++c) {
+      compute(c & 1);
+      if (c + 1 < nch) {
+        store((c + 1) & 1, ra[0], rb[0]);
+        if (c + 2 < nch) load((c + 2) * KC, ra[0], rb[0]);
+      }
+      __syncthreads();
+    }
+  } else {
+    // loads run two chunks ahead of the split into LDS (two compute phases of latency cover)
+    if (nch > 2) load(2 * KC, ra[1], rb[1]);
+    __syncthreads();
+    for (int c = 0; c < nch; c += 2) {
+      compute(0);  // chunk c (even) in stage 0
+      if (c + 1 < nch) {
+        store(1, ra[0], rb[0]);
+        if (c + 3 < nch) load((c + 3) * KC, ra[0], rb[0]);
+      }
+      __syncthreads();
+      if (c + 1 >= nch) break;
+      compute(1);  // chunk c + 1 in stage 1
+      if (c + 2 < nch) {
+        store(0, ra[1], rb[1]);
+        if (c + 4 < nch) load((c + 4) * KC, ra[1], rb[1]);
+      }
+      __syncthreads();
+    }
   }
   store_tile(acc, m0, n0, wm, wn, lane, C, ldc, M, N, bias, rowflag, period);
 }
@@ -521,9 +543,11 @@ __global__ void __launch_bounds__(BM * BN / 64) k_gemm_bf16_db(const float* __re
 struct Bf16Tile {
   int bm, bn;
 };
-// variants 0-6: single LDS stage, K chunk 32; variants 7-10: two LDS stages (kc = 16 or 32)
-static const Bf16Tile kBf16Tiles[] = {{128, 128}, {128, 256}, {256, 128}, {64, 256}, {256, 64}, {64, 128}, {128, 64},
-                                      {128, 128}, {128, 128}, {256, 128}, {128, 256}};
+// variants 0-6: single LDS stage, K chunk 32; variants 7-10: two LDS stages (kc = 16 or 32);
+// variants 11-14: two LDS stages with loads two chunks ahead
+static const Bf16Tile kBf16Tiles[] = {{128, 128}, {128, 256}, {256, 128}, {64, 256}, {256, 64},
+                                      {64, 128},  {128, 64},  {128, 128}, {128, 128}, {256, 128},
+                                      {128, 256}, {256, 128}, {128, 256}, {128, 128}, {128, 128}};
 
 int gemm_bf16_variants() { return (int)(sizeof(kBf16Tiles) / sizeof(kBf16Tiles[0])); }
 void gemm_bf16_tile(int variant, int* bm, int* bn) {
@@ -550,7 +574,7 @@ static void launch_tile(hipStream_t s, const float* A, int64_t lda, const uint16
                                                                      rowflag, period, nT, dr);
 }
 
-template <int NP, int BM, int BN, int KC>
+template <int NP, int BM, int BN, int KC, int PD = 1>
 static void launch_tile_db(hipStream_t s, const float* A, int64_t lda, const uint16_t* const* W, const float* bias,
                            float* C, int64_t ldc, int64_t M, int N, int K, const uint8_t* rowflag, int period,
                            DevRows dr) {
@@ -558,14 +582,14 @@ static void launch_tile_db(hipStream_t s, const float* A, int64_t lda, const uin
   const size_t lds = (size_t)2 * NP * (BM + BN) * (KC + 8) * 2;
   if (!attr_set) {
     if (lds > 64 * 1024)
-      (void)hipFuncSetAttribute((const void*)k_gemm_bf16_db<NP, BM, BN, KC>,
+      (void)hipFuncSetAttribute((const void*)k_gemm_bf16_db<NP, BM, BN, KC, PD>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     attr_set = true;
   }
   const int nT = (N + BN - 1) / BN;
   const int64_t blocks = ((M + BM - 1) / BM) * nT;
-  k_gemm_bf16_db<NP, BM, BN, KC><<<(unsigned)blocks, BM * BN / 64, lds, s>>>(A, lda, W[0], W[1], W[2], bias, C, ldc,
-                                                                           M, N, K, rowflag, period, nT, dr);
+  k_gemm_bf16_db<NP, BM, BN, KC, PD><<<(unsigned)blocks, BM * BN / 64, lds, s>>>(A, lda, W[0], W[1], W[2], bias, C,
+                                                                               ldc, M, N, K, rowflag, period, nT, dr);
 }
 
 template <int NP>
@@ -577,6 +601,10 @@ static void launch_np(hipStream_t s, int variant, const float* A, int64_t lda, c
     case 8: launch_tile_db<NP, 128, 128, 32>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
     case 9: launch_tile_db<NP, 256, 128, 16>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
     case 10: launch_tile_db<NP, 128, 256, 16>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
+    case 11: launch_tile_db<NP, 256, 128, 16, 2>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
+    case 12: launch_tile_db<NP, 128, 256, 16, 2>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
+    case 13: launch_tile_db<NP, 128, 128, 32, 2>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
+    case 14: launch_tile_db<NP, 128, 128, 16, 2>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
     case 1: launch_tile<NP, 128, 256>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
     case 2: launch_tile<NP, 256, 128>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
     case 3: launch_tile<NP, 64, 256>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;

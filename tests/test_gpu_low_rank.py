@@ -238,6 +238,35 @@ def test_many_changed_electrons(gpu_device):
     assert 0.0 < stm["pmove"] < 1.0
 
 
+def test_large_n_generic_update(gpu_device):
+    """200 electrons (hydrogen chain, 1.8 bohr): more than 12 16-wide column tiles, so the
+    low-rank update takes the generic Woodbury kernel and the state's inverses the non-register
+    Gauss-Jordan / LU forms.  Updates against fresh evaluations, then a sweep."""
+    from sparse_wf_amd import systems
+    from sparse_wf_amd.config import FireConfig
+
+    mol = systems.h_chain(200, 1.8)
+    cfg = FireConfig(mol=mol, cutoff=5.0, feature_dim=64, n_determinants=1)
+    wf = FireWaveFunction(cfg, device=gpu_device)
+    r = systems.init_electrons(mol, 4, stddev=0.5, seed=12)
+    prm = wf.init(0, r[0])
+    wf.upload(prm)
+    (_, _), state = wf.log_psi_with_state(prm, r, precision="fp32")
+    cur = r.copy()
+    rng = np.random.default_rng(5)
+    for k in (0, 99, 100, 199):
+        cur = cur.copy()
+        cur[:, k] += rng.normal(scale=0.3, size=(len(cur), 3)).astype(np.float32)
+        (sg_u, lp_u), state = wf.log_psi_low_rank_update(prm, cur, np.array([k]), state=state)
+        sg_f, lp_f = wf.signed(prm, cur, precision="fp32")
+        _close(lp_u, lp_f, 1e-8)
+        assert torch.equal(sg_u.cpu(), sg_f.cpu())
+    r_end, stm = wf.mcmc_steps_low_rank(prm, r, cfg.n_el, width=0.3, seed=4, precision="fp32")
+    sg_f, lp_f = wf.signed(prm, r_end, precision="fp32")
+    _close(stm["logpsi"], lp_f, 1e-8)
+    assert 0.0 < stm["pmove"] < 1.0
+
+
 def test_sweep_graph_matches_eager(setup):
     """Whole sweeps replayed from the captured hipGraph give bit-identical walkers, log psi
     and acceptance counts to the same steps issued eagerly, over several replays with eager
