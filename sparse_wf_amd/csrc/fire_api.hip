@@ -545,7 +545,7 @@ void run_orbitals_det(Ctx& c, const float* r, int B, const Graph& gr, bool tan, 
       HIPCHK(det_lap_set_smem_limit(sm));
       h->smem_set = sm;
     }
-    launch_det_lap(c.s, Ob, ND, gd, h->ptr.as<int64_t>(), h->nbr.as<int32_t>(), h->rev.as<int32_t>(), B, N, h->n_up, D,
+    launch_det_lap(c.s, Ob, ND, gd, h->ENV.as<float>(), h->ptr.as<int64_t>(), h->nbr.as<int32_t>(), h->rev.as<int32_t>(), B, N, h->n_up, D,
                    smax, h->AinvT.as<float>(), h->trip.as<double>(), h->dgrad.as<double>());
     out.grad = h->det_gr.as<double>();
     out.lap = h->det_lap.as<double>();

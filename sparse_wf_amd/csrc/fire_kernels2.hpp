@@ -21,9 +21,9 @@ void launch_det_trace(hipStream_t s, const float* O, int ND, GroupDesc gd, int B
                       const double* Ainv, double* tr);
 size_t det_lap_smem(int smax);
 hipError_t det_lap_set_smem_limit(size_t bytes);
-void launch_det_lap(hipStream_t s, const float* O, int ND, GroupDesc gd, const int64_t* ptr, const int32_t* nbr,
-                    const int32_t* rev, int B, int N, int n_up, int D, int smax, const float* AinvT, double* trip,
-                    double* grad);
+void launch_det_lap(hipStream_t s, const float* O, int ND, GroupDesc gd, const float* env, const int64_t* ptr,
+                    const int32_t* nbr, const int32_t* rev, int B, int N, int n_up, int D, int smax,
+                    const float* AinvT, double* trip, double* grad);
 void launch_det_combine(hipStream_t s, int B, int N, int D, const double* logdet, const double* sgn, const double* tr,
                         const double* trip, const double* grad, double* out_logpsi, double* out_sign,
                         double* out_grad, double* out_lap, bool tangents);

@@ -62,7 +62,10 @@ void launch_env_coef(hipStream_t s, const float* r, const float* R, int B, int N
     k_env_coef<false><<<G, 256, 0, s>>>(r, R, B, N, n_up, M, D, E, sp_up, sp_dn, coef, list, n_list);
 }
 
-// Phi = orbital rows x envelope rows (product with a one-electron function), in place.
+// Phi = orbital rows x envelope rows (product with a one-electron function), in place, for
+// the value, Laplacian and self-gradient rows of each group.  A neighbour row d O_i / d r_j
+// (j != i) becomes E_i d O_i / d r_j; that scale is applied where the rows are consumed
+// (k_det_lap stages them through LDS), so the neighbour rows are not rewritten here.
 template <bool TAN>
 __global__ void k_orb_product(float* __restrict__ O, int ND, GroupDesc gd, const float* __restrict__ env) {
   const int g = blockIdx.x;
@@ -70,8 +73,7 @@ __global__ void k_orb_product(float* __restrict__ O, int ND, GroupDesc gd, const
   if (c >= ND) return;
   const int NR = TAN ? 5 : 1;
   const float* e = env + (size_t)g * NR * ND + c;
-  const int64_t r0 = gd.row(g), r1 = gd.row(g + 1);
-  float* o = O + r0 * ND + c;
+  float* o = O + gd.row(g) * ND + c;
   const float ox = o[0], ex = e[0];
   o[0] = ox * ex;
   if (!TAN) return;
@@ -83,17 +85,6 @@ __global__ void k_orb_product(float* __restrict__ O, int ND, GroupDesc gd, const
     o[(2 + t) * (int64_t)ND] = ox * et + ex * ot;
   }
   o[ND] = ol * ex + el * ox + 2.f * cross;
-  // neighbour rows 8 at a time: all loads before the stores
-  const int64_t nr = r1 - r0;
-  int64_t k = 5;
-  for (; k + 8 <= nr; k += 8) {
-    float v[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) v[u] = o[(k + u) * ND];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) o[(k + u) * ND] = v[u] * ex;
-  }
-  for (; k < nr; ++k) o[k * ND] *= ex;
 }
 
 void launch_orb_product(hipStream_t s, float* O, int ND, int G, GroupDesc gd, const float* env, bool tangents) {
@@ -524,13 +515,16 @@ typedef float float4_t __attribute__((ext_vector_type(4)));
 //   B[a,x,b] = sum_o J_a[x,o] Ainv[o, S_b]       (FP32 MFMA 16x16x4, Ainv rounded from FP64)
 //   trip = sum B[a,x,b] B[b,x,a],  grad_x = sum_a B[a,x,a]   (FP64 sums).
 // Each jacobian row (i, n) belongs to exactly one set S_n, so the orbital jacobian is read
-// once.  Dynamic LDS: J chunk [RT*16][KC+4], X chunk [CT*16][KC+4], B [3s][s], indices.
+// once.  For a != n the stored row is d O_a / d r_n before the envelope product; it is
+// scaled by the envelope value row E_a (env, [group][5][ND], value row first) while it is
+// staged, which is the product rule since E_a depends on r_a only.
+// Dynamic LDS: J chunk [RT*16][KC+4], X chunk [CT*16][KC+4], B [3s][s], indices.
 // VW: width of the vector loads of jacobian and inverse rows (4, 2 or 1 floats; N % VW == 0
 // keeps every row start aligned).  The next orbital chunk is prefetched into registers while
 // the current one is multiplied.
 template <int VW>
 __global__ void __launch_bounds__(256) k_det_lap(const float* __restrict__ O, int ND, GroupDesc gd,
-                                                 const int64_t* __restrict__ ptr, const int32_t* __restrict__ nbr,
+                                                 const float* __restrict__ env, const int64_t* __restrict__ ptr, const int32_t* __restrict__ nbr,
                                                  const int32_t* __restrict__ rev, int B, int N, int n_up, int D,
                                                  int smax, const float* __restrict__ AinvT, double* __restrict__ trip,
                                                  double* __restrict__ grad) {
@@ -543,6 +537,7 @@ __global__ void __launch_bounds__(256) k_det_lap(const float* __restrict__ O, in
   int* s_idx = reinterpret_cast<int*>(Bs + ((3 * smax * smax + 1) & ~1));
   int64_t* s_row = reinterpret_cast<int64_t*>(s_idx + smax + (smax & 1));  // first jacobian row of a
   int64_t* s_rp = s_row + smax;                                             // element offset of row (a, x)
+  int64_t* s_ep = s_rp + 3 * smax;  // envelope value row offset of a (-1 for a = 0)
   __shared__ double red[16 * 4];
   const int blk = blockIdx.x;
   const int n = blk % N;
@@ -556,11 +551,13 @@ __global__ void __launch_bounds__(256) k_det_lap(const float* __restrict__ O, in
     if (a == 0) {
       s_idx[0] = n;
       s_row[0] = gd.row(gn) + 2;
+      s_ep[0] = -1;
     } else {
       const int i = nbr[p0 + a - 1];
       const int gi = group_of(w, i, B, n_up, N - n_up);
       s_idx[a] = i;
       s_row[a] = gd.row(gi) + 5 + 3 * (int64_t)rev[p0 + a - 1];
+      s_ep[a] = (int64_t)gi * 5 * ND + (int64_t)d * N;
     }
   }
   const int RT = (3 * s + 15) / 16, CT = (s + 15) / 16, NT = RT * CT;
@@ -578,15 +575,20 @@ __global__ void __launch_bounds__(256) k_det_lap(const float* __restrict__ O, in
   constexpr int JPT = ((FIRE_SMAX * 3 + 15) / 16 * 16 * UPR + 255) / 256;  // J units per thread
   constexpr int XPT = ((FIRE_SMAX + 15) / 16 * 16 * UPR + 255) / 256;      // X units per thread
   const int nJ = RT * 16 * UPR, nX = CT * 16 * UPR;
-  vec_t rj[JPT], rx[XPT];
+  vec_t rj[JPT], re[JPT], rx[XPT];
   auto load = [&](int o0) {
 #pragma unroll
     for (int q = 0; q < JPT; ++q) {
       const int e = tid + q * 256;
       const int ax = e / UPR, k = (e % UPR) * VW;
-      vec_t v = (vec_t)(0.f);
-      if (e < nJ && ax < 3 * s && o0 + k < N) v = *reinterpret_cast<const vec_t*>(O + s_rp[ax] + o0 + k);
+      vec_t v = (vec_t)(0.f), ev = (vec_t)(1.f);
+      if (e < nJ && ax < 3 * s && o0 + k < N) {
+        v = *reinterpret_cast<const vec_t*>(O + s_rp[ax] + o0 + k);
+        const int64_t ep = s_ep[ax / 3];
+        if (ep >= 0) ev = *reinterpret_cast<const vec_t*>(env + ep + o0 + k);
+      }
       rj[q] = v;
+      re[q] = ev;
     }
 #pragma unroll
     for (int q = 0; q < XPT; ++q) {
@@ -601,7 +603,7 @@ __global__ void __launch_bounds__(256) k_det_lap(const float* __restrict__ O, in
 #pragma unroll
     for (int q = 0; q < JPT; ++q) {
       const int e = tid + q * 256;
-      if (e < nJ) *reinterpret_cast<vec_t*>(Js + (e / UPR) * LDK + (e % UPR) * VW) = rj[q];
+      if (e < nJ) *reinterpret_cast<vec_t*>(Js + (e / UPR) * LDK + (e % UPR) * VW) = rj[q] * re[q];
     }
 #pragma unroll
     for (int q = 0; q < XPT; ++q) {
@@ -663,19 +665,19 @@ __global__ void __launch_bounds__(256) k_det_lap(const float* __restrict__ O, in
 size_t det_lap_smem(int smax) {
   const int RTm = (3 * smax + 15) / 16, CTm = (smax + 15) / 16;
   return (size_t)(RTm * 16 + CTm * 16) * (DL_KC + 4) * 4 + (size_t)((3 * smax * smax + 1) & ~1) * 4 +
-         (size_t)(smax + (smax & 1)) * 4 + (size_t)smax * 8 + (size_t)3 * smax * 8;
+         (size_t)(smax + (smax & 1)) * 4 + (size_t)smax * 8 + (size_t)3 * smax * 8 + (size_t)smax * 8;
 }
 
-void launch_det_lap(hipStream_t s, const float* O, int ND, GroupDesc gd, const int64_t* ptr, const int32_t* nbr,
-                    const int32_t* rev, int B, int N, int n_up, int D, int smax, const float* AinvT, double* trip,
-                    double* grad) {
+void launch_det_lap(hipStream_t s, const float* O, int ND, GroupDesc gd, const float* env, const int64_t* ptr,
+                    const int32_t* nbr, const int32_t* rev, int B, int N, int n_up, int D, int smax,
+                    const float* AinvT, double* trip, double* grad) {
   const size_t sm = det_lap_smem(smax);
   if (N % 4 == 0)
-    k_det_lap<4><<<B * D * N, 256, sm, s>>>(O, ND, gd, ptr, nbr, rev, B, N, n_up, D, smax, AinvT, trip, grad);
+    k_det_lap<4><<<B * D * N, 256, sm, s>>>(O, ND, gd, env, ptr, nbr, rev, B, N, n_up, D, smax, AinvT, trip, grad);
   else if (N % 2 == 0)
-    k_det_lap<2><<<B * D * N, 256, sm, s>>>(O, ND, gd, ptr, nbr, rev, B, N, n_up, D, smax, AinvT, trip, grad);
+    k_det_lap<2><<<B * D * N, 256, sm, s>>>(O, ND, gd, env, ptr, nbr, rev, B, N, n_up, D, smax, AinvT, trip, grad);
   else
-    k_det_lap<1><<<B * D * N, 256, sm, s>>>(O, ND, gd, ptr, nbr, rev, B, N, n_up, D, smax, AinvT, trip, grad);
+    k_det_lap<1><<<B * D * N, 256, sm, s>>>(O, ND, gd, env, ptr, nbr, rev, B, N, n_up, D, smax, AinvT, trip, grad);
 }
 
 hipError_t det_lap_set_smem_limit(size_t bytes) {
