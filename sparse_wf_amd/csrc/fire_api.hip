@@ -97,7 +97,6 @@ struct fire_handle {
   DevBuf JR, mlp_ns, mlp_dg, QV, AV, att_ns, att_dg, RIN, RY1, RY2, att_out;
   DevBuf cval, clap, cgrad, epot;
   int chunk = 256;
-  size_t smem_set = 0;
   bool lu_smem_set = false;
   // optional HIP-event timing of the dense contractions (fire_profile_gemm)
   bool gemm_timing = false;
@@ -540,13 +539,9 @@ void run_orbitals_det(Ctx& c, const float* r, int B, const Graph& gr, bool tan, 
     launch_det_trace(c.s, Ob, ND, gd, B, N, h->n_up, D, h->Ainv.as<double>(), h->tr.as<double>());
     const int smax = gr.max_nb + 1;
     REQUIRE(smax <= FIRE_SMAX, "an electron has more than 63 neighbours within the cutoff (determinant Laplacian limit)");
-    const size_t sm = det_lap_smem(smax);
-    if (sm > 64 * 1024 && sm > h->smem_set) {
-      HIPCHK(det_lap_set_smem_limit(sm));
-      h->smem_set = sm;
-    }
-    launch_det_lap(c.s, Ob, ND, gd, h->ENV.as<float>(), h->ptr.as<int64_t>(), h->nbr.as<int32_t>(), h->rev.as<int32_t>(), B, N, h->n_up, D,
-                   smax, h->AinvT.as<float>(), h->trip.as<double>(), h->dgrad.as<double>());
+    launch_det_lap(c.s, Ob, ND, gd, h->ENV.as<float>(), h->ptr.as<int64_t>(), h->nbr.as<int32_t>(),
+                   h->rev.as<int32_t>(), B, N, h->n_up, D, smax, h->AinvT.as<float>(), h->trip.as<double>(),
+                   h->dgrad.as<double>());
     out.grad = h->det_gr.as<double>();
     out.lap = h->det_lap.as<double>();
   } else {

@@ -19,8 +19,6 @@ size_t lu_lds_bytes(int N);
 hipError_t lu_set_smem_limit(size_t bytes);
 void launch_det_trace(hipStream_t s, const float* O, int ND, GroupDesc gd, int B, int N, int n_up, int D,
                       const double* Ainv, double* tr);
-size_t det_lap_smem(int smax);
-hipError_t det_lap_set_smem_limit(size_t bytes);
 void launch_det_lap(hipStream_t s, const float* O, int ND, GroupDesc gd, const float* env, const int64_t* ptr,
                     const int32_t* nbr, const int32_t* rev, int B, int N, int n_up, int D, int smax,
                     const float* AinvT, double* trip, double* grad);

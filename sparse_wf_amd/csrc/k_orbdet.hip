@@ -505,188 +505,230 @@ void launch_det_trace(hipStream_t s, const float* O, int ND, GroupDesc gd, int B
   k_det_trace<<<B * D, 256, 0, s>>>(O, ND, gd, B, N, n_up, D, Ainv, tr);
 }
 
-#define DL_KC 32  // contraction chunk staged in LDS
-#define DL_TPW 12  // max 16x16 output tiles per wavefront (3*64 x 64 outputs / 4 waves)
-
+// Triplet part of the determinant Laplacian and the determinant gradient.  For electron n with
+// dependency set S = {n} U N(n) (s members) and the jacobian rows J_a[x] = d Phi_a / d r_n,x
+// (3 per member a), per determinant d:
+//   B_x[a,b] = sum_o J_a[x, d N + o] Ainv_d[o, S_b]   (FP32 MFMA 16x16x4)
+//   trip_d = sum_x sum_{a,b} B_x[a,b] B_x[b,a],  grad_d,x = sum_a B_x[a,a]   (FP64 sums).
+// Each jacobian row (i, n) belongs to exactly one set S_n, so the orbital jacobian is read once.
+// For a != n the stored row is d O_a / d r_n before the envelope product; it is scaled by the
+// envelope value row E_a (env, [group][5][ND], value row first) as it is loaded, which is the
+// product rule since E_a depends on r_a only.
+//
+// One wavefront per (walker, electron, determinant); the four wavefronts of a workgroup take
+// consecutive units, so they read the same jacobian rows (different determinant slices) on one
+// CU.  Members are padded to NAT tiles of 16 (padding rows read member 0's rows, padding
+// columns are zero), so B_x is an NAT x NAT grid of 16x16 tiles; XG values of x are computed
+// per pass over the columns.  The jacobian and the inverse rows (AinvT, the transposed
+// inverse) stream from memory straight into MFMA operand registers: in a 16-column chunk
+// starting at c0, lane l holds row l%16 and columns c0 + 4 (l/16) + t (t < 4), and MFMA step t
+// contracts exactly those columns.  That permutes the contraction index identically for both
+// operands, so the product is unchanged while every lane issues one vector load per row.  The
+// chunk frame starts at the slice start rounded down to 4 columns; the off = d N mod 4 columns
+// of the previous slice it takes in meet zero inverse columns.
+// trip and grad come from the accumulators: tile (j,i) is transposed through 1.25 KB of LDS per
+// wavefront into the layout of tile (i,j), so no workgroup barrier is needed.
+// VEC: float4 jacobian/envelope loads and float2 inverse loads (N even, ND % 4 == 0); otherwise
+// scalar loads.
 typedef float float4_t __attribute__((ext_vector_type(4)));
+typedef float float2_t __attribute__((ext_vector_type(2)));
 
-// Triplet part of the determinant Laplacian and the determinant gradient for one
-// (walker, det, electron n): with S = {n} U N(n) and J_a the jacobian rows d Phi_a / d r_n,
-//   B[a,x,b] = sum_o J_a[x,o] Ainv[o, S_b]       (FP32 MFMA 16x16x4, Ainv rounded from FP64)
-//   trip = sum B[a,x,b] B[b,x,a],  grad_x = sum_a B[a,x,a]   (FP64 sums).
-// Each jacobian row (i, n) belongs to exactly one set S_n, so the orbital jacobian is read
-// once.  For a != n the stored row is d O_a / d r_n before the envelope product; it is
-// scaled by the envelope value row E_a (env, [group][5][ND], value row first) while it is
-// staged, which is the product rule since E_a depends on r_a only.
-// Dynamic LDS: J chunk [RT*16][KC+4], X chunk [CT*16][KC+4], B [3s][s], indices.
-// VW: width of the vector loads of jacobian and inverse rows (4, 2 or 1 floats; N % VW == 0
-// keeps every row start aligned).  The next orbital chunk is prefetched into registers while
-// the current one is multiplied.
-template <int VW>
-__global__ void __launch_bounds__(256) k_det_lap(const float* __restrict__ O, int ND, GroupDesc gd,
-                                                 const float* __restrict__ env, const int64_t* __restrict__ ptr, const int32_t* __restrict__ nbr,
-                                                 const int32_t* __restrict__ rev, int B, int N, int n_up, int D,
-                                                 int smax, const float* __restrict__ AinvT, double* __restrict__ trip,
-                                                 double* __restrict__ grad) {
-  extern __shared__ unsigned char smem_raw[];
-  const int RTm = (3 * smax + 15) / 16, CTm = (smax + 15) / 16;
-  const int LDK = DL_KC + 4;
-  float* Js = reinterpret_cast<float*>(smem_raw);
-  float* Xs = Js + RTm * 16 * LDK;
-  float* Bs = Xs + CTm * 16 * LDK;
-  int* s_idx = reinterpret_cast<int*>(Bs + ((3 * smax * smax + 1) & ~1));
-  int64_t* s_row = reinterpret_cast<int64_t*>(s_idx + smax + (smax & 1));  // first jacobian row of a
-  int64_t* s_rp = s_row + smax;                                             // element offset of row (a, x)
-  int64_t* s_ep = s_rp + 3 * smax;  // envelope value row offset of a (-1 for a = 0)
-  __shared__ double red[16 * 4];
-  const int blk = blockIdx.x;
-  const int n = blk % N;
-  const int wd = blk / N;
-  const int w = wd / D, d = wd - w * D;
-  const int gn = group_of(w, n, B, n_up, N - n_up);
-  const int64_t p0 = ptr[gn];
-  const int s = 1 + (int)(ptr[gn + 1] - p0);
-  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  for (int a = tid; a < s; a += blockDim.x) {
-    if (a == 0) {
-      s_idx[0] = n;
-      s_row[0] = gd.row(gn) + 2;
-      s_ep[0] = -1;
-    } else {
-      const int i = nbr[p0 + a - 1];
-      const int gi = group_of(w, i, B, n_up, N - n_up);
-      s_idx[a] = i;
-      s_row[a] = gd.row(gi) + 5 + 3 * (int64_t)rev[p0 + a - 1];
-      s_ep[a] = (int64_t)gi * 5 * ND + (int64_t)d * N;
-    }
-  }
-  const int RT = (3 * s + 15) / 16, CT = (s + 15) / 16, NT = RT * CT;
-  float4_t acc[DL_TPW];
-  for (int t = 0; t < DL_TPW; ++t) acc[t] = (float4_t){0.f, 0.f, 0.f, 0.f};
-  __syncthreads();
-  for (int ax = tid; ax < 3 * s; ax += blockDim.x) {
-    const int a = ax / 3, x = ax - 3 * a;
-    s_rp[ax] = (s_row[a] + x) * ND + (int64_t)d * N;
-  }
-  __syncthreads();
-  const float* XT = AinvT + (size_t)wd * N * N;
-  typedef float vec_t __attribute__((ext_vector_type(VW)));
-  constexpr int UPR = DL_KC / VW;                                   // vector units per chunk row
-  constexpr int JPT = ((FIRE_SMAX * 3 + 15) / 16 * 16 * UPR + 255) / 256;  // J units per thread
-  constexpr int XPT = ((FIRE_SMAX + 15) / 16 * 16 * UPR + 255) / 256;      // X units per thread
-  const int nJ = RT * 16 * UPR, nX = CT * 16 * UPR;
-  vec_t rj[JPT], re[JPT], rx[XPT];
-  auto load = [&](int o0) {
+template <int NAT, int XG, int P, bool VEC>
+__device__ __forceinline__ void det_lap_unit(const float* __restrict__ O, int ND, GroupDesc gd,
+                                             const float* __restrict__ env, const int32_t* __restrict__ nbr,
+                                             const int32_t* __restrict__ rev, int B, int N, int n_up, int D,
+                                             const float* __restrict__ AinvT, double* __restrict__ trip,
+                                             double* __restrict__ grad, int w, int n, int d, int gn, int64_t p0,
+                                             int s, float* tb) {
+  const int lane = threadIdx.x & 63, li = lane & 15, kg = lane >> 4;
+  const int cbase = (d * N) & ~3, off = d * N - cbase;
+  const int ncol = N + off, nch = (ncol + 15) / 16;
+  const float* jb[NAT];  // row x = 0 of member t*16 + li, at the chunk frame
+  const float* eb[NAT];
+  const float* xb[NAT];  // inverse row of member t*16 + li, at the chunk frame
+  bool sc[NAT], bok[NAT];
 #pragma unroll
-    for (int q = 0; q < JPT; ++q) {
-      const int e = tid + q * 256;
-      const int ax = e / UPR, k = (e % UPR) * VW;
-      vec_t v = (vec_t)(0.f), ev = (vec_t)(1.f);
-      if (e < nJ && ax < 3 * s && o0 + k < N) {
-        v = *reinterpret_cast<const vec_t*>(O + s_rp[ax] + o0 + k);
-        const int64_t ep = s_ep[ax / 3];
-        if (ep >= 0) ev = *reinterpret_cast<const vec_t*>(env + ep + o0 + k);
-      }
-      rj[q] = v;
-      re[q] = ev;
+  for (int t = 0; t < NAT; ++t) {
+    const int m = t * 16 + li;
+    const bool ok = m < s;
+    int i = n;
+    int64_t row = gd.row(gn) + 2, ep = -1;
+    if (ok && m > 0) {
+      i = nbr[p0 + m - 1];
+      const int g = group_of(w, i, B, n_up, N - n_up);
+      row = gd.row(g) + 5 + 3 * (int64_t)rev[p0 + m - 1];
+      ep = (int64_t)g * 5 * ND;
     }
-#pragma unroll
-    for (int q = 0; q < XPT; ++q) {
-      const int e = tid + q * 256;
-      const int b = e / UPR, k = (e % UPR) * VW;
-      vec_t v = (vec_t)(0.f);
-      if (e < nX && b < s && o0 + k < N) v = *reinterpret_cast<const vec_t*>(XT + (size_t)s_idx[b] * N + o0 + k);
-      rx[q] = v;
-    }
+    jb[t] = O + row * ND + cbase + kg * 4;
+    sc[t] = ep >= 0;
+    eb[t] = env + (ep >= 0 ? ep : 0) + cbase + kg * 4;
+    bok[t] = ok;
+    xb[t] = AinvT + ((size_t)(w * D + d) * N + i) * N + kg * 4 - off;
+  }
+  struct Chunk {
+    float4_t j[XG][NAT], e[NAT], x[NAT];
   };
-  auto store = [&]() {
+  double tr = 0.0, gx[3] = {0.0, 0.0, 0.0};
+  for (int x0 = 0; x0 < 3; x0 += XG) {
+    auto load = [&](int k, Chunk& c) {
+      const int o = k * 16;  // this lane's columns: o + 4 kg + t in the chunk frame
 #pragma unroll
-    for (int q = 0; q < JPT; ++q) {
-      const int e = tid + q * 256;
-      if (e < nJ) *reinterpret_cast<vec_t*>(Js + (e / UPR) * LDK + (e % UPR) * VW) = rj[q] * re[q];
-    }
+      for (int t = 0; t < NAT; ++t) {
+        if (VEC) {
+          // vectors start inside the slice or not at all; their tail past the slice meets zero
+          // inverse columns (and stays inside the row since ND % 4 == 0)
+          const bool ok = o + kg * 4 < ncol;
 #pragma unroll
-    for (int q = 0; q < XPT; ++q) {
-      const int e = tid + q * 256;
-      if (e < nX) *reinterpret_cast<vec_t*>(Xs + (e / UPR) * LDK + (e % UPR) * VW) = rx[q];
-    }
-  };
-  load(0);
-  for (int o0 = 0; o0 < N; o0 += DL_KC) {
-    store();
-    __syncthreads();
-    if (o0 + DL_KC < N) load(o0 + DL_KC);
+          for (int xi = 0; xi < XG; ++xi)
+            c.j[xi][t] = ok ? *reinterpret_cast<const float4_t*>(jb[t] + (int64_t)(x0 + xi) * ND + o)
+                            : (float4_t)(0.f);
+          c.e[t] = ok && sc[t] ? *reinterpret_cast<const float4_t*>(eb[t] + o) : (float4_t)(1.f);
 #pragma unroll
-    for (int q = 0; q < DL_TPW; ++q) {
-      const int t = wid + 4 * q;
-      if (t < NT) {
-        const int rt = t / CT, ct = t - rt * CT;
-        const float* ja = Js + (rt * 16 + (lane & 15)) * LDK + (lane >> 4);
-        const float* xb = Xs + (ct * 16 + (lane & 15)) * LDK + (lane >> 4);
+          for (int h = 0; h < 2; ++h) {
+            const int cs = o + kg * 4 - off + 2 * h;  // slice column (even: N and off are even)
+            const float2_t v = bok[t] && cs >= 0 && cs < N ? *reinterpret_cast<const float2_t*>(xb[t] + o + 2 * h)
+                                                           : (float2_t)(0.f);
+            c.x[t][2 * h] = v[0];
+            c.x[t][2 * h + 1] = v[1];
+          }
+        } else {
 #pragma unroll
-        for (int kk = 0; kk < DL_KC; kk += 4)
-          acc[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(ja[kk], xb[kk], acc[q], 0, 0, 0);
+          for (int q = 0; q < 4; ++q) {
+            const int cf = o + kg * 4 + q, cs = cf - off;
+            const bool ok = cf < ncol;
+#pragma unroll
+            for (int xi = 0; xi < XG; ++xi) c.j[xi][t][q] = ok ? jb[t][(int64_t)(x0 + xi) * ND + o + q] : 0.f;
+            c.e[t][q] = ok && sc[t] ? eb[t][o + q] : 1.f;
+            c.x[t][q] = bok[t] && cs >= 0 && cs < N ? xb[t][o + q] : 0.f;
+          }
+        }
       }
-    }
-    __syncthreads();
-  }
-  // scatter the 16x16 tiles into B[ax][b]
-  {
+    };
+    float4_t acc[XG][NAT][NAT];
 #pragma unroll
-    for (int q = 0; q < DL_TPW; ++q) {
-      const int t = wid + 4 * q;
-      if (t < NT) {
-        const int rt = t / CT, ct = t - rt * CT;
-        const int b = ct * 16 + (lane & 15);
+    for (int xi = 0; xi < XG; ++xi)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int ax = rt * 16 + 4 * (lane >> 4) + v;
-          if (ax < 3 * s && b < s) Bs[ax * s + b] = acc[q][v];
+      for (int i = 0; i < NAT; ++i)
+#pragma unroll
+        for (int j = 0; j < NAT; ++j) acc[xi][i][j] = (float4_t){0.f, 0.f, 0.f, 0.f};
+    auto mul = [&](const Chunk& c) {
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int xi = 0; xi < XG; ++xi)
+#pragma unroll
+          for (int i = 0; i < NAT; ++i) {
+            const float jv = c.j[xi][i][t] * c.e[i][t];
+#pragma unroll
+            for (int j = 0; j < NAT; ++j)
+              acc[xi][i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(jv, c.x[j][t], acc[xi][i][j], 0, 0, 0);
+          }
+    };
+    Chunk c[P];
+#pragma unroll
+    for (int q = 0; q < P; ++q)
+      if (q < nch) load(q, c[q]);
+    for (int k = 0; k < nch; k += P) {
+#pragma unroll
+      for (int q = 0; q < P; ++q) {
+        if (k + q < nch) {
+          mul(c[q]);
+          if (k + q + P < nch) load(k + q + P, c[q]);
         }
       }
     }
+    // trip_x = sum_{i,j} T_ij o T_ji^T over tiles, grad_x = trace; padding rows meet zero columns
+#pragma unroll
+    for (int xi = 0; xi < XG; ++xi) {
+      double g = 0.0;
+#pragma unroll
+      for (int i = 0; i < NAT; ++i) {
+#pragma unroll
+        for (int j = i; j < NAT; ++j) {
+#pragma unroll
+          for (int v = 0; v < 4; ++v) tb[(4 * kg + v) * 20 + li] = acc[xi][j][i][v];
+          __builtin_amdgcn_wave_barrier();
+          const float4_t tt = *reinterpret_cast<const float4_t*>(tb + li * 20 + 4 * kg);
+          __builtin_amdgcn_wave_barrier();
+          double p = 0.0;
+#pragma unroll
+          for (int v = 0; v < 4; ++v) p += (double)acc[xi][i][j][v] * (double)tt[v];
+          tr += (i == j ? 1.0 : 2.0) * p;
+          if (i == j) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+              if (li == 4 * kg + v) g += (double)acc[xi][i][i][v];
+          }
+        }
+      }
+      const int x = x0 + xi;
+      if (x == 0) gx[0] += g;
+      else if (x == 1) gx[1] += g;
+      else gx[2] += g;
+    }
   }
-  __syncthreads();
-  double v4[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int e = tid; e < 3 * s * s; e += blockDim.x) {
-    const int ax = e / s, b = e - ax * s;
-    const int a = ax / 3, x = ax - 3 * a;
-    const double bv = Bs[e];
-    v4[0] += bv * (double)Bs[(3 * b + x) * s + a];
-    if (a == b) v4[1 + x] += bv;
-  }
-  block_sum<4>(v4, red);
-  if (tid == 0) {
-    trip[blk] = v4[0];
-    for (int x = 0; x < 3; ++x) grad[(size_t)blk * 3 + x] = v4[1 + x];
+  tr = warp_sum(tr);
+#pragma unroll
+  for (int x = 0; x < 3; ++x) gx[x] = warp_sum(gx[x]);
+  if (lane == 0) {
+    const int64_t o = ((int64_t)w * D + d) * N + n;
+    trip[o] = tr;
+#pragma unroll
+    for (int x = 0; x < 3; ++x) grad[o * 3 + x] = gx[x];
   }
 }
 
-size_t det_lap_smem(int smax) {
-  const int RTm = (3 * smax + 15) / 16, CTm = (smax + 15) / 16;
-  return (size_t)(RTm * 16 + CTm * 16) * (DL_KC + 4) * 4 + (size_t)((3 * smax * smax + 1) & ~1) * 4 +
-         (size_t)(smax + (smax & 1)) * 4 + (size_t)smax * 8 + (size_t)3 * smax * 8 + (size_t)smax * 8;
+// Units of NAT = ceil(s / 16) tiles only (one launch per tile count, so each has its own
+// register budget); the other wavefronts return at once.  Workgroups are mapped so that each
+// XCD takes a contiguous range of units and keeps its walkers' inverses in its own L2.
+template <int NAT, int XG, int P, bool VEC>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NAT == 2 && VEC ? 4 : 1)))
+k_det_lap(const float* __restrict__ O, int ND, GroupDesc gd, const float* __restrict__ env,
+          const int64_t* __restrict__ ptr, const int32_t* __restrict__ nbr, const int32_t* __restrict__ rev, int B,
+          int N, int n_up, int D, const float* __restrict__ AinvT, double* __restrict__ trip,
+          double* __restrict__ grad) {
+  __shared__ float tbs[4][16 * 20];
+  const int wid = threadIdx.x >> 6;
+  const int64_t units = (int64_t)B * N * D;
+  const int64_t nblk = (units + 3) / 4, G8 = (nblk + 7) >> 3;
+  const int64_t lb = (int64_t)(blockIdx.x & 7) * G8 + (blockIdx.x >> 3);
+  const int64_t u = lb * 4 + wid;
+  if (lb >= nblk || u >= units) return;
+  const int d = (int)(u % D);
+  const int wn = (int)(u / D);
+  const int w = wn / N, n = wn - w * N;
+  const int gn = group_of(w, n, B, n_up, N - n_up);
+  const int64_t p0 = ptr[gn];
+  const int s = 1 + (int)(ptr[gn + 1] - p0);
+  if (((s + 15) >> 4) != NAT) return;
+  det_lap_unit<NAT, XG, P, VEC>(O, ND, gd, env, nbr, rev, B, N, n_up, D, AinvT, trip, grad, w, n, d, gn, p0, s,
+                                tbs[wid]);
+}
+
+template <bool VEC>
+static void det_lap_launch(hipStream_t st, const float* O, int ND, GroupDesc gd, const float* env, const int64_t* ptr,
+                           const int32_t* nbr, const int32_t* rev, int B, int N, int n_up, int D, int smax,
+                           const float* AinvT, double* trip, double* grad) {
+  const int64_t nblk = ((int64_t)B * N * D + 3) / 4;
+  const dim3 grid((unsigned)(8 * ((nblk + 7) / 8)));
+  const int natm = (smax + 15) / 16;
+  k_det_lap<1, 3, 2, VEC><<<grid, 256, 0, st>>>(O, ND, gd, env, ptr, nbr, rev, B, N, n_up, D, AinvT, trip, grad);
+  if (natm >= 2)
+    k_det_lap<2, 3, 1, VEC><<<grid, 256, 0, st>>>(O, ND, gd, env, ptr, nbr, rev, B, N, n_up, D, AinvT, trip, grad);
+  if (natm >= 3)
+    k_det_lap<3, 1, 1, VEC><<<grid, 256, 0, st>>>(O, ND, gd, env, ptr, nbr, rev, B, N, n_up, D, AinvT, trip, grad);
+  if (natm >= 4)
+    k_det_lap<4, 1, 1, VEC><<<grid, 256, 0, st>>>(O, ND, gd, env, ptr, nbr, rev, B, N, n_up, D, AinvT, trip, grad);
 }
 
 void launch_det_lap(hipStream_t s, const float* O, int ND, GroupDesc gd, const float* env, const int64_t* ptr,
                     const int32_t* nbr, const int32_t* rev, int B, int N, int n_up, int D, int smax,
                     const float* AinvT, double* trip, double* grad) {
-  const size_t sm = det_lap_smem(smax);
-  if (N % 4 == 0)
-    k_det_lap<4><<<B * D * N, 256, sm, s>>>(O, ND, gd, env, ptr, nbr, rev, B, N, n_up, D, smax, AinvT, trip, grad);
-  else if (N % 2 == 0)
-    k_det_lap<2><<<B * D * N, 256, sm, s>>>(O, ND, gd, env, ptr, nbr, rev, B, N, n_up, D, smax, AinvT, trip, grad);
+  if (B * N == 0) return;
+  if (N % 2 == 0 && ND % 4 == 0)
+    det_lap_launch<true>(s, O, ND, gd, env, ptr, nbr, rev, B, N, n_up, D, smax, AinvT, trip, grad);
   else
-    k_det_lap<1><<<B * D * N, 256, sm, s>>>(O, ND, gd, env, ptr, nbr, rev, B, N, n_up, D, smax, AinvT, trip, grad);
-}
-
-hipError_t det_lap_set_smem_limit(size_t bytes) {
-  hipError_t e = hipFuncSetAttribute((const void*)k_det_lap<4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-  if (e == hipSuccess)
-    e = hipFuncSetAttribute((const void*)k_det_lap<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-  if (e == hipSuccess)
-    e = hipFuncSetAttribute((const void*)k_det_lap<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-  return e;
+    det_lap_launch<false>(s, O, ND, gd, env, ptr, nbr, rev, B, N, n_up, D, smax, AinvT, trip, grad);
 }
 
 // Sum of the determinants: log|psi_det|, sign, gradient and Laplacian per walker (FP64).
