@@ -539,15 +539,324 @@ __global__ void __launch_bounds__(BM * BN / 64) k_gemm_bf16_db(const float* __re
   store_tile(acc, m0, n0, wm, wn, lane, C, ldc, M, N, bias, rowflag, period);
 }
 
+// Deep-A form of the double-buffered kernel: the A chunks (HBM) run PA chunks ahead of their
+// split into LDS through a ring of register buffers, the weight planes (L2-resident) one chunk
+// ahead.  With A_PER float4 per thread and chunk the ring costs 4 PA A_PER registers; the
+// bytes of A in flight per CU grow PA-fold, which is what bounds the staged forms on the
+// tall, narrow contractions of the tangent rows (a few hundred columns, K = 256).
+template <int NP, int BM, int BN, int KC, int PA>
+__global__ void __launch_bounds__(BM * BN / 64) k_gemm_bf16_dq(const float* __restrict__ A, int64_t lda,
+                                                              const uint16_t* __restrict__ Wp0,
+                                                              const uint16_t* __restrict__ Wp1,
+                                                              const uint16_t* __restrict__ Wp2,
+                                                              const float* __restrict__ bias, float* __restrict__ C,
+                                                              int64_t ldc, int64_t M, int N, int K,
+                                                              const uint8_t* __restrict__ rowflag, int period, int nT,
+                                                              DevRows dr) {
+  constexpr int NWN = BN / 64;
+  constexpr int NT = (BM / 64) * NWN * 64;
+  constexpr int LDH = KC + 8;
+  constexpr int A_UNITS = BM * (KC / 4), B_UNITS = BN * (KC / 8);
+  constexpr int A_PER = (A_UNITS + NT - 1) / NT;
+  constexpr int B_PER = (B_UNITS + NT - 1) / NT;
+  constexpr size_t STAGE = (size_t)NP * (BM + BN) * LDH;
+  extern __shared__ __attribute__((aligned(16))) unsigned char gemm_lds[];
+  uint16_t* lds = reinterpret_cast<uint16_t*>(gemm_lds);
+  auto As = [&](int st, int p, int r) { return lds + st * STAGE + ((size_t)p * BM + r) * LDH; };
+  auto Bs = [&](int st, int p, int r) { return lds + st * STAGE + (size_t)NP * BM * LDH + ((size_t)p * BN + r) * LDH; };
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  int64_t mt;
+  int nt;
+  tile_of(mt, nt, nT);
+  if (dr.offset) {
+    const int64_t o = *dr.offset;
+    A += o * lda;
+    C += o * ldc;
+    if (rowflag) rowflag += o;
+  }
+  if (dr.count) M = M < (int64_t)*dr.count ? M : (int64_t)*dr.count;
+  if (mt * BM >= M) return;
+  const int64_t m0 = mt * BM;
+  const int n0 = nt * BN;
+  const int wm = (wid / NWN) * 64, wn = (wid % NWN) * 64;
+  float16_t acc[2][2];
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b)
+      for (int v = 0; v < 16; ++v) acc[a][b][v] = 0.f;
+  const uint16_t* Wpl[3] = {Wp0, Wp1, Wp2};
+  float4 ra[PA][A_PER];
+  uint4 rb[NP][B_PER];
+  auto load_a = [&](int c, float4(&xa)[A_PER]) {
+#pragma unroll
+    for (int q = 0; q < A_PER; ++q) {
+      const int e = tid + q * NT;
+      const int rr = e / (KC / 4), kq = (e % (KC / 4)) * 4;
+      const int64_t gr = m0 + rr;
+      xa[q] = (e < A_UNITS && gr < M) ? *reinterpret_cast<const float4*>(A + gr * lda + c * KC + kq)
+                                      : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+  };
+  auto load_b = [&](int c) {
+#pragma unroll
+    for (int q = 0; q < B_PER; ++q) {
+      const int e = tid + q * NT;
+      const int nn = e / (KC / 8), kq = (e % (KC / 8)) * 8;
+      const int gn = n0 + nn;
+#pragma unroll
+      for (int p = 0; p < NP; ++p)
+        rb[p][q] = (e < B_UNITS && gn < N) ? *reinterpret_cast<const uint4*>(Wpl[p] + (int64_t)gn * K + c * KC + kq)
+                                           : make_uint4(0, 0, 0, 0);
+    }
+  };
+  auto store = [&](int st, const float4(&xa)[A_PER]) {
+#pragma unroll
+    for (int q = 0; q < A_PER; ++q) {
+      const int e = tid + q * NT;
+      if (A_UNITS % NT != 0 && e >= A_UNITS) continue;
+      const int rr = e / (KC / 4), kq = (e % (KC / 4)) * 4;
+      float2v_t x01 = {xa[q].x, xa[q].y}, x23 = {xa[q].z, xa[q].w};
+#pragma unroll
+      for (int p = 0; p < NP; ++p) {
+        const bf16x2_t h01 = __builtin_convertvector(x01, bf16x2_t);
+        const bf16x2_t h23 = __builtin_convertvector(x23, bf16x2_t);
+        *reinterpret_cast<uint2*>(As(st, p, rr) + kq) =
+            make_uint2(__builtin_bit_cast(uint32_t, h01), __builtin_bit_cast(uint32_t, h23));
+        if (p + 1 < NP) {
+          x01 = x01 - __builtin_convertvector(h01, float2v_t);
+          x23 = x23 - __builtin_convertvector(h23, float2v_t);
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < B_PER; ++q) {
+      const int e = tid + q * NT;
+      if (B_UNITS % NT != 0 && e >= B_UNITS) continue;
+      const int nn = e / (KC / 8), kq = (e % (KC / 8)) * 8;
+#pragma unroll
+      for (int p = 0; p < NP; ++p) *reinterpret_cast<uint4*>(Bs(st, p, nn) + kq) = rb[p][q];
+    }
+  };
+  auto compute = [&](int st) {
+#pragma unroll
+    for (int kk = 0; kk < KC; kk += 16) {
+      const int ka = kk + 8 * (lane >> 5);
+      bf16x8_t av[NP][2], bv[NP][2];
+#pragma unroll
+      for (int p = 0; p < NP; ++p) {
+#pragma unroll
+        for (int a = 0; a < 2; ++a) av[p][a] = *reinterpret_cast<const bf16x8_t*>(As(st, p, wm + a * 32 + (lane & 31)) + ka);
+#pragma unroll
+        for (int b = 0; b < 2; ++b) bv[p][b] = *reinterpret_cast<const bf16x8_t*>(Bs(st, p, wn + b * 32 + (lane & 31)) + ka);
+      }
+#pragma unroll
+      for (int t = 0; t < n_products(NP); ++t)
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int b = 0; b < 2; ++b)
+            acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[prod_a(NP, t)][a], bv[prod_b(NP, t)][b],
+                                                                acc[a][b], 0, 0, 0);
+    }
+  };
+  const int nch = K / KC;
+  // chunk 0 into stage 0; chunks 1 .. PA in the ring (chunk j in ra[j % PA]); W chunk 1 in rb
+  load_a(0, ra[0]);
+  load_b(0);
+  store(0, ra[0]);
+#pragma unroll
+  for (int j = 1; j <= PA; ++j)
+    if (j < nch) load_a(j, ra[j % PA]);
+  if (nch > 1) load_b(1);
+  __syncthreads();
+  for (int c = 0; c < nch; c += PA) {
+#pragma unroll
+    for (int q = 0; q < PA; ++q) {
+      const int cc = c + q;  // chunk in stage cc & 1; chunk cc + 1 waits in ra[(q + 1) % PA]
+      if (cc < nch) {
+        compute(cc & 1);
+        if (cc + 1 < nch) {
+          store((cc + 1) & 1, ra[(q + 1) % PA]);
+          if (cc + 1 + PA < nch) load_a(cc + 1 + PA, ra[(q + 1) % PA]);
+          if (cc + 2 < nch) load_b(cc + 2);
+        }
+        __syncthreads();
+      }
+    }
+  }
+  store_tile(acc, m0, n0, wm, wn, lane, C, ldc, M, N, bias, rowflag, period);
+}
+
+// A-in-registers form: each wavefront computes a 64 x WN tile; it loads its A fragments
+// straight from memory into MFMA operand registers (lane l: row l % 32 of a 32-row tile,
+// k = 8 (l / 32) .. +7 of each 16-k step), two chunks ahead, and splits them into the NP planes
+// in registers.  Only the weight planes go through LDS (two stages of 16 k, one barrier per
+// chunk), shared by the NWM x NWN wavefronts of the block (BM = 64 NWM rows, BN = WN NWN
+// columns); the NWN wavefronts of a row range read the same A rows, served once from memory
+// through the CU's cache.  Halves the LDS reads per MFMA of the staged forms, whose operands
+// both come from LDS.
+template <int NP, int NWM, int NWN, int WN>
+__global__ void __launch_bounds__(NWM * NWN * 64) __attribute__((amdgpu_waves_per_eu(2))) k_gemm_bf16_ar(const float* __restrict__ A, int64_t lda,
+                                                           const uint16_t* __restrict__ Wp0,
+                                                           const uint16_t* __restrict__ Wp1,
+                                                           const uint16_t* __restrict__ Wp2,
+                                                           const float* __restrict__ bias, float* __restrict__ C,
+                                                           int64_t ldc, int64_t M, int N, int K,
+                                                           const uint8_t* __restrict__ rowflag, int period, int nT,
+                                                           DevRows dr) {
+  constexpr int NT = NWM * NWN * 64, BM = 64 * NWM, BN = WN * NWN, NB = WN / 32;
+  constexpr int KC = 16, LDH = KC + 8;     // halves per LDS row: 48 bytes, 16-byte aligned
+  constexpr int B_UNITS = BN * (KC / 8);   // uint4 per plane and chunk
+  constexpr int B_PER = (B_UNITS + NT - 1) / NT;
+  constexpr int STAGE = NP * BN * LDH;     // halves per stage
+  constexpr int PA = 3;                    // A chunks in registers (loads two chunks ahead)
+  __shared__ __attribute__((aligned(16))) uint16_t Ws[2 * STAGE];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  int64_t mt;
+  int nt;
+  tile_of(mt, nt, nT);
+  if (dr.offset) {
+    const int64_t o = *dr.offset;
+    A += o * lda;
+    C += o * ldc;
+    if (rowflag) rowflag += o;
+  }
+  if (dr.count) M = M < (int64_t)*dr.count ? M : (int64_t)*dr.count;
+  if (mt * BM >= M) return;
+  const int64_t m0 = mt * BM;
+  const int n0 = nt * BN;
+  const int wm = (wid / NWN) * 64, wn = (wid % NWN) * WN;
+  const uint16_t* Wpl[3] = {Wp0, Wp1, Wp2};
+  // this lane's A rows (two 32-row tiles) and k offset inside a 16-k step
+  const float* arow[2];
+  bool aok[2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a) {
+    const int64_t r = m0 + wm + a * 32 + (lane & 31);
+    aok[a] = r < M;
+    arow[a] = A + (aok[a] ? r : 0) * lda + 8 * (lane >> 5);
+  }
+  float4 ra[PA][2][2];  // [buffer][row tile][half of the 8 k]
+  auto load_a = [&](int c, float4(&x)[2][2]) {
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+        x[a][h] = aok[a] ? *reinterpret_cast<const float4*>(arow[a] + c * KC + 4 * h) : make_float4(0.f, 0.f, 0.f, 0.f);
+  };
+  uint4 rb[NP][B_PER];
+  auto load_b = [&](int c) {
+#pragma unroll
+    for (int q = 0; q < B_PER; ++q) {
+      const int e = tid + q * NT;
+      const int nn = e >> 1, kq = (e & 1) * 8;
+      const int gn = n0 + nn;
+#pragma unroll
+      for (int p = 0; p < NP; ++p)
+        rb[p][q] = (e < B_UNITS && gn < N) ? *reinterpret_cast<const uint4*>(Wpl[p] + (int64_t)gn * K + c * KC + kq)
+                                           : make_uint4(0, 0, 0, 0);
+    }
+  };
+  auto store_b = [&](int st) {
+#pragma unroll
+    for (int q = 0; q < B_PER; ++q) {
+      const int e = tid + q * NT;
+      if (B_UNITS % NT != 0 && e >= B_UNITS) continue;
+      const int nn = e >> 1, kq = (e & 1) * 8;
+#pragma unroll
+      for (int p = 0; p < NP; ++p) *reinterpret_cast<uint4*>(Ws + st * STAGE + (p * BN + nn) * LDH + kq) = rb[p][q];
+    }
+  };
+  float16_t acc[2][NB];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) acc[a][b][v] = 0.f;
+  auto compute = [&](int st, const float4(&x)[2][2]) {
+    // x = hi + mid + lo with hardware round-to-nearest conversions; each remainder is exact
+    bf16x8_t av[NP][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      float2v_t r[4] = {{x[a][0].x, x[a][0].y}, {x[a][0].z, x[a][0].w}, {x[a][1].x, x[a][1].y}, {x[a][1].z, x[a][1].w}};
+#pragma unroll
+      for (int p = 0; p < NP; ++p) {
+        bf16x2_t h[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          h[j] = __builtin_convertvector(r[j], bf16x2_t);
+          if (p + 1 < NP) r[j] = r[j] - __builtin_convertvector(h[j], float2v_t);
+        }
+        av[p][a] = (bf16x8_t){h[0][0], h[0][1], h[1][0], h[1][1], h[2][0], h[2][1], h[3][0], h[3][1]};
+      }
+    }
+    const int kb = 8 * (lane >> 5);
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      bf16x8_t bv[NP];
+#pragma unroll
+      for (int p = 0; p < NP; ++p)
+        bv[p] = *reinterpret_cast<const bf16x8_t*>(Ws + st * STAGE + (p * BN + wn + b * 32 + (lane & 31)) * LDH + kb);
+#pragma unroll
+      for (int t = 0; t < n_products(NP); ++t)
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[prod_a(NP, t)][a], bv[prod_b(NP, t)], acc[a][b], 0, 0, 0);
+    }
+  };
+  const int nch = K / KC;
+  load_b(0);
+  load_a(0, ra[0]);
+  if (nch > 1) load_a(1, ra[1]);
+  store_b(0);
+  if (nch > 1) load_b(1);
+  __syncthreads();
+  // chunk c: A in ra[c % 3], W in stage c % 2; chunk c + 1's W is in registers, chunk c + 2's A
+  // is loaded now
+  for (int c = 0; c < nch; c += PA) {
+#pragma unroll
+    for (int q = 0; q < PA; ++q) {
+      const int cc = c + q;
+      if (cc < nch) {
+        if (cc + 2 < nch) load_a(cc + 2, ra[(q + 2) % PA]);
+        compute(cc & 1, ra[q]);
+        if (cc + 1 < nch) {
+          store_b((cc + 1) & 1);
+          if (cc + 2 < nch) load_b(cc + 2);
+        }
+        __syncthreads();
+      }
+    }
+  }
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const int col = n0 + wn + b * 32 + (lane & 31);
+    if (col >= N) continue;
+    const float bcol = bias ? bias[col] : 0.f;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const int64_t row = m0 + wm + a * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+        if (row >= M) continue;
+        float val = acc[a][b][v];
+        if (bias && is_bias_row(row, rowflag, period)) val += bcol;
+        C[row * ldc + col] = val;
+      }
+  }
+}
+
 // Tile shapes offered to the autotuner (index = variant).
 struct Bf16Tile {
   int bm, bn;
 };
 // variants 0-6: single LDS stage, K chunk 32; variants 7-10: two LDS stages (kc = 16 or 32);
-// variants 11-14: two LDS stages with loads two chunks ahead
-static const Bf16Tile kBf16Tiles[] = {{128, 128}, {128, 256}, {256, 128}, {64, 256}, {256, 64},
-                                      {64, 128},  {128, 64},  {128, 128}, {128, 128}, {256, 128},
-                                      {128, 256}, {256, 128}, {128, 256}, {128, 128}, {128, 128}};
+// variants 11-14: two LDS stages with loads two chunks ahead; variants 15-17: A in registers;
+// variants 18-22: two LDS stages with A PA chunks ahead
+static const Bf16Tile kBf16Tiles[] = {{128, 128}, {128, 256}, {256, 128}, {64, 256}, {256, 64},  {64, 128},
+                                      {128, 64},  {128, 128}, {128, 128}, {256, 128}, {128, 256}, {256, 128},
+                                      {128, 256}, {128, 128}, {128, 128}, {128, 256}, {256, 128}, {64, 256},
+                                      {128, 256}, {128, 256}, {256, 128}, {128, 256}, {128, 128}};
 
 int gemm_bf16_variants() { return (int)(sizeof(kBf16Tiles) / sizeof(kBf16Tiles[0])); }
 void gemm_bf16_tile(int variant, int* bm, int* bn) {
@@ -556,12 +865,18 @@ void gemm_bf16_tile(int variant, int* bm, int* bn) {
   *bn = t.bn;
 }
 
+constexpr size_t kMaxLds = 160 * 1024;  // LDS of one CU
+
 template <int NP, int BM, int BN>
 static void launch_tile(hipStream_t s, const float* A, int64_t lda, const uint16_t* const* W, const float* bias,
                         float* C, int64_t ldc, int64_t M, int N, int K, const uint8_t* rowflag, int period,
                         DevRows dr) {
   static bool attr_set = false;  // LDS above 64 KB must be enabled once per kernel
   const size_t lds = (size_t)NP * (BM + BN) * LDB_H * 2;
+  if (lds > kMaxLds) {  // does not fit one CU: the base tile (same result) is launched instead
+    launch_tile<NP, 128, 128>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr);
+    return;
+  }
   if (!attr_set) {
     if (lds > 64 * 1024)
       (void)hipFuncSetAttribute((const void*)k_gemm_bf16<NP, BM, BN>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -580,6 +895,10 @@ static void launch_tile_db(hipStream_t s, const float* A, int64_t lda, const uin
                            DevRows dr) {
   static bool attr_set = false;
   const size_t lds = (size_t)2 * NP * (BM + BN) * (KC + 8) * 2;
+  if (lds > kMaxLds) {
+    launch_tile<NP, 128, 128>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr);
+    return;
+  }
   if (!attr_set) {
     if (lds > 64 * 1024)
       (void)hipFuncSetAttribute((const void*)k_gemm_bf16_db<NP, BM, BN, KC, PD>,
@@ -590,6 +909,38 @@ static void launch_tile_db(hipStream_t s, const float* A, int64_t lda, const uin
   const int64_t blocks = ((M + BM - 1) / BM) * nT;
   k_gemm_bf16_db<NP, BM, BN, KC, PD><<<(unsigned)blocks, BM * BN / 64, lds, s>>>(A, lda, W[0], W[1], W[2], bias, C,
                                                                                ldc, M, N, K, rowflag, period, nT, dr);
+}
+
+template <int NP, int BM, int BN, int KC, int PA>
+static void launch_tile_dq(hipStream_t s, const float* A, int64_t lda, const uint16_t* const* W, const float* bias,
+                           float* C, int64_t ldc, int64_t M, int N, int K, const uint8_t* rowflag, int period,
+                           DevRows dr) {
+  static bool attr_set = false;
+  const size_t lds = (size_t)2 * NP * (BM + BN) * (KC + 8) * 2;
+  if (lds > kMaxLds) {
+    launch_tile<NP, 128, 128>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr);
+    return;
+  }
+  if (!attr_set) {
+    if (lds > 64 * 1024)
+      (void)hipFuncSetAttribute((const void*)k_gemm_bf16_dq<NP, BM, BN, KC, PA>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    attr_set = true;
+  }
+  const int nT = (N + BN - 1) / BN;
+  const int64_t blocks = ((M + BM - 1) / BM) * nT;
+  k_gemm_bf16_dq<NP, BM, BN, KC, PA><<<(unsigned)blocks, BM * BN / 64, lds, s>>>(A, lda, W[0], W[1], W[2], bias, C,
+                                                                               ldc, M, N, K, rowflag, period, nT, dr);
+}
+
+template <int NP, int NWM, int NWN, int WN>
+static void launch_tile_ar(hipStream_t s, const float* A, int64_t lda, const uint16_t* const* W, const float* bias,
+                           float* C, int64_t ldc, int64_t M, int N, int K, const uint8_t* rowflag, int period,
+                           DevRows dr) {
+  const int nT = (N + WN * NWN - 1) / (WN * NWN);
+  const int64_t blocks = ((M + 64 * NWM - 1) / (64 * NWM)) * nT;
+  k_gemm_bf16_ar<NP, NWM, NWN, WN><<<(unsigned)blocks, NWM * NWN * 64, 0, s>>>(A, lda, W[0], W[1], W[2], bias, C, ldc,
+                                                                              M, N, K, rowflag, period, nT, dr);
 }
 
 template <int NP>
@@ -605,6 +956,14 @@ static void launch_np(hipStream_t s, int variant, const float* A, int64_t lda, c
     case 12: launch_tile_db<NP, 128, 256, 16, 2>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
     case 13: launch_tile_db<NP, 128, 128, 32, 2>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
     case 14: launch_tile_db<NP, 128, 128, 16, 2>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
+    case 15: launch_tile_ar<NP, 2, 2, 128>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
+    case 16: launch_tile_ar<NP, 4, 1, 128>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
+    case 17: launch_tile_ar<NP, 1, 4, 64>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
+    case 18: launch_tile_dq<NP, 128, 256, 16, 4>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
+    case 19: launch_tile_dq<NP, 128, 256, 16, 6>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
+    case 20: launch_tile_dq<NP, 256, 128, 16, 4>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
+    case 21: launch_tile_dq<NP, 128, 256, 32, 3>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
+    case 22: launch_tile_dq<NP, 128, 128, 16, 6>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
     case 1: launch_tile<NP, 128, 256>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
     case 2: launch_tile<NP, 256, 128>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
     case 3: launch_tile<NP, 64, 256>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;

@@ -452,6 +452,159 @@ __global__ void __launch_bounds__(NW * 64) k_gj_inv_reg(const double* __restrict
     }
 }
 
+__device__ __forceinline__ double readlane_f64(double x, int l) {  // l uniform
+  const uint64_t u = __builtin_bit_cast(uint64_t, x);
+  const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)u, l), hi = __builtin_amdgcn_readlane((uint32_t)(u >> 32), l);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// Column-distributed register Gauss-Jordan inverse with implicit partial pivoting, FP64: the
+// same pivot sequence and result layout as k_gj_inv_reg.  Column j lives in wavefront j % NW
+// (register c = j / NW), row i in lane i % 64 (register m = i / 64).  The wavefront holding
+// column k picks the unused row p with the largest |A[p][k]| (ties: lowest row) by a wavefront
+// reduction and publishes p, the pivot and column k through LDS (double-buffered by step
+// parity, so one barrier per step suffices); every wavefront then reads row p of its own
+// columns with readlane and updates its elements in registers.
+template <int NW, int RPT, int CPT>
+__global__ void __launch_bounds__(NW * 64) k_gj_inv_col(const double* __restrict__ Aall, int N,
+                                                        double* __restrict__ Ainv_all, float* __restrict__ AinvT32,
+                                                        double* __restrict__ logdet, double* __restrict__ sign,
+                                                        int want_inv) {
+  __shared__ double s_col[2][RPT * 64];
+  __shared__ double s_pv[2];
+  __shared__ int s_p[2];
+  __shared__ int s_piv[RPT * 64];
+  __shared__ int s_inv[RPT * 64];
+  const int mat = blockIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const double* Ag = Aall + (size_t)mat * N * N;
+  double S[RPT][CPT];
+#pragma unroll
+  for (int m = 0; m < RPT; ++m)
+#pragma unroll
+    for (int c = 0; c < CPT; ++c) {
+      const int i = lane + 64 * m, j = wid + NW * c;
+      S[m][c] = (i < N && j < N) ? Ag[(size_t)i * N + j] : 0.0;
+    }
+  uint32_t used = 0;  // bit m: row lane + 64 m was a pivot
+  double ld = 0.0;
+  int neg = 0;
+  for (int k = 0; k < N; ++k) {
+    // buffer k & 1: a wavefront still reading it for step k - 2 has passed the barrier of k - 1
+    const int buf = k & 1;
+    if (wid == k % NW) {
+      const int ck = k / NW;
+      double v[RPT];
+#pragma unroll
+      for (int m = 0; m < RPT; ++m) {
+        double x = 0.0;
+#pragma unroll
+        for (int c = 0; c < CPT; ++c)
+          if (c == ck) x = S[m][c];
+        v[m] = x;
+        s_col[buf][lane + 64 * m] = x;
+      }
+      double best = -1.0;
+      int bi = 1 << 30;
+#pragma unroll
+      for (int m = 0; m < RPT; ++m) {
+        const int i = lane + 64 * m;
+        const double a = fabs(v[m]);
+        if (i < N && !((used >> m) & 1u) && (a > best || (a == best && i < bi))) {
+          best = a;
+          bi = i;
+        }
+      }
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        const double ob = __shfl_xor(best, o, 64);
+        const int oi = __shfl_xor(bi, o, 64);
+        if (ob > best || (ob == best && oi < bi)) {
+          best = ob;
+          bi = oi;
+        }
+      }
+      if (lane == (bi & 63)) {
+        double x = 0.0;
+#pragma unroll
+        for (int m = 0; m < RPT; ++m)
+          if (m == (bi >> 6)) x = v[m];
+        s_pv[buf] = x;
+        s_p[buf] = bi;
+      }
+    }
+    __syncthreads();
+    const int p = __builtin_amdgcn_readfirstlane(s_p[buf]);
+    const double pv = s_pv[buf];
+    const double ip = 1.0 / pv;
+    if (tid == 0) {
+      ld += log(fabs(pv));
+      neg += pv < 0.0;
+      s_piv[k] = p;
+    }
+    const int mp = p >> 6, lp = p & 63;
+    if (lane == lp) used |= 1u << mp;
+    double fk[RPT];
+#pragma unroll
+    for (int m = 0; m < RPT; ++m) fk[m] = s_col[buf][lane + 64 * m];
+    // every row takes S - fk rs (the pivot row is overwritten below); column k becomes -fk ip
+    double rs[CPT];
+#pragma unroll
+    for (int c = 0; c < CPT; ++c) {
+      double x = S[0][c];
+#pragma unroll
+      for (int m = 1; m < RPT; ++m) x = m == mp ? S[m][c] : x;
+      if (wid + NW * c == k) {  // uniform: column k
+        rs[c] = ip;
+#pragma unroll
+        for (int m = 0; m < RPT; ++m) S[m][c] = -fk[m] * ip;
+      } else {
+        rs[c] = readlane_f64(x, lp) * ip;  // row p of this column, scaled
+#pragma unroll
+        for (int m = 0; m < RPT; ++m) S[m][c] = S[m][c] - fk[m] * rs[c];
+      }
+    }
+    if (lane == lp) {
+#pragma unroll
+      for (int m = 0; m < RPT; ++m)
+        if (m == mp) {
+#pragma unroll
+          for (int c = 0; c < CPT; ++c) S[m][c] = rs[c];
+        }
+    }
+  }
+  __syncthreads();
+  for (int k = tid; k < N; k += NW * 64) s_inv[s_piv[k]] = k;
+  __syncthreads();
+  if (tid == 0) {
+    // parity of k -> piv(k): N minus the number of cycles (visited marks kept in s_inv's sign bit)
+    int cycles = 0;
+    for (int s0 = 0; s0 < N; ++s0) {
+      if (s_inv[s0] < 0) continue;
+      ++cycles;
+      for (int t = s0; s_inv[t] >= 0; t = s_piv[t]) s_inv[t] = -1 - s_inv[t];
+    }
+    for (int t = 0; t < N; ++t) s_inv[t] = -1 - s_inv[t];
+    logdet[mat] = ld;
+    sign[mat] = (((N - cycles) + neg) & 1) ? -1.0 : 1.0;
+  }
+  if (!want_inv) return;
+  __syncthreads();
+  double* X = Ainv_all + (size_t)mat * N * N;
+  float* XT = AinvT32 ? AinvT32 + (size_t)mat * N * N : nullptr;
+#pragma unroll
+  for (int m = 0; m < RPT; ++m)
+#pragma unroll
+    for (int c = 0; c < CPT; ++c) {
+      const int r = lane + 64 * m, cc = wid + NW * c;
+      if (r < N && cc < N) {
+        const int i = s_inv[r], j = s_piv[cc];
+        X[(size_t)i * N + j] = S[m][c];
+        if (XT) XT[(size_t)j * N + i] = (float)S[m][c];
+      }
+    }
+}
+
 size_t lu_lds_bytes(int N) { return (size_t)N * N * sizeof(double) + (size_t)N * (sizeof(double) + sizeof(int)); }
 
 hipError_t lu_set_smem_limit(size_t bytes) {
@@ -471,9 +624,9 @@ __global__ void k_transpose_f32(const double* __restrict__ X, int N, float* __re
 void launch_lu_inv(hipStream_t s, double* A, int n_mat, int N, double* Ainv, float* AinvT32, double* logdet,
                    double* sign, bool want_inv, bool use_lds) {
   if (N <= 144) {
-    k_gj_inv_reg<16, 9, 3><<<n_mat, 1024, 0, s>>>(A, N, Ainv, AinvT32, logdet, sign, want_inv ? 1 : 0);
+    k_gj_inv_col<16, 3, 9><<<n_mat, 1024, 0, s>>>(A, N, Ainv, AinvT32, logdet, sign, want_inv ? 1 : 0);
   } else if (N <= 192) {
-    k_gj_inv_reg<16, 12, 3><<<n_mat, 1024, 0, s>>>(A, N, Ainv, AinvT32, logdet, sign, want_inv ? 1 : 0);
+    k_gj_inv_col<16, 3, 12><<<n_mat, 1024, 0, s>>>(A, N, Ainv, AinvT32, logdet, sign, want_inv ? 1 : 0);
   } else if (use_lds) {
     k_gj_inv_lds<<<n_mat, 512, lu_lds_bytes(N), s>>>(A, N, Ainv, AinvT32, logdet, sign, want_inv ? 1 : 0);
   } else {
@@ -716,7 +869,7 @@ static void det_lap_launch(hipStream_t st, const float* O, int ND, GroupDesc gd,
   if (natm >= 2)
     k_det_lap<2, 3, 1, VEC><<<grid, 256, 0, st>>>(O, ND, gd, env, ptr, nbr, rev, B, N, n_up, D, AinvT, trip, grad);
   if (natm >= 3)
-    k_det_lap<3, 1, 1, VEC><<<grid, 256, 0, st>>>(O, ND, gd, env, ptr, nbr, rev, B, N, n_up, D, AinvT, trip, grad);
+    k_det_lap<3, 3, 1, VEC><<<grid, 256, 0, st>>>(O, ND, gd, env, ptr, nbr, rev, B, N, n_up, D, AinvT, trip, grad);
   if (natm >= 4)
     k_det_lap<4, 1, 1, VEC><<<grid, 256, 0, st>>>(O, ND, gd, env, ptr, nbr, rev, B, N, n_up, D, AinvT, trip, grad);
 }
