@@ -335,10 +335,10 @@ void gemm(Ctx& c, const float* A, int64_t lda, int wi, int bi, float* C, int64_t
     HIPCHK(hipEventCreate(&e1));
     HIPCHK(hipEventRecord(e0, c.s));
   }
-  if (use_bf)
-    launch_gemm_bf16(c.s, A, lda, h->Pbf[wi], h->Pbf_lo[wi], h->Pbf_lo2[wi], bias, C, ldc, M, N, K, rowflag, period, ns,
-                     variant, dr);
-  else
+  if (use_bf) {
+    launch_gemm_bf16(c.s, A, lda, h->Pbf[wi], h->Pbf_lo[wi], h->Pbf_lo2[wi], bias, C, ldc, M, N, K, rowflag, period,
+                     ns, variant, dr);
+  } else
     launch_gemm_f32(c.s, A, lda, h->P[wi], h->Pt32[wi], bias, C, ldc, M, N, K, rowflag, period, dr);
   if (timed) {
     HIPCHK(hipEventRecord(e1, c.s));
@@ -1100,7 +1100,8 @@ int fire_create(const fire_config* cfg, const float* R, const float* Z, int devi
     double enn = 0.0;
     for (int I = 0; I < c.n_nuc; ++I)
       for (int J = I + 1; J < c.n_nuc; ++J) {
-        const double dx = R[3 * I] - R[3 * J], dy = R[3 * I + 1] - R[3 * J + 1], dz = R[3 * I + 2] - R[3 * J + 2];
+        const double dx = (double)R[3 * I] - R[3 * J], dy = (double)R[3 * I + 1] - R[3 * J + 1],
+                     dz = (double)R[3 * I + 2] - R[3 * J + 2];
         enn += (double)Z[I] * Z[J] / std::sqrt(dx * dx + dy * dy + dz * dz);
       }
     h->e_nn = enn;

@@ -69,6 +69,9 @@ struct DevRows {
 void launch_gemm_f32(hipStream_t s, const float* A, int64_t lda, const float* W, const float* Wt, const float* bias,
                      float* C, int64_t ldc, int64_t M, int N, int K, const uint8_t* rowflag, int period,
                      DevRows dr = DevRows());
+void launch_gemm_f32_tail(hipStream_t s, const float* A, int64_t lda, const float* Wt, const float* bias, float* C,
+                          int64_t ldc, int64_t M, int n0, int nr, int K, const uint8_t* rowflag, int period,
+                          DevRows dr, int64_t sA, int64_t sW, int64_t sC, int nb);
 void launch_gemm_f32_batched(hipStream_t s, const float* A, int64_t lda, int64_t sA, const float* Wt, int64_t sW,
                              float* C, int64_t ldc, int64_t sC, int64_t M, int N, int K, int nb);
 // nsplit 1: plain BF16; 3: W = hi + lo planes, 3 products; 6: W = hi + mid + lo, 6 products

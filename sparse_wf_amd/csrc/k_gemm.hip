@@ -353,6 +353,91 @@ __global__ void k_gemm_smalln(const float* __restrict__ A, int64_t lda, const fl
   }
 }
 
+// Column tail C[:, n0 .. n0 + nr) (nr <= 16) of a product whose other columns go through
+// 128-wide tiles, FP32 MFMA 16x16x4.  Workgroup = 4 wavefronts x 64 rows (four 16-row tiles);
+// lane l holds row / column l % 16 and, in a chunk of 16 k starting at k0, the four
+// consecutive k0 + 4 (l / 16) + t, and MFMA step t contracts exactly those k (the same
+// permutation of k for both operands).  VEC: K % 16 == 0 and lda % 4 == 0.
+template <bool VEC>
+__global__ void __launch_bounds__(256) k_gemm_f32_tail(const float* __restrict__ A, int64_t lda,
+                                                       const float* __restrict__ Wt, const float* __restrict__ bias,
+                                                       float* __restrict__ C, int64_t ldc, int64_t M, int n0, int nr,
+                                                       int K, const uint8_t* __restrict__ rowflag, int period,
+                                                       DevRows dr, int64_t sA, int64_t sW, int64_t sC) {
+  typedef float f4_t __attribute__((ext_vector_type(4)));
+  A += blockIdx.z * sA;
+  Wt += blockIdx.z * sW;
+  C += blockIdx.z * sC;
+  if (dr.offset) {
+    const int64_t o = *dr.offset;
+    A += o * lda;
+    C += o * ldc;
+    if (rowflag) rowflag += o;
+  }
+  if (dr.count) M = M < (int64_t)*dr.count ? M : (int64_t)*dr.count;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, li = lane & 15, kg = lane >> 4;
+  const int64_t rbase = (int64_t)blockIdx.x * 256 + wid * 64;
+  if (rbase >= M) return;
+  const float* ar[4];
+  bool aok[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int64_t r = rbase + t * 16 + li;
+    aok[t] = r < M;
+    ar[t] = A + (aok[t] ? r : 0) * lda + 4 * kg;
+  }
+  const bool wok = li < nr;
+  const float* wr = Wt + (int64_t)(n0 + (wok ? li : 0)) * K + 4 * kg;
+  f4_t acc[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) acc[t] = (f4_t){0.f, 0.f, 0.f, 0.f};
+  for (int k0 = 0; k0 < K; k0 += 16) {
+    f4_t a[4], w;
+    if (VEC) {
+#pragma unroll
+      for (int t = 0; t < 4; ++t) a[t] = aok[t] ? *reinterpret_cast<const f4_t*>(ar[t] + k0) : (f4_t)(0.f);
+      w = wok ? *reinterpret_cast<const f4_t*>(wr + k0) : (f4_t)(0.f);
+    } else {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const bool kok = k0 + 4 * kg + j < K;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) a[t][j] = aok[t] && kok ? ar[t][k0 + j] : 0.f;
+        w[j] = wok && kok ? wr[k0 + j] : 0.f;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[t][j], w[j], acc[t], 0, 0, 0);
+  }
+  if (!wok) return;
+  const int col = n0 + li;
+  const float bcol = bias ? bias[col] : 0.f;
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int64_t row = rbase + t * 16 + 4 * kg + v;
+      if (row >= M) continue;
+      float val = acc[t][v];
+      if (bias && is_bias_row(row, rowflag, period)) val += bcol;
+      C[row * ldc + col] = val;
+    }
+}
+
+void launch_gemm_f32_tail(hipStream_t s, const float* A, int64_t lda, const float* Wt, const float* bias, float* C,
+                          int64_t ldc, int64_t M, int n0, int nr, int K, const uint8_t* rowflag, int period,
+                          DevRows dr, int64_t sA, int64_t sW, int64_t sC, int nb) {
+  if (M == 0 || nr <= 0 || nb == 0) return;
+  const dim3 grid((unsigned)((M + 255) / 256), 1, (unsigned)nb);
+  if ((K % 16) == 0 && (lda % 4) == 0)
+    k_gemm_f32_tail<true><<<grid, 256, 0, s>>>(A, lda, Wt, bias, C, ldc, M, n0, nr, K, rowflag, period, dr, sA, sW, sC);
+  else
+    k_gemm_f32_tail<false><<<grid, 256, 0, s>>>(A, lda, Wt, bias, C, ldc, M, n0, nr, K, rowflag, period, dr, sA, sW,
+                                                sC);
+}
+
 void launch_gemm_f32(hipStream_t s, const float* A, int64_t lda, const float* W, const float* Wt, const float* bias,
                      float* C, int64_t ldc, int64_t M, int N, int K, const uint8_t* rowflag, int period, DevRows dr) {
   if (M == 0) return;
@@ -362,14 +447,17 @@ void launch_gemm_f32(hipStream_t s, const float* A, int64_t lda, const float* W,
                                                                       dr);
     return;
   }
-  const int nT = (N + GT_N - 1) / GT_N;
+  // a remainder of at most 16 columns past whole 128-wide tiles goes to the tail kernel
+  const int rem = N % GT_N, Nm = rem > 0 && rem <= 16 ? N - rem : N;
+  const int nT = (Nm + GT_N - 1) / GT_N;
   const int64_t blocks = ((M + GT_M - 1) / GT_M) * nT;
   if ((K % GT_K) == 0 && (lda % 4) == 0)
-    k_gemm_f32<true><<<(unsigned)blocks, 256, 0, s>>>(A, lda, Wt, bias, C, ldc, M, N, K, rowflag, period, nT, dr, 0, 0,
-                                                      0);
+    k_gemm_f32<true><<<(unsigned)blocks, 256, 0, s>>>(A, lda, Wt, bias, C, ldc, M, Nm, K, rowflag, period, nT, dr, 0,
+                                                      0, 0);
   else
-    k_gemm_f32<false><<<(unsigned)blocks, 256, 0, s>>>(A, lda, Wt, bias, C, ldc, M, N, K, rowflag, period, nT, dr, 0,
+    k_gemm_f32<false><<<(unsigned)blocks, 256, 0, s>>>(A, lda, Wt, bias, C, ldc, M, Nm, K, rowflag, period, nT, dr, 0,
                                                        0, 0);
+  if (Nm < N) launch_gemm_f32_tail(s, A, lda, Wt, bias, C, ldc, M, Nm, N - Nm, K, rowflag, period, dr, 0, 0, 0, 1);
 }
 
 // nb independent products C_b = A_b W_b (b < nb; operands at strides sA, sW (of Wt), sC) in
@@ -377,12 +465,16 @@ void launch_gemm_f32(hipStream_t s, const float* A, int64_t lda, const float* W,
 void launch_gemm_f32_batched(hipStream_t s, const float* A, int64_t lda, int64_t sA, const float* Wt, int64_t sW,
                              float* C, int64_t ldc, int64_t sC, int64_t M, int N, int K, int nb) {
   if (M == 0 || nb == 0) return;
-  const int nT = (N + GT_N - 1) / GT_N;
+  const int rem = N % GT_N, Nm = rem > 0 && rem <= 16 && N > GT_N ? N - rem : N;
+  const int nT = (Nm + GT_N - 1) / GT_N;
   const dim3 grid((unsigned)(((M + GT_M - 1) / GT_M) * nT), 1, (unsigned)nb);
   if ((K % GT_K) == 0 && (lda % 4) == 0)
-    k_gemm_f32<true><<<grid, 256, 0, s>>>(A, lda, Wt, nullptr, C, ldc, M, N, K, nullptr, 0, nT, DevRows(), sA, sW, sC);
+    k_gemm_f32<true><<<grid, 256, 0, s>>>(A, lda, Wt, nullptr, C, ldc, M, Nm, K, nullptr, 0, nT, DevRows(), sA, sW, sC);
   else
-    k_gemm_f32<false><<<grid, 256, 0, s>>>(A, lda, Wt, nullptr, C, ldc, M, N, K, nullptr, 0, nT, DevRows(), sA, sW, sC);
+    k_gemm_f32<false><<<grid, 256, 0, s>>>(A, lda, Wt, nullptr, C, ldc, M, Nm, K, nullptr, 0, nT, DevRows(), sA, sW,
+                                           sC);
+  if (Nm < N)
+    launch_gemm_f32_tail(s, A, lda, Wt, nullptr, C, ldc, M, Nm, N - Nm, K, nullptr, 0, DevRows(), sA, sW, sC, nb);
 }
 
 // Double-buffered form: two LDS stages of KC (16 or 32) k each; chunk c is multiplied from

@@ -28,6 +28,17 @@ def water_cfg(**kw):
     return FireConfig(mol=mol, **base)
 
 
+def alkane_cfg(**kw):
+    """C16H34 (130 electrons) with one determinant and small layers: N = 130 puts the
+    orbital-count dimension one column tile plus a narrow remainder wide (the FP32 column-tail
+    kernel), and ND = 130 is not a multiple of 4 (the scalar determinant-Laplacian loads)."""
+    mol = systems.zigzag_alkane(16)
+    base = dict(cutoff=3.0, feature_dim=64, pair_mlp_widths=(4, 4), pair_n_envelopes=4, n_determinants=1,
+                n_envelopes=8, mlp_width=32, mlp_depth=2, attention_heads=4, attention_dim=4)
+    base.update(kw)
+    return FireConfig(mol=mol, **base)
+
+
 def make_params(cfg, seed=0, r_init=None, embed_fn=None):
     """Random params with the Jastrow exercised and the normalisation scales set from the
     oracle embedding on ``r_init`` (one walker)."""

@@ -14,7 +14,7 @@ import torch
 from oracle import fire
 from sparse_wf_amd.wave_function import FireWaveFunction
 
-from cases import make_params, small_cfg, walkers, water_cfg
+from cases import alkane_cfg, make_params, small_cfg, walkers, water_cfg
 
 pytestmark = pytest.mark.gpu
 
@@ -29,6 +29,7 @@ CASES = {
     "small": lambda: small_cfg(False),
     "small_restricted": lambda: small_cfg(True),
     "water": lambda: water_cfg(),
+    "alkane130": lambda: alkane_cfg(),
 }
 
 
@@ -123,7 +124,9 @@ def test_local_energy(setup, precision):
     for w in range(len(r)):
         sign, lp = fire.log_psi_fwd_lap(pt, cfg, r[w])
         ek_ref = float(-0.5 * (lp.l + (lp.j * lp.j).sum()))
-        ep_ref = fire.potential_energy(r[w], cfg.mol.R, cfg.mol.Z)
+        # the library receives the nuclear positions as float32 (as the electron positions) and
+        # sums the Coulomb terms of those in FP64: compare with the same rounded positions
+        ep_ref = fire.potential_energy(r[w], np.asarray(cfg.mol.R, np.float32), cfg.mol.Z)
         g = out["grad"][w].cpu().numpy().reshape(-1)
         eg = _rel(g, lp.j.numpy())
         el = abs(float(out["lap"][w]) - float(lp.l)) / max(1.0, abs(float(lp.l)))
