@@ -778,6 +778,168 @@ __global__ void __launch_bounds__(BM * BN / 64) k_gemm_bf16_dq(const float* __re
   store_tile(acc, m0, n0, wm, wn, lane, C, ldc, M, N, bias, rowflag, period);
 }
 
+// Wide-wavefront form of the deep-A kernel: each wavefront computes 64 x 128 (2 x 4 MFMA
+// tiles, 128 accumulators), so a K step reads 6 NP operand fragments from LDS for 8 tiles
+// instead of 4 NP for 4, and a 256 x 256 block stages each weight chunk once for 256 rows.
+// Both cut the LDS traffic per output, which bounds the 64 x 64 forms.
+template <int NP, int BM, int BN, int KC, int PA>
+__global__ void __launch_bounds__(BM * BN / 128) __attribute__((amdgpu_waves_per_eu(2))) k_gemm_bf16_dw(const float* __restrict__ A, int64_t lda,
+                                                              const uint16_t* __restrict__ Wp0,
+                                                              const uint16_t* __restrict__ Wp1,
+                                                              const uint16_t* __restrict__ Wp2,
+                                                              const float* __restrict__ bias, float* __restrict__ C,
+                                                              int64_t ldc, int64_t M, int N, int K,
+                                                              const uint8_t* __restrict__ rowflag, int period, int nT,
+                                                              DevRows dr) {
+  constexpr int NWN = BN / 128;
+  constexpr int NT = (BM / 64) * NWN * 64;
+  constexpr int LDH = KC + 8;
+  constexpr int A_UNITS = BM * (KC / 4), B_UNITS = BN * (KC / 8);
+  constexpr int A_PER = (A_UNITS + NT - 1) / NT;
+  constexpr int B_PER = (B_UNITS + NT - 1) / NT;
+  constexpr size_t STAGE = (size_t)NP * (BM + BN) * LDH;
+  extern __shared__ __attribute__((aligned(16))) unsigned char gemm_lds[];
+  uint16_t* lds = reinterpret_cast<uint16_t*>(gemm_lds);
+  auto As = [&](int st, int p, int r) { return lds + st * STAGE + ((size_t)p * BM + r) * LDH; };
+  auto Bs = [&](int st, int p, int r) { return lds + st * STAGE + (size_t)NP * BM * LDH + ((size_t)p * BN + r) * LDH; };
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  int64_t mt;
+  int nt;
+  tile_of(mt, nt, nT);
+  if (dr.offset) {
+    const int64_t o = *dr.offset;
+    A += o * lda;
+    C += o * ldc;
+    if (rowflag) rowflag += o;
+  }
+  if (dr.count) M = M < (int64_t)*dr.count ? M : (int64_t)*dr.count;
+  if (mt * BM >= M) return;
+  const int64_t m0 = mt * BM;
+  const int n0 = nt * BN;
+  const int wm = (wid / NWN) * 64, wn = (wid % NWN) * 128;
+  float16_t acc[2][4];
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 4; ++b)
+      for (int v = 0; v < 16; ++v) acc[a][b][v] = 0.f;
+  const uint16_t* Wpl[3] = {Wp0, Wp1, Wp2};
+  float4 ra[PA][A_PER];
+  uint4 rb[NP][B_PER];
+  auto load_a = [&](int c, float4(&xa)[A_PER]) {
+#pragma unroll
+    for (int q = 0; q < A_PER; ++q) {
+      const int e = tid + q * NT;
+      const int rr = e / (KC / 4), kq = (e % (KC / 4)) * 4;
+      const int64_t gr = m0 + rr;
+      xa[q] = (e < A_UNITS && gr < M) ? *reinterpret_cast<const float4*>(A + gr * lda + c * KC + kq)
+                                      : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+  };
+  auto load_b = [&](int c) {
+#pragma unroll
+    for (int q = 0; q < B_PER; ++q) {
+      const int e = tid + q * NT;
+      const int nn = e / (KC / 8), kq = (e % (KC / 8)) * 8;
+      const int gn = n0 + nn;
+#pragma unroll
+      for (int p = 0; p < NP; ++p)
+        rb[p][q] = (e < B_UNITS && gn < N) ? *reinterpret_cast<const uint4*>(Wpl[p] + (int64_t)gn * K + c * KC + kq)
+                                           : make_uint4(0, 0, 0, 0);
+    }
+  };
+  auto store = [&](int st, const float4(&xa)[A_PER]) {
+#pragma unroll
+    for (int q = 0; q < A_PER; ++q) {
+      const int e = tid + q * NT;
+      if (A_UNITS % NT != 0 && e >= A_UNITS) continue;
+      const int rr = e / (KC / 4), kq = (e % (KC / 4)) * 4;
+      float2v_t x01 = {xa[q].x, xa[q].y}, x23 = {xa[q].z, xa[q].w};
+#pragma unroll
+      for (int p = 0; p < NP; ++p) {
+        const bf16x2_t h01 = __builtin_convertvector(x01, bf16x2_t);
+        const bf16x2_t h23 = __builtin_convertvector(x23, bf16x2_t);
+        *reinterpret_cast<uint2*>(As(st, p, rr) + kq) =
+            make_uint2(__builtin_bit_cast(uint32_t, h01), __builtin_bit_cast(uint32_t, h23));
+        if (p + 1 < NP) {
+          x01 = x01 - __builtin_convertvector(h01, float2v_t);
+          x23 = x23 - __builtin_convertvector(h23, float2v_t);
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < B_PER; ++q) {
+      const int e = tid + q * NT;
+      if (B_UNITS % NT != 0 && e >= B_UNITS) continue;
+      const int nn = e / (KC / 8), kq = (e % (KC / 8)) * 8;
+#pragma unroll
+      for (int p = 0; p < NP; ++p) *reinterpret_cast<uint4*>(Bs(st, p, nn) + kq) = rb[p][q];
+    }
+  };
+  auto compute = [&](int st) {
+#pragma unroll
+    for (int kk = 0; kk < KC; kk += 16) {
+      const int ka = kk + 8 * (lane >> 5);
+      bf16x8_t av[NP][2];
+#pragma unroll
+      for (int p = 0; p < NP; ++p)
+#pragma unroll
+        for (int a = 0; a < 2; ++a) av[p][a] = *reinterpret_cast<const bf16x8_t*>(As(st, p, wm + a * 32 + (lane & 31)) + ka);
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        bf16x8_t bv[NP];
+#pragma unroll
+        for (int p = 0; p < NP; ++p) bv[p] = *reinterpret_cast<const bf16x8_t*>(Bs(st, p, wn + b * 32 + (lane & 31)) + ka);
+#pragma unroll
+        for (int t = 0; t < n_products(NP); ++t)
+#pragma unroll
+          for (int a = 0; a < 2; ++a)
+            acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[prod_a(NP, t)][a], bv[prod_b(NP, t)], acc[a][b], 0,
+                                                                0, 0);
+      }
+    }
+  };
+  const int nch = K / KC;
+  // chunk 0 into stage 0; chunks 1 .. PA in the ring (chunk j in ra[j % PA]); W chunk 1 in rb
+  load_a(0, ra[0]);
+  load_b(0);
+  store(0, ra[0]);
+#pragma unroll
+  for (int j = 1; j <= PA; ++j)
+    if (j < nch) load_a(j, ra[j % PA]);
+  if (nch > 1) load_b(1);
+  __syncthreads();
+  for (int c = 0; c < nch; c += PA) {
+#pragma unroll
+    for (int q = 0; q < PA; ++q) {
+      const int cc = c + q;  // chunk in stage cc & 1; chunk cc + 1 waits in ra[(q + 1) % PA]
+      if (cc < nch) {
+        compute(cc & 1);
+        if (cc + 1 < nch) {
+          store((cc + 1) & 1, ra[(q + 1) % PA]);
+          if (cc + 1 + PA < nch) load_a(cc + 1 + PA, ra[(q + 1) % PA]);
+          if (cc + 2 < nch) load_b(cc + 2);
+        }
+        __syncthreads();
+      }
+    }
+  }
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    const int col = n0 + wn + b * 32 + (lane & 31);
+    if (col >= N) continue;
+    const float bcol = bias ? bias[col] : 0.f;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const int64_t row = m0 + wm + a * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+        if (row >= M) continue;
+        float val = acc[a][b][v];
+        if (bias && is_bias_row(row, rowflag, period)) val += bcol;
+        C[row * ldc + col] = val;
+      }
+  }
+}
+
 // A-in-registers form: each wavefront computes a 64 x WN tile; it loads its A fragments
 // straight from memory into MFMA operand registers (lane l: row l % 32 of a 32-row tile,
 // k = 8 (l / 32) .. +7 of each 16-k step), two chunks ahead, and splits them into the NP planes
@@ -944,11 +1106,13 @@ struct Bf16Tile {
 };
 // variants 0-6: single LDS stage, K chunk 32; variants 7-10: two LDS stages (kc = 16 or 32);
 // variants 11-14: two LDS stages with loads two chunks ahead; variants 15-17: A in registers;
-// variants 18-22: two LDS stages with A PA chunks ahead
+// variants 18-22: two LDS stages with A PA chunks ahead; variants 23-25: the same with 64 x 128
+// wavefront tiles
 static const Bf16Tile kBf16Tiles[] = {{128, 128}, {128, 256}, {256, 128}, {64, 256}, {256, 64},  {64, 128},
                                       {128, 64},  {128, 128}, {128, 128}, {256, 128}, {128, 256}, {256, 128},
                                       {128, 256}, {128, 128}, {128, 128}, {128, 256}, {256, 128}, {64, 256},
-                                      {128, 256}, {128, 256}, {256, 128}, {128, 256}, {128, 128}};
+                                      {128, 256}, {128, 256}, {256, 128}, {128, 256}, {128, 128}, {256, 256},
+                                      {128, 256}, {256, 128}};
 
 int gemm_bf16_variants() { return (int)(sizeof(kBf16Tiles) / sizeof(kBf16Tiles[0])); }
 void gemm_bf16_tile(int variant, int* bm, int* bn) {
@@ -1025,6 +1189,28 @@ static void launch_tile_dq(hipStream_t s, const float* A, int64_t lda, const uin
                                                                                ldc, M, N, K, rowflag, period, nT, dr);
 }
 
+template <int NP, int BM, int BN, int KC, int PA>
+static void launch_tile_dw(hipStream_t s, const float* A, int64_t lda, const uint16_t* const* W, const float* bias,
+                           float* C, int64_t ldc, int64_t M, int N, int K, const uint8_t* rowflag, int period,
+                           DevRows dr) {
+  static bool attr_set = false;
+  const size_t lds = (size_t)2 * NP * (BM + BN) * (KC + 8) * 2;
+  if (lds > kMaxLds) {
+    launch_tile<NP, 128, 128>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr);
+    return;
+  }
+  if (!attr_set) {
+    if (lds > 64 * 1024)
+      (void)hipFuncSetAttribute((const void*)k_gemm_bf16_dw<NP, BM, BN, KC, PA>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    attr_set = true;
+  }
+  const int nT = (N + BN - 1) / BN;
+  const int64_t blocks = ((M + BM - 1) / BM) * nT;
+  k_gemm_bf16_dw<NP, BM, BN, KC, PA><<<(unsigned)blocks, BM * BN / 128, lds, s>>>(A, lda, W[0], W[1], W[2], bias, C,
+                                                                                ldc, M, N, K, rowflag, period, nT, dr);
+}
+
 template <int NP, int NWM, int NWN, int WN>
 static void launch_tile_ar(hipStream_t s, const float* A, int64_t lda, const uint16_t* const* W, const float* bias,
                            float* C, int64_t ldc, int64_t M, int N, int K, const uint8_t* rowflag, int period,
@@ -1056,6 +1242,9 @@ static void launch_np(hipStream_t s, int variant, const float* A, int64_t lda, c
     case 20: launch_tile_dq<NP, 256, 128, 16, 4>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
     case 21: launch_tile_dq<NP, 128, 256, 32, 3>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
     case 22: launch_tile_dq<NP, 128, 128, 16, 6>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
+    case 23: launch_tile_dw<NP, 256, 256, 16, 4>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
+    case 24: launch_tile_dw<NP, 128, 256, 16, 4>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
+    case 25: launch_tile_dw<NP, 256, 128, 16, 4>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
     case 1: launch_tile<NP, 128, 256>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
     case 2: launch_tile<NP, 256, 128>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
     case 3: launch_tile<NP, 64, 256>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
