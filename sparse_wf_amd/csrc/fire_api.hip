@@ -91,7 +91,7 @@ struct fire_handle {
   int iEN, iEEs, iEEd, iMLP, iORB, iJM, iATT, iCUSP, iSCAL;
   // workspace
   DevBuf cnt, ptr, nbr, rev, rowflag;
-  DevBuf T5, Z2, T5b, M5, XA, XB, XC, O, ENV, ENVC;
+  DevBuf T5, Z2, T5b, M5, XA, XB, XC, O, ENV, ENVC, PF;
   DevBuf A64, Ainv, AinvT, ldet, dsgn, tr, trip, dgrad;
   DevBuf det_lp, det_sg, det_gr, det_lap;
   DevBuf JR, mlp_ns, mlp_dg, QV, AV, att_ns, att_dg, RIN, RY1, RY2, att_out;
@@ -437,8 +437,15 @@ void run_embedding(Ctx& c, const float* r, int B, const Graph& gr, bool tan, Gro
   }
   h->XA.ensure(sizeof(float) * (size_t)R * F);
   h->XB.ensure(sizeof(float) * (size_t)R * F);
+  float* pf = nullptr;
+  if (tan) {
+    h->PF.ensure(sizeof(float) * pair_filter_floats(edge_params(h, h->iEEs).filt.w1) *
+                 (size_t)std::max<int64_t>(gr.n_pairs, 1));
+    pf = h->PF.as<float>();
+  }
   launch_messages(c.s, r, B, N, h->n_up, h->ptr.as<int64_t>(), h->nbr.as<int32_t>(), h->M5.as<float>(),
-                  edge_params(h, h->iEEs), edge_params(h, h->iEEd), h->cut_learned, F, tan, h->XA.as<float>());
+                  edge_params(h, h->iEEs), edge_params(h, h->iEEd), h->cut_learned, F, tan, h->XA.as<float>(), nullptr,
+                  0, -1, nullptr, nullptr, pf);
   const uint8_t* flag = nullptr;
   int period = 1;
   if (tan) {

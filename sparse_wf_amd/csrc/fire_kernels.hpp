@@ -53,7 +53,9 @@ void launch_act(hipStream_t s, float* Y, int64_t ld, int ncols, int G, GroupDesc
 void launch_messages(hipStream_t s, const float* r, int B, int N, int n_up, const int64_t* ptr, const int32_t* nbr,
                      const float* M5, EdgeParams same, EdgeParams diff, float cut, int F, bool tangents, float* out,
                      const int2* list = nullptr, int n_list = 0, int kmove = -1, const float* M5k = nullptr,
-                     const int32_t* dn = nullptr);
+                     const int32_t* dn = nullptr, float* pf = nullptr);
+// floats per pair of the pair-filter buffer pf that launch_messages needs with tangent rows
+size_t pair_filter_floats(int w1);
 void launch_scale(hipStream_t s, float* p, int64_t n, float a);
 void launch_rowflag(hipStream_t s, int G, GroupDesc gd, int64_t n_rows, uint8_t* flag);
 

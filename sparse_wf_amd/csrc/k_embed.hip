@@ -262,17 +262,51 @@ void launch_scale(hipStream_t s, float* p, int64_t n, float a) { k_scale<<<1024,
 
 #define MSG_CHUNK 64
 
+// Forward-Laplacian pair filter of every pair (one thread per pair, workgroups per centre):
+// pf[p][row][0 .. W1) = beta, pf[p][row][W1 .. W1 + 4) = g for the 5 rows of pair p (CSR
+// order).  Kept out of the message kernel, whose register budget it would set.
+template <int W1>
+__global__ void __launch_bounds__(64) k_pair_filter(const float* __restrict__ r, int B, int N, int n_up,
+                                                    const int64_t* __restrict__ ptr, const int32_t* __restrict__ nbr,
+                                                    EdgeParams Ps, EdgeParams Pd, float cut, float* __restrict__ pf) {
+  constexpr int ST = 5 * (W1 + 4);
+  const int g = blockIdx.x;
+  int w, i;
+  group_inv(g, B, n_up, N - n_up, w, i);
+  const int64_t p0 = ptr[g];
+  const int nb = (int)(ptr[g + 1] - p0);
+  const float* rw = r + (size_t)w * N * 3;
+  const bool up_i = i < n_up;
+  for (int k = threadIdx.x; k < nb; k += blockDim.x) {
+    const int j = nbr[p0 + k];
+    const float diff[3] = {rw[3 * i] - rw[3 * j], rw[3 * i + 1] - rw[3 * j + 1], rw[3 * i + 2] - rw[3 * j + 2]};
+    const bool same = up_i == (j < n_up);
+    float beta[5][W1], gg[5][4];
+    pair_filter_fl<W1, 5>(diff, same ? Ps.filt : Pd.filt, cut, beta, gg);
+    float* o = pf + (p0 + k) * ST;
+#pragma unroll
+    for (int rr = 0; rr < 5; ++rr) {
+#pragma unroll
+      for (int q = 0; q < W1; ++q) o[rr * (W1 + 4) + q] = beta[rr][q];
+#pragma unroll
+      for (int m = 0; m < 4; ++m) o[rr * (W1 + 4) + W1 + m] = gg[rr][m];
+    }
+  }
+}
+
 // Message passing h_i = h0_i + sum_j Gamma_ij * silu(e_ij + m_ct[i] + m_nb[j]) with
 // forward-Laplacian rows; one workgroup (blockDim = F) per centre electron.  The pair
-// filter quantities of up to MSG_CHUNK neighbours are computed by one thread each and
-// shared through LDS; each thread then owns one feature column.
+// filter quantities of up to MSG_CHUNK neighbours are staged in LDS (with tangent rows: read
+// from pf, written by k_pair_filter; value rows: computed here by one thread per pair); each
+// thread then owns one feature column.
 template <bool TAN, int W1>
 __global__ void __launch_bounds__(256) k_messages(const float* __restrict__ r, int B, int N, int n_up,
                                                    const int64_t* __restrict__ ptr, const int32_t* __restrict__ nbr,
                                                    const float* __restrict__ M5, EdgeParams Ps, EdgeParams Pd,
                                                    float cut, int F, float* __restrict__ out,
                                                    const int2* __restrict__ list, int kmove,
-                                                   const float* __restrict__ M5k, const int32_t* __restrict__ dn) {
+                                                   const float* __restrict__ M5k, const int32_t* __restrict__ dn,
+                                                   const float* __restrict__ pf) {
   // list mode (value rows only): centres are list[g] = (walker, electron); the h0/message
   // rows of electron `kmove` of every walker come from M5k[walker] instead of M5; dn
   // (optional, device) is the number of list entries in use.
@@ -321,7 +355,23 @@ __global__ void __launch_bounds__(256) k_messages(const float* __restrict__ r, i
   }
   for (int c0 = 0; c0 < nb; c0 += MSG_CHUNK) {
     const int cn = min(MSG_CHUNK, nb - c0);
-    if (threadIdx.x < cn) {
+    if constexpr (TAN) {
+      constexpr int ST = 5 * (W1 + 4);
+      const float* src = pf + (p0 + c0) * ST;
+      for (int e = threadIdx.x; e < cn * ST; e += blockDim.x) {
+        const int k = e / ST, rem = e - k * ST, rr = rem / (W1 + 4), q = rem - rr * (W1 + 4);
+        const float v = src[e];
+        if (q < W1)
+          s_beta[k][rr][q] = v;
+        else
+          s_g[k][rr][q - W1] = v;
+      }
+      if (threadIdx.x < cn) {
+        const int j = nbr[p0 + c0 + threadIdx.x];
+        s_src[threadIdx.x] = m5_of(j);
+        s_same[threadIdx.x] = up_i == (j < n_up);
+      }
+    } else if (threadIdx.x < cn) {
       const int k = c0 + threadIdx.x;
       const int j = nbr[p0 + k];
       const float diff[3] = {rw[3 * i] - rw[3 * j], rw[3 * i + 1] - rw[3 * j + 1], rw[3 * i + 2] - rw[3 * j + 2]};
@@ -447,16 +497,22 @@ __global__ void __launch_bounds__(256) k_messages(const float* __restrict__ r, i
 
 void launch_messages(hipStream_t s, const float* r, int B, int N, int n_up, const int64_t* ptr, const int32_t* nbr,
                      const float* M5, EdgeParams same, EdgeParams diff, float cut, int F, bool tangents, float* out,
-                     const int2* list, int n_list, int kmove, const float* M5k, const int32_t* dn) {
+                     const int2* list, int n_list, int kmove, const float* M5k, const int32_t* dn, float* pf) {
   const int G = list ? n_list : B * N;
   if (G == 0) return;
   if (tangents) {
-    W1_DISPATCH(same.filt.w1, k_messages<true, W1><<<G, F, 0, s>>>(r, B, N, n_up, ptr, nbr, M5, same, diff, cut, F,
-                                                                   out, list, kmove, M5k, dn));
+    W1_DISPATCH(same.filt.w1, {
+      k_pair_filter<W1><<<G, 64, 0, s>>>(r, B, N, n_up, ptr, nbr, same, diff, cut, pf);
+      k_messages<true, W1><<<G, F, 0, s>>>(r, B, N, n_up, ptr, nbr, M5, same, diff, cut, F, out, list, kmove, M5k,
+                                           dn, pf);
+    });
   } else {
     W1_DISPATCH(same.filt.w1, k_messages<false, W1><<<G, F, 0, s>>>(r, B, N, n_up, ptr, nbr, M5, same, diff, cut, F,
-                                                                    out, list, kmove, M5k, dn));
+                                                                    out, list, kmove, M5k, dn, nullptr));
   }
 }
+
+// floats per pair of the k_pair_filter output, for the width W1_DISPATCH picks
+size_t pair_filter_floats(int w1) { return (size_t)5 * ((w1 == 4 ? 4 : w1 == 8 ? 8 : 16) + 4); }
 
 }  // namespace fire
