@@ -91,7 +91,7 @@ struct fire_handle {
   int iEN, iEEs, iEEd, iMLP, iORB, iJM, iATT, iCUSP, iSCAL;
   // workspace
   DevBuf cnt, ptr, nbr, rev, rowflag;
-  DevBuf T5, Z2, T5b, M5, XA, XB, XC, O, ENV, ENVC, PF;
+  DevBuf T5, Z2, T5b, M5, XA, XB, XC, O, ENV, ENVC, PF, EF, EOK;
   DevBuf A64, Ainv, AinvT, ldet, dsgn, tr, trip, dgrad;
   DevBuf det_lp, det_sg, det_gr, det_lap;
   DevBuf JR, mlp_ns, mlp_dg, QV, AV, att_ns, att_dg, RIN, RY1, RY2, att_out;
@@ -419,7 +419,17 @@ void run_embedding(Ctx& c, const float* r, int B, const Graph& gr, bool tan, Gro
   h->Z2.ensure(sizeof(float) * (size_t)G * NR * 2 * F);
   h->M5.ensure(sizeof(float) * (size_t)G * NR * 5 * F);
   const int b = h->iEN;
-  launch_elec_init_en(c.s, r, h->R, B, N, h->n_up, h->M, h->cfg.cutoff_1el, F, en_params(h), tan, h->T5.as<float>());
+  float* ef = nullptr;
+  uint8_t* eok = nullptr;
+  if (tan) {
+    const EnParams ep = en_params(h);
+    h->EF.ensure(sizeof(float) * pair_filter_floats(ep.w1) * (size_t)G * h->M);
+    h->EOK.ensure((size_t)G * h->M);
+    ef = h->EF.as<float>();
+    eok = h->EOK.as<uint8_t>();
+  }
+  launch_elec_init_en(c.s, r, h->R, B, N, h->n_up, h->M, h->cfg.cutoff_1el, F, en_params(h), tan, h->T5.as<float>(),
+                      nullptr, 0, ef, eok);
   const int64_t rows5 = (int64_t)G * NR;
   gemm(c, h->T5.as<float>(), F, b + 11, -1, h->Z2.as<float>(), 2 * F, rows5, 2 * F, F, nullptr, 0);
   launch_glu(c.s, h->Z2.as<float>(), G, F, tan, h->T5b.as<float>());

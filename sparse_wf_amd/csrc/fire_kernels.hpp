@@ -42,8 +42,10 @@ void launch_rev(hipStream_t s, const int64_t* ptr, const int32_t* nbr, int B, in
 
 // embedding
 // list (optional): (walker, electron) pairs to process, output row = list position
+// with tangents: ef (G * M * pair_filter_floats(P.w1) floats) and eok (G * M bytes) are scratch
 void launch_elec_init_en(hipStream_t s, const float* r, const float* R, int B, int N, int n_up, int M, float cutoff_1el,
-                         int F, EnParams P, bool tangents, float* out, const int2* list = nullptr, int n_list = 0);
+                         int F, EnParams P, bool tangents, float* out, const int2* list = nullptr, int n_list = 0,
+                         float* ef = nullptr, uint8_t* eok = nullptr);
 void launch_glu(hipStream_t s, const float* Z, int G, int F, bool tangents, float* out);
 // dG (optional, device): groups actually in use (G is the launch capacity)
 void launch_act(hipStream_t s, float* Y, int64_t ld, int ncols, int G, GroupDesc gd, int act, float act_scale,

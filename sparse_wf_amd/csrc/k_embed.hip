@@ -14,12 +14,58 @@ namespace fire {
 
 #define EN_CHUNK 64
 
+// Forward-Laplacian e-n filter of every (electron, nucleus) pair, one thread per pair:
+// ef[g][J][row][0 .. W1) = beta, [W1 .. W1 + 4) = g for the 5 rows; eok[g][J] = nucleus J
+// within cutoff_1el of electron g.  Kept out of k_elec_init_en, whose register budget it
+// would set.
+template <int W1>
+__global__ void __launch_bounds__(64) k_en_filter(const float* __restrict__ r, const float* __restrict__ R, int B,
+                                                  int N, int n_up, int M, float cutoff_1el, EnParams P,
+                                                  float* __restrict__ ef, uint8_t* __restrict__ eok) {
+  constexpr int ST = 5 * (W1 + 4);
+  const int g = blockIdx.x;
+  int w, i;
+  group_inv(g, B, n_up, N - n_up, w, i);
+  const float* ri = r + ((size_t)w * N + i) * 3;
+  const float rx = ri[0], ry = ri[1], rz = ri[2];
+  for (int J = threadIdx.x; J < M; J += blockDim.x) {
+    const float diff[3] = {__fsub_rn(rx, R[3 * J]), __fsub_rn(ry, R[3 * J + 1]), __fsub_rn(rz, R[3 * J + 2])};
+    const float s2 = __fadd_rn(__fadd_rn(__fmul_rn(diff[0], diff[0]), __fmul_rn(diff[1], diff[1])),
+                               __fmul_rn(diff[2], diff[2]));
+    const bool ok = __fsqrt_rn(s2) < cutoff_1el;
+    eok[(size_t)g * M + J] = ok;
+    if (!ok) continue;
+    FilterParams fp;
+    fp.K = P.K + (size_t)J * 4 * P.w0;
+    fp.b = P.b + (size_t)J * P.w0;
+    fp.sp = P.sp + (size_t)J * P.E;
+    fp.Wd = P.Wd;
+    fp.bd = P.bd;
+    fp.We = P.We;
+    fp.w0 = P.w0;
+    fp.w1 = P.w1;
+    fp.E = P.E;
+    float beta[5][W1], gg[5][4];
+    pair_filter_fl<W1, 5>(diff, fp, cutoff_1el, beta, gg);
+    float* o = ef + ((size_t)g * M + J) * ST;
+#pragma unroll
+    for (int rr = 0; rr < 5; ++rr) {
+#pragma unroll
+      for (int k = 0; k < W1; ++k) o[rr * (W1 + 4) + k] = beta[rr][k];
+#pragma unroll
+      for (int m = 0; m < 4; ++m) o[rr * (W1 + 4) + W1 + m] = gg[rr][m];
+    }
+  }
+}
+
 // One workgroup (blockDim = F) per electron: sum over nuclei within cutoff_1el of
-// Gamma_J * e_J with forward-Laplacian rows, then LayerNorm with forward-Laplacian.
+// Gamma_J * e_J with forward-Laplacian rows, then LayerNorm with forward-Laplacian.  With
+// tangent rows the filter values come from k_en_filter (ef, eok); value rows compute them here.
 template <bool TAN, int W1>
 __global__ void __launch_bounds__(256) k_elec_init_en(const float* __restrict__ r, const float* __restrict__ R, int B,
                                                        int N, int n_up, int M, float cutoff_1el, int F, EnParams P,
-                                                       float* __restrict__ out, const int2* __restrict__ list) {
+                                                       float* __restrict__ out, const int2* __restrict__ list,
+                                                       const float* __restrict__ ef, const uint8_t* __restrict__ eok) {
   constexpr int NRF = TAN ? 5 : 1;  // forward-Laplacian rows staged per nucleus
   __shared__ float s_beta[EN_CHUNK][NRF][W1];
   __shared__ float s_g[EN_CHUNK][NRF][4];
@@ -45,6 +91,20 @@ __global__ void __launch_bounds__(256) k_elec_init_en(const float* __restrict__ 
 
   for (int J0 = 0; J0 < M; J0 += EN_CHUNK) {
     const int J = J0 + threadIdx.x;
+    if constexpr (TAN) {
+      constexpr int ST = 5 * (W1 + 4);
+      const int nvc = min(EN_CHUNK, M - J0);
+      const float* src = ef + ((size_t)g * M + J0) * ST;
+      for (int e = threadIdx.x; e < nvc * ST; e += blockDim.x) {
+        const int k = e / ST, rem = e - k * ST, rr = rem / (W1 + 4), q = rem - rr * (W1 + 4);
+        const float v = src[e];
+        if (q < W1)
+          s_beta[k][rr][q] = v;
+        else
+          s_g[k][rr][q - W1] = v;
+      }
+      if (threadIdx.x < EN_CHUNK) s_ok[threadIdx.x] = J < M ? eok[(size_t)g * M + J] : 0;
+    } else {
     if (threadIdx.x < EN_CHUNK) s_ok[threadIdx.x] = 0;
     if (threadIdx.x < EN_CHUNK && J < M) {
       const float diff[3] = {__fsub_rn(rx, R[3 * J]), __fsub_rn(ry, R[3 * J + 1]), __fsub_rn(rz, R[3 * J + 2])};
@@ -73,6 +133,7 @@ __global__ void __launch_bounds__(256) k_elec_init_en(const float* __restrict__ 
           for (int m = 0; m < 4; ++m) s_g[slot][rr][m] = gg[rr][m];
         }
       }
+    }
     }
     __syncthreads();
     // nuclei in ascending index order (the order of the reference's e-n list)
@@ -142,13 +203,18 @@ __global__ void __launch_bounds__(256) k_elec_init_en(const float* __restrict__ 
   }
 
 void launch_elec_init_en(hipStream_t s, const float* r, const float* R, int B, int N, int n_up, int M, float cutoff_1el,
-                         int F, EnParams P, bool tangents, float* out, const int2* list, int n_list) {
+                         int F, EnParams P, bool tangents, float* out, const int2* list, int n_list, float* ef,
+                         uint8_t* eok) {
   const int G = list ? n_list : B * N;
   if (G == 0) return;
-  if (tangents) {
-    W1_DISPATCH(P.w1, k_elec_init_en<true, W1><<<G, F, 0, s>>>(r, R, B, N, n_up, M, cutoff_1el, F, P, out, list));
+  if (tangents) {  // (no list: every electron of the batch)
+    W1_DISPATCH(P.w1, {
+      k_en_filter<W1><<<G, 64, 0, s>>>(r, R, B, N, n_up, M, cutoff_1el, P, ef, eok);
+      k_elec_init_en<true, W1><<<G, F, 0, s>>>(r, R, B, N, n_up, M, cutoff_1el, F, P, out, nullptr, ef, eok);
+    });
   } else {
-    W1_DISPATCH(P.w1, k_elec_init_en<false, W1><<<G, F, 0, s>>>(r, R, B, N, n_up, M, cutoff_1el, F, P, out, list));
+    W1_DISPATCH(P.w1, k_elec_init_en<false, W1><<<G, F, 0, s>>>(r, R, B, N, n_up, M, cutoff_1el, F, P, out, list,
+                                                                nullptr, nullptr));
   }
 }
 
