@@ -10,47 +10,85 @@
 namespace fire {
 
 // Last MLP-Jastrow layer (W -> 2), silu with forward-Laplacian, * mlp_scale + [0, log_bias].
-// JR gets two columns per tangent row of the group.
+// JR gets two columns per tangent row of the group.  The row dot products go to LDS (rows past
+// 5 + 3 (FIRE_SMAX - 1), which the neighbour limit excludes, to JR itself); the activation rule
+// then runs over the rows in parallel (128 threads per column) with the sum of squares of the
+// jacobian rows reduced in LDS.
+#define JM_MAXR (5 + 3 * (FIRE_SMAX - 1))
 __global__ void __launch_bounds__(256) k_jmlp_last(const float* __restrict__ Y, int W, GroupDesc gd,
                                                    const float* __restrict__ W3, const float* __restrict__ b3,
                                                    const float* __restrict__ scale, const float* __restrict__ zb,
                                                    float* __restrict__ JR, const int32_t* __restrict__ dG) {
+  __shared__ float s_a[2][JM_MAXR];
+  __shared__ float s_ss[2][2];
   const int g = blockIdx.x;
   if (dG && g >= *dG) return;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  const int64_t r0 = gd.row(g), r1 = gd.row(g + 1);
-  for (int64_t rr = r0 + wid; rr < r1; rr += blockDim.x >> 6) {
-    float a0 = 0.f, a1 = 0.f;
-    for (int k = lane; k < W; k += 64) {
-      const float y = Y[rr * W + k];
-      a0 += y * W3[2 * k];
-      a1 += y * W3[2 * k + 1];
+  const int64_t r0 = gd.row(g);
+  const int nr = (int)(gd.row(g + 1) - r0);
+  auto put = [&](int rr, float a0, float a1) {
+    if (rr == 0) { a0 += b3[0]; a1 += b3[1]; }
+    if (rr < JM_MAXR) {
+      s_a[0][rr] = a0;
+      s_a[1][rr] = a1;
+    } else {
+      JR[(r0 + rr) * 2] = a0;
+      JR[(r0 + rr) * 2 + 1] = a1;
     }
-    a0 = warp_sum(a0);
-    a1 = warp_sum(a1);
-    if (lane == 0) {
-      if (rr == r0) { a0 += b3[0]; a1 += b3[1]; }
-      JR[rr * 2] = a0;
-      JR[rr * 2 + 1] = a1;
+  };
+  if (W % 64 == 0) {
+    // four rows per wavefront, 16 lanes per row with float4 loads (W % 4 == 0 keeps rows aligned)
+    const int sub = lane >> 4, l16 = lane & 15;
+    for (int rb = wid * 4; rb < nr; rb += (blockDim.x >> 6) * 4) {
+      const int rr = rb + sub;
+      float a0 = 0.f, a1 = 0.f;
+      if (rr < nr) {
+        const float* y = Y + (r0 + rr) * W;
+        for (int k = 4 * l16; k < W; k += 64) {
+          const float4 v = *reinterpret_cast<const float4*>(y + k);
+          const float4 wa = *reinterpret_cast<const float4*>(W3 + 2 * k);
+          const float4 wb = *reinterpret_cast<const float4*>(W3 + 2 * k + 4);
+          a0 += v.x * wa.x + v.y * wa.z + v.z * wb.x + v.w * wb.z;
+          a1 += v.x * wa.y + v.y * wa.w + v.z * wb.y + v.w * wb.w;
+        }
+      }
+#pragma unroll
+      for (int o = 8; o > 0; o >>= 1) {
+        a0 += __shfl_xor(a0, o, 16);
+        a1 += __shfl_xor(a1, o, 16);
+      }
+      if (l16 == 0 && rr < nr) put(rr, a0, a1);
+    }
+  } else {
+    for (int rr = wid; rr < nr; rr += blockDim.x >> 6) {
+      float a0 = 0.f, a1 = 0.f;
+      for (int k = lane; k < W; k += 64) {
+        const float y = Y[(r0 + rr) * W + k];
+        a0 += y * W3[2 * k];
+        a1 += y * W3[2 * k + 1];
+      }
+      a0 = warp_sum(a0);
+      a1 = warp_sum(a1);
+      if (lane == 0) put(rr, a0, a1);
     }
   }
   __syncthreads();
-  if (threadIdx.x < 2) {
-    const int c = threadIdx.x;
-    const float x = JR[r0 * 2 + c];
-    float f, f1, f2;
-    silu3(x, f, f1, f2);
-    const float sc = scale[c];
+  const int c = threadIdx.x >> 7, t = threadIdx.x & 127;  // column, thread within the column
+  float f, f1, f2;
+  silu3(s_a[c][0], f, f1, f2);
+  const float sc = scale[c];
+  float ss = 0.f;
+  for (int rr = 2 + t; rr < nr; rr += 128) {
+    const float v = rr < JM_MAXR ? s_a[c][rr] : JR[(r0 + rr) * 2 + c];
+    ss += v * v;
+    JR[(r0 + rr) * 2 + c] = v * f1 * sc;
+  }
+  ss = warp_sum(ss);
+  if (lane == 0) s_ss[c][wid & 1] = ss;
+  __syncthreads();
+  if (t == 0) {
     JR[r0 * 2 + c] = f * sc + zb[c];
-    if (r1 - r0 > 1) {
-      float ss = 0.f;
-      for (int64_t rr = r0 + 2; rr < r1; ++rr) {
-        const float v = JR[rr * 2 + c];
-        ss += v * v;
-        JR[rr * 2 + c] = v * f1 * sc;
-      }
-      JR[(r0 + 1) * 2 + c] = (JR[(r0 + 1) * 2 + c] * f1 + f2 * ss) * sc;
-    }
+    if (nr > 1) JR[(r0 + 1) * 2 + c] = (s_a[c][1] * f1 + f2 * (s_ss[c][0] + s_ss[c][1])) * sc;
   }
 }
 
