@@ -487,8 +487,14 @@ void run_orbitals_det(Ctx& c, const float* r, int B, const Graph& gr, bool tan, 
   fire_handle* h = c.h;
   const int N = h->N, F = h->F, G = B * N, ND = h->ND, D = h->D;
   const int NR = tan ? 5 : 1;
-  const size_t ocols = std::max(ND, h->has_att ? 2 * h->HD : 0);
+  // orbital rows at a stride of a multiple of 32 floats (128-byte lines); the padding columns
+  // are read (against zero inverse columns) by the determinant Laplacian's vector loads and are
+  // zeroed whenever the buffer is allocated (every later writer stores finite values)
+  const int ldo = (ND + 31) & ~31;
+  const size_t ocols = std::max(ldo, h->has_att ? 2 * h->HD : 0);
+  const void* o_prev = h->O.p;
   h->O.ensure(sizeof(float) * (size_t)R * ocols);
+  if (h->O.p != o_prev) launch_zero_f32(c.s, h->O.as<float>(), (int64_t)(h->O.bytes / sizeof(float)));
   h->ENV.ensure(sizeof(float) * (size_t)G * NR * ND);
   const int o = h->iORB;
   const int64_t R_up = gd.a * (int64_t)B * h->n_up;  // rows of up-spin groups (value part)
@@ -503,11 +509,13 @@ void run_orbitals_det(Ctx& c, const float* r, int B, const Graph& gr, bool tan, 
   const uint8_t* flag = tan ? h->rowflag.as<uint8_t>() : nullptr;
   const int period = tan ? 0 : 1;
   float* Ob = h->O.as<float>();
-  gemm(c, h->XA.as<float>(), F, o + 0, o + 1, Ob, ND, R_up_rows, ND, F, flag, period);
+  gemm(c, h->XA.as<float>(), F, o + 0, o + 1, Ob, ldo, R_up_rows, ND, F, flag, period);
   if (R > R_up_rows)
-    gemm(c, h->XA.as<float>() + (size_t)R_up_rows * F, F, o + 2, o + 3, Ob + (size_t)R_up_rows * ND, ND,
+    gemm(c, h->XA.as<float>() + (size_t)R_up_rows * F, F, o + 2, o + 3, Ob + (size_t)R_up_rows * ldo, ldo,
          R - R_up_rows, ND, F, flag ? flag + R_up_rows : nullptr, period);
-  if (keep) HIPCHK(hipMemcpyAsync(keep->orb, Ob, sizeof(float) * (size_t)R * ND, hipMemcpyDeviceToDevice, c.s));
+  if (keep)
+    HIPCHK(hipMemcpy2DAsync(keep->orb, sizeof(float) * ND, Ob, sizeof(float) * ldo, sizeof(float) * ND, (size_t)R,
+                            hipMemcpyDeviceToDevice, c.s));
   // envelopes: coefficients per (electron, row, nucleus, envelope), then one FP32 GEMM per
   // (spin block, determinant) against pi stored as [det][nucleus * envelope][orbital]
   {
@@ -526,12 +534,12 @@ void run_orbitals_det(Ctx& c, const float* r, int B, const Graph& gr, bool tan, 
   }
   if (keep)
     HIPCHK(hipMemcpyAsync(keep->env, h->ENV.p, sizeof(float) * (size_t)G * ND, hipMemcpyDeviceToDevice, c.s));
-  launch_orb_product(c.s, Ob, ND, G, gd, h->ENV.as<float>(), tan);
+  launch_orb_product(c.s, Ob, ldo, ND, G, gd, h->ENV.as<float>(), tan);
   const size_t nmat = (size_t)B * D;
   h->A64.ensure(sizeof(double) * nmat * N * N);
   h->ldet.ensure(sizeof(double) * nmat);
   h->dsgn.ensure(sizeof(double) * nmat);
-  launch_gather_A(c.s, Ob, ND, gd, B, N, h->n_up, D, h->A64.as<double>());
+  launch_gather_A(c.s, Ob, ldo, ND, gd, B, N, h->n_up, D, h->A64.as<double>());
   if (keep)
     HIPCHK(hipMemcpyAsync(keep->A, h->A64.p, sizeof(double) * nmat * N * N, hipMemcpyDeviceToDevice, c.s));
   h->det_lp.ensure(sizeof(double) * B);
@@ -553,10 +561,10 @@ void run_orbitals_det(Ctx& c, const float* r, int B, const Graph& gr, bool tan, 
     h->AinvT.ensure(sizeof(float) * nmat * N * N);
     launch_lu_inv(c.s, h->A64.as<double>(), (int)nmat, N, h->Ainv.as<double>(), h->AinvT.as<float>(),
                   h->ldet.as<double>(), h->dsgn.as<double>(), true, lu_lds);
-    launch_det_trace(c.s, Ob, ND, gd, B, N, h->n_up, D, h->Ainv.as<double>(), h->tr.as<double>());
+    launch_det_trace(c.s, Ob, ldo, gd, B, N, h->n_up, D, h->Ainv.as<double>(), h->tr.as<double>());
     const int smax = gr.max_nb + 1;
     REQUIRE(smax <= FIRE_SMAX, "an electron has more than 63 neighbours within the cutoff (determinant Laplacian limit)");
-    launch_det_lap(c.s, Ob, ND, gd, h->ENV.as<float>(), h->ptr.as<int64_t>(), h->nbr.as<int32_t>(),
+    launch_det_lap(c.s, Ob, ldo, ND, gd, h->ENV.as<float>(), h->ptr.as<int64_t>(), h->nbr.as<int32_t>(),
                    h->rev.as<int32_t>(), B, N, h->n_up, D, smax, h->AinvT.as<float>(), h->trip.as<double>(),
                    h->dgrad.as<double>());
     out.grad = h->det_gr.as<double>();

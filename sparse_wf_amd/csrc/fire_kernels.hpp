@@ -59,6 +59,7 @@ void launch_messages(hipStream_t s, const float* r, int B, int N, int n_up, cons
 // floats per pair of the pair-filter buffer pf that launch_messages needs with tangent rows
 size_t pair_filter_floats(int w1);
 void launch_scale(hipStream_t s, float* p, int64_t n, float a);
+void launch_zero_f32(hipStream_t s, float* p, int64_t n);
 void launch_rowflag(hipStream_t s, int G, GroupDesc gd, int64_t n_rows, uint8_t* flag);
 
 // Optional device-side row range of a contraction (for launches sized by a capacity, e.g. under

@@ -11,15 +11,16 @@ namespace fire {
 void launch_env_coef(hipStream_t s, const float* r, const float* R, int B, int N, int n_up, int M, int D, int E,
                      const float* sp_up, const float* sp_dn, bool tangents, float* coef, const int2* list = nullptr,
                      int n_list = 0);
-void launch_orb_product(hipStream_t s, float* O, int ND, int G, GroupDesc gd, const float* env, bool tangents);
-void launch_gather_A(hipStream_t s, const float* O, int ND, GroupDesc gd, int B, int N, int n_up, int D, double* A);
+void launch_orb_product(hipStream_t s, float* O, int ldo, int ND, int G, GroupDesc gd, const float* env, bool tangents);
+void launch_gather_A(hipStream_t s, const float* O, int ldo, int ND, GroupDesc gd, int B, int N, int n_up, int D,
+                     double* A);
 void launch_lu_inv(hipStream_t s, double* A, int n_mat, int N, double* Ainv, float* AinvT32, double* logdet,
                    double* sign, bool want_inv, bool use_lds);
 size_t lu_lds_bytes(int N);
 hipError_t lu_set_smem_limit(size_t bytes);
-void launch_det_trace(hipStream_t s, const float* O, int ND, GroupDesc gd, int B, int N, int n_up, int D,
+void launch_det_trace(hipStream_t s, const float* O, int ldo, GroupDesc gd, int B, int N, int n_up, int D,
                       const double* Ainv, double* tr);
-void launch_det_lap(hipStream_t s, const float* O, int ND, GroupDesc gd, const float* env, const int64_t* ptr,
+void launch_det_lap(hipStream_t s, const float* O, int ldo, int ND, GroupDesc gd, const float* env, const int64_t* ptr,
                     const int32_t* nbr, const int32_t* rev, int B, int N, int n_up, int D, int smax,
                     const float* AinvT, double* trip, double* grad);
 void launch_det_combine(hipStream_t s, int B, int N, int D, const double* logdet, const double* sgn, const double* tr,

@@ -311,6 +311,14 @@ __global__ void k_rowflag(int G, GroupDesc gd, uint8_t* flag) {
   if (g < G) flag[gd.row(g)] = 1;
 }
 
+__global__ void k_zero_f32(float* p, int64_t n) {
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) p[k] = 0.f;
+}
+
+void launch_zero_f32(hipStream_t s, float* p, int64_t n) {
+  if (n > 0) k_zero_f32<<<2048, 256, 0, s>>>(p, n);
+}
+
 __global__ void k_zero_u8(uint8_t* p, int64_t n) {
   for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) p[k] = 0;
 }

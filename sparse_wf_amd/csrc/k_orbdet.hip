@@ -67,49 +67,51 @@ void launch_env_coef(hipStream_t s, const float* r, const float* R, int B, int N
 // (j != i) becomes E_i d O_i / d r_j; that scale is applied where the rows are consumed
 // (k_det_lap stages them through LDS), so the neighbour rows are not rewritten here.
 template <bool TAN>
-__global__ void k_orb_product(float* __restrict__ O, int ND, GroupDesc gd, const float* __restrict__ env) {
+__global__ void k_orb_product(float* __restrict__ O, int ldo, int ND, GroupDesc gd, const float* __restrict__ env) {
   const int g = blockIdx.x;
   const int c = blockIdx.y * blockDim.x + threadIdx.x;
   if (c >= ND) return;
   const int NR = TAN ? 5 : 1;
   const float* e = env + (size_t)g * NR * ND + c;
-  float* o = O + gd.row(g) * ND + c;
+  float* o = O + gd.row(g) * ldo + c;
   const float ox = o[0], ex = e[0];
   o[0] = ox * ex;
   if (!TAN) return;
-  const float ol = o[ND], el = e[ND];
+  const float ol = o[ldo], el = e[ND];
   float cross = 0.f;
   for (int t = 0; t < 3; ++t) {
-    const float ot = o[(2 + t) * ND], et = e[(2 + t) * ND];
+    const float ot = o[(2 + t) * (int64_t)ldo], et = e[(2 + t) * ND];
     cross += ot * et;
-    o[(2 + t) * (int64_t)ND] = ox * et + ex * ot;
+    o[(2 + t) * (int64_t)ldo] = ox * et + ex * ot;
   }
-  o[ND] = ol * ex + el * ox + 2.f * cross;
+  o[ldo] = ol * ex + el * ox + 2.f * cross;
 }
 
-void launch_orb_product(hipStream_t s, float* O, int ND, int G, GroupDesc gd, const float* env, bool tangents) {
+void launch_orb_product(hipStream_t s, float* O, int ldo, int ND, int G, GroupDesc gd, const float* env,
+                        bool tangents) {
   dim3 grid(G, (ND + 255) / 256);
   if (tangents)
-    k_orb_product<true><<<grid, 256, 0, s>>>(O, ND, gd, env);
+    k_orb_product<true><<<grid, 256, 0, s>>>(O, ldo, ND, gd, env);
   else
-    k_orb_product<false><<<grid, 256, 0, s>>>(O, ND, gd, env);
+    k_orb_product<false><<<grid, 256, 0, s>>>(O, ldo, ND, gd, env);
 }
 
 // A[w,d][i][o] (FP64) from the value rows of the orbital groups.
-__global__ void k_gather_A(const float* __restrict__ O, int ND, GroupDesc gd, int B, int N, int n_up, int D,
-                           double* __restrict__ A) {
+__global__ void k_gather_A(const float* __restrict__ O, int ldo, int ND, GroupDesc gd, int B, int N, int n_up,
+                           int D, double* __restrict__ A) {
   const int g = blockIdx.x;
   int w, i;
   group_inv(g, B, n_up, N - n_up, w, i);
-  const float* o = O + gd.row(g) * ND;
+  const float* o = O + gd.row(g) * ldo;
   for (int c = threadIdx.x; c < ND; c += blockDim.x) {
     const int d = c / N, col = c - d * N;
     A[(((size_t)w * D + d) * N + i) * N + col] = (double)o[c];
   }
 }
 
-void launch_gather_A(hipStream_t s, const float* O, int ND, GroupDesc gd, int B, int N, int n_up, int D, double* A) {
-  k_gather_A<<<B * N, 256, 0, s>>>(O, ND, gd, B, N, n_up, D, A);
+void launch_gather_A(hipStream_t s, const float* O, int ldo, int ND, GroupDesc gd, int B, int N, int n_up, int D,
+                     double* A) {
+  k_gather_A<<<B * N, 256, 0, s>>>(O, ldo, ND, gd, B, N, n_up, D, A);
 }
 
 // LU with partial pivoting (first maximum wins), log|det|, sign and the inverse, FP64.
@@ -636,7 +638,7 @@ void launch_lu_inv(hipStream_t s, double* A, int n_mat, int N, double* Ainv, flo
 }
 
 // tr(Ainv * Lap A) for every (walker, det): sum_i sum_o lap_i[d,o] Ainv[o,i].
-__global__ void k_det_trace(const float* __restrict__ O, int ND, GroupDesc gd, int B, int N, int n_up, int D,
+__global__ void k_det_trace(const float* __restrict__ O, int ldo, GroupDesc gd, int B, int N, int n_up, int D,
                             const double* __restrict__ Ainv, double* __restrict__ tr) {
   __shared__ double red[16];
   const int wd = blockIdx.x;
@@ -646,16 +648,16 @@ __global__ void k_det_trace(const float* __restrict__ O, int ND, GroupDesc gd, i
   for (int e = threadIdx.x; e < N * N; e += blockDim.x) {
     const int i = e / N, o = e - i * N;
     const int g = group_of(w, i, B, n_up, N - n_up);
-    acc += (double)O[(gd.row(g) + 1) * ND + d * N + o] * X[(size_t)o * N + i];
+    acc += (double)O[(gd.row(g) + 1) * ldo + d * N + o] * X[(size_t)o * N + i];
   }
   double v[1] = {acc};
   block_sum<1>(v, red);
   if (threadIdx.x == 0) tr[wd] = v[0];
 }
 
-void launch_det_trace(hipStream_t s, const float* O, int ND, GroupDesc gd, int B, int N, int n_up, int D,
+void launch_det_trace(hipStream_t s, const float* O, int ldo, GroupDesc gd, int B, int N, int n_up, int D,
                       const double* Ainv, double* tr) {
-  k_det_trace<<<B * D, 256, 0, s>>>(O, ND, gd, B, N, n_up, D, Ainv, tr);
+  k_det_trace<<<B * D, 256, 0, s>>>(O, ldo, gd, B, N, n_up, D, Ainv, tr);
 }
 
 // Triplet part of the determinant Laplacian and the determinant gradient.  For electron n with
@@ -673,29 +675,40 @@ void launch_det_trace(hipStream_t s, const float* O, int ND, GroupDesc gd, int B
 // CU.  Members are padded to NAT tiles of 16 (padding rows read member 0's rows, padding
 // columns are zero), so B_x is an NAT x NAT grid of 16x16 tiles; XG values of x are computed
 // per pass over the columns.  The jacobian and the inverse rows (AinvT, the transposed
-// inverse) stream from memory straight into MFMA operand registers: in a 16-column chunk
-// starting at c0, lane l holds row l%16 and columns c0 + 4 (l/16) + t (t < 4), and MFMA step t
-// contracts exactly those columns.  That permutes the contraction index identically for both
-// operands, so the product is unchanged while every lane issues one vector load per row.  The
-// chunk frame starts at the slice start rounded down to 4 columns; the off = d N mod 4 columns
-// of the previous slice it takes in meet zero inverse columns.
+// inverse) stream from memory straight into MFMA operand registers: a chunk covers 16 U
+// columns starting at c0; lane l holds row l%16 and the columns c0 + 16 u + 4 (l/16) + t
+// (u < U, t < 4), and the MFMA step (u, t) contracts exactly those columns.  That permutes the
+// contraction index identically for both operands, so the product is unchanged while every
+// lane issues contiguous vector loads.  The chunk frame starts at the slice start rounded down
+// to 16 U columns (to 4 for the scalar form); the off = d N - start columns of the previous
+// slice it takes in meet zero inverse columns.  With a row stride ldo that is a multiple of 32
+// floats, a chunk is one aligned half (U = 1) or all (U = 2) of a 128-byte line of each row.
 // trip and grad come from the accumulators: tile (j,i) is transposed through 1.25 KB of LDS per
 // wavefront into the layout of tile (i,j), so no workgroup barrier is needed.
-// VEC: float4 jacobian/envelope loads and float2 inverse loads (N even, ND % 4 == 0); otherwise
-// scalar loads.
+// VEC: float4 jacobian/envelope loads and float2 inverse loads (N even, ldo and ND multiples
+// of 4); otherwise scalar loads.  Vector loads may run past the slice (into the next slice, or
+// for the last slice into the row padding, which the caller keeps finite) onto zero columns.
 typedef float float4_t __attribute__((ext_vector_type(4)));
 typedef float float2_t __attribute__((ext_vector_type(2)));
 
-template <int NAT, int XG, int P, bool VEC>
-__device__ __forceinline__ void det_lap_unit(const float* __restrict__ O, int ND, GroupDesc gd,
+// vector form: 16-column runs per lane and chunk (two runs, whole 128-byte lines, measured
+// slower: the registers cut the 2-tile class to 3 waves per SIMD); the 2-tile class: x values
+// per column pass and waves per SIMD (its register budget)
+#define DL_U 1
+#define DL_XG2 3
+#define DL_WPE2 4
+
+template <int NAT, int XG, int P, bool VEC, int U>
+__device__ __forceinline__ void det_lap_unit(const float* __restrict__ O, int ldo, int ND, GroupDesc gd,
                                              const float* __restrict__ env, const int32_t* __restrict__ nbr,
                                              const int32_t* __restrict__ rev, int B, int N, int n_up, int D,
                                              const float* __restrict__ AinvT, double* __restrict__ trip,
                                              double* __restrict__ grad, int w, int n, int d, int gn, int64_t p0,
                                              int s, float* tb) {
+  constexpr int CW = 16 * U;
   const int lane = threadIdx.x & 63, li = lane & 15, kg = lane >> 4;
-  const int cbase = (d * N) & ~3, off = d * N - cbase;
-  const int ncol = N + off, nch = (ncol + 15) / 16;
+  const int cbase = VEC ? (d * N) & ~(CW - 1) : (d * N) & ~3, off = d * N - cbase;
+  const int ncol = N + off, nch = (ncol + CW - 1) / CW;
   const float* jb[NAT];  // row x = 0 of member t*16 + li, at the chunk frame
   const float* eb[NAT];
   const float* xb[NAT];  // inverse row of member t*16 + li, at the chunk frame
@@ -712,47 +725,50 @@ __device__ __forceinline__ void det_lap_unit(const float* __restrict__ O, int ND
       row = gd.row(g) + 5 + 3 * (int64_t)rev[p0 + m - 1];
       ep = (int64_t)g * 5 * ND;
     }
-    jb[t] = O + row * ND + cbase + kg * 4;
+    jb[t] = O + row * ldo + cbase + kg * 4;
     sc[t] = ep >= 0;
     eb[t] = env + (ep >= 0 ? ep : 0) + cbase + kg * 4;
     bok[t] = ok;
     xb[t] = AinvT + ((size_t)(w * D + d) * N + i) * N + kg * 4 - off;
   }
   struct Chunk {
-    float4_t j[XG][NAT], e[NAT], x[NAT];
+    float4_t j[XG][NAT][U], e[NAT][U], x[NAT][U];
   };
   double tr = 0.0, gx[3] = {0.0, 0.0, 0.0};
   for (int x0 = 0; x0 < 3; x0 += XG) {
     auto load = [&](int k, Chunk& c) {
-      const int o = k * 16;  // this lane's columns: o + 4 kg + t in the chunk frame
 #pragma unroll
-      for (int t = 0; t < NAT; ++t) {
-        if (VEC) {
-          // vectors start inside the slice or not at all; their tail past the slice meets zero
-          // inverse columns (and stays inside the row since ND % 4 == 0)
-          const bool ok = o + kg * 4 < ncol;
+      for (int u = 0; u < U; ++u) {
+        const int o = k * CW + 16 * u;  // this lane's columns: o + 4 kg + t in the chunk frame
 #pragma unroll
-          for (int xi = 0; xi < XG; ++xi)
-            c.j[xi][t] = ok ? *reinterpret_cast<const float4_t*>(jb[t] + (int64_t)(x0 + xi) * ND + o)
-                            : (float4_t)(0.f);
-          c.e[t] = ok && sc[t] ? *reinterpret_cast<const float4_t*>(eb[t] + o) : (float4_t)(1.f);
+        for (int t = 0; t < NAT; ++t) {
+          if (VEC) {
+            const bool ok = o + kg * 4 < ncol;
 #pragma unroll
-          for (int h = 0; h < 2; ++h) {
-            const int cs = o + kg * 4 - off + 2 * h;  // slice column (even: N and off are even)
-            const float2_t v = bok[t] && cs >= 0 && cs < N ? *reinterpret_cast<const float2_t*>(xb[t] + o + 2 * h)
-                                                           : (float2_t)(0.f);
-            c.x[t][2 * h] = v[0];
-            c.x[t][2 * h + 1] = v[1];
-          }
-        } else {
+            for (int xi = 0; xi < XG; ++xi)
+              c.j[xi][t][u] = ok ? *reinterpret_cast<const float4_t*>(jb[t] + (int64_t)(x0 + xi) * ldo + o)
+                                 : (float4_t)(0.f);
+            c.e[t][u] = ok && sc[t] ? *reinterpret_cast<const float4_t*>(eb[t] + o) : (float4_t)(1.f);
 #pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            const int cf = o + kg * 4 + q, cs = cf - off;
-            const bool ok = cf < ncol;
+            for (int h = 0; h < 2; ++h) {
+              const int cs = o + kg * 4 - off + 2 * h;  // slice column (even: N and off are even)
+              const float2_t v = bok[t] && cs >= 0 && cs < N
+                                     ? *reinterpret_cast<const float2_t*>(xb[t] + o + 2 * h)
+                                     : (float2_t)(0.f);
+              c.x[t][u][2 * h] = v[0];
+              c.x[t][u][2 * h + 1] = v[1];
+            }
+          } else {
 #pragma unroll
-            for (int xi = 0; xi < XG; ++xi) c.j[xi][t][q] = ok ? jb[t][(int64_t)(x0 + xi) * ND + o + q] : 0.f;
-            c.e[t][q] = ok && sc[t] ? eb[t][o + q] : 1.f;
-            c.x[t][q] = bok[t] && cs >= 0 && cs < N ? xb[t][o + q] : 0.f;
+            for (int q = 0; q < 4; ++q) {
+              const int cf = o + kg * 4 + q, cs = cf - off;
+              const bool ok = cf < ncol;
+#pragma unroll
+              for (int xi = 0; xi < XG; ++xi)
+                c.j[xi][t][u][q] = ok ? jb[t][(int64_t)(x0 + xi) * ldo + o + q] : 0.f;
+              c.e[t][u][q] = ok && sc[t] ? eb[t][o + q] : 1.f;
+              c.x[t][u][q] = bok[t] && cs >= 0 && cs < N ? xb[t][o + q] : 0.f;
+            }
           }
         }
       }
@@ -766,16 +782,18 @@ __device__ __forceinline__ void det_lap_unit(const float* __restrict__ O, int ND
         for (int j = 0; j < NAT; ++j) acc[xi][i][j] = (float4_t){0.f, 0.f, 0.f, 0.f};
     auto mul = [&](const Chunk& c) {
 #pragma unroll
-      for (int t = 0; t < 4; ++t)
+      for (int u = 0; u < U; ++u)
 #pragma unroll
-        for (int xi = 0; xi < XG; ++xi)
+        for (int t = 0; t < 4; ++t)
 #pragma unroll
-          for (int i = 0; i < NAT; ++i) {
-            const float jv = c.j[xi][i][t] * c.e[i][t];
+          for (int xi = 0; xi < XG; ++xi)
 #pragma unroll
-            for (int j = 0; j < NAT; ++j)
-              acc[xi][i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(jv, c.x[j][t], acc[xi][i][j], 0, 0, 0);
-          }
+            for (int i = 0; i < NAT; ++i) {
+              const float jv = c.j[xi][i][u][t] * c.e[i][u][t];
+#pragma unroll
+              for (int j = 0; j < NAT; ++j)
+                acc[xi][i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(jv, c.x[j][u][t], acc[xi][i][j], 0, 0, 0);
+            }
     };
     Chunk c[P];
 #pragma unroll
@@ -834,9 +852,9 @@ __device__ __forceinline__ void det_lap_unit(const float* __restrict__ O, int ND
 // Units of NAT = ceil(s / 16) tiles only (one launch per tile count, so each has its own
 // register budget); the other wavefronts return at once.  Workgroups are mapped so that each
 // XCD takes a contiguous range of units and keeps its walkers' inverses in its own L2.
-template <int NAT, int XG, int P, bool VEC>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NAT == 2 && VEC ? 4 : 1)))
-k_det_lap(const float* __restrict__ O, int ND, GroupDesc gd, const float* __restrict__ env,
+template <int NAT, int XG, int P, bool VEC, int U, int WPE>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE)))
+k_det_lap(const float* __restrict__ O, int ldo, int ND, GroupDesc gd, const float* __restrict__ env,
           const int64_t* __restrict__ ptr, const int32_t* __restrict__ nbr, const int32_t* __restrict__ rev, int B,
           int N, int n_up, int D, const float* __restrict__ AinvT, double* __restrict__ trip,
           double* __restrict__ grad) {
@@ -854,34 +872,39 @@ k_det_lap(const float* __restrict__ O, int ND, GroupDesc gd, const float* __rest
   const int64_t p0 = ptr[gn];
   const int s = 1 + (int)(ptr[gn + 1] - p0);
   if (((s + 15) >> 4) != NAT) return;
-  det_lap_unit<NAT, XG, P, VEC>(O, ND, gd, env, nbr, rev, B, N, n_up, D, AinvT, trip, grad, w, n, d, gn, p0, s,
-                                tbs[wid]);
+  det_lap_unit<NAT, XG, P, VEC, U>(O, ldo, ND, gd, env, nbr, rev, B, N, n_up, D, AinvT, trip, grad, w, n, d, gn, p0,
+                                   s, tbs[wid]);
 }
 
-template <bool VEC>
-static void det_lap_launch(hipStream_t st, const float* O, int ND, GroupDesc gd, const float* env, const int64_t* ptr,
-                           const int32_t* nbr, const int32_t* rev, int B, int N, int n_up, int D, int smax,
-                           const float* AinvT, double* trip, double* grad) {
+template <bool VEC, int U>
+static void det_lap_launch(hipStream_t st, const float* O, int ldo, int ND, GroupDesc gd, const float* env,
+                           const int64_t* ptr, const int32_t* nbr, const int32_t* rev, int B, int N, int n_up, int D,
+                           int smax, const float* AinvT, double* trip, double* grad) {
   const int64_t nblk = ((int64_t)B * N * D + 3) / 4;
   const dim3 grid((unsigned)(8 * ((nblk + 7) / 8)));
   const int natm = (smax + 15) / 16;
-  k_det_lap<1, 3, 2, VEC><<<grid, 256, 0, st>>>(O, ND, gd, env, ptr, nbr, rev, B, N, n_up, D, AinvT, trip, grad);
+  k_det_lap<1, 3, 2, VEC, U, 1><<<grid, 256, 0, st>>>(O, ldo, ND, gd, env, ptr, nbr, rev, B, N, n_up, D, AinvT, trip,
+                                                      grad);
   if (natm >= 2)
-    k_det_lap<2, 3, 1, VEC><<<grid, 256, 0, st>>>(O, ND, gd, env, ptr, nbr, rev, B, N, n_up, D, AinvT, trip, grad);
+    k_det_lap<2, DL_XG2, 1, VEC, U, DL_WPE2><<<grid, 256, 0, st>>>(O, ldo, ND, gd, env, ptr, nbr, rev, B, N, n_up, D,
+                                                                  AinvT, trip, grad);
+  // the 3- and 4-tile classes (rare) keep 16-column runs: their accumulators fill the registers
   if (natm >= 3)
-    k_det_lap<3, 3, 1, VEC><<<grid, 256, 0, st>>>(O, ND, gd, env, ptr, nbr, rev, B, N, n_up, D, AinvT, trip, grad);
+    k_det_lap<3, 3, 1, VEC, 1, 1><<<grid, 256, 0, st>>>(O, ldo, ND, gd, env, ptr, nbr, rev, B, N, n_up, D, AinvT,
+                                                        trip, grad);
   if (natm >= 4)
-    k_det_lap<4, 1, 1, VEC><<<grid, 256, 0, st>>>(O, ND, gd, env, ptr, nbr, rev, B, N, n_up, D, AinvT, trip, grad);
+    k_det_lap<4, 1, 1, VEC, 1, 1><<<grid, 256, 0, st>>>(O, ldo, ND, gd, env, ptr, nbr, rev, B, N, n_up, D, AinvT,
+                                                        trip, grad);
 }
 
-void launch_det_lap(hipStream_t s, const float* O, int ND, GroupDesc gd, const float* env, const int64_t* ptr,
-                    const int32_t* nbr, const int32_t* rev, int B, int N, int n_up, int D, int smax,
-                    const float* AinvT, double* trip, double* grad) {
+void launch_det_lap(hipStream_t s, const float* O, int ldo, int ND, GroupDesc gd, const float* env,
+                    const int64_t* ptr, const int32_t* nbr, const int32_t* rev, int B, int N, int n_up, int D,
+                    int smax, const float* AinvT, double* trip, double* grad) {
   if (B * N == 0) return;
-  if (N % 2 == 0 && ND % 4 == 0)
-    det_lap_launch<true>(s, O, ND, gd, env, ptr, nbr, rev, B, N, n_up, D, smax, AinvT, trip, grad);
+  if (N % 2 == 0 && ND % 4 == 0 && ldo % 32 == 0)
+    det_lap_launch<true, DL_U>(s, O, ldo, ND, gd, env, ptr, nbr, rev, B, N, n_up, D, smax, AinvT, trip, grad);
   else
-    det_lap_launch<false>(s, O, ND, gd, env, ptr, nbr, rev, B, N, n_up, D, smax, AinvT, trip, grad);
+    det_lap_launch<false, 1>(s, O, ldo, ND, gd, env, ptr, nbr, rev, B, N, n_up, D, smax, AinvT, trip, grad);
 }
 
 // Sum of the determinants: log|psi_det|, sign, gradient and Laplacian per walker (FP64).
