@@ -611,11 +611,11 @@ void run_jastrow_and_assemble(Ctx& c, const float* r, int B, bool tan, const Gro
     const int j = h->iJM;
     float* cur = h->XB.as<float>();
     float* nxt = h->XC.as<float>();
+    // the silu of the last hidden layer is applied inside k_jmlp_last
     gemm(c, h->XA.as<float>(), F, j + 0, j + 1, cur, W, R, W, F, flag, period);
-    launch_act(c.s, cur, W, W, G, gd, ACT_SILU, 0.f);
     for (int l = 1; l < h->depth; ++l) {
+      launch_act(c.s, cur, W, W, G, gd, ACT_SILU, 0.f);
       gemm(c, cur, W, j + 2 * l, j + 2 * l + 1, nxt, W, R, W, W, flag, period);
-      launch_act(c.s, nxt, W, W, G, gd, ACT_SILU, 0.f);
       std::swap(cur, nxt);
     }
     const int jl = j + 2 * h->depth;
@@ -901,11 +901,11 @@ void state_propose(Ctx& c, int k, int mode, const float* rk_in, float width, uin
     const int j = h->iJM;
     float* cur = S.p_x.as<float>();
     float* nxt = S.p_y.as<float>();
+    // the silu of the last hidden layer is applied inside k_jmlp_last
     gemm(cc, S.p_h.as<float>(), F, j + 0, j + 1, cur, W, T, W, F, nullptr, 1, rT);
-    launch_act(c.s, cur, W, W, (int)T, g1, ACT_SILU, 0.f, dT);
     for (int l = 1; l < h->depth; ++l) {
+      launch_act(c.s, cur, W, W, (int)T, g1, ACT_SILU, 0.f, dT);
       gemm(cc, cur, W, j + 2 * l, j + 2 * l + 1, nxt, W, T, W, W, nullptr, 1, rT);
-      launch_act(c.s, nxt, W, W, (int)T, g1, ACT_SILU, 0.f, dT);
       std::swap(cur, nxt);
     }
     const int jl = j + 2 * h->depth;

@@ -28,6 +28,7 @@ void launch_det_combine(hipStream_t s, int B, int N, int D, const double* logdet
                         double* out_grad, double* out_lap, bool tangents);
 
 // Jastrow
+// Y: pre-activation rows of the last hidden layer (the silu is applied inside)
 void launch_jmlp_last(hipStream_t s, const float* Y, int W, int G, GroupDesc gd, const float* W3, const float* b3,
                       const float* scale, const float* zb, float* JR, const int32_t* dG = nullptr);
 void launch_node_sums(hipStream_t s, const float* X, int64_t ld, int ncols, GroupDesc gd, int B, int N, int n_up,

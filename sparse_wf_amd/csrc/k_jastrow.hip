@@ -10,17 +10,31 @@
 namespace fire {
 
 // Last MLP-Jastrow layer (W -> 2), silu with forward-Laplacian, * mlp_scale + [0, log_bias].
-// JR gets two columns per tangent row of the group.  The row dot products go to LDS (rows past
-// 5 + 3 (FIRE_SMAX - 1), which the neighbour limit excludes, to JR itself); the activation rule
-// then runs over the rows in parallel (128 threads per column) with the sum of squares of the
-// jacobian rows reduced in LDS.
+// Z holds the pre-activation rows of the layer before: its silu (forward-Laplacian rule) is
+// applied here rather than in a separate pass.  With f, f1, f2 = silu and its derivatives at
+// the group's value row (per column k), the activated rows are f (value), f1 z_jac (each
+// jacobian row) and f1 z_lap + f2 sum_jac z_jac^2 (Laplacian row), so the row dot products
+// with W3 are
+//   value: sum f W3,  jacobian: sum z (f1 W3),  Laplacian: sum (f1 z_lap + f2 ss) W3,
+// with ss[k] the column sums of squares of the jacobian rows (partials per row slot in LDS,
+// added in a fixed order).  JR gets two columns per tangent row of the group.  The row dot
+// products go to LDS (rows past 5 + 3 (FIRE_SMAX - 1), which the neighbour limit excludes, to
+// JR itself); the activation rule of the last layer then runs over the rows in parallel (128
+// threads per column).  Dynamic LDS: 20 W floats.
 #define JM_MAXR (5 + 3 * (FIRE_SMAX - 1))
-__global__ void __launch_bounds__(256) k_jmlp_last(const float* __restrict__ Y, int W, GroupDesc gd,
+__global__ void __launch_bounds__(256) k_jmlp_last(const float* __restrict__ Z, int W, GroupDesc gd,
                                                    const float* __restrict__ W3, const float* __restrict__ b3,
                                                    const float* __restrict__ scale, const float* __restrict__ zb,
                                                    float* __restrict__ JR, const int32_t* __restrict__ dG) {
+  extern __shared__ float jm_lds[];
+  float* s_f1 = jm_lds;         // [W]
+  float* s_f2 = s_f1 + W;       // [W]
+  float* s_v0 = s_f2 + W;       // f1 * W3[:, 0]
+  float* s_v1 = s_v0 + W;       // f1 * W3[:, 1]
+  float* s_ss = s_v1 + W;       // [16][W] partial sums of squares
   __shared__ float s_a[2][JM_MAXR];
-  __shared__ float s_ss[2][2];
+  __shared__ float s_red[2][4];
+  __shared__ float s_ss2[2][2];
   const int g = blockIdx.x;
   if (dG && g >= *dG) return;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -36,41 +50,95 @@ __global__ void __launch_bounds__(256) k_jmlp_last(const float* __restrict__ Y, 
       JR[(r0 + rr) * 2 + 1] = a1;
     }
   };
-  if (W % 64 == 0) {
-    // four rows per wavefront, 16 lanes per row with float4 loads (W % 4 == 0 keeps rows aligned)
-    const int sub = lane >> 4, l16 = lane & 15;
-    for (int rb = wid * 4; rb < nr; rb += (blockDim.x >> 6) * 4) {
-      const int rr = rb + sub;
-      float a0 = 0.f, a1 = 0.f;
-      if (rr < nr) {
-        const float* y = Y + (r0 + rr) * W;
-        for (int k = 4 * l16; k < W; k += 64) {
-          const float4 v = *reinterpret_cast<const float4*>(y + k);
-          const float4 wa = *reinterpret_cast<const float4*>(W3 + 2 * k);
-          const float4 wb = *reinterpret_cast<const float4*>(W3 + 2 * k + 4);
-          a0 += v.x * wa.x + v.y * wa.z + v.z * wb.x + v.w * wb.z;
-          a1 += v.x * wa.y + v.y * wa.w + v.z * wb.y + v.w * wb.w;
+  // block sum of two values; every thread gets them
+  auto bsum2 = [&](float& a0, float& a1) {
+    a0 = warp_sum(a0);
+    a1 = warp_sum(a1);
+    if (lane == 0) { s_red[0][wid] = a0; s_red[1][wid] = a1; }
+    __syncthreads();
+    a0 = s_red[0][0] + s_red[0][1] + s_red[0][2] + s_red[0][3];
+    a1 = s_red[1][0] + s_red[1][1] + s_red[1][2] + s_red[1][3];
+    __syncthreads();
+  };
+  // value row: activation coefficients per column and the value-row dot products
+  {
+    float a0 = 0.f, a1 = 0.f;
+    for (int k = threadIdx.x; k < W; k += blockDim.x) {
+      float f, f1, f2;
+      silu3(Z[r0 * W + k], f, f1, f2);
+      const float w0 = W3[2 * k], w1 = W3[2 * k + 1];
+      s_f1[k] = f1;
+      s_f2[k] = f2;
+      s_v0[k] = f1 * w0;
+      s_v1[k] = f1 * w1;
+      a0 += f * w0;
+      a1 += f * w1;
+    }
+    bsum2(a0, a1);  // (its barriers also publish the coefficients)
+    if (threadIdx.x == 0) put(0, a0, a1);
+  }
+  if (nr > 1) {
+    // jacobian rows: dots with f1 W3, and per-column sums of squares in 16 partial slots
+    if (W % 64 == 0) {
+      // four rows per wavefront, 16 lanes per row with float4 loads
+      const int sub = lane >> 4, l16 = lane & 15, slot = wid * 4 + sub;
+      for (int k = 4 * l16; k < W; k += 64)
+        *reinterpret_cast<float4*>(s_ss + slot * W + k) = make_float4(0.f, 0.f, 0.f, 0.f);
+      for (int rb = 2 + wid * 4; rb < nr; rb += (blockDim.x >> 6) * 4) {
+        const int rr = rb + sub;
+        float a0 = 0.f, a1 = 0.f;
+        if (rr < nr) {
+          const float* z = Z + (r0 + rr) * W;
+          for (int k = 4 * l16; k < W; k += 64) {
+            const float4 v = *reinterpret_cast<const float4*>(z + k);
+            const float4 c0 = *reinterpret_cast<const float4*>(s_v0 + k);
+            const float4 c1 = *reinterpret_cast<const float4*>(s_v1 + k);
+            a0 += v.x * c0.x + v.y * c0.y + v.z * c0.z + v.w * c0.w;
+            a1 += v.x * c1.x + v.y * c1.y + v.z * c1.z + v.w * c1.w;
+            float4* sp = reinterpret_cast<float4*>(s_ss + slot * W + k);
+            float4 q = *sp;
+            q.x += v.x * v.x;
+            q.y += v.y * v.y;
+            q.z += v.z * v.z;
+            q.w += v.w * v.w;
+            *sp = q;
+          }
         }
-      }
 #pragma unroll
-      for (int o = 8; o > 0; o >>= 1) {
-        a0 += __shfl_xor(a0, o, 16);
-        a1 += __shfl_xor(a1, o, 16);
+        for (int o = 8; o > 0; o >>= 1) {
+          a0 += __shfl_xor(a0, o, 16);
+          a1 += __shfl_xor(a1, o, 16);
+        }
+        if (l16 == 0 && rr < nr) put(rr, a0, a1);
       }
-      if (l16 == 0 && rr < nr) put(rr, a0, a1);
-    }
-  } else {
-    for (int rr = wid; rr < nr; rr += blockDim.x >> 6) {
-      float a0 = 0.f, a1 = 0.f;
-      for (int k = lane; k < W; k += 64) {
-        const float y = Y[(r0 + rr) * W + k];
-        a0 += y * W3[2 * k];
-        a1 += y * W3[2 * k + 1];
+    } else {
+      for (int k = lane; k < W; k += 64) s_ss[wid * W + k] = 0.f;
+      for (int rr = 2 + wid; rr < nr; rr += blockDim.x >> 6) {
+        float a0 = 0.f, a1 = 0.f;
+        for (int k = lane; k < W; k += 64) {
+          const float v = Z[(r0 + rr) * W + k];
+          a0 += v * s_v0[k];
+          a1 += v * s_v1[k];
+          s_ss[wid * W + k] += v * v;
+        }
+        a0 = warp_sum(a0);
+        a1 = warp_sum(a1);
+        if (lane == 0) put(rr, a0, a1);
       }
-      a0 = warp_sum(a0);
-      a1 = warp_sum(a1);
-      if (lane == 0) put(rr, a0, a1);
     }
+    __syncthreads();
+    // Laplacian row
+    const int nslot = W % 64 == 0 ? 16 : 4;
+    float a0 = 0.f, a1 = 0.f;
+    for (int k = threadIdx.x; k < W; k += blockDim.x) {
+      float ss = 0.f;
+      for (int q = 0; q < nslot; ++q) ss += s_ss[q * W + k];
+      const float y = s_f1[k] * Z[(r0 + 1) * W + k] + s_f2[k] * ss;
+      a0 += y * W3[2 * k];
+      a1 += y * W3[2 * k + 1];
+    }
+    bsum2(a0, a1);
+    if (threadIdx.x == 0) put(1, a0, a1);
   }
   __syncthreads();
   const int c = threadIdx.x >> 7, t = threadIdx.x & 127;  // column, thread within the column
@@ -84,18 +152,24 @@ __global__ void __launch_bounds__(256) k_jmlp_last(const float* __restrict__ Y, 
     JR[(r0 + rr) * 2 + c] = v * f1 * sc;
   }
   ss = warp_sum(ss);
-  if (lane == 0) s_ss[c][wid & 1] = ss;
+  if (lane == 0) s_ss2[c][wid & 1] = ss;
   __syncthreads();
   if (t == 0) {
     JR[r0 * 2 + c] = f * sc + zb[c];
-    if (nr > 1) JR[(r0 + 1) * 2 + c] = (s_a[c][1] * f1 + f2 * (s_ss[c][0] + s_ss[c][1])) * sc;
+    if (nr > 1) JR[(r0 + 1) * 2 + c] = (s_a[c][1] * f1 + f2 * (s_ss2[c][0] + s_ss2[c][1])) * sc;
   }
 }
 
 void launch_jmlp_last(hipStream_t s, const float* Y, int W, int G, GroupDesc gd, const float* W3, const float* b3,
                       const float* scale, const float* zb, float* JR, const int32_t* dG) {
   if (G == 0) return;
-  k_jmlp_last<<<G, 256, 0, s>>>(Y, W, gd, W3, b3, scale, zb, JR, dG);
+  const size_t lds = sizeof(float) * 20 * (size_t)W;
+  static size_t lds_set = 64 * 1024;
+  if (lds > lds_set) {
+    (void)hipFuncSetAttribute((const void*)k_jmlp_last, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    lds_set = lds;
+  }
+  k_jmlp_last<<<G, 256, lds, s>>>(Y, W, gd, W3, b3, scale, zb, JR, dG);
 }
 
 // Per walker and spin: sum over centre electrons of the value row and the Laplacian row.
