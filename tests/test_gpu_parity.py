@@ -30,6 +30,9 @@ CASES = {
     "small_restricted": lambda: small_cfg(True),
     "water": lambda: water_cfg(),
     "alkane130": lambda: alkane_cfg(),
+    # dependency sets of up to 50 members: the 2-, 3- and 4-tile classes of the determinant
+    # Laplacian, on its vector path (N even, D N a multiple of 4)
+    "alkane130_dense": lambda: alkane_cfg(cutoff=7.0, n_determinants=2),
 }
 
 
@@ -139,7 +142,10 @@ def test_local_energy(setup, precision):
         # mixed: FP32-level products, but the BF16 MFMA accumulation is looser than FP32's on
         # ill-conditioned walkers (a near-node water walker with Laplacian ~ -4e4 reaches 2.4e-4)
         if precision == "fp32":
-            assert eg < 1e-4 and el < 1e-3 and ek < 1e-3
+            # 130 electrons with ~38 neighbours each: one of the four walkers reaches 1.8e-4 in
+            # the gradient (walker conditioning, not the set size: the same walkers measure
+            # 4.7e-4 at cutoff 5 and 5.2e-3 at cutoff 4, with mixed as close)
+            assert eg < (3e-4 if name == "alkane130_dense" else 1e-4) and el < 1e-3 and ek < 1e-3
         elif precision == "mixed":
             assert eg < 5e-4 and el < 2e-3 and ek < 2e-3
         else:
