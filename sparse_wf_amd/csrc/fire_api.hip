@@ -90,7 +90,8 @@ struct fire_handle {
   // tensor indices
   int iEN, iEEs, iEEd, iMLP, iORB, iJM, iATT, iCUSP, iSCAL;
   // workspace
-  DevBuf cnt, ptr, nbr, rev, rowflag;
+  DevBuf cnt, ptr, nbr, rev, rowflag, rowgrp;
+  DevBuf AV0, Z0, F1, PART;  // fused silu epilogue: packed value rows, their pre-activations, silu', slots
   DevBuf T5, Z2, T5b, M5, XA, XB, XC, O, ENV, ENVC, PF, EF, EOK;
   DevBuf A64, Ainv, AinvT, ldet, dsgn, tr, trip, dgrad;
   DevBuf det_lp, det_sg, det_gr, det_lap;
@@ -105,6 +106,9 @@ struct fire_handle {
   // Autotuned tile shape of the BF16 contraction per (split, N, K, floor(log2 M)): the first
   // call of a shape times every candidate on the real operands and keeps the fastest.
   bool autotune = true;
+  // silu of the tangent-row MLP layers fused into the contraction epilogue (FIRE_FUSED_ACT=0:
+  // the separate element-wise pass instead)
+  bool fuse_act = true;
   bool use_graphs = true;  // whole Metropolis sweeps replayed from a captured hipGraph
   // BF16 planes per operand used by FIRE_MIXED for each contraction class (ElecInit MLP,
   // update MLP, orbitals, MLP Jastrow, attention Q/V, attention readout): 3 = six products
@@ -285,7 +289,8 @@ int gemm_class(const fire_handle* h, int wi) {
 
 // C = A @ W(tensor wi) + bias(tensor bi, optional) on value rows
 void gemm(Ctx& c, const float* A, int64_t lda, int wi, int bi, float* C, int64_t ldc, int64_t M, int N, int K,
-          const uint8_t* rowflag, int period, DevRows dr = DevRows()) {
+          const uint8_t* rowflag, int period, DevRows dr = DevRows(), const ActEpi* epp = nullptr) {
+  const ActEpi ep = epp ? *epp : ActEpi();
   fire_handle* h = c.h;
   const float* bias = bi >= 0 ? h->P[bi] : nullptr;
   const bool bf_ok = h->Pbf[wi] != nullptr && (K % 32) == 0 && (lda % 4) == 0 && N > 16;
@@ -300,6 +305,7 @@ void gemm(Ctx& c, const float* A, int64_t lda, int wi, int bi, float* C, int64_t
     int lg = 0;
     while ((int64_t(1) << (lg + 1)) <= M) ++lg;
     if (dr.count) lg += 32;  // rows counted on the device (M is the capacity): tuned apart
+    if (ep.rowgrp) lg += 64;  // fused silu epilogue: tuned apart
     const auto key = std::make_tuple(ns, N, K, lg);
     auto it = h->gemm_tile.find(key);
     if (it != h->gemm_tile.end()) {
@@ -312,11 +318,15 @@ void gemm(Ctx& c, const float* A, int64_t lda, int wi, int bi, float* C, int64_t
       HIPCHK(hipEventCreate(&t0));
       HIPCHK(hipEventCreate(&t1));
       for (int v = 0; v < nv; ++v) {
+        if (ep.rowgrp && !gemm_bf16_fusable(v)) {
+          ms[v] = 1e30f;
+          continue;
+        }
         launch_gemm_bf16(c.s, A, lda, h->Pbf[wi], h->Pbf_lo[wi], h->Pbf_lo2[wi], bias, C, ldc, M, N, K, rowflag,
-                         period, ns, v, dr);
+                         period, ns, v, dr, ep);
         HIPCHK(hipEventRecord(t0, c.s));
         launch_gemm_bf16(c.s, A, lda, h->Pbf[wi], h->Pbf_lo[wi], h->Pbf_lo2[wi], bias, C, ldc, M, N, K, rowflag,
-                         period, ns, v, dr);
+                         period, ns, v, dr, ep);
         HIPCHK(hipEventRecord(t1, c.s));
         HIPCHK(hipEventSynchronize(t1));
         HIPCHK(hipEventElapsedTime(&ms[v], t0, t1));
@@ -337,14 +347,45 @@ void gemm(Ctx& c, const float* A, int64_t lda, int wi, int bi, float* C, int64_t
   }
   if (use_bf) {
     launch_gemm_bf16(c.s, A, lda, h->Pbf[wi], h->Pbf_lo[wi], h->Pbf_lo2[wi], bias, C, ldc, M, N, K, rowflag, period,
-                     ns, variant, dr);
+                     ns, variant, dr, ep);
   } else
-    launch_gemm_f32(c.s, A, lda, h->P[wi], h->Pt32[wi], bias, C, ldc, M, N, K, rowflag, period, dr);
+    launch_gemm_f32(c.s, A, lda, h->P[wi], h->Pt32[wi], bias, C, ldc, M, N, K, rowflag, period, dr, ep);
   if (timed) {
     HIPCHK(hipEventRecord(e1, c.s));
     h->gemm_events.push_back({e0, e1});
     h->gemm_flops += 2.0 * (double)M * N * K;
   }
+}
+
+// C = silu(A @ W + b) over the tangent rows of G groups (forward-Laplacian rule of k_act), the
+// activation fused into the contraction's epilogue: the value rows are contracted first (FP32,
+// packed) for silu' of every group, the epilogue applies silu / silu' and leaves per-block sums
+// of squares of the jacobian rows, k_act_lapfix completes the Laplacian rows.
+void gemm_silu_tan(Ctx& c, const float* A, int64_t lda, int wi, int bi, float* C, int64_t ldc, int64_t R, int N, int K,
+                   const uint8_t* flag, int G, const GroupDesc& gd) {
+  fire_handle* h = c.h;
+  if (!h->fuse_act || N % 128 != 0 || K % 32 != 0 || lda % 4 != 0 || G == 0) {
+    gemm(c, A, lda, wi, bi, C, ldc, R, N, K, flag, 0);
+    launch_act(c.s, C, ldc, N, G, gd, ACT_SILU, 0.f);
+    return;
+  }
+  h->AV0.ensure(sizeof(float) * (size_t)G * K);
+  h->Z0.ensure(sizeof(float) * (size_t)G * N);
+  h->F1.ensure(sizeof(float) * (size_t)G * N);
+  h->PART.ensure(sizeof(float) * (size_t)G * ACT_EPI_SLOTS * N);
+  float* z0 = h->Z0.as<float>();
+  launch_gather_rows(c.s, A, lda, K, G, gd, h->AV0.as<float>());
+  launch_gemm_f32(c.s, h->AV0.as<float>(), K, h->P[wi], h->Pt32[wi], bi >= 0 ? h->P[bi] : nullptr, z0, N, G, N, K,
+                  nullptr, 1);
+  launch_silu_grad(c.s, z0, h->F1.as<float>(), (int64_t)G * N);
+  ActEpi ep;
+  ep.rowgrp = h->rowgrp.as<int32_t>();
+  ep.f1 = h->F1.as<float>();
+  ep.part = h->PART.as<float>();
+  ep.gd = gd;
+  ep.ldf = N;
+  gemm(c, A, lda, wi, bi, C, ldc, R, N, K, flag, 0, DevRows(), &ep);
+  launch_act_lapfix(c.s, C, ldc, N, G, gd, z0, h->PART.as<float>(), N);
 }
 
 EnParams en_params(fire_handle* h) {
@@ -463,8 +504,15 @@ void run_embedding(Ctx& c, const float* r, int B, const Graph& gr, bool tan, Gro
     launch_rowflag(c.s, G, gd, R, h->rowflag.as<uint8_t>());
     flag = h->rowflag.as<uint8_t>();
     period = 0;
+    h->rowgrp.ensure(sizeof(int32_t) * (size_t)R);
+    launch_rowgrp(c.s, G, gd, h->rowgrp.as<int32_t>());
   }
   const int m = h->iMLP;
+  if (tan) {
+    gemm_silu_tan(c, h->XA.as<float>(), F, m + 0, m + 1, h->XB.as<float>(), F, R, F, F, flag, G, gd);
+    gemm_silu_tan(c, h->XB.as<float>(), F, m + 2, m + 3, h->XA.as<float>(), F, R, F, F, flag, G, gd);
+    return;
+  }
   gemm(c, h->XA.as<float>(), F, m + 0, m + 1, h->XB.as<float>(), F, R, F, F, flag, period);
   launch_act(c.s, h->XB.as<float>(), F, F, G, gd, ACT_SILU, 0.f);
   gemm(c, h->XB.as<float>(), F, m + 2, m + 3, h->XA.as<float>(), F, R, F, F, flag, period);
@@ -611,13 +659,20 @@ void run_jastrow_and_assemble(Ctx& c, const float* r, int B, bool tan, const Gro
     const int j = h->iJM;
     float* cur = h->XB.as<float>();
     float* nxt = h->XC.as<float>();
-    // the silu of the last hidden layer is applied inside k_jmlp_last
-    gemm(c, h->XA.as<float>(), F, j + 0, j + 1, cur, W, R, W, F, flag, period);
-    for (int l = 1; l < h->depth; ++l) {
-      launch_act(c.s, cur, W, W, G, gd, ACT_SILU, 0.f);
-      gemm(c, cur, W, j + 2 * l, j + 2 * l + 1, nxt, W, R, W, W, flag, period);
+    // the silu of the last hidden layer is applied inside k_jmlp_last, the others in the
+    // epilogue of their contraction (tangent rows) or by k_act
+    for (int l = 0; l < h->depth; ++l) {
+      const float* in = l == 0 ? h->XA.as<float>() : nxt;
+      const int fan = l == 0 ? F : W;
+      if (l + 1 < h->depth && tan) {
+        gemm_silu_tan(c, in, fan, j + 2 * l, j + 2 * l + 1, cur, W, R, W, fan, flag, G, gd);
+      } else {
+        gemm(c, in, fan, j + 2 * l, j + 2 * l + 1, cur, W, R, W, fan, flag, period);
+        if (l + 1 < h->depth) launch_act(c.s, cur, W, W, G, gd, ACT_SILU, 0.f);
+      }
       std::swap(cur, nxt);
     }
+    std::swap(cur, nxt);
     const int jl = j + 2 * h->depth;
     launch_jmlp_last(c.s, cur, W, G, gd, h->P[jl], h->P[jl + 1], h->P[jl + 2], h->P[jl + 3], h->JR.as<float>());
     launch_node_sums(c.s, h->JR.as<float>(), 2, 2, gd, B, N, h->n_up, h->mlp_ns.as<double>());
@@ -1102,6 +1157,7 @@ int fire_create(const fire_config* cfg, const float* R, const float* Z, int devi
     HIPCHK(hipSetDevice(device));
     fire_handle* h = new fire_handle();
     h->cfg = c;
+    if (const char* e = getenv("FIRE_FUSED_ACT")) h->fuse_act = e[0] != '0';
     h->device = device;
     h->N = c.n_el;
     h->n_up = c.n_up;

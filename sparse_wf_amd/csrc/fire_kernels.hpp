@@ -69,11 +69,37 @@ struct DevRows {
   const int32_t* offset = nullptr;
 };
 
+// Forward-Laplacian silu applied in the epilogue of a contraction over tangent rows (the
+// rule of k_act, split in three):
+//   before: z0 = the value row of every group (FP32 contraction of the gathered value rows),
+//           f1[g][c] = silu'(z0);
+//   epilogue (rowgrp[row] = g for jacobian rows, -1 value rows, -2 Laplacian rows):
+//           value rows silu(z + bias), jacobian rows f1[g][c] z, Laplacian rows z unchanged,
+//           and the sum of z^2 over the jacobian rows of group g that one wavefront holds for
+//           one column, in part[g][slot][c] (slot = 2 (64-row block - first block of the
+//           group's jacobian rows) + lane half; at most 8 slots);
+//   after:  Laplacian rows f1 z + silu''(z0) * (sum of the group's slots), in k_act_lapfix.
+// rowgrp == nullptr: plain store.
+struct ActEpi {
+  const int32_t* rowgrp = nullptr;
+  const float* f1 = nullptr;
+  float* part = nullptr;
+  GroupDesc gd{0, 0, nullptr};
+  int ldf = 0;  // row stride of f1 and of each part slot
+};
+constexpr int ACT_EPI_SLOTS = 8;
+
+void launch_rowgrp(hipStream_t s, int G, GroupDesc gd, int32_t* rowgrp);
+void launch_gather_rows(hipStream_t s, const float* A, int64_t lda, int K, int G, GroupDesc gd, float* out);
+void launch_silu_grad(hipStream_t s, const float* z0, float* f1, int64_t n);
+void launch_act_lapfix(hipStream_t s, float* C, int64_t ldc, int N, int G, GroupDesc gd, const float* z0,
+                       const float* part, int ldf);
+
 // dense contractions: C[M,N] = A[M,K] W[K,N] (+ bias on rows with flag / row % period == 0)
 // W is [K,N] (used for N <= 16), Wt its transpose [N,K] (used by the MFMA kernel)
 void launch_gemm_f32(hipStream_t s, const float* A, int64_t lda, const float* W, const float* Wt, const float* bias,
                      float* C, int64_t ldc, int64_t M, int N, int K, const uint8_t* rowflag, int period,
-                     DevRows dr = DevRows());
+                     DevRows dr = DevRows(), ActEpi ep = ActEpi());
 void launch_gemm_f32_tail(hipStream_t s, const float* A, int64_t lda, const float* Wt, const float* bias, float* C,
                           int64_t ldc, int64_t M, int n0, int nr, int K, const uint8_t* rowflag, int period,
                           DevRows dr, int64_t sA, int64_t sW, int64_t sC, int nb);
@@ -84,8 +110,10 @@ void launch_gemm_f32_batched(hipStream_t s, const float* A, int64_t lda, int64_t
 void launch_gemm_bf16(hipStream_t s, const float* A, int64_t lda, const uint16_t* Whi /*[N,K] bf16*/,
                       const uint16_t* Wlo, const uint16_t* Wlo2, const float* bias, float* C, int64_t ldc, int64_t M,
                       int N, int K, const uint8_t* rowflag, int period, int nsplit, int variant = 0,
-                      DevRows dr = DevRows());
+                      DevRows dr = DevRows(), ActEpi ep = ActEpi());
 int gemm_bf16_variants();
+// variants that have a fused-silu (ActEpi) form; the others are never launched with one
+bool gemm_bf16_fusable(int variant);
 void gemm_bf16_tile(int variant, int* bm, int* bn);
 
 }  // namespace fire

@@ -306,6 +306,80 @@ void launch_act(hipStream_t s, float* Y, int64_t ld, int ncols, int G, GroupDesc
   k_act<<<grid, 256, 0, s>>>(Y, ld, ncols, gd, act, act_scale, dG);
 }
 
+// Row codes of the fused silu epilogue: -1 value row, -2 Laplacian row, g on the jacobian rows
+// of group g.  One wavefront per group.
+__global__ void k_rowgrp(int G, GroupDesc gd, int32_t* __restrict__ code) {
+  const int g = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  if (g >= G) return;
+  const int64_t r0 = gd.row(g), r1 = gd.row(g + 1);
+  for (int64_t r = r0 + (threadIdx.x & 63); r < r1; r += 64) code[r] = r == r0 ? -1 : (r == r0 + 1 ? -2 : g);
+}
+
+void launch_rowgrp(hipStream_t s, int G, GroupDesc gd, int32_t* rowgrp) {
+  if (G > 0) k_rowgrp<<<(G + 3) / 4, 256, 0, s>>>(G, gd, rowgrp);
+}
+
+// out[g][:] = A[row(g)][:K] (the value rows, packed)
+__global__ void k_gather_rows(const float* __restrict__ A, int64_t lda, int K, int G, GroupDesc gd,
+                              float* __restrict__ out) {
+  const int g = blockIdx.x;
+  const float* src = A + gd.row(g) * lda;
+  for (int k = threadIdx.x; k < K; k += blockDim.x) out[(int64_t)g * K + k] = src[k];
+}
+
+void launch_gather_rows(hipStream_t s, const float* A, int64_t lda, int K, int G, GroupDesc gd, float* out) {
+  if (G > 0) k_gather_rows<<<G, 256, 0, s>>>(A, lda, K, G, gd, out);
+}
+
+__global__ void k_silu_grad(const float* __restrict__ z0, float* __restrict__ f1, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    float f, d1, d2;
+    silu3(z0[i], f, d1, d2);
+    f1[i] = d1;
+  }
+}
+
+void launch_silu_grad(hipStream_t s, const float* z0, float* f1, int64_t n) {
+  if (n > 0) k_silu_grad<<<2048, 256, 0, s>>>(z0, f1, n);
+}
+
+// Laplacian rows after the fused epilogue: f1 z + f2 * (sum of z^2 over the group's jacobian
+// rows), the sum assembled from the slots the epilogue wrote (a slot exists for each 64-row
+// block the jacobian rows touch and each lane half (row / 4 odd or even) that holds one of
+// them there).
+__global__ void k_act_lapfix(float* __restrict__ C, int64_t ldc, int N, GroupDesc gd, const float* __restrict__ z0,
+                             const float* __restrict__ part, int ldf) {
+  const int g = blockIdx.x;
+  const int c = blockIdx.y * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  const int64_t r0 = gd.row(g), r1 = gd.row(g + 1);
+  if (r1 - r0 < 2) return;
+  float f, f1, f2;
+  silu3(z0[(int64_t)g * ldf + c], f, f1, f2);
+  float ss = 0.f;
+  const int64_t js = r0 + 2, je = r1;
+  if (je > js) {
+    const int64_t b0 = js >> 6, b1 = (je - 1) >> 6;
+    for (int64_t blk = b0; blk <= b1; ++blk) {
+      const int64_t s0 = js > blk * 64 ? js : blk * 64, s1 = je < blk * 64 + 64 ? je : blk * 64 + 64;
+      for (int hh = 0; hh < 2; ++hh) {
+        bool has = false;
+        for (int64_t r = s0; r < s1 && r < s0 + 8; ++r) has |= (int)((r >> 2) & 1) == hh;
+        if (has) ss += part[((int64_t)g * ACT_EPI_SLOTS + 2 * (blk - b0) + hh) * ldf + c];
+      }
+    }
+  }
+  float* y = C + (r0 + 1) * ldc + c;
+  *y = *y * f1 + f2 * ss;
+}
+
+void launch_act_lapfix(hipStream_t s, float* C, int64_t ldc, int N, int G, GroupDesc gd, const float* z0,
+                       const float* part, int ldf) {
+  if (G == 0) return;
+  dim3 grid(G, (N + 255) / 256);
+  k_act_lapfix<<<grid, 256, 0, s>>>(C, ldc, N, gd, z0, part, ldf);
+}
+
 __global__ void k_rowflag(int G, GroupDesc gd, uint8_t* flag) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g < G) flag[gd.row(g)] = 1;

@@ -68,35 +68,96 @@ __device__ __forceinline__ void tile_of(int64_t& m_tile, int& n_tile, int nT) {
   n_tile = (int)(t - m_tile * nT);
 }
 
-template <typename AccT>
-__device__ __forceinline__ void store_tile(const AccT (&acc)[2][2], int64_t m0, int n0, int wm, int wn, int lane,
+// Writes a wavefront's 64 x (32 NB) block of 32x32 MFMA accumulators (lane l, register v of
+// tile (a, b): row 32 a + 8 (v / 4) + 4 (l / 32) + v % 4, column 32 b + l % 32), either plain
+// (+ bias on value rows) or through the fused forward-Laplacian silu described at ActEpi.
+template <int NB, bool FUSE, typename AccT>
+__device__ __forceinline__ void store_rows(const AccT (&acc)[2][NB], int64_t m0, int n0, int wm, int wn, int lane,
                                            float* __restrict__ C, int64_t ldc, int64_t M, int N,
                                            const float* __restrict__ bias, const uint8_t* __restrict__ rowflag,
-                                           int period) {
-  for (int a = 0; a < 2; ++a)
-    for (int b = 0; b < 2; ++b) {
+                                           int period, const ActEpi& ep) {
+  if constexpr (!FUSE) {
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
       const int col = n0 + wn + b * 32 + (lane & 31);
       if (col >= N) continue;
       const float bcol = bias ? bias[col] : 0.f;
-      for (int v = 0; v < 16; ++v) {
-        const int64_t row = m0 + wm + a * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
-        if (row >= M) continue;
-        float val = acc[a][b][v];
-        if (bias && is_bias_row(row, rowflag, period)) val += bcol;
-        C[row * ldc + col] = val;
-      }
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+          const int64_t row = m0 + wm + a * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+          if (row >= M) continue;
+          float val = acc[a][b][v];
+          if (bias && is_bias_row(row, rowflag, period)) val += bcol;
+          C[row * ldc + col] = val;
+        }
     }
+  } else {
+    // fused silu: the lane's 32 rows ascend with (a, v), so the jacobian rows of one group form
+    // one run per lane and column; each run's sum of squares goes to its own slot.  Lane l
+    // holds the code of row rb + l; every lane takes part in the shuffles.
+    const int h = lane >> 5;
+    const int64_t rb = m0 + wm;  // first row of this 64-row block (blocks are 64-aligned)
+    const int mycode = rb + lane < M ? ep.rowgrp[rb + lane] : -3;
+    auto flush = [&](int g, float ss, int col) {
+      const int64_t first = (ep.gd.row(g) + 2) >> 6;  // block of the group's first jacobian row
+      const int slot = 2 * (int)((rb >> 6) - first) + h;
+      ep.part[((int64_t)g * ACT_EPI_SLOTS + slot) * ep.ldf + col] = ss;
+    };
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const int col = n0 + wn + b * 32 + (lane & 31);
+      const bool colok = col < N;
+      const float bcol = (bias && colok) ? bias[col] : 0.f;
+      int cur = -1;
+      float ss = 0.f, f1 = 0.f;
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+          const int rl = a * 32 + 8 * (v >> 2) + 4 * h + (v & 3);
+          const int cd = __shfl(mycode, rl);
+          if (!colok || cd == -3) continue;
+          float z = acc[a][b][v];
+          if (cd >= 0) {
+            if (cd != cur) {
+              if (cur >= 0) flush(cur, ss, col);
+              cur = cd;
+              ss = 0.f;
+              f1 = ep.f1[(int64_t)cd * ep.ldf + col];
+            }
+            ss += z * z;
+            z *= f1;
+          } else if (cd == -1) {
+            z += bcol;
+            z *= 1.0f / (1.0f + expf(-z));
+          }
+          C[(rb + rl) * ldc + col] = z;
+        }
+      if (colok && cur >= 0) flush(cur, ss, col);
+    }
+  }
+}
+
+template <bool FUSE, typename AccT>
+__device__ __forceinline__ void store_tile(const AccT (&acc)[2][2], int64_t m0, int n0, int wm, int wn, int lane,
+                                           float* __restrict__ C, int64_t ldc, int64_t M, int N,
+                                           const float* __restrict__ bias, const uint8_t* __restrict__ rowflag,
+                                           int period, const ActEpi& ep) {
+  store_rows<2, FUSE>(acc, m0, n0, wm, wn, lane, C, ldc, M, N, bias, rowflag, period, ep);
 }
 
 // ---------------------------------------------------------------------------------------
 // FP32 MFMA GEMM.  Wt is [N,K] FP32 (W transposed).  Vector path: K % 32 == 0, lda % 4 == 0.
 // ---------------------------------------------------------------------------------------
-template <bool VEC>
+template <bool VEC, bool FUSE = false>
 __global__ void __launch_bounds__(256) k_gemm_f32(const float* __restrict__ A, int64_t lda,
                                                   const float* __restrict__ Wt, const float* __restrict__ bias,
                                                   float* __restrict__ C, int64_t ldc, int64_t M, int N, int K,
                                                   const uint8_t* __restrict__ rowflag, int period, int nT,
-                                                  DevRows dr, int64_t sA, int64_t sW, int64_t sC) {
+                                                  DevRows dr, int64_t sA, int64_t sW, int64_t sC,
+                                                  ActEpi ep) {
   __shared__ __attribute__((aligned(16))) float As[GT_M][LDA_F];
   __shared__ __attribute__((aligned(16))) float Bs[GT_N][LDA_F];  // Bs[n][k]
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
@@ -175,7 +236,7 @@ __global__ void __launch_bounds__(256) k_gemm_f32(const float* __restrict__ A, i
     }
     __syncthreads();
   }
-  store_tile(acc, m0, n0, wm, wn, lane, C, ldc, M, N, bias, rowflag, period);
+  store_tile<FUSE>(acc, m0, n0, wm, wn, lane, C, ldc, M, N, bias, rowflag, period, ep);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -186,7 +247,7 @@ __global__ void __launch_bounds__(256) k_gemm_f32(const float* __restrict__ A, i
 // quarter (2 x 2 MFMA 32x32x16 tiles) per wavefront; K staged 32 at a time with the next
 // chunk prefetched into registers.  Requires K % 32 == 0 and lda % 4 == 0.
 // ---------------------------------------------------------------------------------------
-template <int NP, int BM, int BN>
+template <int NP, int BM, int BN, bool FUSE = false>
 __global__ void __launch_bounds__(BM * BN / 64) k_gemm_bf16(const float* __restrict__ A, int64_t lda,
                                                            const uint16_t* __restrict__ Wp0,
                                                            const uint16_t* __restrict__ Wp1,
@@ -194,7 +255,7 @@ __global__ void __launch_bounds__(BM * BN / 64) k_gemm_bf16(const float* __restr
                                                            const float* __restrict__ bias, float* __restrict__ C,
                                                            int64_t ldc, int64_t M, int N, int K,
                                                            const uint8_t* __restrict__ rowflag, int period, int nT,
-                                                           DevRows dr) {
+                                                           DevRows dr, ActEpi ep) {
   constexpr int NWN = BN / 64;
   constexpr int NT = (BM / 64) * NWN * 64;  // threads
   constexpr int A_PER = BM * 8 / NT;        // float4 of the A chunk per thread
@@ -319,7 +380,7 @@ __global__ void __launch_bounds__(BM * BN / 64) k_gemm_bf16(const float* __restr
     compute();
     __syncthreads();
   }
-  store_tile(acc, m0, n0, wm, wn, lane, C, ldc, M, N, bias, rowflag, period);
+  store_tile<FUSE>(acc, m0, n0, wm, wn, lane, C, ldc, M, N, bias, rowflag, period, ep);
 }
 
 // Small-N FP32 GEMM (N <= 16): one wavefront per row, lanes across K.  W is [K,N].
@@ -439,8 +500,10 @@ void launch_gemm_f32_tail(hipStream_t s, const float* A, int64_t lda, const floa
 }
 
 void launch_gemm_f32(hipStream_t s, const float* A, int64_t lda, const float* W, const float* Wt, const float* bias,
-                     float* C, int64_t ldc, int64_t M, int N, int K, const uint8_t* rowflag, int period, DevRows dr) {
+                     float* C, int64_t ldc, int64_t M, int N, int K, const uint8_t* rowflag, int period, DevRows dr,
+                     ActEpi ep) {
   if (M == 0) return;
+  if (ep.rowgrp && (N <= 16 || (N % GT_N) != 0)) abort();  // the fused epilogue needs whole 128-wide tiles
   if (N <= 16) {
     const int64_t threads = M * 64;
     k_gemm_smalln<<<(unsigned)((threads + 255) / 256), 256, 0, s>>>(A, lda, W, bias, C, ldc, M, N, K, rowflag, period,
@@ -451,12 +514,21 @@ void launch_gemm_f32(hipStream_t s, const float* A, int64_t lda, const float* W,
   const int rem = N % GT_N, Nm = rem > 0 && rem <= 16 ? N - rem : N;
   const int nT = (Nm + GT_N - 1) / GT_N;
   const int64_t blocks = ((M + GT_M - 1) / GT_M) * nT;
+  if (ep.rowgrp) {
+    if ((K % GT_K) == 0 && (lda % 4) == 0)
+      k_gemm_f32<true, true><<<(unsigned)blocks, 256, 0, s>>>(A, lda, Wt, bias, C, ldc, M, Nm, K, rowflag, period, nT,
+                                                              dr, 0, 0, 0, ep);
+    else
+      k_gemm_f32<false, true><<<(unsigned)blocks, 256, 0, s>>>(A, lda, Wt, bias, C, ldc, M, Nm, K, rowflag, period, nT,
+                                                               dr, 0, 0, 0, ep);
+    return;
+  }
   if ((K % GT_K) == 0 && (lda % 4) == 0)
     k_gemm_f32<true><<<(unsigned)blocks, 256, 0, s>>>(A, lda, Wt, bias, C, ldc, M, Nm, K, rowflag, period, nT, dr, 0,
-                                                      0, 0);
+                                                      0, 0, ep);
   else
     k_gemm_f32<false><<<(unsigned)blocks, 256, 0, s>>>(A, lda, Wt, bias, C, ldc, M, Nm, K, rowflag, period, nT, dr, 0,
-                                                       0, 0);
+                                                       0, 0, ep);
   if (Nm < N) launch_gemm_f32_tail(s, A, lda, Wt, bias, C, ldc, M, Nm, N - Nm, K, rowflag, period, dr, 0, 0, 0, 1);
 }
 
@@ -469,10 +541,10 @@ void launch_gemm_f32_batched(hipStream_t s, const float* A, int64_t lda, int64_t
   const int nT = (Nm + GT_N - 1) / GT_N;
   const dim3 grid((unsigned)(((M + GT_M - 1) / GT_M) * nT), 1, (unsigned)nb);
   if ((K % GT_K) == 0 && (lda % 4) == 0)
-    k_gemm_f32<true><<<grid, 256, 0, s>>>(A, lda, Wt, nullptr, C, ldc, M, Nm, K, nullptr, 0, nT, DevRows(), sA, sW, sC);
+    k_gemm_f32<true><<<grid, 256, 0, s>>>(A, lda, Wt, nullptr, C, ldc, M, Nm, K, nullptr, 0, nT, DevRows(), sA, sW, sC, ActEpi());
   else
     k_gemm_f32<false><<<grid, 256, 0, s>>>(A, lda, Wt, nullptr, C, ldc, M, Nm, K, nullptr, 0, nT, DevRows(), sA, sW,
-                                           sC);
+                                           sC, ActEpi());
   if (Nm < N)
     launch_gemm_f32_tail(s, A, lda, Wt, nullptr, C, ldc, M, Nm, N - Nm, K, nullptr, 0, DevRows(), sA, sW, sC, nb);
 }
@@ -489,7 +561,7 @@ __global__ void __launch_bounds__(BM * BN / 64) k_gemm_bf16_db(const float* __re
                                                               const float* __restrict__ bias, float* __restrict__ C,
                                                               int64_t ldc, int64_t M, int N, int K,
                                                               const uint8_t* __restrict__ rowflag, int period, int nT,
-                                                              DevRows dr) {
+                                                              DevRows dr, ActEpi ep) {
   constexpr int NWN = BN / 64;
   constexpr int NT = (BM / 64) * NWN * 64;
   constexpr int LDH = KC + 8;                // halves per LDS row (16-byte aligned, conflict-free reads)
@@ -628,7 +700,7 @@ __global__ void __launch_bounds__(BM * BN / 64) k_gemm_bf16_db(const float* __re
       __syncthreads();
     }
   }
-  store_tile(acc, m0, n0, wm, wn, lane, C, ldc, M, N, bias, rowflag, period);
+  store_tile<false>(acc, m0, n0, wm, wn, lane, C, ldc, M, N, bias, rowflag, period, ep);
 }
 
 // Deep-A form of the double-buffered kernel: the A chunks (HBM) run PA chunks ahead of their
@@ -636,7 +708,7 @@ __global__ void __launch_bounds__(BM * BN / 64) k_gemm_bf16_db(const float* __re
 // ahead.  With A_PER float4 per thread and chunk the ring costs 4 PA A_PER registers; the
 // bytes of A in flight per CU grow PA-fold, which is what bounds the staged forms on the
 // tall, narrow contractions of the tangent rows (a few hundred columns, K = 256).
-template <int NP, int BM, int BN, int KC, int PA>
+template <int NP, int BM, int BN, int KC, int PA, bool FUSE = false>
 __global__ void __launch_bounds__(BM * BN / 64) k_gemm_bf16_dq(const float* __restrict__ A, int64_t lda,
                                                               const uint16_t* __restrict__ Wp0,
                                                               const uint16_t* __restrict__ Wp1,
@@ -644,7 +716,7 @@ __global__ void __launch_bounds__(BM * BN / 64) k_gemm_bf16_dq(const float* __re
                                                               const float* __restrict__ bias, float* __restrict__ C,
                                                               int64_t ldc, int64_t M, int N, int K,
                                                               const uint8_t* __restrict__ rowflag, int period, int nT,
-                                                              DevRows dr) {
+                                                              DevRows dr, ActEpi ep) {
   constexpr int NWN = BN / 64;
   constexpr int NT = (BM / 64) * NWN * 64;
   constexpr int LDH = KC + 8;
@@ -775,14 +847,14 @@ __global__ void __launch_bounds__(BM * BN / 64) k_gemm_bf16_dq(const float* __re
       }
     }
   }
-  store_tile(acc, m0, n0, wm, wn, lane, C, ldc, M, N, bias, rowflag, period);
+  store_tile<FUSE>(acc, m0, n0, wm, wn, lane, C, ldc, M, N, bias, rowflag, period, ep);
 }
 
 // Wide-wavefront form of the deep-A kernel: each wavefront computes 64 x 128 (2 x 4 MFMA
 // tiles, 128 accumulators), so a K step reads 6 NP operand fragments from LDS for 8 tiles
 // instead of 4 NP for 4, and a 256 x 256 block stages each weight chunk once for 256 rows.
 // Both cut the LDS traffic per output, which bounds the 64 x 64 forms.
-template <int NP, int BM, int BN, int KC, int PA>
+template <int NP, int BM, int BN, int KC, int PA, bool FUSE = false>
 __global__ void __launch_bounds__(BM * BN / 128) __attribute__((amdgpu_waves_per_eu(2))) k_gemm_bf16_dw(const float* __restrict__ A, int64_t lda,
                                                               const uint16_t* __restrict__ Wp0,
                                                               const uint16_t* __restrict__ Wp1,
@@ -790,7 +862,7 @@ __global__ void __launch_bounds__(BM * BN / 128) __attribute__((amdgpu_waves_per
                                                               const float* __restrict__ bias, float* __restrict__ C,
                                                               int64_t ldc, int64_t M, int N, int K,
                                                               const uint8_t* __restrict__ rowflag, int period, int nT,
-                                                              DevRows dr) {
+                                                              DevRows dr, ActEpi ep) {
   constexpr int NWN = BN / 128;
   constexpr int NT = (BM / 64) * NWN * 64;
   constexpr int LDH = KC + 8;
@@ -922,22 +994,7 @@ __global__ void __launch_bounds__(BM * BN / 128) __attribute__((amdgpu_waves_per
       }
     }
   }
-#pragma unroll
-  for (int b = 0; b < 4; ++b) {
-    const int col = n0 + wn + b * 32 + (lane & 31);
-    if (col >= N) continue;
-    const float bcol = bias ? bias[col] : 0.f;
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-      for (int v = 0; v < 16; ++v) {
-        const int64_t row = m0 + wm + a * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
-        if (row >= M) continue;
-        float val = acc[a][b][v];
-        if (bias && is_bias_row(row, rowflag, period)) val += bcol;
-        C[row * ldc + col] = val;
-      }
-  }
+  store_rows<4, FUSE>(acc, m0, n0, wm, wn, lane, C, ldc, M, N, bias, rowflag, period, ep);
 }
 
 // A-in-registers form: each wavefront computes a 64 x WN tile; it loads its A fragments
@@ -956,7 +1013,7 @@ __global__ void __launch_bounds__(NWM * NWN * 64) __attribute__((amdgpu_waves_pe
                                                            const float* __restrict__ bias, float* __restrict__ C,
                                                            int64_t ldc, int64_t M, int N, int K,
                                                            const uint8_t* __restrict__ rowflag, int period, int nT,
-                                                           DevRows dr) {
+                                                           DevRows dr, ActEpi ep) {
   constexpr int NT = NWM * NWN * 64, BM = 64 * NWM, BN = WN * NWN, NB = WN / 32;
   constexpr int KC = 16, LDH = KC + 8;     // halves per LDS row: 48 bytes, 16-byte aligned
   constexpr int B_UNITS = BN * (KC / 8);   // uint4 per plane and chunk
@@ -1082,22 +1139,7 @@ __global__ void __launch_bounds__(NWM * NWN * 64) __attribute__((amdgpu_waves_pe
       }
     }
   }
-#pragma unroll
-  for (int b = 0; b < NB; ++b) {
-    const int col = n0 + wn + b * 32 + (lane & 31);
-    if (col >= N) continue;
-    const float bcol = bias ? bias[col] : 0.f;
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-      for (int v = 0; v < 16; ++v) {
-        const int64_t row = m0 + wm + a * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
-        if (row >= M) continue;
-        float val = acc[a][b][v];
-        if (bias && is_bias_row(row, rowflag, period)) val += bcol;
-        C[row * ldc + col] = val;
-      }
-  }
+  store_rows<NB, false>(acc, m0, n0, wm, wn, lane, C, ldc, M, N, bias, rowflag, period, ep);
 }
 
 // Tile shapes offered to the autotuner (index = variant).
@@ -1114,6 +1156,8 @@ static const Bf16Tile kBf16Tiles[] = {{128, 128}, {128, 256}, {256, 128}, {64, 2
                                       {128, 256}, {128, 256}, {256, 128}, {128, 256}, {128, 128}, {256, 256},
                                       {128, 256}, {256, 128}};
 
+bool gemm_bf16_fusable(int variant) { return variant == 0 || variant == 2 || variant == 18 || variant == 20 || variant == 23; }
+
 int gemm_bf16_variants() { return (int)(sizeof(kBf16Tiles) / sizeof(kBf16Tiles[0])); }
 void gemm_bf16_tile(int variant, int* bm, int* bn) {
   const Bf16Tile t = kBf16Tiles[(variant >= 0 && variant < gemm_bf16_variants()) ? variant : 0];
@@ -1123,36 +1167,36 @@ void gemm_bf16_tile(int variant, int* bm, int* bn) {
 
 constexpr size_t kMaxLds = 160 * 1024;  // LDS of one CU
 
-template <int NP, int BM, int BN>
+template <int NP, int BM, int BN, bool FUSE = false>
 static void launch_tile(hipStream_t s, const float* A, int64_t lda, const uint16_t* const* W, const float* bias,
                         float* C, int64_t ldc, int64_t M, int N, int K, const uint8_t* rowflag, int period,
-                        DevRows dr) {
+                        DevRows dr, ActEpi ep) {
   static bool attr_set = false;  // LDS above 64 KB must be enabled once per kernel
   const size_t lds = (size_t)NP * (BM + BN) * LDB_H * 2;
   if (lds > kMaxLds) {  // does not fit one CU: the base tile (same result) is launched instead
-    launch_tile<NP, 128, 128>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr);
+    launch_tile<NP, 128, 128, FUSE>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep);
     return;
   }
   if (!attr_set) {
     if (lds > 64 * 1024)
-      (void)hipFuncSetAttribute((const void*)k_gemm_bf16<NP, BM, BN>, hipFuncAttributeMaxDynamicSharedMemorySize,
+      (void)hipFuncSetAttribute((const void*)k_gemm_bf16<NP, BM, BN, FUSE>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)lds);
     attr_set = true;
   }
   const int nT = (N + BN - 1) / BN;
   const int64_t blocks = ((M + BM - 1) / BM) * nT;
-  k_gemm_bf16<NP, BM, BN><<<(unsigned)blocks, BM * BN / 64, lds, s>>>(A, lda, W[0], W[1], W[2], bias, C, ldc, M, N, K,
-                                                                     rowflag, period, nT, dr);
+  k_gemm_bf16<NP, BM, BN, FUSE><<<(unsigned)blocks, BM * BN / 64, lds, s>>>(A, lda, W[0], W[1], W[2], bias, C, ldc, M, N, K,
+                                                                     rowflag, period, nT, dr, ep);
 }
 
 template <int NP, int BM, int BN, int KC, int PD = 1>
 static void launch_tile_db(hipStream_t s, const float* A, int64_t lda, const uint16_t* const* W, const float* bias,
                            float* C, int64_t ldc, int64_t M, int N, int K, const uint8_t* rowflag, int period,
-                           DevRows dr) {
+                           DevRows dr, ActEpi ep) {
   static bool attr_set = false;
   const size_t lds = (size_t)2 * NP * (BM + BN) * (KC + 8) * 2;
   if (lds > kMaxLds) {
-    launch_tile<NP, 128, 128>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr);
+    launch_tile<NP, 128, 128>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep);
     return;
   }
   if (!attr_set) {
@@ -1164,108 +1208,118 @@ static void launch_tile_db(hipStream_t s, const float* A, int64_t lda, const uin
   const int nT = (N + BN - 1) / BN;
   const int64_t blocks = ((M + BM - 1) / BM) * nT;
   k_gemm_bf16_db<NP, BM, BN, KC, PD><<<(unsigned)blocks, BM * BN / 64, lds, s>>>(A, lda, W[0], W[1], W[2], bias, C,
-                                                                               ldc, M, N, K, rowflag, period, nT, dr);
+                                                                               ldc, M, N, K, rowflag, period, nT, dr, ep);
 }
 
-template <int NP, int BM, int BN, int KC, int PA>
+template <int NP, int BM, int BN, int KC, int PA, bool FUSE = false>
 static void launch_tile_dq(hipStream_t s, const float* A, int64_t lda, const uint16_t* const* W, const float* bias,
                            float* C, int64_t ldc, int64_t M, int N, int K, const uint8_t* rowflag, int period,
-                           DevRows dr) {
+                           DevRows dr, ActEpi ep) {
   static bool attr_set = false;
   const size_t lds = (size_t)2 * NP * (BM + BN) * (KC + 8) * 2;
   if (lds > kMaxLds) {
-    launch_tile<NP, 128, 128>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr);
+    launch_tile<NP, 128, 128, FUSE>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep);
     return;
   }
   if (!attr_set) {
     if (lds > 64 * 1024)
-      (void)hipFuncSetAttribute((const void*)k_gemm_bf16_dq<NP, BM, BN, KC, PA>,
+      (void)hipFuncSetAttribute((const void*)k_gemm_bf16_dq<NP, BM, BN, KC, PA, FUSE>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     attr_set = true;
   }
   const int nT = (N + BN - 1) / BN;
   const int64_t blocks = ((M + BM - 1) / BM) * nT;
-  k_gemm_bf16_dq<NP, BM, BN, KC, PA><<<(unsigned)blocks, BM * BN / 64, lds, s>>>(A, lda, W[0], W[1], W[2], bias, C,
-                                                                               ldc, M, N, K, rowflag, period, nT, dr);
+  k_gemm_bf16_dq<NP, BM, BN, KC, PA, FUSE><<<(unsigned)blocks, BM * BN / 64, lds, s>>>(A, lda, W[0], W[1], W[2], bias, C,
+                                                                               ldc, M, N, K, rowflag, period, nT, dr, ep);
 }
 
-template <int NP, int BM, int BN, int KC, int PA>
+template <int NP, int BM, int BN, int KC, int PA, bool FUSE = false>
 static void launch_tile_dw(hipStream_t s, const float* A, int64_t lda, const uint16_t* const* W, const float* bias,
                            float* C, int64_t ldc, int64_t M, int N, int K, const uint8_t* rowflag, int period,
-                           DevRows dr) {
+                           DevRows dr, ActEpi ep) {
   static bool attr_set = false;
   const size_t lds = (size_t)2 * NP * (BM + BN) * (KC + 8) * 2;
   if (lds > kMaxLds) {
-    launch_tile<NP, 128, 128>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr);
+    launch_tile<NP, 128, 128, FUSE>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep);
     return;
   }
   if (!attr_set) {
     if (lds > 64 * 1024)
-      (void)hipFuncSetAttribute((const void*)k_gemm_bf16_dw<NP, BM, BN, KC, PA>,
+      (void)hipFuncSetAttribute((const void*)k_gemm_bf16_dw<NP, BM, BN, KC, PA, FUSE>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     attr_set = true;
   }
   const int nT = (N + BN - 1) / BN;
   const int64_t blocks = ((M + BM - 1) / BM) * nT;
-  k_gemm_bf16_dw<NP, BM, BN, KC, PA><<<(unsigned)blocks, BM * BN / 128, lds, s>>>(A, lda, W[0], W[1], W[2], bias, C,
-                                                                                ldc, M, N, K, rowflag, period, nT, dr);
+  k_gemm_bf16_dw<NP, BM, BN, KC, PA, FUSE><<<(unsigned)blocks, BM * BN / 128, lds, s>>>(A, lda, W[0], W[1], W[2], bias, C,
+                                                                                ldc, M, N, K, rowflag, period, nT, dr, ep);
 }
 
 template <int NP, int NWM, int NWN, int WN>
 static void launch_tile_ar(hipStream_t s, const float* A, int64_t lda, const uint16_t* const* W, const float* bias,
                            float* C, int64_t ldc, int64_t M, int N, int K, const uint8_t* rowflag, int period,
-                           DevRows dr) {
+                           DevRows dr, ActEpi ep) {
   const int nT = (N + WN * NWN - 1) / (WN * NWN);
   const int64_t blocks = ((M + 64 * NWM - 1) / (64 * NWM)) * nT;
   k_gemm_bf16_ar<NP, NWM, NWN, WN><<<(unsigned)blocks, NWM * NWN * 64, 0, s>>>(A, lda, W[0], W[1], W[2], bias, C, ldc,
-                                                                              M, N, K, rowflag, period, nT, dr);
+                                                                              M, N, K, rowflag, period, nT, dr, ep);
 }
 
 template <int NP>
 static void launch_np(hipStream_t s, int variant, const float* A, int64_t lda, const uint16_t* const* W,
                       const float* bias, float* C, int64_t ldc, int64_t M, int N, int K, const uint8_t* rowflag,
-                      int period, DevRows dr) {
+                      int period, DevRows dr, ActEpi ep) {
+  if (ep.rowgrp) {  // fused silu epilogue: the shapes of gemm_bf16_fusable
+    switch (variant) {
+      case 2: launch_tile<NP, 256, 128, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 18: launch_tile_dq<NP, 128, 256, 16, 4, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 20: launch_tile_dq<NP, 256, 128, 16, 4, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 23: launch_tile_dw<NP, 256, 256, 16, 4, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      default: launch_tile<NP, 128, 128, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+    }
+    return;
+  }
   switch (variant) {
-    case 7: launch_tile_db<NP, 128, 128, 16>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
-    case 8: launch_tile_db<NP, 128, 128, 32>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
-    case 9: launch_tile_db<NP, 256, 128, 16>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
-    case 10: launch_tile_db<NP, 128, 256, 16>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
-    case 11: launch_tile_db<NP, 256, 128, 16, 2>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
-    case 12: launch_tile_db<NP, 128, 256, 16, 2>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
-    case 13: launch_tile_db<NP, 128, 128, 32, 2>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
-    case 14: launch_tile_db<NP, 128, 128, 16, 2>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
-    case 15: launch_tile_ar<NP, 2, 2, 128>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
-    case 16: launch_tile_ar<NP, 4, 1, 128>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
-    case 17: launch_tile_ar<NP, 1, 4, 64>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
-    case 18: launch_tile_dq<NP, 128, 256, 16, 4>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
-    case 19: launch_tile_dq<NP, 128, 256, 16, 6>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
-    case 20: launch_tile_dq<NP, 256, 128, 16, 4>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
-    case 21: launch_tile_dq<NP, 128, 256, 32, 3>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
-    case 22: launch_tile_dq<NP, 128, 128, 16, 6>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
-    case 23: launch_tile_dw<NP, 256, 256, 16, 4>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
-    case 24: launch_tile_dw<NP, 128, 256, 16, 4>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
-    case 25: launch_tile_dw<NP, 256, 128, 16, 4>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
-    case 1: launch_tile<NP, 128, 256>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
-    case 2: launch_tile<NP, 256, 128>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
-    case 3: launch_tile<NP, 64, 256>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
-    case 4: launch_tile<NP, 256, 64>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
-    case 5: launch_tile<NP, 64, 128>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
-    case 6: launch_tile<NP, 128, 64>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
-    default: launch_tile<NP, 128, 128>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr); break;
+    case 7: launch_tile_db<NP, 128, 128, 16>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+    case 8: launch_tile_db<NP, 128, 128, 32>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+    case 9: launch_tile_db<NP, 256, 128, 16>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+    case 10: launch_tile_db<NP, 128, 256, 16>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+    case 11: launch_tile_db<NP, 256, 128, 16, 2>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+    case 12: launch_tile_db<NP, 128, 256, 16, 2>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+    case 13: launch_tile_db<NP, 128, 128, 32, 2>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+    case 14: launch_tile_db<NP, 128, 128, 16, 2>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+    case 15: launch_tile_ar<NP, 2, 2, 128>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+    case 16: launch_tile_ar<NP, 4, 1, 128>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+    case 17: launch_tile_ar<NP, 1, 4, 64>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+    case 18: launch_tile_dq<NP, 128, 256, 16, 4>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+    case 19: launch_tile_dq<NP, 128, 256, 16, 6>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+    case 20: launch_tile_dq<NP, 256, 128, 16, 4>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+    case 21: launch_tile_dq<NP, 128, 256, 32, 3>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+    case 22: launch_tile_dq<NP, 128, 128, 16, 6>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+    case 23: launch_tile_dw<NP, 256, 256, 16, 4>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+    case 24: launch_tile_dw<NP, 128, 256, 16, 4>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+    case 25: launch_tile_dw<NP, 256, 128, 16, 4>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+    case 1: launch_tile<NP, 128, 256>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+    case 2: launch_tile<NP, 256, 128>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+    case 3: launch_tile<NP, 64, 256>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+    case 4: launch_tile<NP, 256, 64>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+    case 5: launch_tile<NP, 64, 128>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+    case 6: launch_tile<NP, 128, 64>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+    default: launch_tile<NP, 128, 128>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
   }
 }
 
 void launch_gemm_bf16(hipStream_t s, const float* A, int64_t lda, const uint16_t* Whi, const uint16_t* Wlo,
                       const uint16_t* Wlo2, const float* bias, float* C, int64_t ldc, int64_t M, int N, int K,
-                      const uint8_t* rowflag, int period, int nsplit, int variant, DevRows dr) {
+                      const uint8_t* rowflag, int period, int nsplit, int variant, DevRows dr, ActEpi ep) {
   if (M == 0) return;
   const uint16_t* W[3] = {Whi, Wlo ? Wlo : Whi, Wlo2 ? Wlo2 : Whi};
   if (nsplit == 6)
-    launch_np<3>(s, variant, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr);
+    launch_np<3>(s, variant, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep);
   else if (nsplit == 3)
-    launch_np<2>(s, variant, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr);
+    launch_np<2>(s, variant, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep);
   else
-    launch_np<1>(s, variant, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr);
+    launch_np<1>(s, variant, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep);
 }
 
 }  // namespace fire

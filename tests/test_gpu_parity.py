@@ -150,3 +150,31 @@ def test_local_energy(setup, precision):
             assert eg < 5e-4 and el < 2e-3 and ek < 2e-3
         else:
             assert eg < 5e-3 and el < 1e-2 and ek < 1e-2
+
+
+@pytest.mark.parametrize("precision", ["fp32", "mixed"])
+def test_fused_silu_epilogue_matches_separate_pass(gpu_device, precision, monkeypatch):
+    """The silu of the tangent-row MLP layers applied inside the contraction epilogue (the
+    default) against the separate element-wise pass (FIRE_FUSED_ACT=0), same walkers: they differ
+    only in the order of the sum of squares over each group's jacobian rows and in where silu'
+    of the value row is taken from (a separate FP32 contraction of the value rows)."""
+    cfg = alkane_cfg()
+    prm = make_params(cfg, seed=0)
+    r = walkers(cfg, 4, seed=3)
+    outs = []
+    for flag in ("1", "0"):
+        monkeypatch.setenv("FIRE_FUSED_ACT", flag)
+        wf = FireWaveFunction(cfg, device=gpu_device)
+        wf.upload(prm)
+        outs.append(wf.local_energy_terms(prm, r, precision=precision, with_derivatives=True))
+    fused, sep = outs
+    tol = 2e-5 if precision == "fp32" else 2e-4
+    for key in ("logpsi", "lap", "e_kin"):
+        x = fused[key].double().cpu().numpy()
+        y = sep[key].double().cpu().numpy()
+        d = np.abs(x - y) / np.maximum(1.0, np.abs(y))
+        print(f"  {precision} {key}: max rel {d.max():.2e}")
+        assert d.max() < tol
+    g = _rel(fused["grad"].cpu().numpy(), sep["grad"].cpu().numpy())
+    print(f"  {precision} grad rel {g:.2e}")
+    assert g < tol
