@@ -89,6 +89,10 @@ struct fire_handle {
   bool params_ok = false;
   // tensor indices
   int iEN, iEEs, iEEd, iMLP, iORB, iJM, iATT, iCUSP, iSCAL;
+  // tensors derived at upload (after the uploaded ones): the value half of the attention
+  // projection W_v [F,HD], b_v [HD], and the key-weighted head sums of its query half
+  // W_a [F,H] (W_a[:,h] = sum_k W_q[:,h Dk + k] key[h Dk + k]), b_a [H]
+  int iAV = -1;
   // workspace
   DevBuf cnt, ptr, nbr, rev, rowflag, rowgrp;
   DevBuf AV0, Z0, F1, PART;  // fused silu epilogue: packed value rows, their pre-activations, silu', slots
@@ -262,7 +266,7 @@ void gemm_weights(fire_handle* h, std::vector<std::pair<int, std::pair<int, int>
     }
   }
   if (h->has_att) {
-    out.push_back({h->iATT + 0, {F, 2 * HD}});
+    out.push_back({h->iAV, {F, HD}});
     int fan = 2 * HD;
     for (int l = 0; l < h->depth; ++l) {
       out.push_back({h->iATT + 3 + 2 * l, {fan, W}});
@@ -284,7 +288,7 @@ int gemm_class(const fire_handle* h, int wi) {
   if (wi < h->iORB) return 1;
   if (wi < h->iJM) return 2;
   if (wi < h->iATT) return 3;
-  return wi == h->iATT ? 4 : 5;
+  return wi == h->iATT || wi == h->iAV ? 4 : 5;
 }
 
 // C = A @ W(tensor wi) + bias(tensor bi, optional) on value rows
@@ -690,8 +694,15 @@ void run_jastrow_and_assemble(Ctx& c, const float* r, int B, bool tan, const Gro
     float* QV = h->O.as<float>();  // orbital rows are no longer needed
     h->AV.ensure(sizeof(float) * (size_t)R * nc);
     h->att_ns.ensure(sizeof(double) * (size_t)B * 2 * 2 * nc);
-    gemm(c, h->XA.as<float>(), F, ac + 0, ac + 1, QV, 2 * HD, R, 2 * HD, F, flag, period);
-    launch_att_group(c.s, QV, H, h->Dk, G, gd, h->P[ac + 2], tan, h->AV.as<float>());
+    // value half through the split contraction; the H key-weighted head sums of the query half
+    // (W_a = W_q key per head) in FP32 behind it
+    const int av = h->iAV;
+    float* QA = QV + (size_t)R * HD;
+    gemm(c, h->XA.as<float>(), F, av, av + 1, QV, HD, R, HD, F, flag, period);
+    for (int n0 = 0; n0 < H; n0 += 16)
+      launch_gemm_f32_tail(c.s, h->XA.as<float>(), F, h->Pt32[av + 2], h->P[av + 3], QA, H, R, n0, std::min(16, H - n0),
+                           F, flag, period, DevRows(), 0, 0, 0, 1);
+    launch_att_group(c.s, QV, QA, H, h->Dk, G, gd, tan, h->AV.as<float>());
     launch_node_sums(c.s, h->AV.as<float>(), nc, nc, gd, B, N, h->n_up, h->att_ns.as<double>());
     const int nrow = tan ? 2 + 3 * N : 1;
     if (tan) {
@@ -970,9 +981,13 @@ void state_propose(Ctx& c, int k, int mode, const float* rk_in, float width, uin
   const float* att_scale = nullptr;
   const float* att_zb = nullptr;
   if (h->has_att) {
-    const int ac = h->iATT;
-    gemm(cc, S.p_h.as<float>(), F, ac + 0, ac + 1, S.p_qv.as<float>(), 2 * HD, T, 2 * HD, F, nullptr, 1, rT);
-    launch_att_group(c.s, S.p_qv.as<float>(), H, Dk, (int)T, g1, h->P[ac + 2], false, S.p_av.as<float>(), dT);
+    const int av = h->iAV;
+    float* qa = S.p_qv.as<float>() + (size_t)T * HD;
+    gemm(cc, S.p_h.as<float>(), F, av, av + 1, S.p_qv.as<float>(), HD, T, HD, F, nullptr, 1, rT);
+    for (int n0 = 0; n0 < H; n0 += 16)
+      launch_gemm_f32_tail(c.s, S.p_h.as<float>(), F, h->Pt32[av + 2], h->P[av + 3], qa, H, T, n0, std::min(16, H - n0),
+                           F, nullptr, 1, rT, 0, 0, 0, 1);
+    launch_att_group(c.s, S.p_qv.as<float>(), qa, H, Dk, (int)T, g1, false, S.p_av.as<float>(), dT);
   }
   if (h->has_mlp || h->has_att)
     launch_mc_jsum(c.s, S.jr.as<float>(), S.p_jr.as<float>(), S.av.as<float>(), S.p_av.as<float>(),
@@ -1217,7 +1232,8 @@ int fire_destroy(fire_handle* h) {
                     &h->XA, &h->XB, &h->XC, &h->O, &h->ENV, &h->ENVC, &h->A64, &h->Ainv, &h->AinvT, &h->ldet, &h->dsgn,
                     &h->tr, &h->trip, &h->dgrad, &h->det_lp, &h->det_sg, &h->det_gr, &h->det_lap, &h->JR,
                     &h->mlp_ns, &h->mlp_dg, &h->QV, &h->AV, &h->att_ns, &h->att_dg, &h->RIN, &h->RY1,
-                    &h->RY2, &h->att_out, &h->cval, &h->clap, &h->cgrad, &h->epot};
+                    &h->RY2, &h->att_out, &h->cval, &h->clap, &h->cgrad, &h->epot, &h->PF, &h->EF, &h->EOK,
+                    &h->rowgrp, &h->AV0, &h->Z0, &h->F1, &h->PART};
   for (DevBuf* b : bufs) b->release();
   h->st.release();
   delete h;
@@ -1243,31 +1259,63 @@ int fire_upload_params(fire_handle* h, const float* packed, const int64_t* sizes
       if (p) HIPCHK(hipFree(p));
     for (float* p : h->Pt32)
       if (p) HIPCHK(hipFree(p));
-    h->P.assign(n_tensors, nullptr);
-    h->Pbf.assign(n_tensors, nullptr);
-    h->Pbf_lo.assign(n_tensors, nullptr);
-    h->Pbf_lo2.assign(n_tensors, nullptr);
-    h->Pt32.assign(n_tensors, nullptr);
+    const int n_all = n_tensors + (h->has_att ? 4 : 0);
+    h->P.assign(n_all, nullptr);
+    h->Pbf.assign(n_all, nullptr);
+    h->Pbf_lo.assign(n_all, nullptr);
+    h->Pbf_lo2.assign(n_all, nullptr);
+    h->Pt32.assign(n_all, nullptr);
+    // host copies of every tensor (uploaded, then derived)
+    std::vector<const float*> host(n_all, nullptr);
+    std::vector<int64_t> hsize(n_all, 0);
     size_t off = 0;
     for (int k = 0; k < n_tensors; ++k) {
-      HIPCHK(hipMalloc(&h->P[k], sizeof(float) * std::max<int64_t>(sizes[k], 1)));
-      HIPCHK(hipMemcpy(h->P[k], packed + off, sizeof(float) * sizes[k], hipMemcpyHostToDevice));
+      host[k] = packed + off;
+      hsize[k] = sizes[k];
       off += sizes[k];
+    }
+    std::vector<float> derived[4];
+    if (h->has_att) {
+      const int F = h->F, H = h->H, Dk = h->Dk, HD = h->HD;
+      const float* wq = host[h->iATT];  // [F, 2 HD]: query columns, then value columns
+      const float* bq = host[h->iATT + 1];
+      const float* key = host[h->iATT + 2];
+      derived[0].resize((size_t)F * HD);
+      derived[1].resize(HD);
+      derived[2].resize((size_t)F * H);
+      derived[3].resize(H);
+      for (int f = 0; f < F; ++f) {
+        for (int n = 0; n < HD; ++n) derived[0][(size_t)f * HD + n] = wq[(size_t)f * 2 * HD + HD + n];
+        for (int hh = 0; hh < H; ++hh) {
+          double a = 0.0;
+          for (int k = 0; k < Dk; ++k) a += (double)wq[(size_t)f * 2 * HD + hh * Dk + k] * key[hh * Dk + k];
+          derived[2][(size_t)f * H + hh] = (float)a;
+        }
+      }
+      for (int n = 0; n < HD; ++n) derived[1][n] = bq[HD + n];
+      for (int hh = 0; hh < H; ++hh) {
+        double a = 0.0;
+        for (int k = 0; k < Dk; ++k) a += (double)bq[hh * Dk + k] * key[hh * Dk + k];
+        derived[3][hh] = (float)a;
+      }
+      h->iAV = n_tensors;
+      for (int q = 0; q < 4; ++q) {
+        host[n_tensors + q] = derived[q].data();
+        hsize[n_tensors + q] = (int64_t)derived[q].size();
+      }
+    }
+    for (int k = 0; k < n_all; ++k) {
+      HIPCHK(hipMalloc(&h->P[k], sizeof(float) * std::max<int64_t>(hsize[k], 1)));
+      HIPCHK(hipMemcpy(h->P[k], host[k], sizeof(float) * hsize[k], hipMemcpyHostToDevice));
     }
     std::vector<std::pair<int, std::pair<int, int>>> gw;
     gemm_weights(h, gw);
-    size_t base = 0;
-    std::vector<size_t> offs(n_tensors);
-    for (int k = 0; k < n_tensors; ++k) {
-      offs[k] = base;
-      base += sizes[k];
-    }
     for (auto& t : gw) {
       const int k = t.first, K = t.second.first, Nn = t.second.second;
       // three BF16 planes, transposed to [N,K]: x = hi + mid + lo (remainders exact in FP32)
       std::vector<uint16_t> pl[3];
       for (auto& v : pl) v.resize((size_t)K * Nn);
-      const float* src = packed + offs[k];
+      const float* src = host[k];
       auto bf2f_h = [](uint16_t b) {
         uint32_t u = (uint32_t)b << 16;
         float f;
@@ -1294,7 +1342,7 @@ int fire_upload_params(fire_handle* h, const float* packed, const int64_t* sizes
     // envelope weights pi per determinant ([det][ME][N] -> [det][N][ME])
     auto upload_t = [&](int k, int nb, int K, int Nn) {
       std::vector<float> tt((size_t)nb * K * Nn);
-      const float* src = packed + offs[k];
+      const float* src = host[k];
       for (int b = 0; b < nb; ++b)
         for (int kk = 0; kk < K; ++kk)
           for (int n = 0; n < Nn; ++n)
@@ -1303,14 +1351,15 @@ int fire_upload_params(fire_handle* h, const float* packed, const int64_t* sizes
       HIPCHK(hipMemcpy(h->Pt32[k], tt.data(), sizeof(float) * tt.size(), hipMemcpyHostToDevice));
     };
     for (auto& t : gw) upload_t(t.first, 1, t.second.first, t.second.second);
+    if (h->has_att) upload_t(h->iAV + 2, 1, h->F, h->H);
     for (int k : {h->iORB + 5, h->iORB + 7}) upload_t(k, h->D, h->M * h->Eiso, h->N);
     // a captured sweep graph holds the old parameter pointers
     if (h->st.graph) {
       HIPCHK(hipGraphExecDestroy(h->st.graph));
       h->st.graph = nullptr;
     }
-    h->cut_learned = packed[offs[h->iSCAL]];
-    h->s1 = packed[offs[h->iSCAL] + 1];
+    h->cut_learned = host[h->iSCAL][0];
+    h->s1 = host[h->iSCAL][1];
     h->params_ok = true;
     return 0;
   } catch (const std::exception& e) {

@@ -230,78 +230,66 @@ void launch_dep_gather(hipStream_t s, const float* X, int64_t ld, int ncols, Gro
   k_dep_gather<<<B * N, 256, 0, s>>>(X, ld, ncols, gd, ptr, nbr, rev, B, N, n_up, out);
 }
 
-// Attention Jastrow per group: a = sum_k q[h,k] keys[h,k]; att = exp(-a / sqrt(Dk)) and
-// vals = att * v with forward-Laplacian rows.  QV rows = [q (H*Dk) | v (H*Dk)];
-// AV rows = [att (H) | vals (H*Dk)].  Thread (h, k) owns column h*Dk + k; the head sum is
-// a shuffle reduction over the Dk consecutive lanes of the head (Dk a power of two <= 64),
-// so every row is read once with coalesced loads.
+// Attention Jastrow per group: att = exp(-a / sqrt(Dk)) and vals = att * v with
+// forward-Laplacian rows, where a = sum_k q[h,k] keys[h,k] arrives precomputed per row and
+// head (the key-weighted head sums of the query projection are a linear map of the features,
+// contracted as such).  V rows [H*Dk], QA rows [H]; AV rows = [att (H) | vals (H*Dk)].
+// Thread (h, k) owns column h*Dk + k, so every row is read once with coalesced loads.
 template <bool TAN>
-__global__ void __launch_bounds__(256) k_att_group(const float* __restrict__ QV, int H, int Dk, GroupDesc gd,
-                                                   const float* __restrict__ keys, float* __restrict__ AV,
+__global__ void __launch_bounds__(256) k_att_group(const float* __restrict__ V, const float* __restrict__ QA, int H,
+                                                   int Dk, GroupDesc gd, float* __restrict__ AV,
                                                    const int32_t* __restrict__ dG) {
   const int g = blockIdx.x;
   if (dG && g >= *dG) return;
   const int HD = H * Dk;
   const int col = threadIdx.x;  // (h, k)
-  const bool live = col < HD;
-  const int h = live ? col / Dk : 0, k = live ? col - h * Dk : 1;
+  if (col >= HD) return;
+  const int h = col / Dk, k = col - h * Dk;
   const float sc = rsqrtf((float)Dk);
-  const float key = live ? keys[col] : 0.f;
   const int64_t r0 = gd.row(g), r1 = TAN ? gd.row(g + 1) : r0 + 1;
-  const int ldq = 2 * HD, lda = H + HD;
-  auto head_sum = [&](float p) {
-    for (int o = 1; o < Dk; o <<= 1) p += __shfl_xor(p, o, 64);
-    return p;
-  };
-  auto pre = [&](int64_t rr) { return head_sum(live ? QV[rr * ldq + col] * key : 0.f); };
-  const float ax = pre(r0);
+  const int lda = H + HD;
+  const float ax = QA[r0 * H + h];
   const float f = expf(-sc * ax), f1 = -sc * f, f2 = sc * sc * f;
-  const float vx = live ? QV[r0 * ldq + HD + col] : 0.f;
-  if (live) {
-    AV[r0 * lda + H + col] = f * vx;
-    if (k == 0) AV[r0 * lda + h] = f;
-  }
+  const float vx = V[r0 * HD + col];
+  AV[r0 * lda + H + col] = f * vx;
+  if (k == 0) AV[r0 * lda + h] = f;
   if (!TAN) return;
-  const float a_l = pre(r0 + 1);
+  const float a_l = QA[(r0 + 1) * H + h];
   float ss = 0.f, cross = 0.f;
   auto row = [&](int64_t rr, float a_t, float vt) {
     const float at = a_t * f1;
     ss += a_t * a_t;
-    if (live) {
-      cross += at * vt;
-      AV[rr * lda + H + col] = at * vx + f * vt;
-      if (k == 0) AV[rr * lda + h] = at;
-    }
+    cross += at * vt;
+    AV[rr * lda + H + col] = at * vx + f * vt;
+    if (k == 0) AV[rr * lda + h] = at;
   };
-  // jacobian rows 4 at a time: the rows' Q and V values are loaded before the stores
+  // jacobian rows 4 at a time: the rows' a and V values are loaded before the stores
   int64_t rr = r0 + 2;
   for (; rr + 4 <= r1; rr += 4) {
     float q[4], v[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      q[u] = live ? QV[(rr + u) * ldq + col] : 0.f;
-      v[u] = live ? QV[(rr + u) * ldq + HD + col] : 0.f;
+      q[u] = QA[(rr + u) * H + h];
+      v[u] = V[(rr + u) * HD + col];
     }
 #pragma unroll
-    for (int u = 0; u < 4; ++u) row(rr + u, head_sum(q[u] * key), v[u]);
+    for (int u = 0; u < 4; ++u) row(rr + u, q[u], v[u]);
   }
-  for (; rr < r1; ++rr) row(rr, pre(rr), live ? QV[rr * ldq + HD + col] : 0.f);
-  if (live) {
-    const float al = a_l * f1 + f2 * ss;
-    const float vl = QV[(r0 + 1) * ldq + HD + col];
-    AV[(r0 + 1) * lda + H + col] = al * vx + f * vl + 2.f * cross;
-    if (k == 0) AV[(r0 + 1) * lda + h] = al;
-  }
+  for (; rr < r1; ++rr) row(rr, QA[rr * H + h], V[rr * HD + col]);
+  const float al = a_l * f1 + f2 * ss;
+  const float vl = V[(r0 + 1) * HD + col];
+  AV[(r0 + 1) * lda + H + col] = al * vx + f * vl + 2.f * cross;
+  if (k == 0) AV[(r0 + 1) * lda + h] = al;
 }
 
-void launch_att_group(hipStream_t s, const float* QV, int H, int Dk, int G, GroupDesc gd, const float* keys,
+void launch_att_group(hipStream_t s, const float* V, const float* QA, int H, int Dk, int G, GroupDesc gd,
                       bool tangents, float* AV, const int32_t* dG) {
   if (G == 0) return;
   const int threads = ((H * Dk + 63) / 64) * 64;
   if (tangents)
-    k_att_group<true><<<G, threads, 0, s>>>(QV, H, Dk, gd, keys, AV, dG);
+    k_att_group<true><<<G, threads, 0, s>>>(V, QA, H, Dk, gd, AV, dG);
   else
-    k_att_group<false><<<G, threads, 0, s>>>(QV, H, Dk, gd, keys, AV, dG);
+    k_att_group<false><<<G, threads, 0, s>>>(V, QA, H, Dk, gd, AV, dG);
 }
 
 // Readout input u[s,h,k] = vals[s,h,k] / norm[s,h] with dense forward-Laplacian over the
