@@ -16,6 +16,8 @@
 //               accumulates hi*hi + hi*lo + lo*hi ("split BF16 x3"); NSPLIT = 6 carries
 //               hi + mid + lo (x to ~2^-26) and accumulates the six products whose order is
 //               at most 2^-16 relative (x6, FP32-level products); NSPLIT = 1 is plain BF16.
+#include <type_traits>
+
 #include "fire_common.hpp"
 #include "fire_kernels.hpp"
 
@@ -49,6 +51,16 @@ __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint
 #define LDA_F (GT_K + 4)  // fp32 LDS row stride (floats)
 #define LDB_H (GT_K + 8)  // bf16 LDS row stride (halves): 80 bytes, keeps 16-byte alignment
 #define N_XCD 8
+
+template <int V>
+using IC = std::integral_constant<int, V>;
+
+// Number of the wavefront's 32-column MFMA tiles (of nb) that hold columns below N: the last
+// N tile of a contraction whose N is not a multiple of the tile width multiplies only those.
+__device__ __forceinline__ int live_col_tiles(int c0, int N, int nb) {
+  const int n = (N - c0 + 31) >> 5;
+  return n < 0 ? 0 : (n > nb ? nb : n);
+}
 
 __device__ __forceinline__ bool is_bias_row(int64_t row, const uint8_t* rowflag, int period) {
   if (rowflag) return rowflag[row] != 0;
@@ -337,7 +349,9 @@ __global__ void __launch_bounds__(BM * BN / 64) k_gemm_bf16(const float* __restr
       for (int p = 0; p < NP; ++p) *reinterpret_cast<uint4*>(&Bs[p][nn][kq]) = rb[p][q];
     }
   };
-  auto compute = [&]() {
+  const int nbl = live_col_tiles(n0 + wn, N, 2);
+  auto compute_nb = [&](auto NBL) {
+    constexpr int nbc = decltype(NBL)::value;
 #pragma unroll
     for (int kk = 0; kk < GT_K; kk += 16) {
       const int ka = kk + 8 * (lane >> 5);
@@ -347,7 +361,7 @@ __global__ void __launch_bounds__(BM * BN / 64) k_gemm_bf16(const float* __restr
 #pragma unroll
         for (int a = 0; a < 2; ++a) av[p][a] = *reinterpret_cast<const bf16x8_t*>(&As[p][wm + a * 32 + (lane & 31)][ka]);
 #pragma unroll
-        for (int b = 0; b < 2; ++b) bv[p][b] = *reinterpret_cast<const bf16x8_t*>(&Bs[p][wn + b * 32 + (lane & 31)][ka]);
+        for (int b = 0; b < nbc; ++b) bv[p][b] = *reinterpret_cast<const bf16x8_t*>(&Bs[p][wn + b * 32 + (lane & 31)][ka]);
       }
       // product-major order: four independent accumulators between dependent MFMAs
 #pragma unroll
@@ -355,10 +369,16 @@ __global__ void __launch_bounds__(BM * BN / 64) k_gemm_bf16(const float* __restr
 #pragma unroll
         for (int a = 0; a < 2; ++a)
 #pragma unroll
-          for (int b = 0; b < 2; ++b)
+          for (int b = 0; b < nbc; ++b)
             acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[prod_a(NP, t)][a], bv[prod_b(NP, t)][b],
                                                                 acc[a][b], 0, 0, 0);
     }
+  };
+  auto compute = [&]() {
+    if (nbl == 2)
+      compute_nb(IC<2>());
+    else if (nbl == 1)
+      compute_nb(IC<1>());
   };
 
   const int nch = K / GT_K;
@@ -946,7 +966,9 @@ __global__ void __launch_bounds__(BM * BN / 128) __attribute__((amdgpu_waves_per
       for (int p = 0; p < NP; ++p) *reinterpret_cast<uint4*>(Bs(st, p, nn) + kq) = rb[p][q];
     }
   };
-  auto compute = [&](int st) {
+  const int nbl = live_col_tiles(n0 + wn, N, 4);
+  auto compute_nb = [&](int st, auto NBL) {
+    constexpr int nbc = decltype(NBL)::value;
 #pragma unroll
     for (int kk = 0; kk < KC; kk += 16) {
       const int ka = kk + 8 * (lane >> 5);
@@ -956,7 +978,7 @@ __global__ void __launch_bounds__(BM * BN / 128) __attribute__((amdgpu_waves_per
 #pragma unroll
         for (int a = 0; a < 2; ++a) av[p][a] = *reinterpret_cast<const bf16x8_t*>(As(st, p, wm + a * 32 + (lane & 31)) + ka);
 #pragma unroll
-      for (int b = 0; b < 4; ++b) {
+      for (int b = 0; b < nbc; ++b) {
         bf16x8_t bv[NP];
 #pragma unroll
         for (int p = 0; p < NP; ++p) bv[p] = *reinterpret_cast<const bf16x8_t*>(Bs(st, p, wn + b * 32 + (lane & 31)) + ka);
@@ -968,6 +990,16 @@ __global__ void __launch_bounds__(BM * BN / 128) __attribute__((amdgpu_waves_per
                                                                 0, 0);
       }
     }
+  };
+  auto compute = [&](int st) {
+    if (nbl == 4)
+      compute_nb(st, IC<4>());
+    else if (nbl == 3)
+      compute_nb(st, IC<3>());
+    else if (nbl == 2)
+      compute_nb(st, IC<2>());
+    else if (nbl == 1)
+      compute_nb(st, IC<1>());
   };
   const int nch = K / KC;
   // chunk 0 into stage 0; chunks 1 .. PA in the ring (chunk j in ra[j % PA]); W chunk 1 in rb
