@@ -89,10 +89,12 @@ struct fire_handle {
   bool params_ok = false;
   // tensor indices
   int iEN, iEEs, iEEd, iMLP, iORB, iJM, iATT, iCUSP, iSCAL;
-  // tensors derived at upload (after the uploaded ones): the value half of the attention
-  // projection W_v [F,HD], b_v [HD], and the key-weighted head sums of its query half
-  // W_a [F,H] (W_a[:,h] = sum_k W_q[:,h Dk + k] key[h Dk + k]), b_a [H]
+  // tensors derived at upload (after the uploaded ones): the attention projection as
+  // W_va [F, HDP] = [value half W_v | key-weighted head sums of the query half W_a | 0] with
+  // W_a[:,h] = sum_k W_q[:,h Dk + k] key[h Dk + k], and its bias b_va [HDP];
+  // HDP = HD + H rounded up to 16
   int iAV = -1;
+  int HDP = 0;
   // workspace
   DevBuf cnt, ptr, nbr, rev, rowflag, rowgrp;
   DevBuf AV0, Z0, F1, PART;  // fused silu epilogue: packed value rows, their pre-activations, silu', slots
@@ -266,7 +268,7 @@ void gemm_weights(fire_handle* h, std::vector<std::pair<int, std::pair<int, int>
     }
   }
   if (h->has_att) {
-    out.push_back({h->iAV, {F, HD}});
+    out.push_back({h->iAV, {F, h->HDP}});
     int fan = 2 * HD;
     for (int l = 0; l < h->depth; ++l) {
       out.push_back({h->iATT + 3 + 2 * l, {fan, W}});
@@ -543,7 +545,7 @@ void run_orbitals_det(Ctx& c, const float* r, int B, const Graph& gr, bool tan, 
   // are read (against zero inverse columns) by the determinant Laplacian's vector loads and are
   // zeroed whenever the buffer is allocated (every later writer stores finite values)
   const int ldo = (ND + 31) & ~31;
-  const size_t ocols = std::max(ldo, h->has_att ? 2 * h->HD : 0);
+  const size_t ocols = std::max(ldo, h->has_att ? h->HDP : 0);
   const void* o_prev = h->O.p;
   h->O.ensure(sizeof(float) * (size_t)R * ocols);
   if (h->O.p != o_prev) launch_zero_f32(c.s, h->O.as<float>(), (int64_t)(h->O.bytes / sizeof(float)));
@@ -694,15 +696,11 @@ void run_jastrow_and_assemble(Ctx& c, const float* r, int B, bool tan, const Gro
     float* QV = h->O.as<float>();  // orbital rows are no longer needed
     h->AV.ensure(sizeof(float) * (size_t)R * nc);
     h->att_ns.ensure(sizeof(double) * (size_t)B * 2 * 2 * nc);
-    // value half through the split contraction; the H key-weighted head sums of the query half
-    // (W_a = W_q key per head) in FP32 behind it
-    const int av = h->iAV;
-    float* QA = QV + (size_t)R * HD;
-    gemm(c, h->XA.as<float>(), F, av, av + 1, QV, HD, R, HD, F, flag, period);
-    for (int n0 = 0; n0 < H; n0 += 16)
-      launch_gemm_f32_tail(c.s, h->XA.as<float>(), F, h->Pt32[av + 2], h->P[av + 3], QA, H, R, n0, std::min(16, H - n0),
-                           F, flag, period, DevRows(), 0, 0, 0, 1);
-    launch_att_group(c.s, QV, QA, H, h->Dk, G, gd, tan, h->AV.as<float>());
+    // one contraction gives the value half and the H key-weighted head sums of the query half
+    // (W_a = W_q key per head): rows [v (HD) | a (H) | 0]
+    const int HDP = h->HDP;
+    gemm(c, h->XA.as<float>(), F, h->iAV, h->iAV + 1, QV, HDP, R, HDP, F, flag, period);
+    launch_att_group(c.s, QV, QV + HD, HDP, H, h->Dk, G, gd, tan, h->AV.as<float>());
     launch_node_sums(c.s, h->AV.as<float>(), nc, nc, gd, B, N, h->n_up, h->att_ns.as<double>());
     const int nrow = tan ? 2 + 3 * N : 1;
     if (tan) {
@@ -847,7 +845,7 @@ void state_init(Ctx& c, const float* r_dev, int B) {
   S.wbscr.ensure((size_t)B * D * wb_update_scratch(N));
   S.jsum.ensure(sizeof(double) * (size_t)B * 2);
   S.p_jr.ensure(sizeof(float) * (size_t)T * 2);
-  S.p_qv.ensure(sizeof(float) * (size_t)T * 2 * std::max(HD, 1));
+  S.p_qv.ensure(sizeof(float) * (size_t)T * std::max(h->HDP, 1));
   S.p_av.ensure(sizeof(float) * (size_t)T * nc);
   S.rin.ensure(sizeof(float) * (size_t)B * 2 * std::max(HD, 1));
   S.ry1.ensure(sizeof(float) * (size_t)B * W);
@@ -981,13 +979,10 @@ void state_propose(Ctx& c, int k, int mode, const float* rk_in, float width, uin
   const float* att_scale = nullptr;
   const float* att_zb = nullptr;
   if (h->has_att) {
-    const int av = h->iAV;
-    float* qa = S.p_qv.as<float>() + (size_t)T * HD;
-    gemm(cc, S.p_h.as<float>(), F, av, av + 1, S.p_qv.as<float>(), HD, T, HD, F, nullptr, 1, rT);
-    for (int n0 = 0; n0 < H; n0 += 16)
-      launch_gemm_f32_tail(c.s, S.p_h.as<float>(), F, h->Pt32[av + 2], h->P[av + 3], qa, H, T, n0, std::min(16, H - n0),
-                           F, nullptr, 1, rT, 0, 0, 0, 1);
-    launch_att_group(c.s, S.p_qv.as<float>(), qa, H, Dk, (int)T, g1, false, S.p_av.as<float>(), dT);
+    const int HDP = h->HDP;
+    gemm(cc, S.p_h.as<float>(), F, h->iAV, h->iAV + 1, S.p_qv.as<float>(), HDP, T, HDP, F, nullptr, 1, rT);
+    launch_att_group(c.s, S.p_qv.as<float>(), S.p_qv.as<float>() + HD, HDP, H, Dk, (int)T, g1, false,
+                     S.p_av.as<float>(), dT);
   }
   if (h->has_mlp || h->has_att)
     launch_mc_jsum(c.s, S.jr.as<float>(), S.p_jr.as<float>(), S.av.as<float>(), S.p_av.as<float>(),
@@ -1131,7 +1126,8 @@ int fail(const std::exception& e) {
 int tangent_chunk(const fire_handle* h) {
   const int64_t N = h->N;
   const int64_t rows = 5 * N + 3 * N * std::min<int64_t>(N - 1, 40);
-  const int64_t width = std::max<int64_t>(std::max<int64_t>(h->F, h->ND), std::max<int64_t>(h->W, 2 * h->HD));
+  const int64_t width =
+      std::max<int64_t>(std::max<int64_t>(h->F, h->ND), std::max<int64_t>(h->W, std::max(2 * h->HD, h->HDP)));
   const int64_t per_walker = rows * width * 4;
   return (int)std::max<int64_t>(8, std::min<int64_t>(h->chunk, (int64_t)12000000000LL / per_walker));
 }
@@ -1189,6 +1185,7 @@ int fire_create(const fire_config* cfg, const float* R, const float* Z, int devi
     h->H = c.att_heads;
     h->Dk = c.att_dim;
     h->HD = c.att_heads * c.att_dim;
+    h->HDP = h->HD + (c.att_heads + 15) / 16 * 16;
     h->depth = c.mlp_depth;
     h->has_mlp = c.use_mlp_jastrow || c.use_log_jastrow;
     h->has_att = c.use_attention != 0;
@@ -1259,7 +1256,7 @@ int fire_upload_params(fire_handle* h, const float* packed, const int64_t* sizes
       if (p) HIPCHK(hipFree(p));
     for (float* p : h->Pt32)
       if (p) HIPCHK(hipFree(p));
-    const int n_all = n_tensors + (h->has_att ? 4 : 0);
+    const int n_all = n_tensors + (h->has_att ? 2 : 0);
     h->P.assign(n_all, nullptr);
     h->Pbf.assign(n_all, nullptr);
     h->Pbf_lo.assign(n_all, nullptr);
@@ -1274,32 +1271,30 @@ int fire_upload_params(fire_handle* h, const float* packed, const int64_t* sizes
       hsize[k] = sizes[k];
       off += sizes[k];
     }
-    std::vector<float> derived[4];
+    std::vector<float> derived[2];
     if (h->has_att) {
-      const int F = h->F, H = h->H, Dk = h->Dk, HD = h->HD;
+      const int F = h->F, H = h->H, Dk = h->Dk, HD = h->HD, HDP = h->HDP;
       const float* wq = host[h->iATT];  // [F, 2 HD]: query columns, then value columns
       const float* bq = host[h->iATT + 1];
       const float* key = host[h->iATT + 2];
-      derived[0].resize((size_t)F * HD);
-      derived[1].resize(HD);
-      derived[2].resize((size_t)F * H);
-      derived[3].resize(H);
+      derived[0].assign((size_t)F * HDP, 0.f);
+      derived[1].assign(HDP, 0.f);
       for (int f = 0; f < F; ++f) {
-        for (int n = 0; n < HD; ++n) derived[0][(size_t)f * HD + n] = wq[(size_t)f * 2 * HD + HD + n];
+        for (int n = 0; n < HD; ++n) derived[0][(size_t)f * HDP + n] = wq[(size_t)f * 2 * HD + HD + n];
         for (int hh = 0; hh < H; ++hh) {
           double a = 0.0;
           for (int k = 0; k < Dk; ++k) a += (double)wq[(size_t)f * 2 * HD + hh * Dk + k] * key[hh * Dk + k];
-          derived[2][(size_t)f * H + hh] = (float)a;
+          derived[0][(size_t)f * HDP + HD + hh] = (float)a;
         }
       }
       for (int n = 0; n < HD; ++n) derived[1][n] = bq[HD + n];
       for (int hh = 0; hh < H; ++hh) {
         double a = 0.0;
         for (int k = 0; k < Dk; ++k) a += (double)bq[hh * Dk + k] * key[hh * Dk + k];
-        derived[3][hh] = (float)a;
+        derived[1][HD + hh] = (float)a;
       }
       h->iAV = n_tensors;
-      for (int q = 0; q < 4; ++q) {
+      for (int q = 0; q < 2; ++q) {
         host[n_tensors + q] = derived[q].data();
         hsize[n_tensors + q] = (int64_t)derived[q].size();
       }
@@ -1351,7 +1346,6 @@ int fire_upload_params(fire_handle* h, const float* packed, const int64_t* sizes
       HIPCHK(hipMemcpy(h->Pt32[k], tt.data(), sizeof(float) * tt.size(), hipMemcpyHostToDevice));
     };
     for (auto& t : gw) upload_t(t.first, 1, t.second.first, t.second.second);
-    if (h->has_att) upload_t(h->iAV + 2, 1, h->F, h->H);
     for (int k : {h->iORB + 5, h->iORB + 7}) upload_t(k, h->D, h->M * h->Eiso, h->N);
     // a captured sweep graph holds the old parameter pointers
     if (h->st.graph) {

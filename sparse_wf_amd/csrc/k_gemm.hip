@@ -161,7 +161,7 @@ __device__ __forceinline__ void store_tile(const AccT (&acc)[2][2], int64_t m0, 
 }
 
 // ---------------------------------------------------------------------------------------
-// FP32 MFMA GEMM.  Wt is [N,K] FP32 (W transposed).  Vector path: K % 32 == 0, lda % 4 == 0.
+// FP32 MFMA GEMM.  Wt is [N,K] FP32 (W transposed).  Vector path: K % 4 == 0, lda % 4 == 0.
 // ---------------------------------------------------------------------------------------
 template <bool VEC, bool FUSE = false>
 __global__ void __launch_bounds__(256) k_gemm_f32(const float* __restrict__ A, int64_t lda,
@@ -203,9 +203,11 @@ __global__ void __launch_bounds__(256) k_gemm_f32(const float* __restrict__ A, i
       const int rr = e >> 3, kq = (e & 7) * 4;
       const int64_t gr = m0 + rr;
       const int gn = n0 + rr;
-      if (VEC) {
-        ra[q] = gr < M ? *reinterpret_cast<const float4*>(A + gr * lda + k0 + kq) : make_float4(0.f, 0.f, 0.f, 0.f);
-        rb[q] = gn < N ? *reinterpret_cast<const float4*>(Wt + (int64_t)gn * K + k0 + kq) : make_float4(0.f, 0.f, 0.f, 0.f);
+      if (VEC) {  // K % 4 == 0: a float4 lies wholly inside or wholly past K
+        const bool kin = k0 + kq < K;
+        ra[q] = gr < M && kin ? *reinterpret_cast<const float4*>(A + gr * lda + k0 + kq) : make_float4(0.f, 0.f, 0.f, 0.f);
+        rb[q] = gn < N && kin ? *reinterpret_cast<const float4*>(Wt + (int64_t)gn * K + k0 + kq)
+                              : make_float4(0.f, 0.f, 0.f, 0.f);
       } else {
         float va[4], vb[4];
 #pragma unroll
@@ -535,7 +537,7 @@ void launch_gemm_f32(hipStream_t s, const float* A, int64_t lda, const float* W,
   const int nT = (Nm + GT_N - 1) / GT_N;
   const int64_t blocks = ((M + GT_M - 1) / GT_M) * nT;
   if (ep.rowgrp) {
-    if ((K % GT_K) == 0 && (lda % 4) == 0)
+    if ((K % 4) == 0 && (lda % 4) == 0)
       k_gemm_f32<true, true><<<(unsigned)blocks, 256, 0, s>>>(A, lda, Wt, bias, C, ldc, M, Nm, K, rowflag, period, nT,
                                                               dr, 0, 0, 0, ep);
     else
@@ -543,7 +545,7 @@ void launch_gemm_f32(hipStream_t s, const float* A, int64_t lda, const float* W,
                                                                dr, 0, 0, 0, ep);
     return;
   }
-  if ((K % GT_K) == 0 && (lda % 4) == 0)
+  if ((K % 4) == 0 && (lda % 4) == 0)
     k_gemm_f32<true><<<(unsigned)blocks, 256, 0, s>>>(A, lda, Wt, bias, C, ldc, M, Nm, K, rowflag, period, nT, dr, 0,
                                                       0, 0, ep);
   else
@@ -560,7 +562,7 @@ void launch_gemm_f32_batched(hipStream_t s, const float* A, int64_t lda, int64_t
   const int rem = N % GT_N, Nm = rem > 0 && rem <= 16 && N > GT_N ? N - rem : N;
   const int nT = (Nm + GT_N - 1) / GT_N;
   const dim3 grid((unsigned)(((M + GT_M - 1) / GT_M) * nT), 1, (unsigned)nb);
-  if ((K % GT_K) == 0 && (lda % 4) == 0)
+  if ((K % 4) == 0 && (lda % 4) == 0 && (sA % 4) == 0 && (sW % 4) == 0)
     k_gemm_f32<true><<<grid, 256, 0, s>>>(A, lda, Wt, nullptr, C, ldc, M, Nm, K, nullptr, 0, nT, DevRows(), sA, sW, sC, ActEpi());
   else
     k_gemm_f32<false><<<grid, 256, 0, s>>>(A, lda, Wt, nullptr, C, ldc, M, Nm, K, nullptr, 0, nT, DevRows(), sA, sW,

@@ -233,11 +233,12 @@ void launch_dep_gather(hipStream_t s, const float* X, int64_t ld, int ncols, Gro
 // Attention Jastrow per group: att = exp(-a / sqrt(Dk)) and vals = att * v with
 // forward-Laplacian rows, where a = sum_k q[h,k] keys[h,k] arrives precomputed per row and
 // head (the key-weighted head sums of the query projection are a linear map of the features,
-// contracted as such).  V rows [H*Dk], QA rows [H]; AV rows = [att (H) | vals (H*Dk)].
+// contracted as such).  V rows [H*Dk] and QA rows [H] at row stride ldq (one buffer with QA
+// behind V); AV rows = [att (H) | vals (H*Dk)].
 // Thread (h, k) owns column h*Dk + k, so every row is read once with coalesced loads.
 template <bool TAN>
-__global__ void __launch_bounds__(256) k_att_group(const float* __restrict__ V, const float* __restrict__ QA, int H,
-                                                   int Dk, GroupDesc gd, float* __restrict__ AV,
+__global__ void __launch_bounds__(256) k_att_group(const float* __restrict__ V, const float* __restrict__ QA, int ldq,
+                                                   int H, int Dk, GroupDesc gd, float* __restrict__ AV,
                                                    const int32_t* __restrict__ dG) {
   const int g = blockIdx.x;
   if (dG && g >= *dG) return;
@@ -248,13 +249,13 @@ __global__ void __launch_bounds__(256) k_att_group(const float* __restrict__ V, 
   const float sc = rsqrtf((float)Dk);
   const int64_t r0 = gd.row(g), r1 = TAN ? gd.row(g + 1) : r0 + 1;
   const int lda = H + HD;
-  const float ax = QA[r0 * H + h];
+  const float ax = QA[r0 * ldq + h];
   const float f = expf(-sc * ax), f1 = -sc * f, f2 = sc * sc * f;
-  const float vx = V[r0 * HD + col];
+  const float vx = V[r0 * ldq + col];
   AV[r0 * lda + H + col] = f * vx;
   if (k == 0) AV[r0 * lda + h] = f;
   if (!TAN) return;
-  const float a_l = QA[(r0 + 1) * H + h];
+  const float a_l = QA[(r0 + 1) * ldq + h];
   float ss = 0.f, cross = 0.f;
   auto row = [&](int64_t rr, float a_t, float vt) {
     const float at = a_t * f1;
@@ -269,27 +270,27 @@ __global__ void __launch_bounds__(256) k_att_group(const float* __restrict__ V, 
     float q[4], v[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      q[u] = QA[(rr + u) * H + h];
-      v[u] = V[(rr + u) * HD + col];
+      q[u] = QA[(rr + u) * ldq + h];
+      v[u] = V[(rr + u) * ldq + col];
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u) row(rr + u, q[u], v[u]);
   }
-  for (; rr < r1; ++rr) row(rr, QA[rr * H + h], V[rr * HD + col]);
+  for (; rr < r1; ++rr) row(rr, QA[rr * ldq + h], V[rr * ldq + col]);
   const float al = a_l * f1 + f2 * ss;
-  const float vl = V[(r0 + 1) * HD + col];
+  const float vl = V[(r0 + 1) * ldq + col];
   AV[(r0 + 1) * lda + H + col] = al * vx + f * vl + 2.f * cross;
   if (k == 0) AV[(r0 + 1) * lda + h] = al;
 }
 
-void launch_att_group(hipStream_t s, const float* V, const float* QA, int H, int Dk, int G, GroupDesc gd,
+void launch_att_group(hipStream_t s, const float* V, const float* QA, int ldq, int H, int Dk, int G, GroupDesc gd,
                       bool tangents, float* AV, const int32_t* dG) {
   if (G == 0) return;
   const int threads = ((H * Dk + 63) / 64) * 64;
   if (tangents)
-    k_att_group<true><<<G, threads, 0, s>>>(V, QA, H, Dk, gd, AV, dG);
+    k_att_group<true><<<G, threads, 0, s>>>(V, QA, ldq, H, Dk, gd, AV, dG);
   else
-    k_att_group<false><<<G, threads, 0, s>>>(V, QA, H, Dk, gd, AV, dG);
+    k_att_group<false><<<G, threads, 0, s>>>(V, QA, ldq, H, Dk, gd, AV, dG);
 }
 
 // Readout input u[s,h,k] = vals[s,h,k] / norm[s,h] with dense forward-Laplacian over the
