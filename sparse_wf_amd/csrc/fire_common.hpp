@@ -168,6 +168,15 @@ __device__ inline void block_sum(T (&v)[NV], T* smem /* >= 16*NV */) {
 __device__ __forceinline__ int group_of(int w, int i, int B, int n_up, int n_dn) {
   return i < n_up ? w * n_up + i : B * n_up + w * n_dn + (i - n_up);
 }
+// Workgroups are dealt round-robin over the 8 XCDs (block b runs on XCD b % 8).  This maps
+// block b of n to work item x q + min(x, r) + b / 8 (x = b % 8, q = n / 8, r = n % 8), so each
+// XCD takes one contiguous range of work items and data they share (the rows of one walker's
+// electrons) stays in that XCD's L2.  A bijection of [0, n).
+__device__ __forceinline__ int xcd_block(int b, int n) {
+  const int q = n >> 3, r = n & 7, x = b & 7;
+  return x * q + (x < r ? x : r) + (b >> 3);
+}
+
 __device__ __forceinline__ void group_inv(int g, int B, int n_up, int n_dn, int& w, int& i) {
   if (g < B * n_up) { w = g / n_up; i = g - w * n_up; }
   else { int h = g - B * n_up; w = h / n_dn; i = n_up + (h - w * n_dn); }

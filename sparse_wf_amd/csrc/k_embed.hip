@@ -465,7 +465,7 @@ __global__ void __launch_bounds__(256) k_messages(const float* __restrict__ r, i
   __shared__ const float* s_src[MSG_CHUNK];
   __shared__ int s_same[MSG_CHUNK];
   __shared__ int s_nup;  // up-spin neighbours in the chunk (they come first)
-  const int g = blockIdx.x;
+  const int g = list ? (int)blockIdx.x : xcd_block(blockIdx.x, gridDim.x);
   const int n_dn = N - n_up;
   int w, i;
   if (list) {
