@@ -107,48 +107,64 @@ __device__ __forceinline__ void store_rows(const AccT (&acc)[2][NB], int64_t m0,
     }
   } else {
     // fused silu: the lane's 32 rows ascend with (a, v), so the jacobian rows of one group form
-    // one run per lane and column; each run's sum of squares goes to its own slot.  Lane l
-    // holds the code of row rb + l; every lane takes part in the shuffles.
+    // one run per lane (the same for each of its NB columns); each run's sum of squares per
+    // column goes to its own slot.  Lane l holds the code of row rb + l; every lane takes part
+    // in the shuffles.
     const int h = lane >> 5;
     const int64_t rb = m0 + wm;  // first row of this 64-row block (blocks are 64-aligned)
     const int mycode = rb + lane < M ? ep.rowgrp[rb + lane] : -3;
-    auto flush = [&](int g, float ss, int col) {
-      const int64_t first = (ep.gd.row(g) + 2) >> 6;  // block of the group's first jacobian row
-      const int slot = 2 * (int)((rb >> 6) - first) + h;
-      ep.part[((int64_t)g * ACT_EPI_SLOTS + slot) * ep.ldf + col] = ss;
-    };
+    int col[NB];
+    bool colok[NB];
+    float bcol[NB], ss[NB], f1[NB];
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
-      const int col = n0 + wn + b * 32 + (lane & 31);
-      const bool colok = col < N;
-      const float bcol = (bias && colok) ? bias[col] : 0.f;
-      int cur = -1;
-      float ss = 0.f, f1 = 0.f;
+      col[b] = n0 + wn + b * 32 + (lane & 31);
+      colok[b] = col[b] < N;
+      bcol[b] = (bias && colok[b]) ? bias[col[b]] : 0.f;
+      ss[b] = 0.f;
+      f1[b] = 0.f;
+    }
+    int cur = -1;
+    auto flush = [&]() {
+      const int64_t first = (ep.gd.row(cur) + 2) >> 6;  // block of the group's first jacobian row
+      const int slot = 2 * (int)((rb >> 6) - first) + h;
+      float* dst = ep.part + ((int64_t)cur * ACT_EPI_SLOTS + slot) * ep.ldf;
 #pragma unroll
-      for (int a = 0; a < 2; ++a)
+      for (int b = 0; b < NB; ++b)
+        if (colok[b]) dst[col[b]] = ss[b];
+    };
 #pragma unroll
-        for (int v = 0; v < 16; ++v) {
-          const int rl = a * 32 + 8 * (v >> 2) + 4 * h + (v & 3);
-          const int cd = __shfl(mycode, rl);
-          if (!colok || cd == -3) continue;
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const int rl = a * 32 + 8 * (v >> 2) + 4 * h + (v & 3);
+        const int cd = __shfl(mycode, rl);
+        if (cd == -3) continue;
+        if (cd >= 0 && cd != cur) {
+          if (cur >= 0) flush();
+          cur = cd;
+#pragma unroll
+          for (int b = 0; b < NB; ++b) {
+            ss[b] = 0.f;
+            f1[b] = colok[b] ? ep.f1[(int64_t)cd * ep.ldf + col[b]] : 0.f;
+          }
+        }
+        float* crow = C + (rb + rl) * ldc;
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+          if (!colok[b]) continue;
           float z = acc[a][b][v];
           if (cd >= 0) {
-            if (cd != cur) {
-              if (cur >= 0) flush(cur, ss, col);
-              cur = cd;
-              ss = 0.f;
-              f1 = ep.f1[(int64_t)cd * ep.ldf + col];
-            }
-            ss += z * z;
-            z *= f1;
+            ss[b] += z * z;
+            z *= f1[b];
           } else if (cd == -1) {
-            z += bcol;
+            z += bcol[b];
             z *= 1.0f / (1.0f + expf(-z));
           }
-          C[(rb + rl) * ldc + col] = z;
+          crow[col[b]] = z;
         }
-      if (colok && cur >= 0) flush(cur, ss, col);
-    }
+      }
+    if (cur >= 0) flush();
   }
 }
 

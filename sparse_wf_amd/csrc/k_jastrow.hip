@@ -264,17 +264,17 @@ __global__ void __launch_bounds__(256) k_att_group(const float* __restrict__ V, 
     AV[rr * lda + H + col] = at * vx + f * vt;
     if (k == 0) AV[rr * lda + h] = at;
   };
-  // jacobian rows 4 at a time: the rows' a and V values are loaded before the stores
+  // jacobian rows 8 at a time: the rows' a and V values are loaded before the stores
   int64_t rr = r0 + 2;
-  for (; rr + 4 <= r1; rr += 4) {
-    float q[4], v[4];
+  for (; rr + 8 <= r1; rr += 8) {
+    float q[8], v[8];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < 8; ++u) {
       q[u] = QA[(rr + u) * ldq + h];
       v[u] = V[(rr + u) * ldq + col];
     }
 #pragma unroll
-    for (int u = 0; u < 4; ++u) row(rr + u, q[u], v[u]);
+    for (int u = 0; u < 8; ++u) row(rr + u, q[u], v[u]);
   }
   for (; rr < r1; ++rr) row(rr, QA[rr * ldq + h], V[rr * ldq + col]);
   const float al = a_l * f1 + f2 * ss;
