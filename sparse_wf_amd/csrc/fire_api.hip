@@ -95,6 +95,11 @@ struct fire_handle {
   // HDP = HD + H rounded up to 16
   int iAV = -1;
   int HDP = 0;
+  // with attention: the orbital projections with the H head sums appended as columns
+  // ND .. ND + H (up: iORBA, iORBA + 1 bias; down: iORBA + 2, + 3), and with an MLP Jastrow the
+  // first Jastrow layer with the value half appended, [W_j1 | W_v] [F, W + HD] (iJV, iJV + 1)
+  int iORBA = -1, iJV = -1;
+  int ldo_orb = 0;  // row stride of the orbital rows of the last run_orbitals_det
   // workspace
   DevBuf cnt, ptr, nbr, rev, rowflag, rowgrp;
   DevBuf AV0, Z0, F1, PART;  // fused silu epilogue: packed value rows, their pre-activations, silu', slots
@@ -269,6 +274,9 @@ void gemm_weights(fire_handle* h, std::vector<std::pair<int, std::pair<int, int>
   }
   if (h->has_att) {
     out.push_back({h->iAV, {F, h->HDP}});
+    out.push_back({h->iORBA, {F, ND + h->H}});
+    out.push_back({h->iORBA + 2, {F, ND + h->H}});
+    if (h->has_mlp) out.push_back({h->iJV, {F, W + HD}});
     int fan = 2 * HD;
     for (int l = 0; l < h->depth; ++l) {
       out.push_back({h->iATT + 3 + 2 * l, {fan, W}});
@@ -290,6 +298,8 @@ int gemm_class(const fire_handle* h, int wi) {
   if (wi < h->iORB) return 1;
   if (wi < h->iJM) return 2;
   if (wi < h->iATT) return 3;
+  if (wi == h->iORBA || wi == h->iORBA + 2) return 2;
+  if (wi == h->iJV) return 3;  // shared with the attention values: planes of both (gemm())
   return wi == h->iATT || wi == h->iAV ? 4 : 5;
 }
 
@@ -303,7 +313,8 @@ void gemm(Ctx& c, const float* A, int64_t lda, int wi, int bi, float* C, int64_t
   const bool use_bf = (c.prec == FIRE_MIXED || c.prec == FIRE_MIXED_FAST || c.prec == FIRE_BF16) && bf_ok;
   int ns = c.prec == FIRE_MIXED_FAST ? 3 : 1;
   if (c.prec == FIRE_MIXED) {
-    const int np = h->planes[gemm_class(h, wi)];
+    int np = h->planes[gemm_class(h, wi)];
+    if (wi == h->iJV) np = std::max(np, h->planes[4]);
     ns = np == 3 ? 6 : (np == 2 ? 3 : 1);
   }
   int variant = 0;
@@ -367,31 +378,45 @@ void gemm(Ctx& c, const float* A, int64_t lda, int wi, int bi, float* C, int64_t
 // activation fused into the contraction's epilogue: the value rows are contracted first (FP32,
 // packed) for silu' of every group, the epilogue applies silu / silu' and leaves per-block sums
 // of squares of the jacobian rows, k_act_lapfix completes the Laplacian rows.
+// With wf >= 0 only the first Nf columns are activated: they are the product with weight
+// tensor wf (bias bf), the remaining N - Nf columns are stored as they are (+ bias on value
+// rows); requires the fused epilogue.
+bool gemm_silu_fusable(const fire_handle* h, int N, int K, int64_t lda) {
+  return h->fuse_act && N % 128 == 0 && K % 32 == 0 && lda % 4 == 0;
+}
+
 void gemm_silu_tan(Ctx& c, const float* A, int64_t lda, int wi, int bi, float* C, int64_t ldc, int64_t R, int N, int K,
-                   const uint8_t* flag, int G, const GroupDesc& gd) {
+                   const uint8_t* flag, int G, const GroupDesc& gd, int wf = -1, int bf = -1, int Nf = 0) {
   fire_handle* h = c.h;
-  if (!h->fuse_act || N % 128 != 0 || K % 32 != 0 || lda % 4 != 0 || G == 0) {
+  if (wf < 0) {
+    wf = wi;
+    bf = bi;
+    Nf = N;
+  }
+  if (!gemm_silu_fusable(h, Nf, K, lda) || G == 0) {
+    REQUIRE(Nf == N, "partial activation needs the fused epilogue");
     gemm(c, A, lda, wi, bi, C, ldc, R, N, K, flag, 0);
     launch_act(c.s, C, ldc, N, G, gd, ACT_SILU, 0.f);
     return;
   }
   h->AV0.ensure(sizeof(float) * (size_t)G * K);
-  h->Z0.ensure(sizeof(float) * (size_t)G * N);
-  h->F1.ensure(sizeof(float) * (size_t)G * N);
-  h->PART.ensure(sizeof(float) * (size_t)G * ACT_EPI_SLOTS * N);
+  h->Z0.ensure(sizeof(float) * (size_t)G * Nf);
+  h->F1.ensure(sizeof(float) * (size_t)G * Nf);
+  h->PART.ensure(sizeof(float) * (size_t)G * ACT_EPI_SLOTS * Nf);
   float* z0 = h->Z0.as<float>();
   launch_gather_rows(c.s, A, lda, K, G, gd, h->AV0.as<float>());
-  launch_gemm_f32(c.s, h->AV0.as<float>(), K, h->P[wi], h->Pt32[wi], bi >= 0 ? h->P[bi] : nullptr, z0, N, G, N, K,
+  launch_gemm_f32(c.s, h->AV0.as<float>(), K, h->P[wf], h->Pt32[wf], bf >= 0 ? h->P[bf] : nullptr, z0, Nf, G, Nf, K,
                   nullptr, 1);
-  launch_silu_grad(c.s, z0, h->F1.as<float>(), (int64_t)G * N);
+  launch_silu_grad(c.s, z0, h->F1.as<float>(), (int64_t)G * Nf);
   ActEpi ep;
   ep.rowgrp = h->rowgrp.as<int32_t>();
   ep.f1 = h->F1.as<float>();
   ep.part = h->PART.as<float>();
   ep.gd = gd;
-  ep.ldf = N;
+  ep.ldf = Nf;
+  ep.nfuse = Nf;
   gemm(c, A, lda, wi, bi, C, ldc, R, N, K, flag, 0, DevRows(), &ep);
-  launch_act_lapfix(c.s, C, ldc, N, G, gd, z0, h->PART.as<float>(), N);
+  launch_act_lapfix(c.s, C, ldc, Nf, G, gd, z0, h->PART.as<float>(), Nf);
 }
 
 EnParams en_params(fire_handle* h) {
@@ -544,7 +569,12 @@ void run_orbitals_det(Ctx& c, const float* r, int B, const Graph& gr, bool tan, 
   // orbital rows at a stride of a multiple of 32 floats (128-byte lines); the padding columns
   // are read (against zero inverse columns) by the determinant Laplacian's vector loads and are
   // zeroed whenever the buffer is allocated (every later writer stores finite values)
-  const int ldo = (ND + 31) & ~31;
+  // tangent rows with attention: the orbital contraction also yields the H head sums (columns
+  // ND .. ND + H; the value path keeps the orbital-only form the Metropolis updates share)
+  const bool aug = tan && h->has_att;
+  const int NDA = ND + (aug ? h->H : 0);
+  const int ldo = (NDA + 31) & ~31;
+  h->ldo_orb = ldo;
   const size_t ocols = std::max(ldo, h->has_att ? h->HDP : 0);
   const void* o_prev = h->O.p;
   h->O.ensure(sizeof(float) * (size_t)R * ocols);
@@ -563,10 +593,11 @@ void run_orbitals_det(Ctx& c, const float* r, int B, const Graph& gr, bool tan, 
   const uint8_t* flag = tan ? h->rowflag.as<uint8_t>() : nullptr;
   const int period = tan ? 0 : 1;
   float* Ob = h->O.as<float>();
-  gemm(c, h->XA.as<float>(), F, o + 0, o + 1, Ob, ldo, R_up_rows, ND, F, flag, period);
+  const int wup = aug ? h->iORBA : o, wdn = aug ? h->iORBA + 2 : o + 2;
+  gemm(c, h->XA.as<float>(), F, wup, wup + 1, Ob, ldo, R_up_rows, NDA, F, flag, period);
   if (R > R_up_rows)
-    gemm(c, h->XA.as<float>() + (size_t)R_up_rows * F, F, o + 2, o + 3, Ob + (size_t)R_up_rows * ldo, ldo,
-         R - R_up_rows, ND, F, flag ? flag + R_up_rows : nullptr, period);
+    gemm(c, h->XA.as<float>() + (size_t)R_up_rows * F, F, wdn, wdn + 1, Ob + (size_t)R_up_rows * ldo, ldo,
+         R - R_up_rows, NDA, F, flag ? flag + R_up_rows : nullptr, period);
   if (keep)
     HIPCHK(hipMemcpy2DAsync(keep->orb, sizeof(float) * ND, Ob, sizeof(float) * ldo, sizeof(float) * ND, (size_t)R,
                             hipMemcpyDeviceToDevice, c.s));
@@ -657,28 +688,40 @@ void run_jastrow_and_assemble(Ctx& c, const float* r, int B, bool tan, const Gro
   A.det_sign = det.sign;
   A.det_grad = det.grad;
   A.det_lap = det.lap;
+  // tangent rows, two Jastrow layers and attention: the first Jastrow layer and the attention
+  // values are one contraction of the features ([W_j1 | W_v], silu on the first W columns
+  // only), its rows [h1 (W) | v (HD)] in XC; the head sums come with the orbital rows
+  const bool joint = tan && h->has_mlp && h->has_att && h->depth == 2 && h->iJV >= 0 &&
+                     gemm_silu_fusable(h, W, F, F) && (W + h->HD) % 4 == 0;
   if (h->has_mlp) {
-    h->XC.ensure(sizeof(float) * (size_t)R * W);
+    h->XC.ensure(sizeof(float) * (size_t)R * (joint ? W + h->HD : W));
     h->XB.ensure(sizeof(float) * (size_t)R * std::max(W, F));
     h->JR.ensure(sizeof(float) * (size_t)R * 2);
     h->mlp_ns.ensure(sizeof(double) * (size_t)B * 8);
     const int j = h->iJM;
     float* cur = h->XB.as<float>();
     float* nxt = h->XC.as<float>();
-    // the silu of the last hidden layer is applied inside k_jmlp_last, the others in the
-    // epilogue of their contraction (tangent rows) or by k_act
-    for (int l = 0; l < h->depth; ++l) {
-      const float* in = l == 0 ? h->XA.as<float>() : nxt;
-      const int fan = l == 0 ? F : W;
-      if (l + 1 < h->depth && tan) {
-        gemm_silu_tan(c, in, fan, j + 2 * l, j + 2 * l + 1, cur, W, R, W, fan, flag, G, gd);
-      } else {
-        gemm(c, in, fan, j + 2 * l, j + 2 * l + 1, cur, W, R, W, fan, flag, period);
-        if (l + 1 < h->depth) launch_act(c.s, cur, W, W, G, gd, ACT_SILU, 0.f);
+    if (joint) {
+      const int NJ = W + h->HD;
+      gemm_silu_tan(c, h->XA.as<float>(), F, h->iJV, h->iJV + 1, h->XC.as<float>(), NJ, R, NJ, F, flag, G, gd, j + 0,
+                    j + 1, W);
+      gemm(c, h->XC.as<float>(), NJ, j + 2, j + 3, h->XB.as<float>(), W, R, W, W, flag, period);
+    } else {
+      // the silu of the last hidden layer is applied inside k_jmlp_last, the others in the
+      // epilogue of their contraction (tangent rows) or by k_act
+      for (int l = 0; l < h->depth; ++l) {
+        const float* in = l == 0 ? h->XA.as<float>() : nxt;
+        const int fan = l == 0 ? F : W;
+        if (l + 1 < h->depth && tan) {
+          gemm_silu_tan(c, in, fan, j + 2 * l, j + 2 * l + 1, cur, W, R, W, fan, flag, G, gd);
+        } else {
+          gemm(c, in, fan, j + 2 * l, j + 2 * l + 1, cur, W, R, W, fan, flag, period);
+          if (l + 1 < h->depth) launch_act(c.s, cur, W, W, G, gd, ACT_SILU, 0.f);
+        }
+        std::swap(cur, nxt);
       }
       std::swap(cur, nxt);
     }
-    std::swap(cur, nxt);
     const int jl = j + 2 * h->depth;
     launch_jmlp_last(c.s, cur, W, G, gd, h->P[jl], h->P[jl + 1], h->P[jl + 2], h->P[jl + 3], h->JR.as<float>());
     launch_node_sums(c.s, h->JR.as<float>(), 2, 2, gd, B, N, h->n_up, h->mlp_ns.as<double>());
@@ -693,14 +736,19 @@ void run_jastrow_and_assemble(Ctx& c, const float* r, int B, bool tan, const Gro
   if (h->has_att) {
     const int HD = h->HD, H = h->H, ac = h->iATT;
     const int nc = H + HD;
-    float* QV = h->O.as<float>();  // orbital rows are no longer needed
     h->AV.ensure(sizeof(float) * (size_t)R * nc);
     h->att_ns.ensure(sizeof(double) * (size_t)B * 2 * 2 * nc);
-    // one contraction gives the value half and the H key-weighted head sums of the query half
-    // (W_a = W_q key per head): rows [v (HD) | a (H) | 0]
-    const int HDP = h->HDP;
-    gemm(c, h->XA.as<float>(), F, h->iAV, h->iAV + 1, QV, HDP, R, HDP, F, flag, period);
-    launch_att_group(c.s, QV, QV + HD, HDP, H, h->Dk, G, gd, tan, h->AV.as<float>());
+    if (joint) {  // values behind the first Jastrow layer, head sums behind the orbitals
+      launch_att_group(c.s, h->XC.as<float>() + W, W + HD, h->O.as<float>() + h->ND, h->ldo_orb, H, h->Dk, G, gd,
+                       tan, h->AV.as<float>());
+    } else {
+      // one contraction gives the value half and the H key-weighted head sums of the query half
+      // (W_a = W_q key per head): rows [v (HD) | a (H) | 0]; the orbital rows are no longer needed
+      float* QV = h->O.as<float>();
+      const int HDP = h->HDP;
+      gemm(c, h->XA.as<float>(), F, h->iAV, h->iAV + 1, QV, HDP, R, HDP, F, flag, period);
+      launch_att_group(c.s, QV, HDP, QV + HD, HDP, H, h->Dk, G, gd, tan, h->AV.as<float>());
+    }
     launch_node_sums(c.s, h->AV.as<float>(), nc, nc, gd, B, N, h->n_up, h->att_ns.as<double>());
     const int nrow = tan ? 2 + 3 * N : 1;
     if (tan) {
@@ -981,7 +1029,7 @@ void state_propose(Ctx& c, int k, int mode, const float* rk_in, float width, uin
   if (h->has_att) {
     const int HDP = h->HDP;
     gemm(cc, S.p_h.as<float>(), F, h->iAV, h->iAV + 1, S.p_qv.as<float>(), HDP, T, HDP, F, nullptr, 1, rT);
-    launch_att_group(c.s, S.p_qv.as<float>(), S.p_qv.as<float>() + HD, HDP, H, Dk, (int)T, g1, false,
+    launch_att_group(c.s, S.p_qv.as<float>(), HDP, S.p_qv.as<float>() + HD, HDP, H, Dk, (int)T, g1, false,
                      S.p_av.as<float>(), dT);
   }
   if (h->has_mlp || h->has_att)
@@ -1256,7 +1304,7 @@ int fire_upload_params(fire_handle* h, const float* packed, const int64_t* sizes
       if (p) HIPCHK(hipFree(p));
     for (float* p : h->Pt32)
       if (p) HIPCHK(hipFree(p));
-    const int n_all = n_tensors + (h->has_att ? 2 : 0);
+    const int n_all = n_tensors + (h->has_att ? (h->has_mlp ? 8 : 6) : 0);
     h->P.assign(n_all, nullptr);
     h->Pbf.assign(n_all, nullptr);
     h->Pbf_lo.assign(n_all, nullptr);
@@ -1271,7 +1319,7 @@ int fire_upload_params(fire_handle* h, const float* packed, const int64_t* sizes
       hsize[k] = sizes[k];
       off += sizes[k];
     }
-    std::vector<float> derived[2];
+    std::vector<float> derived[8];
     if (h->has_att) {
       const int F = h->F, H = h->H, Dk = h->Dk, HD = h->HD, HDP = h->HDP;
       const float* wq = host[h->iATT];  // [F, 2 HD]: query columns, then value columns
@@ -1294,7 +1342,29 @@ int fire_upload_params(fire_handle* h, const float* packed, const int64_t* sizes
         derived[1][HD + hh] = (float)a;
       }
       h->iAV = n_tensors;
-      for (int q = 0; q < 2; ++q) {
+      // columns [c0, c0 + n) of the [F, HDP] table above: value half (0, HD), head sums (HD, H)
+      auto cat = [&](const float* w1, int n1, const float* b1, int c0, int n, std::vector<float>& w,
+                     std::vector<float>& bb) {
+        w.assign((size_t)F * (n1 + n), 0.f);
+        bb.assign(n1 + n, 0.f);
+        for (int f = 0; f < F; ++f) {
+          for (int q = 0; q < n1; ++q) w[(size_t)f * (n1 + n) + q] = w1[(size_t)f * n1 + q];
+          for (int q = 0; q < n; ++q) w[(size_t)f * (n1 + n) + n1 + q] = derived[0][(size_t)f * HDP + c0 + q];
+        }
+        for (int q = 0; q < n1; ++q) bb[q] = b1[q];
+        for (int q = 0; q < n; ++q) bb[n1 + q] = derived[1][c0 + q];
+      };
+      const int o = h->iORB, ND = h->ND;
+      cat(host[o + 0], ND, host[o + 1], HD, H, derived[2], derived[3]);
+      cat(host[o + 2], ND, host[o + 3], HD, H, derived[4], derived[5]);
+      h->iORBA = n_tensors + 2;
+      int nd = 6;
+      if (h->has_mlp) {
+        cat(host[h->iJM + 0], h->W, host[h->iJM + 1], 0, HD, derived[6], derived[7]);
+        h->iJV = n_tensors + 6;
+        nd = 8;
+      }
+      for (int q = 0; q < nd; ++q) {
         host[n_tensors + q] = derived[q].data();
         hsize[n_tensors + q] = (int64_t)derived[q].size();
       }

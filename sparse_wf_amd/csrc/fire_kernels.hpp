@@ -85,7 +85,8 @@ struct ActEpi {
   const float* f1 = nullptr;
   float* part = nullptr;
   GroupDesc gd{0, 0, nullptr};
-  int ldf = 0;  // row stride of f1 and of each part slot
+  int ldf = 0;            // row stride of f1 and of each part slot
+  int nfuse = 1 << 30;    // columns at or past nfuse are stored plain (+ bias on value rows)
 };
 constexpr int ACT_EPI_SLOTS = 8;
 

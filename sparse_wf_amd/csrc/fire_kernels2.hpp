@@ -36,8 +36,8 @@ void launch_node_sums(hipStream_t s, const float* X, int64_t ld, int ncols, Grou
 void launch_dep_gather(hipStream_t s, const float* X, int64_t ld, int ncols, GroupDesc gd, const int64_t* ptr,
                        const int32_t* nbr, const int32_t* rev, int B, int N, int n_up,
                        double* out /*[B][2 spin][N][3][ncols]*/);
-void launch_att_group(hipStream_t s, const float* V /*[rows][H Dk]*/, const float* QA /*[rows][H]*/, int ldq, int H,
-                      int Dk, int G, GroupDesc gd, bool tangents, float* AV, const int32_t* dG = nullptr);
+void launch_att_group(hipStream_t s, const float* V /*[rows][H Dk]*/, int ldv, const float* QA /*[rows][H]*/, int ldq,
+                      int H, int Dk, int G, GroupDesc gd, bool tangents, float* AV, const int32_t* dG = nullptr);
 void launch_att_readin(hipStream_t s, const double* NS, const double* DG, int B, int N, int H, int Dk, bool tangents,
                        float* RIN);
 void launch_cusp_coulomb(hipStream_t s, const float* r, const float* R, const float* Z, int B, int N, int n_up, int M,

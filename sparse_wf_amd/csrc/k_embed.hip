@@ -365,7 +365,8 @@ __global__ void k_act_lapfix(float* __restrict__ C, int64_t ldc, int N, GroupDes
       for (int hh = 0; hh < 2; ++hh) {
         bool has = false;
         for (int64_t r = s0; r < s1 && r < s0 + 8; ++r) has |= (int)((r >> 2) & 1) == hh;
-        if (has) ss += part[((int64_t)g * ACT_EPI_SLOTS + 2 * (blk - b0) + hh) * ldf + c];
+        const int slot = 2 * (int)(blk - b0) + hh;
+        if (has && slot < ACT_EPI_SLOTS) ss += part[((int64_t)g * ACT_EPI_SLOTS + slot) * ldf + c];
       }
     }
   }

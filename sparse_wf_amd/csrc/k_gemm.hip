@@ -114,12 +114,13 @@ __device__ __forceinline__ void store_rows(const AccT (&acc)[2][NB], int64_t m0,
     const int64_t rb = m0 + wm;  // first row of this 64-row block (blocks are 64-aligned)
     const int mycode = rb + lane < M ? ep.rowgrp[rb + lane] : -3;
     int col[NB];
-    bool colok[NB];
+    bool colok[NB], fz[NB];
     float bcol[NB], ss[NB], f1[NB];
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
       col[b] = n0 + wn + b * 32 + (lane & 31);
       colok[b] = col[b] < N;
+      fz[b] = col[b] < ep.nfuse;
       bcol[b] = (bias && colok[b]) ? bias[col[b]] : 0.f;
       ss[b] = 0.f;
       f1[b] = 0.f;
@@ -128,10 +129,11 @@ __device__ __forceinline__ void store_rows(const AccT (&acc)[2][NB], int64_t m0,
     auto flush = [&]() {
       const int64_t first = (ep.gd.row(cur) + 2) >> 6;  // block of the group's first jacobian row
       const int slot = 2 * (int)((rb >> 6) - first) + h;
+      if (slot >= ACT_EPI_SLOTS) return;  // not within the neighbour limit (at most 4 blocks)
       float* dst = ep.part + ((int64_t)cur * ACT_EPI_SLOTS + slot) * ep.ldf;
 #pragma unroll
       for (int b = 0; b < NB; ++b)
-        if (colok[b]) dst[col[b]] = ss[b];
+        if (colok[b] && fz[b]) dst[col[b]] = ss[b];
     };
 #pragma unroll
     for (int a = 0; a < 2; ++a)
@@ -146,7 +148,7 @@ __device__ __forceinline__ void store_rows(const AccT (&acc)[2][NB], int64_t m0,
 #pragma unroll
           for (int b = 0; b < NB; ++b) {
             ss[b] = 0.f;
-            f1[b] = colok[b] ? ep.f1[(int64_t)cd * ep.ldf + col[b]] : 0.f;
+            f1[b] = colok[b] && fz[b] ? ep.f1[(int64_t)cd * ep.ldf + col[b]] : 0.f;
           }
         }
         float* crow = C + (rb + rl) * ldc;
@@ -154,7 +156,9 @@ __device__ __forceinline__ void store_rows(const AccT (&acc)[2][NB], int64_t m0,
         for (int b = 0; b < NB; ++b) {
           if (!colok[b]) continue;
           float z = acc[a][b][v];
-          if (cd >= 0) {
+          if (!fz[b]) {
+            if (cd == -1) z += bcol[b];
+          } else if (cd >= 0) {
             ss[b] += z * z;
             z *= f1[b];
           } else if (cd == -1) {
@@ -541,15 +545,16 @@ void launch_gemm_f32(hipStream_t s, const float* A, int64_t lda, const float* W,
                      float* C, int64_t ldc, int64_t M, int N, int K, const uint8_t* rowflag, int period, DevRows dr,
                      ActEpi ep) {
   if (M == 0) return;
-  if (ep.rowgrp && (N <= 16 || (N % GT_N) != 0)) abort();  // the fused epilogue needs whole 128-wide tiles
+  if (ep.rowgrp && N <= 16) abort();  // the fused epilogue is in the MFMA kernel only
   if (N <= 16) {
     const int64_t threads = M * 64;
     k_gemm_smalln<<<(unsigned)((threads + 255) / 256), 256, 0, s>>>(A, lda, W, bias, C, ldc, M, N, K, rowflag, period,
                                                                       dr);
     return;
   }
-  // a remainder of at most 16 columns past whole 128-wide tiles goes to the tail kernel
-  const int rem = N % GT_N, Nm = rem > 0 && rem <= 16 ? N - rem : N;
+  // a remainder of at most 16 columns past whole 128-wide tiles goes to the tail kernel (not
+  // with the fused epilogue, whose columns all go through the MFMA kernel)
+  const int rem = N % GT_N, Nm = rem > 0 && rem <= 16 && !ep.rowgrp ? N - rem : N;
   const int nT = (Nm + GT_N - 1) / GT_N;
   const int64_t blocks = ((M + GT_M - 1) / GT_M) * nT;
   if (ep.rowgrp) {

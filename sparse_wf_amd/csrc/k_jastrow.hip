@@ -233,12 +233,12 @@ void launch_dep_gather(hipStream_t s, const float* X, int64_t ld, int ncols, Gro
 // Attention Jastrow per group: att = exp(-a / sqrt(Dk)) and vals = att * v with
 // forward-Laplacian rows, where a = sum_k q[h,k] keys[h,k] arrives precomputed per row and
 // head (the key-weighted head sums of the query projection are a linear map of the features,
-// contracted as such).  V rows [H*Dk] and QA rows [H] at row stride ldq (one buffer with QA
-// behind V); AV rows = [att (H) | vals (H*Dk)].
+// contracted as such).  V rows [H*Dk] at row stride ldv, QA rows [H] at row stride ldq;
+// AV rows = [att (H) | vals (H*Dk)].
 // Thread (h, k) owns column h*Dk + k, so every row is read once with coalesced loads.
 template <bool TAN>
-__global__ void __launch_bounds__(256) k_att_group(const float* __restrict__ V, const float* __restrict__ QA, int ldq,
-                                                   int H, int Dk, GroupDesc gd, float* __restrict__ AV,
+__global__ void __launch_bounds__(256) k_att_group(const float* __restrict__ V, int ldv, const float* __restrict__ QA,
+                                                   int ldq, int H, int Dk, GroupDesc gd, float* __restrict__ AV,
                                                    const int32_t* __restrict__ dG) {
   const int g = blockIdx.x;
   if (dG && g >= *dG) return;
@@ -251,7 +251,7 @@ __global__ void __launch_bounds__(256) k_att_group(const float* __restrict__ V, 
   const int lda = H + HD;
   const float ax = QA[r0 * ldq + h];
   const float f = expf(-sc * ax), f1 = -sc * f, f2 = sc * sc * f;
-  const float vx = V[r0 * ldq + col];
+  const float vx = V[r0 * ldv + col];
   AV[r0 * lda + H + col] = f * vx;
   if (k == 0) AV[r0 * lda + h] = f;
   if (!TAN) return;
@@ -271,26 +271,26 @@ __global__ void __launch_bounds__(256) k_att_group(const float* __restrict__ V, 
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
       q[u] = QA[(rr + u) * ldq + h];
-      v[u] = V[(rr + u) * ldq + col];
+      v[u] = V[(rr + u) * ldv + col];
     }
 #pragma unroll
     for (int u = 0; u < 8; ++u) row(rr + u, q[u], v[u]);
   }
-  for (; rr < r1; ++rr) row(rr, QA[rr * ldq + h], V[rr * ldq + col]);
+  for (; rr < r1; ++rr) row(rr, QA[rr * ldq + h], V[rr * ldv + col]);
   const float al = a_l * f1 + f2 * ss;
-  const float vl = V[(r0 + 1) * ldq + col];
+  const float vl = V[(r0 + 1) * ldv + col];
   AV[(r0 + 1) * lda + H + col] = al * vx + f * vl + 2.f * cross;
   if (k == 0) AV[(r0 + 1) * lda + h] = al;
 }
 
-void launch_att_group(hipStream_t s, const float* V, const float* QA, int ldq, int H, int Dk, int G, GroupDesc gd,
-                      bool tangents, float* AV, const int32_t* dG) {
+void launch_att_group(hipStream_t s, const float* V, int ldv, const float* QA, int ldq, int H, int Dk, int G,
+                      GroupDesc gd, bool tangents, float* AV, const int32_t* dG) {
   if (G == 0) return;
   const int threads = ((H * Dk + 63) / 64) * 64;
   if (tangents)
-    k_att_group<true><<<G, threads, 0, s>>>(V, QA, ldq, H, Dk, gd, AV, dG);
+    k_att_group<true><<<G, threads, 0, s>>>(V, ldv, QA, ldq, H, Dk, gd, AV, dG);
   else
-    k_att_group<false><<<G, threads, 0, s>>>(V, QA, ldq, H, Dk, gd, AV, dG);
+    k_att_group<false><<<G, threads, 0, s>>>(V, ldv, QA, ldq, H, Dk, gd, AV, dG);
 }
 
 // Readout input u[s,h,k] = vals[s,h,k] / norm[s,h] with dense forward-Laplacian over the

@@ -33,6 +33,10 @@ CASES = {
     # dependency sets of up to 50 members: the 2-, 3- and 4-tile classes of the determinant
     # Laplacian, on its vector path (N even, D N a multiple of 4)
     "alkane130_dense": lambda: alkane_cfg(cutoff=7.0, n_determinants=2),
+    # F = W = 128: the silu of the update-MLP and first Jastrow layers in the contraction
+    # epilogue, the first Jastrow layer joint with the attention values, the head sums behind
+    # the orbital rows
+    "small_wide": lambda: small_cfg(False, feature_dim=128, mlp_width=128),
 }
 
 
@@ -158,7 +162,7 @@ def test_fused_silu_epilogue_matches_separate_pass(gpu_device, precision, monkey
     default) against the separate element-wise pass (FIRE_FUSED_ACT=0), same walkers: they differ
     only in the order of the sum of squares over each group's jacobian rows and in where silu'
     of the value row is taken from (a separate FP32 contraction of the value rows)."""
-    cfg = alkane_cfg()
+    cfg = alkane_cfg(feature_dim=128, mlp_width=128)  # F, W multiples of 128: the fused forms apply
     prm = make_params(cfg, seed=0)
     r = walkers(cfg, 4, seed=3)
     outs = []
