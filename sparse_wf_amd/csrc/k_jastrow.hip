@@ -176,7 +176,9 @@ void launch_jmlp_last(hipStream_t s, const float* Y, int W, int G, GroupDesc gd,
 __global__ void k_node_sums(const float* __restrict__ X, int64_t ld, int ncols, GroupDesc gd, int B, int N, int n_up,
                             int tangents, double* __restrict__ out) {
   const int w = blockIdx.x;
-  for (int c = threadIdx.x; c < ncols; c += blockDim.x) {
+  {
+    const int c = blockIdx.y * blockDim.x + threadIdx.x;  // one column per thread
+    if (c >= ncols) return;
     double s[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
     for (int i = 0; i < N; ++i) {
       const int g = group_of(w, i, B, n_up, N - n_up);
@@ -193,7 +195,7 @@ __global__ void k_node_sums(const float* __restrict__ X, int64_t ld, int ncols, 
 void launch_node_sums(hipStream_t s, const float* X, int64_t ld, int ncols, GroupDesc gd, int B, int N, int n_up,
                       double* out) {
   const int tangents = gd.a != 1 || gd.b != 0;
-  k_node_sums<<<B, 256, 0, s>>>(X, ld, ncols, gd, B, N, n_up, tangents, out);
+  k_node_sums<<<dim3(B, (ncols + 63) / 64), 64, 0, s>>>(X, ld, ncols, gd, B, N, n_up, tangents, out);
 }
 
 // Dense jacobian of a sum over electrons: for dependency n, sum the jacobian rows d X_i/d r_n
@@ -227,7 +229,9 @@ __global__ void k_dep_gather(const float* __restrict__ X, int64_t ld, int ncols,
 
 void launch_dep_gather(hipStream_t s, const float* X, int64_t ld, int ncols, GroupDesc gd, const int64_t* ptr,
                        const int32_t* nbr, const int32_t* rev, int B, int N, int n_up, double* out) {
-  k_dep_gather<<<B * N, 256, 0, s>>>(X, ld, ncols, gd, ptr, nbr, rev, B, N, n_up, out);
+  // every column in one pass: a narrow second pass would repeat the whole gather's latency
+  const int threads = ncols > 1024 ? 1024 : (ncols + 63) / 64 * 64;
+  k_dep_gather<<<B * N, threads, 0, s>>>(X, ld, ncols, gd, ptr, nbr, rev, B, N, n_up, out);
 }
 
 // Attention Jastrow per group: att = exp(-a / sqrt(Dk)) and vals = att * v with
@@ -301,14 +305,16 @@ __global__ void k_att_readin(const double* __restrict__ NS, const double* __rest
   const int w = blockIdx.x;
   const int HD = H * Dk, ncol = H + HD;
   const int nrow = TAN ? 2 + 3 * N : 1;
-  for (int col = threadIdx.x; col < 2 * HD; col += blockDim.x) {
+  {
+    const int col = blockIdx.y * blockDim.x + threadIdx.x;  // one column per thread
+    if (col >= 2 * HD) return;
     const int sp = col / HD, hk = col - sp * HD, h = hk / Dk;
     const double* ns = NS + ((size_t)w * 2 + sp) * 2 * ncol;
     const double nx = ns[h], vx = ns[H + hk];
     const double rinv = 1.0 / nx;
     float* out = RIN + (size_t)w * nrow * 2 * HD + col;
     out[0] = (float)(vx * rinv);
-    if (!TAN) continue;
+    if (!TAN) return;
     const double nl = ns[ncol + h], vl = ns[ncol + H + hk];
     double sn2 = 0.0, cross = 0.0;
     const double* dg = DG + ((size_t)w * 2 + sp) * N * 3 * ncol;
@@ -326,10 +332,11 @@ __global__ void k_att_readin(const double* __restrict__ NS, const double* __rest
 
 void launch_att_readin(hipStream_t s, const double* NS, const double* DG, int B, int N, int H, int Dk, bool tangents,
                        float* RIN) {
+  const dim3 grid(B, (2 * H * Dk + 63) / 64);
   if (tangents)
-    k_att_readin<true><<<B, 256, 0, s>>>(NS, DG, N, H, Dk, RIN);
+    k_att_readin<true><<<grid, 64, 0, s>>>(NS, DG, N, H, Dk, RIN);
   else
-    k_att_readin<false><<<B, 256, 0, s>>>(NS, DG, N, H, Dk, RIN);
+    k_att_readin<false><<<grid, 64, 0, s>>>(NS, DG, N, H, Dk, RIN);
 }
 
 // Per (walker, electron i): e-e cusp value/Laplacian (pairs j > i), cusp gradient on i
