@@ -215,12 +215,29 @@ __global__ void k_dep_gather(const float* __restrict__ X, int64_t ld, int ncols,
       const int sp = n < n_up ? 0 : 1;
       for (int x = 0; x < 3; ++x) s[sp][x] += X[(rr + x) * ld + c];
     }
-    for (int a = 0; a < nb; ++a) {
-      const int i = nbr[p0 + a];
-      const int gi = group_of(w, i, B, n_up, N - n_up);
-      const int64_t rr = gd.row(gi) + 5 + 3 * (int64_t)rev[p0 + a];
-      const int sp = i < n_up ? 0 : 1;
-      for (int x = 0; x < 3; ++x) s[sp][x] += X[(rr + x) * ld + c];
+    // dependencies 8 at a time: all their rows are loaded before they are summed (in order)
+    for (int a0 = 0; a0 < nb; a0 += 8) {
+      float v[8][3];
+      int spu[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        spu[u] = -1;
+        if (a0 + u < nb) {
+          const int i = nbr[p0 + a0 + u];
+          const int gi = group_of(w, i, B, n_up, N - n_up);
+          const int64_t rr = gd.row(gi) + 5 + 3 * (int64_t)rev[p0 + a0 + u];
+          spu[u] = i < n_up ? 0 : 1;
+#pragma unroll
+          for (int x = 0; x < 3; ++x) v[u][x] = X[(rr + x) * ld + c];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+#pragma unroll
+        for (int x = 0; x < 3; ++x) {
+          if (spu[u] == 0) s[0][x] += v[u][x];
+          if (spu[u] == 1) s[1][x] += v[u][x];
+        }
     }
     for (int sp = 0; sp < 2; ++sp)
       for (int x = 0; x < 3; ++x) out[((((size_t)w * 2 + sp) * N + n) * 3 + x) * ncols + c] = s[sp][x];
