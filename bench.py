@@ -230,8 +230,10 @@ def main():
     table = (ctypes.c_int32 * (7 * cap))()
     n_tab = ctypes.c_int32()
     _lib.check(wf._lib.fire_autotune(wf._h, -1, table, cap, ctypes.byref(n_tab)))
+    # log2 M carries +64 for the fused-activation epilogue and +32 for device-counted rows
     roofline["autotuned_tiles"] = [
-        {"split": table[7 * i], "N": table[7 * i + 1], "K": table[7 * i + 2], "log2_M": table[7 * i + 3],
+        {"split": table[7 * i], "N": table[7 * i + 1], "K": table[7 * i + 2], "log2_M": table[7 * i + 3] % 32,
+         "fused_silu": table[7 * i + 3] >= 64, "device_rows": (table[7 * i + 3] // 32) % 2 == 1,
          "tile": f"{table[7 * i + 5]}x{table[7 * i + 6]}"} for i in range(min(n_tab.value, cap))]
 
     e_m = out_m["e_loc"].cpu().numpy()

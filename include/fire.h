@@ -166,9 +166,11 @@ int fire_precision_plan(fire_handle* h, const int32_t* planes, int32_t n, int32_
 /* Tile-shape autotuner of the dense contractions (no counterpart in the reference; the
  * reference leaves kernel selection to XLA).  On by default: the first BF16 contraction of
  * each (split, N, K, floor(log2 M)) shape times every tile shape on its real operands
- * (synchronising) and keeps the fastest for the life of the handle.  enable: 1 on, 0 off
- * (tile 128 x 128), 2 on and forget earlier choices, -1 unchanged.  table (optional, int32
- * [cap][7]) receives the current choices as (split, N, K, log2 M, variant, BM, BN);
+ * (synchronising) and keeps the fastest for the life of the process, shared by every handle
+ * on the same device.  enable: 1 on, 0 off (tile 128 x 128), 2 on and forget earlier choices
+ * (this device's), -1 unchanged.  table (optional, int32 [cap][7]) receives this handle's
+ * choices as (split, N, K, log2 M, variant, BM, BN); log2 M is raised by 32 for row counts held
+ * on the device and by 64 for contractions with the fused activation epilogue;
  * *n_entries the number of choices. */
 int fire_autotune(fire_handle* h, int32_t enable, int32_t* table, int32_t cap, int32_t* n_entries);
 

@@ -8,6 +8,7 @@
 #include <stdexcept>
 #include <string>
 #include <map>
+#include <mutex>
 #include <tuple>
 #include <vector>
 
@@ -19,6 +20,17 @@
 namespace {
 
 thread_local std::string g_last_error;
+
+// Tile shapes chosen by the autotuner, kept for the process per device: every handle (one per
+// geometry in ragged batches and in the serving loop) reuses a shape another one measured.
+struct TileCache {
+  std::mutex mu;
+  std::map<std::tuple<int, int, int, int, int>, int> best;  // (device, split, N, K, log2 M) -> variant
+};
+TileCache& tile_cache() {
+  static TileCache c;
+  return c;
+}
 
 struct FireError : std::runtime_error {
   using std::runtime_error::runtime_error;
@@ -324,11 +336,22 @@ void gemm(Ctx& c, const float* A, int64_t lda, int wi, int bi, float* C, int64_t
     if (dr.count) lg += 32;  // rows counted on the device (M is the capacity): tuned apart
     if (ep.rowgrp) lg += 64;  // fused silu epilogue: tuned apart
     const auto key = std::make_tuple(ns, N, K, lg);
+    const auto gkey = std::make_tuple(h->device, ns, N, K, lg);
     auto it = h->gemm_tile.find(key);
-    if (it != h->gemm_tile.end()) {
+    bool known = it != h->gemm_tile.end();
+    if (known) {
       variant = it->second;
-    } else if (!c.fixed) {
-      // time every tile shape twice on these operands (the second launch counts)
+    } else {
+      std::lock_guard<std::mutex> lk(tile_cache().mu);
+      auto g = tile_cache().best.find(gkey);
+      if (g != tile_cache().best.end()) {
+        variant = g->second;
+        h->gemm_tile[key] = variant;
+        known = true;
+      }
+    }
+    if (!known && !c.fixed) {
+      // time every tile shape on these operands: one untimed launch, then the faster of two
       const int nv = gemm_bf16_variants();
       std::vector<float> ms(nv, 0.f);
       hipEvent_t t0, t1;
@@ -341,18 +364,25 @@ void gemm(Ctx& c, const float* A, int64_t lda, int wi, int bi, float* C, int64_t
         }
         launch_gemm_bf16(c.s, A, lda, h->Pbf[wi], h->Pbf_lo[wi], h->Pbf_lo2[wi], bias, C, ldc, M, N, K, rowflag,
                          period, ns, v, dr, ep);
-        HIPCHK(hipEventRecord(t0, c.s));
-        launch_gemm_bf16(c.s, A, lda, h->Pbf[wi], h->Pbf_lo[wi], h->Pbf_lo2[wi], bias, C, ldc, M, N, K, rowflag,
-                         period, ns, v, dr, ep);
-        HIPCHK(hipEventRecord(t1, c.s));
-        HIPCHK(hipEventSynchronize(t1));
-        HIPCHK(hipEventElapsedTime(&ms[v], t0, t1));
+        ms[v] = 1e30f;
+        for (int rep = 0; rep < 2; ++rep) {
+          HIPCHK(hipEventRecord(t0, c.s));
+          launch_gemm_bf16(c.s, A, lda, h->Pbf[wi], h->Pbf_lo[wi], h->Pbf_lo2[wi], bias, C, ldc, M, N, K, rowflag,
+                           period, ns, v, dr, ep);
+          HIPCHK(hipEventRecord(t1, c.s));
+          HIPCHK(hipEventSynchronize(t1));
+          float t = 0.f;
+          HIPCHK(hipEventElapsedTime(&t, t0, t1));
+          ms[v] = std::min(ms[v], t);
+        }
         if (ms[v] < ms[variant]) variant = v;
       }
       HIPCHK(hipEventDestroy(t0));
       HIPCHK(hipEventDestroy(t1));
       h->gemm_tile[key] = variant;
       h->gemm_tile_ms[key] = ms;
+      std::lock_guard<std::mutex> lk(tile_cache().mu);
+      tile_cache().best[gkey] = variant;
     }
   }
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1723,6 +1753,9 @@ int fire_autotune(fire_handle* h, int32_t enable, int32_t* table, int32_t cap, i
     if (enable == 2) {
       h->gemm_tile.clear();
       h->gemm_tile_ms.clear();
+      std::lock_guard<std::mutex> lk(tile_cache().mu);
+      auto& b = tile_cache().best;
+      for (auto i = b.begin(); i != b.end();) i = std::get<0>(i->first) == h->device ? b.erase(i) : std::next(i);
     }
     return 0;
   } catch (const std::exception& e) {

@@ -354,24 +354,30 @@ __global__ void k_act_lapfix(float* __restrict__ C, int64_t ldc, int N, GroupDes
   if (c >= N) return;
   const int64_t r0 = gd.row(g), r1 = gd.row(g + 1);
   if (r1 - r0 < 2) return;
-  float f, f1, f2;
-  silu3(z0[(int64_t)g * ldf + c], f, f1, f2);
-  float ss = 0.f;
+  float* y = C + (r0 + 1) * ldc + c;
+  const float zl = *y;
+  const float zv = z0[(int64_t)g * ldf + c];
+  // which slots exist (uniform over the block): bit 2 (block - first block) + half
+  unsigned mask = 0u;
   const int64_t js = r0 + 2, je = r1;
   if (je > js) {
     const int64_t b0 = js >> 6, b1 = (je - 1) >> 6;
-    for (int64_t blk = b0; blk <= b1; ++blk) {
+    for (int64_t blk = b0; blk <= b1 && blk - b0 < ACT_EPI_SLOTS / 2; ++blk) {
       const int64_t s0 = js > blk * 64 ? js : blk * 64, s1 = je < blk * 64 + 64 ? je : blk * 64 + 64;
-      for (int hh = 0; hh < 2; ++hh) {
-        bool has = false;
-        for (int64_t r = s0; r < s1 && r < s0 + 8; ++r) has |= (int)((r >> 2) & 1) == hh;
-        const int slot = 2 * (int)(blk - b0) + hh;
-        if (has && slot < ACT_EPI_SLOTS) ss += part[((int64_t)g * ACT_EPI_SLOTS + slot) * ldf + c];
-      }
+      for (int64_t r = s0; r < s1 && r < s0 + 8; ++r) mask |= 1u << (2 * (blk - b0) + ((r >> 2) & 1));
     }
   }
-  float* y = C + (r0 + 1) * ldc + c;
-  *y = *y * f1 + f2 * ss;
+  // all slot loads issued before the sum (in slot order)
+  const float* pg = part + (int64_t)g * ACT_EPI_SLOTS * ldf + c;
+  float v[ACT_EPI_SLOTS];
+#pragma unroll
+  for (int q = 0; q < ACT_EPI_SLOTS; ++q) v[q] = (mask >> q) & 1u ? pg[(int64_t)q * ldf] : 0.f;
+  float ss = 0.f;
+#pragma unroll
+  for (int q = 0; q < ACT_EPI_SLOTS; ++q) ss += v[q];
+  float f, f1, f2;
+  silu3(zv, f, f1, f2);
+  *y = zl * f1 + f2 * ss;
 }
 
 void launch_act_lapfix(hipStream_t s, float* C, int64_t ldc, int N, int G, GroupDesc gd, const float* z0,
