@@ -208,8 +208,9 @@ def main():
     g_achieved = (g_flops / (g_ms * 1e-3)) / 1e12 if g_ms > 0 else None
     roofline = {
         "bound": "mfma",
-        "kernel": "k_gemm_bf16 / k_gemm_bf16_db (every dense contraction of one mixed step: embedding and orbital "
-                  "contractions on three BF16 planes, Jastrow and attention on two; autotuned tile shapes)",
+        "kernel": "k_gemm_bf16* (every dense contraction of one mixed step: embedding and orbital "
+                  "contractions on three BF16 planes, Jastrow and attention on two; autotuned tile shapes; "
+                  "the update-MLP and first Jastrow layers with the forward-Laplacian silu in their epilogue)",
         "achieved": g_achieved,
         "peak": PEAK["bf16"],
         "unit": "TFLOP/s",
