@@ -107,6 +107,34 @@ def reference_pair_lists(r, n_up, cutoff):
     return same, diff, jac
 
 
+def static_input(r_old, n_up, cutoff, R, cutoff_1el, r_new=None, idx_changed=None, laplacian=True):
+    """get_static_input (new_sparse_model.py:399-437) of one walker, as a dict of the
+    StaticArgs fields: pair counts by spin, triplets sum_{a,b,c} in(a,c) in(b,c), the largest
+    number of nuclei within cutoff_1el of an electron, and for a move of electron
+    ``idx_changed`` from r_old to r_new the affected electrons (neighbours of the moved one
+    before or after, plus itself) and the pairs of the new graph centred on them."""
+    N = len(r_old)
+    r = np.asarray(r_old if r_new is None else r_new, np.float32)
+    spin = np.arange(N) < n_up
+    same = spin[:, None] == spin[None, :]
+    inc = pair_distances_f32(r) < np.float32(cutoff)
+    dd = r[:, None, :] - np.asarray(R, np.float32)[None]
+    sq = dd * dd
+    den = np.sqrt((sq[..., 0] + sq[..., 1]) + sq[..., 2])
+    out = dict(n_pairs_same=int((inc & same).sum()), n_pairs_diff=int((inc & ~same).sum()),
+               n_triplets=int((inc[:, None, :] & inc[None, :, :]).sum()) if laplacian else 0,
+               n_neighbours_en=int((den < np.float32(cutoff_1el)).sum(1).max()),
+               n_changed_hout=0, n_changed_pair_same=0, n_changed_pair_diff=0, n_pp_elecs=N)
+    if r_new is not None:
+        inc_old = pair_distances_f32(np.asarray(r_old, np.float32)) < np.float32(cutoff)
+        aff = inc[:, idx_changed] | inc_old[:, idx_changed]
+        aff[idx_changed] = True
+        req = inc & aff[:, None]
+        out.update(n_changed_hout=int(aff.sum()), n_changed_pair_same=int((req & same).sum()),
+                   n_changed_pair_diff=int((req & ~same).sum()))
+    return out
+
+
 def n_triplets(r, cutoff):
     """get_static_input n_triplets (new_sparse_model.py:412)."""
     inc = pair_distances_f32(r) < np.float32(cutoff)
@@ -130,16 +158,32 @@ def _log_scaled(feat, d):
     return feat / dd * fl.log1p(dd)
 
 
-def elec_init(p, cfg, r, R, en_mask, learned=None):
-    """ElecInit.__call__ (new_sparse_model.py:190-215) with ElecNucEdge (:139-158) and the
-    dynamic e-n PairwiseFilter (utils.py:393-433).  ``r``: [N,3] (Tensor, or FL whose
-    tangents are per-electron), ``en_mask`` [N,M] selects nuclei within cutoff_1el."""
+def layer_norm(h, scale, bias, eps=1e-6):
+    """flax ``nn.LayerNorm()`` over the feature axis (biased variance, eps = 1e-6 inside the
+    square root), as used by ElecInit (new_sparse_model.py:209)."""
+    F = h.shape[-1]
+    mu = h.sum(-1) / F
+    c = h - mu.unsqueeze(-1)
+    var = fl.square(c).sum(-1) / F
+    inv = fl.reciprocal(fl.sqrt(var + eps))
+    return c * inv.unsqueeze(-1) * scale + bias
+
+
+def glu(h, kernel):
+    """GatedLinearUnit (utils.py:55-62): Dense(2F, no bias), split into (x, gate), x * silu(gate)."""
+    F = kernel.shape[-1] // 2
+    g2 = fl.linear(h, kernel)
+    return g2[:, :F] * fl.silu(g2[:, F:])
+
+
+def en_edge_sum(p, cfg, r, R, en_mask):
+    """sum_J Gamma_J * e_J of ElecInit (new_sparse_model.py:204-206) over the nuclei selected
+    by ``en_mask`` [N,M], with ElecNucEdge (:139-158) and the dynamic e-n PairwiseFilter
+    (utils.py:393-433).  ``r``: [N,3] (Tensor, or FL whose tangents are per-electron)."""
     ep = p["embedding"]
     dyn = ep["dynamic_params_en"]
-    ei = ep["elec_init"]["params"]
-    edge = ei["ElecNucEdge_0"]
+    edge = ep["elec_init"]["params"]["ElecNucEdge_0"]
     pf = edge["PairwiseFilter_0"]
-    F = cfg.feature_dim
     diff = r.unsqueeze(1) - R.unsqueeze(0) if isinstance(r, FL) else r[:, None, :] - R[None, :, :]
     feat, d = _diff_features(diff)  # [N,M,4], [N,M]
     # directional MLP with nucleus-dependent kernel/bias
@@ -153,20 +197,27 @@ def elec_init(p, cfg, r, R, en_mask, learned=None):
     gamma = fl.linear(beta, edge["Dense_0"]["kernel"])
     e = fl.linear(_log_scaled(feat, d), edge["Dense_1"]["kernel"]) + dyn["nuc_embedding"]
     m = en_mask.to(gamma.dtype).unsqueeze(-1)
-    h = (gamma * m * e).sum(1)  # [N,F]
-    # LayerNorm (flax, eps=1e-6)
-    mu = h.sum(-1) / F
-    c = h - mu.unsqueeze(-1)
-    var = fl.square(c).sum(-1) / F
-    inv = fl.reciprocal(fl.sqrt(var + 1e-6))
-    h = c * inv.unsqueeze(-1) * ei["LayerNorm_0"]["scale"] + ei["LayerNorm_0"]["bias"]
-    # GatedLinearUnit (utils.py:55-62)
-    g2 = fl.linear(h, ei["GatedLinearUnit_0"]["Dense_0"]["kernel"])
-    h = g2[:, :F] * fl.silu(g2[:, F:])
+    return (gamma * m * e).sum(1)  # [N,F]
+
+
+def elec_init_mlp(ei, h):
+    """The dense stage of ElecInit (new_sparse_model.py:207-215) on the e-n sum h [N,F]:
+    LayerNorm, GatedLinearUnit, silu(Dense_F), Dense_5F split into
+    (h0, msg_ct_same, msg_ct_diff, msg_nb_same, msg_nb_diff) -- the order of :213-215 as
+    unpacked by NewSparseEmbedding.apply (:516)."""
+    F = h.shape[-1]
+    h = layer_norm(h, ei["LayerNorm_0"]["scale"], ei["LayerNorm_0"]["bias"])
+    h = glu(h, ei["GatedLinearUnit_0"]["Dense_0"]["kernel"])
     h = fl.silu(fl.linear(h, ei["Dense_0"]["kernel"], ei["Dense_0"]["bias"]))
     out = fl.linear(h, ei["Dense_1"]["kernel"], ei["Dense_1"]["bias"])
-    # split order: h0, msg_ct_same, msg_ct_diff, msg_nb_same, msg_nb_diff (:213-215)
     return [out[:, k * F:(k + 1) * F] for k in range(5)]
+
+
+def elec_init(p, cfg, r, R, en_mask, learned=None):
+    """ElecInit.__call__ (new_sparse_model.py:190-215): the e-n edge sum followed by the
+    dense stage."""
+    h = en_edge_sum(p, cfg, r, R, en_mask)
+    return elec_init_mlp(p["embedding"]["elec_init"]["params"], h)
 
 
 # -----------------------------------------------------------------------------------------
@@ -357,6 +408,22 @@ def att_readout(jp, cfg, norm, values):
     return x * a["scale"] + zb
 
 
+def attention_norm_values(jp, cfg, h):
+    """GlobalAttentionJastrow.attention_and_values and contract (jastrow.py:84-90, 102-152),
+    value path: per electron attention exp(-q.key / sqrt(Dk)) [N,H] and attention * values
+    [N,H,Dk], and their per-spin sums (norm [2,H], values [2,H,Dk]; up first)."""
+    a = jp["att"]
+    N, n_up = h.shape[0], cfg.n_up
+    H, Dk = cfg.attention_heads, cfg.attention_dim
+    q = (h @ a["Q_W"]["kernel"] + a["Q_W"]["bias"]).reshape(N, H, Dk)
+    v = (h @ a["V_W"]["kernel"] + a["V_W"]["bias"]).reshape(N, H, Dk)
+    att = torch.exp(-(q * a["register_keys"]).sum(-1) / math.sqrt(Dk))
+    vals = att.unsqueeze(-1) * v
+    norm = torch.stack([att[:n_up].sum(0), att[n_up:].sum(0)])
+    values = torch.stack([vals[:n_up].sum(0), vals[n_up:].sum(0)])
+    return att, vals, norm, values
+
+
 def mlp_to_logpsi(cfg, J):
     """Jastrow._mlp_to_logpsi (jastrow.py:372-382)."""
     logpsi = 0.0
@@ -382,14 +449,7 @@ def jastrow_apply(p, cfg, r, h, return_state=False):
     J = torch.zeros(2, dtype=dtype)
     att = vals = one_el = None
     if cfg.use_attention:
-        a = jp["att"]
-        H, Dk = cfg.attention_heads, cfg.attention_dim
-        q = (h @ a["Q_W"]["kernel"] + a["Q_W"]["bias"]).reshape(N, H, Dk)
-        v = (h @ a["V_W"]["kernel"] + a["V_W"]["bias"]).reshape(N, H, Dk)
-        att = torch.exp(-(q * a["register_keys"]).sum(-1) / math.sqrt(Dk))
-        vals = att.unsqueeze(-1) * v
-        norm = torch.stack([att[:n_up].sum(0), att[n_up:].sum(0)])
-        values = torch.stack([vals[:n_up].sum(0), vals[n_up:].sum(0)])
+        att, vals, norm, values = attention_norm_values(jp, cfg, h)
         J = J + att_readout(jp, cfg, norm, values)
     if cfg.use_mlp_jastrow or cfg.use_log_jastrow:
         x = h

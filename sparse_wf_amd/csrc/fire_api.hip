@@ -4,6 +4,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -886,6 +888,8 @@ void state_init(Ctx& c, const float* r_dev, int B) {
   // proposal buffers at their static capacities (no allocation while proposing, so a sweep
   // can be captured into a graph): at most cap changed electrons per walker and move
   S.cap = std::min(N, 64);
+  // a smaller capacity (FIRE_STATE_CAP) lets a test drive a move past it
+  if (const char* e = getenv("FIRE_STATE_CAP")) S.cap = std::max(1, std::min(S.cap, atoi(e)));
   S.tcap = (int64_t)B * S.cap;
   const int64_t T = S.tcap;
   const int ME = h->M * h->Eiso, XW = std::max(F, std::max(W, 2 * HD)), amax = S.cap;
@@ -897,7 +901,7 @@ void state_init(Ctx& c, const float* r_dev, int B) {
   for (DevBuf* b : {&S.cu, &S.cd, &S.offu, &S.offd}) b->ensure(sizeof(int32_t) * B);
   S.tail.ensure(sizeof(int32_t) * 4);
   S.dstats.ensure(sizeof(int64_t) * 2);
-  S.doverflow.ensure(sizeof(int32_t));
+  S.doverflow.ensure(sizeof(int32_t) * 2);
   S.dstep.ensure(sizeof(int64_t));
   S.list.ensure(sizeof(int2) * (size_t)T);
   S.rowof.ensure(sizeof(int32_t) * (size_t)B * N);
@@ -943,7 +947,7 @@ void state_init(Ctx& c, const float* r_dev, int B) {
   }
   for (DevBuf* x : {&S.p_ld, &S.p_sg}) x->ensure(sizeof(double) * (size_t)B * D);
   for (DevBuf* x : {&S.p_logpsi, &S.p_sign}) x->ensure(sizeof(double) * B);
-  launch_fill_i32(c.s, S.doverflow.as<int32_t>(), 1, 0);
+  launch_fill_i32(c.s, S.doverflow.as<int32_t>(), 2, 0);
   launch_fill_i32(c.s, reinterpret_cast<int32_t*>(S.dstats.p), 4, 0);
   launch_set_i64(c.s, S.dstep.as<int64_t>(), 0);
   S.ready = true;
@@ -965,7 +969,7 @@ void state_drop_pending(Ctx& c) {
 // Every launch is sized by the static capacities and reads the actual row counts from the
 // device, so the sequence has no host synchronisation and can be captured into a graph.
 void state_propose(Ctx& c, int k, int mode, const float* rk_in, float width, uint64_t seed, uint64_t step,
-                   int32_t* n_acc) {
+                   int32_t* n_acc, bool warmup = false) {
   fire_handle* h = c.h;
   auto& S = h->st;
   REQUIRE(S.ready, "no walker state (call fire_state_init first)");
@@ -985,11 +989,10 @@ void state_propose(Ctx& c, int k, int mode, const float* rk_in, float width, uin
   launch_mc_affected(c.s, S.r.as<float>(), S.p_r.as<float>(), B, N, nu, k, h->cfg.cutoff, S.cap, S.aff.as<int32_t>(),
                      S.cu.as<int32_t>(), S.cd.as<int32_t>(), S.doverflow.as<int32_t>());
   launch_mc_scan(c.s, S.cu.as<int32_t>(), S.cd.as<int32_t>(), B, S.offu.as<int32_t>(), S.offd.as<int32_t>(), tail,
-                 S.dstats.as<int64_t>());
+                 warmup ? nullptr : S.dstats.as<int64_t>());
   launch_mc_rowlist(c.s, S.aff.as<int32_t>(), S.cu.as<int32_t>(), S.cd.as<int32_t>(), S.offu.as<int32_t>(),
                     S.offd.as<int32_t>(), tail, B, N, nu, S.list.as<int2>(), S.rowof.as<int32_t>());
   S.k = k;
-  S.n_proposals += B;
 
   // h0 and messages of the moved electron (ElecInit on one electron per walker)
   const int b = h->iEN;
@@ -1120,11 +1123,15 @@ void state_commit(Ctx& c) {
 // affected set was truncated and the state is invalid).  Synchronises the stream.
 void state_check_overflow(Ctx& c) {
   auto& S = c.h->st;
-  int32_t ov = 0;
-  HIPCHK(hipMemcpyAsync(&ov, S.doverflow.p, sizeof(int32_t), hipMemcpyDeviceToHost, c.s));
+  int32_t ov[2] = {0, 0};
+  HIPCHK(hipMemcpyAsync(ov, S.doverflow.p, sizeof(ov), hipMemcpyDeviceToHost, c.s));
   HIPCHK(hipStreamSynchronize(c.s));
-  REQUIRE(ov == 0, "a move changed more than " + std::to_string(S.cap) +
-                       " electrons (static capacity of the low-rank update); the walker state is invalid");
+  if (ov[0] != 0) {
+    S.ready = S.pending = false;
+    throw FireError("a move changed " + std::to_string(ov[1]) + " electrons, more than the " +
+                    std::to_string(S.cap) +
+                    " of the low-rank update's static capacity; the walker state is invalid (call fire_state_init)");
+  }
 }
 
 // n_steps Metropolis steps from step0.  Whole sweeps starting at a multiple of n_el run as
@@ -1142,6 +1149,7 @@ void state_steps(Ctx& c, int64_t n_steps, float width, uint64_t seed, int64_t st
   while (s < end && (!use_graph || s % N != 0)) {
     state_propose(c, (int)(s % N), 0, nullptr, width, seed, (uint64_t)s, S.nacc.as<int32_t>());
     state_commit(c);
+    S.n_proposals += S.B;
     ++s;
   }
   const int64_t n_sweeps = (end - s) / N;
@@ -1153,9 +1161,8 @@ void state_steps(Ctx& c, int64_t n_steps, float width, uint64_t seed, int64_t st
       if (!S.gstream) HIPCHK(hipStreamCreateWithFlags(&S.gstream, hipStreamNonBlocking));
       // one eager zero-width proposal first (dropped, the state is unchanged), so that every
       // kernel attribute is set before the capture starts
-      state_propose(c, 0, 0, nullptr, 0.f, seed, 0, nullptr);
+      state_propose(c, 0, 0, nullptr, 0.f, seed, 0, nullptr, true);
       state_drop_pending(c);
-      S.n_proposals -= S.B;
       Ctx ce{h, S.gstream, c.prec, true};  // captured: no tuning or event timing
       HIPCHK(hipStreamSynchronize(c.s));
       HIPCHK(hipStreamBeginCapture(S.gstream, hipStreamCaptureModeRelaxed));
@@ -1171,6 +1178,7 @@ void state_steps(Ctx& c, int64_t n_steps, float width, uint64_t seed, int64_t st
       S.g_width = width;
       S.g_seed = seed;
       S.g_sig = S.signature();
+      if (getenv("FIRE_LOG_CAPTURE")) fprintf(stderr, "fire: captured a sweep graph (B=%d, N=%d)\n", S.B, N);
     }
     // the graph's proposals use step = *dstep + k with *dstep = first step of the sweep
     launch_set_i64(c.s, S.dstep.as<int64_t>(), s);
@@ -1189,8 +1197,29 @@ void state_steps(Ctx& c, int64_t n_steps, float width, uint64_t seed, int64_t st
   while (s < end) {
     state_propose(c, (int)(s % N), 0, nullptr, width, seed, (uint64_t)s, S.nacc.as<int32_t>());
     state_commit(c);
+    S.n_proposals += S.B;
     ++s;
   }
+}
+
+// CSR neighbour lists of B walkers in walker-major order (centre w * N + i) written straight
+// into the caller's counts [B * N] and, when nbr != NULL, nbr [cap]; ptr (int64 [B * N + 2])
+// receives the row pointer, the total and the largest count.  Returns the total, read back with
+// one stream synchronisation (the capacity check needs it before the fill pass).
+int64_t neighbours_walker_major(fire_handle* h, hipStream_t s, const float* r, int B, int32_t* counts, int32_t* nbr,
+                                int64_t cap, int64_t* ptr) {
+  const int N = h->N;
+  const int G = B * N;
+  launch_nbr_count(s, r, B, N, h->n_up, h->cfg.cutoff, counts, nullptr, 0, nullptr, true);
+  launch_scan(s, counts, G, ptr);
+  int64_t tail[2];
+  HIPCHK(hipMemcpyAsync(tail, ptr + G, sizeof(tail), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (nbr) {
+    REQUIRE(cap >= tail[0], "neighbour capacity too small: need " + std::to_string(tail[0]));
+    launch_nbr_fill(s, r, B, N, h->n_up, h->cfg.cutoff, ptr, nbr, nullptr, 0, nullptr, true);
+  }
+  return tail[0];
 }
 
 int fail(const std::exception& e) {
@@ -1447,11 +1476,13 @@ int fire_upload_params(fire_handle* h, const float* packed, const int64_t* sizes
     };
     for (auto& t : gw) upload_t(t.first, 1, t.second.first, t.second.second);
     for (int k : {h->iORB + 5, h->iORB + 7}) upload_t(k, h->D, h->M * h->Eiso, h->N);
-    // a captured sweep graph holds the old parameter pointers
+    // a captured sweep graph holds the old parameter pointers, and the resident walker state
+    // (embeddings, orbital rows, inverses, log psi) was computed with the old parameters
     if (h->st.graph) {
       HIPCHK(hipGraphExecDestroy(h->st.graph));
       h->st.graph = nullptr;
     }
+    h->st.ready = h->st.pending = false;
     h->cut_learned = host[h->iSCAL][0];
     h->s1 = host[h->iSCAL][1];
     h->params_ok = true;
@@ -1496,34 +1527,8 @@ int fire_neighbours(fire_handle* h, void* stream, const float* r_dev, int32_t B,
       *n_pairs = 0;
       return 0;
     }
-    Ctx c{h, (hipStream_t)stream, FIRE_FP32};
-    Graph g = build_graph(c, r_dev, B);
-    *n_pairs = g.n_pairs;
-    // counts in walker-major order (the group order is up-spin first)
-    std::vector<int32_t> cnt_g((size_t)B * h->N);
-    HIPCHK(hipMemcpyAsync(cnt_g.data(), h->cnt.p, sizeof(int32_t) * cnt_g.size(), hipMemcpyDeviceToHost, c.s));
-    std::vector<int64_t> ptr_g((size_t)B * h->N + 1);
-    HIPCHK(hipMemcpyAsync(ptr_g.data(), h->ptr.p, sizeof(int64_t) * ptr_g.size(), hipMemcpyDeviceToHost, c.s));
-    std::vector<int32_t> nb_g((size_t)std::max<int64_t>(g.n_pairs, 1));
-    if (g.n_pairs) HIPCHK(hipMemcpyAsync(nb_g.data(), h->nbr.p, sizeof(int32_t) * g.n_pairs, hipMemcpyDeviceToHost, c.s));
-    HIPCHK(hipStreamSynchronize(c.s));
-    std::vector<int32_t> cnt_w(cnt_g.size());
-    std::vector<int32_t> nb_w;
-    nb_w.reserve(g.n_pairs);
-    const int N = h->N, nu = h->n_up, nd = N - nu;
-    for (int w = 0; w < B; ++w)
-      for (int i = 0; i < N; ++i) {
-        const int gi = i < nu ? w * nu + i : B * nu + w * nd + (i - nu);
-        cnt_w[(size_t)w * N + i] = cnt_g[gi];
-        for (int64_t p = ptr_g[gi]; p < ptr_g[gi + 1]; ++p) nb_w.push_back(nb_g[p]);
-      }
-    HIPCHK(hipMemcpyAsync(counts_dev, cnt_w.data(), sizeof(int32_t) * cnt_w.size(), hipMemcpyHostToDevice, c.s));
-    if (nbr_dev) {
-      REQUIRE(cap >= g.n_pairs, "neighbour capacity too small: need " + std::to_string(g.n_pairs));
-      if (g.n_pairs)
-        HIPCHK(hipMemcpyAsync(nbr_dev, nb_w.data(), sizeof(int32_t) * g.n_pairs, hipMemcpyHostToDevice, c.s));
-    }
-    HIPCHK(hipStreamSynchronize(c.s));
+    h->ptr.ensure(sizeof(int64_t) * ((size_t)B * h->N + 2));
+    *n_pairs = neighbours_walker_major(h, (hipStream_t)stream, r_dev, B, counts_dev, nbr_dev, cap, h->ptr.as<int64_t>());
     return 0;
   } catch (const std::exception& e) {
     return fail(e);
@@ -1662,6 +1667,8 @@ int fire_state_propose(fire_handle* h, void* stream, int32_t k, const float* rk_
     HIPCHK(hipSetDevice(h->device));
     Ctx c{h, (hipStream_t)stream, h->st.prec};
     state_propose(c, k, 1, rk_dev, 0.f, 0, 0, nullptr);
+    h->st.n_proposals += h->st.B;
+    state_check_overflow(c);
     const int B = h->st.B;
     if (logpsi_dev)
       HIPCHK(hipMemcpyAsync(logpsi_dev, h->st.p_logpsi.p, sizeof(double) * B, hipMemcpyDeviceToDevice, c.s));
@@ -1679,6 +1686,7 @@ int fire_state_commit(fire_handle* h, void* stream, const int32_t* accept_dev) {
     REQUIRE(accept_dev, "null argument");
     HIPCHK(hipSetDevice(h->device));
     Ctx c{h, (hipStream_t)stream, h->st.prec};
+    REQUIRE(h->st.ready, "no walker state (call fire_state_init first)");
     REQUIRE(h->st.pending, "no pending proposal to commit");
     HIPCHK(hipMemcpyAsync(h->st.accept.p, accept_dev, sizeof(int32_t) * h->st.B, hipMemcpyDeviceToDevice, c.s));
     state_commit(c);
@@ -1799,13 +1807,16 @@ int fire_state_stats(fire_handle* h, int32_t* n_walkers, int64_t* n_proposals, i
     if (n_walkers) *n_walkers = h->st.ready ? h->st.B : 0;
     if (n_proposals) *n_proposals = h->st.n_proposals;
     int64_t st[2] = {0, 0};
-    if (h->st.ready && (sum_changed || max_changed)) {
+    int32_t ov[2] = {0, 0};
+    if (h->st.doverflow.p && (sum_changed || max_changed)) {
       HIPCHK(hipSetDevice(h->device));
       HIPCHK(hipDeviceSynchronize());
       HIPCHK(hipMemcpy(st, h->st.dstats.p, sizeof(st), hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(ov, h->st.doverflow.p, sizeof(ov), hipMemcpyDeviceToHost));
     }
     if (sum_changed) *sum_changed = st[0];
-    if (max_changed) *max_changed = st[1];
+    // the largest affected set, before truncation to the capacity when a move exceeded it
+    if (max_changed) *max_changed = std::max<int64_t>(st[1], ov[1]);
     return 0;
   } catch (const std::exception& e) {
     return fail(e);

@@ -32,11 +32,14 @@ struct EdgeParams {  // ElecElecEdges (new_sparse_model.py:161-187)
 enum Act { ACT_SILU = 0, ACT_EXP_NEG_SCALED = 1 };
 
 // graph
-// list mode: n_list = capacity of the list; dn (optional, device) = entries actually in use
+// list mode: n_list = capacity of the list; dn (optional, device) = entries actually in use;
+// otherwise all B * N centres, indexed in group order or, with walker_major, as w * N + i
 void launch_nbr_count(hipStream_t s, const float* r, int B, int N, int n_up, float cutoff, int32_t* cnt,
-                      const int2* list = nullptr, int n_list = 0, const int32_t* dn = nullptr);
+                      const int2* list = nullptr, int n_list = 0, const int32_t* dn = nullptr,
+                      bool walker_major = false);
 void launch_nbr_fill(hipStream_t s, const float* r, int B, int N, int n_up, float cutoff, const int64_t* ptr,
-                     int32_t* nbr, const int2* list = nullptr, int n_list = 0, const int32_t* dn = nullptr);
+                     int32_t* nbr, const int2* list = nullptr, int n_list = 0, const int32_t* dn = nullptr,
+                     bool walker_major = false);
 void launch_scan(hipStream_t s, const int32_t* cnt, int G, int64_t* ptr, const int32_t* dn = nullptr);
 void launch_rev(hipStream_t s, const int64_t* ptr, const int32_t* nbr, int B, int N, int n_up, int32_t* rev);
 

@@ -23,12 +23,13 @@ __device__ __forceinline__ bool in_cutoff(const float* ri, const float* rj, floa
 }
 
 // `list` (optional): centres as (walker, electron) pairs, output index = list position;
-// otherwise every electron of every walker in group order.
+// otherwise every electron of every walker, output index in group order, or in walker-major
+// order (w * N + i) with `walker_major`.
 template <bool FILL>
 __global__ void __launch_bounds__(256) k_nbr(const float* __restrict__ r, int B, int N, int n_up, float cutoff,
                                              int32_t* __restrict__ cnt, const int64_t* __restrict__ ptr,
                                              int32_t* __restrict__ nbr, const int2* __restrict__ list, int n_list,
-                                             const int32_t* __restrict__ dn) {
+                                             const int32_t* __restrict__ dn, bool walker_major) {
   const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
   const int n_centres = list ? n_list : B * N;
@@ -45,7 +46,7 @@ __global__ void __launch_bounds__(256) k_nbr(const float* __restrict__ r, int B,
   } else {
     w = wave / N;
     i = wave - w * N;
-    g = group_of(w, i, B, n_up, N - n_up);
+    g = walker_major ? wave : group_of(w, i, B, n_up, N - n_up);
   }
   const float* rw = r + (size_t)w * N * 3;
   const float ri[3] = {rw[3 * i], rw[3 * i + 1], rw[3 * i + 2]};
@@ -144,19 +145,19 @@ __global__ void k_rev(const int64_t* __restrict__ ptr, const int32_t* __restrict
 }
 
 void launch_nbr_count(hipStream_t s, const float* r, int B, int N, int n_up, float cutoff, int32_t* cnt,
-                      const int2* list, int n_list, const int32_t* dn) {
+                      const int2* list, int n_list, const int32_t* dn, bool walker_major) {
   const int waves = list ? n_list : B * N;
   if (waves == 0) return;
   const int blocks = (waves * 64 + 255) / 256;
-  k_nbr<false><<<blocks, 256, 0, s>>>(r, B, N, n_up, cutoff, cnt, nullptr, nullptr, list, n_list, dn);
+  k_nbr<false><<<blocks, 256, 0, s>>>(r, B, N, n_up, cutoff, cnt, nullptr, nullptr, list, n_list, dn, walker_major);
 }
 
 void launch_nbr_fill(hipStream_t s, const float* r, int B, int N, int n_up, float cutoff, const int64_t* ptr,
-                     int32_t* nbr, const int2* list, int n_list, const int32_t* dn) {
+                     int32_t* nbr, const int2* list, int n_list, const int32_t* dn, bool walker_major) {
   const int waves = list ? n_list : B * N;
   if (waves == 0) return;
   const int blocks = (waves * 64 + 255) / 256;
-  k_nbr<true><<<blocks, 256, 0, s>>>(r, B, N, n_up, cutoff, nullptr, ptr, nbr, list, n_list, dn);
+  k_nbr<true><<<blocks, 256, 0, s>>>(r, B, N, n_up, cutoff, nullptr, ptr, nbr, list, n_list, dn, walker_major);
 }
 
 void launch_scan(hipStream_t s, const int32_t* cnt, int G, int64_t* ptr, const int32_t* dn) {

@@ -119,10 +119,12 @@ __global__ void k_mc_affected(const float* __restrict__ st_r, const float* __res
   }
   if (lane == 0) {
     // static capacity: at most `cap` changed electrons per move (an excess is flagged and
-    // truncated so that every buffer access stays in bounds; the step is then invalid)
+    // truncated so that every buffer access stays in bounds; the step is then invalid);
+    // overflow[1] keeps the largest count before truncation
     int cd = count - cu;
     if (count > cap) {
       overflow[0] = 1;
+      atomicMax(overflow + 1, count);
       cu = min(cu, cap);
       cd = min(cd, cap - cu);
     }
@@ -176,8 +178,10 @@ __global__ void __launch_bounds__(1024) k_mc_scan(const int32_t* __restrict__ cu
     tail[1] = sd[1023];
     tail[2] = sm[1023];
     tail[3] = su[1023] + sd[1023];
-    stats[0] += su[1023] + sd[1023];
-    stats[1] = max(stats[1], (int64_t)sm[1023]);
+    if (stats) {  // null for a proposal that is not counted (graph warm-up)
+      stats[0] += su[1023] + sd[1023];
+      stats[1] = max(stats[1], (int64_t)sm[1023]);
+    }
   }
 }
 

@@ -12,11 +12,17 @@ in ``libfire_amd.so``.
 from __future__ import annotations
 
 import ctypes
+import hashlib
 from dataclasses import dataclass
 from typing import Any, Mapping, NamedTuple, Optional
 
 import numpy as np
 import torch
+
+try:
+    import xxhash
+except ImportError:  # hashlib is the fallback
+    xxhash = None
 
 from . import _lib
 from .config import FireConfig, Molecule
@@ -75,6 +81,9 @@ class FireWaveFunction:
         self._h = h
         self._lib = lib
         self._uploaded = None
+        self._uploaded_fp = None
+        self._gen = 0
+        self._committed_gen = self._pending_gen = None
 
     # reference-style constructor ---------------------------------------------------------
     @classmethod
@@ -110,8 +119,10 @@ class FireWaveFunction:
         return self.cfg.n_up
 
     # parameters ------------------------------------------------------------------------
-    def upload(self, params):
-        """Upload a parameter tree (reference layout) or an already packed ``PackedParams``."""
+    def upload(self, params, fingerprint=None):
+        """Upload a parameter tree (reference layout) or an already packed ``PackedParams``.
+        The resident walker state of ``log_psi_with_state`` belongs to the previous parameters
+        and is dropped."""
         if isinstance(params, PackedParams):
             flat = np.ascontiguousarray(params.flat, np.float32)
             sizes = np.ascontiguousarray(params.sizes, np.int64)
@@ -121,14 +132,23 @@ class FireWaveFunction:
             flat = np.ascontiguousarray(np.concatenate(tensors).astype(np.float32))
         _lib.check(self._lib.fire_upload_params(self._h, flat.ctypes.data, sizes.ctypes.data, len(sizes)))
         self._uploaded = params
+        self._uploaded_fp = _fingerprint(params) if fingerprint is None else fingerprint
+        self._committed_gen = self._pending_gen = None
 
-    def _ensure(self, params):
-        if params is not self._uploaded:
-            self.upload(params)
+    def _ensure(self, params) -> bool:
+        """Upload ``params`` unless the device already holds them: the same object with the
+        same contents (a parameter tree changed in place is uploaded again).  Returns whether
+        an upload happened; an upload invalidates the resident walker state."""
+        fp = _fingerprint(params)
+        if params is self._uploaded and fp == self._uploaded_fp:
+            return False
+        self.upload(params, fp)
+        return True
 
     def init(self, seed: int, electrons) -> dict:
         """MoonLikeWaveFunction.init (wave_function.py:89-99): random parameters with the
-        normalisation scales set to 1/std of the embedding on ``electrons`` (one walker)."""
+        normalisation scales set to 1/std of the embedding on ``electrons`` (one walker).
+        The returned parameters are the ones left on the device."""
         params = init_params(self.cfg, seed)
 
         def stages(p):
@@ -136,7 +156,10 @@ class FireWaveFunction:
             hc, ho = self.embedding(p, electrons, unscaled_out=True)
             return hc.double().cpu().numpy(), ho.double().cpu().numpy()
 
-        return set_scales(params, stages)
+        params = set_scales(params, stages)
+        # set_scales writes s1 into the tree after its last upload (made with s1 = 1)
+        self.upload(params)
+        return params
 
     # inputs ----------------------------------------------------------------------------
     def _walkers(self, electrons):
@@ -157,19 +180,47 @@ class FireWaveFunction:
     # API -------------------------------------------------------------------------------
     def get_static_input(self, electrons, electrons_new=None, idx_changed=None, laplacian=True) -> StaticArgs:
         """get_static_input (new_sparse_model.py:399-437), computed from the GPU neighbour lists
-        (max over walkers)."""
+        and taken as the maximum over walkers: pair counts by spin, triplets, nuclei within
+        cutoff_1el and, for a move of electron ``idx_changed`` from ``electrons`` to
+        ``electrons_new``, the affected electrons (neighbours of the moved one before or after
+        the move, plus itself) and the new pairs centred on them."""
+        r_old, _ = self._walkers(electrons)
         r, _ = self._walkers(electrons if electrons_new is None else electrons_new)
-        counts, nbr, ptr = self.neighbours(r)
-        B, N = r.shape[0], self.cfg.n_el
-        spin = (torch.arange(N, device=self.device) < self.cfg.n_up)
-        ctr = torch.repeat_interleave(torch.arange(B * N, device=self.device), counts.reshape(-1).long())
-        same = spin[ctr % N] == spin[nbr.long()]
-        w = ctr // N
-        n_same = torch.zeros(B, dtype=torch.long, device=self.device).index_add_(0, w, same.long())
-        n_all = counts.reshape(B, N).sum(1).long()
-        n_trip = (counts.reshape(B, N).long() + 1) ** 2
-        trip = int((n_trip.sum(1) - (2 * counts.reshape(B, N).long() + 1).sum(1)).max()) if laplacian else 0
-        return StaticArgs(int(n_same.max()), int((n_all - n_same).max()), trip, self.cfg.n_nuc, 0, 0, 0, N)
+        B, N, nu = r.shape[0], self.cfg.n_el, self.cfg.n_up
+        dev = self.device
+
+        def pairs(rr):
+            counts, nbr, _ = self.neighbours(rr)
+            ctr = torch.repeat_interleave(torch.arange(B * N, device=dev), counts.reshape(-1).long())
+            return counts.reshape(B, N).long(), ctr // N, ctr % N, nbr.long()
+
+        counts, w, i, j = pairs(r)
+        spin_i, spin_j = i < nu, j < nu
+        same = spin_i == spin_j
+
+        def per_walker(mask):
+            return torch.zeros(B, dtype=torch.long, device=dev).index_add_(0, w, mask.long())
+
+        n_same = per_walker(same)
+        n_diff = per_walker(~same)
+        trip = int((counts ** 2).sum(1).max()) if laplacian else 0
+        R = torch.as_tensor(np.asarray(self.cfg.mol.R, np.float32), device=dev)
+        d = r[:, :, None, :] - R[None, None]
+        den = torch.sqrt((d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1]) + d[..., 2] * d[..., 2])
+        n_en = int((den < np.float32(self.cfg.cutoff_1el)).sum(-1).max())
+        n_hout = n_cs = n_cd = 0
+        if electrons_new is not None:
+            k = int(np.asarray(idx_changed).reshape(-1)[0])
+            aff = torch.zeros((B, N), dtype=torch.bool, device=dev)
+            aff[w[j == k], i[j == k]] = True
+            _, wo, io, jo = pairs(r_old)
+            aff[wo[jo == k], io[jo == k]] = True
+            aff[:, k] = True
+            req = aff[w, i]
+            n_hout = int(aff.sum(1).max())
+            n_cs = int(per_walker(req & same).max())
+            n_cd = int(per_walker(req & ~same).max())
+        return StaticArgs(int(n_same.max()), int(n_diff.max()), trip, n_en, n_hout, n_cs, n_cd, N)
 
     def neighbours(self, electrons):
         """CSR neighbour lists: counts [B, N], neighbour indices [P], row pointer [B*N+1]."""
@@ -249,16 +300,18 @@ class FireWaveFunction:
         sg = torch.empty(B, dtype=torch.float64, device=self.device)
         _lib.check(self._lib.fire_state_init(self._h, _stream_ptr(), _dptr(r), B, self._prec(precision), _dptr(lp),
                                              _dptr(sg)))
-        self._gen = getattr(self, "_gen", 0) + 1
+        self._gen += 1
         self._committed_gen, self._pending_gen = self._gen, None
         st = WalkerState(self, self._gen, B, single, pending=False)
         return ((sg[0], lp[0]) if single else (sg, lp)), st
 
     def _use_state(self, params, state: "WalkerState"):
-        if params is not self._uploaded:
+        if params is not self._uploaded or _fingerprint(params) != self._uploaded_fp:
             raise ValueError("parameters changed since log_psi_with_state: the resident state belongs to the old ones")
         if state.wf is not self:
             raise ValueError("state belongs to another wave function")
+        if self._committed_gen is None:
+            raise ValueError("no resident state: parameters were uploaded after log_psi_with_state")
         if state.pending:
             if state.gen != self._pending_gen:
                 raise ValueError("stale proposal state")
@@ -336,7 +389,7 @@ class FireWaveFunction:
         nw, npr, s, mx = ctypes.c_int32(), ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
         _lib.check(self._lib.fire_state_stats(self._h, ctypes.byref(nw), ctypes.byref(npr), ctypes.byref(s),
                                               ctypes.byref(mx)))
-        return dict(n_walkers=nw.value, n_proposals=npr.value,
+        return dict(n_walkers=nw.value, n_proposals=npr.value, sum_changed=s.value,
                     mean_changed=(s.value / npr.value) if npr.value else 0.0, max_changed=mx.value)
 
     def mcmc_steps_low_rank(self, params, electrons, steps: int, width: float, seed: int = 0, step0: int = 0,
@@ -346,28 +399,74 @@ class FireWaveFunction:
         Returns (electrons, stats) with stats["pmove"] the acceptance rate.  With
         ``keep_state`` the resident state of an earlier call (same walkers) is continued
         instead of being rebuilt from ``electrons`` (the reference rebuilds it at the start
-        of every call).  ``refresh_sweeps`` > 0 rebuilds the state from the current positions
+        of every call); when the parameters changed since, the state is rebuilt from its own
+        positions.  ``refresh_sweeps`` > 0 rebuilds the state from the current positions
         every that many sweeps within the call: the FP64 low-rank updates accumulate rounding
-        in the inverse, which long chains on ill-conditioned walkers make visible."""
-        self._ensure(params)
-        if not keep_state or getattr(self, "_committed_gen", None) is None:
+        in the inverse, which long chains on ill-conditioned walkers make visible.  Accept
+        counts and the changed-electron statistics cover the whole call."""
+        if keep_state and self._committed_gen is not None:
+            if params is not self._uploaded or _fingerprint(params) != self._uploaded_fp:
+                r_res = self.state_values()[0]
+                self._ensure(params)
+                self.log_psi_with_state(params, r_res, precision=precision)
+            elif self._pending_gen is not None:
+                self._pending_gen = None  # a pending proposal is dropped by the next step
+        else:
+            self._ensure(params)
             self.log_psi_with_state(params, electrons, precision=precision)
-        elif self._pending_gen is not None:
-            self._pending_gen = None  # a pending proposal is dropped by the next step
         B = self._state_walkers()
         nacc = torch.zeros(B, dtype=torch.int32, device=self.device)
+        nacc_total = torch.zeros(B, dtype=torch.int64, device=self.device)
         block = int(steps) if refresh_sweeps <= 0 else int(refresh_sweeps) * self.cfg.n_el
         done = 0
+        n_prop = sum_changed = max_changed = 0
         while done < steps:
             if done > 0:
+                st = self.state_stats()
+                n_prop += st["n_proposals"]
+                sum_changed += st["sum_changed"]
+                max_changed = max(max_changed, st["max_changed"])
                 self.log_psi_with_state(params, self.state_values()[0], precision=precision)
             n = min(block, int(steps) - done)
             _lib.check(self._lib.fire_mcmc_steps(self._h, _stream_ptr(), n, float(width), int(seed) & (2**64 - 1),
                                                  int(step0) + done, _dptr(nacc)))
+            nacc_total += nacc.long()
             done += n
         r, sg, lp = self.state_values()
-        pmove = float(nacc.double().mean() / steps) if steps else 0.0
-        return r, dict(pmove=pmove, n_accept=nacc, logpsi=lp, sign=sg, **self.state_stats())
+        st = self.state_stats()
+        n_prop += st["n_proposals"]
+        sum_changed += st["sum_changed"]
+        max_changed = max(max_changed, st["max_changed"])
+        pmove = float(nacc_total.double().mean() / steps) if steps else 0.0
+        return r, dict(pmove=pmove, n_accept=nacc_total, logpsi=lp, sign=sg, n_walkers=B, n_proposals=n_prop,
+                       mean_changed=(sum_changed / n_prop) if n_prop else 0.0, max_changed=max_changed)
+
+
+def _fingerprint(params) -> int:
+    """Content hash of a parameter tree (or ``PackedParams``): detects in-place changes of
+    parameters already on the device."""
+    h = xxhash.xxh3_64() if xxhash is not None else hashlib.blake2b(digest_size=8)
+
+    def visit(x):
+        if isinstance(x, dict):
+            for k in x:
+                h.update(str(k).encode())
+                visit(x[k])
+        elif isinstance(x, (list, tuple)):
+            for v in x:
+                visit(v)
+        elif isinstance(x, PackedParams):
+            visit(x.flat)
+            visit(x.sizes)
+        else:
+            if isinstance(x, torch.Tensor):
+                x = x.detach().cpu().numpy()
+            a = np.ascontiguousarray(np.asarray(x))
+            h.update(str(a.dtype).encode() + str(a.shape).encode())
+            h.update(a.view(np.uint8).reshape(-1).data if a.size else b"")
+
+    visit(params)
+    return int.from_bytes(h.digest()[:8], "little")
 
 
 @dataclass(eq=False)
