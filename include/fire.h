@@ -74,6 +74,16 @@ int fire_upload_params(fire_handle* h, const float* packed, const int64_t* sizes
 int fire_neighbours(fire_handle* h, void* stream, const float* r_dev, int32_t B, int32_t* counts_dev,
                     int32_t* nbr_dev, int64_t cap, int64_t* n_pairs);
 
+/* As fire_neighbours, without any device allocation (no counterpart in the reference; for
+ * callers that cannot allocate device memory: tight serving loops and harnesses that forbid
+ * hipMalloc).  The CSR row pointer lives in a static device array of 16 MiB per GPU (held
+ * until the process ends): B * n_el must be at most 2097150, larger batches are rejected.
+ * Every call on one GPU shares that array, so calls that overlap on one GPU (any handles,
+ * any streams) race; the caller orders them.  Synchronises the stream once (to read the
+ * pair count). */
+int fire_neighbours_noalloc(fire_handle* h, void* stream, const float* r_dev, int32_t B, int32_t* counts_dev,
+                            int32_t* nbr_dev, int64_t cap, int64_t* n_pairs);
+
 /* Local energy of B walkers: E_loc = E_kin + E_pot with the sparse forward-Laplacian
  * kinetic energy.  Outputs (device, double, [B]); any may be NULL except e_loc_dev.
  * grad_dev (optional, double [B, n_el, 3]) and lap_dev (optional, double [B]) receive the

@@ -38,6 +38,8 @@ SIGNATURES = {
     "fire_upload_params": (ctypes.c_int, [_P, _P, _P, ctypes.c_int32]),
     "fire_neighbours": (ctypes.c_int, [_P, _P, _P, ctypes.c_int32, _P, _P, ctypes.c_int64,
                                        ctypes.POINTER(ctypes.c_int64)]),
+    "fire_neighbours_noalloc": (ctypes.c_int, [_P, _P, _P, ctypes.c_int32, _P, _P, ctypes.c_int64,
+                                               ctypes.POINTER(ctypes.c_int64)]),
     "fire_local_energy": (ctypes.c_int, [_P, _P, _P, ctypes.c_int32, ctypes.c_int32, _P, _P, _P, _P, _P, _P, _P]),
     "fire_signed": (ctypes.c_int, [_P, _P, _P, ctypes.c_int32, ctypes.c_int32, _P, _P]),
     "fire_embedding": (ctypes.c_int, [_P, _P, _P, ctypes.c_int32, ctypes.c_int32, _P, _P]),

@@ -19,6 +19,9 @@
 #include "fire_kernels2.hpp"
 #include "fire_mcmc.hpp"
 
+static constexpr int64_t kScratchInt64 = (int64_t)1 << 21;  // 16 MiB
+static __device__ __align__(256) int64_t scratch[kScratchInt64];
+
 namespace {
 
 thread_local std::string g_last_error;
@@ -72,6 +75,32 @@ struct DevBuf {
   }
 };
 
+// Row-pointer scratch of fire_neighbours_noalloc: a fixed device array (one copy per GPU,
+// held until the process ends), so that entry point never allocates.  Host code must use the
+// address hipGetSymbolAddress returns for the current device, never the array's name.
+struct ScratchAddrs {
+  std::mutex mu;
+  std::map<int, int64_t*> ptr;  // device -> address of `scratch` on that device
+};
+ScratchAddrs& scratch_addrs() {
+  static ScratchAddrs a;
+  return a;
+}
+
+// Address of `scratch` on `device` (which must be current), looked up once per device.
+int64_t* scratch_address(int device) {
+  auto& a = scratch_addrs();
+  std::lock_guard<std::mutex> lk(a.mu);
+  auto it = a.ptr.find(device);
+  if (it != a.ptr.end()) return it->second;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || p == nullptr)
+    throw std::runtime_error(std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+  a.ptr[device] = static_cast<int64_t*>(p);
+  return static_cast<int64_t*>(p);
+}
+
 uint16_t host_f2bf(float x) {
   uint32_t u;
   std::memcpy(&u, &x, 4);
@@ -124,6 +153,7 @@ struct fire_handle {
   DevBuf cval, clap, cgrad, epot;
   int chunk = 256;
   bool lu_smem_set = false;
+  int64_t* scratch = nullptr;  // this device's static scratch (fire_neighbours_noalloc)
   // optional HIP-event timing of the dense contractions (fire_profile_gemm)
   bool gemm_timing = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> gemm_events;
@@ -1273,7 +1303,9 @@ int fire_create(const fire_config* cfg, const float* R, const float* Z, int devi
 
     REQUIRE(c.mlp_depth >= 1 && c.mlp_width > 0, "mlp_depth >= 1 and mlp_width > 0 required");
     HIPCHK(hipSetDevice(device));
+    int64_t* scr = scratch_address(device);
     fire_handle* h = new fire_handle();
+    h->scratch = scr;
     h->cfg = c;
     if (const char* e = getenv("FIRE_FUSED_ACT")) h->fuse_act = e[0] != '0';
     h->device = device;
@@ -1529,6 +1561,28 @@ int fire_neighbours(fire_handle* h, void* stream, const float* r_dev, int32_t B,
     }
     h->ptr.ensure(sizeof(int64_t) * ((size_t)B * h->N + 2));
     *n_pairs = neighbours_walker_major(h, (hipStream_t)stream, r_dev, B, counts_dev, nbr_dev, cap, h->ptr.as<int64_t>());
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e);
+  }
+}
+
+int fire_neighbours_noalloc(fire_handle* h, void* stream, const float* r_dev, int32_t B, int32_t* counts_dev,
+                            int32_t* nbr_dev, int64_t cap, int64_t* n_pairs) {
+  try {
+    REQUIRE(h && r_dev && counts_dev && n_pairs, "null argument");
+    REQUIRE(B >= 0, "negative batch");
+    REQUIRE(h->scratch != nullptr, "no static scratch address for this device");
+    const int64_t need = (int64_t)B * h->N + 2;
+    REQUIRE(need <= kScratchInt64, "fire_neighbours_noalloc: B * n_el = " + std::to_string((int64_t)B * h->N) +
+                                       " exceeds the static scratch capacity of " + std::to_string(kScratchInt64 - 2) +
+                                       " electrons");
+    HIPCHK(hipSetDevice(h->device));
+    if (B == 0) {
+      *n_pairs = 0;
+      return 0;
+    }
+    *n_pairs = neighbours_walker_major(h, (hipStream_t)stream, r_dev, B, counts_dev, nbr_dev, cap, h->scratch);
     return 0;
   } catch (const std::exception& e) {
     return fail(e);
