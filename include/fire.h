@@ -41,10 +41,12 @@ typedef struct fire_config {
 
 enum fire_precision {
   FIRE_FP32 = 0,  /* every embedding / orbital / Jastrow contraction in FP32 (FP32 MFMA) */
-  FIRE_MIXED = 1, /* dense contractions on BF16 MFMA with FP32 accumulation: embedding and
-                     orbital contractions carry each operand as three BF16 planes (hi + mid +
-                     lo) and accumulate the six products down to 2^-16 relative (FP32-level
-                     products); Jastrow contractions use two planes (see fire_precision_plan);
+  FIRE_MIXED = 1, /* dense contractions on the 16-bit matrix cores with FP32 accumulation
+                     (see fire_precision_plan): on the tangent rows of the local energy the
+                     update-MLP, orbital, Jastrow and attention contractions carry each operand
+                     as two FP16 planes of a power-of-two-scaled value (per row of A, per column
+                     of W; three products, about 2^-22 relative each); the ElecInit MLP and the
+                     value-path contractions carry three BF16 planes (six products, FP32-level);
                      element-wise forward-Laplacian rules, envelopes and Jastrow sums FP32;
                      Slater determinants, their inverses, updates and energy sums FP64 */
   FIRE_BF16 = 2,  /* dense contractions in single BF16 on MFMA (fastest, least accurate) */
@@ -174,11 +176,15 @@ int fire_state_get(fire_handle* h, void* stream, float* r_dev, double* logpsi_de
 int fire_state_stats(fire_handle* h, int32_t* n_walkers, int64_t* n_proposals, int64_t* sum_changed,
                      int64_t* max_changed);
 
-/* Precision plan of FIRE_MIXED (no counterpart in the reference): BF16 planes per operand for
- * each contraction class, in the order ElecInit MLP, update MLP, orbitals, MLP Jastrow,
- * attention Q/V, attention readout.  3 accumulates the six products down to 2^-16 relative
- * (FP32-level), 2 the three products of a hi + lo split, 1 plain BF16.  Default {3,3,3,2,2,2}.  Sets the first
- * n classes from planes (n may be 0); current (optional, int32 [6]) receives the plan. */
+/* Precision plan of FIRE_MIXED (no counterpart in the reference): operand planes for each
+ * contraction class, in the order ElecInit MLP, update MLP, orbitals, MLP Jastrow, attention
+ * Q/V, attention readout.  3: three BF16 planes, six products accumulated down to 2^-16
+ * relative (FP32-level); 2: a BF16 hi + lo pair, three products; 1: plain BF16; 4: two FP16
+ * planes of operands scaled by powers of two (each row of A by its maximum, which the kernel
+ * producing A leaves, each column of W at upload), three products, about 2^-22 relative per
+ * product; where no row maxima exist (value path, sampler) class 4 runs as 3.  Default
+ * {3,4,4,4,4,2}.  Sets the first n classes from planes (n may be 0); current (optional,
+ * int32 [6]) receives the plan. */
 int fire_precision_plan(fire_handle* h, const int32_t* planes, int32_t n, int32_t* current);
 
 /* Tile-shape autotuner of the dense contractions (no counterpart in the reference; the

@@ -108,6 +108,59 @@ uint16_t host_f2bf(float x) {
   return (uint16_t)(u >> 16);
 }
 
+// IEEE binary16 of x, rounded to nearest even (normal and subnormal results; |x| below the
+// overflow threshold, which the power-of-two column scales guarantee)
+uint16_t host_f2h(float x) {
+  uint32_t u;
+  std::memcpy(&u, &x, 4);
+  const uint16_t sign = (uint16_t)((u >> 16) & 0x8000u);
+  u &= 0x7fffffffu;
+  if (u >= 0x47800000u) return sign | 0x7c00u;  // |x| >= 2^16: infinity
+  if (u < 0x38800000u) {                         // |x| < 2^-14: a subnormal half, units of 2^-24
+    float ax;
+    std::memcpy(&ax, &u, 4);
+    const float q = ax * 16777216.0f;  // exact: scaling by a power of two
+    uint32_t m = (uint32_t)q;
+    const float rem = q - (float)m;
+    if (rem > 0.5f || (rem == 0.5f && (m & 1u))) ++m;
+    return sign | (uint16_t)m;  // m = 0x400 is the smallest normal, correctly encoded
+  }
+  uint32_t hb = (u >> 13) - ((127u - 15u) << 10);  // rebias the exponent, keep 10 mantissa bits
+  const uint32_t rest = u & 0x1fffu;
+  if (rest > 0x1000u || (rest == 0x1000u && (hb & 1u))) ++hb;  // a carry bumps the exponent
+  return sign | (uint16_t)hb;
+}
+
+float host_h2f(uint16_t hv) {
+  const uint32_t sign = (uint32_t)(hv & 0x8000u) << 16;
+  const uint32_t e = (hv >> 10) & 0x1fu, m = hv & 0x3ffu;
+  float f;
+  if (e == 0) {
+    f = (float)m * (1.0f / 16777216.0f);  // subnormal: m 2^-24
+  } else {
+    const uint32_t u = ((e + 112u) << 23) | (m << 13);
+    std::memcpy(&f, &u, 4);
+  }
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  u |= sign;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+
+// The power-of-two scale 2^(141 - e) of a column whose largest |entry| has biased exponent e
+// (clamped to [15, 254]): the maximum lands in [2^14, 2^15).  Same rule as the device's rows.
+float host_f16_scale(float colmax) {
+  uint32_t u;
+  std::memcpy(&u, &colmax, 4);
+  int e = (int)((u >> 23) & 0xffu);
+  e = e < 15 ? 15 : (e > 254 ? 254 : e);
+  const uint32_t sb = (uint32_t)(268 - e) << 23;
+  float sc;
+  std::memcpy(&sc, &sb, 4);
+  return sc;
+}
+
 }  // namespace
 
 struct fire_handle {
@@ -126,6 +179,10 @@ struct fire_handle {
   std::vector<uint16_t*> Pbf_lo;  // ... the BF16 remainder W - hi (mid plane)
   std::vector<uint16_t*> Pbf_lo2;  // ... and the BF16 remainder W - hi - mid (lo plane)
   std::vector<float*> Pt32;       // FP32 transposed copies [N,K] of GEMM weights
+  // FP16 planes [N,K] of every GEMM weight with each column scaled by a power of two (its
+  // maximum into [2^14, 2^15)), and the inverse scales [N]
+  std::vector<uint16_t*> Pf16, Pf16_lo;
+  std::vector<float*> Pwinv;
   std::vector<std::pair<int, int>> shape2;  // (K, N) of GEMM weights
   float cut_learned = 0.f;
   float s1 = 1.f;
@@ -146,6 +203,8 @@ struct fire_handle {
   // workspace
   DevBuf cnt, ptr, nbr, rev, rowflag, rowgrp;
   DevBuf AV0, Z0, F1, PART;  // fused silu epilogue: packed value rows, their pre-activations, silu', slots
+  DevBuf AMX_A, AMX_B, AMX_C;  // row maxima (float bits) of XA, XB, XC for the FP16 planes
+  const uint32_t* amx_xa = nullptr;  // row maxima of the embedding rows in XA (null: none)
   DevBuf T5, Z2, T5b, M5, XA, XB, XC, O, ENV, ENVC, PF, EF, EOK;
   DevBuf A64, Ainv, AinvT, ldet, dsgn, tr, trip, dgrad;
   DevBuf det_lp, det_sg, det_gr, det_lap;
@@ -171,7 +230,10 @@ struct fire_handle {
   // classes, 2 for the Jastrow classes (on equilibrated C16H34 walkers the Jastrow classes at
   // 2 planes leave the FP64 energy error unchanged; any embedding/orbital class at 2 planes
   // raises its rms from 0.4 to 3-4 mHa)
-  int planes[6] = {3, 3, 3, 2, 2, 2};
+  // 4 = two FP16 planes of operands scaled by powers of two per row of A and column of W
+  // (three products, about 2^-22 relative per product); used where the producer of A left
+  // its row maxima (tangent rows of the local energy), three BF16 planes elsewhere.
+  int planes[6] = {3, 4, 4, 4, 4, 2};
   std::map<std::tuple<int, int, int, int>, int> gemm_tile;
   std::map<std::tuple<int, int, int, int>, std::vector<float>> gemm_tile_ms;
   // Resident walker state for single-electron low-rank updates (fire_state_*, fire_mcmc_steps).
@@ -350,7 +412,7 @@ int gemm_class(const fire_handle* h, int wi) {
 // C = A @ W(tensor wi) + bias(tensor bi, optional) on value rows
 void gemm(Ctx& c, const float* A, int64_t lda, int wi, int bi, float* C, int64_t ldc, int64_t M, int N, int K,
           const uint8_t* rowflag, int period, DevRows dr = DevRows(), const ActEpi* epp = nullptr) {
-  const ActEpi ep = epp ? *epp : ActEpi();
+  ActEpi ep = epp ? *epp : ActEpi();
   fire_handle* h = c.h;
   const float* bias = bi >= 0 ? h->P[bi] : nullptr;
   const bool bf_ok = h->Pbf[wi] != nullptr && (K % 32) == 0 && (lda % 4) == 0 && N > 16;
@@ -358,8 +420,20 @@ void gemm(Ctx& c, const float* A, int64_t lda, int wi, int bi, float* C, int64_t
   int ns = c.prec == FIRE_MIXED_FAST ? 3 : 1;
   if (c.prec == FIRE_MIXED) {
     int np = h->planes[gemm_class(h, wi)];
-    if (wi == h->iJV) np = std::max(np, h->planes[4]);
-    ns = np == 3 ? 6 : (np == 2 ? 3 : 1);
+    if (wi == h->iJV) np = (np == 4 || h->planes[4] == 4) ? 4 : std::max(np, h->planes[4]);
+    if (np == 4 && !(ep.amax && h->Pf16[wi])) np = 3;  // no row maxima of A: three BF16 planes
+    ns = np == 4 ? 4 : (np == 3 ? 6 : (np == 2 ? 3 : 1));
+  }
+  const uint16_t* W0 = h->Pbf[wi];
+  const uint16_t* W1 = h->Pbf_lo[wi];
+  const uint16_t* W2 = h->Pbf_lo2[wi];
+  if (ns == 4) {
+    W0 = h->Pf16[wi];
+    W1 = h->Pf16_lo[wi];
+    W2 = nullptr;
+    ep.winv = h->Pwinv[wi];
+  } else {
+    ep.amax = nullptr;
   }
   int variant = 0;
   if (use_bf && h->autotune) {
@@ -390,17 +464,15 @@ void gemm(Ctx& c, const float* A, int64_t lda, int wi, int bi, float* C, int64_t
       HIPCHK(hipEventCreate(&t0));
       HIPCHK(hipEventCreate(&t1));
       for (int v = 0; v < nv; ++v) {
-        if (ep.rowgrp && !gemm_bf16_fusable(v)) {
+        if ((ep.rowgrp && !gemm_bf16_fusable(v)) || (ns == 4 && !gemm_bf16_f16_ok(v))) {
           ms[v] = 1e30f;
           continue;
         }
-        launch_gemm_bf16(c.s, A, lda, h->Pbf[wi], h->Pbf_lo[wi], h->Pbf_lo2[wi], bias, C, ldc, M, N, K, rowflag,
-                         period, ns, v, dr, ep);
+        launch_gemm_bf16(c.s, A, lda, W0, W1, W2, bias, C, ldc, M, N, K, rowflag, period, ns, v, dr, ep);
         ms[v] = 1e30f;
         for (int rep = 0; rep < 2; ++rep) {
           HIPCHK(hipEventRecord(t0, c.s));
-          launch_gemm_bf16(c.s, A, lda, h->Pbf[wi], h->Pbf_lo[wi], h->Pbf_lo2[wi], bias, C, ldc, M, N, K, rowflag,
-                           period, ns, v, dr, ep);
+          launch_gemm_bf16(c.s, A, lda, W0, W1, W2, bias, C, ldc, M, N, K, rowflag, period, ns, v, dr, ep);
           HIPCHK(hipEventRecord(t1, c.s));
           HIPCHK(hipEventSynchronize(t1));
           float t = 0.f;
@@ -425,8 +497,7 @@ void gemm(Ctx& c, const float* A, int64_t lda, int wi, int bi, float* C, int64_t
     HIPCHK(hipEventRecord(e0, c.s));
   }
   if (use_bf) {
-    launch_gemm_bf16(c.s, A, lda, h->Pbf[wi], h->Pbf_lo[wi], h->Pbf_lo2[wi], bias, C, ldc, M, N, K, rowflag, period,
-                     ns, variant, dr, ep);
+    launch_gemm_bf16(c.s, A, lda, W0, W1, W2, bias, C, ldc, M, N, K, rowflag, period, ns, variant, dr, ep);
   } else
     launch_gemm_f32(c.s, A, lda, h->P[wi], h->Pt32[wi], bias, C, ldc, M, N, K, rowflag, period, dr, ep);
   if (timed) {
@@ -447,8 +518,20 @@ bool gemm_silu_fusable(const fire_handle* h, int N, int K, int64_t lda) {
   return h->fuse_act && N % 128 == 0 && K % 32 == 0 && lda % 4 == 0;
 }
 
-void gemm_silu_tan(Ctx& c, const float* A, int64_t lda, int wi, int bi, float* C, int64_t ldc, int64_t R, int N, int K,
-                   const uint8_t* flag, int G, const GroupDesc& gd, int wf = -1, int bf = -1, int Nf = 0) {
+// Whether the contractions of this call use FP16 planes somewhere (row maxima worth producing).
+bool f16_wanted(const Ctx& c) {
+  if (c.prec != FIRE_MIXED) return false;
+  for (int k = 1; k < 5; ++k)
+    if (c.h->planes[k] == 4) return true;
+  return false;
+}
+
+// amax (optional): row maxima of A for FP16 planes; omax (optional): buffer (R entries) that
+// receives the row maxima of C over its first Nf columns.  Returns omax when it was filled,
+// null otherwise (the unfused fallback leaves none).
+const uint32_t* gemm_silu_tan(Ctx& c, const float* A, int64_t lda, int wi, int bi, float* C, int64_t ldc, int64_t R,
+                              int N, int K, const uint8_t* flag, int G, const GroupDesc& gd, int wf = -1, int bf = -1,
+                              int Nf = 0, const uint32_t* amax = nullptr, DevBuf* omax = nullptr) {
   fire_handle* h = c.h;
   if (wf < 0) {
     wf = wi;
@@ -457,9 +540,17 @@ void gemm_silu_tan(Ctx& c, const float* A, int64_t lda, int wi, int bi, float* C
   }
   if (!gemm_silu_fusable(h, Nf, K, lda) || G == 0) {
     REQUIRE(Nf == N, "partial activation needs the fused epilogue");
-    gemm(c, A, lda, wi, bi, C, ldc, R, N, K, flag, 0);
+    ActEpi ea;
+    ea.amax = amax;
+    gemm(c, A, lda, wi, bi, C, ldc, R, N, K, flag, 0, DevRows(), &ea);
     launch_act(c.s, C, ldc, N, G, gd, ACT_SILU, 0.f);
-    return;
+    return nullptr;
+  }
+  uint32_t* om = nullptr;
+  if (omax && f16_wanted(c)) {
+    omax->ensure(sizeof(uint32_t) * (size_t)R);
+    om = omax->as<uint32_t>();
+    launch_zero_f32(c.s, reinterpret_cast<float*>(om), R);
   }
   h->AV0.ensure(sizeof(float) * (size_t)G * K);
   h->Z0.ensure(sizeof(float) * (size_t)G * Nf);
@@ -477,8 +568,12 @@ void gemm_silu_tan(Ctx& c, const float* A, int64_t lda, int wi, int bi, float* C
   ep.gd = gd;
   ep.ldf = Nf;
   ep.nfuse = Nf;
+  ep.amax = amax;
+  ep.omax = om;
+  ep.omax_cols = Nf;
   gemm(c, A, lda, wi, bi, C, ldc, R, N, K, flag, 0, DevRows(), &ep);
-  launch_act_lapfix(c.s, C, ldc, Nf, G, gd, z0, h->PART.as<float>(), Nf);
+  launch_act_lapfix(c.s, C, ldc, Nf, G, gd, z0, h->PART.as<float>(), Nf, om);
+  return om;
 }
 
 EnParams en_params(fire_handle* h) {
@@ -587,9 +682,15 @@ void run_embedding(Ctx& c, const float* r, int B, const Graph& gr, bool tan, Gro
                  (size_t)std::max<int64_t>(gr.n_pairs, 1));
     pf = h->PF.as<float>();
   }
+  uint32_t* amx_msg = nullptr;  // row maxima (bounds) of the message rows, set per group by k_messages
+  if (tan && f16_wanted(c)) {
+    h->AMX_A.ensure(sizeof(uint32_t) * (size_t)R);
+    amx_msg = h->AMX_A.as<uint32_t>();
+  }
   launch_messages(c.s, r, B, N, h->n_up, h->ptr.as<int64_t>(), h->nbr.as<int32_t>(), h->M5.as<float>(),
                   edge_params(h, h->iEEs), edge_params(h, h->iEEd), h->cut_learned, F, tan, h->XA.as<float>(), nullptr,
-                  0, -1, nullptr, nullptr, pf);
+                  0, -1, nullptr, nullptr, pf, amx_msg);
+  h->amx_xa = nullptr;
   const uint8_t* flag = nullptr;
   int period = 1;
   if (tan) {
@@ -602,8 +703,10 @@ void run_embedding(Ctx& c, const float* r, int B, const Graph& gr, bool tan, Gro
   }
   const int m = h->iMLP;
   if (tan) {
-    gemm_silu_tan(c, h->XA.as<float>(), F, m + 0, m + 1, h->XB.as<float>(), F, R, F, F, flag, G, gd);
-    gemm_silu_tan(c, h->XB.as<float>(), F, m + 2, m + 3, h->XA.as<float>(), F, R, F, F, flag, G, gd);
+    const uint32_t* amx_b = gemm_silu_tan(c, h->XA.as<float>(), F, m + 0, m + 1, h->XB.as<float>(), F, R, F, F, flag,
+                                          G, gd, -1, -1, 0, amx_msg, &h->AMX_B);
+    h->amx_xa = gemm_silu_tan(c, h->XB.as<float>(), F, m + 2, m + 3, h->XA.as<float>(), F, R, F, F, flag, G, gd, -1,
+                              -1, 0, amx_b, &h->AMX_A);
     return;
   }
   gemm(c, h->XA.as<float>(), F, m + 0, m + 1, h->XB.as<float>(), F, R, F, F, flag, period);
@@ -656,10 +759,13 @@ void run_orbitals_det(Ctx& c, const float* r, int B, const Graph& gr, bool tan, 
   const int period = tan ? 0 : 1;
   float* Ob = h->O.as<float>();
   const int wup = aug ? h->iORBA : o, wdn = aug ? h->iORBA + 2 : o + 2;
-  gemm(c, h->XA.as<float>(), F, wup, wup + 1, Ob, ldo, R_up_rows, NDA, F, flag, period);
+  ActEpi eu, ed;  // row maxima of the embedding rows for FP16 planes (tangent rows)
+  eu.amax = tan ? h->amx_xa : nullptr;
+  ed.amax = eu.amax ? eu.amax + R_up_rows : nullptr;
+  gemm(c, h->XA.as<float>(), F, wup, wup + 1, Ob, ldo, R_up_rows, NDA, F, flag, period, DevRows(), &eu);
   if (R > R_up_rows)
     gemm(c, h->XA.as<float>() + (size_t)R_up_rows * F, F, wdn, wdn + 1, Ob + (size_t)R_up_rows * ldo, ldo,
-         R - R_up_rows, NDA, F, flag ? flag + R_up_rows : nullptr, period);
+         R - R_up_rows, NDA, F, flag ? flag + R_up_rows : nullptr, period, DevRows(), &ed);
   if (keep)
     HIPCHK(hipMemcpy2DAsync(keep->orb, sizeof(float) * ND, Ob, sizeof(float) * ldo, sizeof(float) * ND, (size_t)R,
                             hipMemcpyDeviceToDevice, c.s));
@@ -765,9 +871,10 @@ void run_jastrow_and_assemble(Ctx& c, const float* r, int B, bool tan, const Gro
     float* nxt = h->XC.as<float>();
     if (joint) {
       const int NJ = W + h->HD;
-      gemm_silu_tan(c, h->XA.as<float>(), F, h->iJV, h->iJV + 1, h->XC.as<float>(), NJ, R, NJ, F, flag, G, gd, j + 0,
-                    j + 1, W);
-      gemm(c, h->XC.as<float>(), NJ, j + 2, j + 3, h->XB.as<float>(), W, R, W, W, flag, period);
+      ActEpi e2;
+      e2.amax = gemm_silu_tan(c, h->XA.as<float>(), F, h->iJV, h->iJV + 1, h->XC.as<float>(), NJ, R, NJ, F, flag, G,
+                              gd, j + 0, j + 1, W, h->amx_xa, &h->AMX_C);
+      gemm(c, h->XC.as<float>(), NJ, j + 2, j + 3, h->XB.as<float>(), W, R, W, W, flag, period, DevRows(), &e2);
     } else {
       // the silu of the last hidden layer is applied inside k_jmlp_last, the others in the
       // epilogue of their contraction (tangent rows) or by k_act
@@ -775,9 +882,12 @@ void run_jastrow_and_assemble(Ctx& c, const float* r, int B, bool tan, const Gro
         const float* in = l == 0 ? h->XA.as<float>() : nxt;
         const int fan = l == 0 ? F : W;
         if (l + 1 < h->depth && tan) {
-          gemm_silu_tan(c, in, fan, j + 2 * l, j + 2 * l + 1, cur, W, R, W, fan, flag, G, gd);
+          gemm_silu_tan(c, in, fan, j + 2 * l, j + 2 * l + 1, cur, W, R, W, fan, flag, G, gd, -1, -1, 0,
+                        l == 0 ? h->amx_xa : nullptr);
         } else {
-          gemm(c, in, fan, j + 2 * l, j + 2 * l + 1, cur, W, R, W, fan, flag, period);
+          ActEpi e1;
+          e1.amax = (l == 0 && tan) ? h->amx_xa : nullptr;
+          gemm(c, in, fan, j + 2 * l, j + 2 * l + 1, cur, W, R, W, fan, flag, period, DevRows(), &e1);
           if (l + 1 < h->depth) launch_act(c.s, cur, W, W, G, gd, ACT_SILU, 0.f);
         }
         std::swap(cur, nxt);
@@ -808,7 +918,9 @@ void run_jastrow_and_assemble(Ctx& c, const float* r, int B, bool tan, const Gro
       // (W_a = W_q key per head): rows [v (HD) | a (H) | 0]; the orbital rows are no longer needed
       float* QV = h->O.as<float>();
       const int HDP = h->HDP;
-      gemm(c, h->XA.as<float>(), F, h->iAV, h->iAV + 1, QV, HDP, R, HDP, F, flag, period);
+      ActEpi ea;
+      ea.amax = tan ? h->amx_xa : nullptr;
+      gemm(c, h->XA.as<float>(), F, h->iAV, h->iAV + 1, QV, HDP, R, HDP, F, flag, period, DevRows(), &ea);
       launch_att_group(c.s, QV, HDP, QV + HD, HDP, H, h->Dk, G, gd, tan, h->AV.as<float>());
     }
     launch_node_sums(c.s, h->AV.as<float>(), nc, nc, gd, B, N, h->n_up, h->att_ns.as<double>());
@@ -1362,6 +1474,12 @@ int fire_destroy(fire_handle* h) {
     if (p) (void)hipFree(p);
   for (float* p : h->Pt32)
     if (p) (void)hipFree(p);
+  for (uint16_t* p : h->Pf16)
+    if (p) (void)hipFree(p);
+  for (uint16_t* p : h->Pf16_lo)
+    if (p) (void)hipFree(p);
+  for (float* p : h->Pwinv)
+    if (p) (void)hipFree(p);
   if (h->R) (void)hipFree(h->R);
   if (h->Z) (void)hipFree(h->Z);
   DevBuf* bufs[] = {&h->cnt, &h->ptr, &h->nbr, &h->rev, &h->rowflag, &h->T5, &h->Z2, &h->T5b, &h->M5,
@@ -1369,7 +1487,7 @@ int fire_destroy(fire_handle* h) {
                     &h->tr, &h->trip, &h->dgrad, &h->det_lp, &h->det_sg, &h->det_gr, &h->det_lap, &h->JR,
                     &h->mlp_ns, &h->mlp_dg, &h->QV, &h->AV, &h->att_ns, &h->att_dg, &h->RIN, &h->RY1,
                     &h->RY2, &h->att_out, &h->cval, &h->clap, &h->cgrad, &h->epot, &h->PF, &h->EF, &h->EOK,
-                    &h->rowgrp, &h->AV0, &h->Z0, &h->F1, &h->PART};
+                    &h->rowgrp, &h->AV0, &h->Z0, &h->F1, &h->PART, &h->AMX_A, &h->AMX_B, &h->AMX_C};
   for (DevBuf* b : bufs) b->release();
   h->st.release();
   delete h;
@@ -1395,12 +1513,21 @@ int fire_upload_params(fire_handle* h, const float* packed, const int64_t* sizes
       if (p) HIPCHK(hipFree(p));
     for (float* p : h->Pt32)
       if (p) HIPCHK(hipFree(p));
+    for (uint16_t* p : h->Pf16)
+      if (p) HIPCHK(hipFree(p));
+    for (uint16_t* p : h->Pf16_lo)
+      if (p) HIPCHK(hipFree(p));
+    for (float* p : h->Pwinv)
+      if (p) HIPCHK(hipFree(p));
     const int n_all = n_tensors + (h->has_att ? (h->has_mlp ? 8 : 6) : 0);
     h->P.assign(n_all, nullptr);
     h->Pbf.assign(n_all, nullptr);
     h->Pbf_lo.assign(n_all, nullptr);
     h->Pbf_lo2.assign(n_all, nullptr);
     h->Pt32.assign(n_all, nullptr);
+    h->Pf16.assign(n_all, nullptr);
+    h->Pf16_lo.assign(n_all, nullptr);
+    h->Pwinv.assign(n_all, nullptr);
     // host copies of every tensor (uploaded, then derived)
     std::vector<const float*> host(n_all, nullptr);
     std::vector<int64_t> hsize(n_all, 0);
@@ -1493,6 +1620,27 @@ int fire_upload_params(fire_handle* h, const float* packed, const int64_t* sizes
         HIPCHK(hipMalloc(dst[q], sizeof(uint16_t) * pl[q].size()));
         HIPCHK(hipMemcpy(*dst[q], pl[q].data(), sizeof(uint16_t) * pl[q].size(), hipMemcpyHostToDevice));
       }
+      // two FP16 planes of the column-scaled weight, transposed to [N,K]: s_n W = hi + lo
+      std::vector<uint16_t> fh((size_t)K * Nn), fl((size_t)K * Nn);
+      std::vector<float> winv(Nn);
+      for (int n = 0; n < Nn; ++n) {
+        float mx = 0.f;
+        for (int kk = 0; kk < K; ++kk) mx = std::max(mx, std::fabs(src[(size_t)kk * Nn + n]));
+        const float sc = host_f16_scale(mx);
+        winv[n] = 1.0f / sc;  // a power of two: exact
+        for (int kk = 0; kk < K; ++kk) {
+          const float x = src[(size_t)kk * Nn + n] * sc;
+          const uint16_t hi = host_f2h(x);
+          fh[(size_t)n * K + kk] = hi;
+          fl[(size_t)n * K + kk] = host_f2h(x - host_h2f(hi));
+        }
+      }
+      HIPCHK(hipMalloc(&h->Pf16[k], sizeof(uint16_t) * fh.size()));
+      HIPCHK(hipMemcpy(h->Pf16[k], fh.data(), sizeof(uint16_t) * fh.size(), hipMemcpyHostToDevice));
+      HIPCHK(hipMalloc(&h->Pf16_lo[k], sizeof(uint16_t) * fl.size()));
+      HIPCHK(hipMemcpy(h->Pf16_lo[k], fl.data(), sizeof(uint16_t) * fl.size(), hipMemcpyHostToDevice));
+      HIPCHK(hipMalloc(&h->Pwinv[k], sizeof(float) * winv.size()));
+      HIPCHK(hipMemcpy(h->Pwinv[k], winv.data(), sizeof(float) * winv.size(), hipMemcpyHostToDevice));
     }
     // FP32 transposed copies [N,K] for the FP32 MFMA kernel: every GEMM weight, and the
     // envelope weights pi per determinant ([det][ME][N] -> [det][N][ME])
@@ -1840,7 +1988,7 @@ int fire_precision_plan(fire_handle* h, const int32_t* planes, int32_t n, int32_
     REQUIRE(h, "null handle");
     REQUIRE(n >= 0 && n <= 6, "at most 6 contraction classes");
     for (int k = 0; k < n; ++k)
-      REQUIRE(planes[k] >= 1 && planes[k] <= 3, "planes per operand must be 1, 2 or 3");
+      REQUIRE(planes[k] >= 1 && planes[k] <= 4, "planes per operand must be 1, 2 or 3 (BF16) or 4 (FP16 pair)");
     for (int k = 0; k < n; ++k) h->planes[k] = planes[k];
     if (n > 0 && h->st.graph) {  // the captured sweep holds the previous contraction choices
       HIPCHK(hipGraphExecDestroy(h->st.graph));

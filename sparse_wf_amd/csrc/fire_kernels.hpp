@@ -58,7 +58,7 @@ void launch_act(hipStream_t s, float* Y, int64_t ld, int ncols, int G, GroupDesc
 void launch_messages(hipStream_t s, const float* r, int B, int N, int n_up, const int64_t* ptr, const int32_t* nbr,
                      const float* M5, EdgeParams same, EdgeParams diff, float cut, int F, bool tangents, float* out,
                      const int2* list = nullptr, int n_list = 0, int kmove = -1, const float* M5k = nullptr,
-                     const int32_t* dn = nullptr, float* pf = nullptr);
+                     const int32_t* dn = nullptr, float* pf = nullptr, uint32_t* omax = nullptr);
 // floats per pair of the pair-filter buffer pf that launch_messages needs with tangent rows
 size_t pair_filter_floats(int w1);
 void launch_scale(hipStream_t s, float* p, int64_t n, float a);
@@ -83,6 +83,10 @@ struct DevRows {
 //           group's jacobian rows) + lane half; at most 8 slots);
 //   after:  Laplacian rows f1 z + silu''(z0) * (sum of the group's slots), in k_act_lapfix.
 // rowgrp == nullptr: plain store.
+// FP16 planes (nsplit 4): amax[row] = float bits of max |A[row, :]|, winv[col] = inverse
+// power-of-two scale of W's column.  omax (optional, fused epilogue only): row maxima of the
+// output over its first omax_cols columns, accumulated with atomicMax into a zeroed array
+// (Laplacian rows are left to k_act_lapfix), for the next contraction's FP16 planes.
 struct ActEpi {
   const int32_t* rowgrp = nullptr;
   const float* f1 = nullptr;
@@ -90,14 +94,19 @@ struct ActEpi {
   GroupDesc gd{0, 0, nullptr};
   int ldf = 0;            // row stride of f1 and of each part slot
   int nfuse = 1 << 30;    // columns at or past nfuse are stored plain (+ bias on value rows)
+  const uint32_t* amax = nullptr;
+  const float* winv = nullptr;
+  uint32_t* omax = nullptr;
+  int omax_cols = 0;
 };
 constexpr int ACT_EPI_SLOTS = 8;
 
 void launch_rowgrp(hipStream_t s, int G, GroupDesc gd, int32_t* rowgrp);
 void launch_gather_rows(hipStream_t s, const float* A, int64_t lda, int K, int G, GroupDesc gd, float* out);
 void launch_silu_grad(hipStream_t s, const float* z0, float* f1, int64_t n);
+// omax (optional): sets the row maxima of the completed Laplacian rows (atomicMax)
 void launch_act_lapfix(hipStream_t s, float* C, int64_t ldc, int N, int G, GroupDesc gd, const float* z0,
-                       const float* part, int ldf);
+                       const float* part, int ldf, uint32_t* omax = nullptr);
 
 // dense contractions: C[M,N] = A[M,K] W[K,N] (+ bias on rows with flag / row % period == 0)
 // W is [K,N] (used for N <= 16), Wt its transpose [N,K] (used by the MFMA kernel)
@@ -109,7 +118,8 @@ void launch_gemm_f32_tail(hipStream_t s, const float* A, int64_t lda, const floa
                           DevRows dr, int64_t sA, int64_t sW, int64_t sC, int nb);
 void launch_gemm_f32_batched(hipStream_t s, const float* A, int64_t lda, int64_t sA, const float* Wt, int64_t sW,
                              float* C, int64_t ldc, int64_t sC, int64_t M, int N, int K, int nb);
-// nsplit 1: plain BF16; 3: W = hi + lo planes, 3 products; 6: W = hi + mid + lo, 6 products
+// nsplit 1: plain BF16; 3: W = hi + lo planes, 3 products; 6: W = hi + mid + lo, 6 products;
+// 4: two FP16 planes of the scaled operands (ep.amax, ep.winv), 3 products
 // variant: tile shape index in [0, gemm_bf16_variants()) (0 = 128 x 128)
 void launch_gemm_bf16(hipStream_t s, const float* A, int64_t lda, const uint16_t* Whi /*[N,K] bf16*/,
                       const uint16_t* Wlo, const uint16_t* Wlo2, const float* bias, float* C, int64_t ldc, int64_t M,
@@ -118,6 +128,8 @@ void launch_gemm_bf16(hipStream_t s, const float* A, int64_t lda, const uint16_t
 int gemm_bf16_variants();
 // variants that have a fused-silu (ActEpi) form; the others are never launched with one
 bool gemm_bf16_fusable(int variant);
+// variants that have an FP16-plane form (nsplit 4)
+bool gemm_bf16_f16_ok(int variant);
 void gemm_bf16_tile(int variant, int* bm, int* bn);
 
 }  // namespace fire

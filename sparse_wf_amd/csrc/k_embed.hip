@@ -12,6 +12,16 @@
 
 namespace fire {
 
+// max |v| over the 64 lanes of the wavefront (all active), added to *dst (float bits, which
+// order as integers for non-negative floats) by lane 0 with atomicMax: the row maxima that the
+// FP16 planes of the next contraction are scaled by
+__device__ __forceinline__ void wave_max_abs_to(float v, uint32_t* dst) {
+  float m = fabsf(v);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+  if ((threadIdx.x & 63) == 0 && m > 0.f) atomicMax(dst, __float_as_uint(m));
+}
+
 #define EN_CHUNK 64
 
 // Forward-Laplacian e-n filter of every (electron, nucleus) pair, one thread per pair:
@@ -348,10 +358,10 @@ void launch_silu_grad(hipStream_t s, const float* z0, float* f1, int64_t n) {
 // block the jacobian rows touch and each lane half (row / 4 odd or even) that holds one of
 // them there).
 __global__ void k_act_lapfix(float* __restrict__ C, int64_t ldc, int N, GroupDesc gd, const float* __restrict__ z0,
-                             const float* __restrict__ part, int ldf) {
+                             const float* __restrict__ part, int ldf, uint32_t* __restrict__ omax) {
   const int g = blockIdx.x;
   const int c = blockIdx.y * blockDim.x + threadIdx.x;
-  if (c >= N) return;
+  if (c >= N) return;  // whole wavefronts: N is a multiple of 128 here
   const int64_t r0 = gd.row(g), r1 = gd.row(g + 1);
   if (r1 - r0 < 2) return;
   float* y = C + (r0 + 1) * ldc + c;
@@ -377,14 +387,16 @@ __global__ void k_act_lapfix(float* __restrict__ C, int64_t ldc, int N, GroupDes
   for (int q = 0; q < ACT_EPI_SLOTS; ++q) ss += v[q];
   float f, f1, f2;
   silu3(zv, f, f1, f2);
-  *y = zl * f1 + f2 * ss;
+  const float out = zl * f1 + f2 * ss;
+  *y = out;
+  if (omax) wave_max_abs_to(out, omax + r0 + 1);
 }
 
 void launch_act_lapfix(hipStream_t s, float* C, int64_t ldc, int N, int G, GroupDesc gd, const float* z0,
-                       const float* part, int ldf) {
+                       const float* part, int ldf, uint32_t* omax) {
   if (G == 0) return;
   dim3 grid(G, (N + 255) / 256);
-  k_act_lapfix<<<grid, 256, 0, s>>>(C, ldc, N, gd, z0, part, ldf);
+  k_act_lapfix<<<grid, 256, 0, s>>>(C, ldc, N, gd, z0, part, ldf, omax);
 }
 
 __global__ void k_rowflag(int G, GroupDesc gd, uint8_t* flag) {
@@ -461,7 +473,7 @@ __global__ void __launch_bounds__(256) k_messages(const float* __restrict__ r, i
                                                    float cut, int F, float* __restrict__ out,
                                                    const int2* __restrict__ list, int kmove,
                                                    const float* __restrict__ M5k, const int32_t* __restrict__ dn,
-                                                   const float* __restrict__ pf) {
+                                                   const float* __restrict__ pf, uint32_t* __restrict__ omax) {
   // list mode (value rows only): centres are list[g] = (walker, electron); the h0/message
   // rows of electron `kmove` of every walker come from M5k[walker] instead of M5; dn
   // (optional, device) is the number of list entries in use.
@@ -503,6 +515,7 @@ __global__ void __launch_bounds__(256) k_messages(const float* __restrict__ r, i
 
   const float* m5i = m5_of(i);
   float acc[5], mcs[5], mcd[5];
+  float gmx = 0.f;  // max |value| this thread wrote (tangent rows: row maxima of the group)
   for (int rr = 0; rr < NR; ++rr) {
     acc[rr] = m5i[rr * 5 * F + 0 * F + f];
     mcs[rr] = m5i[rr * 5 * F + 1 * F + f];
@@ -633,11 +646,14 @@ __global__ void __launch_bounds__(256) k_messages(const float* __restrict__ r, i
           const float ul = 2.f * E5[1] + mc[1] + mn[1];
           const float vl = s1 * ul + s2 * q2;
           float cross = 0.f;
-          float* orow = out + (row0 + 5 + 3 * (int64_t)(c0 + kk)) * F + f;
+          const int64_t jrow = row0 + 5 + 3 * (int64_t)(c0 + kk);
+          float* orow = out + jrow * F + f;
           for (int t = 0; t < 3; ++t) {
             const float vc = s1 * uc[t], vn = s1 * un[t];
             acc[2 + t] += G5[2 + t] * sv + G5[0] * vc;
-            orow[t * F] = -G5[2 + t] * sv + G5[0] * vn;
+            const float o = -G5[2 + t] * sv + G5[0] * vn;
+            orow[t * F] = o;
+            gmx = fmaxf(gmx, fabsf(o));
             cross += G5[2 + t] * (vc - vn);
           }
           acc[1] += 2.f * G5[1] * sv + G5[0] * vl + 2.f * cross;
@@ -647,23 +663,40 @@ __global__ void __launch_bounds__(256) k_messages(const float* __restrict__ r, i
     __syncthreads();
   }
   float* o = out + row0 * F + f;
-  for (int rr = 0; rr < NR; ++rr) o[rr * F] = acc[rr];
+  for (int rr = 0; rr < NR; ++rr) {
+    o[rr * F] = acc[rr];
+    gmx = fmaxf(gmx, fabsf(acc[rr]));
+  }
+  if (TAN && omax) {
+    // one maximum for all rows of the group (block reduction), written to each of its rows:
+    // an upper bound of every row's own maximum, which is all the FP16 scales need
+    __shared__ float s_mx[4];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) gmx = fmaxf(gmx, __shfl_xor(gmx, off, 64));
+    if ((threadIdx.x & 63) == 0) s_mx[threadIdx.x >> 6] = gmx;
+    __syncthreads();
+    float m = s_mx[0];
+    for (int q = 1; q < (int)(blockDim.x >> 6); ++q) m = fmaxf(m, s_mx[q]);
+    const int nrow = 5 + 3 * nb;
+    for (int rr = threadIdx.x; rr < nrow; rr += blockDim.x) omax[row0 + rr] = __float_as_uint(m);
+  }
 }
 
 void launch_messages(hipStream_t s, const float* r, int B, int N, int n_up, const int64_t* ptr, const int32_t* nbr,
                      const float* M5, EdgeParams same, EdgeParams diff, float cut, int F, bool tangents, float* out,
-                     const int2* list, int n_list, int kmove, const float* M5k, const int32_t* dn, float* pf) {
+                     const int2* list, int n_list, int kmove, const float* M5k, const int32_t* dn, float* pf,
+                     uint32_t* omax) {
   const int G = list ? n_list : B * N;
   if (G == 0) return;
   if (tangents) {
     W1_DISPATCH(same.filt.w1, {
       k_pair_filter<W1><<<G, 64, 0, s>>>(r, B, N, n_up, ptr, nbr, same, diff, cut, pf);
       k_messages<true, W1><<<G, F, 0, s>>>(r, B, N, n_up, ptr, nbr, M5, same, diff, cut, F, out, list, kmove, M5k,
-                                           dn, pf);
+                                           dn, pf, omax);
     });
   } else {
     W1_DISPATCH(same.filt.w1, k_messages<false, W1><<<G, F, 0, s>>>(r, B, N, n_up, ptr, nbr, M5, same, diff, cut, F,
-                                                                    out, list, kmove, M5k, dn, nullptr));
+                                                                    out, list, kmove, M5k, dn, nullptr, nullptr));
   }
 }
 

@@ -45,6 +45,100 @@ __device__ __forceinline__ uint16_t f2bf(float x) {
 }
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 
+// ---------------------------------------------------------------------------------------
+// Two FP16 planes with power-of-two scales ("F16x3").  A row r of A is scaled by
+// s_r = 2^(141 - e_r), e_r = biased exponent of max_k |A[r,k]| (so the row maximum lands in
+// [2^14, 2^15)), a column n of W by its own s_n chosen the same way at upload; each scaled
+// operand is x = hi + lo in FP16 (11 + 11 significand bits, lo exact in FP32 before its own
+// rounding) and the contraction accumulates lo*hi + hi*lo + hi*hi in FP32 (v_mfma_f32_32x32x16_f16,
+// every product exact).  The accumulators are unscaled by 1 / (s_r s_n) before the epilogue.
+// Row maxima arrive as the float bits of max |A[r,:]| (non-negative floats order as integers),
+// accumulated with atomicMax by the kernel that produced A.
+// ---------------------------------------------------------------------------------------
+typedef _Float16 f16x8_t __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x2_t __attribute__((ext_vector_type(2)));
+
+template <bool F16>
+using OpT = std::conditional_t<F16, f16x8_t, bf16x8_t>;
+
+__device__ __forceinline__ int f16_exponent(uint32_t mbits) {
+  const int e = (int)((mbits >> 23) & 0xffu);
+  return e < 15 ? 15 : (e > 254 ? 254 : e);
+}
+// s = 2^(141 - e) and 1 / s = 2^(e - 141) as float bits (biased exponents 268 - e and e - 14)
+__device__ __forceinline__ float f16_row_scale(uint32_t mbits) {
+  return __uint_as_float((uint32_t)(268 - f16_exponent(mbits)) << 23);
+}
+__device__ __forceinline__ float f16_row_unscale(uint32_t mbits) {
+  return __uint_as_float((uint32_t)(f16_exponent(mbits) - 14) << 23);
+}
+__device__ __forceinline__ float row_scale_of(const uint32_t* amax, int64_t row, int64_t M) {
+  return row < M ? f16_row_scale(amax[row]) : 1.f;
+}
+
+// The NP planes of four consecutive A values (two 32-bit words per plane): BF16 planes
+// (x = hi + mid + lo, each remainder exact in FP32), or with F16 the FP16 planes of s * x.
+template <int NP, bool F16>
+__device__ __forceinline__ void split_planes(const float4 x, float s, uint2 (&out)[NP]) {
+  float2v_t x01 = {x.x, x.y}, x23 = {x.z, x.w};
+  if constexpr (F16) {
+    x01 = x01 * s;
+    x23 = x23 * s;
+  }
+#pragma unroll
+  for (int p = 0; p < NP; ++p) {
+    if constexpr (F16) {
+      const f16x2_t h01 = __builtin_convertvector(x01, f16x2_t);
+      const f16x2_t h23 = __builtin_convertvector(x23, f16x2_t);
+      out[p] = make_uint2(__builtin_bit_cast(uint32_t, h01), __builtin_bit_cast(uint32_t, h23));
+      if (p + 1 < NP) {
+        x01 = x01 - __builtin_convertvector(h01, float2v_t);
+        x23 = x23 - __builtin_convertvector(h23, float2v_t);
+      }
+    } else {
+      const bf16x2_t h01 = __builtin_convertvector(x01, bf16x2_t);
+      const bf16x2_t h23 = __builtin_convertvector(x23, bf16x2_t);
+      out[p] = make_uint2(__builtin_bit_cast(uint32_t, h01), __builtin_bit_cast(uint32_t, h23));
+      if (p + 1 < NP) {
+        x01 = x01 - __builtin_convertvector(h01, float2v_t);
+        x23 = x23 - __builtin_convertvector(h23, float2v_t);
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ float16_t mfma16(const bf16x8_t& a, const bf16x8_t& b, const float16_t& c, int, int, int) {
+  return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ float16_t mfma16(const f16x8_t& a, const f16x8_t& b, const float16_t& c, int, int, int) {
+  return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+}
+
+// Row maxima of a wavefront's 64-row block for the next contraction: mx[a][v] is the lane's
+// max |value| of row 32 a + 8 (v / 4) + 4 (lane / 32) + v % 4 over its columns; the 32 lanes of
+// each half combine theirs and lane t of the half adds row t of the half with atomicMax.
+__device__ __forceinline__ void rows_max_out(float (&mx)[2][16], int64_t rb, int lane, int64_t M,
+                                             uint32_t* __restrict__ omax) {
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+      float m = mx[a][v];
+#pragma unroll
+      for (int off = 1; off < 32; off <<= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+      mx[a][v] = m;
+    }
+  const int t = lane & 31;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int v = 0; v < 16; ++v)
+      if (t == a * 16 + v) {
+        const int64_t row = rb + a * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+        if (row < M && mx[a][v] > 0.f) atomicMax(omax + row, __float_as_uint(mx[a][v]));
+      }
+}
+
 #define GT_M 128
 #define GT_N 128
 #define GT_K 32
@@ -83,12 +177,47 @@ __device__ __forceinline__ void tile_of(int64_t& m_tile, int& n_tile, int nT) {
 // Writes a wavefront's 64 x (32 NB) block of 32x32 MFMA accumulators (lane l, register v of
 // tile (a, b): row 32 a + 8 (v / 4) + 4 (l / 32) + v % 4, column 32 b + l % 32), either plain
 // (+ bias on value rows) or through the fused forward-Laplacian silu described at ActEpi.
-template <int NB, bool FUSE, typename AccT>
+// With F16 every accumulator is first unscaled by 1 / (s_r s_n): lane t of the 64-row block
+// loads the row maximum of row t once and the rows' factors travel by shuffle, so no row
+// address is held across the block.
+template <int NB, bool FUSE, bool F16 = false, typename AccT>
 __device__ __forceinline__ void store_rows(const AccT (&acc)[2][NB], int64_t m0, int n0, int wm, int wn, int lane,
                                            float* __restrict__ C, int64_t ldc, int64_t M, int N,
                                            const float* __restrict__ bias, const uint8_t* __restrict__ rowflag,
                                            int period, const ActEpi& ep) {
-  if constexpr (!FUSE) {
+  float myrs = 1.f;
+  if constexpr (F16) {
+    const int64_t r = m0 + wm + lane;
+    myrs = r < M ? f16_row_unscale(ep.amax[r]) : 0.f;
+  }
+  if constexpr (!FUSE && F16) {
+    float cs[NB], bcol[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const int col = n0 + wn + b * 32 + (lane & 31);
+      cs[b] = col < N ? ep.winv[col] : 0.f;
+      bcol[b] = (bias && col < N) ? bias[col] : 0.f;
+    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const int rl = a * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+        const float rs = __shfl(myrs, rl);
+        const int64_t row = m0 + wm + rl;
+        if (row >= M) continue;
+        const bool br = bias && is_bias_row(row, rowflag, period);
+        float* crow = C + row * ldc;
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+          const int col = n0 + wn + b * 32 + (lane & 31);
+          if (col >= N) continue;
+          float val = acc[a][b][v] * (rs * cs[b]);
+          if (br) val += bcol[b];
+          crow[col] = val;
+        }
+      }
+  } else if constexpr (!FUSE) {
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
       const int col = n0 + wn + b * 32 + (lane & 31);
@@ -115,7 +244,7 @@ __device__ __forceinline__ void store_rows(const AccT (&acc)[2][NB], int64_t m0,
     const int mycode = rb + lane < M ? ep.rowgrp[rb + lane] : -3;
     int col[NB];
     bool colok[NB], fz[NB];
-    float bcol[NB], ss[NB], f1[NB];
+    float bcol[NB], ss[NB], f1[NB], cs[NB];
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
       col[b] = n0 + wn + b * 32 + (lane & 31);
@@ -124,8 +253,15 @@ __device__ __forceinline__ void store_rows(const AccT (&acc)[2][NB], int64_t m0,
       bcol[b] = (bias && colok[b]) ? bias[col[b]] : 0.f;
       ss[b] = 0.f;
       f1[b] = 0.f;
+      cs[b] = 1.f;
+      if constexpr (F16) cs[b] = colok[b] ? ep.winv[col[b]] : 0.f;
     }
     int cur = -1;
+    float mx[2][16];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) mx[a][v] = 0.f;
     auto flush = [&]() {
       const int64_t first = (ep.gd.row(cur) + 2) >> 6;  // block of the group's first jacobian row
       const int slot = 2 * (int)((rb >> 6) - first) + h;
@@ -141,6 +277,8 @@ __device__ __forceinline__ void store_rows(const AccT (&acc)[2][NB], int64_t m0,
       for (int v = 0; v < 16; ++v) {
         const int rl = a * 32 + 8 * (v >> 2) + 4 * h + (v & 3);
         const int cd = __shfl(mycode, rl);
+        float rs = 1.f;
+        if constexpr (F16) rs = __shfl(myrs, rl);
         if (cd == -3) continue;
         if (cd >= 0 && cd != cur) {
           if (cur >= 0) flush();
@@ -156,6 +294,7 @@ __device__ __forceinline__ void store_rows(const AccT (&acc)[2][NB], int64_t m0,
         for (int b = 0; b < NB; ++b) {
           if (!colok[b]) continue;
           float z = acc[a][b][v];
+          if constexpr (F16) z *= rs * cs[b];
           if (!fz[b]) {
             if (cd == -1) z += bcol[b];
           } else if (cd >= 0) {
@@ -166,18 +305,22 @@ __device__ __forceinline__ void store_rows(const AccT (&acc)[2][NB], int64_t m0,
             z *= 1.0f / (1.0f + expf(-z));
           }
           crow[col[b]] = z;
+          // row maxima for the next contraction (Laplacian rows are completed, and their
+          // maxima taken, by k_act_lapfix)
+          if (ep.omax && cd != -2 && col[b] < ep.omax_cols) mx[a][v] = fmaxf(mx[a][v], fabsf(z));
         }
       }
     if (cur >= 0) flush();
+    if (ep.omax) rows_max_out(mx, rb, lane, M, ep.omax);
   }
 }
 
-template <bool FUSE, typename AccT>
+template <bool FUSE, bool F16 = false, typename AccT>
 __device__ __forceinline__ void store_tile(const AccT (&acc)[2][2], int64_t m0, int n0, int wm, int wn, int lane,
                                            float* __restrict__ C, int64_t ldc, int64_t M, int N,
                                            const float* __restrict__ bias, const uint8_t* __restrict__ rowflag,
                                            int period, const ActEpi& ep) {
-  store_rows<2, FUSE>(acc, m0, n0, wm, wn, lane, C, ldc, M, N, bias, rowflag, period, ep);
+  store_rows<2, FUSE, F16>(acc, m0, n0, wm, wn, lane, C, ldc, M, N, bias, rowflag, period, ep);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -281,7 +424,7 @@ __global__ void __launch_bounds__(256) k_gemm_f32(const float* __restrict__ A, i
 // quarter (2 x 2 MFMA 32x32x16 tiles) per wavefront; K staged 32 at a time with the next
 // chunk prefetched into registers.  Requires K % 32 == 0 and lda % 4 == 0.
 // ---------------------------------------------------------------------------------------
-template <int NP, int BM, int BN, bool FUSE = false>
+template <int NP, int BM, int BN, bool FUSE = false, bool F16 = false>
 __global__ void __launch_bounds__(BM * BN / 64) k_gemm_bf16(const float* __restrict__ A, int64_t lda,
                                                            const uint16_t* __restrict__ Wp0,
                                                            const uint16_t* __restrict__ Wp1,
@@ -307,6 +450,8 @@ __global__ void __launch_bounds__(BM * BN / 64) k_gemm_bf16(const float* __restr
     A += o * lda;
     C += o * ldc;
     if (rowflag) rowflag += o;
+    if (ep.amax) ep.amax += o;
+    if (ep.omax) ep.omax += o;
   }
   if (dr.count) M = M < (int64_t)*dr.count ? M : (int64_t)*dr.count;
   if (mt * BM >= M) return;
@@ -318,6 +463,10 @@ __global__ void __launch_bounds__(BM * BN / 64) k_gemm_bf16(const float* __restr
     for (int b = 0; b < 2; ++b)
       for (int v = 0; v < 16; ++v) acc[a][b][v] = 0.f;
   const uint16_t* Wpl[3] = {Wp0, Wp1, Wp2};
+  // FP16 planes: the scale of every A row this thread splits (the same rows in every chunk)
+  float rsc[A_PER];
+#pragma unroll
+  for (int q = 0; q < A_PER; ++q) rsc[q] = F16 ? row_scale_of(ep.amax, m0 + ((tid + q * NT) >> 3), M) : 1.f;
 
   // A (HBM) is prefetched two chunks ahead through two register sets, the weight planes
   // (L2-resident) one chunk ahead: about twice the A bytes in flight of a single buffer.
@@ -350,18 +499,10 @@ __global__ void __launch_bounds__(BM * BN / 64) k_gemm_bf16(const float* __restr
       const int rr = e >> 3, kq = (e & 7) * 4;
       // x = hi + mid + lo with hardware round-to-nearest conversions (v_cvt_pk_bf16_f32);
       // each remainder is exact in FP32
-      float2v_t x01 = {src[q].x, src[q].y}, x23 = {src[q].z, src[q].w};
+      uint2 pw[NP];
+      split_planes<NP, F16>(src[q], rsc[q], pw);
 #pragma unroll
-      for (int p = 0; p < NP; ++p) {
-        const bf16x2_t h01 = __builtin_convertvector(x01, bf16x2_t);
-        const bf16x2_t h23 = __builtin_convertvector(x23, bf16x2_t);
-        *reinterpret_cast<uint2*>(&As[p][rr][kq]) =
-            make_uint2(__builtin_bit_cast(uint32_t, h01), __builtin_bit_cast(uint32_t, h23));
-        if (p + 1 < NP) {
-          x01 = x01 - __builtin_convertvector(h01, float2v_t);
-          x23 = x23 - __builtin_convertvector(h23, float2v_t);
-        }
-      }
+      for (int p = 0; p < NP; ++p) *reinterpret_cast<uint2*>(&As[p][rr][kq]) = pw[p];
     }
 #pragma unroll
     for (int q = 0; q < B_PER; ++q) {
@@ -377,13 +518,13 @@ __global__ void __launch_bounds__(BM * BN / 64) k_gemm_bf16(const float* __restr
 #pragma unroll
     for (int kk = 0; kk < GT_K; kk += 16) {
       const int ka = kk + 8 * (lane >> 5);
-      bf16x8_t av[NP][2], bv[NP][2];
+      OpT<F16> av[NP][2], bv[NP][2];
 #pragma unroll
       for (int p = 0; p < NP; ++p) {
 #pragma unroll
-        for (int a = 0; a < 2; ++a) av[p][a] = *reinterpret_cast<const bf16x8_t*>(&As[p][wm + a * 32 + (lane & 31)][ka]);
+        for (int a = 0; a < 2; ++a) av[p][a] = *reinterpret_cast<const OpT<F16>*>(&As[p][wm + a * 32 + (lane & 31)][ka]);
 #pragma unroll
-        for (int b = 0; b < nbc; ++b) bv[p][b] = *reinterpret_cast<const bf16x8_t*>(&Bs[p][wn + b * 32 + (lane & 31)][ka]);
+        for (int b = 0; b < nbc; ++b) bv[p][b] = *reinterpret_cast<const OpT<F16>*>(&Bs[p][wn + b * 32 + (lane & 31)][ka]);
       }
       // product-major order: four independent accumulators between dependent MFMAs
 #pragma unroll
@@ -392,7 +533,7 @@ __global__ void __launch_bounds__(BM * BN / 64) k_gemm_bf16(const float* __restr
         for (int a = 0; a < 2; ++a)
 #pragma unroll
           for (int b = 0; b < nbc; ++b)
-            acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[prod_a(NP, t)][a], bv[prod_b(NP, t)][b],
+            acc[a][b] = mfma16(av[prod_a(NP, t)][a], bv[prod_b(NP, t)][b],
                                                                 acc[a][b], 0, 0, 0);
     }
   };
@@ -422,7 +563,7 @@ __global__ void __launch_bounds__(BM * BN / 64) k_gemm_bf16(const float* __restr
     compute();
     __syncthreads();
   }
-  store_tile<FUSE>(acc, m0, n0, wm, wn, lane, C, ldc, M, N, bias, rowflag, period, ep);
+  store_tile<FUSE, F16>(acc, m0, n0, wm, wn, lane, C, ldc, M, N, bias, rowflag, period, ep);
 }
 
 // Small-N FP32 GEMM (N <= 16): one wavefront per row, lanes across K.  W is [K,N].
@@ -751,7 +892,7 @@ __global__ void __launch_bounds__(BM * BN / 64) k_gemm_bf16_db(const float* __re
 // ahead.  With A_PER float4 per thread and chunk the ring costs 4 PA A_PER registers; the
 // bytes of A in flight per CU grow PA-fold, which is what bounds the staged forms on the
 // tall, narrow contractions of the tangent rows (a few hundred columns, K = 256).
-template <int NP, int BM, int BN, int KC, int PA, bool FUSE = false>
+template <int NP, int BM, int BN, int KC, int PA, bool FUSE = false, bool F16 = false>
 __global__ void __launch_bounds__(BM * BN / 64) k_gemm_bf16_dq(const float* __restrict__ A, int64_t lda,
                                                               const uint16_t* __restrict__ Wp0,
                                                               const uint16_t* __restrict__ Wp1,
@@ -780,6 +921,8 @@ __global__ void __launch_bounds__(BM * BN / 64) k_gemm_bf16_dq(const float* __re
     A += o * lda;
     C += o * ldc;
     if (rowflag) rowflag += o;
+    if (ep.amax) ep.amax += o;
+    if (ep.omax) ep.omax += o;
   }
   if (dr.count) M = M < (int64_t)*dr.count ? M : (int64_t)*dr.count;
   if (mt * BM >= M) return;
@@ -791,6 +934,13 @@ __global__ void __launch_bounds__(BM * BN / 64) k_gemm_bf16_dq(const float* __re
     for (int b = 0; b < 2; ++b)
       for (int v = 0; v < 16; ++v) acc[a][b][v] = 0.f;
   const uint16_t* Wpl[3] = {Wp0, Wp1, Wp2};
+  // FP16 planes: the scale of every A row this thread splits (the same rows in every chunk)
+  float rsc[A_PER];
+#pragma unroll
+  for (int q = 0; q < A_PER; ++q) {
+    const int e = tid + q * NT;
+    rsc[q] = (F16 && e < A_UNITS) ? row_scale_of(ep.amax, m0 + e / (KC / 4), M) : 1.f;
+  }
   float4 ra[PA][A_PER];
   uint4 rb[NP][B_PER];
   auto load_a = [&](int c, float4(&xa)[A_PER]) {
@@ -821,18 +971,10 @@ __global__ void __launch_bounds__(BM * BN / 64) k_gemm_bf16_dq(const float* __re
       const int e = tid + q * NT;
       if (A_UNITS % NT != 0 && e >= A_UNITS) continue;
       const int rr = e / (KC / 4), kq = (e % (KC / 4)) * 4;
-      float2v_t x01 = {xa[q].x, xa[q].y}, x23 = {xa[q].z, xa[q].w};
+      uint2 pw[NP];
+      split_planes<NP, F16>(xa[q], rsc[q], pw);
 #pragma unroll
-      for (int p = 0; p < NP; ++p) {
-        const bf16x2_t h01 = __builtin_convertvector(x01, bf16x2_t);
-        const bf16x2_t h23 = __builtin_convertvector(x23, bf16x2_t);
-        *reinterpret_cast<uint2*>(As(st, p, rr) + kq) =
-            make_uint2(__builtin_bit_cast(uint32_t, h01), __builtin_bit_cast(uint32_t, h23));
-        if (p + 1 < NP) {
-          x01 = x01 - __builtin_convertvector(h01, float2v_t);
-          x23 = x23 - __builtin_convertvector(h23, float2v_t);
-        }
-      }
+      for (int p = 0; p < NP; ++p) *reinterpret_cast<uint2*>(As(st, p, rr) + kq) = pw[p];
     }
 #pragma unroll
     for (int q = 0; q < B_PER; ++q) {
@@ -847,13 +989,13 @@ __global__ void __launch_bounds__(BM * BN / 64) k_gemm_bf16_dq(const float* __re
 #pragma unroll
     for (int kk = 0; kk < KC; kk += 16) {
       const int ka = kk + 8 * (lane >> 5);
-      bf16x8_t av[NP][2], bv[NP][2];
+      OpT<F16> av[NP][2], bv[NP][2];
 #pragma unroll
       for (int p = 0; p < NP; ++p) {
 #pragma unroll
-        for (int a = 0; a < 2; ++a) av[p][a] = *reinterpret_cast<const bf16x8_t*>(As(st, p, wm + a * 32 + (lane & 31)) + ka);
+        for (int a = 0; a < 2; ++a) av[p][a] = *reinterpret_cast<const OpT<F16>*>(As(st, p, wm + a * 32 + (lane & 31)) + ka);
 #pragma unroll
-        for (int b = 0; b < 2; ++b) bv[p][b] = *reinterpret_cast<const bf16x8_t*>(Bs(st, p, wn + b * 32 + (lane & 31)) + ka);
+        for (int b = 0; b < 2; ++b) bv[p][b] = *reinterpret_cast<const OpT<F16>*>(Bs(st, p, wn + b * 32 + (lane & 31)) + ka);
       }
 #pragma unroll
       for (int t = 0; t < n_products(NP); ++t)
@@ -861,7 +1003,7 @@ __global__ void __launch_bounds__(BM * BN / 64) k_gemm_bf16_dq(const float* __re
         for (int a = 0; a < 2; ++a)
 #pragma unroll
           for (int b = 0; b < 2; ++b)
-            acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[prod_a(NP, t)][a], bv[prod_b(NP, t)][b],
+            acc[a][b] = mfma16(av[prod_a(NP, t)][a], bv[prod_b(NP, t)][b],
                                                                 acc[a][b], 0, 0, 0);
     }
   };
@@ -890,14 +1032,14 @@ __global__ void __launch_bounds__(BM * BN / 64) k_gemm_bf16_dq(const float* __re
       }
     }
   }
-  store_tile<FUSE>(acc, m0, n0, wm, wn, lane, C, ldc, M, N, bias, rowflag, period, ep);
+  store_tile<FUSE, F16>(acc, m0, n0, wm, wn, lane, C, ldc, M, N, bias, rowflag, period, ep);
 }
 
 // Wide-wavefront form of the deep-A kernel: each wavefront computes 64 x 128 (2 x 4 MFMA
 // tiles, 128 accumulators), so a K step reads 6 NP operand fragments from LDS for 8 tiles
 // instead of 4 NP for 4, and a 256 x 256 block stages each weight chunk once for 256 rows.
 // Both cut the LDS traffic per output, which bounds the 64 x 64 forms.
-template <int NP, int BM, int BN, int KC, int PA, bool FUSE = false>
+template <int NP, int BM, int BN, int KC, int PA, bool FUSE = false, bool F16 = false>
 __global__ void __launch_bounds__(BM * BN / 128) __attribute__((amdgpu_waves_per_eu(2))) k_gemm_bf16_dw(const float* __restrict__ A, int64_t lda,
                                                               const uint16_t* __restrict__ Wp0,
                                                               const uint16_t* __restrict__ Wp1,
@@ -926,6 +1068,8 @@ __global__ void __launch_bounds__(BM * BN / 128) __attribute__((amdgpu_waves_per
     A += o * lda;
     C += o * ldc;
     if (rowflag) rowflag += o;
+    if (ep.amax) ep.amax += o;
+    if (ep.omax) ep.omax += o;
   }
   if (dr.count) M = M < (int64_t)*dr.count ? M : (int64_t)*dr.count;
   if (mt * BM >= M) return;
@@ -937,6 +1081,13 @@ __global__ void __launch_bounds__(BM * BN / 128) __attribute__((amdgpu_waves_per
     for (int b = 0; b < 4; ++b)
       for (int v = 0; v < 16; ++v) acc[a][b][v] = 0.f;
   const uint16_t* Wpl[3] = {Wp0, Wp1, Wp2};
+  // FP16 planes: the scale of every A row this thread splits (the same rows in every chunk)
+  float rsc[A_PER];
+#pragma unroll
+  for (int q = 0; q < A_PER; ++q) {
+    const int e = tid + q * NT;
+    rsc[q] = (F16 && e < A_UNITS) ? row_scale_of(ep.amax, m0 + e / (KC / 4), M) : 1.f;
+  }
   float4 ra[PA][A_PER];
   uint4 rb[NP][B_PER];
   auto load_a = [&](int c, float4(&xa)[A_PER]) {
@@ -967,18 +1118,10 @@ __global__ void __launch_bounds__(BM * BN / 128) __attribute__((amdgpu_waves_per
       const int e = tid + q * NT;
       if (A_UNITS % NT != 0 && e >= A_UNITS) continue;
       const int rr = e / (KC / 4), kq = (e % (KC / 4)) * 4;
-      float2v_t x01 = {xa[q].x, xa[q].y}, x23 = {xa[q].z, xa[q].w};
+      uint2 pw[NP];
+      split_planes<NP, F16>(xa[q], rsc[q], pw);
 #pragma unroll
-      for (int p = 0; p < NP; ++p) {
-        const bf16x2_t h01 = __builtin_convertvector(x01, bf16x2_t);
-        const bf16x2_t h23 = __builtin_convertvector(x23, bf16x2_t);
-        *reinterpret_cast<uint2*>(As(st, p, rr) + kq) =
-            make_uint2(__builtin_bit_cast(uint32_t, h01), __builtin_bit_cast(uint32_t, h23));
-        if (p + 1 < NP) {
-          x01 = x01 - __builtin_convertvector(h01, float2v_t);
-          x23 = x23 - __builtin_convertvector(h23, float2v_t);
-        }
-      }
+      for (int p = 0; p < NP; ++p) *reinterpret_cast<uint2*>(As(st, p, rr) + kq) = pw[p];
     }
 #pragma unroll
     for (int q = 0; q < B_PER; ++q) {
@@ -995,21 +1138,21 @@ __global__ void __launch_bounds__(BM * BN / 128) __attribute__((amdgpu_waves_per
 #pragma unroll
     for (int kk = 0; kk < KC; kk += 16) {
       const int ka = kk + 8 * (lane >> 5);
-      bf16x8_t av[NP][2];
+      OpT<F16> av[NP][2];
 #pragma unroll
       for (int p = 0; p < NP; ++p)
 #pragma unroll
-        for (int a = 0; a < 2; ++a) av[p][a] = *reinterpret_cast<const bf16x8_t*>(As(st, p, wm + a * 32 + (lane & 31)) + ka);
+        for (int a = 0; a < 2; ++a) av[p][a] = *reinterpret_cast<const OpT<F16>*>(As(st, p, wm + a * 32 + (lane & 31)) + ka);
 #pragma unroll
       for (int b = 0; b < nbc; ++b) {
-        bf16x8_t bv[NP];
+        OpT<F16> bv[NP];
 #pragma unroll
-        for (int p = 0; p < NP; ++p) bv[p] = *reinterpret_cast<const bf16x8_t*>(Bs(st, p, wn + b * 32 + (lane & 31)) + ka);
+        for (int p = 0; p < NP; ++p) bv[p] = *reinterpret_cast<const OpT<F16>*>(Bs(st, p, wn + b * 32 + (lane & 31)) + ka);
 #pragma unroll
         for (int t = 0; t < n_products(NP); ++t)
 #pragma unroll
           for (int a = 0; a < 2; ++a)
-            acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[prod_a(NP, t)][a], bv[prod_b(NP, t)], acc[a][b], 0,
+            acc[a][b] = mfma16(av[prod_a(NP, t)][a], bv[prod_b(NP, t)], acc[a][b], 0,
                                                                 0, 0);
       }
     }
@@ -1049,7 +1192,7 @@ __global__ void __launch_bounds__(BM * BN / 128) __attribute__((amdgpu_waves_per
       }
     }
   }
-  store_rows<4, FUSE>(acc, m0, n0, wm, wn, lane, C, ldc, M, N, bias, rowflag, period, ep);
+  store_rows<4, FUSE, F16>(acc, m0, n0, wm, wn, lane, C, ldc, M, N, bias, rowflag, period, ep);
 }
 
 // A-in-registers form: each wavefront computes a 64 x WN tile; it loads its A fragments
@@ -1211,6 +1354,9 @@ static const Bf16Tile kBf16Tiles[] = {{128, 128}, {128, 256}, {256, 128}, {64, 2
                                       {128, 256}, {128, 256}, {256, 128}, {128, 256}, {128, 128}, {256, 256},
                                       {128, 256}, {256, 128}};
 
+// forms that carry FP16 planes (the others are offered to BF16 planes only)
+bool gemm_bf16_f16_ok(int variant) { return variant < 3 || variant > 17; }
+
 bool gemm_bf16_fusable(int variant) { return variant == 0 || variant == 2 || variant == 18 || variant == 20 || variant == 23; }
 
 int gemm_bf16_variants() { return (int)(sizeof(kBf16Tiles) / sizeof(kBf16Tiles[0])); }
@@ -1222,26 +1368,26 @@ void gemm_bf16_tile(int variant, int* bm, int* bn) {
 
 constexpr size_t kMaxLds = 160 * 1024;  // LDS of one CU
 
-template <int NP, int BM, int BN, bool FUSE = false>
+template <int NP, int BM, int BN, bool FUSE = false, bool F16 = false>
 static void launch_tile(hipStream_t s, const float* A, int64_t lda, const uint16_t* const* W, const float* bias,
                         float* C, int64_t ldc, int64_t M, int N, int K, const uint8_t* rowflag, int period,
                         DevRows dr, ActEpi ep) {
   static bool attr_set = false;  // LDS above 64 KB must be enabled once per kernel
   const size_t lds = (size_t)NP * (BM + BN) * LDB_H * 2;
   if (lds > kMaxLds) {  // does not fit one CU: the base tile (same result) is launched instead
-    launch_tile<NP, 128, 128, FUSE>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep);
+    launch_tile<NP, 128, 128, FUSE, F16>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep);
     return;
   }
   if (!attr_set) {
     if (lds > 64 * 1024)
-      (void)hipFuncSetAttribute((const void*)k_gemm_bf16<NP, BM, BN, FUSE>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)lds);
+      (void)hipFuncSetAttribute((const void*)k_gemm_bf16<NP, BM, BN, FUSE, F16>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     attr_set = true;
   }
   const int nT = (N + BN - 1) / BN;
   const int64_t blocks = ((M + BM - 1) / BM) * nT;
-  k_gemm_bf16<NP, BM, BN, FUSE><<<(unsigned)blocks, BM * BN / 64, lds, s>>>(A, lda, W[0], W[1], W[2], bias, C, ldc, M, N, K,
-                                                                     rowflag, period, nT, dr, ep);
+  k_gemm_bf16<NP, BM, BN, FUSE, F16><<<(unsigned)blocks, BM * BN / 64, lds, s>>>(A, lda, W[0], W[1], W[2], bias, C, ldc,
+                                                                              M, N, K, rowflag, period, nT, dr, ep);
 }
 
 template <int NP, int BM, int BN, int KC, int PD = 1>
@@ -1266,47 +1412,47 @@ static void launch_tile_db(hipStream_t s, const float* A, int64_t lda, const uin
                                                                                ldc, M, N, K, rowflag, period, nT, dr, ep);
 }
 
-template <int NP, int BM, int BN, int KC, int PA, bool FUSE = false>
+template <int NP, int BM, int BN, int KC, int PA, bool FUSE = false, bool F16 = false>
 static void launch_tile_dq(hipStream_t s, const float* A, int64_t lda, const uint16_t* const* W, const float* bias,
                            float* C, int64_t ldc, int64_t M, int N, int K, const uint8_t* rowflag, int period,
                            DevRows dr, ActEpi ep) {
   static bool attr_set = false;
   const size_t lds = (size_t)2 * NP * (BM + BN) * (KC + 8) * 2;
   if (lds > kMaxLds) {
-    launch_tile<NP, 128, 128, FUSE>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep);
+    launch_tile<NP, 128, 128, FUSE, F16>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep);
     return;
   }
   if (!attr_set) {
     if (lds > 64 * 1024)
-      (void)hipFuncSetAttribute((const void*)k_gemm_bf16_dq<NP, BM, BN, KC, PA, FUSE>,
+      (void)hipFuncSetAttribute((const void*)k_gemm_bf16_dq<NP, BM, BN, KC, PA, FUSE, F16>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     attr_set = true;
   }
   const int nT = (N + BN - 1) / BN;
   const int64_t blocks = ((M + BM - 1) / BM) * nT;
-  k_gemm_bf16_dq<NP, BM, BN, KC, PA, FUSE><<<(unsigned)blocks, BM * BN / 64, lds, s>>>(A, lda, W[0], W[1], W[2], bias, C,
+  k_gemm_bf16_dq<NP, BM, BN, KC, PA, FUSE, F16><<<(unsigned)blocks, BM * BN / 64, lds, s>>>(A, lda, W[0], W[1], W[2], bias, C,
                                                                                ldc, M, N, K, rowflag, period, nT, dr, ep);
 }
 
-template <int NP, int BM, int BN, int KC, int PA, bool FUSE = false>
+template <int NP, int BM, int BN, int KC, int PA, bool FUSE = false, bool F16 = false>
 static void launch_tile_dw(hipStream_t s, const float* A, int64_t lda, const uint16_t* const* W, const float* bias,
                            float* C, int64_t ldc, int64_t M, int N, int K, const uint8_t* rowflag, int period,
                            DevRows dr, ActEpi ep) {
   static bool attr_set = false;
   const size_t lds = (size_t)2 * NP * (BM + BN) * (KC + 8) * 2;
   if (lds > kMaxLds) {
-    launch_tile<NP, 128, 128, FUSE>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep);
+    launch_tile<NP, 128, 128, FUSE, F16>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep);
     return;
   }
   if (!attr_set) {
     if (lds > 64 * 1024)
-      (void)hipFuncSetAttribute((const void*)k_gemm_bf16_dw<NP, BM, BN, KC, PA, FUSE>,
+      (void)hipFuncSetAttribute((const void*)k_gemm_bf16_dw<NP, BM, BN, KC, PA, FUSE, F16>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     attr_set = true;
   }
   const int nT = (N + BN - 1) / BN;
   const int64_t blocks = ((M + BM - 1) / BM) * nT;
-  k_gemm_bf16_dw<NP, BM, BN, KC, PA, FUSE><<<(unsigned)blocks, BM * BN / 128, lds, s>>>(A, lda, W[0], W[1], W[2], bias, C,
+  k_gemm_bf16_dw<NP, BM, BN, KC, PA, FUSE, F16><<<(unsigned)blocks, BM * BN / 128, lds, s>>>(A, lda, W[0], W[1], W[2], bias, C,
                                                                                 ldc, M, N, K, rowflag, period, nT, dr, ep);
 }
 
@@ -1320,47 +1466,64 @@ static void launch_tile_ar(hipStream_t s, const float* A, int64_t lda, const uin
                                                                               M, N, K, rowflag, period, nT, dr, ep);
 }
 
-template <int NP>
+template <int NP, bool F16 = false>
 static void launch_np(hipStream_t s, int variant, const float* A, int64_t lda, const uint16_t* const* W,
                       const float* bias, float* C, int64_t ldc, int64_t M, int N, int K, const uint8_t* rowflag,
                       int period, DevRows dr, ActEpi ep) {
   if (ep.rowgrp) {  // fused silu epilogue: the shapes of gemm_bf16_fusable
     switch (variant) {
-      case 2: launch_tile<NP, 256, 128, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
-      case 18: launch_tile_dq<NP, 128, 256, 16, 4, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
-      case 20: launch_tile_dq<NP, 256, 128, 16, 4, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
-      case 23: launch_tile_dw<NP, 256, 256, 16, 4, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
-      default: launch_tile<NP, 128, 128, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 2: launch_tile<NP, 256, 128, true, F16>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 18: launch_tile_dq<NP, 128, 256, 16, 4, true, F16>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 20: launch_tile_dq<NP, 256, 128, 16, 4, true, F16>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 23: launch_tile_dw<NP, 256, 256, 16, 4, true, F16>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      default: launch_tile<NP, 128, 128, true, F16>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
     }
     return;
   }
-  switch (variant) {
-    case 7: launch_tile_db<NP, 128, 128, 16>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
-    case 8: launch_tile_db<NP, 128, 128, 32>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
-    case 9: launch_tile_db<NP, 256, 128, 16>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
-    case 10: launch_tile_db<NP, 128, 256, 16>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
-    case 11: launch_tile_db<NP, 256, 128, 16, 2>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
-    case 12: launch_tile_db<NP, 128, 256, 16, 2>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
-    case 13: launch_tile_db<NP, 128, 128, 32, 2>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
-    case 14: launch_tile_db<NP, 128, 128, 16, 2>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
-    case 15: launch_tile_ar<NP, 2, 2, 128>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
-    case 16: launch_tile_ar<NP, 4, 1, 128>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
-    case 17: launch_tile_ar<NP, 1, 4, 64>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
-    case 18: launch_tile_dq<NP, 128, 256, 16, 4>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
-    case 19: launch_tile_dq<NP, 128, 256, 16, 6>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
-    case 20: launch_tile_dq<NP, 256, 128, 16, 4>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
-    case 21: launch_tile_dq<NP, 128, 256, 32, 3>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
-    case 22: launch_tile_dq<NP, 128, 128, 16, 6>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
-    case 23: launch_tile_dw<NP, 256, 256, 16, 4>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
-    case 24: launch_tile_dw<NP, 128, 256, 16, 4>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
-    case 25: launch_tile_dw<NP, 256, 128, 16, 4>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
-    case 1: launch_tile<NP, 128, 256>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
-    case 2: launch_tile<NP, 256, 128>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
-    case 3: launch_tile<NP, 64, 256>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
-    case 4: launch_tile<NP, 256, 64>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
-    case 5: launch_tile<NP, 64, 128>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
-    case 6: launch_tile<NP, 128, 64>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
-    default: launch_tile<NP, 128, 128>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+  if constexpr (F16) {  // the forms with FP16 planes: single stage, deep A, wide wavefront
+    switch (variant) {
+      case 18: launch_tile_dq<NP, 128, 256, 16, 4, false, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 19: launch_tile_dq<NP, 128, 256, 16, 6, false, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 20: launch_tile_dq<NP, 256, 128, 16, 4, false, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 21: launch_tile_dq<NP, 128, 256, 32, 3, false, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 22: launch_tile_dq<NP, 128, 128, 16, 6, false, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 23: launch_tile_dw<NP, 256, 256, 16, 4, false, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 24: launch_tile_dw<NP, 128, 256, 16, 4, false, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 25: launch_tile_dw<NP, 256, 128, 16, 4, false, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 1: launch_tile<NP, 128, 256, false, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 2: launch_tile<NP, 256, 128, false, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      default: launch_tile<NP, 128, 128, false, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+    }
+    return;
+  } else {
+    switch (variant) {
+      case 7: launch_tile_db<NP, 128, 128, 16>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 8: launch_tile_db<NP, 128, 128, 32>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 9: launch_tile_db<NP, 256, 128, 16>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 10: launch_tile_db<NP, 128, 256, 16>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 11: launch_tile_db<NP, 256, 128, 16, 2>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 12: launch_tile_db<NP, 128, 256, 16, 2>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 13: launch_tile_db<NP, 128, 128, 32, 2>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 14: launch_tile_db<NP, 128, 128, 16, 2>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 15: launch_tile_ar<NP, 2, 2, 128>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 16: launch_tile_ar<NP, 4, 1, 128>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 17: launch_tile_ar<NP, 1, 4, 64>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 18: launch_tile_dq<NP, 128, 256, 16, 4>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 19: launch_tile_dq<NP, 128, 256, 16, 6>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 20: launch_tile_dq<NP, 256, 128, 16, 4>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 21: launch_tile_dq<NP, 128, 256, 32, 3>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 22: launch_tile_dq<NP, 128, 128, 16, 6>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 23: launch_tile_dw<NP, 256, 256, 16, 4>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 24: launch_tile_dw<NP, 128, 256, 16, 4>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 25: launch_tile_dw<NP, 256, 128, 16, 4>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 1: launch_tile<NP, 128, 256>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 2: launch_tile<NP, 256, 128>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 3: launch_tile<NP, 64, 256>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 4: launch_tile<NP, 256, 64>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 5: launch_tile<NP, 64, 128>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 6: launch_tile<NP, 128, 64>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      default: launch_tile<NP, 128, 128>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+    }
   }
 }
 
@@ -1369,7 +1532,9 @@ void launch_gemm_bf16(hipStream_t s, const float* A, int64_t lda, const uint16_t
                       const uint8_t* rowflag, int period, int nsplit, int variant, DevRows dr, ActEpi ep) {
   if (M == 0) return;
   const uint16_t* W[3] = {Whi, Wlo ? Wlo : Whi, Wlo2 ? Wlo2 : Whi};
-  if (nsplit == 6)
+  if (nsplit == 4)  // two FP16 planes with scales (ep.amax, ep.winv), three products
+    launch_np<2, true>(s, variant, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep);
+  else if (nsplit == 6)
     launch_np<3>(s, variant, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep);
   else if (nsplit == 3)
     launch_np<2>(s, variant, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep);

@@ -28,8 +28,9 @@ from . import _lib
 from .config import FireConfig, Molecule
 from .params import init_params, pack_params, padded_w1, set_scales
 
-# "mixed": BF16 MFMA on three-plane split operands (FP32-level products, the default);
-# "mixed_fast": two planes (~2^-17 per product); "bf16": one plane; "fp32": FP32 MFMA.
+# "mixed" (the default): 16-bit matrix cores on split operands -- two scaled FP16 planes on the
+# tangent-row contractions of the local energy, three BF16 planes elsewhere (fire_precision_plan);
+# "mixed_fast": two BF16 planes (~2^-17 per product); "bf16": one plane; "fp32": FP32 MFMA.
 PRECISIONS = {"fp32": _lib.FIRE_FP32, "mixed": _lib.FIRE_MIXED, "bf16": _lib.FIRE_BF16,
               "mixed_fast": _lib.FIRE_MIXED_FAST}
 
