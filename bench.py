@@ -78,7 +78,9 @@ def main():
     ap.add_argument("--sweeps", type=int, default=50, help="Metropolis sweeps (n_el single-electron steps each)")
     ap.add_argument("--width", type=float, default=0.3, help="Gaussian step (bohr)")
     ap.add_argument("--cpu-sample", type=int, default=192, help="walkers timed on the CPU oracle (FP32)")
-    ap.add_argument("--bias-sample", type=int, default=64, help="walkers checked against the FP64 oracle")
+    ap.add_argument("--bias-sample", type=int, default=512, help="walkers checked against the FP64 oracle")
+    ap.add_argument("--config-legs", default="h2o_like,h64",
+                    help="other benchmark configurations evaluated (throughput, error against FP64); '' to skip")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--ragged", type=int, default=32, help="geometries of the ragged-batch leg (0: skip)")
     ap.add_argument("--ragged-sweeps", type=int, default=2, help="timed Metropolis sweeps of the ragged leg")
@@ -87,6 +89,9 @@ def main():
     ap.add_argument("--serve-nel", default="20,400", help="electron-count range of the serving requests")
     ap.add_argument("--serve-walkers", default="128,1024", help="walker range of the serving requests")
     ap.add_argument("--serve-sweeps", default="1,2", help="sweep range of the serving requests")
+    ap.add_argument("--plan", default=None,
+                    help="operand planes per contraction class of the mixed precision, comma-separated "
+                         "(ElecInit MLP, update MLP, orbitals, MLP Jastrow, attention Q/V, readout)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device copy of this many MiB (calibrates HBM byte counters) and exit")
     args = ap.parse_args()
@@ -122,6 +127,7 @@ def main():
     params = wf.init(0, r0[0])
     P.exercise_jastrow(params)
     wf.upload(params)
+    plan = wf.precision_plan([int(x) for x in args.plan.split(",")] if args.plan else None)
 
     # 1. equilibrate with the GPU sampler (mixed precision state)
     mcmc = None
@@ -257,6 +263,7 @@ def main():
                                f"D={cfg.n_determinants}", "global_batch": B, "parallelism": "none (1 GPU)"},
         "fp32_walkers_per_s": wps_f,
         "mixed_over_fp32": wps_m / wps_f,
+        "mixed_plan": plan,
         "mixed_fast_walkers_per_s": wps_x,
         "latency_ms": {"p50": 1e3 * float(np.percentile(lat_m, 50)), "p99": 1e3 * float(np.percentile(lat_m, 99)),
                        "what": "one local-energy call of the whole batch (mixed)"},
@@ -322,34 +329,87 @@ def main():
         result["latency_ms"]["serving_p99"] = 1e3 * serving["mixed"]["p99_s"]
 
     if not args.no_cpu:
-        # CPU baseline leg: the oracle (FP32) timed on a bounded sample of the same walkers,
-        # and the FP64 oracle as the reference for the energy error of the GPU paths.
-        from oracle import fire
+        # CPU baseline leg: the oracle (FP32) timed on a bounded sample of the same walkers over a
+        # pool of single-threaded processes (one per host core of the share, at most 16), and the
+        # FP64 oracle the same way as the reference for the energy error of the GPU paths.
+        from oracle import pool
 
-        torch.set_num_threads(len(os.sched_getaffinity(0)))
+        procs = pool.default_procs()
         r_np = r.cpu().numpy()
-        log(f"CPU baseline on {args.cpu_sample} walkers")
-        p32 = fire.to_torch(params, torch.float32)
-        fire.local_energy(p32, cfg, r_np[0])
-        t0 = time.perf_counter()
-        for w in range(args.cpu_sample):
-            fire.local_energy(p32, cfg, r_np[w])
-        dt_cpu = time.perf_counter() - t0
+        log(f"CPU baseline on {args.cpu_sample} walkers ({procs} processes)")
+        with pool.OraclePool(params, cfg, "float32", procs) as p32:
+            p32.warm()
+            t0 = time.perf_counter()
+            p32(r_np[:args.cpu_sample])
+            dt_cpu = time.perf_counter() - t0
         result["cpu_baseline"] = {"value": args.cpu_sample / dt_cpu, "unit": "walkers/s",
-                                  "cores": torch.get_num_threads(), "kind": "port",
-                                  "sample": f"{args.cpu_sample} walkers of the same batch, oracle in FP32 "
-                                            f"(torch CPU, sparse forward Laplacian), {dt_cpu:.1f} s"}
+                                  "cores": procs, "kind": "port",
+                                  "sample": f"{args.cpu_sample} walkers of the same batch, oracle in FP32 (torch CPU, "
+                                            f"sparse forward Laplacian), {procs} single-threaded processes, "
+                                            f"{dt_cpu:.1f} s (process start-up excluded)"}
         if args.bias_sample > 0:
-            log(f"FP64 oracle on {args.bias_sample} walkers")
-            p64 = fire.to_torch(params, torch.float64)
-            ref = np.array([fire.local_energy(p64, cfg, r_np[w])[0] for w in range(args.bias_sample)])
-            n = args.bias_sample
+            n = min(args.bias_sample, B)
+            log(f"FP64 oracle on {n} walkers")
+            ref, _, _ = pool.local_energies(params, cfg, r_np[:n], procs=procs)
             result["energy_error_vs_fp64_mHa"] = {
                 "walkers": n, "mixed": err_stats(e_m[:n] - ref), "fp32": err_stats(e_f[:n] - ref),
                 "mixed_fast": err_stats(e_x[:n] - ref),
-                "gate": "|bias| <= 1 mHa",
+                "gate": "|bias| <= 1 mHa; bias_se = standard error of the bias",
             }
+            for k in ("mixed", "fp32", "mixed_fast"):
+                e = {"mixed": e_m, "fp32": e_f, "mixed_fast": e_x}[k][:n]
+                result["energy_error_vs_fp64_mHa"][k]["bias_se"] = float(np.std((e - ref) * 1e3, ddof=1) / np.sqrt(n))
+        legs = [x for x in args.config_legs.split(",") if x]
+        if legs:
+            result["configs"] = {mol.name or args.workload: {
+                "walkers": B, "mixed_walkers_per_s": wps_m, "fp32_walkers_per_s": wps_f,
+                "energy_error_vs_fp64_mHa": result.get("energy_error_vs_fp64_mHa")}}
+            for name in legs:
+                result["configs"][name] = config_leg(name, procs, args)
     print(json.dumps(result))
+
+
+def config_leg(name, procs, args):
+    """Another benchmark configuration at its own size: walkers equilibrated by 5 sweeps of the
+    GPU sampler, mixed and FP32 local energy of the whole batch (timed), and the error of both
+    against the FP64 oracle on a sample of 256 walkers."""
+    import torch
+
+    from oracle import pool
+    from sparse_wf_amd import params as P
+    from sparse_wf_amd import systems
+    from sparse_wf_amd.config import FireConfig
+    from sparse_wf_amd.wave_function import FireWaveFunction
+
+    mol, over, n_walk, sigma = systems.workload(name)
+    cfg = FireConfig(mol=mol, **over)
+    log(f"config leg {name}: N={cfg.n_el}, {n_walk} walkers")
+    r0 = systems.init_electrons(mol, n_walk, stddev=sigma, seed=0)
+    wf = FireWaveFunction(cfg, device=0)
+    params = wf.init(0, r0[0])
+    P.exercise_jastrow(params)
+    r, _ = wf.mcmc_steps_low_rank(params, r0, 5 * cfg.n_el, width=args.width, seed=1, precision="mixed")
+    out = {"workload": f"{mol.name} N={cfg.n_el} walkers={n_walk} cutoff={cfg.cutoff} F={cfg.feature_dim} "
+                       f"D={cfg.n_determinants}", "walkers": n_walk}
+    e = {}
+    for prec in ("mixed", "fp32"):
+        wf.local_energy_terms(params, r, precision=prec)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(3):
+            e[prec] = wf.local_energy_terms(params, r, precision=prec)["e_loc"]
+        torch.cuda.synchronize()
+        out[f"{prec}_walkers_per_s"] = 3 * n_walk / (time.perf_counter() - t0)
+        e[prec] = e[prec].cpu().numpy()
+    n = min(256, n_walk)
+    ref, _, _ = pool.local_energies(params, cfg, r[:n].cpu().numpy(), procs=procs)
+    out["energy_error_vs_fp64_mHa"] = {"walkers": n}
+    for prec in ("mixed", "fp32"):
+        d = e[prec][:n] - ref
+        st = err_stats(d)
+        st["bias_se"] = float(np.std(d * 1e3, ddof=1) / np.sqrt(n))
+        out["energy_error_vs_fp64_mHa"][prec] = st
+    return out
 
 
 if __name__ == "__main__":

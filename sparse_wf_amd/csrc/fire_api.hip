@@ -232,8 +232,10 @@ struct fire_handle {
   // raises its rms from 0.4 to 3-4 mHa)
   // 4 = two FP16 planes of operands scaled by powers of two per row of A and column of W
   // (three products, about 2^-22 relative per product); used where the producer of A left
-  // its row maxima (tangent rows of the local energy), three BF16 planes elsewhere.
-  int planes[6] = {3, 4, 4, 4, 4, 2};
+  // its row maxima (tangent rows of the local energy), three BF16 planes elsewhere.  The
+  // Jastrow classes stay on a BF16 pair: on C16H34 the FP16 planes there leave the error
+  // against FP64 unchanged and are slower (N = 512 joint contraction).
+  int planes[6] = {3, 4, 4, 2, 2, 2};
   std::map<std::tuple<int, int, int, int>, int> gemm_tile;
   std::map<std::tuple<int, int, int, int>, std::vector<float>> gemm_tile_ms;
   // Resident walker state for single-electron low-rank updates (fire_state_*, fire_mcmc_steps).
@@ -872,8 +874,9 @@ void run_jastrow_and_assemble(Ctx& c, const float* r, int B, bool tan, const Gro
     if (joint) {
       const int NJ = W + h->HD;
       ActEpi e2;
+      // the second Jastrow layer reads FP16 planes of XC only with class 3 on them
       e2.amax = gemm_silu_tan(c, h->XA.as<float>(), F, h->iJV, h->iJV + 1, h->XC.as<float>(), NJ, R, NJ, F, flag, G,
-                              gd, j + 0, j + 1, W, h->amx_xa, &h->AMX_C);
+                              gd, j + 0, j + 1, W, h->amx_xa, h->planes[3] == 4 ? &h->AMX_C : nullptr);
       gemm(c, h->XC.as<float>(), NJ, j + 2, j + 3, h->XB.as<float>(), W, R, W, W, flag, period, DevRows(), &e2);
     } else {
       // the silu of the last hidden layer is applied inside k_jmlp_last, the others in the

@@ -379,6 +379,16 @@ class FireWaveFunction:
             raise ValueError("no resident state (call log_psi_with_state first)")
         return nw.value
 
+    def precision_plan(self, planes=None):
+        """Operand planes per contraction class of the "mixed" precision (ElecInit MLP, update MLP,
+        orbitals, MLP Jastrow, attention Q/V, attention readout; 1-3 BF16 planes, 4 two scaled
+        FP16 planes): sets the leading classes from ``planes`` (if given) and returns the plan."""
+        planes = [] if planes is None else [int(p) for p in planes]
+        arr = (ctypes.c_int32 * 6)(*(planes + [0] * (6 - len(planes))))
+        cur = (ctypes.c_int32 * 6)()
+        _lib.check(self._lib.fire_precision_plan(self._h, arr, len(planes), cur))
+        return list(cur)
+
     def mcmc_options(self, use_graphs: bool = True):
         """Replay whole sweeps from a captured hipGraph (default) or issue every step eagerly;
         both give identical results."""
