@@ -84,11 +84,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--ragged", type=int, default=32, help="geometries of the ragged-batch leg (0: skip)")
     ap.add_argument("--ragged-sweeps", type=int, default=2, help="timed Metropolis sweeps of the ragged leg")
-    ap.add_argument("--serve-requests", type=int, default=16, help="requests of the serving leg (0: skip)")
+    ap.add_argument("--serve-requests", type=int, default=8, help="requests of the serving leg (0: skip)")
     ap.add_argument("--serve-qps", type=float, default=4.0, help="Poisson arrival rate of the serving leg")
     ap.add_argument("--serve-nel", default="20,400", help="electron-count range of the serving requests")
-    ap.add_argument("--serve-walkers", default="128,1024", help="walker range of the serving requests")
-    ap.add_argument("--serve-sweeps", default="1,2", help="sweep range of the serving requests")
+    ap.add_argument("--serve-walkers", default="128,512", help="walker range of the serving requests")
+    ap.add_argument("--serve-sweeps", default="10,12", help="sweep range of the serving requests")
     ap.add_argument("--plan", default=None,
                     help="operand planes per contraction class of the mixed precision, comma-separated "
                          "(ElecInit MLP, update MLP, orbitals, MLP Jastrow, attention Q/V, readout)")
@@ -311,7 +311,11 @@ def main():
         rng2 = lambda s: tuple(int(x) for x in s.split(","))
         serving = {"workload": f"{args.serve_requests} requests, Poisson {args.serve_qps}/s, random H/C chains "
                                f"N={args.serve_nel}, walkers={args.serve_walkers}, sweeps={args.serve_sweeps}, "
-                               "F=256, D=4, cutoff 5; result = local energy after the sweeps"}
+                               "F=256, D=4, cutoff 5; result = local energy after the sweeps; one host thread "
+                               "dispatching sweep-graph replays over a pool of reusable handles; walker and sweep "
+                               "ranges scaled down from 128-4096 walkers and 10-100 sweeps so the leg stays within a "
+                               "minute (a 4096-walker, 400-electron, 100-sweep request alone is 164 M walker-moves, "
+                               "about 7 minutes at the sampler's rate)"}
         for prec in ("mixed", "fp32"):
             log(f"serving leg ({prec}): preparing {args.serve_requests} requests")
             reqs = poisson_requests(args.serve_requests, args.serve_qps, seed=5, n_el=rng2(args.serve_nel),

@@ -60,6 +60,18 @@ enum fire_precision {
 int fire_create(const fire_config* cfg, const float* R, const float* Z, int device, fire_handle** out);
 int fire_destroy(fire_handle* h);
 
+/* Gives the handle a new molecule and model (as fire_create) while keeping its device
+ * workspace, which only grows: a serving loop reuses a pool of handles across requests
+ * without device allocations for buffers that are already large enough.  Waits for the
+ * device; the parameters and the walker state belong to the old model and are dropped
+ * (fire_upload_params is needed again).  No counterpart in the reference (a serving-side
+ * extension of MoonLikeWaveFunction.create, wave_function.py:102-150). */
+int fire_reconfigure(fire_handle* h, const fire_config* cfg, const float* R, const float* Z);
+
+/* Device bytes held by the handle: workspace, walker state and parameters (host output).
+ * No counterpart in the reference (admission by HBM footprint in the serving loop). */
+int fire_workspace_bytes(fire_handle* h, int64_t* bytes);
+
 /* Uploads parameters from host memory: `n_tensors` float32 tensors back to back in the
  * canonical order listed in sparse_wf_amd/params.py (pack_params), `sizes[k]` = element
  * count of tensor k; sizes are validated against the config.  The resident walker state

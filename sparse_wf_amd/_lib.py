@@ -35,6 +35,8 @@ _P = ctypes.c_void_p
 SIGNATURES = {
     "fire_create": (ctypes.c_int, [ctypes.POINTER(FireConfigC), _P, _P, ctypes.c_int, ctypes.POINTER(_P)]),
     "fire_destroy": (ctypes.c_int, [_P]),
+    "fire_reconfigure": (ctypes.c_int, [_P, ctypes.POINTER(FireConfigC), _P, _P]),
+    "fire_workspace_bytes": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int64)]),
     "fire_upload_params": (ctypes.c_int, [_P, _P, _P, ctypes.c_int32]),
     "fire_neighbours": (ctypes.c_int, [_P, _P, _P, ctypes.c_int32, _P, _P, ctypes.c_int64,
                                        ctypes.POINTER(ctypes.c_int64)]),

@@ -1391,6 +1391,94 @@ void check_ready(fire_handle* h) {
   REQUIRE(h->params_ok, "parameters not uploaded (call fire_upload_params first)");
 }
 
+void check_config(const fire_config& c) {
+  REQUIRE(c.n_el > 0 && c.n_up >= 0 && c.n_up <= c.n_el && c.n_nuc > 0, "invalid electron / nucleus counts");
+  REQUIRE(c.feature_dim % 64 == 0 && c.feature_dim <= 256,
+          "feature_dim must be a multiple of 64, at most 256 (one feature column per thread of a 256-thread "
+          "workgroup in the embedding kernels)");
+  REQUIRE(!c.use_attention || (c.att_dim <= 64 && (c.att_dim & (c.att_dim - 1)) == 0 && c.att_heads * c.att_dim <= 256),
+          "attention_dim must be a power of two <= 64 and attention_heads * attention_dim <= 256");
+  REQUIRE(c.filter_w0 > 0 && c.filter_w0 <= FIRE_W0MAX && (c.filter_w1 == 4 || c.filter_w1 == 8 || c.filter_w1 == 16),
+          "pair_mlp_widths out of range (w0 <= 32; w1 zero-padded to 4, 8 or 16)");
+  REQUIRE(c.n_env_en > 0 && c.n_env_en <= FIRE_EEMAX, "pair_n_envelopes out of range (<= 64)");
+  REQUIRE(c.n_det > 0 && c.n_det <= 16 && c.n_el * c.n_det <= 2048,
+          "n_determinants must be <= 16 and n_el * n_determinants <= 2048");
+  REQUIRE(c.n_det * c.n_env_iso <= 1600, "n_determinants * n_envelopes must be <= 1600");
+  REQUIRE(c.e_e_cusp == 0 || c.e_e_cusp == 1, "e_e_cusp must be 0 (none) or 1 (psiformer)");
+  REQUIRE(c.mlp_depth >= 1 && c.mlp_width > 0, "mlp_depth >= 1 and mlp_width > 0 required");
+}
+
+// Every workspace buffer of the handle outside the walker state.
+std::vector<DevBuf*> handle_bufs(fire_handle* h) {
+  return {&h->cnt,    &h->ptr,     &h->nbr,    &h->rev,     &h->rowflag, &h->T5,     &h->Z2,      &h->T5b,
+          &h->M5,     &h->XA,      &h->XB,     &h->XC,      &h->O,       &h->ENV,    &h->ENVC,    &h->A64,
+          &h->Ainv,   &h->AinvT,   &h->ldet,   &h->dsgn,    &h->tr,      &h->trip,   &h->dgrad,   &h->det_lp,
+          &h->det_sg, &h->det_gr,  &h->det_lap, &h->JR,     &h->mlp_ns,  &h->mlp_dg, &h->QV,      &h->AV,
+          &h->att_ns, &h->att_dg,  &h->RIN,    &h->RY1,     &h->RY2,     &h->att_out, &h->cval,   &h->clap,
+          &h->cgrad,  &h->epot,    &h->PF,     &h->EF,      &h->EOK,     &h->rowgrp, &h->AV0,     &h->Z0,
+          &h->F1,     &h->PART,    &h->AMX_A,  &h->AMX_B,   &h->AMX_C};
+}
+
+void free_params(fire_handle* h) {
+  for (auto* v : {&h->Pbf, &h->Pbf_lo, &h->Pbf_lo2, &h->Pf16, &h->Pf16_lo})
+    for (uint16_t* p : *v)
+      if (p) (void)hipFree(p);
+  for (auto* v : {&h->P, &h->Pt32, &h->Pwinv})
+    for (float* p : *v)
+      if (p) (void)hipFree(p);
+  for (auto* v : {&h->Pbf, &h->Pbf_lo, &h->Pbf_lo2, &h->Pf16, &h->Pf16_lo}) v->clear();
+  for (auto* v : {&h->P, &h->Pt32, &h->Pwinv}) v->clear();
+  h->params_ok = false;
+}
+
+// Molecule and model of the handle (fire_create, fire_reconfigure): the workspace buffers are
+// kept (they grow on demand), parameters and the walker state belong to the old model and go.
+void apply_config(fire_handle* h, const fire_config& c, const float* R, const float* Z) {
+  free_params(h);
+  if (h->st.graph) (void)hipGraphExecDestroy(h->st.graph);
+  h->st.graph = nullptr;
+  h->st.ready = h->st.pending = false;
+  h->cfg = c;
+  h->N = c.n_el;
+  h->n_up = c.n_up;
+  h->n_dn = c.n_el - c.n_up;
+  h->M = c.n_nuc;
+  h->F = c.feature_dim;
+  h->w0 = c.filter_w0;
+  h->w1 = c.filter_w1;
+  h->Een = c.n_env_en;
+  h->D = c.n_det;
+  h->Eiso = c.n_env_iso;
+  h->ND = c.n_el * c.n_det;
+  h->W = c.mlp_width;
+  h->H = c.att_heads;
+  h->Dk = c.att_dim;
+  h->HD = c.att_heads * c.att_dim;
+  h->HDP = h->HD + (c.att_heads + 15) / 16 * 16;
+  h->depth = c.mlp_depth;
+  h->has_mlp = c.use_mlp_jastrow || c.use_log_jastrow;
+  h->has_att = c.use_attention != 0;
+  h->cusp = c.e_e_cusp;
+  h->iAV = h->iORBA = h->iJV = -1;
+  h->amx_xa = nullptr;
+  double enn = 0.0;
+  for (int I = 0; I < c.n_nuc; ++I)
+    for (int J = I + 1; J < c.n_nuc; ++J) {
+      const double dx = (double)R[3 * I] - R[3 * J], dy = (double)R[3 * I + 1] - R[3 * J + 1],
+                   dz = (double)R[3 * I + 2] - R[3 * J + 2];
+      enn += (double)Z[I] * Z[J] / std::sqrt(dx * dx + dy * dy + dz * dz);
+    }
+  h->e_nn = enn;
+  if (h->R) HIPCHK(hipFree(h->R));
+  if (h->Z) HIPCHK(hipFree(h->Z));
+  h->R = h->Z = nullptr;
+  HIPCHK(hipMalloc(&h->R, sizeof(float) * 3 * c.n_nuc));
+  HIPCHK(hipMalloc(&h->Z, sizeof(float) * c.n_nuc));
+  HIPCHK(hipMemcpy(h->R, R, sizeof(float) * 3 * c.n_nuc, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(h->Z, Z, sizeof(float) * c.n_nuc, hipMemcpyHostToDevice));
+  build_expect(h);
+}
+
 }  // namespace
 
 extern "C" {
@@ -1400,64 +1488,47 @@ const char* fire_last_error(void) { return g_last_error.c_str(); }
 int fire_create(const fire_config* cfg, const float* R, const float* Z, int device, fire_handle** out) {
   try {
     REQUIRE(cfg && R && Z && out, "null argument");
-    const fire_config& c = *cfg;
-    REQUIRE(c.n_el > 0 && c.n_up >= 0 && c.n_up <= c.n_el && c.n_nuc > 0, "invalid electron / nucleus counts");
-    REQUIRE(c.feature_dim % 64 == 0 && c.feature_dim <= 256,
-            "feature_dim must be a multiple of 64, at most 256 (one feature column per thread of a 256-thread "
-            "workgroup in the embedding kernels)");
-    REQUIRE(!c.use_attention || (c.att_dim <= 64 && (c.att_dim & (c.att_dim - 1)) == 0 && c.att_heads * c.att_dim <= 256),
-            "attention_dim must be a power of two <= 64 and attention_heads * attention_dim <= 256");
-    REQUIRE(c.filter_w0 > 0 && c.filter_w0 <= FIRE_W0MAX &&
-                (c.filter_w1 == 4 || c.filter_w1 == 8 || c.filter_w1 == 16),
-            "pair_mlp_widths out of range (w0 <= 32; w1 zero-padded to 4, 8 or 16)");
-    REQUIRE(c.n_env_en > 0 && c.n_env_en <= FIRE_EEMAX, "pair_n_envelopes out of range (<= 64)");
-    REQUIRE(c.n_det > 0 && c.n_det <= 16 && c.n_el * c.n_det <= 2048,
-            "n_determinants must be <= 16 and n_el * n_determinants <= 2048");
-    REQUIRE(c.n_det * c.n_env_iso <= 1600, "n_determinants * n_envelopes must be <= 1600");
-    REQUIRE(c.e_e_cusp == 0 || c.e_e_cusp == 1, "e_e_cusp must be 0 (none) or 1 (psiformer)");
-
-    REQUIRE(c.mlp_depth >= 1 && c.mlp_width > 0, "mlp_depth >= 1 and mlp_width > 0 required");
+    check_config(*cfg);
     HIPCHK(hipSetDevice(device));
     int64_t* scr = scratch_address(device);
     fire_handle* h = new fire_handle();
     h->scratch = scr;
-    h->cfg = c;
     if (const char* e = getenv("FIRE_FUSED_ACT")) h->fuse_act = e[0] != '0';
     h->device = device;
-    h->N = c.n_el;
-    h->n_up = c.n_up;
-    h->n_dn = c.n_el - c.n_up;
-    h->M = c.n_nuc;
-    h->F = c.feature_dim;
-    h->w0 = c.filter_w0;
-    h->w1 = c.filter_w1;
-    h->Een = c.n_env_en;
-    h->D = c.n_det;
-    h->Eiso = c.n_env_iso;
-    h->ND = c.n_el * c.n_det;
-    h->W = c.mlp_width;
-    h->H = c.att_heads;
-    h->Dk = c.att_dim;
-    h->HD = c.att_heads * c.att_dim;
-    h->HDP = h->HD + (c.att_heads + 15) / 16 * 16;
-    h->depth = c.mlp_depth;
-    h->has_mlp = c.use_mlp_jastrow || c.use_log_jastrow;
-    h->has_att = c.use_attention != 0;
-    h->cusp = c.e_e_cusp;
-    double enn = 0.0;
-    for (int I = 0; I < c.n_nuc; ++I)
-      for (int J = I + 1; J < c.n_nuc; ++J) {
-        const double dx = (double)R[3 * I] - R[3 * J], dy = (double)R[3 * I + 1] - R[3 * J + 1],
-                     dz = (double)R[3 * I + 2] - R[3 * J + 2];
-        enn += (double)Z[I] * Z[J] / std::sqrt(dx * dx + dy * dy + dz * dz);
-      }
-    h->e_nn = enn;
-    HIPCHK(hipMalloc(&h->R, sizeof(float) * 3 * c.n_nuc));
-    HIPCHK(hipMalloc(&h->Z, sizeof(float) * c.n_nuc));
-    HIPCHK(hipMemcpy(h->R, R, sizeof(float) * 3 * c.n_nuc, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(h->Z, Z, sizeof(float) * c.n_nuc, hipMemcpyHostToDevice));
-    build_expect(h);
+    try {
+      apply_config(h, *cfg, R, Z);
+    } catch (...) {
+      delete h;
+      throw;
+    }
     *out = h;
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e);
+  }
+}
+
+int fire_reconfigure(fire_handle* h, const fire_config* cfg, const float* R, const float* Z) {
+  try {
+    REQUIRE(h && cfg && R && Z, "null argument");
+    check_config(*cfg);
+    HIPCHK(hipSetDevice(h->device));
+    HIPCHK(hipDeviceSynchronize());  // work still queued on the old molecule must finish first
+    apply_config(h, *cfg, R, Z);
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e);
+  }
+}
+
+int fire_workspace_bytes(fire_handle* h, int64_t* bytes) {
+  try {
+    REQUIRE(h && bytes, "null argument");
+    int64_t n = 0;
+    for (DevBuf* b : handle_bufs(h)) n += (int64_t)b->bytes;
+    for (DevBuf* b : h->st.bufs()) n += (int64_t)b->bytes;
+    for (size_t k = 0; k < h->P.size(); ++k) n += 4 * (k < h->expect.size() ? h->expect[k] : 0);
+    *bytes = n;
     return 0;
   } catch (const std::exception& e) {
     return fail(e);
@@ -1467,31 +1538,10 @@ int fire_create(const fire_config* cfg, const float* R, const float* Z, int devi
 int fire_destroy(fire_handle* h) {
   if (!h) return 0;
   (void)hipSetDevice(h->device);
-  for (float* p : h->P)
-    if (p) (void)hipFree(p);
-  for (uint16_t* p : h->Pbf)
-    if (p) (void)hipFree(p);
-  for (uint16_t* p : h->Pbf_lo)
-    if (p) (void)hipFree(p);
-  for (uint16_t* p : h->Pbf_lo2)
-    if (p) (void)hipFree(p);
-  for (float* p : h->Pt32)
-    if (p) (void)hipFree(p);
-  for (uint16_t* p : h->Pf16)
-    if (p) (void)hipFree(p);
-  for (uint16_t* p : h->Pf16_lo)
-    if (p) (void)hipFree(p);
-  for (float* p : h->Pwinv)
-    if (p) (void)hipFree(p);
+  free_params(h);
   if (h->R) (void)hipFree(h->R);
   if (h->Z) (void)hipFree(h->Z);
-  DevBuf* bufs[] = {&h->cnt, &h->ptr, &h->nbr, &h->rev, &h->rowflag, &h->T5, &h->Z2, &h->T5b, &h->M5,
-                    &h->XA, &h->XB, &h->XC, &h->O, &h->ENV, &h->ENVC, &h->A64, &h->Ainv, &h->AinvT, &h->ldet, &h->dsgn,
-                    &h->tr, &h->trip, &h->dgrad, &h->det_lp, &h->det_sg, &h->det_gr, &h->det_lap, &h->JR,
-                    &h->mlp_ns, &h->mlp_dg, &h->QV, &h->AV, &h->att_ns, &h->att_dg, &h->RIN, &h->RY1,
-                    &h->RY2, &h->att_out, &h->cval, &h->clap, &h->cgrad, &h->epot, &h->PF, &h->EF, &h->EOK,
-                    &h->rowgrp, &h->AV0, &h->Z0, &h->F1, &h->PART, &h->AMX_A, &h->AMX_B, &h->AMX_C};
-  for (DevBuf* b : bufs) b->release();
+  for (DevBuf* b : handle_bufs(h)) b->release();
   h->st.release();
   delete h;
   return 0;
@@ -1506,22 +1556,7 @@ int fire_upload_params(fire_handle* h, const float* packed, const int64_t* sizes
     for (int k = 0; k < n_tensors; ++k)
       REQUIRE(sizes[k] == h->expect[k], "parameter tensor " + std::to_string(k) + " has " + std::to_string(sizes[k]) +
                                             " elements, expected " + std::to_string(h->expect[k]));
-    for (float* p : h->P)
-      if (p) HIPCHK(hipFree(p));
-    for (uint16_t* p : h->Pbf)
-      if (p) HIPCHK(hipFree(p));
-    for (uint16_t* p : h->Pbf_lo)
-      if (p) HIPCHK(hipFree(p));
-    for (uint16_t* p : h->Pbf_lo2)
-      if (p) HIPCHK(hipFree(p));
-    for (float* p : h->Pt32)
-      if (p) HIPCHK(hipFree(p));
-    for (uint16_t* p : h->Pf16)
-      if (p) HIPCHK(hipFree(p));
-    for (uint16_t* p : h->Pf16_lo)
-      if (p) HIPCHK(hipFree(p));
-    for (float* p : h->Pwinv)
-      if (p) HIPCHK(hipFree(p));
+    free_params(h);
     const int n_all = n_tensors + (h->has_att ? (h->has_mlp ? 8 : 6) : 0);
     h->P.assign(n_all, nullptr);
     h->Pbf.assign(n_all, nullptr);

@@ -49,6 +49,21 @@ class StaticArgs(NamedTuple):
     n_pp_elecs: int
 
 
+def _config_c(cfg: FireConfig):
+    """(fire_config, R float32 [M, 3], Z float32 [M]) of a configuration."""
+    c = _lib.FireConfigC(
+        n_el=cfg.n_el, n_up=cfg.n_up, n_nuc=cfg.n_nuc, feature_dim=cfg.feature_dim,
+        filter_w0=cfg.pair_mlp_widths[0], filter_w1=padded_w1(cfg.pair_mlp_widths[1]),
+        n_env_en=cfg.pair_n_envelopes, n_det=cfg.n_determinants, n_env_iso=cfg.n_envelopes,
+        e_e_cusp=1 if cfg.e_e_cusps == "psiformer" else 0,
+        use_mlp_jastrow=int(cfg.use_mlp_jastrow), use_log_jastrow=int(cfg.use_log_jastrow),
+        use_attention=int(cfg.use_attention), mlp_width=cfg.mlp_width, mlp_depth=cfg.mlp_depth,
+        att_heads=cfg.attention_heads, att_dim=cfg.attention_dim,
+        cutoff=cfg.cutoff, cutoff_1el=cfg.cutoff_1el,
+    )
+    return c, np.ascontiguousarray(cfg.mol.R, np.float32), np.ascontiguousarray(cfg.mol.Z, np.float32)
+
+
 def _stream_ptr() -> int:
     return torch.cuda.current_stream().cuda_stream
 
@@ -65,18 +80,7 @@ class FireWaveFunction:
         self.device = torch.device("cuda", device)
         self.precision = precision
         lib = _lib.load()
-        c = _lib.FireConfigC(
-            n_el=cfg.n_el, n_up=cfg.n_up, n_nuc=cfg.n_nuc, feature_dim=cfg.feature_dim,
-            filter_w0=cfg.pair_mlp_widths[0], filter_w1=padded_w1(cfg.pair_mlp_widths[1]),
-            n_env_en=cfg.pair_n_envelopes, n_det=cfg.n_determinants, n_env_iso=cfg.n_envelopes,
-            e_e_cusp=1 if cfg.e_e_cusps == "psiformer" else 0,
-            use_mlp_jastrow=int(cfg.use_mlp_jastrow), use_log_jastrow=int(cfg.use_log_jastrow),
-            use_attention=int(cfg.use_attention), mlp_width=cfg.mlp_width, mlp_depth=cfg.mlp_depth,
-            att_heads=cfg.attention_heads, att_dim=cfg.attention_dim,
-            cutoff=cfg.cutoff, cutoff_1el=cfg.cutoff_1el,
-        )
-        R = np.ascontiguousarray(cfg.mol.R, np.float32)
-        Z = np.ascontiguousarray(cfg.mol.Z, np.float32)
+        c, R, Z = _config_c(cfg)
         h = ctypes.c_void_p()
         _lib.check(lib.fire_create(ctypes.byref(c), R.ctypes.data, Z.ctypes.data, device, ctypes.byref(h)))
         self._h = h
@@ -85,6 +89,21 @@ class FireWaveFunction:
         self._uploaded_fp = None
         self._gen = 0
         self._committed_gen = self._pending_gen = None
+
+    def reconfigure(self, cfg: FireConfig):
+        """Switch this wave function (and its GPU handle) to another molecule / model, keeping the
+        device workspace; parameters must be uploaded again and the resident state is gone."""
+        c, R, Z = _config_c(cfg)
+        _lib.check(self._lib.fire_reconfigure(self._h, ctypes.byref(c), R.ctypes.data, Z.ctypes.data))
+        self.cfg = cfg
+        self._uploaded = self._uploaded_fp = None
+        self._committed_gen = self._pending_gen = None
+
+    def workspace_bytes(self) -> int:
+        """Device bytes held by the handle (workspace, walker state, parameters)."""
+        n = ctypes.c_int64()
+        _lib.check(self._lib.fire_workspace_bytes(self._h, ctypes.byref(n)))
+        return n.value
 
     # reference-style constructor ---------------------------------------------------------
     @classmethod

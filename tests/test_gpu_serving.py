@@ -1,7 +1,8 @@
 """Continuous-batching server: every request's result is the same alone and in the mix
 (bit-identical: kernels work per walker, random numbers are keyed by (seed, walker, step),
-the autotuner is off in the server), concurrent requests overlap on the device, and
-admission respects the HBM budget."""
+tile shapes are chosen once per contraction shape and process), concurrent requests overlap
+on the device, the handle pool is reused across requests, and admission respects the HBM
+budget."""
 from __future__ import annotations
 
 import dataclasses
@@ -38,6 +39,8 @@ def test_same_alone_and_in_mix(requests):
         np.testing.assert_array_equal(m.e_loc, a.e_loc)
         assert np.isfinite(m.e_mean)
     assert srv.max_concurrent >= 2
+    # the pool's handles are retargeted between requests, not created per request
+    assert srv.handles_created <= 4 < len(requests) + 1
     # execution intervals of at least two requests overlapped
     iv = sorted((r.started, r.finished) for r in mix.values())
     assert any(iv[k + 1][0] < iv[k][1] for k in range(len(iv) - 1))
