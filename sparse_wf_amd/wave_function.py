@@ -181,6 +181,16 @@ class FireWaveFunction:
         self.upload(params)
         return params
 
+    def load_checkpoint(self, src) -> dict:
+        """Read a reference training checkpoint (``chkpt*.msgpk``, path or bytes; written by
+        ``TrainingState.serialize`` api.py:294-303) and upload its parameters.  Returns
+        ``{"params", "electrons", "step", "key", "width"}``."""
+        from .checkpoint import load_training_state
+
+        st = load_training_state(src, self.cfg)
+        self.upload(st["params"])
+        return st
+
     # inputs ----------------------------------------------------------------------------
     def _walkers(self, electrons):
         r = torch.as_tensor(electrons, dtype=torch.float32, device=self.device)
