@@ -217,6 +217,21 @@ int fire_autotune(fire_handle* h, int32_t enable, int32_t* table, int32_t cap, i
  * launch) and the launch count -- then resets and applies `enable`.  Synchronises. */
 int fire_profile_gemm(fire_handle* h, int32_t enable, double* total_ms, double* total_flops, int64_t* n_launches);
 
+/* Statistics and clipping of a batch of local energies e_loc_dev (double [B], B > 0) on the
+ * handle's device (h may be NULL: the calling thread's current device), one workgroup:
+ * stats_dev (double [5]) = E_mean, E_std (population), clip centre, mean absolute deviation
+ * from it, mean of the clipped energies.  With clip_local_energy > 0 the energies are clipped
+ * to centre +- clip_local_energy * deviation, the centre being the median (stat = 0, the
+ * midpoint of the elements of rank (B-1)/2 and B/2, selected exactly) or the mean (stat = 1);
+ * with clip_local_energy <= 0 nothing is clipped and centre and deviation are NaN.
+ * e_clipped_dev (double [B], optional) receives the clipped energies, e_diff_dev (optional) the
+ * clipped energies minus their mean (dloss/dlogpsi of the energy loss).  A NaN energy makes
+ * the mean, median and deviation NaN and stays NaN when clipped.  Sums are deterministic.
+ * Replaces clip_local_energies / local_energy_diff (src/sparse_wf/update.py:45-69) and the
+ * energy mean / std of Trainer.step (update.py:166-168), on one GPU's batch. */
+int fire_energy_stats(fire_handle* h, void* stream, const double* e_loc_dev, int32_t B, double clip_local_energy,
+                      int32_t stat, double* e_clipped_dev, double* e_diff_dev, double* stats_dev);
+
 /* Message of the last failure on this thread ("" if none). */
 const char* fire_last_error(void);
 

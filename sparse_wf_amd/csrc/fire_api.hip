@@ -2063,4 +2063,20 @@ int fire_state_stats(fire_handle* h, int32_t* n_walkers, int64_t* n_proposals, i
   }
 }
 
+int fire_energy_stats(fire_handle* h, void* stream, const double* e_loc_dev, int32_t B, double clip_local_energy,
+                      int32_t stat, double* e_clipped_dev, double* e_diff_dev, double* stats_dev) {
+  try {
+    REQUIRE(e_loc_dev, "null argument");
+    REQUIRE(B > 0, "fire_energy_stats: empty batch");
+    REQUIRE(stat == 0 || stat == 1, "fire_energy_stats: stat must be 0 (median) or 1 (mean)");
+    if (h) HIPCHK(hipSetDevice(h->device));
+    fire::launch_energy_stats((hipStream_t)stream, e_loc_dev, B, clip_local_energy, stat, e_clipped_dev, e_diff_dev,
+                              stats_dev);
+    HIPCHK(hipGetLastError());
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e);
+  }
+}
+
 }  // extern "C"

@@ -132,4 +132,8 @@ bool gemm_bf16_fusable(int variant);
 bool gemm_bf16_f16_ok(int variant);
 void gemm_bf16_tile(int variant, int* bm, int* bn);
 
+// local-energy statistics and clipping of one batch (one workgroup)
+void launch_energy_stats(hipStream_t s, const double* e, int B, double clip, int stat, double* clipped, double* diff,
+                         double* stats);
+
 }  // namespace fire
