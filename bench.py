@@ -62,6 +62,22 @@ def algorithmic_flops(cfg, counts_per_walker):
     return float(sum(terms.values())), {k: float(v) for k, v in terms.items()}, dict(P=P, T=T, R=R)
 
 
+TRAFFIC_PROFILE = "profiles/local_energy_c16h34_hbm_traffic_mfma.md"
+
+
+def measured_traffic():
+    """HBM bytes per split-contraction launch of the C16H34 mixed evaluation, from the
+    rocprofv3 counter summary committed with the repository (None if it is absent)."""
+    try:
+        with open(os.path.join(ROOT, TRAFFIC_PROFILE)) as f:
+            for line in f:
+                if line.strip().startswith("traffic_bytes_per_launch"):
+                    return float(line.split("=")[1])
+    except OSError:
+        pass
+    return None
+
+
 def err_stats(d):
     d = np.asarray(d, np.float64) * 1e3
     return {"bias": float(np.mean(d)), "rms": float(np.sqrt(np.mean(d ** 2))), "median_abs": float(np.median(np.abs(d))),
@@ -212,6 +228,7 @@ def main():
 
     g_ms, g_flops, g_n = prof_m
     g_achieved = (g_flops / (g_ms * 1e-3)) / 1e12 if g_ms > 0 else None
+    traffic = measured_traffic() if args.workload == "c16h34" and B == 1024 else None
     roofline = {
         "bound": "mfma",
         "kernel": "k_gemm_bf16* (every dense contraction of one mixed step: embedding and orbital "
@@ -221,14 +238,16 @@ def main():
         "peak": PEAK["bf16"],
         "unit": "TFLOP/s",
         "frac": (g_achieved / PEAK["bf16"]) if g_achieved else None,
-        "traffic": None,
+        "traffic": traffic,
+        "traffic_source": TRAFFIC_PROFILE if traffic else None,
         "avg_launch_ms": g_ms / g_n if g_n else None,
         "launches_per_step": g_n,
         "algorithmic_flop_per_step": g_flops,
         "note": "achieved = algorithmic 2*M*N*K per launch over the HIP-event duration of the launches on "
-                "their stream; the matrix cores execute 6x (three planes) or 3x (two planes) these flops; "
-                "traffic (HBM bytes per launch) is measured separately with rocprofv3 --pmc; whole-step "
-                "rate in whole_step_tflops",
+                "their stream; the matrix cores execute 6x (three BF16 planes) or 3x (a BF16 pair or two FP16 "
+                "planes) these flops; traffic = HBM bytes per launch from two rocprofv3 --pmc runs "
+                "(FETCH_SIZE x2, WRITE_SIZE x1 by a 1 GiB copy calibration) of this workload, read from the "
+                "committed profile summary; whole-step rate in whole_step_tflops",
         "whole_step_tflops": flops_w * wps_m / 1e12,
     }
 
