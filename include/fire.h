@@ -217,6 +217,37 @@ int fire_autotune(fire_handle* h, int32_t enable, int32_t* table, int32_t cap, i
  * launch) and the launch count -- then resets and applies `enable`.  Synchronises. */
 int fire_profile_gemm(fire_handle* h, int32_t enable, double* total_ms, double* total_flops, int64_t* n_launches);
 
+/* ---- pseudopotentials ----------------------------------------------------------------------
+ * Effective-core potentials of n_ecp nuclei (atom[e] indexes the molecule's nuclei), tabulated
+ * on one ascending radial grid (double [n_grid]): the local channel v_loc [n_ecp][n_grid] and the
+ * non-local channels l = 0 .. n_l - 1 (n_l <= 4) v_nl [n_ecp][n_l][n_grid], both already divided by
+ * r^2 as eval_ecp_on_grid does; cutoff [n_ecp] is the radius beyond which the non-local channels
+ * are neglected, n_quad [n_ecp] the points of the atom's spherical quadrature and quad_dirs
+ * [n_ecp][q_max][3] its unit directions (weights 1 / n_quad).  z_eff (optional, [n_nuc]) replaces
+ * the nuclear charges of the Coulomb terms of fire_local_energy (Z - core electrons).  n_ecp = 0
+ * removes the pseudopotentials.  Synchronises the device.
+ * Replaces make_pseudopotential / eval_ecp_on_grid (src/sparse_wf/pseudopotentials.py:136-202,
+ * 411-454) given the ECP coefficients (the reference loads them with pyscf). */
+int fire_ecp_upload(fire_handle* h, int32_t n_ecp, const int32_t* atom, int32_t n_l, int32_t n_grid,
+                    const double* grid, const double* v_loc, const double* v_nl, const double* cutoff,
+                    const int32_t* n_quad, int32_t q_max, const double* quad_dirs, const float* z_eff);
+
+/* Pseudopotential energy of the walkers of the resident state (fire_state_init): v_loc_dev
+ * (double [B]) the local part, sum over electrons and ECP nuclei of the interpolated local
+ * channel; v_nl_dev (double [B]) the non-local part: each electron within the cutoff of its
+ * closest ECP nucleus is placed on that nucleus's quadrature, rotated at random (Philox4x32-10
+ * normals under (seed, walker) and counter (electron, 0x45435000 + block)), at its own distance;
+ * each point adds sum_l (2l+1) V_l(r') P_l(cos theta) / n_quad times psi(r')/psi(r), the ratio
+ * from a low-rank proposal of the state (one per electron and quadrature point for all walkers,
+ * each a fire_state_propose).  *n_points (optional, host) receives the number of active
+ * integration points.  The state is unchanged; proposals are counted in fire_state_stats.
+ * Synchronises the stream (the move's capacity check, the active points).
+ * Replaces the pp_local / pp_nonloc calls of make_local_energy (src/sparse_wf/hamiltonian.py:
+ * 69-81) with make_local_pseudopotential, build_all_integration_points and
+ * make_nonlocal_pseudopotential (pseudopotentials.py:204-218, 269-398). */
+int fire_ecp_energy(fire_handle* h, void* stream, uint64_t seed, double* v_loc_dev, double* v_nl_dev,
+                    int64_t* n_points);
+
 /* Statistics and clipping of a batch of local energies e_loc_dev (double [B], B > 0) on the
  * handle's device (h may be NULL: the calling thread's current device), one workgroup:
  * stats_dev (double [5]) = E_mean, E_std (population), clip centre, mean absolute deviation

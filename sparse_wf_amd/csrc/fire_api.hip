@@ -213,6 +213,10 @@ struct fire_handle {
   int chunk = 256;
   bool lu_smem_set = false;
   int64_t* scratch = nullptr;  // this device's static scratch (fire_neighbours_noalloc)
+  // pseudopotentials (fire_ecp_upload): radial tables and quadrature directions on the device,
+  // and the integration points of the last fire_ecp_energy
+  int ecp_n = 0, ecp_nl = 0, ecp_ng = 0, ecp_qmax = 0;
+  DevBuf ecp_atom, ecp_grid, ecp_vloc, ecp_vnl, ecp_cut, ecp_nq, ecp_qpts, ecp_pts, ecp_coef, ecp_kq, ecp_npts;
   // optional HIP-event timing of the dense contractions (fire_profile_gemm)
   bool gemm_timing = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> gemm_events;
@@ -1416,7 +1420,8 @@ std::vector<DevBuf*> handle_bufs(fire_handle* h) {
           &h->det_sg, &h->det_gr,  &h->det_lap, &h->JR,     &h->mlp_ns,  &h->mlp_dg, &h->QV,      &h->AV,
           &h->att_ns, &h->att_dg,  &h->RIN,    &h->RY1,     &h->RY2,     &h->att_out, &h->cval,   &h->clap,
           &h->cgrad,  &h->epot,    &h->PF,     &h->EF,      &h->EOK,     &h->rowgrp, &h->AV0,     &h->Z0,
-          &h->F1,     &h->PART,    &h->AMX_A,  &h->AMX_B,   &h->AMX_C};
+          &h->F1,     &h->PART,    &h->AMX_A,  &h->AMX_B,   &h->AMX_C,  &h->ecp_atom, &h->ecp_grid, &h->ecp_vloc,
+          &h->ecp_vnl, &h->ecp_cut, &h->ecp_nq, &h->ecp_qpts, &h->ecp_pts, &h->ecp_coef, &h->ecp_kq, &h->ecp_npts};
 }
 
 void free_params(fire_handle* h) {
@@ -1438,6 +1443,7 @@ void apply_config(fire_handle* h, const fire_config& c, const float* R, const fl
   if (h->st.graph) (void)hipGraphExecDestroy(h->st.graph);
   h->st.graph = nullptr;
   h->st.ready = h->st.pending = false;
+  h->ecp_n = 0;  // pseudopotentials belong to the previous molecule
   h->cfg = c;
   h->N = c.n_el;
   h->n_up = c.n_up;
@@ -2057,6 +2063,112 @@ int fire_state_stats(fire_handle* h, int32_t* n_walkers, int64_t* n_proposals, i
     if (sum_changed) *sum_changed = st[0];
     // the largest affected set, before truncation to the capacity when a move exceeded it
     if (max_changed) *max_changed = std::max<int64_t>(st[1], ov[1]);
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e);
+  }
+}
+
+int fire_ecp_upload(fire_handle* h, int32_t n_ecp, const int32_t* atom, int32_t n_l, int32_t n_grid,
+                    const double* grid, const double* v_loc, const double* v_nl, const double* cutoff,
+                    const int32_t* n_quad, int32_t q_max, const double* quad_dirs, const float* z_eff) {
+  try {
+    REQUIRE(h, "null handle");
+    REQUIRE(n_ecp >= 0 && n_ecp <= h->M, "fire_ecp_upload: n_ecp out of range");
+    HIPCHK(hipSetDevice(h->device));
+    HIPCHK(hipDeviceSynchronize());
+    if (z_eff) {
+      // Coulomb terms of fire_local_energy use these charges from now on
+      std::vector<float> R(3 * h->M);
+      HIPCHK(hipMemcpy(R.data(), h->R, sizeof(float) * 3 * h->M, hipMemcpyDeviceToHost));
+      double enn = 0.0;
+      for (int I = 0; I < h->M; ++I)
+        for (int J = I + 1; J < h->M; ++J) {
+          const double dx = (double)R[3 * I] - R[3 * J], dy = (double)R[3 * I + 1] - R[3 * J + 1],
+                       dz = (double)R[3 * I + 2] - R[3 * J + 2];
+          enn += (double)z_eff[I] * z_eff[J] / std::sqrt(dx * dx + dy * dy + dz * dz);
+        }
+      h->e_nn = enn;
+      HIPCHK(hipMemcpy(h->Z, z_eff, sizeof(float) * h->M, hipMemcpyHostToDevice));
+    }
+    h->ecp_n = 0;
+    if (n_ecp == 0) return 0;
+    REQUIRE(atom && grid && v_loc && v_nl && cutoff && n_quad && quad_dirs, "null argument");
+    REQUIRE(n_l >= 0 && n_l <= 4, "fire_ecp_upload: non-local channels must be l = 0 .. 3");
+    REQUIRE(n_grid >= 2 && q_max >= 1, "fire_ecp_upload: empty radial grid or quadrature");
+    for (int e = 0; e < n_ecp; ++e) {
+      REQUIRE(atom[e] >= 0 && atom[e] < h->M, "fire_ecp_upload: atom index out of range");
+      REQUIRE(n_quad[e] >= 1 && n_quad[e] <= q_max, "fire_ecp_upload: quadrature size out of range");
+    }
+    for (int g = 1; g < n_grid; ++g) REQUIRE(grid[g] >= grid[g - 1], "fire_ecp_upload: radial grid not ascending");
+    auto put = [](DevBuf& b, const void* src, size_t bytes) {
+      b.ensure(bytes);
+      HIPCHK(hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice));
+    };
+    put(h->ecp_atom, atom, sizeof(int32_t) * n_ecp);
+    put(h->ecp_grid, grid, sizeof(double) * n_grid);
+    put(h->ecp_vloc, v_loc, sizeof(double) * n_ecp * n_grid);
+    if (n_l > 0) put(h->ecp_vnl, v_nl, sizeof(double) * n_ecp * n_l * n_grid);
+    put(h->ecp_cut, cutoff, sizeof(double) * n_ecp);
+    put(h->ecp_nq, n_quad, sizeof(int32_t) * n_ecp);
+    put(h->ecp_qpts, quad_dirs, sizeof(double) * n_ecp * q_max * 3);
+    h->ecp_n = n_ecp;
+    h->ecp_nl = n_l;
+    h->ecp_ng = n_grid;
+    h->ecp_qmax = q_max;
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e);
+  }
+}
+
+int fire_ecp_energy(fire_handle* h, void* stream, uint64_t seed, double* v_loc_dev, double* v_nl_dev,
+                    int64_t* n_points) {
+  try {
+    check_ready(h);
+    REQUIRE(h->st.ready, "no walker state (call fire_state_init first)");
+    REQUIRE(v_loc_dev && v_nl_dev, "null argument");
+    HIPCHK(hipSetDevice(h->device));
+    Ctx c{h, (hipStream_t)stream, h->st.prec};
+    auto& S = h->st;
+    const int B = S.B, N = h->N;
+    launch_fill_f64(c.s, v_loc_dev, B, 0.0);
+    launch_fill_f64(c.s, v_nl_dev, B, 0.0);
+    if (n_points) *n_points = 0;
+    if (h->ecp_n == 0 || B == 0) return 0;
+    state_drop_pending(c);
+    const int Q = h->ecp_qmax;
+    h->ecp_pts.ensure(sizeof(float) * 3 * (size_t)N * Q * B);
+    h->ecp_coef.ensure(sizeof(double) * (size_t)N * Q * B);
+    h->ecp_kq.ensure(sizeof(int32_t) * N);
+    h->ecp_npts.ensure(sizeof(unsigned long long));
+    launch_fill_i32(c.s, h->ecp_kq.as<int32_t>(), N, 0);
+    launch_set_i64(c.s, h->ecp_npts.as<int64_t>(), 0);
+    EcpTables t{h->ecp_n,          h->ecp_nl,          h->ecp_ng,           Q,
+                h->ecp_atom.as<int32_t>(), h->ecp_grid.as<double>(), h->ecp_vloc.as<double>(),
+                h->ecp_nl ? h->ecp_vnl.as<double>() : nullptr, h->ecp_cut.as<double>(), h->ecp_nq.as<int32_t>(),
+                h->ecp_qpts.as<double>()};
+    launch_ecp_points(c.s, S.r.as<float>(), h->R, B, N, t, seed, h->ecp_pts.as<float>(), h->ecp_coef.as<double>(),
+                      h->ecp_kq.as<int32_t>(), v_loc_dev, h->ecp_npts.as<unsigned long long>());
+    std::vector<int32_t> kq(N);
+    int64_t npts = 0;
+    HIPCHK(hipMemcpyAsync(kq.data(), h->ecp_kq.p, sizeof(int32_t) * N, hipMemcpyDeviceToHost, c.s));
+    HIPCHK(hipMemcpyAsync(&npts, h->ecp_npts.p, sizeof(int64_t), hipMemcpyDeviceToHost, c.s));
+    HIPCHK(hipStreamSynchronize(c.s));
+    if (n_points) *n_points = npts;
+    // psi ratios: one low-rank proposal per (electron, quadrature point) for all walkers; a
+    // walker whose electron is not active there proposes its own position (coefficient 0)
+    for (int k = 0; k < N; ++k)
+      for (int q = 0; q < kq[k]; ++q) {
+        const size_t slot = ((size_t)k * Q + q) * B;
+        state_propose(c, k, 1, h->ecp_pts.as<float>() + slot * 3, 0.f, 0, 0, nullptr);
+        S.n_proposals += B;
+        launch_ecp_accum(c.s, B, h->ecp_coef.as<double>() + slot, S.p_logpsi.as<double>(), S.p_sign.as<double>(),
+                         S.logpsi.as<double>(), S.sign.as<double>(), v_nl_dev);
+      }
+    state_drop_pending(c);
+    state_check_overflow(c);
+    HIPCHK(hipGetLastError());
     return 0;
   } catch (const std::exception& e) {
     return fail(e);

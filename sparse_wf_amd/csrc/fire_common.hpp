@@ -27,6 +27,24 @@ struct FilterParams {  // PairwiseFilter (model/utils.py:393-433) with precomput
   int w0, w1, E;
 };
 
+// Philox4x32-10 counter-based generator (Salmon, Moraes, Dror and Shaw, SC'11): 10 rounds
+// on a 4 x 32-bit counter under a 2 x 32-bit key.
+__device__ __forceinline__ uint4 philox4x32_10(uint4 c, uint2 k) {
+  const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u, W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    const uint32_t lo0 = M0 * c.x, hi0 = __umulhi(M0, c.x);
+    const uint32_t lo1 = M1 * c.z, hi1 = __umulhi(M1, c.z);
+    c = make_uint4(hi1 ^ c.y ^ k.x, lo1, hi0 ^ c.w ^ k.y, lo0);
+    k.x += W0;
+    k.y += W1;
+  }
+  return c;
+}
+
+// (x >> 8 + 1/2) / 2^24: a uniform variate strictly inside (0, 1)
+__device__ __forceinline__ double u01(uint32_t x) { return ((double)(x >> 8) + 0.5) * (1.0 / 16777216.0); }
+
 __device__ __forceinline__ float sigmoidf_(float x) { return 1.0f / (1.0f + __expf(-x)); }
 
 // silu and its first two derivatives

@@ -132,6 +132,22 @@ bool gemm_bf16_fusable(int variant);
 bool gemm_bf16_f16_ok(int variant);
 void gemm_bf16_tile(int variant, int* bm, int* bn);
 
+// pseudopotential tables on the device (radial grids in FP64)
+struct EcpTables {
+  int n_ecp, n_l, n_grid, qmax;
+  const int32_t* atom;  // [n_ecp] nucleus index
+  const double* grid;   // [n_grid] radii
+  const double* vloc;   // [n_ecp][n_grid] local channel
+  const double* vnl;    // [n_ecp][n_l][n_grid] non-local channels l = 0 .. n_l - 1
+  const double* cut;    // [n_ecp] non-local cutoff radius
+  const int32_t* nq;    // [n_ecp] quadrature points
+  const double* qpts;   // [n_ecp][qmax][3] unrotated quadrature directions
+};
+void launch_ecp_points(hipStream_t s, const float* r, const float* R, int B, int N, const EcpTables& t, uint64_t seed,
+                       float* pts, double* coef, int32_t* kq, double* vloc, unsigned long long* n_pts);
+void launch_ecp_accum(hipStream_t s, int B, const double* coef, const double* lp_new, const double* sg_new,
+                      const double* lp, const double* sg, double* v);
+
 // local-energy statistics and clipping of one batch (one workgroup)
 void launch_energy_stats(hipStream_t s, const double* e, int B, double clip, int stat, double* clipped, double* diff,
                          double* stats);

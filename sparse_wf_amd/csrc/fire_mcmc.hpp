@@ -53,5 +53,6 @@ void launch_mc_commit(hipStream_t s, const int32_t* accept, const int2* list, in
                       double* st_sg, double* st_logpsi, double* st_sign, float* p_r, const double* cdelta,
                       double* ccusp);
 void launch_fill_i32(hipStream_t s, int32_t* p, int n, int v);
+void launch_fill_f64(hipStream_t s, double* p, int n, double v);
 
 }  // namespace fire

@@ -18,25 +18,6 @@
 
 namespace fire {
 
-// ---------------------------------------------------------------------------------------
-// Philox4x32-10 counter-based generator (Salmon et al., SC'11): key (seed, walker),
-// counter (step, lane, 0, 0).
-// ---------------------------------------------------------------------------------------
-__device__ __forceinline__ uint4 philox4x32_10(uint4 c, uint2 k) {
-  const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u, W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
-#pragma unroll
-  for (int r = 0; r < 10; ++r) {
-    const uint32_t lo0 = M0 * c.x, hi0 = __umulhi(M0, c.x);
-    const uint32_t lo1 = M1 * c.z, hi1 = __umulhi(M1, c.z);
-    c = make_uint4(hi1 ^ c.y ^ k.x, lo1, hi0 ^ c.w ^ k.y, lo0);
-    k.x += W0;
-    k.y += W1;
-  }
-  return c;
-}
-
-__device__ __forceinline__ double u01(uint32_t x) { return ((double)(x >> 8) + 0.5) * (1.0 / 16777216.0); }
-
 // Three N(0,1) variates and one U(0,1) variate for (seed, walker, step).
 __device__ inline void step_randoms(uint64_t seed, int w, uint64_t step, double n3[3], double& u) {
   const uint2 key = make_uint2((uint32_t)seed ^ (uint32_t)(seed >> 32), (uint32_t)w);
@@ -940,6 +921,15 @@ __global__ void k_fill_i32(int32_t* p, int n, int v) {
 
 void launch_fill_i32(hipStream_t s, int32_t* p, int n, int v) {
   if (n > 0) k_fill_i32<<<(n + 255) / 256, 256, 0, s>>>(p, n, v);
+}
+
+__global__ void k_fill_f64(double* p, int n, double v) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) p[e] = v;
+}
+
+void launch_fill_f64(hipStream_t s, double* p, int n, double v) {
+  if (n > 0) k_fill_f64<<<(n + 255) / 256, 256, 0, s>>>(p, n, v);
 }
 
 }  // namespace fire

@@ -98,6 +98,7 @@ class FireWaveFunction:
         self.cfg = cfg
         self._uploaded = self._uploaded_fp = None
         self._committed_gen = self._pending_gen = None
+        self._pp = None
 
     def workspace_bytes(self) -> int:
         """Device bytes held by the handle (workspace, walker state, parameters)."""
@@ -298,6 +299,37 @@ class FireWaveFunction:
         if single:
             out = {k: v[0] for k, v in out.items()}
         return out
+
+    # pseudopotentials -------------------------------------------------------------------
+    def set_pseudopotential(self, pp):
+        """Pseudopotential tables (``pseudopotentials.make_pseudopotential``) for
+        ``pseudopotential_energy``; from now on the Coulomb terms of the local energy use the
+        effective charges (make_local_energy passes them to potential_energy,
+        hamiltonian.py:66-75).  ``None`` removes them and restores the nuclear charges."""
+        from .pseudopotentials import make_pseudopotential, upload
+
+        if pp is None:
+            pp = make_pseudopotential(self.cfg.mol.Z, {})
+        upload(self._lib, self._h, pp)
+        self._pp = pp if pp.n_ecp else None
+
+    def pseudopotential_energy(self, params, electrons, seed: int = 0, precision=None) -> dict:
+        """Local and non-local pseudopotential energy per walker (pp_local / pp_nonloc of
+        make_local_energy, hamiltonian.py:72-75): the walkers become the resident state
+        (log_psi_with_state), the psi ratios of the quadrature points come from its low-rank
+        proposals.  Returns ``{"v_loc", "v_nl"}`` (double [B]) and ``n_points``, the active
+        integration points (the reference's n_pp_elecs statistic)."""
+        if getattr(self, "_pp", None) is None:
+            raise ValueError("no pseudopotential set (set_pseudopotential)")
+        r, single = self._walkers(electrons)
+        self.log_psi_with_state(params, r, precision=precision)
+        B = r.shape[0]
+        vl = torch.empty(B, dtype=torch.float64, device=self.device)
+        vn = torch.empty(B, dtype=torch.float64, device=self.device)
+        n = ctypes.c_int64()
+        _lib.check(self._lib.fire_ecp_energy(self._h, _stream_ptr(), ctypes.c_uint64(int(seed) & (2 ** 64 - 1)),
+                                             _dptr(vl), _dptr(vn), ctypes.byref(n)))
+        return {"v_loc": vl[0] if single else vl, "v_nl": vn[0] if single else vn, "n_points": n.value}
 
     def kinetic_energy(self, params, electrons, static=None, precision=None):
         """-1/2 (Laplacian + |grad|^2) of log|psi| -- wave_function.py:215-221."""
@@ -530,14 +562,25 @@ class WalkerState:
     pending: bool
 
 
-def make_local_energy(wf: FireWaveFunction, energy_operator: str = "sparse", precision: Optional[str] = None):
-    """make_local_energy (hamiltonian.py:49-81) for all-electron molecules: returns
-    ``local_energy(key, params, electrons, static) -> (E_loc, static)``."""
+def make_local_energy(wf: FireWaveFunction, energy_operator: str = "sparse", precision: Optional[str] = None,
+                      pseudopotential=None):
+    """make_local_energy (hamiltonian.py:49-81): returns
+    ``local_energy(key, params, electrons, static) -> (E_loc, static)``.  With a
+    ``pseudopotential`` (pseudopotentials.make_pseudopotential) E_loc adds its local and
+    non-local parts, the Coulomb terms use the effective charges, and ``key`` (an integer)
+    seeds the quadrature rotations."""
     if energy_operator.lower() != "sparse":
         raise ValueError("only the sparse forward-Laplacian energy operator is provided")
+    if pseudopotential is not None:
+        wf.set_pseudopotential(pseudopotential)
 
     def local_energy(key, params, electrons, static=None):
         terms = wf.local_energy_terms(params, electrons, static, precision)
-        return terms["e_loc"], static
+        e = terms["e_loc"]
+        if pseudopotential is not None and pseudopotential.n_ecp:
+            seed = int(np.asarray(key).reshape(-1).astype(np.uint64).sum()) if key is not None else 0
+            pp = wf.pseudopotential_energy(params, electrons, seed, precision)
+            e = e + pp["v_loc"] + pp["v_nl"]
+        return e, static
 
     return local_energy
