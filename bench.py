@@ -222,9 +222,16 @@ def main():
     out_f, dt_f, lat_f, _ = run("fp32", args.steps, 1)
     log("timing mixed_fast")
     out_x, dt_x, _, _ = run("mixed_fast", max(1, args.steps // 2), 1)
+    # the determinant Laplacian's triplet products in FP64 (FP64 transposed inverse on the FP64
+    # matrix cores) instead of on an FP32 copy of the inverse
+    log("timing mixed with FP64 determinant-Laplacian products")
+    wf.det_options(True)
+    out_d, dt_d, _, _ = run("mixed", max(1, args.steps // 2), 1)
+    wf.det_options(False)
     wps_m = B * args.steps / dt_m
     wps_f = B * args.steps / dt_f
     wps_x = B * max(1, args.steps // 2) / dt_x
+    wps_d = B * max(1, args.steps // 2) / dt_d
 
     g_ms, g_flops, g_n = prof_m
     g_achieved = (g_flops / (g_ms * 1e-3)) / 1e12 if g_ms > 0 else None
@@ -265,6 +272,7 @@ def main():
     e_m = out_m["e_loc"].cpu().numpy()
     e_f = out_f["e_loc"].cpu().numpy()
     e_x = out_x["e_loc"].cpu().numpy()
+    e_d = out_d["e_loc"].cpu().numpy()
     result = {
         "metric": METRIC,
         "value": wps_m,
@@ -284,6 +292,7 @@ def main():
         "mixed_over_fp32": wps_m / wps_f,
         "mixed_plan": plan,
         "mixed_fast_walkers_per_s": wps_x,
+        "mixed_fp64_laplacian_walkers_per_s": wps_d,
         "latency_ms": {"p50": 1e3 * float(np.percentile(lat_m, 50)), "p99": 1e3 * float(np.percentile(lat_m, 99)),
                        "what": "one local-energy call of the whole batch (mixed)"},
         "mixed_minus_fp32_mHa": err_stats(e_m - e_f),
@@ -303,7 +312,7 @@ def main():
 
         log(f"ragged leg: {args.ragged} geometries")
         geoms = random_geometries(args.ragged, seed=3, n_el=(10, 200), walkers=256, feature_dim=256)
-        g_run = run_ragged(geoms, args.ragged_sweeps, precision="mixed", use_graphs=True)
+        g_run = run_ragged(geoms, args.ragged_sweeps, precision="mixed", use_graphs=True, local_energy=True)
         e_run = run_ragged(geoms, args.ragged_sweeps, precision="mixed", use_graphs=False)
         same = all(np.array_equal(a.cpu().numpy(), b.cpu().numpy())
                    for a, b in zip(g_run["walkers"], e_run["walkers"]))
@@ -316,6 +325,10 @@ def main():
             "graph_over_eager": g_run["walker_moves_per_s"] / e_run["walker_moves_per_s"],
             "pmove_mean": float(np.mean(g_run["pmove"])),
             "walkers_identical_graph_vs_eager": bool(same),
+            "local_energy_walkers_per_s": g_run["local_energy_walkers_per_s"],
+            "local_energy_seconds": g_run["local_energy_seconds"],
+            "local_energy_what": "mixed local energy of every geometry's final walkers, each geometry on its own "
+                                 "stream from 8 host threads",
         }
         log(f"ragged leg: {g_run['walker_moves_per_s']:.3g} moves/s (graphs), "
             f"{e_run['walker_moves_per_s']:.3g} (eager)")
@@ -377,10 +390,11 @@ def main():
             result["energy_error_vs_fp64_mHa"] = {
                 "walkers": n, "mixed": err_stats(e_m[:n] - ref), "fp32": err_stats(e_f[:n] - ref),
                 "mixed_fast": err_stats(e_x[:n] - ref),
+                "mixed_fp64_laplacian": err_stats(e_d[:n] - ref),
                 "gate": "|bias| <= 1 mHa; bias_se = standard error of the bias",
             }
-            for k in ("mixed", "fp32", "mixed_fast"):
-                e = {"mixed": e_m, "fp32": e_f, "mixed_fast": e_x}[k][:n]
+            for k in ("mixed", "fp32", "mixed_fast", "mixed_fp64_laplacian"):
+                e = {"mixed": e_m, "fp32": e_f, "mixed_fast": e_x, "mixed_fp64_laplacian": e_d}[k][:n]
                 result["energy_error_vs_fp64_mHa"][k]["bias_se"] = float(np.std((e - ref) * 1e3, ddof=1) / np.sqrt(n))
         legs = [x for x in args.config_legs.split(",") if x]
         if legs:

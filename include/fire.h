@@ -200,6 +200,15 @@ int fire_state_stats(fire_handle* h, int32_t* n_walkers, int64_t* n_proposals, i
  * int32 [6]) receives the plan. */
 int fire_precision_plan(fire_handle* h, const int32_t* planes, int32_t n, int32_t* current);
 
+/* Precision of the determinant Laplacian's triplet products B_x = J_x A^-1[:, S] (no counterpart
+ * in the reference, which runs the whole model in one precision): fp64_laplacian = 1 multiplies
+ * the FP32 orbital jacobian rows, widened, with an FP64 transposed copy of the FP64 inverse on the
+ * FP64 matrix cores; 0 uses an FP32 copy of the inverse on the FP32 matrix cores.  The inverse
+ * itself, the trace with the Laplacian rows and every sum are FP64 either way.  The FP64 form
+ * needs n_el even and n_el * n_det a multiple of 4 (otherwise the FP32 form runs).  Applies to
+ * every precision mode. */
+int fire_det_options(fire_handle* h, int32_t fp64_laplacian);
+
 /* Tile-shape autotuner of the dense contractions (no counterpart in the reference; the
  * reference leaves kernel selection to XLA).  On by default: the first BF16 contraction of
  * each (split, N, K, floor(log2 M)) shape times every tile shape on its real operands

@@ -57,6 +57,7 @@ SIGNATURES = {
     "fire_autotune": (ctypes.c_int, [_P, ctypes.c_int32, _P, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]),
     "fire_precision_plan": (ctypes.c_int, [_P, _P, ctypes.c_int32, _P]),
     "fire_mcmc_options": (ctypes.c_int, [_P, ctypes.c_int32]),
+    "fire_det_options": (ctypes.c_int, [_P, ctypes.c_int32]),
     "fire_ecp_upload": (ctypes.c_int, [_P, ctypes.c_int32, _P, ctypes.c_int32, ctypes.c_int32, _P, _P, _P, _P, _P,
                                        ctypes.c_int32, _P, _P]),
     "fire_ecp_energy": (ctypes.c_int, [_P, _P, ctypes.c_uint64, _P, _P, ctypes.POINTER(ctypes.c_int64)]),

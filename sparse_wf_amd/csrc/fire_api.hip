@@ -228,6 +228,9 @@ struct fire_handle {
   // the separate element-wise pass instead)
   bool fuse_act = true;
   bool use_graphs = true;  // whole Metropolis sweeps replayed from a captured hipGraph
+  // determinant Laplacian's products J Ainv[:, S] in FP64 (FP64 transposed inverse, FP64 matrix
+  // cores) instead of FP32 on an FP32 copy of the inverse (fire_det_options)
+  bool det_lap64 = false;
   // BF16 planes per operand used by FIRE_MIXED for each contraction class (ElecInit MLP,
   // update MLP, orbitals, MLP Jastrow, attention Q/V, attention readout): 3 = six products
   // (FP32-level), 2 = three products, 1 = plain.  Default: 3 for the embedding and orbital
@@ -817,15 +820,23 @@ void run_orbitals_det(Ctx& c, const float* r, int B, const Graph& gr, bool tan, 
     h->dgrad.ensure(sizeof(double) * nmat * N * 3);
     h->det_gr.ensure(sizeof(double) * (size_t)B * 3 * N);
     h->det_lap.ensure(sizeof(double) * B);
-    h->AinvT.ensure(sizeof(float) * nmat * N * N);
-    launch_lu_inv(c.s, h->A64.as<double>(), (int)nmat, N, h->Ainv.as<double>(), h->AinvT.as<float>(),
+    const bool lap64 = h->det_lap64 && det_lap64_supported(N, ND, ldo);
+    h->AinvT.ensure((lap64 ? sizeof(double) : sizeof(float)) * nmat * N * N);
+    launch_lu_inv(c.s, h->A64.as<double>(), (int)nmat, N, h->Ainv.as<double>(), lap64 ? nullptr : h->AinvT.as<float>(),
                   h->ldet.as<double>(), h->dsgn.as<double>(), true, lu_lds);
     launch_det_trace(c.s, Ob, ldo, gd, B, N, h->n_up, D, h->Ainv.as<double>(), h->tr.as<double>());
     const int smax = gr.max_nb + 1;
     REQUIRE(smax <= FIRE_SMAX, "an electron has more than 63 neighbours within the cutoff (determinant Laplacian limit)");
-    launch_det_lap(c.s, Ob, ldo, ND, gd, h->ENV.as<float>(), h->ptr.as<int64_t>(), h->nbr.as<int32_t>(),
-                   h->rev.as<int32_t>(), B, N, h->n_up, D, smax, h->AinvT.as<float>(), h->trip.as<double>(),
-                   h->dgrad.as<double>());
+    if (lap64) {
+      launch_transpose_f64(c.s, h->Ainv.as<double>(), (int)nmat, N, h->AinvT.as<double>());
+      launch_det_lap64(c.s, Ob, ldo, ND, gd, h->ENV.as<float>(), h->ptr.as<int64_t>(), h->nbr.as<int32_t>(),
+                       h->rev.as<int32_t>(), B, N, h->n_up, D, smax, h->AinvT.as<double>(), h->trip.as<double>(),
+                       h->dgrad.as<double>());
+    } else {
+      launch_det_lap(c.s, Ob, ldo, ND, gd, h->ENV.as<float>(), h->ptr.as<int64_t>(), h->nbr.as<int32_t>(),
+                     h->rev.as<int32_t>(), B, N, h->n_up, D, smax, h->AinvT.as<float>(), h->trip.as<double>(),
+                     h->dgrad.as<double>());
+    }
     out.grad = h->det_gr.as<double>();
     out.lap = h->det_lap.as<double>();
   } else {
@@ -2169,6 +2180,17 @@ int fire_ecp_energy(fire_handle* h, void* stream, uint64_t seed, double* v_loc_d
     state_drop_pending(c);
     state_check_overflow(c);
     HIPCHK(hipGetLastError());
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e);
+  }
+}
+
+int fire_det_options(fire_handle* h, int32_t fp64_laplacian) {
+  try {
+    REQUIRE(h, "null handle");
+    REQUIRE(fp64_laplacian == 0 || fp64_laplacian == 1, "fire_det_options: fp64_laplacian must be 0 or 1");
+    h->det_lap64 = fp64_laplacian != 0;
     return 0;
   } catch (const std::exception& e) {
     return fail(e);

@@ -23,6 +23,13 @@ void launch_det_trace(hipStream_t s, const float* O, int ldo, GroupDesc gd, int 
 void launch_det_lap(hipStream_t s, const float* O, int ldo, int ND, GroupDesc gd, const float* env, const int64_t* ptr,
                     const int32_t* nbr, const int32_t* rev, int B, int N, int n_up, int D, int smax,
                     const float* AinvT, double* trip, double* grad);
+// FP64 form of launch_det_lap on the FP64 transposed inverse (det_lap64_supported: N even, ND
+// and ldo multiples of 4 and 32)
+void launch_det_lap64(hipStream_t s, const float* O, int ldo, int ND, GroupDesc gd, const float* env,
+                      const int64_t* ptr, const int32_t* nbr, const int32_t* rev, int B, int N, int n_up, int D,
+                      int smax, const double* AinvT, double* trip, double* grad);
+bool det_lap64_supported(int N, int ND, int ldo);
+void launch_transpose_f64(hipStream_t s, const double* X, int n_mat, int N, double* XT);
 void launch_det_combine(hipStream_t s, int B, int N, int D, const double* logdet, const double* sgn, const double* tr,
                         const double* trip, const double* grad, double* out_logpsi, double* out_sign,
                         double* out_grad, double* out_lap, bool tangents);

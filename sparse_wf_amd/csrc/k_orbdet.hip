@@ -907,6 +907,193 @@ void launch_det_lap(hipStream_t s, const float* O, int ldo, int ND, GroupDesc gd
     det_lap_launch<false, 1>(s, O, ldo, ND, gd, env, ptr, nbr, rev, B, N, n_up, D, smax, AinvT, trip, grad);
 }
 
+// FP64 form of the triplet kernel: the same units, chunking and column permutation, with the
+// jacobian rows (FP32, times the envelope scale) widened to FP64 and multiplied with the FP64
+// transposed inverse on v_mfma_f64_16x16x4f64, FP64 accumulators.  Lane l holds A[l%16][l/16]
+// and B[l/16][l%16]; register v of lane l holds D[4v + l/16][l%16], so the tile transposes
+// through LDS in that layout.  Vector loads only (N even, ldo and ND multiples of 4); the
+// inverse rows are read as double2.
+typedef double dbl4_t __attribute__((ext_vector_type(4)));
+typedef double dbl2_t __attribute__((ext_vector_type(2)));
+
+template <int NAT, int XG>
+__device__ __forceinline__ void det_lap_unit64(const float* __restrict__ O, int ldo, int ND, GroupDesc gd,
+                                               const float* __restrict__ env, const int32_t* __restrict__ nbr,
+                                               const int32_t* __restrict__ rev, int B, int N, int n_up, int D,
+                                               const double* __restrict__ AinvT, double* __restrict__ trip,
+                                               double* __restrict__ grad, int w, int n, int d, int gn, int64_t p0,
+                                               int s, double* tb) {
+  const int lane = threadIdx.x & 63, li = lane & 15, kg = lane >> 4;
+  const int cbase = (d * N) & ~15, off = d * N - cbase;
+  const int ncol = N + off, nch = (ncol + 15) / 16;
+  const float* jb[NAT];
+  const float* eb[NAT];
+  const double* xb[NAT];
+  bool sc[NAT], bok[NAT];
+#pragma unroll
+  for (int t = 0; t < NAT; ++t) {
+    const int m = t * 16 + li;
+    const bool ok = m < s;
+    int i = n;
+    int64_t row = gd.row(gn) + 2, ep = -1;
+    if (ok && m > 0) {
+      i = nbr[p0 + m - 1];
+      const int g = group_of(w, i, B, n_up, N - n_up);
+      row = gd.row(g) + 5 + 3 * (int64_t)rev[p0 + m - 1];
+      ep = (int64_t)g * 5 * ND;
+    }
+    jb[t] = O + row * ldo + cbase + kg * 4;
+    sc[t] = ep >= 0;
+    eb[t] = env + (ep >= 0 ? ep : 0) + cbase + kg * 4;
+    bok[t] = ok;
+    xb[t] = AinvT + ((size_t)(w * D + d) * N + i) * N + kg * 4 - off;
+  }
+  double tr = 0.0, gx[3] = {0.0, 0.0, 0.0};
+  for (int x0 = 0; x0 < 3; x0 += XG) {
+    dbl4_t acc[XG][NAT][NAT];
+#pragma unroll
+    for (int xi = 0; xi < XG; ++xi)
+#pragma unroll
+      for (int i = 0; i < NAT; ++i)
+#pragma unroll
+        for (int j = 0; j < NAT; ++j) acc[xi][i][j] = (dbl4_t){0.0, 0.0, 0.0, 0.0};
+    for (int k = 0; k < nch; ++k) {
+      const int o = k * 16;
+      const bool okc = o + kg * 4 < ncol;
+      float4_t jv[XG][NAT], ev[NAT];
+      double xv[NAT][4];
+#pragma unroll
+      for (int t = 0; t < NAT; ++t) {
+#pragma unroll
+        for (int xi = 0; xi < XG; ++xi)
+          jv[xi][t] = okc ? *reinterpret_cast<const float4_t*>(jb[t] + (int64_t)(x0 + xi) * ldo + o) : (float4_t)(0.f);
+        ev[t] = okc && sc[t] ? *reinterpret_cast<const float4_t*>(eb[t] + o) : (float4_t)(1.f);
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int cs = o + kg * 4 - off + 2 * h;
+          const dbl2_t v = bok[t] && cs >= 0 && cs < N ? *reinterpret_cast<const dbl2_t*>(xb[t] + o + 2 * h)
+                                                       : (dbl2_t)(0.0);
+          xv[t][2 * h] = v[0];
+          xv[t][2 * h + 1] = v[1];
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int xi = 0; xi < XG; ++xi)
+#pragma unroll
+          for (int i = 0; i < NAT; ++i) {
+            const double a = (double)jv[xi][i][q] * (double)ev[i][q];
+#pragma unroll
+            for (int j = 0; j < NAT; ++j)
+              acc[xi][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, xv[j][q], acc[xi][i][j], 0, 0, 0);
+          }
+    }
+#pragma unroll
+    for (int xi = 0; xi < XG; ++xi) {
+      double g = 0.0;
+#pragma unroll
+      for (int i = 0; i < NAT; ++i) {
+#pragma unroll
+        for (int j = i; j < NAT; ++j) {
+          // tile (j,i) into LDS as [row][col], then this lane's element (4v + kg, li) of tile
+          // (i,j) meets element (li, 4v + kg) of tile (j,i)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) tb[(4 * v + kg) * 17 + li] = acc[xi][j][i][v];
+          __builtin_amdgcn_wave_barrier();
+          double p = 0.0;
+#pragma unroll
+          for (int v = 0; v < 4; ++v) p += acc[xi][i][j][v] * tb[li * 17 + 4 * v + kg];
+          __builtin_amdgcn_wave_barrier();
+          tr += (i == j ? 1.0 : 2.0) * p;
+          if (i == j) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+              if (li == 4 * v + kg) g += acc[xi][i][i][v];
+          }
+        }
+      }
+      const int x = x0 + xi;
+      if (x == 0) gx[0] += g;
+      else if (x == 1) gx[1] += g;
+      else gx[2] += g;
+    }
+  }
+  tr = warp_sum(tr);
+#pragma unroll
+  for (int x = 0; x < 3; ++x) gx[x] = warp_sum(gx[x]);
+  if (lane == 0) {
+    const int64_t o = ((int64_t)w * D + d) * N + n;
+    trip[o] = tr;
+#pragma unroll
+    for (int x = 0; x < 3; ++x) grad[o * 3 + x] = gx[x];
+  }
+}
+
+template <int NAT, int XG, int WPE>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE)))
+k_det_lap64(const float* __restrict__ O, int ldo, int ND, GroupDesc gd, const float* __restrict__ env,
+            const int64_t* __restrict__ ptr, const int32_t* __restrict__ nbr, const int32_t* __restrict__ rev, int B,
+            int N, int n_up, int D, const double* __restrict__ AinvT, double* __restrict__ trip,
+            double* __restrict__ grad) {
+  __shared__ double tbs[4][16 * 17];
+  const int wid = threadIdx.x >> 6;
+  const int64_t units = (int64_t)B * N * D;
+  const int64_t nblk = (units + 3) / 4, G8 = (nblk + 7) >> 3;
+  const int64_t lb = (int64_t)(blockIdx.x & 7) * G8 + (blockIdx.x >> 3);
+  const int64_t u = lb * 4 + wid;
+  if (lb >= nblk || u >= units) return;
+  const int d = (int)(u % D);
+  const int wn = (int)(u / D);
+  const int w = wn / N, n = wn - w * N;
+  const int gn = group_of(w, n, B, n_up, N - n_up);
+  const int64_t p0 = ptr[gn];
+  const int s = 1 + (int)(ptr[gn + 1] - p0);
+  if (((s + 15) >> 4) != NAT) return;
+  det_lap_unit64<NAT, XG>(O, ldo, ND, gd, env, nbr, rev, B, N, n_up, D, AinvT, trip, grad, w, n, d, gn, p0, s,
+                          tbs[wid]);
+}
+
+// FP64 transposed copy of n_mat N x N matrices (32 x 32 tiles through LDS).
+__global__ void k_transpose_f64(const double* __restrict__ X, int N, double* __restrict__ XT) {
+  __shared__ double t[32][33];
+  const size_t base = (size_t)blockIdx.z * N * N;
+  const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+  for (int j = threadIdx.y; j < 32; j += 8) {
+    const int r = by + j, c = bx + threadIdx.x;
+    if (r < N && c < N) t[j][threadIdx.x] = X[base + (size_t)r * N + c];
+  }
+  __syncthreads();
+  for (int j = threadIdx.y; j < 32; j += 8) {
+    const int r = bx + j, c = by + threadIdx.x;
+    if (r < N && c < N) XT[base + (size_t)r * N + c] = t[threadIdx.x][j];
+  }
+}
+
+void launch_transpose_f64(hipStream_t s, const double* X, int n_mat, int N, double* XT) {
+  if (n_mat == 0) return;
+  const dim3 grid((N + 31) / 32, (N + 31) / 32, n_mat);
+  k_transpose_f64<<<grid, dim3(32, 8), 0, s>>>(X, N, XT);
+}
+
+bool det_lap64_supported(int N, int ND, int ldo) { return N % 2 == 0 && ND % 4 == 0 && ldo % 32 == 0; }
+
+void launch_det_lap64(hipStream_t st, const float* O, int ldo, int ND, GroupDesc gd, const float* env,
+                      const int64_t* ptr, const int32_t* nbr, const int32_t* rev, int B, int N, int n_up, int D,
+                      int smax, const double* AinvT, double* trip, double* grad) {
+  if (B * N == 0) return;
+  const int64_t nblk = ((int64_t)B * N * D + 3) / 4;
+  const dim3 grid((unsigned)(8 * ((nblk + 7) / 8)));
+  const int natm = (smax + 15) / 16;
+  k_det_lap64<1, 3, 2><<<grid, 256, 0, st>>>(O, ldo, ND, gd, env, ptr, nbr, rev, B, N, n_up, D, AinvT, trip, grad);
+  if (natm >= 2)
+    k_det_lap64<2, 1, 2><<<grid, 256, 0, st>>>(O, ldo, ND, gd, env, ptr, nbr, rev, B, N, n_up, D, AinvT, trip, grad);
+  if (natm >= 3)
+    k_det_lap64<3, 1, 1><<<grid, 256, 0, st>>>(O, ldo, ND, gd, env, ptr, nbr, rev, B, N, n_up, D, AinvT, trip, grad);
+  if (natm >= 4)
+    k_det_lap64<4, 1, 1><<<grid, 256, 0, st>>>(O, ldo, ND, gd, env, ptr, nbr, rev, B, N, n_up, D, AinvT, trip, grad);
+}
+
 // Sum of the determinants: log|psi_det|, sign, gradient and Laplacian per walker (FP64).
 //   w_d = s_d exp(L_d - m) / sum;  grad = sum_d w_d g_d;
 //   lap = sum_d w_d (lap_d + |g_d|^2) - |grad|^2,  lap_d = tr_d - sum_n trip_{d,n}.

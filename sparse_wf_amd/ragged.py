@@ -52,11 +52,14 @@ def random_geometries(n: int, seed: int, n_el=(10, 200), walkers: int = 256, fea
 
 
 def run_ragged(geoms: List[Geometry], sweeps: int, width: float = 0.3, precision: str = "mixed",
-               use_graphs: bool = True) -> dict:
+               use_graphs: bool = True, local_energy: bool = False, threads: int = 8) -> dict:
     """Sample every geometry for ``sweeps`` sweeps (after one warm-up sweep that also captures
     the graphs) with all geometries in flight at once; returns throughput and per-geometry
     acceptance.  With ``use_graphs=False`` the same steps are issued eagerly, one geometry
-    after the other on the default stream (the reference point)."""
+    after the other on the default stream (the reference point).  With ``local_energy`` the
+    final walkers of every geometry are then evaluated (local energy, each geometry on its own
+    stream from a pool of ``threads`` host threads, so the evaluations overlap on the device);
+    the result carries ``e_loc`` per geometry and the evaluation throughput."""
     wfs, streams, nacc, states = [], [], [], []
     for g in geoms:
         wf = FireWaveFunction(g.cfg, precision=precision)
@@ -89,6 +92,24 @@ def run_ragged(geoms: List[Geometry], sweeps: int, width: float = 0.3, precision
     for wf in wfs:
         r, _, _ = wf.state_values()  # also reports a capacity overflow as an error
         walkers_out.append(r)
-    return {"geometries": len(geoms), "sweeps": sweeps, "seconds": dt, "walker_moves_per_s": moves / dt,
-            "electrons": [g.cfg.n_el for g in geoms], "pmove": pmove, "walkers": walkers_out,
-            "graphs": use_graphs}
+    out = {"geometries": len(geoms), "sweeps": sweeps, "seconds": dt, "walker_moves_per_s": moves / dt,
+           "electrons": [g.cfg.n_el for g in geoms], "pmove": pmove, "walkers": walkers_out,
+           "graphs": use_graphs}
+    if local_energy:
+        from concurrent.futures import ThreadPoolExecutor
+
+        def evaluate(k):
+            with torch.cuda.stream(streams[k]):
+                e = wfs[k].local_energy_terms(geoms[k].params, walkers_out[k], precision=precision)["e_loc"]
+                streams[k].synchronize()
+            return e
+
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(max_workers=max(1, threads)) as ex:
+            e_loc = list(ex.map(evaluate, range(len(geoms))))
+        dt_e = time.perf_counter() - t0
+        out["e_loc"] = e_loc
+        out["local_energy_seconds"] = dt_e
+        out["local_energy_walkers_per_s"] = sum(len(g.walkers) for g in geoms) / dt_e
+    return out

@@ -450,6 +450,10 @@ class FireWaveFunction:
         _lib.check(self._lib.fire_precision_plan(self._h, arr, len(planes), cur))
         return list(cur)
 
+    def det_options(self, fp64_laplacian: bool):
+        """Determinant-Laplacian products in FP64 (True) or on an FP32 copy of the inverse."""
+        _lib.check(self._lib.fire_det_options(self._h, int(bool(fp64_laplacian))))
+
     def mcmc_options(self, use_graphs: bool = True):
         """Replay whole sweeps from a captured hipGraph (default) or issue every step eagerly;
         both give identical results."""

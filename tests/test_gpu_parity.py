@@ -185,3 +185,33 @@ def test_fused_silu_epilogue_matches_separate_pass(gpu_device, precision, monkey
     g = _rel(fused["grad"].cpu().numpy(), sep["grad"].cpu().numpy())
     print(f"  {precision} grad rel {g:.2e}")
     assert g < tol
+
+
+@pytest.mark.parametrize("precision", ["fp32", "mixed"])
+def test_local_energy_fp64_laplacian(setup, precision):
+    """The determinant Laplacian's triplet products in FP64 (fire_det_options) against the
+    oracle, at the bounds of the FP32 form; the printed errors of both forms show what the FP64
+    products remove."""
+    name, cfg, prm, wf, r = setup
+    pt = fire.to_torch(prm)
+    outs = {}
+    try:
+        for fp64 in (False, True):
+            wf.det_options(fp64)
+            outs[fp64] = wf.local_energy_terms(prm, r, precision=precision, with_derivatives=True)
+    finally:
+        wf.det_options(False)
+    tol_lp, tol_g, a, c = KINETIC_BOUNDS[precision]
+    for w in range(len(r)):
+        _, lp = fire.log_psi_fwd_lap(pt, cfg, r[w])
+        ek_ref = float(-0.5 * (lp.l + (lp.j * lp.j).sum()))
+        d32 = float(outs[False]["e_kin"][w]) - ek_ref
+        d64 = float(outs[True]["e_kin"][w]) - ek_ref
+        g = out_g = _rel(outs[True]["grad"][w].cpu().numpy().reshape(-1), lp.j.numpy())
+        print(f"  {precision} walker {w}: E_kin error FP32 products {d32 * 1e3:+.4f} mHa, FP64 products "
+              f"{d64 * 1e3:+.4f} mHa (lap {float(lp.l):.0f}); grad rel {g:.2e}")
+        if precision == "fp32" and name == "alkane130_dense":
+            tol_g = 3e-4
+        assert abs(d64) < a + c * abs(float(lp.l))
+        assert out_g < tol_g
+        np.testing.assert_array_equal(outs[True]["logpsi"][w].cpu().numpy(), outs[False]["logpsi"][w].cpu().numpy())
