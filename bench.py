@@ -327,8 +327,8 @@ def main():
             "walkers_identical_graph_vs_eager": bool(same),
             "local_energy_walkers_per_s": g_run["local_energy_walkers_per_s"],
             "local_energy_seconds": g_run["local_energy_seconds"],
-            "local_energy_what": "mixed local energy of every geometry's final walkers, each geometry on its own "
-                                 "stream from 8 host threads",
+            "local_energy_what": "mixed local energy of every geometry's final walkers: two host threads, each "
+                                 "with one evaluation handle and stream retargeted per geometry",
         }
         log(f"ragged leg: {g_run['walker_moves_per_s']:.3g} moves/s (graphs), "
             f"{e_run['walker_moves_per_s']:.3g} (eager)")

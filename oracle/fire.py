@@ -1026,3 +1026,30 @@ def log_psi_low_rank_update(p, cfg, r_new32, idx, state, method="woodbury"):
         determinant=dict(matrices=phi, inverses=inv, signs=sl, logdets=ld), jastrow=jst,
         r32=np.asarray(r_new32, np.float32).copy())
     return (sgn * sJ, lp + lJ), new_state, aff
+
+
+# -----------------------------------------------------------------------------------------
+# parameter gradient of log|psi| (the per-walker jacobian of preconditioner.py:186-190)
+# -----------------------------------------------------------------------------------------
+def grad_logpsi(params, cfg, r32):
+    """d log|psi| / d theta for every leaf of the parameter tree (numpy FP64 params): the
+    jax.grad(log_p) of make_dense_spring_preconditioner (preconditioner.py:186-190) for one
+    walker, by torch autograd of :func:`signed` in FP64.  Returns {leaf path: array}, paths as
+    '/embedding/elec_init/params/...' in the tree's order; leaves that do not reach log|psi|
+    get zeros."""
+    leaves, paths = [], []
+
+    def mark(t, path):
+        if isinstance(t, dict):
+            return {k: mark(v, f"{path}/{k}") for k, v in t.items()}
+        if isinstance(t, (list, tuple)):
+            return [mark(v, f"{path}/{i}") for i, v in enumerate(t)]
+        x = torch.as_tensor(np.asarray(t, np.float64)).clone().requires_grad_(True)
+        leaves.append(x)
+        paths.append(path)
+        return x
+
+    pt = mark(params, "")
+    _, lp = signed(pt, cfg, r32)
+    gs = torch.autograd.grad(lp, leaves, allow_unused=True)
+    return {p: (np.zeros(tuple(x.shape)) if g is None else g.detach().numpy()) for p, x, g in zip(paths, leaves, gs)}

@@ -473,7 +473,7 @@ void gemm(Ctx& c, const float* A, int64_t lda, int wi, int bi, float* C, int64_t
       HIPCHK(hipEventCreate(&t0));
       HIPCHK(hipEventCreate(&t1));
       for (int v = 0; v < nv; ++v) {
-        if ((ep.rowgrp && !gemm_bf16_fusable(v)) || (ns == 4 && !gemm_bf16_f16_ok(v))) {
+        if ((ep.rowgrp && !gemm_bf16_fusable(v)) || (ns == 4 && !gemm_bf16_f16_ok(v)) || gemm_bf16_spills(v, ns)) {
           ms[v] = 1e30f;
           continue;
         }

@@ -130,6 +130,7 @@ int gemm_bf16_variants();
 bool gemm_bf16_fusable(int variant);
 // variants that have an FP16-plane form (nsplit 4)
 bool gemm_bf16_f16_ok(int variant);
+bool gemm_bf16_spills(int variant, int nsplit);
 void gemm_bf16_tile(int variant, int* bm, int* bn);
 
 // pseudopotential tables on the device (radial grids in FP64)

@@ -1357,6 +1357,15 @@ static const Bf16Tile kBf16Tiles[] = {{128, 128}, {128, 256}, {256, 128}, {64, 2
 // forms that carry FP16 planes (the others are offered to BF16 planes only)
 bool gemm_bf16_f16_ok(int variant) { return variant < 3 || variant > 17; }
 
+// forms whose registers spill to scratch for this plane count (hipcc resource report): the
+// 256 x 128 wide-wavefront tile (four float4 A loads per thread and chunk) at every plane
+// count, the A-in-registers and 128 x 256 wide-wavefront forms with three planes.  They are
+// not offered: a spilling kernel is slower, and every stream running it reserves scratch.
+bool gemm_bf16_spills(int variant, int nsplit) {
+  if (variant == 25) return true;
+  return nsplit == 6 && (variant == 15 || variant == 24);
+}
+
 bool gemm_bf16_fusable(int variant) { return variant == 0 || variant == 2 || variant == 18 || variant == 20 || variant == 23; }
 
 int gemm_bf16_variants() { return (int)(sizeof(kBf16Tiles) / sizeof(kBf16Tiles[0])); }
@@ -1489,7 +1498,7 @@ static void launch_np(hipStream_t s, int variant, const float* A, int64_t lda, c
       case 22: launch_tile_dq<NP, 128, 128, 16, 6, false, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
       case 23: launch_tile_dw<NP, 256, 256, 16, 4, false, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
       case 24: launch_tile_dw<NP, 128, 256, 16, 4, false, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
-      case 25: launch_tile_dw<NP, 256, 128, 16, 4, false, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 25: launch_tile_dw<NP, 256, 128, 16, 2, false, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
       case 1: launch_tile<NP, 128, 256, false, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
       case 2: launch_tile<NP, 256, 128, false, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
       default: launch_tile<NP, 128, 128, false, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
@@ -1515,7 +1524,7 @@ static void launch_np(hipStream_t s, int variant, const float* A, int64_t lda, c
       case 22: launch_tile_dq<NP, 128, 128, 16, 6>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
       case 23: launch_tile_dw<NP, 256, 256, 16, 4>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
       case 24: launch_tile_dw<NP, 128, 256, 16, 4>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
-      case 25: launch_tile_dw<NP, 256, 128, 16, 4>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 25: launch_tile_dw<NP, 256, 128, 16, 2>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
       case 1: launch_tile<NP, 128, 256>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
       case 2: launch_tile<NP, 256, 128>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
       case 3: launch_tile<NP, 64, 256>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;

@@ -52,14 +52,17 @@ def random_geometries(n: int, seed: int, n_el=(10, 200), walkers: int = 256, fea
 
 
 def run_ragged(geoms: List[Geometry], sweeps: int, width: float = 0.3, precision: str = "mixed",
-               use_graphs: bool = True, local_energy: bool = False, threads: int = 8) -> dict:
+               use_graphs: bool = True, local_energy: bool = False, threads: int = 2) -> dict:
     """Sample every geometry for ``sweeps`` sweeps (after one warm-up sweep that also captures
     the graphs) with all geometries in flight at once; returns throughput and per-geometry
     acceptance.  With ``use_graphs=False`` the same steps are issued eagerly, one geometry
     after the other on the default stream (the reference point).  With ``local_energy`` the
-    final walkers of every geometry are then evaluated (local energy, each geometry on its own
-    stream from a pool of ``threads`` host threads, so the evaluations overlap on the device);
-    the result carries ``e_loc`` per geometry and the evaluation throughput."""
+    final walkers of every geometry are then evaluated: ``threads`` host threads, each with its
+    own evaluation handle and stream, take the geometries in turn and retarget their handle to
+    each (``fire_reconfigure`` keeps the device workspace, so the local-energy workspace held
+    is that of ``threads`` handles, not one per geometry), and the evaluations of different
+    threads overlap on the device; the result carries ``e_loc`` per geometry and the
+    evaluation throughput."""
     wfs, streams, nacc, states = [], [], [], []
     for g in geoms:
         wf = FireWaveFunction(g.cfg, precision=precision)
@@ -96,19 +99,47 @@ def run_ragged(geoms: List[Geometry], sweeps: int, width: float = 0.3, precision
            "electrons": [g.cfg.n_el for g in geoms], "pmove": pmove, "walkers": walkers_out,
            "graphs": use_graphs}
     if local_energy:
-        from concurrent.futures import ThreadPoolExecutor
+        import queue
+        import threading
 
-        def evaluate(k):
-            with torch.cuda.stream(streams[k]):
-                e = wfs[k].local_energy_terms(geoms[k].params, walkers_out[k], precision=precision)["e_loc"]
-                streams[k].synchronize()
-            return e
+        todo = queue.Queue()
+        for k in sorted(range(len(geoms)), key=lambda k: -geoms[k].cfg.n_el):
+            todo.put(k)
+        e_loc = [None] * len(geoms)
+        errors = []
+
+        def worker():
+            s = torch.cuda.Stream()
+            wf = None
+            try:
+                while True:
+                    try:
+                        k = todo.get_nowait()
+                    except queue.Empty:
+                        return
+                    g = geoms[k]
+                    if wf is None:
+                        wf = FireWaveFunction(g.cfg, precision=precision)
+                    else:
+                        wf.reconfigure(g.cfg)
+                    with torch.cuda.stream(s):
+                        e_loc[k] = wf.local_energy_terms(g.params, walkers_out[k], precision=precision)["e_loc"]
+                    s.synchronize()
+            except Exception as e:  # reported after the pool has drained
+                errors.append(e)
+            finally:
+                del wf
 
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        with ThreadPoolExecutor(max_workers=max(1, threads)) as ex:
-            e_loc = list(ex.map(evaluate, range(len(geoms))))
+        pool = [threading.Thread(target=worker) for _ in range(max(1, threads))]
+        for t in pool:
+            t.start()
+        for t in pool:
+            t.join()
         dt_e = time.perf_counter() - t0
+        if errors:
+            raise errors[0]
         out["e_loc"] = e_loc
         out["local_energy_seconds"] = dt_e
         out["local_energy_walkers_per_s"] = sum(len(g.walkers) for g in geoms) / dt_e

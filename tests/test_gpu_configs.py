@@ -91,7 +91,7 @@ def test_ragged_real_shapes(gpu_device):
     geoms = random_geometries(4, seed=17, n_el=(10, 200), walkers=64, feature_dim=256)
     geoms.sort(key=lambda g: g.cfg.n_el)
     print("\n[ragged real shapes] N:", [g.cfg.n_el for g in geoms], flush=True)
-    res = run_ragged(geoms, sweeps=1, use_graphs=True, local_energy=True, threads=4)
+    res = run_ragged(geoms, sweeps=1, use_graphs=True, local_energy=True, threads=2)
     print(f"  local energy of all geometries: {res['local_energy_walkers_per_s']:.0f} walkers/s", flush=True)
     for g, rw, el in zip(geoms, res["walkers"], res["e_loc"]):
         rr = rw[:2].cpu().numpy()
