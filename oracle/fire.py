@@ -1029,11 +1029,11 @@ def log_psi_low_rank_update(p, cfg, r_new32, idx, state, method="woodbury"):
 
 
 # -----------------------------------------------------------------------------------------
-# parameter gradient of log|psi| (the per-walker jacobian of preconditioner.py:186-190)
+# parameter gradient of log|psi| (the per-walker jacobian of preconditioner.py:191-194)
 # -----------------------------------------------------------------------------------------
 def grad_logpsi(params, cfg, r32):
     """d log|psi| / d theta for every leaf of the parameter tree (numpy FP64 params): the
-    jax.grad(log_p) of make_dense_spring_preconditioner (preconditioner.py:186-190) for one
+    jax.grad(log_p) of make_dense_spring_preconditioner (preconditioner.py:191-194) for one
     walker, by torch autograd of :func:`signed` in FP64.  Returns {leaf path: array}, paths as
     '/embedding/elec_init/params/...' in the tree's order; leaves that do not reach log|psi|
     get zeros."""

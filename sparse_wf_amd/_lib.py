@@ -62,6 +62,8 @@ SIGNATURES = {
                                        ctypes.c_int32, _P, _P]),
     "fire_ecp_energy": (ctypes.c_int, [_P, _P, ctypes.c_uint64, _P, _P, ctypes.POINTER(ctypes.c_int64)]),
     "fire_energy_stats": (ctypes.c_int, [_P, _P, _P, ctypes.c_int32, ctypes.c_double, ctypes.c_int32, _P, _P, _P]),
+    "fire_grad_logpsi": (ctypes.c_int, [_P, _P, _P, ctypes.c_int32, _P, _P]),
+    "fire_param_count": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int64)]),
     "fire_last_error": (ctypes.c_char_p, []),
 }
 

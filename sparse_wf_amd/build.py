@@ -9,7 +9,7 @@ from pathlib import Path
 HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 LIB = HERE / "libfire_amd.so"
-SOURCES = ["k_graph.hip", "k_embed.hip", "k_gemm.hip", "k_orbdet.hip", "k_jastrow.hip", "k_mcmc.hip", "k_stats.hip", "k_ecp.hip", "fire_api.hip"]
+SOURCES = ["k_graph.hip", "k_embed.hip", "k_gemm.hip", "k_orbdet.hip", "k_jastrow.hip", "k_mcmc.hip", "k_stats.hip", "k_ecp.hip", "k_grad.hip", "fire_api.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",

@@ -18,6 +18,7 @@
 #include "fire_kernels.hpp"
 #include "fire_kernels2.hpp"
 #include "fire_mcmc.hpp"
+#include "fire_grad.hpp"
 
 static constexpr int64_t kScratchInt64 = (int64_t)1 << 21;  // 16 MiB
 static __device__ __align__(256) int64_t scratch[kScratchInt64];
@@ -215,6 +216,9 @@ struct fire_handle {
   int64_t* scratch = nullptr;  // this device's static scratch (fire_neighbours_noalloc)
   // pseudopotentials (fire_ecp_upload): radial tables and quadrature directions on the device,
   // and the integration points of the last fire_ecp_energy
+  // parameter gradient (fire_grad_logpsi): per-chunk forward intermediates and cotangents
+  static constexpr int kGradSlots = 128;
+  DevBuf grw[kGradSlots];
   int ecp_n = 0, ecp_nl = 0, ecp_ng = 0, ecp_qmax = 0;
   DevBuf ecp_atom, ecp_grid, ecp_vloc, ecp_vnl, ecp_cut, ecp_nq, ecp_qpts, ecp_pts, ecp_coef, ecp_kq, ecp_npts;
   // optional HIP-event timing of the dense contractions (fire_profile_gemm)
@@ -1425,7 +1429,7 @@ void check_config(const fire_config& c) {
 
 // Every workspace buffer of the handle outside the walker state.
 std::vector<DevBuf*> handle_bufs(fire_handle* h) {
-  return {&h->cnt,    &h->ptr,     &h->nbr,    &h->rev,     &h->rowflag, &h->T5,     &h->Z2,      &h->T5b,
+  std::vector<DevBuf*> v{&h->cnt,    &h->ptr,     &h->nbr,    &h->rev,     &h->rowflag, &h->T5,     &h->Z2,      &h->T5b,
           &h->M5,     &h->XA,      &h->XB,     &h->XC,      &h->O,       &h->ENV,    &h->ENVC,    &h->A64,
           &h->Ainv,   &h->AinvT,   &h->ldet,   &h->dsgn,    &h->tr,      &h->trip,   &h->dgrad,   &h->det_lp,
           &h->det_sg, &h->det_gr,  &h->det_lap, &h->JR,     &h->mlp_ns,  &h->mlp_dg, &h->QV,      &h->AV,
@@ -1433,6 +1437,8 @@ std::vector<DevBuf*> handle_bufs(fire_handle* h) {
           &h->cgrad,  &h->epot,    &h->PF,     &h->EF,      &h->EOK,     &h->rowgrp, &h->AV0,     &h->Z0,
           &h->F1,     &h->PART,    &h->AMX_A,  &h->AMX_B,   &h->AMX_C,  &h->ecp_atom, &h->ecp_grid, &h->ecp_vloc,
           &h->ecp_vnl, &h->ecp_cut, &h->ecp_nq, &h->ecp_qpts, &h->ecp_pts, &h->ecp_coef, &h->ecp_kq, &h->ecp_npts};
+  for (DevBuf& b : h->grw) v.push_back(&b);
+  return v;
 }
 
 void free_params(fire_handle* h) {
@@ -1496,6 +1502,381 @@ void apply_config(fire_handle* h, const fire_config& c, const float* R, const fl
   build_expect(h);
 }
 
+
+// Parameter gradient of log|psi| for a chunk of Bc walkers (value path, FP32 forward with FP64
+// determinants): gout [Bc][n_total] w.r.t. the packed tensors, logpsi [Bc] (optional).
+void run_grad(Ctx& c, const float* r, int Bc, float* gout, int64_t gstride, const std::vector<int64_t>& off,
+              double* logpsi_out) {
+  fire_handle* h = c.h;
+  hipStream_t s = c.s;
+  const int N = h->N, nu = h->n_up, nd = h->n_dn, F = h->F, M = h->M, D = h->D, Ei = h->Eiso, ND = h->ND;
+  const int W = h->W, depth = h->depth, H = h->H, Dk = h->Dk, HD = h->HD, w1 = h->w1, w0 = h->w0;
+  const int64_t G = (int64_t)Bc * N, rowsEN = G * M;
+  // one thread per feature in the LayerNorm kernels, per attention channel in the head kernels
+  REQUIRE(F <= 256, "gradient: feature_dim above 256");
+  REQUIRE(!h->has_att || (HD <= 256 && Dk <= 64 && (Dk & (Dk - 1)) == 0),
+          "gradient: attention heads x dim above 256 or a dim that is not a power of two");
+  int nb = 0;
+  auto buf = [&](size_t bytes) -> void* {
+    REQUIRE(nb < fire_handle::kGradSlots, "gradient workspace slots exhausted");
+    DevBuf& b = h->grw[nb++];
+    b.ensure(std::max<size_t>(bytes, 256));
+    return b.p;
+  };
+  auto fb = [&](int64_t n) { return (float*)buf(sizeof(float) * (size_t)n); };
+  auto g32 = [&](const float* A, int64_t lda, int wi, int bi, float* C, int64_t ldc, int64_t Mr, int Nc, int K) {
+    launch_gemm_f32(s, A, lda, h->P[wi], h->Pt32[wi], bi >= 0 ? h->P[bi] : nullptr, C, ldc, Mr, Nc, K, nullptr, 1);
+  };
+  // C = A W^T for a weight W stored [K_w][N_w] row-major: the transposed product takes W itself
+  // as its [N][K] operand
+  // (products with at most 16 columns run on the small-N kernel, which reads the [K][N] operand:
+  // the weight is transposed into a workspace slot of its own first)
+  float* wsmall = nullptr;
+  int64_t wsmall_n = 0;
+  auto g32t = [&](const float* A, int64_t lda, const float* Wrow, float* C, int64_t ldc, int64_t Mr, int Nc, int K) {
+    const float* Wk = nullptr;
+    if (Nc <= 16) {
+      REQUIRE((int64_t)Nc * K <= wsmall_n, "gradient: small-product workspace too small");
+      launch_transpose2d(s, Wrow, Nc, K, wsmall);
+      Wk = wsmall;
+    }
+    launch_gemm_f32(s, A, lda, Wk, Wrow, nullptr, C, ldc, Mr, Nc, K, nullptr, 1);
+  };
+  wsmall_n = std::max<int64_t>({(int64_t)M * Ei * N, (int64_t)F * W, (int64_t)W * W, 2 * (int64_t)W, 16 * (int64_t)F});
+  wsmall = fb(wsmall_n);
+  launch_fill_f32(s, gout, (int64_t)Bc * gstride, 0.f);
+  const WRows both{nullptr, 0, nu, nu, (int64_t)Bc * nu, nd, nd};
+  const WRows up{nullptr, 0, nu, nu, 0, 0, 0};
+  const WRows dn{nullptr, (int64_t)Bc * nu, nd, nd, 0, 0, 0};
+  auto wg = [&](const float* X, int64_t ldx, const float* Y, int64_t ldy, int K, int J, const WRows& wr, int64_t o,
+                const uint8_t* mask = nullptr, int want = 1) {
+    launch_wgrad(s, X, ldx, Y, ldy, K, J, Bc, wr, gout + o, gstride, false, mask, want);
+  };
+
+  // ---- forward ---------------------------------------------------------------------------
+  Graph gr = build_graph(c, r, Bc);
+  const int64_t P = gr.n_pairs;
+  const EnParams ep = en_params(h);
+  float* en_rec = fb(rowsEN * PR_FLOATS);
+  float* en_beta = fb(rowsEN * w1);
+  float* en_lsf = fb(rowsEN * 4);
+  uint8_t* en_mask = (uint8_t*)buf((size_t)rowsEN);
+  launch_gr_en_fwd(s, r, h->R, Bc, N, nu, M, h->cfg.cutoff_1el, ep, rowsEN, en_rec, en_beta, en_lsf, en_mask);
+  float* hen = fb(G * F);
+  launch_gr_en_sum(s, en_beta, en_lsf, ep, en_mask, G, M, F, hen);
+  float* y1 = fb(G * F);
+  float* lnn = fb(G * F);
+  float* lninv = fb(G);
+  launch_gr_ln_fwd(s, hen, G, F, ep.ln_g, ep.ln_b, y1, lnn, lninv);
+  const int b = h->iEN;
+  float* z2 = fb(G * 2 * F);
+  g32(y1, F, b + 11, -1, z2, 2 * F, G, 2 * F, F);
+  float* y2 = fb(G * F);
+  launch_gr_glu_fwd(s, z2, G, F, y2);
+  float* z3 = fb(G * F);
+  g32(y2, F, b + 12, b + 13, z3, F, G, F, F);
+  float* y3 = fb(G * F);
+  launch_silu_fwd(s, y3, z3, G * F);
+  float* m5 = fb(G * 5 * F);
+  g32(y3, F, b + 14, b + 15, m5, 5 * F, G, 5 * F, F);
+  const EdgeParams eps = edge_params(h, h->iEEs), epd = edge_params(h, h->iEEd);
+  float* hc = fb(G * F);
+  launch_messages(s, r, Bc, N, nu, h->ptr.as<int64_t>(), h->nbr.as<int32_t>(), m5, eps, epd, h->cut_learned, F, false,
+                  hc);
+  const int m = h->iMLP;
+  float* zu1 = fb(G * F);
+  g32(hc, F, m + 0, m + 1, zu1, F, G, F, F);
+  float* au1 = fb(G * F);
+  launch_silu_fwd(s, au1, zu1, G * F);
+  float* zu2 = fb(G * F);
+  g32(au1, F, m + 2, m + 3, zu2, F, G, F, F);
+  float* x = fb(G * F);
+  launch_silu_fwd(s, x, zu2, G * F);
+  // orbitals, envelopes, determinants
+  const int o = h->iORB;
+  const int64_t Gu = (int64_t)Bc * nu, Gd = (int64_t)Bc * nd;
+  float* orb = fb(G * ND);
+  g32(x, F, o + 0, o + 1, orb, ND, Gu, ND, F);
+  if (Gd) g32(x + Gu * F, F, o + 2, o + 3, orb + Gu * ND, ND, Gd, ND, F);
+  const int DME = D * M * Ei;
+  float* ex = fb(G * DME);
+  float* env = fb(G * ND);
+  float* dist = fb(G * M);
+  launch_gr_env(s, r, h->R, Bc, N, nu, M, D, Ei, h->P[o + 4], h->P[o + 5], h->P[o + 6], h->P[o + 7], ex, env, dist);
+  const int64_t nmat = (int64_t)Bc * D;
+  double* A64 = (double*)buf(sizeof(double) * (size_t)nmat * N * N);
+  double* Ainv = (double*)buf(sizeof(double) * (size_t)nmat * N * N);
+  double* ldet = (double*)buf(sizeof(double) * (size_t)nmat);
+  double* dsgn = (double*)buf(sizeof(double) * (size_t)nmat);
+  launch_gr_phi(s, orb, env, Bc, N, nu, D, A64);
+  const bool lu_lds = lu_lds_bytes(N) <= 160 * 1024;
+  if (lu_lds && lu_lds_bytes(N) > 64 * 1024 && !h->lu_smem_set) {
+    HIPCHK(lu_set_smem_limit(lu_lds_bytes(N)));
+    h->lu_smem_set = true;
+  }
+  launch_lu_inv(s, A64, (int)nmat, N, Ainv, nullptr, ldet, dsgn, true, lu_lds);
+  // Jastrow MLP
+  const int jm = h->iJM;
+  std::vector<float*> jz(depth + 1, nullptr), ja(depth, nullptr);
+  if (h->has_mlp) {
+    const float* in = x;
+    int fan = F;
+    for (int l = 0; l < depth; ++l) {
+      jz[l] = fb(G * W);
+      ja[l] = fb(G * W);
+      g32(in, fan, jm + 2 * l, jm + 2 * l + 1, jz[l], W, G, W, fan);
+      launch_silu_fwd(s, ja[l], jz[l], G * W);
+      in = ja[l];
+      fan = W;
+    }
+    jz[depth] = fb(G * 2);
+    g32(in, W, jm + 2 * depth, jm + 2 * depth + 1, jz[depth], 2, G, 2, W);
+  }
+  // attention projections [q | v] from the packed [W_q | W_v]
+  const int at = h->iATT;
+  float* qv = nullptr;
+  if (h->has_att) {
+    float* qvT = fb((int64_t)2 * HD * F);
+    launch_transpose2d(s, h->P[at], F, 2 * HD, qvT);
+    qv = fb(G * 2 * HD);
+    launch_gemm_f32(s, x, F, h->P[at], qvT, h->P[at + 1], qv, 2 * HD, G, 2 * HD, F, nullptr, 1);
+  }
+
+  // ---- per walker: determinants, Jastrow, attention readout, cusp -----------------------------
+  double* wdet = (double*)buf(sizeof(double) * (size_t)nmat);
+  double* lp = logpsi_out ? logpsi_out : (double*)buf(sizeof(double) * (size_t)Bc);
+  float* dJ = fb((int64_t)Bc * 2);
+  float* dval = fb((int64_t)Bc * 2 * std::max(HD, 1));
+  float* dnorm = fb((int64_t)Bc * 2 * std::max(H, 1));
+  GradHead ha{};
+  ha.B = Bc;
+  ha.N = N;
+  ha.nu = nu;
+  ha.D = D;
+  ha.W = W;
+  ha.depth = depth;
+  ha.H = H;
+  ha.Dk = Dk;
+  ha.HD = HD;
+  ha.lwmax = std::max(2 * HD, W);
+  ha.has_mlp = h->has_mlp;
+  ha.use_mlp = h->cfg.use_mlp_jastrow;
+  ha.use_log = h->cfg.use_log_jastrow;
+  ha.has_att = h->has_att;
+  ha.cusp = h->cusp;
+  ha.ldet = ldet;
+  ha.dsgn = dsgn;
+  ha.z3 = jz[depth];
+  if (h->has_mlp) {
+    ha.mlp_scale = h->P[jm + 2 * depth + 2];
+    ha.mlp_zb = h->P[jm + 2 * depth + 3];
+    ha.o_mlp_scale = off[jm + 2 * depth + 2];
+    ha.o_mlp_zb = off[jm + 2 * depth + 3];
+  }
+  if (h->has_att) {
+    REQUIRE(depth + 1 <= 8, "gradient: at most 7 hidden readout layers");
+    ha.qv = qv;
+    ha.keys = h->P[at + 2];
+    for (int l = 0; l <= depth; ++l) {
+      ha.ro_W[l] = h->P[at + 3 + 2 * l];
+      ha.ro_b[l] = h->P[at + 4 + 2 * l];
+      ha.o_ro_W[l] = off[at + 3 + 2 * l];
+      ha.o_ro_b[l] = off[at + 4 + 2 * l];
+    }
+    ha.att_scale = h->P[at + 5 + 2 * depth];
+    ha.att_zb = h->P[at + 6 + 2 * depth];
+    ha.o_att_scale = off[at + 5 + 2 * depth];
+    ha.o_att_zb = off[at + 6 + 2 * depth];
+  }
+  if (h->cusp) {
+    ha.cusp_p = h->P[h->iCUSP];
+    ha.o_cusp = off[h->iCUSP];
+  }
+  ha.r = r;
+  ha.wdet = wdet;
+  ha.logpsi = lp;
+  ha.dJ = dJ;
+  ha.dval = dval;
+  ha.dnorm = dnorm;
+  ha.gout = gout;
+  ha.gstride = gstride;
+  REQUIRE(gr_head_lds(ha) <= 64 * 1024, "gradient: readout layers too wide for one workgroup's LDS");
+  launch_gr_head(s, ha);
+  // ---- per row cotangents --------------------------------------------------------------------
+  float* dorb = fb(G * ND);
+  float* denv = fb(G * ND);
+  float* dz3 = h->has_mlp ? fb(G * 2) : nullptr;
+  float* dqv = h->has_att ? fb(G * 2 * HD) : nullptr;
+  float* dkey = h->has_att ? fb(G * HD) : nullptr;
+  GradRows ra{};
+  ra.B = Bc;
+  ra.N = N;
+  ra.nu = nu;
+  ra.D = D;
+  ra.H = H;
+  ra.Dk = Dk;
+  ra.HD = HD;
+  ra.has_mlp = h->has_mlp;
+  ra.has_att = h->has_att;
+  ra.wdet = wdet;
+  ra.Ainv = Ainv;
+  ra.orb = orb;
+  ra.env = env;
+  ra.z3 = jz[depth];
+  ra.dJ = dJ;
+  ra.mlp_scale = ha.mlp_scale;
+  ra.qv = qv;
+  ra.keys = ha.keys;
+  ra.dval = dval;
+  ra.dnorm = dnorm;
+  ra.dorb = dorb;
+  ra.denv = denv;
+  ra.dz3 = dz3;
+  ra.dqv = dqv;
+  ra.dkey = dkey;
+  launch_gr_rows(s, ra, G);
+
+  // ---- orbitals and envelopes --------------------------------------------------------------
+  wg(x, F, dorb, ND, F, ND, up, off[o + 0]);
+  wg(nullptr, 0, dorb, ND, 1, ND, up, off[o + 1]);
+  if (nd) {
+    wg(x, F, dorb, ND, F, ND, dn, off[o + 2]);
+    wg(nullptr, 0, dorb, ND, 1, ND, dn, off[o + 3]);
+  }
+  const int ME = M * Ei;
+  float* tt = fb(G * DME);
+  for (int d = 0; d < D; ++d) {
+    wg(ex + (int64_t)d * ME, DME, denv + (int64_t)d * N, ND, ME, N, up, off[o + 5] + (int64_t)d * ME * N);
+    g32t(denv + (int64_t)d * N, ND, h->P[o + 5] + (int64_t)d * ME * N, tt + (int64_t)d * ME, DME, Gu, ME, N);
+    if (nd) {
+      wg(ex + (int64_t)d * ME, DME, denv + (int64_t)d * N, ND, ME, N, dn, off[o + 7] + (int64_t)d * ME * N);
+      g32t(denv + Gu * ND + (int64_t)d * N, ND, h->P[o + 7] + (int64_t)d * ME * N, tt + Gu * DME + (int64_t)d * ME,
+           DME, Gd, ME, N);
+    }
+  }
+  launch_gr_sp(s, ex, tt, dist, Bc, N, nu, M, D, Ei, gout, gstride, off[o + 4], off[o + 6]);
+  // d x from the orbitals
+  float* dx = fb(G * F);
+  g32t(dorb, ND, h->P[o + 0], dx, F, Gu, F, ND);
+  if (nd) g32t(dorb + Gu * ND, ND, h->P[o + 2], dx + Gu * F, F, Gd, F, ND);
+  float* tmpF = fb(G * std::max(F, W));
+  float* tmpF2 = fb(G * std::max(F, W));
+  // ---- MLP Jastrow -----------------------------------------------------------------------
+  if (h->has_mlp) {
+    wg(ja[depth - 1], W, dz3, 2, W, 2, both, off[jm + 2 * depth]);
+    wg(nullptr, 0, dz3, 2, 1, 2, both, off[jm + 2 * depth + 1]);
+    float* da = tmpF;
+    g32t(dz3, 2, h->P[jm + 2 * depth], da, W, G, W, 2);
+    for (int l = depth - 1; l >= 0; --l) {
+      const float* in = l == 0 ? x : ja[l - 1];
+      const int fan = l == 0 ? F : W;
+      float* dz = tmpF2;
+      launch_silu_bwd(s, dz, W, da, W, jz[l], W, G, W);
+      wg(in, fan, dz, W, fan, W, both, off[jm + 2 * l]);
+      wg(nullptr, 0, dz, W, 1, W, both, off[jm + 2 * l + 1]);
+      g32t(dz, W, h->P[jm + 2 * l], da, fan, G, fan, W);
+    }
+    launch_axpy(s, dx, da, G * F);
+  }
+  // ---- attention -----------------------------------------------------------------------
+  if (h->has_att) {
+    wg(x, F, dqv, 2 * HD, F, 2 * HD, both, off[at + 0]);
+    wg(nullptr, 0, dqv, 2 * HD, 1, 2 * HD, both, off[at + 1]);
+    wg(nullptr, 0, dkey, HD, 1, HD, both, off[at + 2]);
+    g32t(dqv, 2 * HD, h->P[at], tmpF, F, G, F, 2 * HD);
+    launch_axpy(s, dx, tmpF, G * F);
+  }
+  // ---- update MLP --------------------------------------------------------------------------
+  float* dz = tmpF;
+  launch_silu_bwd(s, dz, F, dx, F, zu2, F, G, F);
+  wg(au1, F, dz, F, F, F, both, off[m + 2]);
+  wg(nullptr, 0, dz, F, 1, F, both, off[m + 3]);
+  g32t(dz, F, h->P[m + 2], tmpF2, F, G, F, F);
+  launch_silu_bwd(s, dz, F, tmpF2, F, zu1, F, G, F);
+  wg(hc, F, dz, F, F, F, both, off[m + 0]);
+  wg(nullptr, 0, dz, F, 1, F, both, off[m + 1]);
+  float* dhc = fb(G * F);
+  g32t(dz, F, h->P[m + 0], dhc, F, G, F, F);
+
+  // ---- messages and e-e edges -------------------------------------------------------------------
+  float* dm5 = fb(G * 5 * F);
+  HIPCHK(hipMemcpy2DAsync(dm5, sizeof(float) * 5 * F, dhc, sizeof(float) * F, sizeof(float) * F, (size_t)G,
+                          hipMemcpyDeviceToDevice, s));
+  if (P > 0) {
+    float* prec = fb(P * PR_FLOATS);
+    float* pbeta = fb(P * w1);
+    float* plsf = fb(P * 4);
+    uint8_t* psame = (uint8_t*)buf((size_t)P);
+    int32_t* pctr = (int32_t*)buf(sizeof(int32_t) * (size_t)P);
+    launch_gr_pair_fwd(s, r, Bc, N, nu, h->ptr.as<int64_t>(), h->nbr.as<int32_t>(), eps, epd, h->cut_learned, P, prec,
+                       pbeta, plsf, psame, pctr);
+    float* pdgam = fb(P * F);
+    float* pdu = fb(P * F);
+    launch_gr_msg_bwd(s, pbeta, plsf, eps, epd, m5, dhc, pctr, h->nbr.as<int32_t>(), psame, Bc, N, nu, F, P, pdgam, pdu);
+    launch_gr_msg_collect(s, pdu, h->ptr.as<int64_t>(), h->nbr.as<int32_t>(), h->rev.as<int32_t>(), psame, Bc, N, nu, F,
+                          dm5);
+    int64_t* prng = (int64_t*)buf(sizeof(int64_t) * 4 * (size_t)Bc);
+    launch_pair_ranges(s, h->ptr.as<int64_t>(), Bc, nu, nd, prng);
+    WRows pr;
+    pr.rng = prng;
+    float* pdbeta = fb(P * w1);
+    launch_gr_dbeta(s, pdgam, F, eps.WG, epd.WG, psame, w1, P, pdbeta);
+    float* ppg = fb(P * PG_FLOATS);
+    launch_gr_pair_bwd(s, prec, pdbeta, psame, eps, epd, h->cut_learned, P, ppg);
+    for (int k = 0; k < 2; ++k) {
+      const int base = k == 0 ? h->iEEs : h->iEEd;
+      const int want = k == 0 ? 1 : 0;
+      wg(prec + PR_FEAT, PR_FLOATS, ppg + PG_DPRE, PG_FLOATS, 4, w0, pr, off[base + 0], psame, want);
+      wg(nullptr, 0, ppg + PG_DPRE, PG_FLOATS, 1, w0, pr, off[base + 1], psame, want);
+      wg(nullptr, 0, ppg + PG_DSP, PG_FLOATS, 1, w0, pr, off[base + 2], psame, want);
+      wg(ppg + PG_Y, PG_FLOATS, ppg + PG_DDIR, PG_FLOATS, w0, w1, pr, off[base + 3], psame, want);
+      wg(nullptr, 0, ppg + PG_DDIR, PG_FLOATS, 1, w1, pr, off[base + 4], psame, want);
+      wg(prec + PR_ENV, PR_FLOATS, ppg + PG_DENVD, PG_FLOATS, w0, w1, pr, off[base + 5], psame, want);
+      wg(pbeta, w1, pdgam, F, w1, F, pr, off[base + 6], psame, want);
+      wg(plsf, 4, pdu, F, 4, F, pr, off[base + 7], psame, want);
+      wg(nullptr, 0, pdu, F, 1, F, pr, off[base + 8], psame, want);
+    }
+    // learned cutoff (the first entry of the last packed tensor)
+    wg(nullptr, 0, ppg + PG_DCUT, PG_FLOATS, 1, 1, pr, off[h->iSCAL]);
+  }
+  // ---- ElecInit dense stage ------------------------------------------------------------------
+  wg(y3, F, dm5, 5 * F, F, 5 * F, both, off[b + 14]);
+  wg(nullptr, 0, dm5, 5 * F, 1, 5 * F, both, off[b + 15]);
+  float* dy = tmpF;
+  g32t(dm5, 5 * F, h->P[b + 14], dy, F, G, F, 5 * F);
+  float* dz3e = tmpF2;
+  launch_silu_bwd(s, dz3e, F, dy, F, z3, F, G, F);
+  wg(y2, F, dz3e, F, F, F, both, off[b + 12]);
+  wg(nullptr, 0, dz3e, F, 1, F, both, off[b + 13]);
+  g32t(dz3e, F, h->P[b + 12], dy, F, G, F, F);
+  float* dz2 = fb(G * 2 * F);
+  launch_gr_glu_bwd(s, dy, z2, G, F, dz2);
+  wg(y1, F, dz2, 2 * F, F, 2 * F, both, off[b + 11]);
+  float* dy1 = tmpF2;
+  g32t(dz2, 2 * F, h->P[b + 11], dy1, F, G, F, 2 * F);
+  launch_mul(s, tmpF, dy1, lnn, G * F);
+  wg(nullptr, 0, tmpF, F, 1, F, both, off[b + 9]);
+  wg(nullptr, 0, dy1, F, 1, F, both, off[b + 10]);
+  float* dhen = fb(G * F);
+  launch_gr_ln_bwd(s, dy1, lnn, lninv, ep.ln_g, G, F, dhen);
+  // ---- e-n edges -----------------------------------------------------------------------------
+  float* edgam = fb(rowsEN * F);
+  float* ede = fb(rowsEN * F);
+  launch_gr_en_bwd_f(s, en_beta, en_lsf, ep, en_mask, dhen, rowsEN, M, F, edgam, ede);
+  const WRows en{nullptr, 0, (int64_t)nu * M, nu * M, (int64_t)Bc * nu * M, (int64_t)nd * M, nd * M};
+  wg(en_beta, w1, edgam, F, w1, F, en, off[b + 7]);
+  wg(en_lsf, 4, ede, F, 4, F, en, off[b + 8]);
+  float* edbeta = fb(rowsEN * w1);
+  launch_gr_dbeta(s, edgam, F, ep.WG, ep.WG, nullptr, w1, rowsEN, edbeta);
+  float* epg = fb(rowsEN * PG_FLOATS);
+  launch_gr_en_bwd(s, en_rec, edbeta, ep, h->cfg.cutoff_1el, rowsEN, M, epg);
+  wg(epg + PG_Y, PG_FLOATS, epg + PG_DDIR, PG_FLOATS, w0, w1, en, off[b + 4]);
+  wg(nullptr, 0, epg + PG_DDIR, PG_FLOATS, 1, w1, en, off[b + 5]);
+  wg(epg + PG_ENVR, PG_FLOATS, epg + PG_DENVD, PG_FLOATS, ep.E, w1, en, off[b + 6]);
+  launch_gr_en_nuc(s, en_rec, epg, ede, Bc, N, nu, M, F, ep, gout, gstride, off[b + 0], off[b + 1], off[b + 2],
+                   off[b + 3]);
+  HIPCHK(hipGetLastError());
+}
 }  // namespace
 
 extern "C" {
@@ -2191,6 +2572,45 @@ int fire_det_options(fire_handle* h, int32_t fp64_laplacian) {
     REQUIRE(h, "null handle");
     REQUIRE(fp64_laplacian == 0 || fp64_laplacian == 1, "fire_det_options: fp64_laplacian must be 0 or 1");
     h->det_lap64 = fp64_laplacian != 0;
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e);
+  }
+}
+
+int fire_grad_logpsi(fire_handle* h, void* stream, const float* r_dev, int32_t B, float* grad_dev,
+                     double* logpsi_dev) {
+  try {
+    check_ready(h);
+    REQUIRE(r_dev && grad_dev, "null argument");
+    REQUIRE(B >= 0, "negative batch");
+    HIPCHK(hipSetDevice(h->device));
+    Ctx c{h, (hipStream_t)stream, FIRE_FP32};
+    std::vector<int64_t> off(h->expect.size() + 1, 0);
+    for (size_t k = 0; k < h->expect.size(); ++k) off[k + 1] = off[k] + h->expect[k];
+    const int64_t n_total = off.back();
+    const int N = h->N;
+    // chunks bounded by the e-n edge rows (each carries about 2 F + 350 floats of workspace)
+    const int64_t per_walker = (int64_t)N * h->M * (2 * h->F + 360) * 4;
+    const int chunk = (int)std::max<int64_t>(1, std::min<int64_t>(128, (int64_t)24000000000LL / per_walker));
+    for (int w0 = 0; w0 < B; w0 += chunk) {
+      const int Bc = std::min(chunk, B - w0);
+      run_grad(c, r_dev + (size_t)w0 * N * 3, Bc, grad_dev + (size_t)w0 * n_total, n_total, off,
+               logpsi_dev ? logpsi_dev + w0 : nullptr);
+    }
+    HIPCHK(hipStreamSynchronize(c.s));
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e);
+  }
+}
+
+int fire_param_count(fire_handle* h, int64_t* n_floats) {
+  try {
+    REQUIRE(h && n_floats, "null argument");
+    int64_t n = 0;
+    for (int64_t v : h->expect) n += v;
+    *n_floats = n;
     return 0;
   } catch (const std::exception& e) {
     return fail(e);

@@ -1,0 +1,1098 @@
+// Parameter gradient of log|psi| per walker (value path): the jax.grad(log_p) whose batch the
+// natural-gradient preconditioner contracts (src/sparse_wf/preconditioner.py:191-194).
+//
+// The gradient is taken with respect to the packed parameter tensors of fire_upload_params;
+// the host maps it onto the reference's parameter tree (softplus, folded normalisation scales,
+// padded filter widths, spin-restricted columns).  The forward intermediates come from the
+// value path in FP32 and from the recomputations below; every per-walker sum over electrons,
+// pairs or e-n edges is a fixed-order loop (no atomics), so results are deterministic.
+//
+// Row layouts: per-electron rows in group order (up-spin electrons of all walkers first,
+// walker-major within a spin); e-e pair rows in the CSR order of the neighbour lists; e-n edge
+// rows as group * M + nucleus.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fire_common.hpp"
+#include "fire_kernels.hpp"
+#include "fire_grad.hpp"
+
+namespace fire {
+
+namespace {
+
+__device__ __forceinline__ float sigm(float x) { return 1.0f / (1.0f + expf(-x)); }
+__device__ __forceinline__ float silu_f(float x) { return x * sigm(x); }
+__device__ __forceinline__ float silu_d(float x) {
+  const float s = sigm(x);
+  return s * (1.0f + x * (1.0f - s));
+}
+
+// cutoff polynomial 1 - 15 x^4 + 24 x^5 - 10 x^6 (x < 1) and its derivative
+__device__ __forceinline__ void cutoff_poly(float x, float& f, float& df) {
+  if (x >= 1.0f) {
+    f = 0.f;
+    df = 0.f;
+    return;
+  }
+  const float x2 = x * x, x3 = x2 * x, x4 = x2 * x2;
+  f = 1.0f - 15.0f * x4 + 24.0f * x4 * x - 10.0f * x4 * x2;
+  df = -60.0f * x3 + 120.0f * x4 - 60.0f * x4 * x;
+}
+
+__device__ __forceinline__ void group_walker(int64_t g, int B, int nu, int nd, int& w, int& i) {
+  const int64_t up = (int64_t)B * nu;
+  if (g < up) {
+    w = (int)(g / nu);
+    i = (int)(g - (int64_t)w * nu);
+  } else {
+    const int64_t q = g - up;
+    w = (int)(q / nd);
+    i = nu + (int)(q - (int64_t)w * nd);
+  }
+}
+
+__device__ __forceinline__ int64_t group_index(int w, int i, int B, int nu, int nd) {
+  return i < nu ? (int64_t)w * nu + i : (int64_t)B * nu + (int64_t)w * nd + (i - nu);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------
+// out[b][k][j] (+)= sum over the rows of walker b of X[row][k] Y[row][j] (X == nullptr: 1).
+// Rows of walker b: [rng[4b], rng[4b+1]) and [rng[4b+2], rng[4b+3]), or without rng the fixed
+// ranges a0 + b as .. + na and c0 + b cs .. + nc.  rowmask (optional): only rows with
+// rowmask[row] == want.  One block per (64 x 64 output tile, walker); 16 rows per LDS chunk.
+// ---------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_wgrad(const float* __restrict__ X, int64_t ldx, const float* __restrict__ Y,
+                                               int64_t ldy, int K, int J, WRows wr, const uint8_t* __restrict__ rowmask,
+                                               int want, float* __restrict__ out, int64_t so, int accumulate) {
+  __shared__ float xs[16][64 + 1];
+  __shared__ float ys[16][64 + 1];
+  const int b = blockIdx.y;
+  const int tk = (K + 63) / 64;
+  const int k0 = (blockIdx.x % tk) * 64, j0 = (blockIdx.x / tk) * 64;
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  int64_t s[2], e[2];
+  if (wr.rng) {
+    s[0] = wr.rng[4 * b];
+    e[0] = wr.rng[4 * b + 1];
+    s[1] = wr.rng[4 * b + 2];
+    e[1] = wr.rng[4 * b + 3];
+  } else {
+    s[0] = wr.a0 + (int64_t)b * wr.as;
+    e[0] = s[0] + wr.na;
+    s[1] = wr.c0 + (int64_t)b * wr.cs;
+    e[1] = s[1] + wr.nc;
+  }
+  float acc[4][4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) acc[u][v] = 0.f;
+  for (int part = 0; part < 2; ++part) {
+    for (int64_t r0 = s[part]; r0 < e[part]; r0 += 16) {
+      for (int t = threadIdx.x; t < 16 * 64; t += 256) {
+        const int rr = t >> 6, cc = t & 63;
+        const int64_t row = r0 + rr;
+        bool ok = row < e[part];
+        if (ok && rowmask) ok = rowmask[row] == want;
+        const int k = k0 + cc, j = j0 + cc;
+        xs[rr][cc] = ok && k < K ? (X ? X[row * ldx + k] : 1.f) : 0.f;
+        ys[rr][cc] = ok && j < J ? Y[row * ldy + j] : 0.f;
+      }
+      __syncthreads();
+#pragma unroll 4
+      for (int rr = 0; rr < 16; ++rr) {
+        float xv[4], yv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) xv[u] = xs[rr][ty * 4 + u];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) yv[v] = ys[rr][tx * 4 + v];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) acc[u][v] += xv[u] * yv[v];
+      }
+      __syncthreads();
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int k = k0 + ty * 4 + u, j = j0 + tx * 4 + v;
+      if (k < K && j < J) {
+        float* o = out + (int64_t)b * so + (int64_t)k * J + j;
+        *o = accumulate ? *o + acc[u][v] : acc[u][v];
+      }
+    }
+}
+
+void launch_wgrad(hipStream_t s, const float* X, int64_t ldx, const float* Y, int64_t ldy, int K, int J, int B,
+                  const WRows& wr, float* out, int64_t so, bool accumulate, const uint8_t* rowmask, int want) {
+  if (B == 0 || K == 0 || J == 0) return;
+  const dim3 grid((unsigned)(((K + 63) / 64) * ((J + 63) / 64)), (unsigned)B);
+  k_wgrad<<<grid, 256, 0, s>>>(X, ldx, Y, ldy, K, J, wr, rowmask, want, out, so, accumulate ? 1 : 0);
+}
+
+// ---------------------------------------------------------------------------------------
+// element-wise helpers
+// ---------------------------------------------------------------------------------------
+__global__ void k_silu_bwd(float* __restrict__ dz, const float* __restrict__ dy, int64_t ldy, const float* __restrict__ z,
+                           int64_t ldz, int64_t rows, int cols, int64_t ldo) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= rows * cols) return;
+  const int64_t r = e / cols;
+  const int c = (int)(e - r * cols);
+  dz[r * ldo + c] = dy[r * ldy + c] * silu_d(z[r * ldz + c]);
+}
+
+void launch_silu_bwd(hipStream_t s, float* dz, int64_t ldo, const float* dy, int64_t ldy, const float* z, int64_t ldz,
+                     int64_t rows, int cols) {
+  const int64_t n = rows * cols;
+  if (n == 0) return;
+  k_silu_bwd<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(dz, dy, ldy, z, ldz, rows, cols, ldo);
+}
+
+__global__ void k_silu_fwd(float* __restrict__ y, const float* __restrict__ z, int64_t n) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) y[e] = silu_f(z[e]);
+}
+
+void launch_silu_fwd(hipStream_t s, float* y, const float* z, int64_t n) {
+  if (n) k_silu_fwd<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(y, z, n);
+}
+
+__global__ void k_axpy(float* __restrict__ a, const float* __restrict__ b, int64_t n) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) a[e] += b[e];
+}
+
+void launch_axpy(hipStream_t s, float* a, const float* b, int64_t n) {
+  if (n) k_axpy<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(a, b, n);
+}
+
+__global__ void k_fill_f32(float* __restrict__ a, int64_t n, float v) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) a[e] = v;
+}
+
+void launch_fill_f32(hipStream_t s, float* a, int64_t n, float v) {
+  if (n) k_fill_f32<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(a, n, v);
+}
+
+__global__ void k_mul(float* __restrict__ o, const float* __restrict__ a, const float* __restrict__ b, int64_t n) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) o[e] = a[e] * b[e];
+}
+
+void launch_mul(hipStream_t s, float* o, const float* a, const float* b, int64_t n) {
+  if (n) k_mul<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(o, a, b, n);
+}
+
+__global__ void k_transpose2d(const float* __restrict__ A, int R, int C, float* __restrict__ AT) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)R * C) return;
+  const int r = (int)(e / C), c = (int)(e - (int64_t)r * C);
+  AT[(int64_t)c * R + r] = A[e];
+}
+
+void launch_transpose2d(hipStream_t s, const float* A, int R, int C, float* AT) {
+  const int64_t n = (int64_t)R * C;
+  if (n) k_transpose2d<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(A, R, C, AT);
+}
+
+// Per-walker row ranges of CSR pairs: rng[4w..4w+3] = the pair slots of the walker's up-spin
+// and down-spin centres.
+__global__ void k_pair_ranges(const int64_t* __restrict__ ptr, int B, int nu, int nd, int64_t* __restrict__ rng) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= B) return;
+  const int64_t up = (int64_t)B * nu;
+  rng[4 * w + 0] = ptr[(int64_t)w * nu];
+  rng[4 * w + 1] = ptr[(int64_t)(w + 1) * nu];
+  rng[4 * w + 2] = ptr[up + (int64_t)w * nd];
+  rng[4 * w + 3] = ptr[up + (int64_t)(w + 1) * nd];
+}
+
+void launch_pair_ranges(hipStream_t s, const int64_t* ptr, int B, int nu, int nd, int64_t* rng) {
+  if (B) k_pair_ranges<<<(B + 255) / 256, 256, 0, s>>>(ptr, B, nu, nd, rng);
+}
+
+// ---------------------------------------------------------------------------------------
+// envelopes and Slater matrices
+// ---------------------------------------------------------------------------------------
+// Per group row g: ex[g][d][m][e] = exp(-sp_s[m][d][e] |r - R_m|) and
+// env[g][d N + o] = sum_{m,e} pi_s[d][m E + e][o] ex[g][d][m][e] (s = spin of the electron).
+__global__ void __launch_bounds__(256) k_gr_env(const float* __restrict__ r, const float* __restrict__ R, int B, int N,
+                                               int nu, int M, int D, int E, const float* __restrict__ sp_up,
+                                               const float* __restrict__ pi_up, const float* __restrict__ sp_dn,
+                                               const float* __restrict__ pi_dn, float* __restrict__ ex,
+                                               float* __restrict__ env, float* __restrict__ dist) {
+  extern __shared__ float gx[];  // [D][M][E]
+  const int64_t g = blockIdx.x;
+  int w, i;
+  group_walker(g, B, nu, N - nu, w, i);
+  const bool up = i < nu;
+  const float* sp = up ? sp_up : sp_dn;
+  const float* pi = up ? pi_up : pi_dn;
+  const float* ri = r + ((int64_t)w * N + i) * 3;
+  const int DME = D * M * E;
+  for (int t = threadIdx.x; t < DME; t += blockDim.x) {
+    const int d = t / (M * E), me = t - d * M * E, m = me / E, e = me - m * E;
+    const float dx = ri[0] - R[3 * m], dy = ri[1] - R[3 * m + 1], dz = ri[2] - R[3 * m + 2];
+    const float dm = sqrtf(dx * dx + dy * dy + dz * dz);
+    const float x = expf(-sp[((int64_t)m * D + d) * E + e] * dm);
+    gx[t] = x;
+    ex[g * DME + t] = x;
+    if (d == 0 && e == 0) dist[g * M + m] = dm;
+  }
+  __syncthreads();
+  const int ND = D * N;
+  for (int c = threadIdx.x; c < ND; c += blockDim.x) {
+    const int d = c / N, o = c - d * N;
+    const float* p = pi + (int64_t)d * M * E * N + o;
+    const float* xg = gx + d * M * E;
+    float acc = 0.f;
+    for (int me = 0; me < M * E; ++me) acc += p[(int64_t)me * N] * xg[me];
+    env[g * ND + c] = acc;
+  }
+}
+
+void launch_gr_env(hipStream_t s, const float* r, const float* R, int B, int N, int nu, int M, int D, int E,
+                   const float* sp_up, const float* pi_up, const float* sp_dn, const float* pi_dn, float* ex,
+                   float* env, float* dist) {
+  const int64_t G = (int64_t)B * N;
+  if (G == 0) return;
+  k_gr_env<<<(unsigned)G, 256, sizeof(float) * D * M * E, s>>>(r, R, B, N, nu, M, D, E, sp_up, pi_up, sp_dn, pi_dn,
+                                                                ex, env, dist);
+}
+
+// A[w][d][i][o] = orb[g][d N + o] env[g][d N + o] in FP64
+__global__ void k_gr_phi(const float* __restrict__ orb, const float* __restrict__ env, int B, int N, int nu, int D,
+                         double* __restrict__ A) {
+  const int64_t g = blockIdx.x;
+  int w, i;
+  group_walker(g, B, nu, N - nu, w, i);
+  const int ND = D * N;
+  for (int c = threadIdx.x; c < ND; c += blockDim.x) {
+    const int d = c / N, o = c - d * N;
+    A[(((int64_t)w * D + d) * N + i) * N + o] = (double)orb[g * ND + c] * (double)env[g * ND + c];
+  }
+}
+
+void launch_gr_phi(hipStream_t s, const float* orb, const float* env, int B, int N, int nu, int D, double* A) {
+  if (B * N) k_gr_phi<<<(unsigned)((int64_t)B * N), 256, 0, s>>>(orb, env, B, N, nu, D, A);
+}
+
+// ---------------------------------------------------------------------------------------
+// Per walker: determinant weights, Jastrow sums, attention readout forward and backward,
+// e-e cusp; writes the readout / scale / cusp parameter gradients and the per-walker
+// cotangents the row kernel needs.
+// ---------------------------------------------------------------------------------------
+__device__ float block_sum_f(float v, float* red) {
+  v = warp_sum(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  float t = 0.f;
+  for (int k = 0; k < (int)(blockDim.x >> 6); ++k) t += red[k];
+  return t;
+}
+
+__device__ double block_sum_d(double v, double* red) {
+  v = warp_sum(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  double t = 0.0;
+  for (int k = 0; k < (int)(blockDim.x >> 6); ++k) t += red[k];
+  return t;
+}
+
+__global__ void __launch_bounds__(256) k_gr_head(GradHead a) {
+  extern __shared__ float hs[];
+  __shared__ float red[8];
+  __shared__ double redd[8];
+  __shared__ float sJ[4];
+  const int w = blockIdx.x, t = threadIdx.x;
+  const int N = a.N, nu = a.nu, nd = N - nu, B = a.B;
+  // determinants: weights w_d = s_d exp(L_d - m) / sum
+  if (t == 0) {
+    double m = -1e300;
+    for (int d = 0; d < a.D; ++d) m = fmax(m, a.ldet[w * a.D + d]);
+    double sum = 0.0;
+    for (int d = 0; d < a.D; ++d) sum += a.dsgn[w * a.D + d] * exp(a.ldet[w * a.D + d] - m);
+    for (int d = 0; d < a.D; ++d) a.wdet[w * a.D + d] = a.dsgn[w * a.D + d] * exp(a.ldet[w * a.D + d] - m) / sum;
+    a.logpsi[w] = m + log(fabs(sum));
+  }
+  float J[2] = {0.f, 0.f};
+  float x3s[2] = {0.f, 0.f};
+  // MLP Jastrow: sum_i silu(z3_i) * scale + N zb
+  if (a.has_mlp) {
+    for (int k = 0; k < 2; ++k) {
+      float v = 0.f;
+      for (int i = t; i < N; i += blockDim.x) v += silu_f(a.z3[group_index(w, i, B, nu, nd) * 2 + k]);
+      x3s[k] = block_sum_f(v, red);
+      J[k] += x3s[k] * a.mlp_scale[k] + (float)N * a.mlp_zb[k];
+    }
+  }
+  // attention: per-spin sums of exp(-q.key/sqrt(Dk)) and of its products with the values
+  float* u = hs;                        // [2 HD] normalised values (readout input)
+  float* nrm = u + 2 * a.HD;            // [2 H]
+  float* buf0 = nrm + 2 * a.H;          // [max(2HD, W)] layer activations
+  float* zst = buf0 + a.lwmax;          // [depth + 2][lwmax]: raw value sums, then the readout pre-activations
+  float* dbuf = zst + (a.depth + 2) * a.lwmax;  // [lwmax] cotangent
+  float* dbuf2 = dbuf + a.lwmax;
+  const int HD = a.HD, Dk = a.Dk;
+  const float rs = rsqrtf((float)Dk);
+  if (a.has_att) {
+    if (t < HD) {
+      const int h = t / Dk, k = t - h * Dk;
+      const float key = a.keys[t];
+      for (int sp = 0; sp < 2; ++sp) {
+        float nv = 0.f, vv = 0.f;
+        const int i0 = sp == 0 ? 0 : nu, i1 = sp == 0 ? nu : N;
+        for (int i = i0; i < i1; ++i) {
+          const float* row = a.qv + group_index(w, i, B, nu, nd) * 2 * HD;
+          float qk = row[t] * key;
+          for (int o = Dk >> 1; o > 0; o >>= 1) qk += __shfl_xor(qk, o, Dk);
+          const float at = expf(-qk * rs);
+          nv += at;
+          vv += at * row[HD + t];
+        }
+        u[sp * HD + t] = vv;  // values sums; normalised below
+        if (k == 0) nrm[sp * a.H + h] = nv;
+      }
+    }
+    __syncthreads();
+    for (int c = t; c < 2 * HD; c += blockDim.x) {
+      const int sp = c / HD, h = (c - sp * HD) / Dk;
+      zst[c] = u[c];  // keep the raw value sums in the first zst row while the layers run
+      u[c] = u[c] / nrm[sp * a.H + h];
+    }
+    __syncthreads();
+    // readout MLP forward: layers l = 0 .. depth (silu on hidden layers)
+    int fan = 2 * HD;
+    const float* in = u;
+    for (int l = 0; l <= a.depth; ++l) {
+      const int nout = l < a.depth ? a.W : 2;
+      const float* Wl = a.ro_W[l];
+      const float* bl = a.ro_b[l];
+      float* zl = zst + (l + 1) * a.lwmax;
+      if (nout > 2) {
+        for (int j = t; j < nout; j += blockDim.x) {
+          float acc = bl[j];
+          for (int q = 0; q < fan; ++q) acc += in[q] * Wl[(int64_t)q * nout + j];
+          zl[j] = acc;
+        }
+      } else {
+        for (int j = 0; j < 2; ++j) {
+          float v = 0.f;
+          for (int q = t; q < fan; q += blockDim.x) v += in[q] * Wl[(int64_t)q * 2 + j];
+          v = block_sum_f(v, red);
+          if (t == 0) zl[j] = v + bl[j];
+        }
+      }
+      __syncthreads();
+      if (l < a.depth) {
+        for (int j = t; j < nout; j += blockDim.x) buf0[j + (l & 1) * 0] = silu_f(zl[j]);
+        __syncthreads();
+        // copy the activation next to its pre-activation for the backward pass
+        for (int j = t; j < nout; j += blockDim.x) dbuf2[j] = buf0[j];
+        __syncthreads();
+        in = dbuf2;
+        // keep activations of layer l in zst's spare row? recomputed in the backward from zl
+      }
+      fan = nout;
+    }
+    __syncthreads();
+    for (int k = 0; k < 2; ++k) J[k] += zst[(a.depth + 1) * a.lwmax + k] * a.att_scale[k] + a.att_zb[k];
+  }
+  // e-e cusp value and its parameter derivatives
+  double cusp = 0.0;
+  if (a.cusp) {
+    double v[6] = {0, 0, 0, 0, 0, 0};  // sum_same q, sum_diff q, d/dalpha_same, d/dalpha_diff
+    const float* rw = a.r + (int64_t)w * N * 3;
+    const double als = a.cusp_p[0], ald = a.cusp_p[1];
+    const int64_t npair = (int64_t)N * (N - 1) / 2;
+    for (int64_t p = t; p < npair; p += blockDim.x) {
+      // p -> (i, j), i < j
+      int i = (int)((2.0 * N - 1 - sqrt((2.0 * N - 1) * (2.0 * N - 1) - 8.0 * (double)p)) / 2);
+      while ((int64_t)(i + 1) * (2 * N - i - 2) / 2 <= p) ++i;
+      while ((int64_t)i * (2 * N - i - 1) / 2 > p) --i;
+      const int j = (int)(p - (int64_t)i * (2 * N - i - 1) / 2) + i + 1;
+      const double dx = (double)rw[3 * i] - rw[3 * j], dy = (double)rw[3 * i + 1] - rw[3 * j + 1],
+                   dz = (double)rw[3 * i + 2] - rw[3 * j + 2];
+      const double d = sqrt(dx * dx + dy * dy + dz * dz);
+      const bool same = (i < nu) == (j < nu);
+      const double al = same ? als : ald;
+      const double q = al * al / (al + d);
+      const double dq = (al * al + 2.0 * al * d) / ((al + d) * (al + d));
+      v[same ? 0 : 1] += q;
+      v[same ? 2 : 3] += dq;
+    }
+    for (int k = 0; k < 4; ++k) v[k] = block_sum_d(v[k], redd);
+    cusp = a.cusp_p[2] * v[0] + a.cusp_p[3] * v[1];
+    if (t == 0) {
+      float* gc = a.gout + (int64_t)w * a.gstride + a.o_cusp;
+      gc[0] = (float)(a.cusp_p[2] * v[2]);
+      gc[1] = (float)(a.cusp_p[3] * v[3]);
+      gc[2] = (float)v[0];
+      gc[3] = (float)v[1];
+    }
+  }
+  // log psi of the Jastrow and its cotangent dJ
+  if (t == 0) {
+    float dJ0 = a.use_mlp ? 1.f : 0.f, dJ1 = a.use_log ? 1.f / J[1] : 0.f;
+    sJ[0] = dJ0;
+    sJ[1] = dJ1;
+    a.dJ[2 * w] = dJ0;
+    a.dJ[2 * w + 1] = dJ1;
+    a.logpsi[w] += cusp + (a.use_mlp ? J[0] : 0.f) + (a.use_log ? logf(fabsf(J[1])) : 0.f);
+    if (a.has_mlp) {
+      float* g = a.gout + (int64_t)w * a.gstride;
+      g[a.o_mlp_scale] = x3s[0] * dJ0;
+      g[a.o_mlp_scale + 1] = x3s[1] * dJ1;
+      g[a.o_mlp_zb] = (float)N * dJ0;
+      g[a.o_mlp_zb + 1] = (float)N * dJ1;
+    }
+  }
+  __syncthreads();
+  if (!a.has_att) return;
+  float* gw = a.gout + (int64_t)w * a.gstride;
+  // readout backward: last layer (no silu), then hidden layers with silu
+  const float* zlast = zst + (a.depth + 1) * a.lwmax;
+  if (t == 0) {
+    gw[a.o_att_scale] = zlast[0] * sJ[0];
+    gw[a.o_att_scale + 1] = zlast[1] * sJ[1];
+    gw[a.o_att_zb] = sJ[0];
+    gw[a.o_att_zb + 1] = sJ[1];
+  }
+  if (t < 2) dbuf[t] = sJ[t] * a.att_scale[t];
+  __syncthreads();
+  for (int l = a.depth; l >= 0; --l) {
+    const int nout = l < a.depth ? a.W : 2;
+    const int fan = l == 0 ? 2 * HD : a.W;
+    // input activation of layer l
+    const float* in;
+    if (l == 0) {
+      in = u;
+    } else {
+      const float* zp = zst + l * a.lwmax;
+      for (int j = t; j < fan; j += blockDim.x) buf0[j] = silu_f(zp[j]);
+      __syncthreads();
+      in = buf0;
+    }
+    // dW[q][j] = in[q] dz[j], db[j] = dz[j]
+    float* gW = gw + a.o_ro_W[l];
+    for (int64_t e = t; e < (int64_t)fan * nout; e += blockDim.x) {
+      const int q = (int)(e / nout), j = (int)(e - (int64_t)q * nout);
+      gW[e] = in[q] * dbuf[j];
+    }
+    for (int j = t; j < nout; j += blockDim.x) gw[a.o_ro_b[l] + j] = dbuf[j];
+    // d in[q] = sum_j W[q][j] dz[j]; then through the silu of layer l - 1
+    const float* Wl = a.ro_W[l];
+    for (int q = t; q < fan; q += blockDim.x) {
+      float acc = 0.f;
+      for (int j = 0; j < nout; ++j) acc += Wl[(int64_t)q * nout + j] * dbuf[j];
+      dbuf2[q] = l > 0 ? acc * silu_d(zst[l * a.lwmax + q]) : acc;
+    }
+    __syncthreads();
+    for (int q = t; q < fan; q += blockDim.x) dbuf[q] = dbuf2[q];
+    __syncthreads();
+  }
+  // dbuf = d u [2 HD]: d values = du / norm; d norm = -sum_k du values / norm^2
+  for (int c = t; c < 2 * HD; c += blockDim.x) {
+    const int sp = c / HD, h = (c - sp * HD) / Dk;
+    const float n = nrm[sp * a.H + h];
+    a.dval[(int64_t)w * 2 * HD + c] = dbuf[c] / n;
+  }
+  if (t < HD) {
+    const int h = t / Dk, k = t - h * Dk;
+    for (int sp = 0; sp < 2; ++sp) {
+      const float n = nrm[sp * a.H + h];
+      float v = -dbuf[sp * HD + t] * zst[sp * HD + t] / (n * n);
+      for (int o = Dk >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o, Dk);
+      if (k == 0) a.dnorm[((int64_t)w * 2 + sp) * a.H + h] = v;
+    }
+  }
+}
+
+size_t gr_head_lds(const GradHead& a) {
+  return sizeof(float) * ((size_t)2 * a.HD + 2 * a.H + (size_t)a.lwmax * (a.depth + 5));
+}
+
+void launch_gr_head(hipStream_t s, const GradHead& a) {
+  if (a.B == 0) return;
+  k_gr_head<<<a.B, 256, gr_head_lds(a), s>>>(a);
+}
+
+// ---------------------------------------------------------------------------------------
+// Per group row: cotangents of the raw orbital and envelope rows, of the last MLP-Jastrow
+// pre-activations, and of the attention projections (with the per-row register-key term).
+// ---------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_gr_rows(GradRows a) {
+  const int64_t g = blockIdx.x;
+  const int t = threadIdx.x;
+  int w, i;
+  group_walker(g, a.B, a.nu, a.N - a.nu, w, i);
+  const int N = a.N, ND = a.D * N;
+  for (int c = t; c < ND; c += blockDim.x) {
+    const int d = c / N, o = c - d * N;
+    const double dphi = a.wdet[w * a.D + d] * a.Ainv[(((int64_t)w * a.D + d) * N + o) * N + i];
+    a.dorb[g * ND + c] = (float)(dphi * (double)a.env[g * ND + c]);
+    a.denv[g * ND + c] = (float)(dphi * (double)a.orb[g * ND + c]);
+  }
+  if (a.has_mlp && t < 2) {
+    const float z = a.z3[g * 2 + t];
+    a.dz3[g * 2 + t] = a.dJ[2 * w + t] * a.mlp_scale[t] * silu_d(z);
+  }
+  if (a.has_att && t < a.HD) {
+    const int HD = a.HD, Dk = a.Dk, h = t / Dk;
+    const int sp = i < a.nu ? 0 : 1;
+    const float rs = rsqrtf((float)Dk);
+    const float* row = a.qv + g * 2 * HD;
+    const float q = row[t], v = row[HD + t], key = a.keys[t];
+    float qk = q * key;
+    for (int o = Dk >> 1; o > 0; o >>= 1) qk += __shfl_xor(qk, o, Dk);
+    const float at = expf(-qk * rs);
+    const float dv = a.dval[(int64_t)w * 2 * HD + sp * HD + t];
+    float datt = dv * v;
+    for (int o = Dk >> 1; o > 0; o >>= 1) datt += __shfl_xor(datt, o, Dk);
+    datt += a.dnorm[((int64_t)w * 2 + sp) * a.H + h];
+    const float dqk = datt * at * (-rs);
+    a.dqv[g * 2 * HD + t] = dqk * key;
+    a.dqv[g * 2 * HD + HD + t] = dv * at;
+    a.dkey[g * HD + t] = dqk * q;
+  }
+}
+
+void launch_gr_rows(hipStream_t s, const GradRows& a, int64_t G) {
+  if (G) k_gr_rows<<<(unsigned)G, 256, 0, s>>>(a);
+}
+
+// d sp_s[m][d][e] = -sum over the walker's spin-s electrons of dist_m ex[d][m][e] t[d][m][e]
+// (t = d env_d pi_d^T); one block per walker and spin, threads over (m, d, e).
+__global__ void k_gr_sp(const float* __restrict__ ex, const float* __restrict__ tt, const float* __restrict__ dist,
+                        int B, int N, int nu, int M, int D, int E, float* __restrict__ gout, int64_t gstride,
+                        int64_t o_up, int64_t o_dn) {
+  const int w = blockIdx.x, sp = blockIdx.y;
+  const int nd = N - nu;
+  const int i0 = sp == 0 ? 0 : nu, i1 = sp == 0 ? nu : N;
+  const int DME = D * M * E;
+  float* out = gout + (int64_t)w * gstride + (sp == 0 ? o_up : o_dn);
+  for (int c = threadIdx.x; c < DME; c += blockDim.x) {
+    const int m = c / (D * E), de = c - m * D * E, d = de / E, e = de - d * E;  // packed [M][D][E]
+    const int col = (d * M + m) * E + e;                                         // [D][M][E]
+    float acc = 0.f;
+    for (int i = i0; i < i1; ++i) {
+      const int64_t g = group_index(w, i, B, nu, nd);
+      acc -= dist[g * M + m] * ex[g * DME + col] * tt[g * DME + col];
+    }
+    out[c] = acc;
+  }
+}
+
+void launch_gr_sp(hipStream_t s, const float* ex, const float* tt, const float* dist, int B, int N, int nu, int M,
+                  int D, int E, float* gout, int64_t gstride, int64_t o_up, int64_t o_dn) {
+  if (B == 0) return;
+  const dim3 grid(B, N - nu > 0 ? 2 : 1);
+  k_gr_sp<<<grid, 256, 0, s>>>(ex, tt, dist, B, N, nu, M, D, E, gout, gstride, o_up, o_dn);
+}
+
+// ---------------------------------------------------------------------------------------
+// e-e edges and messages
+// ---------------------------------------------------------------------------------------
+// Forward of the pair filter per CSR pair (thread per pair): the small quantities the edge
+// backward needs, laid out per pair as PairRec (see fire_grad.hpp), and beta / the log-scaled
+// features as separate row arrays for the contractions.
+__global__ void k_gr_pair_fwd(const float* __restrict__ r, int B, int N, int nu, const int64_t* __restrict__ ptr,
+                              const int32_t* __restrict__ nbr, EdgeParams ps, EdgeParams pd, float cut,
+                              int64_t P, float* __restrict__ rec, float* __restrict__ beta, float* __restrict__ lsf,
+                              uint8_t* __restrict__ same, int32_t* __restrict__ ctr) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  // centre group: binary search of p in ptr[0 .. G]
+  const int64_t G = (int64_t)B * N;
+  int64_t lo = 0, hi = G;
+  while (hi - lo > 1) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (ptr[mid] <= p)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  const int64_t g = lo;
+  int w, i;
+  group_walker(g, B, nu, N - nu, w, i);
+  const int j = nbr[p];
+  const bool sm = (i < nu) == (j < nu);
+  same[p] = sm ? 1 : 0;
+  ctr[p] = (int32_t)g;
+  const EdgeParams& e = sm ? ps : pd;
+  const FilterParams& f = e.filt;
+  const float* ri = r + ((int64_t)w * N + i) * 3;
+  const float* rj = r + ((int64_t)w * N + j) * 3;
+  const float df[3] = {ri[0] - rj[0], ri[1] - rj[1], ri[2] - rj[2]};
+  const float d = sqrtf(df[0] * df[0] + df[1] * df[1] + df[2] * df[2]);
+  const float feat[4] = {d, df[0], df[1], df[2]};
+  const float ls = log1pf(d) / d;
+  float* rc = rec + p * PR_FLOATS;
+  for (int c = 0; c < 4; ++c) {
+    rc[PR_FEAT + c] = feat[c];
+    lsf[p * 4 + c] = feat[c] * ls;
+  }
+  rc[PR_D] = d;
+  float pre[FIRE_W0MAX], y[FIRE_W0MAX];
+  for (int a = 0; a < f.w0; ++a) {
+    float v = f.b[a];
+    for (int c = 0; c < 4; ++c) v += feat[c] * f.K[c * f.w0 + a];
+    pre[a] = v;
+    y[a] = silu_f(v);
+    rc[PR_PRE + a] = v;
+  }
+  float fc, dfc;
+  cutoff_poly(d / cut, fc, dfc);
+  float envr[FIRE_W0MAX];
+  for (int q = 0; q < f.E; ++q) {
+    const float x = d / f.sp[q];
+    envr[q] = expf(-x * x) * fc;
+    rc[PR_ENV + q] = envr[q];
+  }
+  for (int k = 0; k < f.w1; ++k) {
+    float dv = f.bd[k], ev = 0.f;
+    for (int a = 0; a < f.w0; ++a) dv += y[a] * f.Wd[a * f.w1 + k];
+    for (int q = 0; q < f.E; ++q) ev += envr[q] * f.We[q * f.w1 + k];
+    rc[PR_DIR + k] = dv;
+    rc[PR_ENVD + k] = ev;
+    beta[p * f.w1 + k] = dv * ev;
+  }
+}
+
+// Messages backward per (pair, feature): Gamma = beta WG, e = lsf WF + BF (same- or
+// different-spin edge weights), u = e + m_ct[i] + m_nb[j], msg = Gamma silu(u):
+// dGamma = dh_i silu(u), du = dh_i Gamma silu'(u).
+__global__ void k_gr_msg_bwd(const float* __restrict__ beta, const float* __restrict__ lsf, EdgeParams ps,
+                             EdgeParams pd, const float* __restrict__ M5, const float* __restrict__ dh,
+                             const int32_t* __restrict__ ctr, const int32_t* __restrict__ nbr,
+                             const uint8_t* __restrict__ same, int B, int N, int nu, int F, int64_t P,
+                             float* __restrict__ dgam, float* __restrict__ du) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= P * F) return;
+  const int64_t p = e / F;
+  const int f = (int)(e - p * F);
+  const int64_t g = ctr[p];
+  int w, i;
+  group_walker(g, B, nu, N - nu, w, i);
+  const int j = nbr[p];
+  const int64_t gj = group_index(w, j, B, nu, N - nu);
+  const bool sm = same[p] != 0;
+  const EdgeParams& ep = sm ? ps : pd;
+  const int w1 = ep.filt.w1;
+  float gam = 0.f, ev = ep.BF[f];
+  for (int k = 0; k < w1; ++k) gam += beta[p * w1 + k] * ep.WG[k * F + f];
+  for (int c = 0; c < 4; ++c) ev += lsf[p * 4 + c] * ep.WF[c * F + f];
+  const float mct = M5[g * 5 * F + (sm ? 1 : 2) * F + f];
+  const float mnb = M5[gj * 5 * F + (sm ? 3 : 4) * F + f];
+  const float u = ev + mct + mnb;
+  const float s = sigm(u);
+  const float dhv = dh[g * F + f];
+  dgam[e] = dhv * u * s;
+  du[e] = dhv * gam * s * (1.0f + u * (1.0f - s));
+}
+
+// d beta[p][k] = sum_f dGamma[p][f] WG[k][f] (same- or different-spin weights); one
+// wavefront per row, four features per lane.
+__global__ void k_gr_dbeta(const float* __restrict__ dgam, int F, const float* __restrict__ WGs,
+                           const float* __restrict__ WGd, const uint8_t* __restrict__ same, int w1, int64_t P,
+                           float* __restrict__ dbeta) {
+  const int64_t p = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (p >= P) return;
+  const float* WG = (same == nullptr || same[p]) ? WGs : WGd;
+  float acc[FIRE_W1MAX];
+  for (int k = 0; k < w1; ++k) acc[k] = 0.f;
+  for (int f = lane; f < F; f += 64) {
+    const float v = dgam[p * F + f];
+    for (int k = 0; k < w1; ++k) acc[k] += v * WG[k * F + f];
+  }
+  for (int k = 0; k < w1; ++k) {
+    const float v = warp_sum(acc[k]);
+    if (lane == 0) dbeta[p * w1 + k] = v;
+  }
+}
+
+// d out5 rows of the messages: d m_ct_s[i] = sum over i's pairs of spin class s of du,
+// d m_nb_s[j] = sum over the pairs (k -> j) of spin class s (found through j's own list and the
+// reverse slots); thread per (group, feature).
+__global__ void k_gr_msg_collect(const float* __restrict__ du, const int64_t* __restrict__ ptr,
+                                 const int32_t* __restrict__ nbr, const int32_t* __restrict__ rev,
+                                 const uint8_t* __restrict__ same, int B, int N, int nu, int F, float* __restrict__ dM5) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t G = (int64_t)B * N;
+  if (e >= G * F) return;
+  const int64_t g = e / F;
+  const int f = (int)(e - g * F);
+  int w, i;
+  group_walker(g, B, nu, N - nu, w, i);
+  float cs = 0.f, cd = 0.f, ns = 0.f, nd_ = 0.f;
+  for (int64_t p = ptr[g]; p < ptr[g + 1]; ++p) {
+    const float v = du[p * F + f];
+    if (same[p])
+      cs += v;
+    else
+      cd += v;
+    // the pair (k -> i) in k's list
+    const int k = nbr[p];
+    const int64_t gk = group_index(w, k, B, nu, N - nu);
+    const int64_t q = ptr[gk] + rev[p];
+    const float vr = du[q * F + f];
+    if (same[q])
+      ns += vr;
+    else
+      nd_ += vr;
+  }
+  dM5[g * 5 * F + 1 * F + f] = cs;
+  dM5[g * 5 * F + 2 * F + f] = cd;
+  dM5[g * 5 * F + 3 * F + f] = ns;
+  dM5[g * 5 * F + 4 * F + f] = nd_;
+}
+
+// Edge filter backward per pair (thread per pair) from d beta [P, w1]: per-pair contributions
+// to the filter parameters laid out as PairGrad (the host reduces them per walker).
+__global__ void k_gr_pair_bwd(const float* __restrict__ rec, const float* __restrict__ dbeta,
+                              const uint8_t* __restrict__ same, EdgeParams ps, EdgeParams pd, float cut, int64_t P,
+                              float* __restrict__ pg) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  const bool sm = same[p] != 0;
+  const FilterParams& f = (sm ? ps : pd).filt;
+  const float* rc = rec + p * PR_FLOATS;
+  const float d = rc[PR_D];
+  float* o = pg + p * PG_FLOATS;
+  float ddir[FIRE_W1MAX], denvd[FIRE_W1MAX];
+  for (int k = 0; k < f.w1; ++k) {
+    const float db = dbeta[p * f.w1 + k];
+    ddir[k] = db * rc[PR_ENVD + k];
+    denvd[k] = db * rc[PR_DIR + k];
+    o[PG_DDIR + k] = ddir[k];
+    o[PG_DENVD + k] = denvd[k];
+  }
+  float fc, dfc;
+  cutoff_poly(d / cut, fc, dfc);
+  float dcut = 0.f;
+  for (int a = 0; a < f.w0; ++a) {
+    // y = silu(pre); d y = sum_k Wd[a][k] d dir[k]; d pre = d y silu'(pre)
+    float dy = 0.f;
+    for (int k = 0; k < f.w1; ++k) dy += f.Wd[a * f.w1 + k] * ddir[k];
+    const float pre = rc[PR_PRE + a];
+    o[PG_Y + a] = silu_f(pre);
+    o[PG_DPRE + a] = dy * silu_d(pre);
+  }
+  for (int q = 0; q < f.E; ++q) {
+    float den = 0.f;
+    for (int k = 0; k < f.w1; ++k) den += f.We[q * f.w1 + k] * denvd[k];
+    // env_r = exp(-(d/sp)^2) fc(d/cut)
+    const float s = f.sp[q], x = d / s, g0 = expf(-x * x);
+    o[PG_DSP + q] = den * g0 * fc * 2.f * x * x / s;
+    dcut += den * g0 * dfc * (-d / (cut * cut));
+  }
+  o[PG_DCUT] = dcut;
+}
+
+// ---------------------------------------------------------------------------------------
+// ElecInit: e-n edges (rows g * M + m) and the dense stage
+// ---------------------------------------------------------------------------------------
+__global__ void k_gr_en_fwd(const float* __restrict__ r, const float* __restrict__ R, int B, int N, int nu, int M,
+                            float cutoff_1el, EnParams ep, int64_t rows, float* __restrict__ rec,
+                            float* __restrict__ beta, float* __restrict__ lsf, uint8_t* __restrict__ mask) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= rows) return;
+  const int64_t g = q / M;
+  const int m = (int)(q - g * M);
+  int w, i;
+  group_walker(g, B, nu, N - nu, w, i);
+  const float* ri = r + ((int64_t)w * N + i) * 3;
+  const float df[3] = {ri[0] - R[3 * m], ri[1] - R[3 * m + 1], ri[2] - R[3 * m + 2]};
+  const float d = sqrtf(df[0] * df[0] + df[1] * df[1] + df[2] * df[2]);
+  // the e-n mask of ElecInit: the float32 distance strictly inside cutoff_1el
+  const float dd = __fsqrt_rn(__fadd_rn(__fadd_rn(__fmul_rn(df[0], df[0]), __fmul_rn(df[1], df[1])), __fmul_rn(df[2], df[2])));
+  mask[q] = dd < cutoff_1el ? 1 : 0;
+  const float feat[4] = {d, df[0], df[1], df[2]};
+  const float ls = log1pf(d) / d;
+  float* rc = rec + q * PR_FLOATS;
+  for (int c = 0; c < 4; ++c) {
+    rc[PR_FEAT + c] = feat[c];
+    lsf[q * 4 + c] = feat[c] * ls;
+  }
+  rc[PR_D] = d;
+  const float* K = ep.K + (int64_t)m * 4 * ep.w0;
+  const float* bb = ep.b + (int64_t)m * ep.w0;
+  const float* sp = ep.sp + (int64_t)m * ep.E;
+  float y[FIRE_W0MAX];
+  for (int a = 0; a < ep.w0; ++a) {
+    float v = bb[a];
+    for (int c = 0; c < 4; ++c) v += feat[c] * K[c * ep.w0 + a];
+    rc[PR_PRE + a] = v;
+    y[a] = silu_f(v);
+  }
+  float fc, dfc;
+  cutoff_poly(d / cutoff_1el, fc, dfc);
+  float envr[FIRE_EEMAX];
+  for (int e = 0; e < ep.E; ++e) {
+    const float x = d / sp[e];
+    envr[e] = expf(-x * x) * fc;
+  }
+  for (int k = 0; k < ep.w1; ++k) {
+    float dv = ep.bd[k], ev = 0.f;
+    for (int a = 0; a < ep.w0; ++a) dv += y[a] * ep.Wd[a * ep.w1 + k];
+    for (int e = 0; e < ep.E; ++e) ev += envr[e] * ep.We[e * ep.w1 + k];
+    rc[PR_DIR + k] = dv;
+    rc[PR_ENVD + k] = ev;
+    beta[q * ep.w1 + k] = dv * ev;
+  }
+}
+
+// h_en[g][f] = sum_m mask Gamma_gm[f] (e_gm[f] + h_nuc[m][f]) with Gamma = beta WG and
+// e = lsf WF (rows g M + m); thread per (group, feature).
+__global__ void k_gr_en_sum(const float* __restrict__ beta, const float* __restrict__ lsf, EnParams ep,
+                            const uint8_t* __restrict__ mask, int64_t G, int M, int F, float* __restrict__ hen) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= G * F) return;
+  const int64_t g = e / F;
+  const int f = (int)(e - g * F);
+  float acc = 0.f;
+  for (int m = 0; m < M; ++m) {
+    const int64_t q = g * M + m;
+    if (!mask[q]) continue;
+    float gam = 0.f, ev = ep.hnuc[(int64_t)m * F + f];
+    for (int k = 0; k < ep.w1; ++k) gam += beta[q * ep.w1 + k] * ep.WG[k * F + f];
+    for (int c = 0; c < 4; ++c) ev += lsf[q * 4 + c] * ep.WF[c * F + f];
+    acc += gam * ev;
+  }
+  hen[e] = acc;
+}
+
+// GLU forward: y = x silu(gate), [x | gate] = Z2 [G, 2F]
+__global__ void k_gr_glu_fwd(const float* __restrict__ Z2, int64_t G, int F, float* __restrict__ y) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= G * F) return;
+  const int64_t g = e / F;
+  const int f = (int)(e - g * F);
+  y[e] = Z2[g * 2 * F + f] * silu_f(Z2[g * 2 * F + F + f]);
+}
+
+// LayerNorm forward (flax: biased variance, eps 1e-6) keeping n = (h - mean) / sqrt(var + eps)
+// and inv per row; one block of 256 threads per row (F <= 256).
+__global__ void k_gr_ln_fwd(const float* __restrict__ h, int F, const float* __restrict__ gsc, const float* __restrict__ gb,
+                            float* __restrict__ y, float* __restrict__ nrm, float* __restrict__ inv) {
+  __shared__ float red[8];
+  const int64_t g = blockIdx.x;
+  const int t = threadIdx.x;
+  const float x = t < F ? h[g * F + t] : 0.f;
+  const float mean = block_sum_f(x, red) / F;
+  const float c = t < F ? x - mean : 0.f;
+  const float var = block_sum_f(c * c, red) / F;
+  const float iv = rsqrtf(var + 1e-6f);
+  if (t < F) {
+    nrm[g * F + t] = c * iv;
+    y[g * F + t] = c * iv * gsc[t] + gb[t];
+  }
+  if (t == 0) inv[g] = iv;
+}
+
+// LayerNorm backward: dn = dy * scale; dh = inv (dn - mean(dn) - n mean(dn n))
+__global__ void k_gr_ln_bwd(const float* __restrict__ dy, const float* __restrict__ nrm, const float* __restrict__ inv,
+                            const float* __restrict__ gsc, int F, float* __restrict__ dh) {
+  __shared__ float red[8];
+  const int64_t g = blockIdx.x;
+  const int t = threadIdx.x;
+  const float n = t < F ? nrm[g * F + t] : 0.f;
+  const float dn = t < F ? dy[g * F + t] * gsc[t] : 0.f;
+  const float m1 = block_sum_f(dn, red) / F;
+  const float m2 = block_sum_f(dn * n, red) / F;
+  if (t < F) dh[g * F + t] = inv[g] * (dn - m1 - n * m2);
+}
+
+// GLU backward: y = x silu(gate), [x | gate] = Z2 [G, 2F]; dZ2 = [dy silu(gate) | dy x silu'(gate)]
+__global__ void k_gr_glu_bwd(const float* __restrict__ dy, const float* __restrict__ Z2, int64_t G, int F,
+                             float* __restrict__ dZ2) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= G * F) return;
+  const int64_t g = e / F;
+  const int f = (int)(e - g * F);
+  const float x = Z2[g * 2 * F + f], gt = Z2[g * 2 * F + F + f], d = dy[e];
+  dZ2[g * 2 * F + f] = d * silu_f(gt);
+  dZ2[g * 2 * F + F + f] = d * x * silu_d(gt);
+}
+
+// e-n edges backward per (edge, feature): dGamma = mask dh_en (e + h_nuc), de = mask dh_en Gamma
+__global__ void k_gr_en_bwd_f(const float* __restrict__ beta, const float* __restrict__ lsf, EnParams ep,
+                              const uint8_t* __restrict__ mask, const float* __restrict__ dhen, int64_t rows, int M,
+                              int F, float* __restrict__ dgam, float* __restrict__ de) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= rows * F) return;
+  const int64_t q = e / F;
+  const int f = (int)(e - q * F);
+  const int64_t g = q / M;
+  const int m = (int)(q - g * M);
+  if (!mask[q]) {
+    dgam[e] = 0.f;
+    de[e] = 0.f;
+    return;
+  }
+  float gam = 0.f, ev = ep.hnuc[(int64_t)m * F + f];
+  for (int k = 0; k < ep.w1; ++k) gam += beta[q * ep.w1 + k] * ep.WG[k * F + f];
+  for (int c = 0; c < 4; ++c) ev += lsf[q * 4 + c] * ep.WF[c * F + f];
+  const float dh = dhen[g * F + f];
+  dgam[e] = dh * ev;
+  de[e] = dh * gam;
+}
+
+// e-n filter backward per edge (thread per edge) from d beta [rows, w1]
+__global__ void k_gr_en_bwd(const float* __restrict__ rec, const float* __restrict__ dbeta, EnParams ep,
+                            float cutoff_1el, int64_t rows, int M, float* __restrict__ pg) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= rows) return;
+  const int m = (int)(q % M);
+  const float* rc = rec + q * PR_FLOATS;
+  const float d = rc[PR_D];
+  const float* sp = ep.sp + (int64_t)m * ep.E;
+  float* o = pg + q * PG_FLOATS;
+  float ddir[FIRE_W1MAX], denvd[FIRE_W1MAX];
+  for (int k = 0; k < ep.w1; ++k) {
+    const float db = dbeta[q * ep.w1 + k];
+    ddir[k] = db * rc[PR_ENVD + k];
+    denvd[k] = db * rc[PR_DIR + k];
+    o[PG_DDIR + k] = ddir[k];
+    o[PG_DENVD + k] = denvd[k];
+  }
+  float fc, dfc;
+  cutoff_poly(d / cutoff_1el, fc, dfc);
+  for (int a = 0; a < ep.w0; ++a) {
+    float dy = 0.f;
+    for (int k = 0; k < ep.w1; ++k) dy += ep.Wd[a * ep.w1 + k] * ddir[k];
+    const float pre = rc[PR_PRE + a];
+    o[PG_Y + a] = silu_f(pre);
+    o[PG_DPRE + a] = dy * silu_d(pre);
+  }
+  for (int e = 0; e < ep.E; ++e) {
+    float den = 0.f;
+    for (int k = 0; k < ep.w1; ++k) den += ep.We[e * ep.w1 + k] * denvd[k];
+    const float s = sp[e], x = d / s, g0 = expf(-x * x);
+    o[PG_ENVR + e] = g0 * fc;
+    o[PG_DSP + e] = den * g0 * fc * 2.f * x * x / s;
+  }
+}
+
+// Nucleus-dependent e-n parameters per (walker, nucleus): d K_m [4][w0] = sum feat (x) d pre,
+// d b_m = sum d pre, d sp_m = sum d sp, d h_nuc[m] = sum de (over the walker's electrons).
+__global__ void k_gr_en_nuc(const float* __restrict__ rec, const float* __restrict__ pg, const float* __restrict__ de,
+                            int B, int N, int nu, int M, int F, EnParams ep, float* __restrict__ gout, int64_t gstride,
+                            int64_t oK, int64_t ob, int64_t osp, int64_t ohn) {
+  const int w = blockIdx.x, m = blockIdx.y, t = threadIdx.x;
+  const int nd = N - nu;
+  float* g = gout + (int64_t)w * gstride;
+  const int nK = 4 * ep.w0;
+  for (int c = t; c < nK + ep.w0 + ep.E; c += blockDim.x) {
+    float acc = 0.f;
+    for (int i = 0; i < N; ++i) {
+      const int64_t q = group_index(w, i, B, nu, nd) * M + m;
+      const float* rc = rec + q * PR_FLOATS;
+      const float* o = pg + q * PG_FLOATS;
+      if (c < nK) {
+        const int cc = c / ep.w0, a = c - cc * ep.w0;
+        acc += rc[PR_FEAT + cc] * o[PG_DPRE + a];
+      } else if (c < nK + ep.w0) {
+        acc += o[PG_DPRE + (c - nK)];
+      } else {
+        acc += o[PG_DSP + (c - nK - ep.w0)];
+      }
+    }
+    if (c < nK)
+      g[oK + (int64_t)m * nK + c] = acc;
+    else if (c < nK + ep.w0)
+      g[ob + (int64_t)m * ep.w0 + (c - nK)] = acc;
+    else
+      g[osp + (int64_t)m * ep.E + (c - nK - ep.w0)] = acc;
+  }
+  for (int f = t; f < F; f += blockDim.x) {
+    float acc = 0.f;
+    for (int i = 0; i < N; ++i) {
+      const int64_t q = group_index(w, i, B, nu, nd) * M + m;
+      acc += de[q * F + f];
+    }
+    g[ohn + (int64_t)m * F + f] = acc;
+  }
+}
+
+void launch_gr_pair_fwd(hipStream_t s, const float* r, int B, int N, int nu, const int64_t* ptr, const int32_t* nbr,
+                        const EdgeParams& ps, const EdgeParams& pd, float cut, int64_t P, float* rec, float* beta,
+                        float* lsf, uint8_t* same, int32_t* ctr) {
+  if (P) k_gr_pair_fwd<<<(unsigned)((P + 127) / 128), 128, 0, s>>>(r, B, N, nu, ptr, nbr, ps, pd, cut, P, rec, beta, lsf, same, ctr);
+}
+void launch_gr_msg_bwd(hipStream_t s, const float* beta, const float* lsf, const EdgeParams& ps, const EdgeParams& pd,
+                       const float* M5, const float* dh, const int32_t* ctr, const int32_t* nbr, const uint8_t* same,
+                       int B, int N, int nu, int F, int64_t P, float* dgam, float* du) {
+  const int64_t n = P * F;
+  if (n)
+    k_gr_msg_bwd<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(beta, lsf, ps, pd, M5, dh, ctr, nbr, same, B, N, nu, F, P,
+                                                             dgam, du);
+}
+void launch_gr_dbeta(hipStream_t s, const float* dgam, int F, const float* WG_same, const float* WG_diff,
+                     const uint8_t* same, int w1, int64_t P, float* dbeta) {
+  if (P) k_gr_dbeta<<<(unsigned)((P + 3) / 4), 256, 0, s>>>(dgam, F, WG_same, WG_diff, same, w1, P, dbeta);
+}
+void launch_gr_msg_collect(hipStream_t s, const float* du, const int64_t* ptr, const int32_t* nbr, const int32_t* rev,
+                           const uint8_t* same, int B, int N, int nu, int F, float* dM5) {
+  const int64_t n = (int64_t)B * N * F;
+  if (n) k_gr_msg_collect<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(du, ptr, nbr, rev, same, B, N, nu, F, dM5);
+}
+void launch_gr_pair_bwd(hipStream_t s, const float* rec, const float* dbeta, const uint8_t* same, const EdgeParams& ps,
+                        const EdgeParams& pd, float cut, int64_t P, float* pg) {
+  if (P) k_gr_pair_bwd<<<(unsigned)((P + 127) / 128), 128, 0, s>>>(rec, dbeta, same, ps, pd, cut, P, pg);
+}
+void launch_gr_en_fwd(hipStream_t s, const float* r, const float* R, int B, int N, int nu, int M, float cutoff_1el,
+                      const EnParams& ep, int64_t rows, float* rec, float* beta, float* lsf, uint8_t* mask) {
+  if (rows) k_gr_en_fwd<<<(unsigned)((rows + 127) / 128), 128, 0, s>>>(r, R, B, N, nu, M, cutoff_1el, ep, rows, rec, beta, lsf, mask);
+}
+void launch_gr_en_sum(hipStream_t s, const float* beta, const float* lsf, const EnParams& ep, const uint8_t* mask,
+                      int64_t G, int M, int F, float* hen) {
+  const int64_t n = G * F;
+  if (n) k_gr_en_sum<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(beta, lsf, ep, mask, G, M, F, hen);
+}
+void launch_gr_glu_fwd(hipStream_t s, const float* Z2, int64_t G, int F, float* y) {
+  const int64_t n = G * F;
+  if (n) k_gr_glu_fwd<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(Z2, G, F, y);
+}
+void launch_gr_ln_fwd(hipStream_t s, const float* h, int64_t G, int F, const float* gsc, const float* gb, float* y,
+                      float* nrm, float* inv) {
+  if (G) k_gr_ln_fwd<<<(unsigned)G, 256, 0, s>>>(h, F, gsc, gb, y, nrm, inv);
+}
+void launch_gr_ln_bwd(hipStream_t s, const float* dy, const float* nrm, const float* inv, const float* gsc, int64_t G,
+                      int F, float* dh) {
+  if (G) k_gr_ln_bwd<<<(unsigned)G, 256, 0, s>>>(dy, nrm, inv, gsc, F, dh);
+}
+void launch_gr_glu_bwd(hipStream_t s, const float* dy, const float* Z2, int64_t G, int F, float* dZ2) {
+  const int64_t n = G * F;
+  if (n) k_gr_glu_bwd<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(dy, Z2, G, F, dZ2);
+}
+void launch_gr_en_bwd_f(hipStream_t s, const float* beta, const float* lsf, const EnParams& ep, const uint8_t* mask,
+                        const float* dhen, int64_t rows, int M, int F, float* dgam, float* de) {
+  const int64_t n = rows * F;
+  if (n) k_gr_en_bwd_f<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(beta, lsf, ep, mask, dhen, rows, M, F, dgam, de);
+}
+void launch_gr_en_bwd(hipStream_t s, const float* rec, const float* dbeta, const EnParams& ep, float cutoff_1el,
+                      int64_t rows, int M, float* pg) {
+  if (rows) k_gr_en_bwd<<<(unsigned)((rows + 127) / 128), 128, 0, s>>>(rec, dbeta, ep, cutoff_1el, rows, M, pg);
+}
+void launch_gr_en_nuc(hipStream_t s, const float* rec, const float* pg, const float* de, int B, int N, int nu, int M,
+                      int F, const EnParams& ep, float* gout, int64_t gstride, int64_t oK, int64_t ob, int64_t osp,
+                      int64_t ohn) {
+  if (B) k_gr_en_nuc<<<dim3(B, M), 256, 0, s>>>(rec, pg, de, B, N, nu, M, F, ep, gout, gstride, oK, ob, osp, ohn);
+}
+
+}  // namespace fire

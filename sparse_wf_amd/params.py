@@ -268,7 +268,7 @@ def _pad_rows(a, n):
     return out
 
 
-def pack_params(cfg: FireConfig, params: dict) -> list:
+def pack_params(cfg: FireConfig, params: dict, dtype=np.float32) -> list:
     """Flatten ``params`` into the canonical float32 tensor list of ``fire_upload_params``.
 
     Transformations applied here (all exact re-arrangements or precomputations):
@@ -278,7 +278,7 @@ def pack_params(cfg: FireConfig, params: dict) -> list:
     Q and V weights concatenated; for ``spin_restricted`` the down-spin orbital / envelope
     parameters are the up-spin ones with the columns of _swap_spin_updn permuted
     (orbitals.py:88-107); learned cutoff = cutoff * sigmoid(cutoff_param) (:644)."""
-    f32 = lambda a: np.ascontiguousarray(np.asarray(a, np.float64).astype(np.float32))
+    f32 = lambda a: np.ascontiguousarray(np.asarray(a, np.float64).astype(dtype))
     W1p = padded_w1(cfg.pair_mlp_widths[1])
     ep = params["embedding"]
     s0, s1 = (float(v) for v in ep["scales"])
@@ -346,3 +346,162 @@ def pack_params(cfg: FireConfig, params: dict) -> list:
         out.append(np.asarray([c["alpha_same"], c["alpha_diff"], c["factor_same"], c["factor_diff"]], np.float64))
     out.append(np.asarray([cfg.cutoff * _sigmoid(ep["cutoff_param"]), s1]))
     return [f32(t).reshape(-1) for t in out]
+
+
+def pack_params_vjp(cfg: FireConfig, params: dict, g):
+    """Adjoint of :func:`pack_params` for a batch of cotangents: ``g`` (torch [B, n], the packed
+    tensors back to back, as ``fire_grad_logpsi`` writes them) -> a tree shaped like ``params``
+    whose leaves are torch [B, *leaf shape] on ``g``'s device, the gradients with respect to the
+    tree's own leaves.  Chain rules of the packing: softplus'd scales take sigmoid(x); folded
+    normalisation scales give s0 * g, s1 * g to the weights and sum(W * g) to s0 / s1;
+    zero-padded filter columns / rows are dropped; pi is transposed back; for
+    ``spin_restricted`` the down-spin copies are scattered back through the column
+    permutation; the learned cutoff takes cutoff * sigmoid'(cutoff_param); the Jastrow
+    log-bias and attention bias are the second entries of their [0, bias] pairs."""
+    import torch
+
+    B = g.shape[0]
+    sizes = [t.size for t in pack_params(cfg, params)]
+    if g.shape[1] != sum(sizes):
+        raise ValueError(f"gradient rows hold {g.shape[1]} floats, the packed parameters {sum(sizes)}")
+    parts = iter(torch.split(g, sizes, dim=1))
+    dev, dt = g.device, g.dtype
+    out = tree_map(lambda x: torch.zeros((B,) + np.shape(x), dtype=dt, device=dev), params)
+    T = lambda x: torch.as_tensor(np.asarray(x, np.float64), dtype=dt, device=dev)
+    wsum = lambda W, gw: (T(W).reshape(1, -1) * gw.reshape(B, -1)).sum(1)
+    W1p, w1 = padded_w1(cfg.pair_mlp_widths[1]), cfg.pair_mlp_widths[1]
+
+    def take(shape=None):
+        p = next(parts)
+        return p if shape is None else p.reshape((B,) + tuple(shape))
+
+    def add(node, key, v):
+        node[key] += v.reshape(node[key].shape)
+
+    def cols(x, leaf):  # _pad_cols
+        return take(np.shape(leaf)[:-1] + (W1p,))[..., :w1]
+
+    def rows(x, leaf):  # _pad_rows
+        return take((W1p,) + np.shape(leaf)[1:])[:, :w1]
+
+    ep, gp = params["embedding"], out["embedding"]
+    s0, s1 = (float(v) for v in ep["scales"])
+    gs0 = torch.zeros(B, dtype=dt, device=dev)
+    gs1 = torch.zeros(B, dtype=dt, device=dev)
+    dyn, gd = ep["dynamic_params_en"], gp["dynamic_params_en"]
+    add(gd["filter"], "kernel", take())
+    add(gd["filter"], "bias", take())
+    add(gd["filter"], "scales", take() * T(_sigmoid(dyn["filter"]["scales"])).reshape(1, -1))
+    add(gd, "nuc_embedding", take())
+    ei, ge = ep["elec_init"]["params"], gp["elec_init"]["params"]
+    e, ge_ = ei["ElecNucEdge_0"], ge["ElecNucEdge_0"]
+    pf, gpf = e["PairwiseFilter_0"], ge_["PairwiseFilter_0"]
+    add(gpf["Dense_0"], "kernel", cols(None, pf["Dense_0"]["kernel"]))
+    add(gpf["Dense_0"], "bias", cols(None, pf["Dense_0"]["bias"]))
+    add(gpf["Dense_1"], "kernel", cols(None, pf["Dense_1"]["kernel"]))
+    add(ge_["Dense_0"], "kernel", rows(None, e["Dense_0"]["kernel"]))
+    add(ge_["Dense_1"], "kernel", take())
+    add(ge["LayerNorm_0"], "scale", take())
+    add(ge["LayerNorm_0"], "bias", take())
+    add(ge["GatedLinearUnit_0"]["Dense_0"], "kernel", take())
+    for k in ("Dense_0", "Dense_1"):
+        add(ge[k], "kernel", take())
+        add(ge[k], "bias", take())
+    for name in ("edge_same", "edge_diff"):
+        q, gq = ep[name]["params"], gp[name]["params"]
+        b, gb = q["beta_ee"], gq["beta_ee"]
+        add(gb, "kernel", take())
+        add(gb, "bias", take())
+        add(gb, "scale", take() * T(_sigmoid(b["scale"])).reshape(1, -1))
+        add(gb["Dense_0"], "kernel", cols(None, b["Dense_0"]["kernel"]))
+        add(gb["Dense_0"], "bias", cols(None, b["Dense_0"]["bias"]))
+        add(gb["Dense_1"], "kernel", cols(None, b["Dense_1"]["kernel"]))
+        add(gq["Dense_0"], "kernel", rows(None, q["Dense_0"]["kernel"]))
+        add(gq["Dense_1"], "kernel", take())
+        add(gq["Dense_1"], "bias", take())
+    up, gu = ep["updates"][0]["params"], gp["updates"][0]["params"]
+    x = take()
+    gs0 += wsum(up["Linear_0"]["kernel"], x)
+    add(gu["Linear_0"], "kernel", s0 * x)
+    add(gu["Linear_0"], "bias", take())
+    add(gu["Linear_1"], "kernel", take())
+    add(gu["Linear_1"], "bias", take())
+
+    op, go = params["to_orbitals"]["params"], out["to_orbitals"]["params"]
+    N, D, nu = cfg.n_el, cfg.n_determinants, cfg.n_up
+    W_up = np.asarray(op["to_orbitals_up"]["kernel"], np.float64)
+    gWu, gbu = take(W_up.shape), take()
+    gWd, gbd = take(W_up.shape), take()
+    gspu, gpiu, gspd, gpid = take(), take(), take(), take()
+    pi_shape = np.shape(op["env_up"]["pi"])  # [nucleus, det, envelope, orbital]
+    back = lambda p: p.reshape((B, pi_shape[1], pi_shape[0]) + tuple(pi_shape[2:])).transpose(1, 2)
+    gs1 += wsum(W_up, gWu)
+    add(go["to_orbitals_up"], "kernel", s1 * gWu)
+    add(go["to_orbitals_up"], "bias", gbu)
+    add(go["env_up"], "pi", back(gpiu))
+    sig_up = T(_sigmoid(op["env_up"]["sigma"])).reshape(1, -1)
+    add(go["env_up"], "sigma", gspu * sig_up)
+    if cfg.spin_restricted:
+        perm = torch.as_tensor(_swap_perm(N, D, nu), device=dev)
+        gs1 += wsum(W_up[:, _swap_perm(N, D, nu)], gWd)
+        t = torch.zeros_like(gWd)
+        t[:, :, perm] = s1 * gWd
+        add(go["to_orbitals_up"], "kernel", t)
+        t = torch.zeros_like(gbd)
+        t[:, perm] = gbd
+        add(go["to_orbitals_up"], "bias", t)
+        add(go["env_up"], "sigma", gspd * sig_up)
+        t = torch.zeros((B,) + tuple(pi_shape), dtype=dt, device=dev)
+        t[..., torch.as_tensor((np.arange(N) + nu) % N, device=dev)] = back(gpid)
+        add(go["env_up"], "pi", t)
+    else:
+        gs1 += wsum(op["to_orbitals_dn"]["kernel"], gWd)
+        add(go["to_orbitals_dn"], "kernel", s1 * gWd)
+        add(go["to_orbitals_dn"], "bias", gbd)
+        add(go["env_dn"], "sigma", gspd * T(_sigmoid(op["env_dn"]["sigma"])).reshape(1, -1))
+        add(go["env_dn"], "pi", back(gpid))
+
+    jp, gj = params["jastrow"]["params"], out["jastrow"]["params"]
+    if cfg.use_mlp_jastrow or cfg.use_log_jastrow:
+        mlp, gm = jp["mlp"], gj["mlp"]
+        for l in range(cfg.mlp_depth):
+            x = take()
+            if l == 0:
+                gs1 += wsum(mlp["Linear_0"]["kernel"], x)
+                x = s1 * x
+            add(gm[f"Linear_{l}"], "kernel", x)
+            add(gm[f"Linear_{l}"], "bias", take())
+        add(gm[f"Linear_{cfg.mlp_depth}"], "kernel", take())
+        add(gm[f"Linear_{cfg.mlp_depth}"], "bias", take())
+        add(gj, "mlp_scale", take())
+        zb = take()
+        if cfg.use_log_jastrow:
+            add(gj, "log_bias", zb[:, 1])
+    if cfg.use_attention:
+        a, ga = jp["att"], gj["att"]
+        hq = np.shape(a["Q_W"]["kernel"])[1]
+        x = take((np.shape(a["Q_W"]["kernel"])[0], -1))
+        qv = np.concatenate([a["Q_W"]["kernel"], a["V_W"]["kernel"]], axis=1)
+        gs1 += wsum(qv, x)
+        add(ga["Q_W"], "kernel", s1 * x[:, :, :hq])
+        add(ga["V_W"], "kernel", s1 * x[:, :, hq:])
+        x = take()
+        add(ga["Q_W"], "bias", x[:, :hq])
+        add(ga["V_W"], "bias", x[:, hq:])
+        add(ga, "register_keys", take())
+        for l in range(cfg.mlp_depth + 1):
+            add(ga["out"][f"Dense_{l}"], "kernel", take())
+            add(ga["out"][f"Dense_{l}"], "bias", take())
+        add(ga, "scale", take())
+        add(ga, "bias", take()[:, 1])
+    if cfg.e_e_cusps == "psiformer":
+        x, gc = take(), gj["pairwise_cusps"]
+        for i, k in enumerate(("alpha_same", "alpha_diff", "factor_same", "factor_diff")):
+            add(gc, k, x[:, i])
+    x = take()
+    sg = float(_sigmoid(ep["cutoff_param"]))
+    add(gp, "cutoff_param", x[:, 0] * (cfg.cutoff * sg * (1.0 - sg)))
+    gs1 += x[:, 1]
+    gp["scales"][0] += gs0
+    gp["scales"][1] += gs1
+    return out

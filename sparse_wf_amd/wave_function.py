@@ -26,7 +26,7 @@ except ImportError:  # hashlib is the fallback
 
 from . import _lib
 from .config import FireConfig, Molecule
-from .params import init_params, pack_params, padded_w1, set_scales
+from .params import init_params, pack_params, pack_params_vjp, padded_w1, set_scales, tree_map
 
 # "mixed" (the default): 16-bit matrix cores on split operands -- two scaled FP16 planes on the
 # tangent-row contractions of the local energy, three BF16 planes elsewhere (fire_precision_plan);
@@ -281,6 +281,27 @@ class FireWaveFunction:
 
     def __call__(self, params, electrons, static=None, precision=None):
         return self.signed(params, electrons, static, precision)[1]
+
+    def grad_logpsi(self, params, electrons, packed: bool = False):
+        """Per-walker d log|psi| / d theta -- the jax.grad(log_p) mapped over walkers by the
+        natural-gradient preconditioners (preconditioner.py:191-194, 290-294).  Returns
+        (grad, logpsi): grad a tree shaped like ``params`` with leaves [B, *shape] (float32,
+        through the adjoint of the packing), or with ``packed`` (always for ``PackedParams``)
+        the [B, n] rows over the packed tensors; logpsi [B] float64."""
+        self._ensure(params)
+        r, single = self._walkers(electrons)
+        B = r.shape[0]
+        n = ctypes.c_int64()
+        _lib.check(self._lib.fire_param_count(self._h, ctypes.byref(n)))
+        g = torch.empty((B, n.value), dtype=torch.float32, device=self.device)
+        lp = torch.empty(B, dtype=torch.float64, device=self.device)
+        _lib.check(self._lib.fire_grad_logpsi(self._h, _stream_ptr(), _dptr(r), B, _dptr(g), _dptr(lp)))
+        if not (packed or isinstance(params, PackedParams)):
+            g = pack_params_vjp(self.cfg, params, g)
+        if single:
+            g = tree_map(lambda t: t[0], g) if isinstance(g, dict) else g[0]
+            lp = lp[0]
+        return g, lp
 
     def local_energy_terms(self, params, electrons, static=None, precision=None, with_derivatives=False):
         """dict with e_loc, e_kin, e_pot, logpsi, sign (and grad [.., N, 3], lap of log|psi|)."""
