@@ -98,6 +98,8 @@ def main():
     ap.add_argument("--config-legs", default="h2o_like,h64",
                     help="other benchmark configurations evaluated (throughput, error against FP64); '' to skip")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--grad-walkers", type=int, default=1024,
+                    help="walkers of the parameter-gradient leg (0: skip)")
     ap.add_argument("--ragged", type=int, default=32, help="geometries of the ragged-batch leg (0: skip)")
     ap.add_argument("--ragged-sweeps", type=int, default=2, help="timed Metropolis sweeps of the ragged leg")
     ap.add_argument("--serve-requests", type=int, default=8, help="requests of the serving leg (0: skip)")
@@ -233,6 +235,32 @@ def main():
     wps_x = B * max(1, args.steps // 2) / dt_x
     wps_d = B * max(1, args.steps // 2) / dt_d
 
+    # 3. per-walker parameter gradient of log|psi| (FP32 value path and its reverse mode; the
+    # rows over the packed parameter tensors, as the preconditioner's Jacobian consumes them)
+    grad_leg = None
+    if args.grad_walkers > 0:
+        Bg = min(B, args.grad_walkers)
+        rg = r[:Bg].contiguous()
+        log(f"timing parameter gradient ({Bg} walkers)")
+        wf.grad_logpsi(params, rg, packed=True)  # sizes the workspace
+        torch.cuda.synchronize()
+        n_rep = 2
+        t0 = time.perf_counter()
+        for _ in range(n_rep):
+            g_rows, lp_g = wf.grad_logpsi(params, rg, packed=True)
+        torch.cuda.synchronize()
+        dt_g = (time.perf_counter() - t0) / n_rep
+        lp_v = wf.signed(params, rg, precision="fp32")[1]
+        grad_leg = {
+            "walkers": Bg, "parameters": int(g_rows.shape[1]), "seconds_per_batch": dt_g,
+            "walkers_per_s": Bg / dt_g, "finite": bool(torch.isfinite(g_rows).all()),
+            "logpsi_minus_fp32_value_max": float((lp_g - lp_v).abs().max()),
+            "what": "d log|psi| / d theta per walker over every packed parameter float (fire_grad_logpsi), "
+                    "FP32 forward with FP64 determinants and its reverse mode, walkers in chunks of up to 128",
+        }
+        del g_rows, lp_g
+        torch.cuda.empty_cache()
+
     g_ms, g_flops, g_n = prof_m
     g_achieved = (g_flops / (g_ms * 1e-3)) / 1e12 if g_ms > 0 else None
     traffic = measured_traffic() if args.workload == "c16h34" and B == 1024 else None
@@ -302,6 +330,7 @@ def main():
         "graph": {"pairs_per_walker": sizes["P"], "triplets_per_walker": sizes["T"], "rows_per_walker": sizes["R"]},
         "roofline": roofline,
         "mcmc": mcmc,
+        "param_grad": grad_leg,
     }
 
     if args.ragged > 0:

@@ -1548,9 +1548,12 @@ void run_grad(Ctx& c, const float* r, int Bc, float* gout, int64_t gstride, cons
   const WRows both{nullptr, 0, nu, nu, (int64_t)Bc * nu, nd, nd};
   const WRows up{nullptr, 0, nu, nu, 0, 0, 0};
   const WRows dn{nullptr, (int64_t)Bc * nu, nd, nd, 0, 0, 0};
+  // partial products of walkers whose rows are split over several workgroups
+  const int64_t part_cap = (int64_t)16 << 20;
+  float* wpart = fb(part_cap);
   auto wg = [&](const float* X, int64_t ldx, const float* Y, int64_t ldy, int K, int J, const WRows& wr, int64_t o,
                 const uint8_t* mask = nullptr, int want = 1) {
-    launch_wgrad(s, X, ldx, Y, ldy, K, J, Bc, wr, gout + o, gstride, false, mask, want);
+    launch_wgrad(s, X, ldx, Y, ldy, K, J, Bc, wr, gout + o, gstride, false, mask, want, wpart, part_cap);
   };
 
   // ---- forward ---------------------------------------------------------------------------
@@ -1819,6 +1822,7 @@ void run_grad(Ctx& c, const float* r, int Bc, float* gout, int64_t gstride, cons
     launch_pair_ranges(s, h->ptr.as<int64_t>(), Bc, nu, nd, prng);
     WRows pr;
     pr.rng = prng;
+    pr.avg_rows = P / std::max(Bc, 1);
     float* pdbeta = fb(P * w1);
     launch_gr_dbeta(s, pdgam, F, eps.WG, epd.WG, psame, w1, P, pdbeta);
     float* ppg = fb(P * PG_FLOATS);

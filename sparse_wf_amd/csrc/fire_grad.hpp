@@ -8,13 +8,15 @@
 namespace fire {
 
 // Rows of walker b: [rng[4b], rng[4b+1]) and [rng[4b+2], rng[4b+3]) when rng is set, else
-// [a0 + b as, + na) and [c0 + b cs, + nc).
+// [a0 + b as, + na) and [c0 + b cs, + nc).  avg_rows: the mean row count per walker when rng is
+// set (sizes the split of each walker's rows over workgroups).
 struct WRows {
   const int64_t* rng = nullptr;
   int64_t a0 = 0, as = 0;
   int na = 0;
   int64_t c0 = 0, cs = 0;
   int nc = 0;
+  int64_t avg_rows = 0;
 };
 
 // per-edge forward record (e-e pairs and e-n edges)
@@ -61,9 +63,11 @@ struct GradRows {
   float *dorb, *denv, *dz3, *dqv, *dkey;
 };
 
+// part (optional, part_cap floats): scratch for per-slice partial products when a walker's
+// rows are split over several workgroups; without it every walker's rows run in one.
 void launch_wgrad(hipStream_t s, const float* X, int64_t ldx, const float* Y, int64_t ldy, int K, int J, int B,
                   const WRows& wr, float* out, int64_t so, bool accumulate = false, const uint8_t* rowmask = nullptr,
-                  int want = 1);
+                  int want = 1, float* part = nullptr, int64_t part_cap = 0);
 void launch_silu_bwd(hipStream_t s, float* dz, int64_t ldo, const float* dy, int64_t ldy, const float* z, int64_t ldz,
                      int64_t rows, int cols);
 void launch_silu_fwd(hipStream_t s, float* y, const float* z, int64_t n);

@@ -13,6 +13,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "fire_common.hpp"
 #include "fire_kernels.hpp"
 #include "fire_grad.hpp"
@@ -40,6 +42,13 @@ __device__ __forceinline__ void cutoff_poly(float x, float& f, float& df) {
   df = -60.0f * x3 + 120.0f * x4 - 60.0f * x4 * x;
 }
 
+__device__ __forceinline__ float4 ld4(const float* p) { return *reinterpret_cast<const float4*>(p); }
+__device__ __forceinline__ void st4(float* p, float4 v) { *reinterpret_cast<float4*>(p) = v; }
+__device__ __forceinline__ float4 fma4(float a, float4 b, float4 c) {
+  return make_float4(fmaf(a, b.x, c.x), fmaf(a, b.y, c.y), fmaf(a, b.z, c.z), fmaf(a, b.w, c.w));
+}
+__device__ __forceinline__ float4 mul4(float4 a, float4 b) { return make_float4(a.x * b.x, a.y * b.y, a.z * b.z, a.w * b.w); }
+
 __device__ __forceinline__ void group_walker(int64_t g, int B, int nu, int nd, int& w, int& i) {
   const int64_t up = (int64_t)B * nu;
   if (g < up) {
@@ -62,17 +71,24 @@ __device__ __forceinline__ int64_t group_index(int w, int i, int B, int nu, int 
 // out[b][k][j] (+)= sum over the rows of walker b of X[row][k] Y[row][j] (X == nullptr: 1).
 // Rows of walker b: [rng[4b], rng[4b+1]) and [rng[4b+2], rng[4b+3]), or without rng the fixed
 // ranges a0 + b as .. + na and c0 + b cs .. + nc.  rowmask (optional): only rows with
-// rowmask[row] == want.  One block per (64 x 64 output tile, walker); 16 rows per LDS chunk.
+// rowmask[row] == want.  One block per (64 x 64 output tile, walker, slice): with S > 1 slices
+// the walker's rows (both ranges, concatenated) are cut into S consecutive parts whose partial
+// products go to part[(slice B + b) K J ..] and are summed in slice order by k_wgrad_reduce;
+// with S == 1 the block writes out directly.  16 rows per LDS chunk.
 // ---------------------------------------------------------------------------------------
+template <int TK, int TJ>
 __global__ void __launch_bounds__(256) k_wgrad(const float* __restrict__ X, int64_t ldx, const float* __restrict__ Y,
                                                int64_t ldy, int K, int J, WRows wr, const uint8_t* __restrict__ rowmask,
-                                               int want, float* __restrict__ out, int64_t so, int accumulate) {
-  __shared__ float xs[16][64 + 1];
-  __shared__ float ys[16][64 + 1];
-  const int b = blockIdx.y;
-  const int tk = (K + 63) / 64;
-  const int k0 = (blockIdx.x % tk) * 64, j0 = (blockIdx.x / tk) * 64;
-  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+                                               int want, float* __restrict__ out, int64_t so, int accumulate,
+                                               float* __restrict__ part) {
+  static_assert(TK * TJ == 4096, "16 outputs per thread");
+  constexpr int TXN = TJ / 4;  // threads along j; each takes columns tx, tx + TXN, tx + 2 TXN, tx + 3 TXN
+  __shared__ float xs[16][TK + 1];
+  __shared__ float ys[16][TJ + 1];
+  const int b = blockIdx.y, S = gridDim.z, sl = blockIdx.z;
+  const int tk = (K + TK - 1) / TK;
+  const int k0 = (blockIdx.x % tk) * TK, j0 = (blockIdx.x / tk) * TJ;
+  const int tx = threadIdx.x % TXN, ty = threadIdx.x / TXN;
   int64_t s[2], e[2];
   if (wr.rng) {
     s[0] = wr.rng[4 * b];
@@ -85,20 +101,36 @@ __global__ void __launch_bounds__(256) k_wgrad(const float* __restrict__ X, int6
     s[1] = wr.c0 + (int64_t)b * wr.cs;
     e[1] = s[1] + wr.nc;
   }
+  if (S > 1) {  // this slice's part of the concatenated ranges
+    const int64_t n0 = e[0] - s[0], n = n0 + (e[1] - s[1]);
+    const int64_t f0 = n * sl / S, f1 = n * (sl + 1) / S;
+    const int64_t a = s[0], c = s[1];
+    s[0] = a + (f0 < n0 ? f0 : n0);
+    e[0] = a + (f1 < n0 ? f1 : n0);
+    s[1] = c + (f0 > n0 ? f0 - n0 : 0);
+    e[1] = c + (f1 > n0 ? f1 - n0 : 0);
+  }
   float acc[4][4];
 #pragma unroll
   for (int u = 0; u < 4; ++u)
 #pragma unroll
     for (int v = 0; v < 4; ++v) acc[u][v] = 0.f;
-  for (int part = 0; part < 2; ++part) {
-    for (int64_t r0 = s[part]; r0 < e[part]; r0 += 16) {
-      for (int t = threadIdx.x; t < 16 * 64; t += 256) {
-        const int rr = t >> 6, cc = t & 63;
+  for (int pt = 0; pt < 2; ++pt) {
+    for (int64_t r0 = s[pt]; r0 < e[pt]; r0 += 16) {
+      for (int t = threadIdx.x; t < 16 * TK; t += 256) {
+        const int rr = t / TK, cc = t - rr * TK;
         const int64_t row = r0 + rr;
-        bool ok = row < e[part];
+        bool ok = row < e[pt];
         if (ok && rowmask) ok = rowmask[row] == want;
-        const int k = k0 + cc, j = j0 + cc;
+        const int k = k0 + cc;
         xs[rr][cc] = ok && k < K ? (X ? X[row * ldx + k] : 1.f) : 0.f;
+      }
+      for (int t = threadIdx.x; t < 16 * TJ; t += 256) {
+        const int rr = t / TJ, cc = t - rr * TJ;
+        const int64_t row = r0 + rr;
+        bool ok = row < e[pt];
+        if (ok && rowmask) ok = rowmask[row] == want;
+        const int j = j0 + cc;
         ys[rr][cc] = ok && j < J ? Y[row * ldy + j] : 0.f;
       }
       __syncthreads();
@@ -108,7 +140,7 @@ __global__ void __launch_bounds__(256) k_wgrad(const float* __restrict__ X, int6
 #pragma unroll
         for (int u = 0; u < 4; ++u) xv[u] = xs[rr][ty * 4 + u];
 #pragma unroll
-        for (int v = 0; v < 4; ++v) yv[v] = ys[rr][tx * 4 + v];
+        for (int v = 0; v < 4; ++v) yv[v] = ys[rr][tx + v * TXN];
 #pragma unroll
         for (int u = 0; u < 4; ++u)
 #pragma unroll
@@ -121,19 +153,57 @@ __global__ void __launch_bounds__(256) k_wgrad(const float* __restrict__ X, int6
   for (int u = 0; u < 4; ++u)
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      const int k = k0 + ty * 4 + u, j = j0 + tx * 4 + v;
+      const int k = k0 + ty * 4 + u, j = j0 + tx + v * TXN;
       if (k < K && j < J) {
-        float* o = out + (int64_t)b * so + (int64_t)k * J + j;
-        *o = accumulate ? *o + acc[u][v] : acc[u][v];
+        if (S > 1) {
+          part[((int64_t)sl * gridDim.y + b) * K * J + (int64_t)k * J + j] = acc[u][v];
+        } else {
+          float* o = out + (int64_t)b * so + (int64_t)k * J + j;
+          *o = accumulate ? *o + acc[u][v] : acc[u][v];
+        }
       }
     }
 }
 
+// out[b][kj] (+)= sum over slices (in order) of part[(slice B + b) K J + kj]
+__global__ void k_wgrad_reduce(const float* __restrict__ part, int S, int B, int64_t KJ, float* __restrict__ out,
+                               int64_t so, int accumulate) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)B * KJ) return;
+  const int b = (int)(e / KJ);
+  const int64_t kj = e - (int64_t)b * KJ;
+  float acc = 0.f;
+  for (int sl = 0; sl < S; ++sl) acc += part[((int64_t)sl * B + b) * KJ + kj];
+  float* o = out + (int64_t)b * so + kj;
+  *o = accumulate ? *o + acc : acc;
+}
+
 void launch_wgrad(hipStream_t s, const float* X, int64_t ldx, const float* Y, int64_t ldy, int K, int J, int B,
-                  const WRows& wr, float* out, int64_t so, bool accumulate, const uint8_t* rowmask, int want) {
+                  const WRows& wr, float* out, int64_t so, bool accumulate, const uint8_t* rowmask, int want,
+                  float* part, int64_t part_cap) {
   if (B == 0 || K == 0 || J == 0) return;
-  const dim3 grid((unsigned)(((K + 63) / 64) * ((J + 63) / 64)), (unsigned)B);
-  k_wgrad<<<grid, 256, 0, s>>>(X, ldx, Y, ldy, K, J, wr, rowmask, want, out, so, accumulate ? 1 : 0);
+  // tile shape: 16 x 256 for narrow products (K <= 16 against J > 64), else 64 x 64
+  const int TK = K <= 16 && J > 64 ? 16 : 64, TJ = 4096 / TK;
+  const int tiles = ((K + TK - 1) / TK) * ((J + TJ - 1) / TJ);
+  // slices: at least about 96 rows each, enough blocks for 8192 in flight, at most 32, and
+  // the partial products within the scratch
+  const int64_t rows = wr.rng ? wr.avg_rows : (int64_t)wr.na + wr.nc;
+  int S = 1;
+  if (part) {
+    const int64_t want_blocks = (8192 + (int64_t)tiles * B - 1) / ((int64_t)tiles * B);
+    S = (int)std::max<int64_t>(1, std::min<int64_t>({32, rows / 96, want_blocks}));
+    while (S > 1 && (int64_t)S * B * K * J > part_cap) --S;
+  }
+  const dim3 grid((unsigned)tiles, (unsigned)B, (unsigned)S);
+  const int acc = accumulate ? 1 : 0;
+  if (TK == 16)
+    k_wgrad<16, 256><<<grid, 256, 0, s>>>(X, ldx, Y, ldy, K, J, wr, rowmask, want, out, so, acc, part);
+  else
+    k_wgrad<64, 64><<<grid, 256, 0, s>>>(X, ldx, Y, ldy, K, J, wr, rowmask, want, out, so, acc, part);
+  if (S > 1) {
+    const int64_t n = (int64_t)B * K * J;
+    k_wgrad_reduce<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(part, S, B, (int64_t)K * J, out, so, accumulate ? 1 : 0);
+  }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -672,18 +742,19 @@ __global__ void k_gr_pair_fwd(const float* __restrict__ r, int B, int N, int nu,
   }
 }
 
-// Messages backward per (pair, feature): Gamma = beta WG, e = lsf WF + BF (same- or
+// Messages backward per (pair, 4 features): Gamma = beta WG, e = lsf WF + BF (same- or
 // different-spin edge weights), u = e + m_ct[i] + m_nb[j], msg = Gamma silu(u):
-// dGamma = dh_i silu(u), du = dh_i Gamma silu'(u).
+// dGamma = dh_i silu(u), du = dh_i Gamma silu'(u).  F % 4 == 0.
 __global__ void k_gr_msg_bwd(const float* __restrict__ beta, const float* __restrict__ lsf, EdgeParams ps,
                              EdgeParams pd, const float* __restrict__ M5, const float* __restrict__ dh,
                              const int32_t* __restrict__ ctr, const int32_t* __restrict__ nbr,
                              const uint8_t* __restrict__ same, int B, int N, int nu, int F, int64_t P,
                              float* __restrict__ dgam, float* __restrict__ du) {
+  const int F4 = F >> 2;
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= P * F) return;
-  const int64_t p = e / F;
-  const int f = (int)(e - p * F);
+  if (e >= P * F4) return;
+  const int64_t p = e / F4;
+  const int f = (int)(e - p * F4) * 4;
   const int64_t g = ctr[p];
   int w, i;
   group_walker(g, B, nu, N - nu, w, i);
@@ -692,36 +763,84 @@ __global__ void k_gr_msg_bwd(const float* __restrict__ beta, const float* __rest
   const bool sm = same[p] != 0;
   const EdgeParams& ep = sm ? ps : pd;
   const int w1 = ep.filt.w1;
-  float gam = 0.f, ev = ep.BF[f];
-  for (int k = 0; k < w1; ++k) gam += beta[p * w1 + k] * ep.WG[k * F + f];
-  for (int c = 0; c < 4; ++c) ev += lsf[p * 4 + c] * ep.WF[c * F + f];
-  const float mct = M5[g * 5 * F + (sm ? 1 : 2) * F + f];
-  const float mnb = M5[gj * 5 * F + (sm ? 3 : 4) * F + f];
-  const float u = ev + mct + mnb;
-  const float s = sigm(u);
-  const float dhv = dh[g * F + f];
-  dgam[e] = dhv * u * s;
-  du[e] = dhv * gam * s * (1.0f + u * (1.0f - s));
+  float4 gam = make_float4(0.f, 0.f, 0.f, 0.f), ev = ld4(ep.BF + f);
+  for (int k = 0; k < w1; ++k) gam = fma4(beta[p * w1 + k], ld4(ep.WG + k * F + f), gam);
+  for (int c = 0; c < 4; ++c) ev = fma4(lsf[p * 4 + c], ld4(ep.WF + c * F + f), ev);
+  const float4 mct = ld4(M5 + g * 5 * F + (sm ? 1 : 2) * F + f);
+  const float4 mnb = ld4(M5 + gj * 5 * F + (sm ? 3 : 4) * F + f);
+  const float4 dhv = ld4(dh + g * F + f);
+  const float uu[4] = {ev.x + mct.x + mnb.x, ev.y + mct.y + mnb.y, ev.z + mct.z + mnb.z, ev.w + mct.w + mnb.w};
+  const float gg[4] = {gam.x, gam.y, gam.z, gam.w}, dd[4] = {dhv.x, dhv.y, dhv.z, dhv.w};
+  float og[4], ou[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const float s = sigm(uu[c]);
+    og[c] = dd[c] * uu[c] * s;
+    ou[c] = dd[c] * gg[c] * s * (1.0f + uu[c] * (1.0f - s));
+  }
+  st4(dgam + p * F + f, make_float4(og[0], og[1], og[2], og[3]));
+  st4(du + p * F + f, make_float4(ou[0], ou[1], ou[2], ou[3]));
 }
 
-// d beta[p][k] = sum_f dGamma[p][f] WG[k][f] (same- or different-spin weights); one
-// wavefront per row, four features per lane.
-__global__ void k_gr_dbeta(const float* __restrict__ dgam, int F, const float* __restrict__ WGs,
-                           const float* __restrict__ WGd, const uint8_t* __restrict__ same, int w1, int64_t P,
-                           float* __restrict__ dbeta) {
-  const int64_t p = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (p >= P) return;
-  const float* WG = (same == nullptr || same[p]) ? WGs : WGd;
-  float acc[FIRE_W1MAX];
-  for (int k = 0; k < w1; ++k) acc[k] = 0.f;
-  for (int f = lane; f < F; f += 64) {
-    const float v = dgam[p * F + f];
-    for (int k = 0; k < w1; ++k) acc[k] += v * WG[k * F + f];
+// d beta[p][k] = sum_f dGamma[p][f] WG[k][f] (same- or different-spin weights).  A block takes
+// 256 rows; dGamma and both weight variants pass through LDS in coalesced 32-feature chunks;
+// thread (r, quarter) accumulates rows r, r + 64, r + 128, r + 192 times the quarter's w1 / 4
+// outputs in registers (no cross-lane sums; the weights are LDS broadcasts).
+__global__ void __launch_bounds__(256) k_gr_dbeta(const float* __restrict__ dgam, int F, const float* __restrict__ WGs,
+                                                  const float* __restrict__ WGd, const uint8_t* __restrict__ same,
+                                                  int w1, int64_t P, float* __restrict__ dbeta) {
+  constexpr int RB = 256, FC = 32;
+  __shared__ float tile[RB][FC + 1];
+  __shared__ float wgl[2][FIRE_W1MAX][FC];
+  const int64_t p0 = (int64_t)blockIdx.x * RB;
+  const int rl = threadIdx.x & 63, qt = threadIdx.x >> 6;
+  const int kq = w1 >> 2;  // outputs per quarter (1, 2 or 4)
+  bool sel[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const int64_t p = p0 + rl + 64 * v;
+    sel[v] = same == nullptr || (p < P && same[p]);
   }
-  for (int k = 0; k < w1; ++k) {
-    const float v = warp_sum(acc[k]);
-    if (lane == 0) dbeta[p * w1 + k] = v;
+  float acc[4][4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+#pragma unroll
+    for (int u = 0; u < 4; ++u) acc[v][u] = 0.f;
+  for (int f0 = 0; f0 < F; f0 += FC) {
+    for (int t = threadIdx.x; t < RB * FC; t += 256) {
+      const int r = t / FC, c = t - r * FC;
+      const int64_t q = p0 + r;
+      tile[r][c] = (q < P && f0 + c < F) ? dgam[q * F + f0 + c] : 0.f;
+    }
+    for (int t = threadIdx.x; t < 2 * w1 * FC; t += 256) {
+      const int vr = t / (w1 * FC), rem = t - vr * w1 * FC, k = rem / FC, c = rem - k * FC;
+      wgl[vr][k][c] = f0 + c < F ? (vr == 0 ? WGs : WGd)[k * F + f0 + c] : 0.f;
+    }
+    __syncthreads();
+    const int fn = F - f0 < FC ? F - f0 : FC;
+    for (int c = 0; c < fn; ++c) {
+      float ws[4], wd[4], x[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        ws[u] = u < kq ? wgl[0][qt * kq + u][c] : 0.f;
+        wd[u] = u < kq ? wgl[1][qt * kq + u][c] : 0.f;
+      }
+#pragma unroll
+      for (int v = 0; v < 4; ++v) x[v] = tile[rl + 64 * v][c];
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc[v][u] += x[v] * (sel[v] ? ws[u] : wd[u]);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const int64_t p = p0 + rl + 64 * v;
+    if (p < P)
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (u < kq) dbeta[p * w1 + qt * kq + u] = acc[v][u];
   }
 }
 
@@ -857,23 +976,27 @@ __global__ void k_gr_en_fwd(const float* __restrict__ r, const float* __restrict
 }
 
 // h_en[g][f] = sum_m mask Gamma_gm[f] (e_gm[f] + h_nuc[m][f]) with Gamma = beta WG and
-// e = lsf WF (rows g M + m); thread per (group, feature).
+// e = lsf WF (rows g M + m); thread per (group, 4 features), F % 4 == 0.
 __global__ void k_gr_en_sum(const float* __restrict__ beta, const float* __restrict__ lsf, EnParams ep,
                             const uint8_t* __restrict__ mask, int64_t G, int M, int F, float* __restrict__ hen) {
+  const int F4 = F >> 2;
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= G * F) return;
-  const int64_t g = e / F;
-  const int f = (int)(e - g * F);
-  float acc = 0.f;
+  if (e >= G * F4) return;
+  const int64_t g = e / F4;
+  const int f = (int)(e - g * F4) * 4;
+  float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
   for (int m = 0; m < M; ++m) {
     const int64_t q = g * M + m;
     if (!mask[q]) continue;
-    float gam = 0.f, ev = ep.hnuc[(int64_t)m * F + f];
-    for (int k = 0; k < ep.w1; ++k) gam += beta[q * ep.w1 + k] * ep.WG[k * F + f];
-    for (int c = 0; c < 4; ++c) ev += lsf[q * 4 + c] * ep.WF[c * F + f];
-    acc += gam * ev;
+    float4 gam = make_float4(0.f, 0.f, 0.f, 0.f), ev = ld4(ep.hnuc + (int64_t)m * F + f);
+    for (int k = 0; k < ep.w1; ++k) gam = fma4(beta[q * ep.w1 + k], ld4(ep.WG + k * F + f), gam);
+    for (int c = 0; c < 4; ++c) ev = fma4(lsf[q * 4 + c], ld4(ep.WF + c * F + f), ev);
+    acc.x += gam.x * ev.x;
+    acc.y += gam.y * ev.y;
+    acc.z += gam.z * ev.z;
+    acc.w += gam.w * ev.w;
   }
-  hen[e] = acc;
+  st4(hen + g * F + f, acc);
 }
 
 // GLU forward: y = x silu(gate), [x | gate] = Z2 [G, 2F]
@@ -929,27 +1052,29 @@ __global__ void k_gr_glu_bwd(const float* __restrict__ dy, const float* __restri
   dZ2[g * 2 * F + F + f] = d * x * silu_d(gt);
 }
 
-// e-n edges backward per (edge, feature): dGamma = mask dh_en (e + h_nuc), de = mask dh_en Gamma
+// e-n edges backward per (edge, 4 features): dGamma = mask dh_en (e + h_nuc),
+// de = mask dh_en Gamma; F % 4 == 0.
 __global__ void k_gr_en_bwd_f(const float* __restrict__ beta, const float* __restrict__ lsf, EnParams ep,
                               const uint8_t* __restrict__ mask, const float* __restrict__ dhen, int64_t rows, int M,
                               int F, float* __restrict__ dgam, float* __restrict__ de) {
+  const int F4 = F >> 2;
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= rows * F) return;
-  const int64_t q = e / F;
-  const int f = (int)(e - q * F);
+  if (e >= rows * F4) return;
+  const int64_t q = e / F4;
+  const int f = (int)(e - q * F4) * 4;
   const int64_t g = q / M;
   const int m = (int)(q - g * M);
   if (!mask[q]) {
-    dgam[e] = 0.f;
-    de[e] = 0.f;
+    st4(dgam + q * F + f, make_float4(0.f, 0.f, 0.f, 0.f));
+    st4(de + q * F + f, make_float4(0.f, 0.f, 0.f, 0.f));
     return;
   }
-  float gam = 0.f, ev = ep.hnuc[(int64_t)m * F + f];
-  for (int k = 0; k < ep.w1; ++k) gam += beta[q * ep.w1 + k] * ep.WG[k * F + f];
-  for (int c = 0; c < 4; ++c) ev += lsf[q * 4 + c] * ep.WF[c * F + f];
-  const float dh = dhen[g * F + f];
-  dgam[e] = dh * ev;
-  de[e] = dh * gam;
+  float4 gam = make_float4(0.f, 0.f, 0.f, 0.f), ev = ld4(ep.hnuc + (int64_t)m * F + f);
+  for (int k = 0; k < ep.w1; ++k) gam = fma4(beta[q * ep.w1 + k], ld4(ep.WG + k * F + f), gam);
+  for (int c = 0; c < 4; ++c) ev = fma4(lsf[q * 4 + c], ld4(ep.WF + c * F + f), ev);
+  const float4 dh = ld4(dhen + g * F + f);
+  st4(dgam + q * F + f, mul4(dh, ev));
+  st4(de + q * F + f, mul4(dh, gam));
 }
 
 // e-n filter backward per edge (thread per edge) from d beta [rows, w1]
@@ -1037,14 +1162,14 @@ void launch_gr_pair_fwd(hipStream_t s, const float* r, int B, int N, int nu, con
 void launch_gr_msg_bwd(hipStream_t s, const float* beta, const float* lsf, const EdgeParams& ps, const EdgeParams& pd,
                        const float* M5, const float* dh, const int32_t* ctr, const int32_t* nbr, const uint8_t* same,
                        int B, int N, int nu, int F, int64_t P, float* dgam, float* du) {
-  const int64_t n = P * F;
+  const int64_t n = P * (F / 4);
   if (n)
     k_gr_msg_bwd<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(beta, lsf, ps, pd, M5, dh, ctr, nbr, same, B, N, nu, F, P,
                                                              dgam, du);
 }
 void launch_gr_dbeta(hipStream_t s, const float* dgam, int F, const float* WG_same, const float* WG_diff,
                      const uint8_t* same, int w1, int64_t P, float* dbeta) {
-  if (P) k_gr_dbeta<<<(unsigned)((P + 3) / 4), 256, 0, s>>>(dgam, F, WG_same, WG_diff, same, w1, P, dbeta);
+  if (P) k_gr_dbeta<<<(unsigned)((P + 255) / 256), 256, 0, s>>>(dgam, F, WG_same, WG_diff, same, w1, P, dbeta);
 }
 void launch_gr_msg_collect(hipStream_t s, const float* du, const int64_t* ptr, const int32_t* nbr, const int32_t* rev,
                            const uint8_t* same, int B, int N, int nu, int F, float* dM5) {
@@ -1061,7 +1186,7 @@ void launch_gr_en_fwd(hipStream_t s, const float* r, const float* R, int B, int 
 }
 void launch_gr_en_sum(hipStream_t s, const float* beta, const float* lsf, const EnParams& ep, const uint8_t* mask,
                       int64_t G, int M, int F, float* hen) {
-  const int64_t n = G * F;
+  const int64_t n = G * (F / 4);
   if (n) k_gr_en_sum<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(beta, lsf, ep, mask, G, M, F, hen);
 }
 void launch_gr_glu_fwd(hipStream_t s, const float* Z2, int64_t G, int F, float* y) {
@@ -1082,7 +1207,7 @@ void launch_gr_glu_bwd(hipStream_t s, const float* dy, const float* Z2, int64_t 
 }
 void launch_gr_en_bwd_f(hipStream_t s, const float* beta, const float* lsf, const EnParams& ep, const uint8_t* mask,
                         const float* dhen, int64_t rows, int M, int F, float* dgam, float* de) {
-  const int64_t n = rows * F;
+  const int64_t n = rows * (F / 4);
   if (n) k_gr_en_bwd_f<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(beta, lsf, ep, mask, dhen, rows, M, F, dgam, de);
 }
 void launch_gr_en_bwd(hipStream_t s, const float* rec, const float* dbeta, const EnParams& ep, float cutoff_1el,
