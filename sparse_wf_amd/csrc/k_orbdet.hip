@@ -460,6 +460,43 @@ __device__ __forceinline__ double readlane_f64(double x, int l) {  // l uniform
   return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
 
+// (best, bi) <- the better of itself and the lane CTRL (a DPP lane pattern) names: larger
+// |value|, ties to the lower row
+template <int CTRL>
+__device__ __forceinline__ void argmax_dpp_step(double& best, int& bi) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, best);
+  const int lo = __builtin_amdgcn_update_dpp((int)(uint32_t)u, (int)(uint32_t)u, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)(uint32_t)(u >> 32), (int)(uint32_t)(u >> 32), CTRL, 0xF, 0xF, false);
+  const int oi = __builtin_amdgcn_update_dpp(bi, bi, CTRL, 0xF, 0xF, false);
+  const double ob = __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+  const bool take = ob > best || (ob == best && oi < bi);
+  best = take ? ob : best;
+  bi = take ? oi : bi;
+}
+
+// Wavefront arg-max of (best, bi) (larger value, ties to the lower index; the order is total,
+// so any reduction tree gives the same winner): DPP within rows of 16 lanes (quad swaps, half-
+// row and row mirrors), then the four row winners through readlane.  Returns the winner's
+// index, uniform over the wavefront.
+__device__ __forceinline__ int argmax_wave(double best, int bi) {
+  argmax_dpp_step<0xB1>(best, bi);   // quad_perm [1,0,3,2]
+  argmax_dpp_step<0x4E>(best, bi);   // quad_perm [2,3,0,1]
+  argmax_dpp_step<0x141>(best, bi);  // row_half_mirror
+  argmax_dpp_step<0x140>(best, bi);  // row_mirror
+  double b = readlane_f64(best, 0);
+  int i = __builtin_amdgcn_readlane(bi, 0);
+#pragma unroll
+  for (int r = 1; r < 4; ++r) {
+    const double ob = readlane_f64(best, 16 * r);
+    const int oi = __builtin_amdgcn_readlane(bi, 16 * r);
+    if (ob > b || (ob == b && oi < i)) {
+      b = ob;
+      i = oi;
+    }
+  }
+  return i;
+}
+
 // Column-distributed register Gauss-Jordan inverse with implicit partial pivoting, FP64: the
 // same pivot sequence and result layout as k_gj_inv_reg.  Column j lives in wavefront j % NW
 // (register c = j / NW), row i in lane i % 64 (register m = i / 64).  The wavefront holding
@@ -477,6 +514,7 @@ __global__ void __launch_bounds__(NW * 64) k_gj_inv_col(const double* __restrict
   __shared__ int s_p[2];
   __shared__ int s_piv[RPT * 64];
   __shared__ int s_inv[RPT * 64];
+  __shared__ double s_pvk[RPT * 64];  // pivot of step k (log |det| summed after the loop)
   const int mat = blockIdx.x;
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   const double* Ag = Aall + (size_t)mat * N * N;
@@ -517,15 +555,7 @@ __global__ void __launch_bounds__(NW * 64) k_gj_inv_col(const double* __restrict
           bi = i;
         }
       }
-#pragma unroll
-      for (int o = 32; o > 0; o >>= 1) {
-        const double ob = __shfl_xor(best, o, 64);
-        const int oi = __shfl_xor(bi, o, 64);
-        if (ob > best || (ob == best && oi < bi)) {
-          best = ob;
-          bi = oi;
-        }
-      }
+      bi = argmax_wave(best, bi);
       if (lane == (bi & 63)) {
         double x = 0.0;
 #pragma unroll
@@ -540,8 +570,8 @@ __global__ void __launch_bounds__(NW * 64) k_gj_inv_col(const double* __restrict
     const double pv = s_pv[buf];
     const double ip = 1.0 / pv;
     if (tid == 0) {
-      ld += log(fabs(pv));
       neg += pv < 0.0;
+      s_pvk[k] = pv;
       s_piv[k] = p;
     }
     const int mp = p >> 6, lp = p & 63;
@@ -576,9 +606,13 @@ __global__ void __launch_bounds__(NW * 64) k_gj_inv_col(const double* __restrict
     }
   }
   __syncthreads();
-  for (int k = tid; k < N; k += NW * 64) s_inv[s_piv[k]] = k;
+  for (int k = tid; k < N; k += NW * 64) {
+    s_inv[s_piv[k]] = k;
+    s_pvk[k] = log(fabs(s_pvk[k]));
+  }
   __syncthreads();
   if (tid == 0) {
+    for (int k = 0; k < N; ++k) ld += s_pvk[k];  // summed in step order
     // parity of k -> piv(k): N minus the number of cycles (visited marks kept in s_inv's sign bit)
     int cycles = 0;
     for (int s0 = 0; s0 < N; ++s0) {
