@@ -43,11 +43,11 @@ enum fire_precision {
   FIRE_FP32 = 0,  /* every embedding / orbital / Jastrow contraction in FP32 (FP32 MFMA) */
   FIRE_MIXED = 1, /* dense contractions on the 16-bit matrix cores with FP32 accumulation
                      (see fire_precision_plan): on the tangent rows of the local energy the
-                     update-MLP and orbital contractions carry each operand as two FP16 planes
-                     of a power-of-two-scaled value (per row of A, per column of W; three
-                     products, about 2^-22 relative each); the ElecInit MLP and the value-path
-                     contractions carry three BF16 planes (six products, FP32-level), the
-                     Jastrow and attention contractions a BF16 hi + lo pair (three products);
+                     ElecInit MLP, update-MLP and orbital contractions carry each operand as two
+                     FP16 planes of a power-of-two-scaled value (per row of A, per column of W;
+                     three products, about 2^-22 relative each); the value-path contractions
+                     carry three BF16 planes (six products, FP32-level), the Jastrow and
+                     attention contractions a BF16 hi + lo pair (three products);
                      element-wise forward-Laplacian rules, envelopes and Jastrow sums FP32;
                      Slater determinants, their inverses, updates and energy sums FP64 */
   FIRE_BF16 = 2,  /* dense contractions in single BF16 on MFMA (fastest, least accurate) */
@@ -196,7 +196,7 @@ int fire_state_stats(fire_handle* h, int32_t* n_walkers, int64_t* n_proposals, i
  * planes of operands scaled by powers of two (each row of A by its maximum, which the kernel
  * producing A leaves, each column of W at upload), three products, about 2^-22 relative per
  * product; where no row maxima exist (value path, sampler) class 4 runs as 3.  Default
- * {3,4,4,2,2,2}.  Sets the first n classes from planes (n may be 0); current (optional,
+ * {4,4,4,2,2,2}.  Sets the first n classes from planes (n may be 0); current (optional,
  * int32 [6]) receives the plan. */
 int fire_precision_plan(fire_handle* h, const int32_t* planes, int32_t n, int32_t* current);
 

@@ -202,7 +202,7 @@ struct fire_handle {
   int iORBA = -1, iJV = -1;
   int ldo_orb = 0;  // row stride of the orbital rows of the last run_orbitals_det
   // workspace
-  DevBuf cnt, ptr, nbr, rev, rowflag, rowgrp;
+  DevBuf cnt, ptr, nbr, rev, rowflag, rowgrp, AMX_E;
   DevBuf AV0, Z0, F1, PART;  // fused silu epilogue: packed value rows, their pre-activations, silu', slots
   DevBuf AMX_A, AMX_B, AMX_C;  // row maxima (float bits) of XA, XB, XC for the FP16 planes
   const uint32_t* amx_xa = nullptr;  // row maxima of the embedding rows in XA (null: none)
@@ -246,7 +246,7 @@ struct fire_handle {
   // its row maxima (tangent rows of the local energy), three BF16 planes elsewhere.  The
   // Jastrow classes stay on a BF16 pair: on C16H34 the FP16 planes there leave the error
   // against FP64 unchanged and are slower (N = 512 joint contraction).
-  int planes[6] = {3, 4, 4, 2, 2, 2};
+  int planes[6] = {4, 4, 4, 2, 2, 2};
   std::map<std::tuple<int, int, int, int>, int> gemm_tile;
   std::map<std::tuple<int, int, int, int>, std::vector<float>> gemm_tile_ms;
   // Resident walker state for single-electron low-rank updates (fire_state_*, fire_mcmc_steps).
@@ -673,12 +673,24 @@ void run_embedding(Ctx& c, const float* r, int B, const Graph& gr, bool tan, Gro
   launch_elec_init_en(c.s, r, h->R, B, N, h->n_up, h->M, h->cfg.cutoff_1el, F, en_params(h), tan, h->T5.as<float>(),
                       nullptr, 0, ef, eok);
   const int64_t rows5 = (int64_t)G * NR;
-  gemm(c, h->T5.as<float>(), F, b + 11, -1, h->Z2.as<float>(), 2 * F, rows5, 2 * F, F, nullptr, 0);
+  // FP16 planes for the ElecInit MLP (tangent rows, class 0 set to 4): row maxima of each input
+  ActEpi e0;
+  const ActEpi* pe0 = nullptr;
+  if (tan && c.prec == FIRE_MIXED && h->planes[0] == 4) {
+    h->AMX_E.ensure(sizeof(uint32_t) * (size_t)rows5);
+    e0.amax = h->AMX_E.as<uint32_t>();
+    pe0 = &e0;
+  }
+  if (pe0) launch_rowmax(c.s, h->T5.as<float>(), F, rows5, F, h->AMX_E.as<uint32_t>());
+  gemm(c, h->T5.as<float>(), F, b + 11, -1, h->Z2.as<float>(), 2 * F, rows5, 2 * F, F, nullptr, 0, DevRows(), pe0);
   launch_glu(c.s, h->Z2.as<float>(), G, F, tan, h->T5b.as<float>());
-  gemm(c, h->T5b.as<float>(), F, b + 12, b + 13, h->T5.as<float>(), F, rows5, F, F, nullptr, NR);
+  if (pe0) launch_rowmax(c.s, h->T5b.as<float>(), F, rows5, F, h->AMX_E.as<uint32_t>());
+  gemm(c, h->T5b.as<float>(), F, b + 12, b + 13, h->T5.as<float>(), F, rows5, F, F, nullptr, NR, DevRows(), pe0);
   GroupDesc g5{NR, 0, nullptr};
   launch_act(c.s, h->T5.as<float>(), F, F, G, g5, ACT_SILU, 0.f);
-  gemm(c, h->T5.as<float>(), F, b + 14, b + 15, h->M5.as<float>(), 5 * F, rows5, 5 * F, F, nullptr, NR);
+  if (pe0) launch_rowmax(c.s, h->T5.as<float>(), F, rows5, F, h->AMX_E.as<uint32_t>());
+  gemm(c, h->T5.as<float>(), F, b + 14, b + 15, h->M5.as<float>(), 5 * F, rows5, 5 * F, F, nullptr, NR, DevRows(),
+       pe0);
 
   if (tan) {
     gd = GroupDesc{5, 3, h->ptr.as<int64_t>()};
@@ -1429,7 +1441,7 @@ void check_config(const fire_config& c) {
 
 // Every workspace buffer of the handle outside the walker state.
 std::vector<DevBuf*> handle_bufs(fire_handle* h) {
-  std::vector<DevBuf*> v{&h->cnt,    &h->ptr,     &h->nbr,    &h->rev,     &h->rowflag, &h->T5,     &h->Z2,      &h->T5b,
+  std::vector<DevBuf*> v{&h->AMX_E, &h->cnt,    &h->ptr,     &h->nbr,    &h->rev,     &h->rowflag, &h->T5,     &h->Z2,      &h->T5b,
           &h->M5,     &h->XA,      &h->XB,     &h->XC,      &h->O,       &h->ENV,    &h->ENVC,    &h->A64,
           &h->Ainv,   &h->AinvT,   &h->ldet,   &h->dsgn,    &h->tr,      &h->trip,   &h->dgrad,   &h->det_lp,
           &h->det_sg, &h->det_gr,  &h->det_lap, &h->JR,     &h->mlp_ns,  &h->mlp_dg, &h->QV,      &h->AV,
