@@ -273,15 +273,16 @@ int fire_energy_stats(fire_handle* h, void* stream, const double* e_loc_dev, int
                       int32_t stat, double* e_clipped_dev, double* e_diff_dev, double* stats_dev);
 
 /* Per-walker gradient of log|psi| with respect to the uploaded parameters: grad_dev (float
- * [B][n], n from fire_param_count) holds, for each walker of r_dev (float [B][N][3], group
- * order), d log|psi| / d theta for every float of the packed parameter tensors, in upload order
+ * [B][n], n from fire_param_count) holds, for each walker of r_dev (float [B][N][3]),
+ * d log|psi| / d theta for every float of the packed parameter tensors, in upload order
  * (fire_upload_params), each tensor in its packed layout (folded scales, softplus'd envelope
  * widths, transposed pi); logpsi_dev (double [B], optional) receives log|psi|.  The forward
- * pass runs in FP32 with FP64 determinants (LU with partial pivoting); the backward pass is the
- * reverse mode of that value path, determinants through their FP64 inverses.  Walkers are
- * processed in chunks; synchronises the stream.  Replaces the per-walker jax.grad(log_p) of
- * the natural-gradient preconditioners (src/sparse_wf/preconditioner.py:191-194, 290-294,
- * 382-383); the adjoint of the packing to the reference's parameter tree is host work. */
+ * pass runs in FP32 with FP64 determinants (Gauss-Jordan with partial pivoting); the backward
+ * pass is the reverse mode of that value path, determinants through their FP64 inverses.
+ * Walkers are processed in chunks; synchronises the stream; B = 0 does nothing.  Replaces the
+ * per-walker jax.grad(log_p) of the natural-gradient preconditioners
+ * (src/sparse_wf/preconditioner.py:191-194, 290-294, 382-383); the adjoint of the packing to
+ * the reference's parameter tree is host work. */
 int fire_grad_logpsi(fire_handle* h, void* stream, const float* r_dev, int32_t B, float* grad_dev,
                      double* logpsi_dev);
 

@@ -2598,8 +2598,9 @@ int fire_grad_logpsi(fire_handle* h, void* stream, const float* r_dev, int32_t B
                      double* logpsi_dev) {
   try {
     check_ready(h);
-    REQUIRE(r_dev && grad_dev, "null argument");
     REQUIRE(B >= 0, "negative batch");
+    if (B == 0) return 0;
+    REQUIRE(r_dev && grad_dev, "null argument");
     HIPCHK(hipSetDevice(h->device));
     Ctx c{h, (hipStream_t)stream, FIRE_FP32};
     std::vector<int64_t> off(h->expect.size() + 1, 0);

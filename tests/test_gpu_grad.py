@@ -54,3 +54,26 @@ def test_grad_logpsi_matches_oracle(name, gpu_device):
             assert err <= 5e-4 * scale + 1e-5 * total, (path, b, err, scale, total)
     worst.sort(reverse=True)
     print("  worst:", worst[:3], flush=True)
+
+
+def test_grad_logpsi_across_chunks_and_empty(gpu_device):
+    """260 walkers run as chunks of 128, 128 and 4: walkers on either side of each chunk
+    boundary match the oracle to the same bound; an empty batch returns empty rows."""
+    cfg = small_cfg(False)
+    prm = make_params(cfg, seed=0)
+    wf = FireWaveFunction(cfg, device=gpu_device)
+    r = walkers(cfg, 260, seed=11)
+    print("\n[grad chunks] B=260", flush=True)
+    g, lp = wf.grad_logpsi(prm, torch.as_tensor(r, device="cuda"))
+    torch.cuda.synchronize()
+    gl = {p: v.cpu().numpy().astype(np.float64) for p, v in P.leaves_with_path(g)}
+    prm64 = P.tree_map(lambda x: np.asarray(x, np.float64), prm)
+    for b in (0, 127, 128, 255, 256, 259):
+        ref = fire.grad_logpsi(prm64, cfg, r[b])
+        total = np.sqrt(sum(float((v ** 2).sum()) for v in ref.values()))
+        for path, want in ref.items():
+            err = float(np.linalg.norm(gl[path][b].reshape(want.shape) - want))
+            assert err <= 5e-4 * float(np.linalg.norm(want)) + 1e-5 * total, (b, path, err)
+    n = int(sum(t.size for t in P.pack_params(cfg, prm)))
+    g0, lp0 = wf.grad_logpsi(prm, torch.empty((0, cfg.n_el, 3), device="cuda"), packed=True)
+    assert tuple(g0.shape) == (0, n) and tuple(lp0.shape) == (0,)
