@@ -266,9 +266,10 @@ def main():
     traffic = measured_traffic() if args.workload == "c16h34" and B == 1024 else None
     roofline = {
         "bound": "mfma",
-        "kernel": "k_gemm_bf16* (every dense contraction of one mixed step: embedding and orbital "
-                  "contractions on three BF16 planes, Jastrow and attention on two; autotuned tile shapes; "
-                  "the update-MLP and first Jastrow layers with the forward-Laplacian silu in their epilogue)",
+        "kernel": "k_gemm_bf16* (every dense contraction of one mixed step: ElecInit-MLP, update-MLP and "
+                  "orbital contractions on two FP16 planes of power-of-two-scaled operands, Jastrow and "
+                  "attention on a BF16 hi + lo pair, three products each; autotuned tile shapes; the "
+                  "update-MLP and first Jastrow layers with the forward-Laplacian silu in their epilogue)",
         "achieved": g_achieved,
         "peak": PEAK["bf16"],
         "unit": "TFLOP/s",
@@ -279,8 +280,8 @@ def main():
         "launches_per_step": g_n,
         "algorithmic_flop_per_step": g_flops,
         "note": "achieved = algorithmic 2*M*N*K per launch over the HIP-event duration of the launches on "
-                "their stream; the matrix cores execute 6x (three BF16 planes) or 3x (a BF16 pair or two FP16 "
-                "planes) these flops; traffic = HBM bytes per launch from two rocprofv3 --pmc runs "
+                "their stream; the matrix cores execute 3x these flops (three products per multiply-add); "
+                "traffic = HBM bytes per launch from two rocprofv3 --pmc runs "
                 "(FETCH_SIZE x2, WRITE_SIZE x1 by a 1 GiB copy calibration) of this workload, read from the "
                 "committed profile summary; whole-step rate in whole_step_tflops",
         "whole_step_tflops": flops_w * wps_m / 1e12,
