@@ -165,6 +165,108 @@ __global__ void __launch_bounds__(256) k_wgrad(const float* __restrict__ X, int6
     }
 }
 
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+// Dense per-walker products on the FP32 matrix cores (X non-null, fixed row ranges, no row
+// mask): out[b][k][j] = sum over the walker's rows of X[row][k] Y[row][j].  Block = (128 x 128
+// output tile, walker, slice), 16 rows of X[:, k0 .. +127] and Y[:, j0 .. +127] per LDS stage;
+// wavefront w computes a 64 x 64 quarter as 2 x 2 tiles of v_mfma_f32_32x32x2f32, two rows per
+// step: lane l supplies X[row 2s + l/32][k0 + 32 a + l%32] as A and Y[row 2s + l/32][j0 + 32 b +
+// l%32] as B, so the contraction index is the row.  Slices and partial sums as k_wgrad.
+__global__ void __launch_bounds__(256) k_wgrad_mfma(const float* __restrict__ X, int64_t ldx,
+                                                    const float* __restrict__ Y, int64_t ldy, int K, int J, WRows wr,
+                                                    float* __restrict__ out, int64_t so, float* __restrict__ part) {
+  constexpr int PITCH = 128 + 4;
+  __shared__ float xs[16][PITCH];
+  __shared__ float ys[16][PITCH];
+  const int b = blockIdx.y, S = gridDim.z, sl = blockIdx.z;
+  const int tk = (K + 127) / 128;
+  const int k0 = (blockIdx.x % tk) * 128, j0 = (blockIdx.x / tk) * 128;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int wm = (wid >> 1) * 64, wn = (wid & 1) * 64;
+  int64_t s[2], e[2];
+  s[0] = wr.a0 + (int64_t)b * wr.as;
+  e[0] = s[0] + wr.na;
+  s[1] = wr.c0 + (int64_t)b * wr.cs;
+  e[1] = s[1] + wr.nc;
+  if (S > 1) {
+    const int64_t n0 = e[0] - s[0], n = n0 + (e[1] - s[1]);
+    const int64_t f0 = n * sl / S, f1 = n * (sl + 1) / S;
+    const int64_t a = s[0], c = s[1];
+    s[0] = a + (f0 < n0 ? f0 : n0);
+    e[0] = a + (f1 < n0 ? f1 : n0);
+    s[1] = c + (f0 > n0 ? f0 - n0 : 0);
+    e[1] = c + (f1 > n0 ? f1 - n0 : 0);
+  }
+  f32x16 acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) acc[a][c][v] = 0.f;
+  // chunks of 16 rows over both ranges; chunk q + 1 is loaded into registers while chunk q is
+  // multiplied from LDS
+  const int64_t nc0 = (e[0] - s[0] + 15) / 16, nch = nc0 + (e[1] - s[1] + 15) / 16;
+  float rx[8], ry[8];
+  auto load = [&](int64_t q) {
+    const int pt = q < nc0 ? 0 : 1;
+    const int64_t r0 = s[pt] + 16 * (q < nc0 ? q : q - nc0);
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int t = threadIdx.x + 256 * u;
+      const int rr = t >> 7, cc = t & 127;
+      const int64_t row = r0 + rr;
+      const bool ok = row < e[pt];
+      rx[u] = ok && k0 + cc < K ? X[row * ldx + k0 + cc] : 0.f;
+      ry[u] = ok && j0 + cc < J ? Y[row * ldy + j0 + cc] : 0.f;
+    }
+  };
+  if (nch > 0) load(0);
+  for (int64_t q = 0; q < nch; ++q) {
+    {
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int t = threadIdx.x + 256 * u;
+        xs[t >> 7][t & 127] = rx[u];
+        ys[t >> 7][t & 127] = ry[u];
+      }
+      __syncthreads();
+      if (q + 1 < nch) load(q + 1);
+#pragma unroll
+      for (int kk = 0; kk < 16; kk += 2) {
+        const int rr = kk + (lane >> 5);
+        float av[2], bv[2];
+#pragma unroll
+        for (int a = 0; a < 2; ++a) av[a] = xs[rr][wm + a * 32 + (lane & 31)];
+#pragma unroll
+        for (int c = 0; c < 2; ++c) bv[c] = ys[rr][wn + c * 32 + (lane & 31)];
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int c = 0; c < 2; ++c) acc[a][c] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[a], bv[c], acc[a][c], 0, 0, 0);
+      }
+      __syncthreads();
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const int j = j0 + wn + c * 32 + (lane & 31);
+      if (j >= J) continue;
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const int k = k0 + wm + a * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+        if (k >= K) continue;
+        if (S > 1)
+          part[((int64_t)sl * gridDim.y + b) * K * J + (int64_t)k * J + j] = acc[a][c][v];
+        else
+          out[(int64_t)b * so + (int64_t)k * J + j] = acc[a][c][v];
+      }
+    }
+}
+
 // out[b][kj] (+)= sum over slices (in order) of part[(slice B + b) K J + kj]
 __global__ void k_wgrad_reduce(const float* __restrict__ part, int S, int B, int64_t KJ, float* __restrict__ out,
                                int64_t so, int accumulate) {
@@ -182,6 +284,24 @@ void launch_wgrad(hipStream_t s, const float* X, int64_t ldx, const float* Y, in
                   const WRows& wr, float* out, int64_t so, bool accumulate, const uint8_t* rowmask, int want,
                   float* part, int64_t part_cap) {
   if (B == 0 || K == 0 || J == 0) return;
+  if (X && !wr.rng && !rowmask && !accumulate && K >= 32 && J >= 32) {
+    // dense product on the matrix cores; rows split in two or more slices when few blocks
+    const int tiles = ((K + 127) / 128) * ((J + 127) / 128);
+    const int64_t rows = (int64_t)wr.na + wr.nc;
+    int S = 1;
+    if (part) {
+      const int64_t want_blocks = (4096 + (int64_t)tiles * B - 1) / ((int64_t)tiles * B);
+      S = (int)std::max<int64_t>(1, std::min<int64_t>({8, rows / 32, want_blocks}));
+      while (S > 1 && (int64_t)S * B * K * J > part_cap) --S;
+    }
+    k_wgrad_mfma<<<dim3((unsigned)tiles, (unsigned)B, (unsigned)S), 256, 0, s>>>(X, ldx, Y, ldy, K, J, wr, out, so,
+                                                                                 part);
+    if (S > 1) {
+      const int64_t n = (int64_t)B * K * J;
+      k_wgrad_reduce<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(part, S, B, (int64_t)K * J, out, so, 0);
+    }
+    return;
+  }
   // tile shape: 16 x 256 for narrow products (K <= 16 against J > 64), else 64 x 64
   const int TK = K <= 16 && J > 64 ? 16 : 64, TJ = 4096 / TK;
   const int tiles = ((K + TK - 1) / TK) * ((J + TJ - 1) / TJ);
