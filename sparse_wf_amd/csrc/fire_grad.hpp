@@ -68,6 +68,9 @@ struct GradRows {
 void launch_wgrad(hipStream_t s, const float* X, int64_t ldx, const float* Y, int64_t ldy, int K, int J, int B,
                   const WRows& wr, float* out, int64_t so, bool accumulate = false, const uint8_t* rowmask = nullptr,
                   int want = 1, float* part = nullptr, int64_t part_cap = 0);
+void launch_gr_en_cot(hipStream_t s, const float* beta, const float* lsf, const uint8_t* mask, const float* dhen,
+                      const EnParams& ep, int B, int N, int nu, int M, int F, float* dbeta, float* part, float* gout,
+                      int64_t gstride, int64_t o_wg, int64_t o_wf, int64_t o_hn);
 void launch_silu_bwd(hipStream_t s, float* dz, int64_t ldo, const float* dy, int64_t ldy, const float* z, int64_t ldz,
                      int64_t rows, int cols);
 void launch_silu_fwd(hipStream_t s, float* y, const float* z, int64_t n);

@@ -1264,6 +1264,7 @@ __global__ void k_gr_en_nuc(const float* __restrict__ rec, const float* __restri
     else
       g[osp + (int64_t)m * ep.E + (c - nK - ep.w0)] = acc;
   }
+  if (!de) return;  // h_nuc's gradient written elsewhere (k_gr_en_cot)
   for (int f = t; f < F; f += blockDim.x) {
     float acc = 0.f;
     for (int i = 0; i < N; ++i) {
@@ -1272,6 +1273,175 @@ __global__ void k_gr_en_nuc(const float* __restrict__ rec, const float* __restri
     }
     g[ohn + (int64_t)m * F + f] = acc;
   }
+}
+
+// Sum of v[k] over the 64 lanes for every k < W1, spread over the lanes: halving steps over
+// lane bits 5, 4, ... each hand half of the remaining values to the partner lane (W1 - 1
+// shuffles), plain butterflies finish the lower bits.  Lane l returns the total for
+// k = xr_index<W1>(l); the lanes with the low 6 - log2(W1) bits clear hold each k once.
+template <int W1>
+__device__ __forceinline__ int xr_index(int lane) {
+  int k = 0;
+#pragma unroll
+  for (int st = 0, n = W1; n > 1; ++st, n >>= 1)
+    if (lane & (32 >> st)) k += n >> 1;
+  return k;
+}
+
+template <int W1>
+__device__ __forceinline__ float xreduce(float (&v)[W1], int lane) {
+  int st = 0;
+#pragma unroll
+  for (int n = W1; n > 1; n >>= 1, ++st) {
+    const int o = 32 >> st, half = n >> 1;
+    const bool up = (lane & o) != 0;
+#pragma unroll
+    for (int j = 0; j < half; ++j) {
+      const float send = up ? v[j] : v[half + j];
+      const float keep = up ? v[half + j] : v[j];
+      v[j] = keep + __shfl_xor(send, o, 64);
+    }
+  }
+  float t = v[0];
+  for (int o = 32 >> st; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+  return t;
+}
+
+// Fused e-n edge backward (rows g M + m), one workgroup per (nucleus m, walker b): the four
+// wavefronts take the walker's electrons in turn, lane features f = lane + 64 t (F <= 256).
+// Gamma = beta WG and e = lsf WF + h_nuc[m] are rebuilt per edge, dGamma = dh_en e and
+// de = dh_en Gamma are used in registers only: the WG / WF products accumulate per lane
+// (partials per (walker, nucleus) to part, summed over nuclei by k_gr_en_cot_reduce), the
+// h_nuc gradient of (walker, m) is complete here, and d beta[q][k] = sum_f dGamma WG[k][f] is a
+// wavefront reduction per edge.  Masked edges (outside cutoff_1el) contribute nothing.
+template <int W1>
+__global__ void __launch_bounds__(256) k_gr_en_cot(const float* __restrict__ beta, const float* __restrict__ lsf,
+                                                   const uint8_t* __restrict__ mask, const float* __restrict__ dhen,
+                                                   EnParams ep, int B, int N, int nu, int M, int F,
+                                                   float* __restrict__ dbeta, float* __restrict__ part,
+                                                   float* __restrict__ gout, int64_t gstride, int64_t o_hn) {
+  __shared__ float red[4][256];
+  const int m = blockIdx.x, b = blockIdx.y;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int T = F >> 6, nd = N - nu;
+  float wg[W1][4], wf[4][4], hn[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int f = lane + 64 * t;
+    const bool ok = t < T;
+#pragma unroll
+    for (int k = 0; k < W1; ++k) wg[k][t] = ok ? ep.WG[k * F + f] : 0.f;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) wf[c][t] = ok ? ep.WF[c * F + f] : 0.f;
+    hn[t] = ok ? ep.hnuc[(int64_t)m * F + f] : 0.f;
+  }
+  float aw[W1][4], af[4][4], ah[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+#pragma unroll
+    for (int k = 0; k < W1; ++k) aw[k][t] = 0.f;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) af[c][t] = 0.f;
+    ah[t] = 0.f;
+  }
+  const int kmine = xr_index<W1>(lane);
+  const bool writer = (lane & ((64 / W1) - 1)) == 0;
+  for (int i = wid; i < N; i += 4) {
+    const int64_t g = group_index(b, i, B, nu, nd);
+    const int64_t q = g * M + m;
+    if (!mask[q]) {
+      if (writer) dbeta[q * W1 + kmine] = 0.f;
+      continue;
+    }
+    float bk[W1], lc[4], db[W1];
+#pragma unroll
+    for (int k = 0; k < W1; ++k) {
+      bk[k] = beta[q * W1 + k];
+      db[k] = 0.f;
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) lc[c] = lsf[q * 4 + c];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      if (t >= T) break;
+      const float dh = dhen[g * F + lane + 64 * t];
+      float gam = 0.f, ev = hn[t];
+#pragma unroll
+      for (int k = 0; k < W1; ++k) gam += bk[k] * wg[k][t];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) ev += lc[c] * wf[c][t];
+      const float dgam = dh * ev, de = dh * gam;
+#pragma unroll
+      for (int k = 0; k < W1; ++k) {
+        aw[k][t] += bk[k] * dgam;
+        db[k] += dgam * wg[k][t];
+      }
+#pragma unroll
+      for (int c = 0; c < 4; ++c) af[c][t] += lc[c] * de;
+      ah[t] += de;
+    }
+    const float tot = xreduce<W1>(db, lane);
+    if (writer) dbeta[q * W1 + kmine] = tot;
+  }
+  // the four wavefronts' accumulators, one field (a row of F) at a time through LDS
+  const int f_out = (threadIdx.x >> 2) + 64 * (threadIdx.x & 3);  // thread -> (lane, t) slot
+  const bool f_ok = (threadIdx.x & 3) < T;
+  float* pb = part + ((int64_t)b * M + m) * (W1 + 4) * F;
+  for (int r = 0; r < W1 + 5; ++r) {
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      float v;
+      if (r < W1) {
+        v = aw[0][t];
+#pragma unroll
+        for (int k = 1; k < W1; ++k) v = r == k ? aw[k][t] : v;
+      } else if (r < W1 + 4) {
+        v = af[0][t];
+#pragma unroll
+        for (int c = 1; c < 4; ++c) v = r == W1 + c ? af[c][t] : v;
+      } else {
+        v = ah[t];
+      }
+      red[wid][lane * 4 + t] = v;
+    }
+    __syncthreads();
+    const float sum = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
+    if (f_ok) {
+      if (r < W1 + 4)
+        pb[(int64_t)r * F + f_out] = sum;
+      else
+        gout[(int64_t)b * gstride + o_hn + (int64_t)m * F + f_out] = sum;
+    }
+    __syncthreads();
+  }
+}
+
+// gout[b][WG (r < W1) | WF (r - W1)] = sum over nuclei (in order) of the partials of k_gr_en_cot
+__global__ void k_gr_en_cot_reduce(const float* __restrict__ part, int B, int M, int W1, int F, float* __restrict__ gout,
+                                   int64_t gstride, int64_t o_wg, int64_t o_wf) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t per = (int64_t)(W1 + 4) * F;
+  if (e >= (int64_t)B * per) return;
+  const int b = (int)(e / per);
+  const int64_t rf = e - (int64_t)b * per;
+  const int r = (int)(rf / F), f = (int)(rf - (int64_t)r * F);
+  float acc = 0.f;
+  for (int m = 0; m < M; ++m) acc += part[((int64_t)b * M + m) * per + rf];
+  gout[(int64_t)b * gstride + (r < W1 ? o_wg + (int64_t)r * F : o_wf + (int64_t)(r - W1) * F) + f] = acc;
+}
+
+void launch_gr_en_cot(hipStream_t s, const float* beta, const float* lsf, const uint8_t* mask, const float* dhen,
+                      const EnParams& ep, int B, int N, int nu, int M, int F, float* dbeta, float* part, float* gout,
+                      int64_t gstride, int64_t o_wg, int64_t o_wf, int64_t o_hn) {
+  if (B == 0 || M == 0) return;
+  const dim3 grid((unsigned)M, (unsigned)B);
+  switch (ep.w1) {
+    case 4: k_gr_en_cot<4><<<grid, 256, 0, s>>>(beta, lsf, mask, dhen, ep, B, N, nu, M, F, dbeta, part, gout, gstride, o_hn); break;
+    case 8: k_gr_en_cot<8><<<grid, 256, 0, s>>>(beta, lsf, mask, dhen, ep, B, N, nu, M, F, dbeta, part, gout, gstride, o_hn); break;
+    default: k_gr_en_cot<16><<<grid, 256, 0, s>>>(beta, lsf, mask, dhen, ep, B, N, nu, M, F, dbeta, part, gout, gstride, o_hn); break;
+  }
+  const int64_t n = (int64_t)B * (ep.w1 + 4) * F;
+  k_gr_en_cot_reduce<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(part, B, M, ep.w1, F, gout, gstride, o_wg, o_wf);
 }
 
 void launch_gr_pair_fwd(hipStream_t s, const float* r, int B, int N, int nu, const int64_t* ptr, const int32_t* nbr,
