@@ -1876,19 +1876,13 @@ void run_grad(Ctx& c, const float* r, int Bc, float* gout, int64_t gstride, cons
   float* dhen = fb(G * F);
   launch_gr_ln_bwd(s, dy1, lnn, lninv, ep.ln_g, G, F, dhen);
   // ---- e-n edges -----------------------------------------------------------------------------
-  // Gamma / e cotangents in registers: d beta per edge, WG / WF / h_nuc gradients per walker
-  const WRows en{nullptr, 0, (int64_t)nu * M, nu * M, (int64_t)Bc * nu * M, (int64_t)nd * M, nd * M};
-  float* edbeta = fb(rowsEN * w1);
+  // the e-n edge backward in one pass: Gamma / e cotangents and d beta in registers, then the
+  // filter backward per edge; WG, WF, Wd, bd, We per walker, K, b, sp, h_nuc per nucleus
   float* enpart = fb((int64_t)Bc * M * (w1 + 4) * F);
-  launch_gr_en_cot(s, en_beta, en_lsf, en_mask, dhen, ep, Bc, N, nu, M, F, edbeta, enpart, gout, gstride, off[b + 7],
-                   off[b + 8], off[b + 3]);
-  float* epg = fb(rowsEN * PG_FLOATS);
-  launch_gr_en_bwd(s, en_rec, edbeta, ep, h->cfg.cutoff_1el, rowsEN, M, epg);
-  wg(epg + PG_Y, PG_FLOATS, epg + PG_DDIR, PG_FLOATS, w0, w1, en, off[b + 4]);
-  wg(nullptr, 0, epg + PG_DDIR, PG_FLOATS, 1, w1, en, off[b + 5]);
-  wg(epg + PG_ENVR, PG_FLOATS, epg + PG_DENVD, PG_FLOATS, ep.E, w1, en, off[b + 6]);
-  launch_gr_en_nuc(s, en_rec, epg, nullptr, Bc, N, nu, M, F, ep, gout, gstride, off[b + 0], off[b + 1], off[b + 2],
-                   off[b + 3]);
+  float* enpart2 = fb((int64_t)Bc * M * w1 * (w0 + ep.E + 1));
+  launch_gr_en_cot(s, en_beta, en_lsf, en_mask, dhen, en_rec, h->cfg.cutoff_1el, ep, Bc, N, nu, M, F, enpart, enpart2,
+                   gout, gstride, off[b + 7], off[b + 8], off[b + 3], off[b + 0], off[b + 1], off[b + 2], off[b + 4],
+                   off[b + 5], off[b + 6]);
   HIPCHK(hipGetLastError());
 }
 }  // namespace

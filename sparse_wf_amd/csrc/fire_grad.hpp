@@ -69,8 +69,9 @@ void launch_wgrad(hipStream_t s, const float* X, int64_t ldx, const float* Y, in
                   const WRows& wr, float* out, int64_t so, bool accumulate = false, const uint8_t* rowmask = nullptr,
                   int want = 1, float* part = nullptr, int64_t part_cap = 0);
 void launch_gr_en_cot(hipStream_t s, const float* beta, const float* lsf, const uint8_t* mask, const float* dhen,
-                      const EnParams& ep, int B, int N, int nu, int M, int F, float* dbeta, float* part, float* gout,
-                      int64_t gstride, int64_t o_wg, int64_t o_wf, int64_t o_hn);
+                      const float* rec, float cutoff_1el, const EnParams& ep, int B, int N, int nu, int M, int F,
+                      float* part, float* part2, float* gout, int64_t gstride, int64_t o_wg, int64_t o_wf,
+                      int64_t o_hn, int64_t o_K, int64_t o_b, int64_t o_sp, int64_t o_wd, int64_t o_bd, int64_t o_we);
 void launch_silu_bwd(hipStream_t s, float* dz, int64_t ldo, const float* dy, int64_t ldy, const float* z, int64_t ldz,
                      int64_t rows, int cols);
 void launch_silu_fwd(hipStream_t s, float* y, const float* z, int64_t n);

@@ -1313,14 +1313,18 @@ __device__ __forceinline__ float xreduce(float (&v)[W1], int lane) {
 // de = dh_en Gamma are used in registers only: the WG / WF products accumulate per lane
 // (partials per (walker, nucleus) to part, summed over nuclei by k_gr_en_cot_reduce), the
 // h_nuc gradient of (walker, m) is complete here, and d beta[q][k] = sum_f dGamma WG[k][f] is a
-// wavefront reduction per edge.  Masked edges (outside cutoff_1el) contribute nothing.
+// wavefront reduction per edge.  The filter backward of the edge follows in the same wavefront
+// (d beta is uniform): lane a < w0 takes pre-activation a (Wd, K[m], b[m] gradients), lane
+// e < E envelope e (We, sp[m]); Wd / We / bd partials per (walker, nucleus) are summed over
+// nuclei by k_gr_en_filt_reduce.  Masked edges (outside cutoff_1el) contribute nothing.
 template <int W1>
 __global__ void __launch_bounds__(256) k_gr_en_cot(const float* __restrict__ beta, const float* __restrict__ lsf,
                                                    const uint8_t* __restrict__ mask, const float* __restrict__ dhen,
-                                                   EnParams ep, int B, int N, int nu, int M, int F,
-                                                   float* __restrict__ dbeta, float* __restrict__ part,
-                                                   float* __restrict__ gout, int64_t gstride, int64_t o_hn) {
-  __shared__ float red[4][256];
+                                                   const float* __restrict__ rec, float cutoff_1el, EnParams ep, int B,
+                                                   int N, int nu, int M, int F, float* __restrict__ part,
+                                                   float* __restrict__ part2, float* __restrict__ gout, int64_t gstride,
+                                                   int64_t o_hn, int64_t o_K, int64_t o_b, int64_t o_sp) {
+  __shared__ float red[4][64 * FIRE_W1MAX];
   const int m = blockIdx.x, b = blockIdx.y;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int T = F >> 6, nd = N - nu;
@@ -1335,6 +1339,17 @@ __global__ void __launch_bounds__(256) k_gr_en_cot(const float* __restrict__ bet
     for (int c = 0; c < 4; ++c) wf[c][t] = ok ? ep.WF[c * F + f] : 0.f;
     hn[t] = ok ? ep.hnuc[(int64_t)m * F + f] : 0.f;
   }
+  // filter backward: lane a < w0 takes pre-activation a, lane e < E envelope e of the edge
+  const bool la = lane < ep.w0, le = lane < ep.E;
+  float wd[W1], we[W1], aWd[W1], aWe[W1], abd[W1];
+#pragma unroll
+  for (int k = 0; k < W1; ++k) {
+    wd[k] = la ? ep.Wd[lane * W1 + k] : 0.f;
+    we[k] = le ? ep.We[lane * W1 + k] : 0.f;
+    aWd[k] = aWe[k] = abd[k] = 0.f;
+  }
+  const float spl = le ? ep.sp[(int64_t)m * ep.E + lane] : 1.f;
+  float aK[4] = {0.f, 0.f, 0.f, 0.f}, ab = 0.f, asp = 0.f;
   float aw[W1][4], af[4][4], ah[4];
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
@@ -1344,15 +1359,10 @@ __global__ void __launch_bounds__(256) k_gr_en_cot(const float* __restrict__ bet
     for (int c = 0; c < 4; ++c) af[c][t] = 0.f;
     ah[t] = 0.f;
   }
-  const int kmine = xr_index<W1>(lane);
-  const bool writer = (lane & ((64 / W1) - 1)) == 0;
   for (int i = wid; i < N; i += 4) {
     const int64_t g = group_index(b, i, B, nu, nd);
     const int64_t q = g * M + m;
-    if (!mask[q]) {
-      if (writer) dbeta[q * W1 + kmine] = 0.f;
-      continue;
-    }
+    if (!mask[q]) continue;
     float bk[W1], lc[4], db[W1];
 #pragma unroll
     for (int k = 0; k < W1; ++k) {
@@ -1381,7 +1391,40 @@ __global__ void __launch_bounds__(256) k_gr_en_cot(const float* __restrict__ bet
       ah[t] += de;
     }
     const float tot = xreduce<W1>(db, lane);
-    if (writer) dbeta[q * W1 + kmine] = tot;
+    // d beta of every k in every lane, then the filter backward of the edge
+    const float* rc = rec + q * PR_FLOATS;
+    const float d = rc[PR_D];
+    float ddir[W1], denvd[W1];
+#pragma unroll
+    for (int k = 0; k < W1; ++k) {
+      const float dbk = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(tot), k * (64 / W1)));
+      ddir[k] = dbk * rc[PR_ENVD + k];
+      denvd[k] = dbk * rc[PR_DIR + k];
+      abd[k] += ddir[k];
+    }
+    if (la) {
+      const float pre = rc[PR_PRE + lane];
+      float dy = 0.f;
+#pragma unroll
+      for (int k = 0; k < W1; ++k) dy += wd[k] * ddir[k];
+      const float y = silu_f(pre), dpre = dy * silu_d(pre);
+#pragma unroll
+      for (int k = 0; k < W1; ++k) aWd[k] += y * ddir[k];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) aK[c] += rc[PR_FEAT + c] * dpre;
+      ab += dpre;
+    }
+    if (le) {
+      float fc, dfc;
+      cutoff_poly(d / cutoff_1el, fc, dfc);
+      float den = 0.f;
+#pragma unroll
+      for (int k = 0; k < W1; ++k) den += we[k] * denvd[k];
+      const float x = d / spl, g0 = expf(-x * x);
+#pragma unroll
+      for (int k = 0; k < W1; ++k) aWe[k] += g0 * fc * denvd[k];
+      asp += den * g0 * fc * 2.f * x * x / spl;
+    }
   }
   // the four wavefronts' accumulators, one field (a row of F) at a time through LDS
   const int f_out = (threadIdx.x >> 2) + 64 * (threadIdx.x & 3);  // thread -> (lane, t) slot
@@ -1414,6 +1457,52 @@ __global__ void __launch_bounds__(256) k_gr_en_cot(const float* __restrict__ bet
     }
     __syncthreads();
   }
+  // filter parameters: Wd [w0][W1], We [E][W1], bd [W1] partials per (walker, nucleus); K[m],
+  // b[m], sp[m] complete here
+  const int w0 = ep.w0, E = ep.E;
+  const int P2 = W1 * (w0 + E + 1);
+  float* p2 = part2 + ((int64_t)b * M + m) * P2;
+  float* gb = gout + (int64_t)b * gstride;
+  for (int r = 0; r < 3; ++r) {  // 0: Wd, 1: We, 2: bd with K / b / sp
+#pragma unroll
+    for (int k = 0; k < W1; ++k) red[wid][lane * W1 + k] = r == 0 ? aWd[k] : (r == 1 ? aWe[k] : abd[k]);
+    __syncthreads();
+    for (int e = threadIdx.x; e < 64 * W1; e += blockDim.x) {
+      const int ln = e / W1, k = e - ln * W1;
+      const float v = red[0][e] + red[1][e] + red[2][e] + red[3][e];
+      if (r == 0 && ln < w0) p2[ln * W1 + k] = v;
+      if (r == 1 && ln < E) p2[w0 * W1 + ln * W1 + k] = v;
+      if (r == 2 && ln == 0) p2[(w0 + E) * W1 + k] = v;
+    }
+    __syncthreads();
+  }
+  // per-nucleus filter gradients: 4 + 1 values per pre-activation lane, 1 per envelope lane
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    red[wid][lane] = c < 4 ? aK[c] : (c == 4 ? ab : asp);
+    __syncthreads();
+    if (threadIdx.x < 64) {
+      const int ln = threadIdx.x;
+      const float v = red[0][ln] + red[1][ln] + red[2][ln] + red[3][ln];
+      if (c < 4 && ln < w0) gb[o_K + (int64_t)m * 4 * w0 + c * w0 + ln] = v;
+      if (c == 4 && ln < w0) gb[o_b + (int64_t)m * w0 + ln] = v;
+      if (c == 5 && ln < E) gb[o_sp + (int64_t)m * E + ln] = v;
+    }
+    __syncthreads();
+  }
+}
+
+// gout[b][Wd | We | bd] = sum over nuclei (in order) of the filter partials of k_gr_en_cot
+__global__ void k_gr_en_filt_reduce(const float* __restrict__ part2, int B, int M, int P2, int n_wd, int n_we,
+                                    float* __restrict__ gout, int64_t gstride, int64_t o_wd, int64_t o_we,
+                                    int64_t o_bd) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)B * P2) return;
+  const int b = (int)(e / P2), j = (int)(e - (int64_t)b * P2);
+  float acc = 0.f;
+  for (int m = 0; m < M; ++m) acc += part2[((int64_t)b * M + m) * P2 + j];
+  const int64_t o = j < n_wd ? o_wd + j : (j < n_wd + n_we ? o_we + (j - n_wd) : o_bd + (j - n_wd - n_we));
+  gout[(int64_t)b * gstride + o] = acc;
 }
 
 // gout[b][WG (r < W1) | WF (r - W1)] = sum over nuclei (in order) of the partials of k_gr_en_cot
@@ -1431,17 +1520,22 @@ __global__ void k_gr_en_cot_reduce(const float* __restrict__ part, int B, int M,
 }
 
 void launch_gr_en_cot(hipStream_t s, const float* beta, const float* lsf, const uint8_t* mask, const float* dhen,
-                      const EnParams& ep, int B, int N, int nu, int M, int F, float* dbeta, float* part, float* gout,
-                      int64_t gstride, int64_t o_wg, int64_t o_wf, int64_t o_hn) {
+                      const float* rec, float cutoff_1el, const EnParams& ep, int B, int N, int nu, int M, int F,
+                      float* part, float* part2, float* gout, int64_t gstride, int64_t o_wg, int64_t o_wf,
+                      int64_t o_hn, int64_t o_K, int64_t o_b, int64_t o_sp, int64_t o_wd, int64_t o_bd, int64_t o_we) {
   if (B == 0 || M == 0) return;
   const dim3 grid((unsigned)M, (unsigned)B);
   switch (ep.w1) {
-    case 4: k_gr_en_cot<4><<<grid, 256, 0, s>>>(beta, lsf, mask, dhen, ep, B, N, nu, M, F, dbeta, part, gout, gstride, o_hn); break;
-    case 8: k_gr_en_cot<8><<<grid, 256, 0, s>>>(beta, lsf, mask, dhen, ep, B, N, nu, M, F, dbeta, part, gout, gstride, o_hn); break;
-    default: k_gr_en_cot<16><<<grid, 256, 0, s>>>(beta, lsf, mask, dhen, ep, B, N, nu, M, F, dbeta, part, gout, gstride, o_hn); break;
+    case 4: k_gr_en_cot<4><<<grid, 256, 0, s>>>(beta, lsf, mask, dhen, rec, cutoff_1el, ep, B, N, nu, M, F, part, part2, gout, gstride, o_hn, o_K, o_b, o_sp); break;
+    case 8: k_gr_en_cot<8><<<grid, 256, 0, s>>>(beta, lsf, mask, dhen, rec, cutoff_1el, ep, B, N, nu, M, F, part, part2, gout, gstride, o_hn, o_K, o_b, o_sp); break;
+    default: k_gr_en_cot<16><<<grid, 256, 0, s>>>(beta, lsf, mask, dhen, rec, cutoff_1el, ep, B, N, nu, M, F, part, part2, gout, gstride, o_hn, o_K, o_b, o_sp); break;
   }
   const int64_t n = (int64_t)B * (ep.w1 + 4) * F;
   k_gr_en_cot_reduce<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(part, B, M, ep.w1, F, gout, gstride, o_wg, o_wf);
+  const int P2 = ep.w1 * (ep.w0 + ep.E + 1);
+  const int64_t n2 = (int64_t)B * P2;
+  k_gr_en_filt_reduce<<<(unsigned)((n2 + 255) / 256), 256, 0, s>>>(part2, B, M, P2, ep.w0 * ep.w1, ep.E * ep.w1, gout,
+                                                                   gstride, o_wd, o_we, o_bd);
 }
 
 void launch_gr_pair_fwd(hipStream_t s, const float* r, int B, int N, int nu, const int64_t* ptr, const int32_t* nbr,
