@@ -213,8 +213,11 @@ int fire_det_options(fire_handle* h, int32_t fp64_laplacian);
  * reference leaves kernel selection to XLA).  On by default: the first BF16 contraction of
  * each (split, N, K, floor(log2 M)) shape times every tile shape on its real operands
  * (synchronising) and keeps the fastest for the life of the process, shared by every handle
- * on the same device.  enable: 1 on, 0 off (tile 128 x 128), 2 on and forget earlier choices
- * (this device's), -1 unchanged.  table (optional, int32 [cap][7]) receives this handle's
+ * on the same device; a shape with a tuned neighbour of the same split, K, floor(log2 M) and
+ * number of 128-column tiles takes the nearest neighbour's choice without timing.  enable: 1 on,
+ * 0 off (tile 128 x 128), 2 on and forget earlier choices (this device's), 3 on and also reuse
+ * a neighbour up to two row buckets away (for ragged and serving batches, whose geometries each
+ * bring new shapes), -1 unchanged.  table (optional, int32 [cap][7]) receives this handle's
  * choices as (split, N, K, log2 M, variant, BM, BN); log2 M is raised by 32 for row counts held
  * on the device and by 64 for contractions with the fused activation epilogue;
  * *n_entries the number of choices. */

@@ -228,6 +228,7 @@ struct fire_handle {
   // Autotuned tile shape of the BF16 contraction per (split, N, K, floor(log2 M)): the first
   // call of a shape times every candidate on the real operands and keeps the fastest.
   bool autotune = true;
+  bool autotune_near = false;  // reuse tuned tiles across neighbouring row buckets
   // silu of the tangent-row MLP layers fused into the contraction epilogue (FIRE_FUSED_ACT=0:
   // the separate element-wise pass instead)
   bool fuse_act = true;
@@ -467,6 +468,30 @@ void gemm(Ctx& c, const float* A, int64_t lda, int wi, int bi, float* C, int64_t
         variant = g->second;
         h->gemm_tile[key] = variant;
         known = true;
+      } else {
+        // nearest tuned shape of the same class (device, split, K, epilogue / row-count flags,
+        // number of 128-column tiles, and row bucket: the same one, or with near_rows within
+        // two): ragged and serving batches bring new N and row counts with every geometry
+        int64_t dbest = -1;
+        for (const auto& kv : tile_cache().best) {
+          const auto& k = kv.first;
+          const int lg2 = std::get<4>(k);
+          if (std::get<0>(k) != h->device || std::get<1>(k) != ns || std::get<3>(k) != K || (lg2 & ~31) != (lg & ~31))
+            continue;
+          const int dl = std::abs((lg2 & 31) - (lg & 31));
+          if (dl > (h->autotune_near ? 2 : 0)) continue;
+          const int n2 = std::get<2>(k);
+          if ((n2 + 127) / 128 != (N + 127) / 128) continue;
+          const int64_t dist = (int64_t)dl * 100000 + std::abs(n2 - N);
+          if (dbest < 0 || dist < dbest) {
+            dbest = dist;
+            variant = kv.second;
+          }
+        }
+        if (dbest >= 0) {
+          h->gemm_tile[key] = variant;
+          known = true;
+        }
       }
     }
     if (!known && !c.fixed) {
@@ -2404,6 +2429,7 @@ int fire_autotune(fire_handle* h, int32_t enable, int32_t* table, int32_t cap, i
     }
     if (n_entries) *n_entries = n;
     if (enable >= 0) h->autotune = enable != 0;
+    if (enable >= 0) h->autotune_near = enable == 3;
     if (enable == 2) {
       h->gemm_tile.clear();
       h->gemm_tile_ms.clear();

@@ -111,12 +111,4 @@ void launch_gr_ln_bwd(hipStream_t s, const float* dy, const float* nrm, const fl
                       int F, float* dh);
 void launch_gr_glu_fwd(hipStream_t s, const float* Z2, int64_t G, int F, float* y);
 void launch_gr_glu_bwd(hipStream_t s, const float* dy, const float* Z2, int64_t G, int F, float* dZ2);
-void launch_gr_en_bwd_f(hipStream_t s, const float* beta, const float* lsf, const EnParams& ep, const uint8_t* mask,
-                        const float* dhen, int64_t rows, int M, int F, float* dgam, float* de);
-void launch_gr_en_bwd(hipStream_t s, const float* rec, const float* dbeta, const EnParams& ep, float cutoff_1el,
-                      int64_t rows, int M, float* pg);
-void launch_gr_en_nuc(hipStream_t s, const float* rec, const float* pg, const float* de, int B, int N, int nu, int M,
-                      int F, const EnParams& ep, float* gout, int64_t gstride, int64_t oK, int64_t ob, int64_t osp,
-                      int64_t ohn);
-
 }  // namespace fire

@@ -1172,109 +1172,6 @@ __global__ void k_gr_glu_bwd(const float* __restrict__ dy, const float* __restri
   dZ2[g * 2 * F + F + f] = d * x * silu_d(gt);
 }
 
-// e-n edges backward per (edge, 4 features): dGamma = mask dh_en (e + h_nuc),
-// de = mask dh_en Gamma; F % 4 == 0.
-__global__ void k_gr_en_bwd_f(const float* __restrict__ beta, const float* __restrict__ lsf, EnParams ep,
-                              const uint8_t* __restrict__ mask, const float* __restrict__ dhen, int64_t rows, int M,
-                              int F, float* __restrict__ dgam, float* __restrict__ de) {
-  const int F4 = F >> 2;
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= rows * F4) return;
-  const int64_t q = e / F4;
-  const int f = (int)(e - q * F4) * 4;
-  const int64_t g = q / M;
-  const int m = (int)(q - g * M);
-  if (!mask[q]) {
-    st4(dgam + q * F + f, make_float4(0.f, 0.f, 0.f, 0.f));
-    st4(de + q * F + f, make_float4(0.f, 0.f, 0.f, 0.f));
-    return;
-  }
-  float4 gam = make_float4(0.f, 0.f, 0.f, 0.f), ev = ld4(ep.hnuc + (int64_t)m * F + f);
-  for (int k = 0; k < ep.w1; ++k) gam = fma4(beta[q * ep.w1 + k], ld4(ep.WG + k * F + f), gam);
-  for (int c = 0; c < 4; ++c) ev = fma4(lsf[q * 4 + c], ld4(ep.WF + c * F + f), ev);
-  const float4 dh = ld4(dhen + g * F + f);
-  st4(dgam + q * F + f, mul4(dh, ev));
-  st4(de + q * F + f, mul4(dh, gam));
-}
-
-// e-n filter backward per edge (thread per edge) from d beta [rows, w1]
-__global__ void k_gr_en_bwd(const float* __restrict__ rec, const float* __restrict__ dbeta, EnParams ep,
-                            float cutoff_1el, int64_t rows, int M, float* __restrict__ pg) {
-  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= rows) return;
-  const int m = (int)(q % M);
-  const float* rc = rec + q * PR_FLOATS;
-  const float d = rc[PR_D];
-  const float* sp = ep.sp + (int64_t)m * ep.E;
-  float* o = pg + q * PG_FLOATS;
-  float ddir[FIRE_W1MAX], denvd[FIRE_W1MAX];
-  for (int k = 0; k < ep.w1; ++k) {
-    const float db = dbeta[q * ep.w1 + k];
-    ddir[k] = db * rc[PR_ENVD + k];
-    denvd[k] = db * rc[PR_DIR + k];
-    o[PG_DDIR + k] = ddir[k];
-    o[PG_DENVD + k] = denvd[k];
-  }
-  float fc, dfc;
-  cutoff_poly(d / cutoff_1el, fc, dfc);
-  for (int a = 0; a < ep.w0; ++a) {
-    float dy = 0.f;
-    for (int k = 0; k < ep.w1; ++k) dy += ep.Wd[a * ep.w1 + k] * ddir[k];
-    const float pre = rc[PR_PRE + a];
-    o[PG_Y + a] = silu_f(pre);
-    o[PG_DPRE + a] = dy * silu_d(pre);
-  }
-  for (int e = 0; e < ep.E; ++e) {
-    float den = 0.f;
-    for (int k = 0; k < ep.w1; ++k) den += ep.We[e * ep.w1 + k] * denvd[k];
-    const float s = sp[e], x = d / s, g0 = expf(-x * x);
-    o[PG_ENVR + e] = g0 * fc;
-    o[PG_DSP + e] = den * g0 * fc * 2.f * x * x / s;
-  }
-}
-
-// Nucleus-dependent e-n parameters per (walker, nucleus): d K_m [4][w0] = sum feat (x) d pre,
-// d b_m = sum d pre, d sp_m = sum d sp, d h_nuc[m] = sum de (over the walker's electrons).
-__global__ void k_gr_en_nuc(const float* __restrict__ rec, const float* __restrict__ pg, const float* __restrict__ de,
-                            int B, int N, int nu, int M, int F, EnParams ep, float* __restrict__ gout, int64_t gstride,
-                            int64_t oK, int64_t ob, int64_t osp, int64_t ohn) {
-  const int w = blockIdx.x, m = blockIdx.y, t = threadIdx.x;
-  const int nd = N - nu;
-  float* g = gout + (int64_t)w * gstride;
-  const int nK = 4 * ep.w0;
-  for (int c = t; c < nK + ep.w0 + ep.E; c += blockDim.x) {
-    float acc = 0.f;
-    for (int i = 0; i < N; ++i) {
-      const int64_t q = group_index(w, i, B, nu, nd) * M + m;
-      const float* rc = rec + q * PR_FLOATS;
-      const float* o = pg + q * PG_FLOATS;
-      if (c < nK) {
-        const int cc = c / ep.w0, a = c - cc * ep.w0;
-        acc += rc[PR_FEAT + cc] * o[PG_DPRE + a];
-      } else if (c < nK + ep.w0) {
-        acc += o[PG_DPRE + (c - nK)];
-      } else {
-        acc += o[PG_DSP + (c - nK - ep.w0)];
-      }
-    }
-    if (c < nK)
-      g[oK + (int64_t)m * nK + c] = acc;
-    else if (c < nK + ep.w0)
-      g[ob + (int64_t)m * ep.w0 + (c - nK)] = acc;
-    else
-      g[osp + (int64_t)m * ep.E + (c - nK - ep.w0)] = acc;
-  }
-  if (!de) return;  // h_nuc's gradient written elsewhere (k_gr_en_cot)
-  for (int f = t; f < F; f += blockDim.x) {
-    float acc = 0.f;
-    for (int i = 0; i < N; ++i) {
-      const int64_t q = group_index(w, i, B, nu, nd) * M + m;
-      acc += de[q * F + f];
-    }
-    g[ohn + (int64_t)m * F + f] = acc;
-  }
-}
-
 // Sum of v[k] over the 64 lanes for every k < W1, spread over the lanes: halving steps over
 // lane bits 5, 4, ... each hand half of the remaining values to the partner lane (W1 - 1
 // shuffles), plain butterflies finish the lower bits.  Lane l returns the total for
@@ -1588,20 +1485,6 @@ void launch_gr_ln_bwd(hipStream_t s, const float* dy, const float* nrm, const fl
 void launch_gr_glu_bwd(hipStream_t s, const float* dy, const float* Z2, int64_t G, int F, float* dZ2) {
   const int64_t n = G * F;
   if (n) k_gr_glu_bwd<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(dy, Z2, G, F, dZ2);
-}
-void launch_gr_en_bwd_f(hipStream_t s, const float* beta, const float* lsf, const EnParams& ep, const uint8_t* mask,
-                        const float* dhen, int64_t rows, int M, int F, float* dgam, float* de) {
-  const int64_t n = rows * (F / 4);
-  if (n) k_gr_en_bwd_f<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(beta, lsf, ep, mask, dhen, rows, M, F, dgam, de);
-}
-void launch_gr_en_bwd(hipStream_t s, const float* rec, const float* dbeta, const EnParams& ep, float cutoff_1el,
-                      int64_t rows, int M, float* pg) {
-  if (rows) k_gr_en_bwd<<<(unsigned)((rows + 127) / 128), 128, 0, s>>>(rec, dbeta, ep, cutoff_1el, rows, M, pg);
-}
-void launch_gr_en_nuc(hipStream_t s, const float* rec, const float* pg, const float* de, int B, int N, int nu, int M,
-                      int F, const EnParams& ep, float* gout, int64_t gstride, int64_t oK, int64_t ob, int64_t osp,
-                      int64_t ohn) {
-  if (B) k_gr_en_nuc<<<dim3(B, M), 256, 0, s>>>(rec, pg, de, B, N, nu, M, F, ep, gout, gstride, oK, ob, osp, ohn);
 }
 
 }  // namespace fire

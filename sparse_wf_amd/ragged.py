@@ -66,6 +66,7 @@ def run_ragged(geoms: List[Geometry], sweeps: int, width: float = 0.3, precision
     wfs, streams, nacc, states = [], [], [], []
     for g in geoms:
         wf = FireWaveFunction(g.cfg, precision=precision)
+        wf.autotune("near")
         wf.upload(g.params)
         wf.mcmc_options(use_graphs)
         s = torch.cuda.Stream() if use_graphs else torch.cuda.current_stream()
@@ -120,6 +121,7 @@ def run_ragged(geoms: List[Geometry], sweeps: int, width: float = 0.3, precision
                     g = geoms[k]
                     if wf is None:
                         wf = FireWaveFunction(g.cfg, precision=precision)
+                        wf.autotune("near")
                     else:
                         wf.reconfigure(g.cfg)
                     with torch.cuda.stream(s):

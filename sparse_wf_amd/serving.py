@@ -105,6 +105,7 @@ class _Slot:
     def take(self, cfg: FireConfig, precision: str) -> FireWaveFunction:
         if self.wf is None:
             self.wf = FireWaveFunction(cfg, device=self.device, precision=precision)
+            self.wf.autotune("near")  # every request brings new shapes
             self.created += 1
         else:
             self.wf.reconfigure(cfg)

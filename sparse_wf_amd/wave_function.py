@@ -461,6 +461,17 @@ class FireWaveFunction:
             raise ValueError("no resident state (call log_psi_with_state first)")
         return nw.value
 
+    def autotune(self, mode: str = "on"):
+        """Contraction tile autotuner of this process and device (``fire_autotune``): "on"
+        (time each new shape class), "off", "reset" (on, earlier choices forgotten) or "near"
+        (on, and shapes up to two row buckets from a tuned one reuse its choice: ragged and
+        serving batches, whose geometries each bring new shapes)."""
+        codes = {"off": 0, "on": 1, "reset": 2, "near": 3}
+        if mode not in codes:
+            raise ValueError(f"autotune mode {mode!r} (one of {sorted(codes)})")
+        n = ctypes.c_int32()
+        _lib.check(self._lib.fire_autotune(self._h, codes[mode], None, 0, ctypes.byref(n)))
+
     def precision_plan(self, planes=None):
         """Operand planes per contraction class of the "mixed" precision (ElecInit MLP, update MLP,
         orbitals, MLP Jastrow, attention Q/V, attention readout; 1-3 BF16 planes, 4 two scaled
