@@ -1867,12 +1867,8 @@ void run_grad(Ctx& c, const float* r, int Bc, float* gout, int64_t gstride, cons
     for (int k = 0; k < 2; ++k) {
       const int base = k == 0 ? h->iEEs : h->iEEd;
       const int want = k == 0 ? 1 : 0;
-      wg(prec + PR_FEAT, PR_FLOATS, ppg + PG_DPRE, PG_FLOATS, 4, w0, pr, off[base + 0], psame, want);
-      wg(nullptr, 0, ppg + PG_DPRE, PG_FLOATS, 1, w0, pr, off[base + 1], psame, want);
-      wg(nullptr, 0, ppg + PG_DSP, PG_FLOATS, 1, w0, pr, off[base + 2], psame, want);
-      wg(ppg + PG_Y, PG_FLOATS, ppg + PG_DDIR, PG_FLOATS, w0, w1, pr, off[base + 3], psame, want);
-      wg(nullptr, 0, ppg + PG_DDIR, PG_FLOATS, 1, w1, pr, off[base + 4], psame, want);
-      wg(prec + PR_ENV, PR_FLOATS, ppg + PG_DENVD, PG_FLOATS, w0, w1, pr, off[base + 5], psame, want);
+      launch_pair_filt(s, prec, ppg, psame, want, prng, Bc, pr.avg_rows, w0, w1, off[base + 0], off[base + 1],
+                       off[base + 2], off[base + 3], off[base + 4], off[base + 5], gout, gstride, wpart, part_cap);
       wg(pbeta, w1, pdgam, F, w1, F, pr, off[base + 6], psame, want);
       wg(plsf, 4, pdu, F, 4, F, pr, off[base + 7], psame, want);
       wg(nullptr, 0, pdu, F, 1, F, pr, off[base + 8], psame, want);

@@ -72,6 +72,10 @@ void launch_gr_en_cot(hipStream_t s, const float* beta, const float* lsf, const 
                       const float* rec, float cutoff_1el, const EnParams& ep, int B, int N, int nu, int M, int F,
                       float* part, float* part2, float* gout, int64_t gstride, int64_t o_wg, int64_t o_wf,
                       int64_t o_hn, int64_t o_K, int64_t o_b, int64_t o_sp, int64_t o_wd, int64_t o_bd, int64_t o_we);
+void launch_pair_filt(hipStream_t s, const float* rec, const float* pg, const uint8_t* same, int want,
+                      const int64_t* rng, int B, int64_t avg_rows, int w0, int w1, int64_t oK, int64_t ob,
+                      int64_t osp, int64_t oWd, int64_t obd, int64_t oWe, float* gout, int64_t gstride, float* part,
+                      int64_t part_cap);
 void launch_silu_bwd(hipStream_t s, float* dz, int64_t ldo, const float* dy, int64_t ldy, const float* z, int64_t ldz,
                      int64_t rows, int cols);
 void launch_silu_fwd(hipStream_t s, float* y, const float* z, int64_t n);

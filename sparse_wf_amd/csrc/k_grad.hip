@@ -165,6 +165,20 @@ __global__ void __launch_bounds__(256) k_wgrad(const float* __restrict__ X, int6
     }
 }
 
+// Column sums per walker over fixed row ranges (the bias gradients of the dense layers):
+// out[b][j] = sum over the rows of walker b (first range, then second, in order) of Y[row][j].
+__global__ void k_wcolsum(const float* __restrict__ Y, int64_t ldy, int J, WRows wr, float* __restrict__ out,
+                          int64_t so) {
+  const int b = blockIdx.y;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= J) return;
+  float acc = 0.f;
+  const int64_t a0 = wr.a0 + (int64_t)b * wr.as, c0 = wr.c0 + (int64_t)b * wr.cs;
+  for (int r = 0; r < wr.na; ++r) acc += Y[(a0 + r) * ldy + j];
+  for (int r = 0; r < wr.nc; ++r) acc += Y[(c0 + r) * ldy + j];
+  out[(int64_t)b * so + j] = acc;
+}
+
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 // Dense per-walker products on the FP32 matrix cores (X non-null, fixed row ranges, no row
@@ -284,6 +298,10 @@ void launch_wgrad(hipStream_t s, const float* X, int64_t ldx, const float* Y, in
                   const WRows& wr, float* out, int64_t so, bool accumulate, const uint8_t* rowmask, int want,
                   float* part, int64_t part_cap) {
   if (B == 0 || K == 0 || J == 0) return;
+  if (!X && K == 1 && !wr.rng && !rowmask && !accumulate) {  // column sums over fixed ranges
+    k_wcolsum<<<dim3((unsigned)((J + 255) / 256), (unsigned)B), 256, 0, s>>>(Y, ldy, J, wr, out, so);
+    return;
+  }
   if (X && !wr.rng && !rowmask && !accumulate && K >= 32 && J >= 32) {
     // dense product on the matrix cores; rows split in two or more slices when few blocks
     const int tiles = ((K + 127) / 128) * ((J + 127) / 128);
@@ -1170,6 +1188,148 @@ __global__ void k_gr_glu_bwd(const float* __restrict__ dy, const float* __restri
   const float x = Z2[g * 2 * F + f], gt = Z2[g * 2 * F + F + f], d = dy[e];
   dZ2[g * 2 * F + f] = d * silu_f(gt);
   dZ2[g * 2 * F + F + f] = d * x * silu_d(gt);
+}
+
+// The six filter-parameter gradients of one e-e edge class in one pass over its pairs (rows of
+// walker b from its two pair ranges, rows with same[row] != want skipped): K [4][w0] = sum feat
+// (x) d pre, b = sum d pre, sp = sum d sp, Wd [w0][w1] = sum y (x) d dir, bd = sum d dir,
+// We [w0][w1] = sum env (x) d envd.  32 rows at a time are staged in LDS as [feat | env | d pre |
+// d sp | y | d dir | d envd]; thread t owns outputs t, t + 256, ...; with S slices per walker the
+// partials go to part[(slice B + b) n_out + o] and k_pair_filt_reduce sums them in slice order.
+struct PairFiltOut {
+  int w0, w1, n_out;
+  int64_t oK, ob, osp, oWd, obd, oWe;
+};
+
+__device__ __forceinline__ void pair_filt_slot(const PairFiltOut& d, int o, int& xi, int& yi, int64_t& dst) {
+  const int w0 = d.w0, w1 = d.w1;
+  const int sENV = 4, sPRE = 4 + w0, sSP = 4 + 2 * w0, sY = 4 + 3 * w0, sDIR = 4 + 4 * w0, sENVD = 4 + 4 * w0 + w1;
+  if (o < 4 * w0) {
+    xi = o / w0;
+    yi = sPRE + o % w0;
+    dst = d.oK + o;
+  } else if ((o -= 4 * w0) < w0) {
+    xi = -1;
+    yi = sPRE + o;
+    dst = d.ob + o;
+  } else if ((o -= w0) < w0) {
+    xi = -1;
+    yi = sSP + o;
+    dst = d.osp + o;
+  } else if ((o -= w0) < w0 * w1) {
+    xi = sY + o / w1;
+    yi = sDIR + o % w1;
+    dst = d.oWd + o;
+  } else if ((o -= w0 * w1) < w1) {
+    xi = -1;
+    yi = sDIR + o;
+    dst = d.obd + o;
+  } else {
+    o -= w1;
+    xi = sENV + o / w1;
+    yi = sENVD + o % w1;
+    dst = d.oWe + o;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_pair_filt(const float* __restrict__ rec, const float* __restrict__ pg,
+                                                   const uint8_t* __restrict__ same, int want,
+                                                   const int64_t* __restrict__ rng, PairFiltOut d,
+                                                   float* __restrict__ gout, int64_t gstride, float* __restrict__ part) {
+  constexpr int RB = 32, ROWMAX = 4 + 4 * FIRE_W0MAX + 2 * FIRE_W1MAX;
+  __shared__ float st[RB][ROWMAX + 1];
+  const int b = blockIdx.x, sl = blockIdx.y, S = gridDim.y, B = gridDim.x;
+  const int w0 = d.w0, w1 = d.w1, RW = 4 + 4 * w0 + 2 * w1;
+  int64_t s0 = rng[4 * b], e0 = rng[4 * b + 1], s1 = rng[4 * b + 2], e1 = rng[4 * b + 3];
+  {
+    const int64_t n0 = e0 - s0, n = n0 + (e1 - s1);
+    const int64_t f0 = n * sl / S, f1 = n * (sl + 1) / S;
+    const int64_t a = s0, c = s1;
+    s0 = a + (f0 < n0 ? f0 : n0);
+    e0 = a + (f1 < n0 ? f1 : n0);
+    s1 = c + (f0 > n0 ? f0 - n0 : 0);
+    e1 = c + (f1 > n0 ? f1 - n0 : 0);
+  }
+  constexpr int NU = (4 * FIRE_W0MAX + 3 * FIRE_W0MAX + 2 * FIRE_W0MAX * FIRE_W1MAX + FIRE_W1MAX + 255) / 256;
+  float acc[NU];
+  int xi[NU], yi[NU];
+  int64_t dst[NU];
+#pragma unroll
+  for (int u = 0; u < NU; ++u) {
+    acc[u] = 0.f;
+    const int o = threadIdx.x + 256 * u;
+    xi[u] = yi[u] = 0;
+    dst[u] = -1;
+    if (o < d.n_out) pair_filt_slot(d, o, xi[u], yi[u], dst[u]);
+  }
+  const int64_t n0 = e0 - s0, nr = n0 + (e1 - s1);
+  for (int64_t r0 = 0; r0 < nr; r0 += RB) {
+    for (int t = threadIdx.x; t < RB * RW; t += 256) {
+      const int rr = t / RW, f = t - rr * RW;
+      const int64_t ri = r0 + rr;
+      float v = 0.f;
+      if (ri < nr) {
+        const int64_t row = ri < n0 ? s0 + ri : s1 + (ri - n0);
+        if (same[row] == want) {
+          const float* rc = rec + row * PR_FLOATS;
+          const float* gp = pg + row * PG_FLOATS;
+          if (f < 4) v = rc[PR_FEAT + f];
+          else if (f < 4 + w0) v = rc[PR_ENV + f - 4];
+          else if (f < 4 + 2 * w0) v = gp[PG_DPRE + f - 4 - w0];
+          else if (f < 4 + 3 * w0) v = gp[PG_DSP + f - 4 - 2 * w0];
+          else if (f < 4 + 4 * w0) v = gp[PG_Y + f - 4 - 3 * w0];
+          else if (f < 4 + 4 * w0 + w1) v = gp[PG_DDIR + f - 4 - 4 * w0];
+          else v = gp[PG_DENVD + f - 4 - 4 * w0 - w1];
+        }
+      }
+      st[rr][f] = v;
+    }
+    __syncthreads();
+    const int nrow = nr - r0 < RB ? (int)(nr - r0) : RB;
+    for (int rr = 0; rr < nrow; ++rr) {
+#pragma unroll
+      for (int u = 0; u < NU; ++u)
+        if (dst[u] >= 0) acc[u] += (xi[u] < 0 ? 1.f : st[rr][xi[u]]) * st[rr][yi[u]];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int u = 0; u < NU; ++u) {
+    if (dst[u] < 0) continue;
+    const int o = threadIdx.x + 256 * u;
+    if (S > 1)
+      part[((int64_t)sl * B + b) * d.n_out + o] = acc[u];
+    else
+      gout[(int64_t)b * gstride + dst[u]] = acc[u];
+  }
+}
+
+__global__ void k_pair_filt_reduce(const float* __restrict__ part, int S, int B, PairFiltOut d, float* __restrict__ gout,
+                                   int64_t gstride) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)B * d.n_out) return;
+  const int b = (int)(e / d.n_out), o = (int)(e - (int64_t)b * d.n_out);
+  float acc = 0.f;
+  for (int sl = 0; sl < S; ++sl) acc += part[((int64_t)sl * B + b) * d.n_out + o];
+  int xi, yi;
+  int64_t dst;
+  pair_filt_slot(d, o, xi, yi, dst);
+  gout[(int64_t)b * gstride + dst] = acc;
+}
+
+void launch_pair_filt(hipStream_t s, const float* rec, const float* pg, const uint8_t* same, int want,
+                      const int64_t* rng, int B, int64_t avg_rows, int w0, int w1, int64_t oK, int64_t ob,
+                      int64_t osp, int64_t oWd, int64_t obd, int64_t oWe, float* gout, int64_t gstride, float* part,
+                      int64_t part_cap) {
+  if (B == 0) return;
+  PairFiltOut d{w0, w1, 6 * w0 + 2 * w0 * w1 + w1, oK, ob, osp, oWd, obd, oWe};
+  int S = (int)std::max<int64_t>(1, std::min<int64_t>(16, avg_rows / 128));
+  while (S > 1 && (int64_t)S * B * d.n_out > part_cap) --S;
+  k_pair_filt<<<dim3((unsigned)B, (unsigned)S), 256, 0, s>>>(rec, pg, same, want, rng, d, gout, gstride, part);
+  if (S > 1) {
+    const int64_t n = (int64_t)B * d.n_out;
+    k_pair_filt_reduce<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(part, S, B, d, gout, gstride);
+  }
 }
 
 // Sum of v[k] over the 64 lanes for every k < W1, spread over the lanes: halving steps over
