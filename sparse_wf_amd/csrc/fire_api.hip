@@ -1850,31 +1850,20 @@ void run_grad(Ctx& c, const float* r, int Bc, float* gout, int64_t gstride, cons
     int32_t* pctr = (int32_t*)buf(sizeof(int32_t) * (size_t)P);
     launch_gr_pair_fwd(s, r, Bc, N, nu, h->ptr.as<int64_t>(), h->nbr.as<int32_t>(), eps, epd, h->cut_learned, P, prec,
                        pbeta, plsf, psame, pctr);
-    float* pdgam = fb(P * F);
+    // per edge class: Gamma / e cotangents, d beta and the filter backward in one pass (du kept
+    // for the message collection), then the messages' rows of the centres and neighbours
     float* pdu = fb(P * F);
-    launch_gr_msg_bwd(s, pbeta, plsf, eps, epd, m5, dhc, pctr, h->nbr.as<int32_t>(), psame, Bc, N, nu, F, P, pdgam, pdu);
-    launch_gr_msg_collect(s, pdu, h->ptr.as<int64_t>(), h->nbr.as<int32_t>(), h->rev.as<int32_t>(), psame, Bc, N, nu, F,
-                          dm5);
-    int64_t* prng = (int64_t*)buf(sizeof(int64_t) * 4 * (size_t)Bc);
-    launch_pair_ranges(s, h->ptr.as<int64_t>(), Bc, nu, nd, prng);
-    WRows pr;
-    pr.rng = prng;
-    pr.avg_rows = P / std::max(Bc, 1);
-    float* pdbeta = fb(P * w1);
-    launch_gr_dbeta(s, pdgam, F, eps.WG, epd.WG, psame, w1, P, pdbeta);
-    float* ppg = fb(P * PG_FLOATS);
-    launch_gr_pair_bwd(s, prec, pdbeta, psame, eps, epd, h->cut_learned, P, ppg);
+    const int64_t nblk = (N + 3) / 4;
+    float* ppart = fb((int64_t)Bc * nblk * ((int64_t)(w1 + 5) * F + 6 * w0 + 2 * w0 * w1 + w1 + 1));
     for (int k = 0; k < 2; ++k) {
       const int base = k == 0 ? h->iEEs : h->iEEd;
-      const int want = k == 0 ? 1 : 0;
-      launch_pair_filt(s, prec, ppg, psame, want, prng, Bc, pr.avg_rows, w0, w1, off[base + 0], off[base + 1],
-                       off[base + 2], off[base + 3], off[base + 4], off[base + 5], gout, gstride, wpart, part_cap);
-      wg(pbeta, w1, pdgam, F, w1, F, pr, off[base + 6], psame, want);
-      wg(plsf, 4, pdu, F, 4, F, pr, off[base + 7], psame, want);
-      wg(nullptr, 0, pdu, F, 1, F, pr, off[base + 8], psame, want);
+      launch_gr_pair_cot(s, h->ptr.as<int64_t>(), h->nbr.as<int32_t>(), psame, k == 0 ? 1 : 0, pbeta, plsf, prec, m5,
+                         dhc, k == 0 ? eps : epd, h->cut_learned, Bc, N, nu, F, pdu, off[base + 0], off[base + 1],
+                         off[base + 2], off[base + 3], off[base + 4], off[base + 5], off[base + 6], off[base + 7],
+                         off[base + 8], off[h->iSCAL], k == 1, ppart, gout, gstride);
     }
-    // learned cutoff (the first entry of the last packed tensor)
-    wg(nullptr, 0, ppg + PG_DCUT, PG_FLOATS, 1, 1, pr, off[h->iSCAL]);
+    launch_gr_msg_collect(s, pdu, h->ptr.as<int64_t>(), h->nbr.as<int32_t>(), h->rev.as<int32_t>(), psame, Bc, N, nu, F,
+                          dm5);
   }
   // ---- ElecInit dense stage ------------------------------------------------------------------
   wg(y3, F, dm5, 5 * F, F, 5 * F, both, off[b + 14]);

@@ -76,6 +76,11 @@ void launch_pair_filt(hipStream_t s, const float* rec, const float* pg, const ui
                       const int64_t* rng, int B, int64_t avg_rows, int w0, int w1, int64_t oK, int64_t ob,
                       int64_t osp, int64_t oWd, int64_t obd, int64_t oWe, float* gout, int64_t gstride, float* part,
                       int64_t part_cap);
+void launch_gr_pair_cot(hipStream_t s, const int64_t* ptr, const int32_t* nbr, const uint8_t* same, int want,
+                        const float* beta, const float* lsf, const float* rec, const float* M5, const float* dh,
+                        const EdgeParams& ep, float cut, int B, int N, int nu, int F, float* du, int64_t oK, int64_t ob,
+                        int64_t osp, int64_t oWd, int64_t obd, int64_t oWe, int64_t oWG, int64_t oWF, int64_t oBF,
+                        int64_t ocut, bool add_cut, float* part, float* gout, int64_t gstride);
 void launch_silu_bwd(hipStream_t s, float* dz, int64_t ldo, const float* dy, int64_t ldy, const float* z, int64_t ldz,
                      int64_t rows, int cols);
 void launch_silu_fwd(hipStream_t s, float* y, const float* z, int64_t n);

@@ -1595,6 +1595,255 @@ void launch_gr_en_cot(hipStream_t s, const float* beta, const float* lsf, const 
                                                                    gstride, o_wd, o_we, o_bd);
 }
 
+// Fused e-e edge backward for one edge class (want = 1: same spin, 0: opposite spin), one
+// wavefront per centre electron (four per workgroup), lane features f = lane + 64 t.  For each of
+// the centre's pairs of the class: Gamma = beta WG, e = lsf WF + BF, u = e + m_ct[i] + m_nb[j]
+// rebuilt, dGamma = dh_i u sigma(u) and du = dh_i Gamma silu'(u) in registers (du is also written:
+// the message collection needs it for the neighbour side), WG / WF / BF products accumulated,
+// d beta reduced across the wavefront, then the pair filter's backward lane-parallel as in
+// k_gr_en_cot (K, b, sp, Wd, bd, We and the learned cutoff).  Partials per (walker, workgroup)
+// in part: [WG (W1 F) | WF (4 F) | BF (F) | filter outputs (pair_filt_slot order) | d cutoff].
+template <int W1>
+__global__ void __launch_bounds__(256) k_gr_pair_cot(const int64_t* __restrict__ ptr, const int32_t* __restrict__ nbr,
+                                                     const uint8_t* __restrict__ same, int want,
+                                                     const float* __restrict__ beta, const float* __restrict__ lsf,
+                                                     const float* __restrict__ rec, const float* __restrict__ M5,
+                                                     const float* __restrict__ dh, EdgeParams ep, float cut, int B,
+                                                     int N, int nu, int F, float* __restrict__ du, PairFiltOut d,
+                                                     float* __restrict__ part) {
+  __shared__ float red[4][64 * FIRE_W1MAX];
+  const int b = blockIdx.y, blk = blockIdx.x;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int i = blk * 4 + wid;
+  const int T = F >> 6, nd = N - nu;
+  const FilterParams& fp = ep.filt;
+  const int w0 = fp.w0;
+  float wg[W1][4], wf[4][4], bf[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int f = lane + 64 * t;
+    const bool ok = t < T;
+#pragma unroll
+    for (int k = 0; k < W1; ++k) wg[k][t] = ok ? ep.WG[k * F + f] : 0.f;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) wf[c][t] = ok ? ep.WF[c * F + f] : 0.f;
+    bf[t] = ok ? ep.BF[f] : 0.f;
+  }
+  const bool la = lane < w0;  // pre-activation a = lane; envelope q = lane (E = w0 for e-e edges)
+  float wd[W1], we[W1], aWd[W1], aWe[W1], abd[W1];
+#pragma unroll
+  for (int k = 0; k < W1; ++k) {
+    wd[k] = la ? fp.Wd[lane * W1 + k] : 0.f;
+    we[k] = la ? fp.We[lane * W1 + k] : 0.f;
+    aWd[k] = aWe[k] = abd[k] = 0.f;
+  }
+  const float spl = la ? fp.sp[lane] : 1.f;
+  float aK[4] = {0.f, 0.f, 0.f, 0.f}, abb = 0.f, asp = 0.f, adc = 0.f;
+  float aw[W1][4], af[4][4], ab[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+#pragma unroll
+    for (int k = 0; k < W1; ++k) aw[k][t] = 0.f;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) af[c][t] = 0.f;
+    ab[t] = 0.f;
+  }
+  if (i < N) {
+    const int64_t g = group_index(b, i, B, nu, nd);
+    float dhv[4], mct[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const bool ok = t < T;
+      const int f = lane + 64 * t;
+      dhv[t] = ok ? dh[g * F + f] : 0.f;
+      mct[t] = ok ? M5[g * 5 * F + (want ? 1 : 2) * F + f] : 0.f;
+    }
+    const int64_t p1 = ptr[g + 1];
+    for (int64_t p = ptr[g]; p < p1; ++p) {
+      if ((same[p] != 0) != (want != 0)) continue;
+      const int64_t gj = group_index(b, nbr[p], B, nu, nd);
+      float bk[W1], lc[4], db[W1];
+#pragma unroll
+      for (int k = 0; k < W1; ++k) {
+        bk[k] = beta[p * W1 + k];
+        db[k] = 0.f;
+      }
+#pragma unroll
+      for (int c = 0; c < 4; ++c) lc[c] = lsf[p * 4 + c];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        if (t >= T) break;
+        const int f = lane + 64 * t;
+        const float mnb = M5[gj * 5 * F + (want ? 3 : 4) * F + f];
+        float gam = 0.f, ev = bf[t];
+#pragma unroll
+        for (int k = 0; k < W1; ++k) gam += bk[k] * wg[k][t];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) ev += lc[c] * wf[c][t];
+        const float u = ev + mct[t] + mnb;
+        const float sg = sigm(u);
+        const float dgam = dhv[t] * u * sg;
+        const float duv = dhv[t] * gam * sg * (1.0f + u * (1.0f - sg));
+        du[p * F + f] = duv;
+#pragma unroll
+        for (int k = 0; k < W1; ++k) {
+          aw[k][t] += bk[k] * dgam;
+          db[k] += dgam * wg[k][t];
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) af[c][t] += lc[c] * duv;
+        ab[t] += duv;
+      }
+      const float tot = xreduce<W1>(db, lane);
+      const float* rc = rec + p * PR_FLOATS;
+      const float dd = rc[PR_D];
+      float ddir[W1], denvd[W1];
+#pragma unroll
+      for (int k = 0; k < W1; ++k) {
+        const float dbk = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(tot), k * (64 / W1)));
+        ddir[k] = dbk * rc[PR_ENVD + k];
+        denvd[k] = dbk * rc[PR_DIR + k];
+        abd[k] += ddir[k];
+      }
+      if (la) {
+        const float pre = rc[PR_PRE + lane];
+        float dy = 0.f, den = 0.f;
+#pragma unroll
+        for (int k = 0; k < W1; ++k) {
+          dy += wd[k] * ddir[k];
+          den += we[k] * denvd[k];
+        }
+        const float y = silu_f(pre), dpre = dy * silu_d(pre);
+        const float envr = rc[PR_ENV + lane];
+#pragma unroll
+        for (int k = 0; k < W1; ++k) {
+          aWd[k] += y * ddir[k];
+          aWe[k] += envr * denvd[k];
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) aK[c] += rc[PR_FEAT + c] * dpre;
+        abb += dpre;
+        float fc, dfc;
+        cutoff_poly(dd / cut, fc, dfc);
+        const float x = dd / spl, g0 = expf(-x * x);
+        asp += den * g0 * fc * 2.f * x * x / spl;
+        adc += den * g0 * dfc * (-dd / (cut * cut));
+      }
+    }
+  }
+  // combine the four wavefronts into this workgroup's partials
+  const int64_t PL = (int64_t)(W1 + 5) * F + d.n_out + 1;
+  float* pb = part + ((int64_t)b * gridDim.x + blk) * PL;
+  const int f_out = (threadIdx.x >> 2) + 64 * (threadIdx.x & 3);
+  const bool f_ok = (threadIdx.x & 3) < T;
+  for (int r = 0; r < W1 + 5; ++r) {
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      float v;
+      if (r < W1) {
+        v = aw[0][t];
+#pragma unroll
+        for (int k = 1; k < W1; ++k) v = r == k ? aw[k][t] : v;
+      } else if (r < W1 + 4) {
+        v = af[0][t];
+#pragma unroll
+        for (int c = 1; c < 4; ++c) v = r == W1 + c ? af[c][t] : v;
+      } else {
+        v = ab[t];
+      }
+      red[wid][lane * 4 + t] = v;
+    }
+    __syncthreads();
+    if (f_ok) pb[(int64_t)r * F + f_out] = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
+    __syncthreads();
+  }
+  float* pf = pb + (int64_t)(W1 + 5) * F;  // filter outputs in pair_filt_slot order
+  const int w1 = W1;
+  for (int r = 0; r < 3; ++r) {  // 0: Wd, 1: We, 2: bd
+#pragma unroll
+    for (int k = 0; k < W1; ++k) red[wid][lane * W1 + k] = r == 0 ? aWd[k] : (r == 1 ? aWe[k] : abd[k]);
+    __syncthreads();
+    for (int e = threadIdx.x; e < 64 * W1; e += blockDim.x) {
+      const int ln = e / W1, k = e - ln * W1;
+      const float v = red[0][e] + red[1][e] + red[2][e] + red[3][e];
+      if (r == 0 && ln < w0) pf[6 * w0 + ln * w1 + k] = v;
+      if (r == 1 && ln < w0) pf[6 * w0 + w0 * w1 + w1 + ln * w1 + k] = v;
+      if (r == 2 && ln == 0) pf[6 * w0 + w0 * w1 + k] = v;
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int c = 0; c < 7; ++c) {
+    red[wid][lane] = c < 4 ? aK[c] : (c == 4 ? abb : (c == 5 ? asp : adc));
+    __syncthreads();
+    if (threadIdx.x < 64) {
+      const int ln = threadIdx.x;
+      const float v = red[0][ln] + red[1][ln] + red[2][ln] + red[3][ln];
+      if (c < 4 && ln < w0) pf[c * w0 + ln] = v;
+      if (c == 4 && ln < w0) pf[4 * w0 + ln] = v;
+      if (c == 5 && ln < w0) pf[5 * w0 + ln] = v;
+      if (c == 6) red[0][64 + ln] = v;  // per-lane cutoff terms, summed below
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    float sdc = 0.f;
+    for (int ln = 0; ln < w0; ++ln) sdc += red[0][64 + ln];
+    pf[d.n_out] = sdc;
+  }
+}
+
+// Sums the partials of k_gr_pair_cot over the walker's workgroups (in order) into the gradient
+// row: WG, WF, BF and the filter outputs of the class; the learned-cutoff term is written
+// (first class) or added (second class).
+__global__ void k_gr_pair_cot_reduce(const float* __restrict__ part, int nblk, int B, int W1, int F, PairFiltOut d,
+                                     float* __restrict__ gout, int64_t gstride, int64_t oWG, int64_t oWF, int64_t oBF,
+                                     int64_t ocut, int add_cut) {
+  const int64_t PL = (int64_t)(W1 + 5) * F + d.n_out + 1;
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)B * PL) return;
+  const int b = (int)(e / PL);
+  const int64_t x = e - (int64_t)b * PL;
+  float acc = 0.f;
+  for (int k = 0; k < nblk; ++k) acc += part[((int64_t)b * nblk + k) * PL + x];
+  float* g = gout + (int64_t)b * gstride;
+  if (x < (int64_t)W1 * F) {
+    g[oWG + x] = acc;
+  } else if (x < (int64_t)(W1 + 4) * F) {
+    g[oWF + x - (int64_t)W1 * F] = acc;
+  } else if (x < (int64_t)(W1 + 5) * F) {
+    g[oBF + x - (int64_t)(W1 + 4) * F] = acc;
+  } else if (x < PL - 1) {
+    int xi, yi;
+    int64_t dst;
+    pair_filt_slot(d, (int)(x - (int64_t)(W1 + 5) * F), xi, yi, dst);
+    g[dst] = acc;
+  } else {
+    g[ocut] = add_cut ? g[ocut] + acc : acc;
+  }
+}
+
+void launch_gr_pair_cot(hipStream_t s, const int64_t* ptr, const int32_t* nbr, const uint8_t* same, int want,
+                        const float* beta, const float* lsf, const float* rec, const float* M5, const float* dh,
+                        const EdgeParams& ep, float cut, int B, int N, int nu, int F, float* du, int64_t oK, int64_t ob,
+                        int64_t osp, int64_t oWd, int64_t obd, int64_t oWe, int64_t oWG, int64_t oWF, int64_t oBF,
+                        int64_t ocut, bool add_cut, float* part, float* gout, int64_t gstride) {
+  if (B == 0 || N == 0) return;
+  const int W1 = ep.filt.w1, w0 = ep.filt.w0;
+  PairFiltOut d{w0, W1, 6 * w0 + 2 * w0 * W1 + W1, oK, ob, osp, oWd, obd, oWe};
+  const int nblk = (N + 3) / 4;
+  const dim3 grid((unsigned)nblk, (unsigned)B);
+  switch (W1) {
+    case 4: k_gr_pair_cot<4><<<grid, 256, 0, s>>>(ptr, nbr, same, want, beta, lsf, rec, M5, dh, ep, cut, B, N, nu, F, du, d, part); break;
+    case 8: k_gr_pair_cot<8><<<grid, 256, 0, s>>>(ptr, nbr, same, want, beta, lsf, rec, M5, dh, ep, cut, B, N, nu, F, du, d, part); break;
+    default: k_gr_pair_cot<16><<<grid, 256, 0, s>>>(ptr, nbr, same, want, beta, lsf, rec, M5, dh, ep, cut, B, N, nu, F, du, d, part); break;
+  }
+  const int64_t PL = (int64_t)(W1 + 5) * F + d.n_out + 1;
+  const int64_t n = (int64_t)B * PL;
+  k_gr_pair_cot_reduce<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(part, nblk, B, W1, F, d, gout, gstride, oWG, oWF, oBF,
+                                                                   ocut, add_cut ? 1 : 0);
+}
+
 void launch_gr_pair_fwd(hipStream_t s, const float* r, int B, int N, int nu, const int64_t* ptr, const int32_t* nbr,
                         const EdgeParams& ps, const EdgeParams& pd, float cut, int64_t P, float* rec, float* beta,
                         float* lsf, uint8_t* same, int32_t* ctr) {
