@@ -27,15 +27,6 @@ constexpr int PR_ENV = 37;   // [E] radial envelopes (e-e pairs, E = w0)
 constexpr int PR_DIR = 69;   // [w1] directional part
 constexpr int PR_ENVD = 85;  // [w1] radial part after Dense_1
 constexpr int PR_FLOATS = 104;
-// per-edge backward record
-constexpr int PG_DDIR = 0;    // [w1]
-constexpr int PG_DENVD = 16;  // [w1]
-constexpr int PG_Y = 32;      // [w0] silu(pre)
-constexpr int PG_DPRE = 64;   // [w0]
-constexpr int PG_DSP = 96;    // [E]
-constexpr int PG_ENVR = 160;  // [E] radial envelopes (e-n edges)
-constexpr int PG_DCUT = 224;  // d learned cutoff (e-e pairs)
-constexpr int PG_FLOATS = 232;
 
 struct GradHead {
   int B, N, nu, D, W, depth, H, Dk, HD, lwmax;
@@ -72,10 +63,6 @@ void launch_gr_en_cot(hipStream_t s, const float* beta, const float* lsf, const 
                       const float* rec, float cutoff_1el, const EnParams& ep, int B, int N, int nu, int M, int F,
                       float* part, float* part2, float* gout, int64_t gstride, int64_t o_wg, int64_t o_wf,
                       int64_t o_hn, int64_t o_K, int64_t o_b, int64_t o_sp, int64_t o_wd, int64_t o_bd, int64_t o_we);
-void launch_pair_filt(hipStream_t s, const float* rec, const float* pg, const uint8_t* same, int want,
-                      const int64_t* rng, int B, int64_t avg_rows, int w0, int w1, int64_t oK, int64_t ob,
-                      int64_t osp, int64_t oWd, int64_t obd, int64_t oWe, float* gout, int64_t gstride, float* part,
-                      int64_t part_cap);
 void launch_gr_pair_cot(hipStream_t s, const int64_t* ptr, const int32_t* nbr, const uint8_t* same, int want,
                         const float* beta, const float* lsf, const float* rec, const float* M5, const float* dh,
                         const EdgeParams& ep, float cut, int B, int N, int nu, int F, float* du, int64_t oK, int64_t ob,
@@ -88,7 +75,6 @@ void launch_axpy(hipStream_t s, float* a, const float* b, int64_t n);
 void launch_fill_f32(hipStream_t s, float* a, int64_t n, float v);
 void launch_mul(hipStream_t s, float* o, const float* a, const float* b, int64_t n);
 void launch_transpose2d(hipStream_t s, const float* A, int R, int C, float* AT);
-void launch_pair_ranges(hipStream_t s, const int64_t* ptr, int B, int nu, int nd, int64_t* rng);
 void launch_gr_env(hipStream_t s, const float* r, const float* R, int B, int N, int nu, int M, int D, int E,
                    const float* sp_up, const float* pi_up, const float* sp_dn, const float* pi_dn, float* ex,
                    float* env, float* dist);
@@ -101,15 +87,8 @@ void launch_gr_sp(hipStream_t s, const float* ex, const float* tt, const float* 
 void launch_gr_pair_fwd(hipStream_t s, const float* r, int B, int N, int nu, const int64_t* ptr, const int32_t* nbr,
                         const EdgeParams& ps, const EdgeParams& pd, float cut, int64_t P, float* rec, float* beta,
                         float* lsf, uint8_t* same, int32_t* ctr);
-void launch_gr_msg_bwd(hipStream_t s, const float* beta, const float* lsf, const EdgeParams& ps, const EdgeParams& pd,
-                       const float* M5, const float* dh, const int32_t* ctr, const int32_t* nbr, const uint8_t* same,
-                       int B, int N, int nu, int F, int64_t P, float* dgam, float* du);
 void launch_gr_msg_collect(hipStream_t s, const float* du, const int64_t* ptr, const int32_t* nbr, const int32_t* rev,
                            const uint8_t* same, int B, int N, int nu, int F, float* dM5);
-void launch_gr_dbeta(hipStream_t s, const float* dgam, int F, const float* WG_same, const float* WG_diff,
-                     const uint8_t* same, int w1, int64_t P, float* dbeta);
-void launch_gr_pair_bwd(hipStream_t s, const float* rec, const float* dbeta, const uint8_t* same, const EdgeParams& ps,
-                        const EdgeParams& pd, float cut, int64_t P, float* pg);
 void launch_gr_en_fwd(hipStream_t s, const float* r, const float* R, int B, int N, int nu, int M, float cutoff_1el,
                       const EnParams& ep, int64_t rows, float* rec, float* beta, float* lsf, uint8_t* mask);
 void launch_gr_en_sum(hipStream_t s, const float* beta, const float* lsf, const EnParams& ep, const uint8_t* mask,

@@ -411,22 +411,6 @@ void launch_transpose2d(hipStream_t s, const float* A, int R, int C, float* AT) 
   if (n) k_transpose2d<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(A, R, C, AT);
 }
 
-// Per-walker row ranges of CSR pairs: rng[4w..4w+3] = the pair slots of the walker's up-spin
-// and down-spin centres.
-__global__ void k_pair_ranges(const int64_t* __restrict__ ptr, int B, int nu, int nd, int64_t* __restrict__ rng) {
-  const int w = blockIdx.x * blockDim.x + threadIdx.x;
-  if (w >= B) return;
-  const int64_t up = (int64_t)B * nu;
-  rng[4 * w + 0] = ptr[(int64_t)w * nu];
-  rng[4 * w + 1] = ptr[(int64_t)(w + 1) * nu];
-  rng[4 * w + 2] = ptr[up + (int64_t)w * nd];
-  rng[4 * w + 3] = ptr[up + (int64_t)(w + 1) * nd];
-}
-
-void launch_pair_ranges(hipStream_t s, const int64_t* ptr, int B, int nu, int nd, int64_t* rng) {
-  if (B) k_pair_ranges<<<(B + 255) / 256, 256, 0, s>>>(ptr, B, nu, nd, rng);
-}
-
 // ---------------------------------------------------------------------------------------
 // envelopes and Slater matrices
 // ---------------------------------------------------------------------------------------
@@ -880,108 +864,6 @@ __global__ void k_gr_pair_fwd(const float* __restrict__ r, int B, int N, int nu,
   }
 }
 
-// Messages backward per (pair, 4 features): Gamma = beta WG, e = lsf WF + BF (same- or
-// different-spin edge weights), u = e + m_ct[i] + m_nb[j], msg = Gamma silu(u):
-// dGamma = dh_i silu(u), du = dh_i Gamma silu'(u).  F % 4 == 0.
-__global__ void k_gr_msg_bwd(const float* __restrict__ beta, const float* __restrict__ lsf, EdgeParams ps,
-                             EdgeParams pd, const float* __restrict__ M5, const float* __restrict__ dh,
-                             const int32_t* __restrict__ ctr, const int32_t* __restrict__ nbr,
-                             const uint8_t* __restrict__ same, int B, int N, int nu, int F, int64_t P,
-                             float* __restrict__ dgam, float* __restrict__ du) {
-  const int F4 = F >> 2;
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= P * F4) return;
-  const int64_t p = e / F4;
-  const int f = (int)(e - p * F4) * 4;
-  const int64_t g = ctr[p];
-  int w, i;
-  group_walker(g, B, nu, N - nu, w, i);
-  const int j = nbr[p];
-  const int64_t gj = group_index(w, j, B, nu, N - nu);
-  const bool sm = same[p] != 0;
-  const EdgeParams& ep = sm ? ps : pd;
-  const int w1 = ep.filt.w1;
-  float4 gam = make_float4(0.f, 0.f, 0.f, 0.f), ev = ld4(ep.BF + f);
-  for (int k = 0; k < w1; ++k) gam = fma4(beta[p * w1 + k], ld4(ep.WG + k * F + f), gam);
-  for (int c = 0; c < 4; ++c) ev = fma4(lsf[p * 4 + c], ld4(ep.WF + c * F + f), ev);
-  const float4 mct = ld4(M5 + g * 5 * F + (sm ? 1 : 2) * F + f);
-  const float4 mnb = ld4(M5 + gj * 5 * F + (sm ? 3 : 4) * F + f);
-  const float4 dhv = ld4(dh + g * F + f);
-  const float uu[4] = {ev.x + mct.x + mnb.x, ev.y + mct.y + mnb.y, ev.z + mct.z + mnb.z, ev.w + mct.w + mnb.w};
-  const float gg[4] = {gam.x, gam.y, gam.z, gam.w}, dd[4] = {dhv.x, dhv.y, dhv.z, dhv.w};
-  float og[4], ou[4];
-#pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    const float s = sigm(uu[c]);
-    og[c] = dd[c] * uu[c] * s;
-    ou[c] = dd[c] * gg[c] * s * (1.0f + uu[c] * (1.0f - s));
-  }
-  st4(dgam + p * F + f, make_float4(og[0], og[1], og[2], og[3]));
-  st4(du + p * F + f, make_float4(ou[0], ou[1], ou[2], ou[3]));
-}
-
-// d beta[p][k] = sum_f dGamma[p][f] WG[k][f] (same- or different-spin weights).  A block takes
-// 256 rows; dGamma and both weight variants pass through LDS in coalesced 32-feature chunks;
-// thread (r, quarter) accumulates rows r, r + 64, r + 128, r + 192 times the quarter's w1 / 4
-// outputs in registers (no cross-lane sums; the weights are LDS broadcasts).
-__global__ void __launch_bounds__(256) k_gr_dbeta(const float* __restrict__ dgam, int F, const float* __restrict__ WGs,
-                                                  const float* __restrict__ WGd, const uint8_t* __restrict__ same,
-                                                  int w1, int64_t P, float* __restrict__ dbeta) {
-  constexpr int RB = 256, FC = 32;
-  __shared__ float tile[RB][FC + 1];
-  __shared__ float wgl[2][FIRE_W1MAX][FC];
-  const int64_t p0 = (int64_t)blockIdx.x * RB;
-  const int rl = threadIdx.x & 63, qt = threadIdx.x >> 6;
-  const int kq = w1 >> 2;  // outputs per quarter (1, 2 or 4)
-  bool sel[4];
-#pragma unroll
-  for (int v = 0; v < 4; ++v) {
-    const int64_t p = p0 + rl + 64 * v;
-    sel[v] = same == nullptr || (p < P && same[p]);
-  }
-  float acc[4][4];
-#pragma unroll
-  for (int v = 0; v < 4; ++v)
-#pragma unroll
-    for (int u = 0; u < 4; ++u) acc[v][u] = 0.f;
-  for (int f0 = 0; f0 < F; f0 += FC) {
-    for (int t = threadIdx.x; t < RB * FC; t += 256) {
-      const int r = t / FC, c = t - r * FC;
-      const int64_t q = p0 + r;
-      tile[r][c] = (q < P && f0 + c < F) ? dgam[q * F + f0 + c] : 0.f;
-    }
-    for (int t = threadIdx.x; t < 2 * w1 * FC; t += 256) {
-      const int vr = t / (w1 * FC), rem = t - vr * w1 * FC, k = rem / FC, c = rem - k * FC;
-      wgl[vr][k][c] = f0 + c < F ? (vr == 0 ? WGs : WGd)[k * F + f0 + c] : 0.f;
-    }
-    __syncthreads();
-    const int fn = F - f0 < FC ? F - f0 : FC;
-    for (int c = 0; c < fn; ++c) {
-      float ws[4], wd[4], x[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        ws[u] = u < kq ? wgl[0][qt * kq + u][c] : 0.f;
-        wd[u] = u < kq ? wgl[1][qt * kq + u][c] : 0.f;
-      }
-#pragma unroll
-      for (int v = 0; v < 4; ++v) x[v] = tile[rl + 64 * v][c];
-#pragma unroll
-      for (int v = 0; v < 4; ++v)
-#pragma unroll
-        for (int u = 0; u < 4; ++u) acc[v][u] += x[v] * (sel[v] ? ws[u] : wd[u]);
-    }
-    __syncthreads();
-  }
-#pragma unroll
-  for (int v = 0; v < 4; ++v) {
-    const int64_t p = p0 + rl + 64 * v;
-    if (p < P)
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-        if (u < kq) dbeta[p * w1 + qt * kq + u] = acc[v][u];
-  }
-}
-
 // d out5 rows of the messages: d m_ct_s[i] = sum over i's pairs of spin class s of du,
 // d m_nb_s[j] = sum over the pairs (k -> j) of spin class s (found through j's own list and the
 // reverse slots); thread per (group, feature).
@@ -1016,48 +898,6 @@ __global__ void k_gr_msg_collect(const float* __restrict__ du, const int64_t* __
   dM5[g * 5 * F + 2 * F + f] = cd;
   dM5[g * 5 * F + 3 * F + f] = ns;
   dM5[g * 5 * F + 4 * F + f] = nd_;
-}
-
-// Edge filter backward per pair (thread per pair) from d beta [P, w1]: per-pair contributions
-// to the filter parameters laid out as PairGrad (the host reduces them per walker).
-__global__ void k_gr_pair_bwd(const float* __restrict__ rec, const float* __restrict__ dbeta,
-                              const uint8_t* __restrict__ same, EdgeParams ps, EdgeParams pd, float cut, int64_t P,
-                              float* __restrict__ pg) {
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= P) return;
-  const bool sm = same[p] != 0;
-  const FilterParams& f = (sm ? ps : pd).filt;
-  const float* rc = rec + p * PR_FLOATS;
-  const float d = rc[PR_D];
-  float* o = pg + p * PG_FLOATS;
-  float ddir[FIRE_W1MAX], denvd[FIRE_W1MAX];
-  for (int k = 0; k < f.w1; ++k) {
-    const float db = dbeta[p * f.w1 + k];
-    ddir[k] = db * rc[PR_ENVD + k];
-    denvd[k] = db * rc[PR_DIR + k];
-    o[PG_DDIR + k] = ddir[k];
-    o[PG_DENVD + k] = denvd[k];
-  }
-  float fc, dfc;
-  cutoff_poly(d / cut, fc, dfc);
-  float dcut = 0.f;
-  for (int a = 0; a < f.w0; ++a) {
-    // y = silu(pre); d y = sum_k Wd[a][k] d dir[k]; d pre = d y silu'(pre)
-    float dy = 0.f;
-    for (int k = 0; k < f.w1; ++k) dy += f.Wd[a * f.w1 + k] * ddir[k];
-    const float pre = rc[PR_PRE + a];
-    o[PG_Y + a] = silu_f(pre);
-    o[PG_DPRE + a] = dy * silu_d(pre);
-  }
-  for (int q = 0; q < f.E; ++q) {
-    float den = 0.f;
-    for (int k = 0; k < f.w1; ++k) den += f.We[q * f.w1 + k] * denvd[k];
-    // env_r = exp(-(d/sp)^2) fc(d/cut)
-    const float s = f.sp[q], x = d / s, g0 = expf(-x * x);
-    o[PG_DSP + q] = den * g0 * fc * 2.f * x * x / s;
-    dcut += den * g0 * dfc * (-d / (cut * cut));
-  }
-  o[PG_DCUT] = dcut;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -1190,12 +1030,9 @@ __global__ void k_gr_glu_bwd(const float* __restrict__ dy, const float* __restri
   dZ2[g * 2 * F + F + f] = d * x * silu_d(gt);
 }
 
-// The six filter-parameter gradients of one e-e edge class in one pass over its pairs (rows of
-// walker b from its two pair ranges, rows with same[row] != want skipped): K [4][w0] = sum feat
-// (x) d pre, b = sum d pre, sp = sum d sp, Wd [w0][w1] = sum y (x) d dir, bd = sum d dir,
-// We [w0][w1] = sum env (x) d envd.  32 rows at a time are staged in LDS as [feat | env | d pre |
-// d sp | y | d dir | d envd]; thread t owns outputs t, t + 256, ...; with S slices per walker the
-// partials go to part[(slice B + b) n_out + o] and k_pair_filt_reduce sums them in slice order.
+// Where the filter-parameter gradients of one e-e edge class go in the gradient row: output o
+// in the order [K (4 w0) | b (w0) | sp (w0) | Wd (w0 w1) | bd (w1) | We (w0 w1)] (the partials
+// of k_gr_pair_cot), its packed destination dst, and the row fields (xi, yi) it multiplies.
 struct PairFiltOut {
   int w0, w1, n_out;
   int64_t oK, ob, osp, oWd, obd, oWe;
@@ -1229,106 +1066,6 @@ __device__ __forceinline__ void pair_filt_slot(const PairFiltOut& d, int o, int&
     xi = sENV + o / w1;
     yi = sENVD + o % w1;
     dst = d.oWe + o;
-  }
-}
-
-__global__ void __launch_bounds__(256) k_pair_filt(const float* __restrict__ rec, const float* __restrict__ pg,
-                                                   const uint8_t* __restrict__ same, int want,
-                                                   const int64_t* __restrict__ rng, PairFiltOut d,
-                                                   float* __restrict__ gout, int64_t gstride, float* __restrict__ part) {
-  constexpr int RB = 32, ROWMAX = 4 + 4 * FIRE_W0MAX + 2 * FIRE_W1MAX;
-  __shared__ float st[RB][ROWMAX + 1];
-  const int b = blockIdx.x, sl = blockIdx.y, S = gridDim.y, B = gridDim.x;
-  const int w0 = d.w0, w1 = d.w1, RW = 4 + 4 * w0 + 2 * w1;
-  int64_t s0 = rng[4 * b], e0 = rng[4 * b + 1], s1 = rng[4 * b + 2], e1 = rng[4 * b + 3];
-  {
-    const int64_t n0 = e0 - s0, n = n0 + (e1 - s1);
-    const int64_t f0 = n * sl / S, f1 = n * (sl + 1) / S;
-    const int64_t a = s0, c = s1;
-    s0 = a + (f0 < n0 ? f0 : n0);
-    e0 = a + (f1 < n0 ? f1 : n0);
-    s1 = c + (f0 > n0 ? f0 - n0 : 0);
-    e1 = c + (f1 > n0 ? f1 - n0 : 0);
-  }
-  constexpr int NU = (4 * FIRE_W0MAX + 3 * FIRE_W0MAX + 2 * FIRE_W0MAX * FIRE_W1MAX + FIRE_W1MAX + 255) / 256;
-  float acc[NU];
-  int xi[NU], yi[NU];
-  int64_t dst[NU];
-#pragma unroll
-  for (int u = 0; u < NU; ++u) {
-    acc[u] = 0.f;
-    const int o = threadIdx.x + 256 * u;
-    xi[u] = yi[u] = 0;
-    dst[u] = -1;
-    if (o < d.n_out) pair_filt_slot(d, o, xi[u], yi[u], dst[u]);
-  }
-  const int64_t n0 = e0 - s0, nr = n0 + (e1 - s1);
-  for (int64_t r0 = 0; r0 < nr; r0 += RB) {
-    for (int t = threadIdx.x; t < RB * RW; t += 256) {
-      const int rr = t / RW, f = t - rr * RW;
-      const int64_t ri = r0 + rr;
-      float v = 0.f;
-      if (ri < nr) {
-        const int64_t row = ri < n0 ? s0 + ri : s1 + (ri - n0);
-        if (same[row] == want) {
-          const float* rc = rec + row * PR_FLOATS;
-          const float* gp = pg + row * PG_FLOATS;
-          if (f < 4) v = rc[PR_FEAT + f];
-          else if (f < 4 + w0) v = rc[PR_ENV + f - 4];
-          else if (f < 4 + 2 * w0) v = gp[PG_DPRE + f - 4 - w0];
-          else if (f < 4 + 3 * w0) v = gp[PG_DSP + f - 4 - 2 * w0];
-          else if (f < 4 + 4 * w0) v = gp[PG_Y + f - 4 - 3 * w0];
-          else if (f < 4 + 4 * w0 + w1) v = gp[PG_DDIR + f - 4 - 4 * w0];
-          else v = gp[PG_DENVD + f - 4 - 4 * w0 - w1];
-        }
-      }
-      st[rr][f] = v;
-    }
-    __syncthreads();
-    const int nrow = nr - r0 < RB ? (int)(nr - r0) : RB;
-    for (int rr = 0; rr < nrow; ++rr) {
-#pragma unroll
-      for (int u = 0; u < NU; ++u)
-        if (dst[u] >= 0) acc[u] += (xi[u] < 0 ? 1.f : st[rr][xi[u]]) * st[rr][yi[u]];
-    }
-    __syncthreads();
-  }
-#pragma unroll
-  for (int u = 0; u < NU; ++u) {
-    if (dst[u] < 0) continue;
-    const int o = threadIdx.x + 256 * u;
-    if (S > 1)
-      part[((int64_t)sl * B + b) * d.n_out + o] = acc[u];
-    else
-      gout[(int64_t)b * gstride + dst[u]] = acc[u];
-  }
-}
-
-__global__ void k_pair_filt_reduce(const float* __restrict__ part, int S, int B, PairFiltOut d, float* __restrict__ gout,
-                                   int64_t gstride) {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (int64_t)B * d.n_out) return;
-  const int b = (int)(e / d.n_out), o = (int)(e - (int64_t)b * d.n_out);
-  float acc = 0.f;
-  for (int sl = 0; sl < S; ++sl) acc += part[((int64_t)sl * B + b) * d.n_out + o];
-  int xi, yi;
-  int64_t dst;
-  pair_filt_slot(d, o, xi, yi, dst);
-  gout[(int64_t)b * gstride + dst] = acc;
-}
-
-void launch_pair_filt(hipStream_t s, const float* rec, const float* pg, const uint8_t* same, int want,
-                      const int64_t* rng, int B, int64_t avg_rows, int w0, int w1, int64_t oK, int64_t ob,
-                      int64_t osp, int64_t oWd, int64_t obd, int64_t oWe, float* gout, int64_t gstride, float* part,
-                      int64_t part_cap) {
-  if (B == 0) return;
-  PairFiltOut d{w0, w1, 6 * w0 + 2 * w0 * w1 + w1, oK, ob, osp, oWd, obd, oWe};
-  int S = (int)std::max<int64_t>(1, std::min<int64_t>(16, avg_rows / 128));
-  while (S > 1 && (int64_t)S * B * d.n_out > part_cap) --S;
-  k_pair_filt<<<dim3((unsigned)B, (unsigned)S), 256, 0, s>>>(rec, pg, same, want, rng, d, gout, gstride, part);
-  if (S > 1) {
-    const int64_t n = (int64_t)B * d.n_out;
-    k_pair_filt_reduce<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(part, S, B, d, gout, gstride);
   }
 }
 
@@ -1849,26 +1586,10 @@ void launch_gr_pair_fwd(hipStream_t s, const float* r, int B, int N, int nu, con
                         float* lsf, uint8_t* same, int32_t* ctr) {
   if (P) k_gr_pair_fwd<<<(unsigned)((P + 127) / 128), 128, 0, s>>>(r, B, N, nu, ptr, nbr, ps, pd, cut, P, rec, beta, lsf, same, ctr);
 }
-void launch_gr_msg_bwd(hipStream_t s, const float* beta, const float* lsf, const EdgeParams& ps, const EdgeParams& pd,
-                       const float* M5, const float* dh, const int32_t* ctr, const int32_t* nbr, const uint8_t* same,
-                       int B, int N, int nu, int F, int64_t P, float* dgam, float* du) {
-  const int64_t n = P * (F / 4);
-  if (n)
-    k_gr_msg_bwd<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(beta, lsf, ps, pd, M5, dh, ctr, nbr, same, B, N, nu, F, P,
-                                                             dgam, du);
-}
-void launch_gr_dbeta(hipStream_t s, const float* dgam, int F, const float* WG_same, const float* WG_diff,
-                     const uint8_t* same, int w1, int64_t P, float* dbeta) {
-  if (P) k_gr_dbeta<<<(unsigned)((P + 255) / 256), 256, 0, s>>>(dgam, F, WG_same, WG_diff, same, w1, P, dbeta);
-}
 void launch_gr_msg_collect(hipStream_t s, const float* du, const int64_t* ptr, const int32_t* nbr, const int32_t* rev,
                            const uint8_t* same, int B, int N, int nu, int F, float* dM5) {
   const int64_t n = (int64_t)B * N * F;
   if (n) k_gr_msg_collect<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(du, ptr, nbr, rev, same, B, N, nu, F, dM5);
-}
-void launch_gr_pair_bwd(hipStream_t s, const float* rec, const float* dbeta, const uint8_t* same, const EdgeParams& ps,
-                        const EdgeParams& pd, float cut, int64_t P, float* pg) {
-  if (P) k_gr_pair_bwd<<<(unsigned)((P + 127) / 128), 128, 0, s>>>(rec, dbeta, same, ps, pd, cut, P, pg);
 }
 void launch_gr_en_fwd(hipStream_t s, const float* r, const float* R, int B, int N, int nu, int M, float cutoff_1el,
                       const EnParams& ep, int64_t rows, float* rec, float* beta, float* lsf, uint8_t* mask) {
