@@ -1642,7 +1642,14 @@ void run_grad(Ctx& c, const float* r, int Bc, float* gout, int64_t gstride, cons
   float* ex = fb(G * DME);
   float* env = fb(G * ND);
   float* dist = fb(G * M);
-  launch_gr_env(s, r, h->R, Bc, N, nu, M, D, Ei, h->P[o + 4], h->P[o + 5], h->P[o + 6], h->P[o + 7], ex, env, dist);
+  // exp(-sigma |r - R|) per (row, det, nucleus, envelope); the envelope sums as FP32 matrix-core
+  // contractions per spin block, batched over determinants (pi transposed per determinant)
+  launch_gr_env(s, r, h->R, Bc, N, nu, M, D, Ei, h->P[o + 4], h->P[o + 5], h->P[o + 6], h->P[o + 7], ex, nullptr,
+                dist);
+  launch_gemm_f32_batched(s, ex, DME, M * Ei, h->Pt32[o + 5], (int64_t)M * Ei * N, env, ND, N, Gu, N, M * Ei, D);
+  if (Gd)
+    launch_gemm_f32_batched(s, ex + Gu * DME, DME, M * Ei, h->Pt32[o + 7], (int64_t)M * Ei * N, env + Gu * ND, ND, N,
+                            Gd, N, M * Ei, D);
   const int64_t nmat = (int64_t)Bc * D;
   double* A64 = (double*)buf(sizeof(double) * (size_t)nmat * N * N);
   double* Ainv = (double*)buf(sizeof(double) * (size_t)nmat * N * N);

@@ -439,6 +439,7 @@ __global__ void __launch_bounds__(256) k_gr_env(const float* __restrict__ r, con
     ex[g * DME + t] = x;
     if (d == 0 && e == 0) dist[g * M + m] = dm;
   }
+  if (!env) return;  // the sums run as a contraction elsewhere
   __syncthreads();
   const int ND = D * N;
   for (int c = threadIdx.x; c < ND; c += blockDim.x) {
