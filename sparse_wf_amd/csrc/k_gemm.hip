@@ -27,6 +27,7 @@ typedef float float16_t __attribute__((ext_vector_type(16)));
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
 typedef float float2v_t __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
 
 // Split-plane product terms (plane of A, plane of B), smallest first: for NP = 3 the six
 // products of order >= 2^-18 relative, for NP = 2 the three of order >= 2^-9, for NP = 1 one.
@@ -148,6 +149,17 @@ __device__ __forceinline__ void rows_max_out(float (&mx)[2][16], int64_t rb, int
 
 template <int V>
 using IC = std::integral_constant<int, V>;
+
+// f(IC<I>()), f(IC<I + 1>()), ..., f(IC<N - 1>()): indices into register arrays stay compile-time
+// constants from the front end on (a #pragma unroll loop is unrolled only after the arrays it
+// indexes may already have been placed in scratch memory)
+template <int I, int N, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (I < N) {
+    f(IC<I>());
+    static_for<I + 1, N>(f);
+  }
+}
 
 // Number of the wavefront's 32-column MFMA tiles (of nb) that hold columns below N: the last
 // N tile of a contraction whose N is not a multiple of the tile width multiplies only those.
@@ -887,11 +899,16 @@ __global__ void __launch_bounds__(BM * BN / 64) k_gemm_bf16_db(const float* __re
   store_tile<false>(acc, m0, n0, wm, wn, lane, C, ldc, M, N, bias, rowflag, period, ep);
 }
 
-// Deep-A form of the double-buffered kernel: the A chunks (HBM) run PA chunks ahead of their
-// split into LDS through a ring of register buffers, the weight planes (L2-resident) one chunk
-// ahead.  With A_PER float4 per thread and chunk the ring costs 4 PA A_PER registers; the
-// bytes of A in flight per CU grow PA-fold, which is what bounds the staged forms on the
-// tall, narrow contractions of the tangent rows (a few hundred columns, K = 256).
+// Deep-A form of the double-buffered kernel: chunk c + PA (its A values from HBM and its weight
+// planes from L2) is loaded into a ring of register buffers while chunk c is multiplied, and
+// split into LDS one chunk ahead.  Every chunk issues the same loads in the same order, all of
+// them unconditionally (rows past M read row M - 1, columns past N column N - 1, chunks past
+// the end the last chunk again; none of these is ever stored), so the wait before a chunk's
+// split is a memory counter of (PA - 1) chunks' loads and PA - 1 chunks stay in flight: the
+// bytes in flight per CU are what bounds the staged forms on the tall, narrow contractions
+// of the tangent rows (a few hundred columns, K = 256).  A load under a lane condition, or a
+// weight load issued after the A loads it has to be waited with, would force a wait for every
+// outstanding load at each chunk.
 template <int NP, int BM, int BN, int KC, int PA, bool FUSE = false, bool F16 = false>
 __global__ void __launch_bounds__(BM * BN / 64) k_gemm_bf16_dq(const float* __restrict__ A, int64_t lda,
                                                               const uint16_t* __restrict__ Wp0,
@@ -935,37 +952,37 @@ __global__ void __launch_bounds__(BM * BN / 64) k_gemm_bf16_dq(const float* __re
       for (int v = 0; v < 16; ++v) acc[a][b][v] = 0.f;
   const uint16_t* Wpl[3] = {Wp0, Wp1, Wp2};
   // FP16 planes: the scale of every A row this thread splits (the same rows in every chunk)
+  // the A rows and weight columns this thread loads (the same in every chunk), clamped into
+  // the operands; M >= 1 and N >= 1 here
+  const float* arow[A_PER];
   float rsc[A_PER];
 #pragma unroll
   for (int q = 0; q < A_PER; ++q) {
-    const int e = tid + q * NT;
-    rsc[q] = (F16 && e < A_UNITS) ? row_scale_of(ep.amax, m0 + e / (KC / 4), M) : 1.f;
+    const int e = (tid + q * NT) % A_UNITS;
+    const int64_t gr = m0 + e / (KC / 4) < M ? m0 + e / (KC / 4) : M - 1;
+    arow[q] = A + gr * lda + (e % (KC / 4)) * 4;
+    rsc[q] = F16 ? f16_row_scale(ep.amax[gr]) : 1.f;
   }
+  int64_t bcol[B_PER];
+#pragma unroll
+  for (int q = 0; q < B_PER; ++q) {
+    const int e = (tid + q * NT) % B_UNITS;
+    const int gn = n0 + e / (KC / 8) < N ? n0 + e / (KC / 8) : N - 1;
+    bcol[q] = (int64_t)gn * K + (e % (KC / 8)) * 8;
+  }
+  const int nch = K / KC;
   float4 ra[PA][A_PER];
-  uint4 rb[NP][B_PER];
-  auto load_a = [&](int c, float4(&xa)[A_PER]) {
+  u32x4_t rb[PA][NP][B_PER];
+  auto load = [&](int c, float4(&xa)[A_PER], u32x4_t(&xb)[NP][B_PER]) __attribute__((always_inline)) {
+    const int k0 = (c < nch ? c : nch - 1) * KC;
 #pragma unroll
-    for (int q = 0; q < A_PER; ++q) {
-      const int e = tid + q * NT;
-      const int rr = e / (KC / 4), kq = (e % (KC / 4)) * 4;
-      const int64_t gr = m0 + rr;
-      xa[q] = (e < A_UNITS && gr < M) ? *reinterpret_cast<const float4*>(A + gr * lda + c * KC + kq)
-                                      : make_float4(0.f, 0.f, 0.f, 0.f);
-    }
+    for (int q = 0; q < A_PER; ++q) xa[q] = *reinterpret_cast<const float4*>(arow[q] + k0);
+#pragma unroll
+    for (int q = 0; q < B_PER; ++q)
+#pragma unroll
+      for (int p = 0; p < NP; ++p) xb[p][q] = *reinterpret_cast<const u32x4_t*>(Wpl[p] + bcol[q] + k0);
   };
-  auto load_b = [&](int c) {
-#pragma unroll
-    for (int q = 0; q < B_PER; ++q) {
-      const int e = tid + q * NT;
-      const int nn = e / (KC / 8), kq = (e % (KC / 8)) * 8;
-      const int gn = n0 + nn;
-#pragma unroll
-      for (int p = 0; p < NP; ++p)
-        rb[p][q] = (e < B_UNITS && gn < N) ? *reinterpret_cast<const uint4*>(Wpl[p] + (int64_t)gn * K + c * KC + kq)
-                                           : make_uint4(0, 0, 0, 0);
-    }
-  };
-  auto store = [&](int st, const float4(&xa)[A_PER]) {
+  auto store = [&](int st, const float4(&xa)[A_PER], const u32x4_t(&xb)[NP][B_PER]) __attribute__((always_inline)) {
 #pragma unroll
     for (int q = 0; q < A_PER; ++q) {
       const int e = tid + q * NT;
@@ -982,7 +999,7 @@ __global__ void __launch_bounds__(BM * BN / 64) k_gemm_bf16_dq(const float* __re
       if (B_UNITS % NT != 0 && e >= B_UNITS) continue;
       const int nn = e / (KC / 8), kq = (e % (KC / 8)) * 8;
 #pragma unroll
-      for (int p = 0; p < NP; ++p) *reinterpret_cast<uint4*>(Bs(st, p, nn) + kq) = rb[p][q];
+      for (int p = 0; p < NP; ++p) *reinterpret_cast<u32x4_t*>(Bs(st, p, nn) + kq) = xb[p][q];
     }
   };
   auto compute = [&](int st) {
@@ -1007,30 +1024,21 @@ __global__ void __launch_bounds__(BM * BN / 64) k_gemm_bf16_dq(const float* __re
                                                                 acc[a][b], 0, 0, 0);
     }
   };
-  const int nch = K / KC;
-  // chunk 0 into stage 0; chunks 1 .. PA in the ring (chunk j in ra[j % PA]); W chunk 1 in rb
-  load_a(0, ra[0]);
-  load_b(0);
-  store(0, ra[0]);
-#pragma unroll
-  for (int j = 1; j <= PA; ++j)
-    if (j < nch) load_a(j, ra[j % PA]);
-  if (nch > 1) load_b(1);
+  // chunk j lives in ring slot j % PA: chunks 0 .. PA - 1 first, chunk 0 split into stage 0,
+  // chunk PA into its slot
+  static_for<0, PA>([&](auto J) __attribute__((always_inline)) { load(J.value, ra[J.value], rb[J.value]); });
+  store(0, ra[0], rb[0]);
+  load(PA, ra[0], rb[0]);
   __syncthreads();
   for (int c = 0; c < nch; c += PA) {
-#pragma unroll
-    for (int q = 0; q < PA; ++q) {
-      const int cc = c + q;  // chunk in stage cc & 1; chunk cc + 1 waits in ra[(q + 1) % PA]
-      if (cc < nch) {
-        compute(cc & 1);
-        if (cc + 1 < nch) {
-          store((cc + 1) & 1, ra[(q + 1) % PA]);
-          if (cc + 1 + PA < nch) load_a(cc + 1 + PA, ra[(q + 1) % PA]);
-          if (cc + 2 < nch) load_b(cc + 2);
-        }
-        __syncthreads();
-      }
-    }
+    static_for<0, PA>([&](auto Q) __attribute__((always_inline)) {
+      constexpr int slot = (decltype(Q)::value + 1) % PA;
+      const int cc = c + decltype(Q)::value;  // chunk cc in stage cc & 1; chunk cc + 1 in `slot`
+      if (cc < nch) compute(cc & 1);
+      if (cc + 1 < nch) store((cc + 1) & 1, ra[slot], rb[slot]);
+      load(cc + 1 + PA, ra[slot], rb[slot]);
+      __syncthreads();
+    });
   }
   store_tile<FUSE, F16>(acc, m0, n0, wm, wn, lane, C, ldc, M, N, bias, rowflag, period, ep);
 }
@@ -1082,37 +1090,37 @@ __global__ void __launch_bounds__(BM * BN / 128) __attribute__((amdgpu_waves_per
       for (int v = 0; v < 16; ++v) acc[a][b][v] = 0.f;
   const uint16_t* Wpl[3] = {Wp0, Wp1, Wp2};
   // FP16 planes: the scale of every A row this thread splits (the same rows in every chunk)
+  // the A rows and weight columns this thread loads (the same in every chunk), clamped into
+  // the operands; M >= 1 and N >= 1 here
+  const float* arow[A_PER];
   float rsc[A_PER];
 #pragma unroll
   for (int q = 0; q < A_PER; ++q) {
-    const int e = tid + q * NT;
-    rsc[q] = (F16 && e < A_UNITS) ? row_scale_of(ep.amax, m0 + e / (KC / 4), M) : 1.f;
+    const int e = (tid + q * NT) % A_UNITS;
+    const int64_t gr = m0 + e / (KC / 4) < M ? m0 + e / (KC / 4) : M - 1;
+    arow[q] = A + gr * lda + (e % (KC / 4)) * 4;
+    rsc[q] = F16 ? f16_row_scale(ep.amax[gr]) : 1.f;
   }
+  int64_t bcol[B_PER];
+#pragma unroll
+  for (int q = 0; q < B_PER; ++q) {
+    const int e = (tid + q * NT) % B_UNITS;
+    const int gn = n0 + e / (KC / 8) < N ? n0 + e / (KC / 8) : N - 1;
+    bcol[q] = (int64_t)gn * K + (e % (KC / 8)) * 8;
+  }
+  const int nch = K / KC;
   float4 ra[PA][A_PER];
-  uint4 rb[NP][B_PER];
-  auto load_a = [&](int c, float4(&xa)[A_PER]) {
+  u32x4_t rb[PA][NP][B_PER];
+  auto load = [&](int c, float4(&xa)[A_PER], u32x4_t(&xb)[NP][B_PER]) __attribute__((always_inline)) {
+    const int k0 = (c < nch ? c : nch - 1) * KC;
 #pragma unroll
-    for (int q = 0; q < A_PER; ++q) {
-      const int e = tid + q * NT;
-      const int rr = e / (KC / 4), kq = (e % (KC / 4)) * 4;
-      const int64_t gr = m0 + rr;
-      xa[q] = (e < A_UNITS && gr < M) ? *reinterpret_cast<const float4*>(A + gr * lda + c * KC + kq)
-                                      : make_float4(0.f, 0.f, 0.f, 0.f);
-    }
+    for (int q = 0; q < A_PER; ++q) xa[q] = *reinterpret_cast<const float4*>(arow[q] + k0);
+#pragma unroll
+    for (int q = 0; q < B_PER; ++q)
+#pragma unroll
+      for (int p = 0; p < NP; ++p) xb[p][q] = *reinterpret_cast<const u32x4_t*>(Wpl[p] + bcol[q] + k0);
   };
-  auto load_b = [&](int c) {
-#pragma unroll
-    for (int q = 0; q < B_PER; ++q) {
-      const int e = tid + q * NT;
-      const int nn = e / (KC / 8), kq = (e % (KC / 8)) * 8;
-      const int gn = n0 + nn;
-#pragma unroll
-      for (int p = 0; p < NP; ++p)
-        rb[p][q] = (e < B_UNITS && gn < N) ? *reinterpret_cast<const uint4*>(Wpl[p] + (int64_t)gn * K + c * KC + kq)
-                                           : make_uint4(0, 0, 0, 0);
-    }
-  };
-  auto store = [&](int st, const float4(&xa)[A_PER]) {
+  auto store = [&](int st, const float4(&xa)[A_PER], const u32x4_t(&xb)[NP][B_PER]) __attribute__((always_inline)) {
 #pragma unroll
     for (int q = 0; q < A_PER; ++q) {
       const int e = tid + q * NT;
@@ -1129,7 +1137,7 @@ __global__ void __launch_bounds__(BM * BN / 128) __attribute__((amdgpu_waves_per
       if (B_UNITS % NT != 0 && e >= B_UNITS) continue;
       const int nn = e / (KC / 8), kq = (e % (KC / 8)) * 8;
 #pragma unroll
-      for (int p = 0; p < NP; ++p) *reinterpret_cast<uint4*>(Bs(st, p, nn) + kq) = rb[p][q];
+      for (int p = 0; p < NP; ++p) *reinterpret_cast<u32x4_t*>(Bs(st, p, nn) + kq) = xb[p][q];
     }
   };
   const int nbl = live_col_tiles(n0 + wn, N, 4);
@@ -1167,30 +1175,21 @@ __global__ void __launch_bounds__(BM * BN / 128) __attribute__((amdgpu_waves_per
     else if (nbl == 1)
       compute_nb(st, IC<1>());
   };
-  const int nch = K / KC;
-  // chunk 0 into stage 0; chunks 1 .. PA in the ring (chunk j in ra[j % PA]); W chunk 1 in rb
-  load_a(0, ra[0]);
-  load_b(0);
-  store(0, ra[0]);
-#pragma unroll
-  for (int j = 1; j <= PA; ++j)
-    if (j < nch) load_a(j, ra[j % PA]);
-  if (nch > 1) load_b(1);
+  // chunk j lives in ring slot j % PA: chunks 0 .. PA - 1 first, chunk 0 split into stage 0,
+  // chunk PA into its slot
+  static_for<0, PA>([&](auto J) __attribute__((always_inline)) { load(J.value, ra[J.value], rb[J.value]); });
+  store(0, ra[0], rb[0]);
+  load(PA, ra[0], rb[0]);
   __syncthreads();
   for (int c = 0; c < nch; c += PA) {
-#pragma unroll
-    for (int q = 0; q < PA; ++q) {
-      const int cc = c + q;  // chunk in stage cc & 1; chunk cc + 1 waits in ra[(q + 1) % PA]
-      if (cc < nch) {
-        compute(cc & 1);
-        if (cc + 1 < nch) {
-          store((cc + 1) & 1, ra[(q + 1) % PA]);
-          if (cc + 1 + PA < nch) load_a(cc + 1 + PA, ra[(q + 1) % PA]);
-          if (cc + 2 < nch) load_b(cc + 2);
-        }
-        __syncthreads();
-      }
-    }
+    static_for<0, PA>([&](auto Q) __attribute__((always_inline)) {
+      constexpr int slot = (decltype(Q)::value + 1) % PA;
+      const int cc = c + decltype(Q)::value;  // chunk cc in stage cc & 1; chunk cc + 1 in `slot`
+      if (cc < nch) compute(cc & 1);
+      if (cc + 1 < nch) store((cc + 1) & 1, ra[slot], rb[slot]);
+      load(cc + 1 + PA, ra[slot], rb[slot]);
+      __syncthreads();
+    });
   }
   store_rows<4, FUSE, F16>(acc, m0, n0, wm, wn, lane, C, ldc, M, N, bias, rowflag, period, ep);
 }
@@ -1492,12 +1491,12 @@ static void launch_np(hipStream_t s, int variant, const float* A, int64_t lda, c
   if constexpr (F16) {  // the forms with FP16 planes: single stage, deep A, wide wavefront
     switch (variant) {
       case 18: launch_tile_dq<NP, 128, 256, 16, 4, false, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
-      case 19: launch_tile_dq<NP, 128, 256, 16, 6, false, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 19: launch_tile_dq<NP, 128, 256, 16, 2, false, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
       case 20: launch_tile_dq<NP, 256, 128, 16, 4, false, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
-      case 21: launch_tile_dq<NP, 128, 256, 32, 3, false, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
-      case 22: launch_tile_dq<NP, 128, 128, 16, 6, false, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 21: launch_tile_dq<NP, 128, 256, 32, 2, false, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 22: launch_tile_dq<NP, 128, 128, 16, 3, false, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
       case 23: launch_tile_dw<NP, 256, 256, 16, 4, false, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
-      case 24: launch_tile_dw<NP, 128, 256, 16, 4, false, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 24: launch_tile_dw<NP, 128, 256, 16, 3, false, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
       case 25: launch_tile_dw<NP, 256, 128, 16, 2, false, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
       case 1: launch_tile<NP, 128, 256, false, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
       case 2: launch_tile<NP, 256, 128, false, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
@@ -1518,12 +1517,12 @@ static void launch_np(hipStream_t s, int variant, const float* A, int64_t lda, c
       case 16: launch_tile_ar<NP, 4, 1, 128>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
       case 17: launch_tile_ar<NP, 1, 4, 64>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
       case 18: launch_tile_dq<NP, 128, 256, 16, 4>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
-      case 19: launch_tile_dq<NP, 128, 256, 16, 6>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 19: launch_tile_dq<NP, 128, 256, 16, 2>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
       case 20: launch_tile_dq<NP, 256, 128, 16, 4>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
-      case 21: launch_tile_dq<NP, 128, 256, 32, 3>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
-      case 22: launch_tile_dq<NP, 128, 128, 16, 6>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 21: launch_tile_dq<NP, 128, 256, 32, 2>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 22: launch_tile_dq<NP, 128, 128, 16, 3>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
       case 23: launch_tile_dw<NP, 256, 256, 16, 4>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
-      case 24: launch_tile_dw<NP, 128, 256, 16, 4>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 24: launch_tile_dw<NP, 128, 256, 16, 3>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
       case 25: launch_tile_dw<NP, 256, 128, 16, 2>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
       case 1: launch_tile<NP, 128, 256>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
       case 2: launch_tile<NP, 256, 128>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
