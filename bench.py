@@ -193,9 +193,10 @@ def main():
     flops_w, flop_terms, sizes = algorithmic_flops(cfg, mean_counts)
 
     def _prof(enable):
-        ms, fl, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_int64()
-        _lib.check(wf._lib.fire_profile_gemm(wf._h, int(enable), ctypes.byref(ms), ctypes.byref(fl), ctypes.byref(n)))
-        return ms.value, fl.value, n.value
+        ms, fl, by, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_double(), ctypes.c_int64()
+        _lib.check(wf._lib.fire_profile_gemm(wf._h, int(enable), ctypes.byref(ms), ctypes.byref(fl), ctypes.byref(by),
+                                             ctypes.byref(n)))
+        return ms.value, fl.value, by.value, n.value
 
     # 2. timed local-energy steps
     def run(prec, steps, warmup, profile=False):
@@ -261,26 +262,37 @@ def main():
         del g_rows, lp_g
         torch.cuda.empty_cache()
 
-    g_ms, g_flops, g_n = prof_m
+    g_ms, g_flops, g_bytes, g_n = prof_m
     g_achieved = (g_flops / (g_ms * 1e-3)) / 1e12 if g_ms > 0 else None
+    g_gbs = (g_bytes / (g_ms * 1e-3)) / 1e9 if g_ms > 0 else None
     traffic = measured_traffic() if args.workload == "c16h34" and B == 1024 else None
+    # the contractions' algorithmic intensity (2MNK flops over A, C and the weights moved once)
+    # is far below MI355X's ridge point (2.5 PFLOP/s over 8 TB/s = 312 flop/byte), so their
+    # roofline ceiling is HBM; the matrix-core view is kept beside it
     roofline = {
-        "bound": "mfma",
+        "bound": "hbm",
         "kernel": "k_gemm_bf16* (every dense contraction of one mixed step: ElecInit-MLP, update-MLP and "
                   "orbital contractions on two FP16 planes of power-of-two-scaled operands, Jastrow and "
                   "attention on a BF16 hi + lo pair, three products each; autotuned tile shapes; the "
                   "update-MLP and first Jastrow layers with the forward-Laplacian silu in their epilogue)",
-        "achieved": g_achieved,
-        "peak": PEAK["bf16"],
-        "unit": "TFLOP/s",
-        "frac": (g_achieved / PEAK["bf16"]) if g_achieved else None,
+        "achieved": g_gbs,
+        "peak": PEAK["hbm_gbs"],
+        "unit": "GB/s",
+        "frac": (g_gbs / PEAK["hbm_gbs"]) if g_gbs else None,
         "traffic": traffic,
+        "algorithmic_bytes_per_launch": g_bytes / g_n if g_n else None,
+        "intensity_flop_per_byte": g_flops / g_bytes if g_bytes else None,
+        "ridge_flop_per_byte": PEAK["bf16"] * 1e12 / (PEAK["hbm_gbs"] * 1e9),
+        "mfma_view": {"achieved": g_achieved, "peak": PEAK["bf16"], "unit": "TFLOP/s",
+                      "frac": (g_achieved / PEAK["bf16"]) if g_achieved else None,
+                      "what": "algorithmic 2*M*N*K per launch; the matrix cores execute three 16-bit products "
+                              "per multiply-add on top of these"},
         "traffic_source": TRAFFIC_PROFILE if traffic else None,
         "avg_launch_ms": g_ms / g_n if g_n else None,
         "launches_per_step": g_n,
         "algorithmic_flop_per_step": g_flops,
-        "note": "achieved = algorithmic 2*M*N*K per launch over the HIP-event duration of the launches on "
-                "their stream; the matrix cores execute 3x these flops (three products per multiply-add); "
+        "note": "achieved = algorithmic bytes per launch (A read once, C written once, the weight planes once: "
+                "4*M*(K+N) + planes*2*N*K) over the HIP-event duration of the launches on their stream; "
                 "traffic = HBM bytes per launch from two rocprofv3 --pmc runs "
                 "(FETCH_SIZE x2, WRITE_SIZE x1 by a 1 GiB copy calibration) of this workload, read from the "
                 "committed profile summary; whole-step rate in whole_step_tflops",

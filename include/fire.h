@@ -226,8 +226,10 @@ int fire_autotune(fire_handle* h, int32_t enable, int32_t* table, int32_t cap, i
 /* Instrumentation (no counterpart in the reference): enables / disables HIP-event timing of
  * every dense-contraction launch of this handle.  Each call first reports what was recorded
  * since the previous call -- total kernel milliseconds, algorithmic FLOPs (2*M*N*K per
- * launch) and the launch count -- then resets and applies `enable`.  Synchronises. */
-int fire_profile_gemm(fire_handle* h, int32_t enable, double* total_ms, double* total_flops, int64_t* n_launches);
+ * launch), algorithmic bytes (4*M*(K + N) for A read and C written once, plus the weight
+ * planes once) and the launch count -- then resets and applies `enable`.  Synchronises. */
+int fire_profile_gemm(fire_handle* h, int32_t enable, double* total_ms, double* total_flops, double* total_bytes,
+                      int64_t* n_launches);
 
 /* ---- pseudopotentials ----------------------------------------------------------------------
  * Effective-core potentials of n_ecp nuclei (atom[e] indexes the molecule's nuclei), tabulated

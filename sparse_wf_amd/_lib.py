@@ -46,7 +46,8 @@ SIGNATURES = {
     "fire_signed": (ctypes.c_int, [_P, _P, _P, ctypes.c_int32, ctypes.c_int32, _P, _P]),
     "fire_embedding": (ctypes.c_int, [_P, _P, _P, ctypes.c_int32, ctypes.c_int32, _P, _P]),
     "fire_profile_gemm": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.POINTER(ctypes.c_double),
-                                         ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]),
+                                         ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                         ctypes.POINTER(ctypes.c_int64)]),
     "fire_state_init": (ctypes.c_int, [_P, _P, _P, ctypes.c_int32, ctypes.c_int32, _P, _P]),
     "fire_state_propose": (ctypes.c_int, [_P, _P, ctypes.c_int32, _P, _P, _P]),
     "fire_state_commit": (ctypes.c_int, [_P, _P, _P]),

@@ -225,6 +225,7 @@ struct fire_handle {
   bool gemm_timing = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> gemm_events;
   double gemm_flops = 0.0;
+  double gemm_bytes = 0.0;  // algorithmic: A read once, C written once, the weight planes once
   // Autotuned tile shape of the BF16 contraction per (split, N, K, floor(log2 M)): the first
   // call of a shape times every candidate on the real operands and keeps the fastest.
   bool autotune = true;
@@ -542,6 +543,8 @@ void gemm(Ctx& c, const float* A, int64_t lda, int wi, int bi, float* C, int64_t
     HIPCHK(hipEventRecord(e1, c.s));
     h->gemm_events.push_back({e0, e1});
     h->gemm_flops += 2.0 * (double)M * N * K;
+    const double wbytes = use_bf ? 2.0 * (ns == 6 ? 3 : (ns == 1 ? 1 : 2)) : 4.0;
+    h->gemm_bytes += 4.0 * (double)M * (K + N) + wbytes * N * K;
   }
 }
 
@@ -2133,7 +2136,8 @@ int fire_upload_params(fire_handle* h, const float* packed, const int64_t* sizes
   }
 }
 
-int fire_profile_gemm(fire_handle* h, int32_t enable, double* total_ms, double* total_flops, int64_t* n_launches) {
+int fire_profile_gemm(fire_handle* h, int32_t enable, double* total_ms, double* total_flops, double* total_bytes,
+                      int64_t* n_launches) {
   try {
     REQUIRE(h, "null handle");
     HIPCHK(hipSetDevice(h->device));
@@ -2148,9 +2152,11 @@ int fire_profile_gemm(fire_handle* h, int32_t enable, double* total_ms, double* 
     }
     if (total_ms) *total_ms = ms;
     if (total_flops) *total_flops = h->gemm_flops;
+    if (total_bytes) *total_bytes = h->gemm_bytes;
     if (n_launches) *n_launches = (int64_t)h->gemm_events.size();
     h->gemm_events.clear();
     h->gemm_flops = 0.0;
+    h->gemm_bytes = 0.0;
     h->gemm_timing = enable != 0;
     return 0;
   } catch (const std::exception& e) {

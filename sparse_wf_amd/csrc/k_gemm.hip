@@ -191,7 +191,8 @@ __device__ __forceinline__ void tile_of(int64_t& m_tile, int& n_tile, int nT) {
 // (+ bias on value rows) or through the fused forward-Laplacian silu described at ActEpi.
 // With F16 every accumulator is first unscaled by 1 / (s_r s_n): lane t of the 64-row block
 // loads the row maximum of row t once and the rows' factors travel by shuffle, so no row
-// address is held across the block.
+// address is held across the block.  C goes out with streaming (nontemporal) stores: its rows
+// are far larger than the caches and a later kernel reads them once.
 template <int NB, bool FUSE, bool F16 = false, typename AccT>
 __device__ __forceinline__ void store_rows(const AccT (&acc)[2][NB], int64_t m0, int n0, int wm, int wn, int lane,
                                            float* __restrict__ C, int64_t ldc, int64_t M, int N,
@@ -226,7 +227,7 @@ __device__ __forceinline__ void store_rows(const AccT (&acc)[2][NB], int64_t m0,
           if (col >= N) continue;
           float val = acc[a][b][v] * (rs * cs[b]);
           if (br) val += bcol[b];
-          crow[col] = val;
+          __builtin_nontemporal_store(val, crow + col);
         }
       }
   } else if constexpr (!FUSE) {
@@ -243,7 +244,7 @@ __device__ __forceinline__ void store_rows(const AccT (&acc)[2][NB], int64_t m0,
           if (row >= M) continue;
           float val = acc[a][b][v];
           if (bias && is_bias_row(row, rowflag, period)) val += bcol;
-          C[row * ldc + col] = val;
+          __builtin_nontemporal_store(val, C + row * ldc + col);
         }
     }
   } else {
@@ -316,7 +317,7 @@ __device__ __forceinline__ void store_rows(const AccT (&acc)[2][NB], int64_t m0,
             z += bcol[b];
             z *= 1.0f / (1.0f + expf(-z));
           }
-          crow[col[b]] = z;
+          __builtin_nontemporal_store(z, crow + col[b]);
           // row maxima for the next contraction (Laplacian rows are completed, and their
           // maxima taken, by k_act_lapfix)
           if (ep.omax && cd != -2 && col[b] < ep.omax_cols) mx[a][v] = fmaxf(mx[a][v], fabsf(z));
