@@ -27,6 +27,16 @@ struct FilterParams {  // PairwiseFilter (model/utils.py:393-433) with precomput
   int w0, w1, E;
 };
 
+// Store of a tangent-row output that is far larger than the caches and read once by a later
+// kernel: streaming (nontemporal) when NT, plain otherwise (value rows, sampler lists).
+template <bool NT>
+__device__ __forceinline__ void store_rows_out(float* p, float v) {
+  if constexpr (NT)
+    __builtin_nontemporal_store(v, p);
+  else
+    *p = v;
+}
+
 // Philox4x32-10 counter-based generator (Salmon, Moraes, Dror and Shaw, SC'11): 10 rounds
 // on a 4 x 32-bit counter under a 2 x 32-bit key.
 __device__ __forceinline__ uint4 philox4x32_10(uint4 c, uint2 k) {

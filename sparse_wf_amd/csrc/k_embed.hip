@@ -673,7 +673,7 @@ __global__ void __launch_bounds__(256) k_messages(const float* __restrict__ r, i
             const float vc = s1 * uc[t], vn = s1 * un[t];
             acc[2 + t] += G5[2 + t] * sv + G5[0] * vc;
             const float o = -G5[2 + t] * sv + G5[0] * vn;
-            orow[t * F] = o;
+            store_rows_out<TAN>(orow + t * F, o);
             gmx = fmaxf(gmx, fabsf(o));
             cross += G5[2 + t] * (vc - vn);
           }
@@ -685,7 +685,7 @@ __global__ void __launch_bounds__(256) k_messages(const float* __restrict__ r, i
   }
   float* o = out + row0 * F + f;
   for (int rr = 0; rr < NR; ++rr) {
-    o[rr * F] = acc[rr];
+    store_rows_out<TAN>(o + rr * F, acc[rr]);
     gmx = fmaxf(gmx, fabsf(acc[rr]));
   }
   if (TAN && omax) {
