@@ -176,14 +176,18 @@ struct fire_handle {
   // parameters
   std::vector<int64_t> expect;  // element counts in canonical order
   std::vector<float*> P;
-  std::vector<uint16_t*> Pbf;   // BF16 transposed copies of GEMM weights (null if none): hi part
+  // 16-bit planes of the GEMM weights (null if none) in the K-blocked layout the split
+  // contractions read: element (k, n) at ((k / 16) N + n) 16 + k % 16, so the 16-wide K chunk
+  // of every column is one contiguous run
+  std::vector<uint16_t*> Pbf;   // BF16 planes: hi part
   std::vector<uint16_t*> Pbf_lo;  // ... the BF16 remainder W - hi (mid plane)
   std::vector<uint16_t*> Pbf_lo2;  // ... and the BF16 remainder W - hi - mid (lo plane)
   std::vector<float*> Pt32;       // FP32 transposed copies [N,K] of GEMM weights
-  // FP16 planes [N,K] of every GEMM weight with each column scaled by a power of two (its
-  // maximum into [2^14, 2^15)), and the inverse scales [N]
+  // FP16 planes (same layout) of every GEMM weight with each column scaled by a power of two
+  // (its maximum into [2^14, 2^15)), and the inverse scales [N]
   std::vector<uint16_t*> Pf16, Pf16_lo;
   std::vector<float*> Pwinv;
+  std::vector<int> Pwn;  // columns N of each GEMM weight with planes (the planes' row length)
   std::vector<std::pair<int, int>> shape2;  // (K, N) of GEMM weights
   float cut_learned = 0.f;
   float s1 = 1.f;
@@ -432,6 +436,8 @@ void gemm(Ctx& c, const float* A, int64_t lda, int wi, int bi, float* C, int64_t
   const float* bias = bi >= 0 ? h->P[bi] : nullptr;
   const bool bf_ok = h->Pbf[wi] != nullptr && (K % 32) == 0 && (lda % 4) == 0 && N > 16;
   const bool use_bf = (c.prec == FIRE_MIXED || c.prec == FIRE_MIXED_FAST || c.prec == FIRE_BF16) && bf_ok;
+  REQUIRE(!use_bf || N == h->Pwn[wi], "contraction of weight " + std::to_string(wi) + " over " + std::to_string(N) +
+                                          " of its " + std::to_string(h->Pwn[wi]) + " columns");
   int ns = c.prec == FIRE_MIXED_FAST ? 3 : 1;
   if (c.prec == FIRE_MIXED) {
     int np = h->planes[gemm_class(h, wi)];
@@ -1992,6 +1998,7 @@ int fire_upload_params(fire_handle* h, const float* packed, const int64_t* sizes
     h->Pf16.assign(n_all, nullptr);
     h->Pf16_lo.assign(n_all, nullptr);
     h->Pwinv.assign(n_all, nullptr);
+    h->Pwn.assign(n_all, 0);
     // host copies of every tensor (uploaded, then derived)
     std::vector<const float*> host(n_all, nullptr);
     std::vector<int64_t> hsize(n_all, 0);
@@ -2059,9 +2066,12 @@ int fire_upload_params(fire_handle* h, const float* packed, const int64_t* sizes
     gemm_weights(h, gw);
     for (auto& t : gw) {
       const int k = t.first, K = t.second.first, Nn = t.second.second;
-      // three BF16 planes, transposed to [N,K]: x = hi + mid + lo (remainders exact in FP32)
+      const size_t Kp = ((size_t)K + 15) / 16 * 16;  // K padded to whole 16-wide chunks (zeros)
+      auto at = [Nn](int kk, int n) { return ((size_t)(kk >> 4) * Nn + n) * 16 + (kk & 15); };
+      h->Pwn[k] = Nn;
+      // three BF16 planes, K-blocked: x = hi + mid + lo (remainders exact in FP32)
       std::vector<uint16_t> pl[3];
-      for (auto& v : pl) v.resize((size_t)K * Nn);
+      for (auto& v : pl) v.assign(Kp * Nn, 0);
       const float* src = host[k];
       auto bf2f_h = [](uint16_t b) {
         uint32_t u = (uint32_t)b << 16;
@@ -2075,17 +2085,17 @@ int fire_upload_params(fire_handle* h, const float* packed, const int64_t* sizes
           const uint16_t hi = host_f2bf(x);
           const float r1 = x - bf2f_h(hi);
           const uint16_t mid = host_f2bf(r1);
-          pl[0][(size_t)n * K + kk] = hi;
-          pl[1][(size_t)n * K + kk] = mid;
-          pl[2][(size_t)n * K + kk] = host_f2bf(r1 - bf2f_h(mid));
+          pl[0][at(kk, n)] = hi;
+          pl[1][at(kk, n)] = mid;
+          pl[2][at(kk, n)] = host_f2bf(r1 - bf2f_h(mid));
         }
       uint16_t** dst[3] = {&h->Pbf[k], &h->Pbf_lo[k], &h->Pbf_lo2[k]};
       for (int q = 0; q < 3; ++q) {
         HIPCHK(hipMalloc(dst[q], sizeof(uint16_t) * pl[q].size()));
         HIPCHK(hipMemcpy(*dst[q], pl[q].data(), sizeof(uint16_t) * pl[q].size(), hipMemcpyHostToDevice));
       }
-      // two FP16 planes of the column-scaled weight, transposed to [N,K]: s_n W = hi + lo
-      std::vector<uint16_t> fh((size_t)K * Nn), fl((size_t)K * Nn);
+      // two FP16 planes of the column-scaled weight, K-blocked: s_n W = hi + lo
+      std::vector<uint16_t> fh(Kp * Nn, 0), fl(Kp * Nn, 0);
       std::vector<float> winv(Nn);
       for (int n = 0; n < Nn; ++n) {
         float mx = 0.f;
@@ -2095,8 +2105,8 @@ int fire_upload_params(fire_handle* h, const float* packed, const int64_t* sizes
         for (int kk = 0; kk < K; ++kk) {
           const float x = src[(size_t)kk * Nn + n] * sc;
           const uint16_t hi = host_f2h(x);
-          fh[(size_t)n * K + kk] = hi;
-          fl[(size_t)n * K + kk] = host_f2h(x - host_h2f(hi));
+          fh[at(kk, n)] = hi;
+          fl[at(kk, n)] = host_f2h(x - host_h2f(hi));
         }
       }
       HIPCHK(hipMalloc(&h->Pf16[k], sizeof(uint16_t) * fh.size()));

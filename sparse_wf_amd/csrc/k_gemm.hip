@@ -168,6 +168,12 @@ __device__ __forceinline__ int live_col_tiles(int c0, int N, int nb) {
   return n < 0 ? 0 : (n > nb ? nb : n);
 }
 
+// Offset of weight element (k, n) in a K-blocked plane of N columns (the 16-wide K chunk of a
+// column is contiguous, consecutive columns follow each other): ((k / 16) N + n) 16 + k % 16.
+__device__ __forceinline__ int64_t wpl_off(int64_t n, int k, int N) {
+  return ((int64_t)(k >> 4) * N + n) * 16 + (k & 15);
+}
+
 __device__ __forceinline__ bool is_bias_row(int64_t row, const uint8_t* rowflag, int period) {
   if (rowflag) return rowflag[row] != 0;
   if (period > 0) return (row % period) == 0;
@@ -502,7 +508,7 @@ __global__ void __launch_bounds__(BM * BN / 64) k_gemm_bf16(const float* __restr
       const int gn = n0 + nn;
 #pragma unroll
       for (int p = 0; p < NP; ++p)
-        rb[p][q] = gn < N ? *reinterpret_cast<const uint4*>(Wpl[p] + (int64_t)gn * K + k0 + kq) : make_uint4(0, 0, 0, 0);
+        rb[p][q] = gn < N ? *reinterpret_cast<const uint4*>(Wpl[p] + wpl_off(gn, k0 + kq, N)) : make_uint4(0, 0, 0, 0);
     }
   };
   auto store = [&](const float4(&src)[A_PER]) {
@@ -809,7 +815,7 @@ __global__ void __launch_bounds__(BM * BN / 64) k_gemm_bf16_db(const float* __re
       const int gn = n0 + nn;
 #pragma unroll
       for (int p = 0; p < NP; ++p)
-        xb[p][q] = (e < B_UNITS && gn < N) ? *reinterpret_cast<const uint4*>(Wpl[p] + (int64_t)gn * K + k0 + kq)
+        xb[p][q] = (e < B_UNITS && gn < N) ? *reinterpret_cast<const uint4*>(Wpl[p] + wpl_off(gn, k0 + kq, N))
                                            : make_uint4(0, 0, 0, 0);
     }
   };
@@ -969,7 +975,7 @@ __global__ void __launch_bounds__(BM * BN / 64) k_gemm_bf16_dq(const float* __re
   for (int q = 0; q < B_PER; ++q) {
     const int e = (tid + q * NT) % B_UNITS;
     const int gn = n0 + e / (KC / 8) < N ? n0 + e / (KC / 8) : N - 1;
-    bcol[q] = (int64_t)gn * K + (e % (KC / 8)) * 8;
+    bcol[q] = wpl_off(gn, (e % (KC / 8)) * 8, N);
   }
   const int nch = K / KC;
   float4 ra[PA][A_PER];
@@ -981,7 +987,7 @@ __global__ void __launch_bounds__(BM * BN / 64) k_gemm_bf16_dq(const float* __re
 #pragma unroll
     for (int q = 0; q < B_PER; ++q)
 #pragma unroll
-      for (int p = 0; p < NP; ++p) xb[p][q] = *reinterpret_cast<const u32x4_t*>(Wpl[p] + bcol[q] + k0);
+      for (int p = 0; p < NP; ++p) xb[p][q] = *reinterpret_cast<const u32x4_t*>(Wpl[p] + bcol[q] + (int64_t)k0 * N);
   };
   auto store = [&](int st, const float4(&xa)[A_PER], const u32x4_t(&xb)[NP][B_PER]) __attribute__((always_inline)) {
 #pragma unroll
@@ -1107,7 +1113,7 @@ __global__ void __launch_bounds__(BM * BN / 128) __attribute__((amdgpu_waves_per
   for (int q = 0; q < B_PER; ++q) {
     const int e = (tid + q * NT) % B_UNITS;
     const int gn = n0 + e / (KC / 8) < N ? n0 + e / (KC / 8) : N - 1;
-    bcol[q] = (int64_t)gn * K + (e % (KC / 8)) * 8;
+    bcol[q] = wpl_off(gn, (e % (KC / 8)) * 8, N);
   }
   const int nch = K / KC;
   float4 ra[PA][A_PER];
@@ -1119,7 +1125,7 @@ __global__ void __launch_bounds__(BM * BN / 128) __attribute__((amdgpu_waves_per
 #pragma unroll
     for (int q = 0; q < B_PER; ++q)
 #pragma unroll
-      for (int p = 0; p < NP; ++p) xb[p][q] = *reinterpret_cast<const u32x4_t*>(Wpl[p] + bcol[q] + k0);
+      for (int p = 0; p < NP; ++p) xb[p][q] = *reinterpret_cast<const u32x4_t*>(Wpl[p] + bcol[q] + (int64_t)k0 * N);
   };
   auto store = [&](int st, const float4(&xa)[A_PER], const u32x4_t(&xb)[NP][B_PER]) __attribute__((always_inline)) {
 #pragma unroll
@@ -1261,7 +1267,7 @@ __global__ void __launch_bounds__(NWM * NWN * 64) __attribute__((amdgpu_waves_pe
       const int gn = n0 + nn;
 #pragma unroll
       for (int p = 0; p < NP; ++p)
-        rb[p][q] = (e < B_UNITS && gn < N) ? *reinterpret_cast<const uint4*>(Wpl[p] + (int64_t)gn * K + c * KC + kq)
+        rb[p][q] = (e < B_UNITS && gn < N) ? *reinterpret_cast<const uint4*>(Wpl[p] + wpl_off(gn, c * KC + kq, N))
                                            : make_uint4(0, 0, 0, 0);
     }
   };
@@ -1346,24 +1352,24 @@ struct Bf16Tile {
 };
 // variants 0-6: single LDS stage, K chunk 32; variants 7-10: two LDS stages (kc = 16 or 32);
 // variants 11-14: two LDS stages with loads two chunks ahead; variants 15-17: A in registers;
-// variants 18-22: two LDS stages with A PA chunks ahead; variants 23-25: the same with 64 x 128
-// wavefront tiles
+// variants 18-22 and 26: two LDS stages with the loads PA chunks ahead; variants 23-25: the same
+// with 64 x 128 wavefront tiles
 static const Bf16Tile kBf16Tiles[] = {{128, 128}, {128, 256}, {256, 128}, {64, 256}, {256, 64},  {64, 128},
                                       {128, 64},  {128, 128}, {128, 128}, {256, 128}, {128, 256}, {256, 128},
                                       {128, 256}, {128, 128}, {128, 128}, {128, 256}, {256, 128}, {64, 256},
                                       {128, 256}, {128, 256}, {256, 128}, {128, 256}, {128, 128}, {256, 256},
-                                      {128, 256}, {256, 128}};
+                                      {128, 256}, {256, 256}, {128, 256}};
 
 // forms that carry FP16 planes (the others are offered to BF16 planes only)
 bool gemm_bf16_f16_ok(int variant) { return variant < 3 || variant > 17; }
 
 // forms whose registers spill to scratch for this plane count (hipcc resource report): the
-// 256 x 128 wide-wavefront tile (four float4 A loads per thread and chunk) at every plane
-// count, the A-in-registers and 128 x 256 wide-wavefront forms with three planes.  They are
-// not offered: a spilling kernel is slower, and every stream running it reserves scratch.
+// 2 x 2 A-in-registers form with two or three planes; with three planes also the 4 x 1
+// A-in-registers form and every wide-wavefront form.  They are not offered: a spilling kernel
+// is slower, and every stream running it reserves scratch.
 bool gemm_bf16_spills(int variant, int nsplit) {
-  if (variant == 25) return true;
-  return nsplit == 6 && (variant == 15 || variant == 24);
+  if (variant == 15) return nsplit != 1;
+  return nsplit == 6 && (variant == 16 || variant == 23 || variant == 24 || variant == 25);
 }
 
 bool gemm_bf16_fusable(int variant) { return variant == 0 || variant == 2 || variant == 18 || variant == 20 || variant == 23; }
@@ -1498,7 +1504,8 @@ static void launch_np(hipStream_t s, int variant, const float* A, int64_t lda, c
       case 22: launch_tile_dq<NP, 128, 128, 16, 3, false, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
       case 23: launch_tile_dw<NP, 256, 256, 16, 4, false, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
       case 24: launch_tile_dw<NP, 128, 256, 16, 3, false, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
-      case 25: launch_tile_dw<NP, 256, 128, 16, 2, false, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 25: launch_tile_dw<NP, 256, 256, 16, 2, false, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 26: launch_tile_dq<NP, 128, 256, 16, 1, false, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
       case 1: launch_tile<NP, 128, 256, false, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
       case 2: launch_tile<NP, 256, 128, false, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
       default: launch_tile<NP, 128, 128, false, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
@@ -1524,7 +1531,8 @@ static void launch_np(hipStream_t s, int variant, const float* A, int64_t lda, c
       case 22: launch_tile_dq<NP, 128, 128, 16, 3>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
       case 23: launch_tile_dw<NP, 256, 256, 16, 4>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
       case 24: launch_tile_dw<NP, 128, 256, 16, 3>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
-      case 25: launch_tile_dw<NP, 256, 128, 16, 2>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 25: launch_tile_dw<NP, 256, 256, 16, 2>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 26: launch_tile_dq<NP, 128, 256, 16, 1>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
       case 1: launch_tile<NP, 128, 256>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
       case 2: launch_tile<NP, 256, 128>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
       case 3: launch_tile<NP, 64, 256>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
