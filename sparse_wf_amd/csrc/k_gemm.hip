@@ -1372,7 +1372,9 @@ bool gemm_bf16_spills(int variant, int nsplit) {
   return nsplit == 6 && (variant == 16 || variant == 23 || variant == 24 || variant == 25);
 }
 
-bool gemm_bf16_fusable(int variant) { return variant == 0 || variant == 2 || variant == 18 || variant == 20 || variant == 23; }
+bool gemm_bf16_fusable(int variant) {
+  return variant == 0 || variant == 2 || variant == 18 || variant == 20 || variant == 23 || variant == 25 || variant == 26;
+}
 
 int gemm_bf16_variants() { return (int)(sizeof(kBf16Tiles) / sizeof(kBf16Tiles[0])); }
 void gemm_bf16_tile(int variant, int* bm, int* bn) {
@@ -1491,6 +1493,8 @@ static void launch_np(hipStream_t s, int variant, const float* A, int64_t lda, c
       case 18: launch_tile_dq<NP, 128, 256, 16, 4, true, F16>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
       case 20: launch_tile_dq<NP, 256, 128, 16, 4, true, F16>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
       case 23: launch_tile_dw<NP, 256, 256, 16, 4, true, F16>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 25: launch_tile_dw<NP, 256, 256, 16, 2, true, F16>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 26: launch_tile_dq<NP, 128, 256, 16, 1, true, F16>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
       default: launch_tile<NP, 128, 128, true, F16>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
     }
     return;
