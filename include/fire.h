@@ -217,11 +217,16 @@ int fire_det_options(fire_handle* h, int32_t fp64_laplacian);
  * number of 128-column tiles takes the nearest neighbour's choice without timing.  enable: 1 on,
  * 0 off (tile 128 x 128), 2 on and forget earlier choices (this device's), 3 on and also reuse
  * a neighbour up to two row buckets away (for ragged and serving batches, whose geometries each
- * bring new shapes), -1 unchanged.  table (optional, int32 [cap][7]) receives this handle's
+ * bring new shapes), 100 + v every contraction of this handle on tile form v (a diagnostic: where
+ * form v does not apply to a contraction, the base tile; modes 0-3 end it), -1 unchanged.  table (optional, int32 [cap][7]) receives this handle's
  * choices as (split, N, K, log2 M, variant, BM, BN); log2 M is raised by 32 for row counts held
  * on the device and by 64 for contractions with the fused activation epilogue;
  * *n_entries the number of choices. */
 int fire_autotune(fire_handle* h, int32_t enable, int32_t* table, int32_t cap, int32_t* n_entries);
+
+/* Number of tile forms of the split contractions: fire_autotune's mode 100 + v takes
+ * v = 0 .. fire_tile_forms() - 1 (no counterpart in the reference). */
+int fire_tile_forms(void);
 
 /* Instrumentation (no counterpart in the reference): enables / disables HIP-event timing of
  * every dense-contraction launch of this handle.  Each call first reports what was recorded

@@ -56,6 +56,7 @@ SIGNATURES = {
     "fire_state_stats": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int64),
                                         ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]),
     "fire_autotune": (ctypes.c_int, [_P, ctypes.c_int32, _P, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]),
+    "fire_tile_forms": (ctypes.c_int, []),
     "fire_precision_plan": (ctypes.c_int, [_P, _P, ctypes.c_int32, _P]),
     "fire_mcmc_options": (ctypes.c_int, [_P, ctypes.c_int32]),
     "fire_det_options": (ctypes.c_int, [_P, ctypes.c_int32]),

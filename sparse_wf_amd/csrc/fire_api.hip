@@ -233,7 +233,8 @@ struct fire_handle {
   // Autotuned tile shape of the BF16 contraction per (split, N, K, floor(log2 M)): the first
   // call of a shape times every candidate on the real operands and keeps the fastest.
   bool autotune = true;
-  bool autotune_near = false;  // reuse tuned tiles across neighbouring row buckets
+  bool autotune_near = false;
+  int force_tile = -1;  // >= 0: every contraction uses this tile form where it applies (fire_autotune)  // reuse tuned tiles across neighbouring row buckets
   // silu of the tangent-row MLP layers fused into the contraction epilogue (FIRE_FUSED_ACT=0:
   // the separate element-wise pass instead)
   bool fuse_act = true;
@@ -457,11 +458,18 @@ void gemm(Ctx& c, const float* A, int64_t lda, int wi, int bi, float* C, int64_t
     ep.amax = nullptr;
   }
   int variant = 0;
-  if (use_bf && h->autotune) {
-    int lg = 0;
-    while ((int64_t(1) << (lg + 1)) <= M) ++lg;
-    if (dr.count) lg += 32;  // rows counted on the device (M is the capacity): tuned apart
-    if (ep.rowgrp) lg += 64;  // fused silu epilogue: tuned apart
+  int lg = 0;
+  while ((int64_t(1) << (lg + 1)) <= M) ++lg;
+  if (dr.count) lg += 32;  // rows counted on the device (M is the capacity): tuned apart
+  if (ep.rowgrp) lg += 64;  // fused silu epilogue: tuned apart
+  if (use_bf && h->force_tile >= 0) {
+    // one tile form for every contraction; where that form does not apply, the base tile
+    variant = h->force_tile;
+    if (variant >= gemm_bf16_variants() || (ep.rowgrp && !gemm_bf16_fusable(variant)) ||
+        (ns == 4 && !gemm_bf16_f16_ok(variant)) || gemm_bf16_spills(variant, ns))
+      variant = 0;
+    h->gemm_tile[std::make_tuple(ns, N, K, lg)] = variant;  // reported by fire_autotune's table
+  } else if (use_bf && h->autotune) {
     const auto key = std::make_tuple(ns, N, K, lg);
     const auto gkey = std::make_tuple(h->device, ns, N, K, lg);
     auto it = h->gemm_tile.find(key);
@@ -2436,8 +2444,14 @@ int fire_autotune(fire_handle* h, int32_t enable, int32_t* table, int32_t cap, i
       ++n;
     }
     if (n_entries) *n_entries = n;
-    if (enable >= 0) h->autotune = enable != 0;
-    if (enable >= 0) h->autotune_near = enable == 3;
+    REQUIRE(enable <= 3 || (enable >= 100 && enable < 100 + gemm_bf16_variants()), "autotune mode out of range");
+    if (enable >= 100) {
+      h->force_tile = enable - 100;
+    } else if (enable >= 0) {
+      h->force_tile = -1;
+      h->autotune = enable != 0;
+      h->autotune_near = enable == 3;
+    }
     if (enable == 2) {
       h->gemm_tile.clear();
       h->gemm_tile_ms.clear();
@@ -2450,6 +2464,8 @@ int fire_autotune(fire_handle* h, int32_t enable, int32_t* table, int32_t cap, i
     return fail(e);
   }
 }
+
+int fire_tile_forms(void) { return gemm_bf16_variants(); }
 
 int fire_mcmc_options(fire_handle* h, int32_t use_graphs) {
   try {

@@ -16,6 +16,7 @@
 //               accumulates hi*hi + hi*lo + lo*hi ("split BF16 x3"); NSPLIT = 6 carries
 //               hi + mid + lo (x to ~2^-26) and accumulates the six products whose order is
 //               at most 2^-16 relative (x6, FP32-level products); NSPLIT = 1 is plain BF16.
+#include <algorithm>
 #include <type_traits>
 
 #include "fire_common.hpp"
@@ -1054,7 +1055,7 @@ __global__ void __launch_bounds__(BM * BN / 64) k_gemm_bf16_dq(const float* __re
 // tiles, 128 accumulators), so a K step reads 6 NP operand fragments from LDS for 8 tiles
 // instead of 4 NP for 4, and a 256 x 256 block stages each weight chunk once for 256 rows.
 // Both cut the LDS traffic per output, which bounds the 64 x 64 forms.
-template <int NP, int BM, int BN, int KC, int PA, bool FUSE = false, bool F16 = false>
+template <int NP, int BM, int BN, int KC, int PA, bool FUSE = false, bool F16 = false, bool PERSIST = false>
 __global__ void __launch_bounds__(BM * BN / 128) __attribute__((amdgpu_waves_per_eu(2))) k_gemm_bf16_dw(const float* __restrict__ A, int64_t lda,
                                                               const uint16_t* __restrict__ Wp0,
                                                               const uint16_t* __restrict__ Wp1,
@@ -1077,7 +1078,12 @@ __global__ void __launch_bounds__(BM * BN / 128) __attribute__((amdgpu_waves_per
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   int64_t mt;
   int nt;
-  tile_of(mt, nt, nT);
+  if constexpr (PERSIST) {
+    mt = blockIdx.x;
+    nt = 0;
+  } else {
+    tile_of(mt, nt, nT);
+  }
   if (dr.offset) {
     const int64_t o = *dr.offset;
     A += o * lda;
@@ -1087,34 +1093,34 @@ __global__ void __launch_bounds__(BM * BN / 128) __attribute__((amdgpu_waves_per
     if (ep.omax) ep.omax += o;
   }
   if (dr.count) M = M < (int64_t)*dr.count ? M : (int64_t)*dr.count;
-  if (mt * BM >= M) return;
-  const int64_t m0 = mt * BM;
-  const int n0 = nt * BN;
+  const int64_t MT = (M + BM - 1) / BM;  // row tiles
+  if (mt >= MT) return;
+  int64_t m0 = mt * BM;
+  int n0 = nt * BN;
   const int wm = (wid / NWN) * 64, wn = (wid % NWN) * 128;
   float16_t acc[2][4];
-  for (int a = 0; a < 2; ++a)
-    for (int b = 0; b < 4; ++b)
-      for (int v = 0; v < 16; ++v) acc[a][b][v] = 0.f;
   const uint16_t* Wpl[3] = {Wp0, Wp1, Wp2};
-  // FP16 planes: the scale of every A row this thread splits (the same rows in every chunk)
-  // the A rows and weight columns this thread loads (the same in every chunk), clamped into
-  // the operands; M >= 1 and N >= 1 here
+  // the A rows (with their FP16 scales) and weight columns this thread loads for the tile at
+  // (m0, n0), the same in every chunk, clamped into the operands; M >= 1 and N >= 1 here
   const float* arow[A_PER];
   float rsc[A_PER];
-#pragma unroll
-  for (int q = 0; q < A_PER; ++q) {
-    const int e = (tid + q * NT) % A_UNITS;
-    const int64_t gr = m0 + e / (KC / 4) < M ? m0 + e / (KC / 4) : M - 1;
-    arow[q] = A + gr * lda + (e % (KC / 4)) * 4;
-    rsc[q] = F16 ? f16_row_scale(ep.amax[gr]) : 1.f;
-  }
   int64_t bcol[B_PER];
+  auto setup = [&]() __attribute__((always_inline)) {
 #pragma unroll
-  for (int q = 0; q < B_PER; ++q) {
-    const int e = (tid + q * NT) % B_UNITS;
-    const int gn = n0 + e / (KC / 8) < N ? n0 + e / (KC / 8) : N - 1;
-    bcol[q] = wpl_off(gn, (e % (KC / 8)) * 8, N);
-  }
+    for (int q = 0; q < A_PER; ++q) {
+      const int e = (tid + q * NT) % A_UNITS;
+      const int64_t gr = m0 + e / (KC / 4) < M ? m0 + e / (KC / 4) : M - 1;
+      arow[q] = A + gr * lda + (e % (KC / 4)) * 4;
+      rsc[q] = F16 ? f16_row_scale(ep.amax[gr]) : 1.f;
+    }
+#pragma unroll
+    for (int q = 0; q < B_PER; ++q) {
+      const int e = (tid + q * NT) % B_UNITS;
+      const int gn = n0 + e / (KC / 8) < N ? n0 + e / (KC / 8) : N - 1;
+      bcol[q] = wpl_off(gn, (e % (KC / 8)) * 8, N);
+    }
+  };
+  setup();
   const int nch = K / KC;
   float4 ra[PA][A_PER];
   u32x4_t rb[PA][NP][B_PER];
@@ -1185,20 +1191,49 @@ __global__ void __launch_bounds__(BM * BN / 128) __attribute__((amdgpu_waves_per
   // chunk j lives in ring slot j % PA: chunks 0 .. PA - 1 first, chunk 0 split into stage 0,
   // chunk PA into its slot
   static_for<0, PA>([&](auto J) __attribute__((always_inline)) { load(J.value, ra[J.value], rb[J.value]); });
-  store(0, ra[0], rb[0]);
-  load(PA, ra[0], rb[0]);
-  __syncthreads();
-  for (int c = 0; c < nch; c += PA) {
-    static_for<0, PA>([&](auto Q) __attribute__((always_inline)) {
-      constexpr int slot = (decltype(Q)::value + 1) % PA;
-      const int cc = c + decltype(Q)::value;  // chunk cc in stage cc & 1; chunk cc + 1 in `slot`
-      if (cc < nch) compute(cc & 1);
-      if (cc + 1 < nch) store((cc + 1) & 1, ra[slot], rb[slot]);
-      load(cc + 1 + PA, ra[slot], rb[slot]);
-      __syncthreads();
-    });
+  // PERSIST: the block walks row tiles blockIdx.x, + gridDim.x, ..., all N tiles of each in
+  // turn (the A rows come back from the cache for the second); the first PA chunks of the
+  // next tile are loaded before the C stores of the current one, so those stores overlap loads
+  while (true) {
+    for (int a = 0; a < 2; ++a)
+      for (int b = 0; b < 4; ++b)
+        for (int v = 0; v < 16; ++v) acc[a][b][v] = 0.f;
+    store(0, ra[0], rb[0]);
+    load(PA, ra[0], rb[0]);
+    __syncthreads();
+    for (int c = 0; c < nch; c += PA) {
+      static_for<0, PA>([&](auto Q) __attribute__((always_inline)) {
+        constexpr int slot = (decltype(Q)::value + 1) % PA;
+        const int cc = c + decltype(Q)::value;  // chunk cc in stage cc & 1; chunk cc + 1 in `slot`
+        if (cc < nch) compute(cc & 1);
+        if (cc + 1 < nch) store((cc + 1) & 1, ra[slot], rb[slot]);
+        load(cc + 1 + PA, ra[slot], rb[slot]);
+        __syncthreads();
+      });
+    }
+    if constexpr (!PERSIST) {
+      store_rows<4, FUSE, F16>(acc, m0, n0, wm, wn, lane, C, ldc, M, N, bias, rowflag, period, ep);
+      return;
+    } else {
+      const int64_t m0c = m0;
+      const int n0c = n0;
+      if (nt + 1 < nT) {
+        ++nt;
+      } else {
+        nt = 0;
+        mt += gridDim.x;
+      }
+      const bool more = mt < MT;
+      if (more) {
+        m0 = mt * BM;
+        n0 = nt * BN;
+        setup();
+        static_for<0, PA>([&](auto J) __attribute__((always_inline)) { load(J.value, ra[J.value], rb[J.value]); });
+      }
+      store_rows<4, FUSE, F16>(acc, m0c, n0c, wm, wn, lane, C, ldc, M, N, bias, rowflag, period, ep);
+      if (!more) return;
+    }
   }
-  store_rows<4, FUSE, F16>(acc, m0, n0, wm, wn, lane, C, ldc, M, N, bias, rowflag, period, ep);
 }
 
 // A-in-registers form: each wavefront computes a 64 x WN tile; it loads its A fragments
@@ -1353,12 +1388,12 @@ struct Bf16Tile {
 // variants 0-6: single LDS stage, K chunk 32; variants 7-10: two LDS stages (kc = 16 or 32);
 // variants 11-14: two LDS stages with loads two chunks ahead; variants 15-17: A in registers;
 // variants 18-22 and 26: two LDS stages with the loads PA chunks ahead; variants 23-25: the same
-// with 64 x 128 wavefront tiles
+// with 64 x 128 wavefront tiles; 27-28: those with one persistent block per CU
 static const Bf16Tile kBf16Tiles[] = {{128, 128}, {128, 256}, {256, 128}, {64, 256}, {256, 64},  {64, 128},
                                       {128, 64},  {128, 128}, {128, 128}, {256, 128}, {128, 256}, {256, 128},
                                       {128, 256}, {128, 128}, {128, 128}, {128, 256}, {256, 128}, {64, 256},
                                       {128, 256}, {128, 256}, {256, 128}, {128, 256}, {128, 128}, {256, 256},
-                                      {128, 256}, {256, 256}, {128, 256}};
+                                      {128, 256}, {256, 256}, {128, 256}, {256, 256}, {256, 256}};
 
 // forms that carry FP16 planes (the others are offered to BF16 planes only)
 bool gemm_bf16_f16_ok(int variant) { return variant < 3 || variant > 17; }
@@ -1369,11 +1404,11 @@ bool gemm_bf16_f16_ok(int variant) { return variant < 3 || variant > 17; }
 // is slower, and every stream running it reserves scratch.
 bool gemm_bf16_spills(int variant, int nsplit) {
   if (variant == 15) return nsplit != 1;
-  return nsplit == 6 && (variant == 16 || variant == 23 || variant == 24 || variant == 25);
+  return nsplit == 6 && (variant == 16 || variant == 23 || variant == 24 || variant == 25 || variant >= 27);
 }
 
 bool gemm_bf16_fusable(int variant) {
-  return variant == 0 || variant == 2 || variant == 18 || variant == 20 || variant == 23 || variant == 25 || variant == 26;
+  return variant == 0 || variant == 2 || variant == 18 || variant == 20 || (variant >= 23 && variant != 24);
 }
 
 int gemm_bf16_variants() { return (int)(sizeof(kBf16Tiles) / sizeof(kBf16Tiles[0])); }
@@ -1451,7 +1486,22 @@ static void launch_tile_dq(hipStream_t s, const float* A, int64_t lda, const uin
                                                                                ldc, M, N, K, rowflag, period, nT, dr, ep);
 }
 
-template <int NP, int BM, int BN, int KC, int PA, bool FUSE = false, bool F16 = false>
+// compute units of the current device (cached per device)
+static int device_cus() {
+  static int cus[64] = {0};
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  if (dev < 0 || dev >= 64) dev = 0;
+  if (cus[dev] == 0) {
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+    cus[dev] = n;
+  }
+  return cus[dev];
+}
+
+// PERSIST: one block per CU (the form's LDS and registers admit one) walking the row tiles
+template <int NP, int BM, int BN, int KC, int PA, bool FUSE = false, bool F16 = false, bool PERSIST = false>
 static void launch_tile_dw(hipStream_t s, const float* A, int64_t lda, const uint16_t* const* W, const float* bias,
                            float* C, int64_t ldc, int64_t M, int N, int K, const uint8_t* rowflag, int period,
                            DevRows dr, ActEpi ep) {
@@ -1463,14 +1513,15 @@ static void launch_tile_dw(hipStream_t s, const float* A, int64_t lda, const uin
   }
   if (!attr_set) {
     if (lds > 64 * 1024)
-      (void)hipFuncSetAttribute((const void*)k_gemm_bf16_dw<NP, BM, BN, KC, PA, FUSE, F16>,
+      (void)hipFuncSetAttribute((const void*)k_gemm_bf16_dw<NP, BM, BN, KC, PA, FUSE, F16, PERSIST>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     attr_set = true;
   }
   const int nT = (N + BN - 1) / BN;
-  const int64_t blocks = ((M + BM - 1) / BM) * nT;
-  k_gemm_bf16_dw<NP, BM, BN, KC, PA, FUSE, F16><<<(unsigned)blocks, BM * BN / 128, lds, s>>>(A, lda, W[0], W[1], W[2], bias, C,
-                                                                                ldc, M, N, K, rowflag, period, nT, dr, ep);
+  const int64_t mtiles = (M + BM - 1) / BM;
+  const int64_t blocks = PERSIST ? std::min<int64_t>(mtiles, device_cus()) : mtiles * nT;
+  k_gemm_bf16_dw<NP, BM, BN, KC, PA, FUSE, F16, PERSIST><<<(unsigned)blocks, BM * BN / 128, lds, s>>>(
+      A, lda, W[0], W[1], W[2], bias, C, ldc, M, N, K, rowflag, period, nT, dr, ep);
 }
 
 template <int NP, int NWM, int NWN, int WN>
@@ -1495,6 +1546,8 @@ static void launch_np(hipStream_t s, int variant, const float* A, int64_t lda, c
       case 23: launch_tile_dw<NP, 256, 256, 16, 4, true, F16>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
       case 25: launch_tile_dw<NP, 256, 256, 16, 2, true, F16>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
       case 26: launch_tile_dq<NP, 128, 256, 16, 1, true, F16>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 27: launch_tile_dw<NP, 256, 256, 16, 4, true, F16, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 28: launch_tile_dw<NP, 256, 256, 16, 2, true, F16, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
       default: launch_tile<NP, 128, 128, true, F16>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
     }
     return;
@@ -1510,6 +1563,8 @@ static void launch_np(hipStream_t s, int variant, const float* A, int64_t lda, c
       case 24: launch_tile_dw<NP, 128, 256, 16, 3, false, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
       case 25: launch_tile_dw<NP, 256, 256, 16, 2, false, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
       case 26: launch_tile_dq<NP, 128, 256, 16, 1, false, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 27: launch_tile_dw<NP, 256, 256, 16, 4, false, true, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 28: launch_tile_dw<NP, 256, 256, 16, 2, false, true, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
       case 1: launch_tile<NP, 128, 256, false, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
       case 2: launch_tile<NP, 256, 128, false, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
       default: launch_tile<NP, 128, 128, false, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
@@ -1537,6 +1592,8 @@ static void launch_np(hipStream_t s, int variant, const float* A, int64_t lda, c
       case 24: launch_tile_dw<NP, 128, 256, 16, 3>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
       case 25: launch_tile_dw<NP, 256, 256, 16, 2>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
       case 26: launch_tile_dq<NP, 128, 256, 16, 1>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 27: launch_tile_dw<NP, 256, 256, 16, 4, false, false, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 28: launch_tile_dw<NP, 256, 256, 16, 2, false, false, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
       case 1: launch_tile<NP, 128, 256>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
       case 2: launch_tile<NP, 256, 128>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
       case 3: launch_tile<NP, 64, 256>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;

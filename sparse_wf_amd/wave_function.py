@@ -461,16 +461,34 @@ class FireWaveFunction:
             raise ValueError("no resident state (call log_psi_with_state first)")
         return nw.value
 
-    def autotune(self, mode: str = "on"):
+    def autotune(self, mode="on"):
         """Contraction tile autotuner of this process and device (``fire_autotune``): "on"
         (time each new shape class), "off", "reset" (on, earlier choices forgotten) or "near"
         (on, and shapes up to two row buckets from a tuned one reuse its choice: ragged and
-        serving batches, whose geometries each bring new shapes)."""
+        serving batches, whose geometries each bring new shapes).  An int v runs every
+        contraction of this handle on tile form v where that form applies (a diagnostic)."""
         codes = {"off": 0, "on": 1, "reset": 2, "near": 3}
-        if mode not in codes:
-            raise ValueError(f"autotune mode {mode!r} (one of {sorted(codes)})")
+        if isinstance(mode, int) and not isinstance(mode, bool):
+            code = 100 + mode
+        elif mode in codes:
+            code = codes[mode]
+        else:
+            raise ValueError(f"autotune mode {mode!r} (one of {sorted(codes)} or a tile form number)")
         n = ctypes.c_int32()
-        _lib.check(self._lib.fire_autotune(self._h, codes[mode], None, 0, ctypes.byref(n)))
+        _lib.check(self._lib.fire_autotune(self._h, code, None, 0, ctypes.byref(n)))
+
+    def tile_choices(self):
+        """This handle's tile form per contraction shape, as (split, N, K, log2 M code, form)."""
+        cap = 512
+        table = (ctypes.c_int32 * (7 * cap))()
+        n = ctypes.c_int32()
+        _lib.check(self._lib.fire_autotune(self._h, -1, table, cap, ctypes.byref(n)))
+        return [tuple(table[7 * i:7 * i + 5]) for i in range(min(n.value, cap))]
+
+    @staticmethod
+    def tile_forms() -> int:
+        """Number of tile forms of the split contractions (valid arguments of autotune(int))."""
+        return int(_lib.load().fire_tile_forms())
 
     def precision_plan(self, planes=None):
         """Operand planes per contraction class of the "mixed" precision (ElecInit MLP, update MLP,
