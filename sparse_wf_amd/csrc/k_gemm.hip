@@ -917,7 +917,7 @@ __global__ void __launch_bounds__(BM * BN / 64) k_gemm_bf16_db(const float* __re
 // of the tangent rows (a few hundred columns, K = 256).  A load under a lane condition, or a
 // weight load issued after the A loads it has to be waited with, would force a wait for every
 // outstanding load at each chunk.
-template <int NP, int BM, int BN, int KC, int PA, bool FUSE = false, bool F16 = false>
+template <int NP, int BM, int BN, int KC, int PA, bool FUSE = false, bool F16 = false, bool PERSIST = false>
 __global__ void __launch_bounds__(BM * BN / 64) k_gemm_bf16_dq(const float* __restrict__ A, int64_t lda,
                                                               const uint16_t* __restrict__ Wp0,
                                                               const uint16_t* __restrict__ Wp1,
@@ -940,7 +940,12 @@ __global__ void __launch_bounds__(BM * BN / 64) k_gemm_bf16_dq(const float* __re
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   int64_t mt;
   int nt;
-  tile_of(mt, nt, nT);
+  if constexpr (PERSIST) {
+    mt = blockIdx.x;
+    nt = 0;
+  } else {
+    tile_of(mt, nt, nT);
+  }
   if (dr.offset) {
     const int64_t o = *dr.offset;
     A += o * lda;
@@ -950,34 +955,34 @@ __global__ void __launch_bounds__(BM * BN / 64) k_gemm_bf16_dq(const float* __re
     if (ep.omax) ep.omax += o;
   }
   if (dr.count) M = M < (int64_t)*dr.count ? M : (int64_t)*dr.count;
-  if (mt * BM >= M) return;
-  const int64_t m0 = mt * BM;
-  const int n0 = nt * BN;
+  const int64_t MT = (M + BM - 1) / BM;  // row tiles
+  if (mt >= MT) return;
+  int64_t m0 = mt * BM;
+  int n0 = nt * BN;
   const int wm = (wid / NWN) * 64, wn = (wid % NWN) * 64;
   float16_t acc[2][2];
-  for (int a = 0; a < 2; ++a)
-    for (int b = 0; b < 2; ++b)
-      for (int v = 0; v < 16; ++v) acc[a][b][v] = 0.f;
   const uint16_t* Wpl[3] = {Wp0, Wp1, Wp2};
-  // FP16 planes: the scale of every A row this thread splits (the same rows in every chunk)
-  // the A rows and weight columns this thread loads (the same in every chunk), clamped into
-  // the operands; M >= 1 and N >= 1 here
+  // the A rows (with their FP16 scales) and weight columns this thread loads for the tile at
+  // (m0, n0), the same in every chunk, clamped into the operands; M >= 1 and N >= 1 here
   const float* arow[A_PER];
   float rsc[A_PER];
-#pragma unroll
-  for (int q = 0; q < A_PER; ++q) {
-    const int e = (tid + q * NT) % A_UNITS;
-    const int64_t gr = m0 + e / (KC / 4) < M ? m0 + e / (KC / 4) : M - 1;
-    arow[q] = A + gr * lda + (e % (KC / 4)) * 4;
-    rsc[q] = F16 ? f16_row_scale(ep.amax[gr]) : 1.f;
-  }
   int64_t bcol[B_PER];
+  auto setup = [&]() __attribute__((always_inline)) {
 #pragma unroll
-  for (int q = 0; q < B_PER; ++q) {
-    const int e = (tid + q * NT) % B_UNITS;
-    const int gn = n0 + e / (KC / 8) < N ? n0 + e / (KC / 8) : N - 1;
-    bcol[q] = wpl_off(gn, (e % (KC / 8)) * 8, N);
-  }
+    for (int q = 0; q < A_PER; ++q) {
+      const int e = (tid + q * NT) % A_UNITS;
+      const int64_t gr = m0 + e / (KC / 4) < M ? m0 + e / (KC / 4) : M - 1;
+      arow[q] = A + gr * lda + (e % (KC / 4)) * 4;
+      rsc[q] = F16 ? f16_row_scale(ep.amax[gr]) : 1.f;
+    }
+#pragma unroll
+    for (int q = 0; q < B_PER; ++q) {
+      const int e = (tid + q * NT) % B_UNITS;
+      const int gn = n0 + e / (KC / 8) < N ? n0 + e / (KC / 8) : N - 1;
+      bcol[q] = wpl_off(gn, (e % (KC / 8)) * 8, N);
+    }
+  };
+  setup();
   const int nch = K / KC;
   float4 ra[PA][A_PER];
   u32x4_t rb[PA][NP][B_PER];
@@ -1035,20 +1040,48 @@ __global__ void __launch_bounds__(BM * BN / 64) k_gemm_bf16_dq(const float* __re
   // chunk j lives in ring slot j % PA: chunks 0 .. PA - 1 first, chunk 0 split into stage 0,
   // chunk PA into its slot
   static_for<0, PA>([&](auto J) __attribute__((always_inline)) { load(J.value, ra[J.value], rb[J.value]); });
-  store(0, ra[0], rb[0]);
-  load(PA, ra[0], rb[0]);
-  __syncthreads();
-  for (int c = 0; c < nch; c += PA) {
-    static_for<0, PA>([&](auto Q) __attribute__((always_inline)) {
-      constexpr int slot = (decltype(Q)::value + 1) % PA;
-      const int cc = c + decltype(Q)::value;  // chunk cc in stage cc & 1; chunk cc + 1 in `slot`
-      if (cc < nch) compute(cc & 1);
-      if (cc + 1 < nch) store((cc + 1) & 1, ra[slot], rb[slot]);
-      load(cc + 1 + PA, ra[slot], rb[slot]);
-      __syncthreads();
-    });
+  // PERSIST: the block walks row tiles blockIdx.x, + gridDim.x, ..., all N tiles of each in
+  // turn; the first PA chunks of the next tile are loaded before the current tile's C stores
+  while (true) {
+    for (int a = 0; a < 2; ++a)
+      for (int b = 0; b < 2; ++b)
+        for (int v = 0; v < 16; ++v) acc[a][b][v] = 0.f;
+    store(0, ra[0], rb[0]);
+    load(PA, ra[0], rb[0]);
+    __syncthreads();
+    for (int c = 0; c < nch; c += PA) {
+      static_for<0, PA>([&](auto Q) __attribute__((always_inline)) {
+        constexpr int slot = (decltype(Q)::value + 1) % PA;
+        const int cc = c + decltype(Q)::value;  // chunk cc in stage cc & 1; chunk cc + 1 in `slot`
+        if (cc < nch) compute(cc & 1);
+        if (cc + 1 < nch) store((cc + 1) & 1, ra[slot], rb[slot]);
+        load(cc + 1 + PA, ra[slot], rb[slot]);
+        __syncthreads();
+      });
+    }
+    if constexpr (!PERSIST) {
+      store_tile<FUSE, F16>(acc, m0, n0, wm, wn, lane, C, ldc, M, N, bias, rowflag, period, ep);
+      return;
+    } else {
+      const int64_t m0c = m0;
+      const int n0c = n0;
+      if (nt + 1 < nT) {
+        ++nt;
+      } else {
+        nt = 0;
+        mt += gridDim.x;
+      }
+      const bool more = mt < MT;
+      if (more) {
+        m0 = mt * BM;
+        n0 = nt * BN;
+        setup();
+        static_for<0, PA>([&](auto J) __attribute__((always_inline)) { load(J.value, ra[J.value], rb[J.value]); });
+      }
+      store_tile<FUSE, F16>(acc, m0c, n0c, wm, wn, lane, C, ldc, M, N, bias, rowflag, period, ep);
+      if (!more) return;
+    }
   }
-  store_tile<FUSE, F16>(acc, m0, n0, wm, wn, lane, C, ldc, M, N, bias, rowflag, period, ep);
 }
 
 // Wide-wavefront form of the deep-A kernel: each wavefront computes 64 x 128 (2 x 4 MFMA
@@ -1388,12 +1421,14 @@ struct Bf16Tile {
 // variants 0-6: single LDS stage, K chunk 32; variants 7-10: two LDS stages (kc = 16 or 32);
 // variants 11-14: two LDS stages with loads two chunks ahead; variants 15-17: A in registers;
 // variants 18-22 and 26: two LDS stages with the loads PA chunks ahead; variants 23-25: the same
-// with 64 x 128 wavefront tiles; 27-28: those with one persistent block per CU
+// with 64 x 128 wavefront tiles; 27-28: those with one persistent block per CU; 29-30: 128 x 256
+// two-stage tiles with persistent blocks
 static const Bf16Tile kBf16Tiles[] = {{128, 128}, {128, 256}, {256, 128}, {64, 256}, {256, 64},  {64, 128},
                                       {128, 64},  {128, 128}, {128, 128}, {256, 128}, {128, 256}, {256, 128},
                                       {128, 256}, {128, 128}, {128, 128}, {128, 256}, {256, 128}, {64, 256},
                                       {128, 256}, {128, 256}, {256, 128}, {128, 256}, {128, 128}, {256, 256},
-                                      {128, 256}, {256, 256}, {128, 256}, {256, 256}, {256, 256}};
+                                      {128, 256}, {256, 256}, {128, 256}, {256, 256}, {256, 256}, {128, 256},
+                                      {128, 256}};
 
 // forms that carry FP16 planes (the others are offered to BF16 planes only)
 bool gemm_bf16_f16_ok(int variant) { return variant < 3 || variant > 17; }
@@ -1464,28 +1499,6 @@ static void launch_tile_db(hipStream_t s, const float* A, int64_t lda, const uin
                                                                                ldc, M, N, K, rowflag, period, nT, dr, ep);
 }
 
-template <int NP, int BM, int BN, int KC, int PA, bool FUSE = false, bool F16 = false>
-static void launch_tile_dq(hipStream_t s, const float* A, int64_t lda, const uint16_t* const* W, const float* bias,
-                           float* C, int64_t ldc, int64_t M, int N, int K, const uint8_t* rowflag, int period,
-                           DevRows dr, ActEpi ep) {
-  static bool attr_set = false;
-  const size_t lds = (size_t)2 * NP * (BM + BN) * (KC + 8) * 2;
-  if (lds > kMaxLds) {
-    launch_tile<NP, 128, 128, FUSE, F16>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep);
-    return;
-  }
-  if (!attr_set) {
-    if (lds > 64 * 1024)
-      (void)hipFuncSetAttribute((const void*)k_gemm_bf16_dq<NP, BM, BN, KC, PA, FUSE, F16>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    attr_set = true;
-  }
-  const int nT = (N + BN - 1) / BN;
-  const int64_t blocks = ((M + BM - 1) / BM) * nT;
-  k_gemm_bf16_dq<NP, BM, BN, KC, PA, FUSE, F16><<<(unsigned)blocks, BM * BN / 64, lds, s>>>(A, lda, W[0], W[1], W[2], bias, C,
-                                                                               ldc, M, N, K, rowflag, period, nT, dr, ep);
-}
-
 // compute units of the current device (cached per device)
 static int device_cus() {
   static int cus[64] = {0};
@@ -1498,6 +1511,38 @@ static int device_cus() {
     cus[dev] = n;
   }
   return cus[dev];
+}
+
+// PERSIST: per CU as many blocks as its registers and LDS hold, each walking row tiles
+template <int NP, int BM, int BN, int KC, int PA, bool FUSE = false, bool F16 = false, bool PERSIST = false>
+static void launch_tile_dq(hipStream_t s, const float* A, int64_t lda, const uint16_t* const* W, const float* bias,
+                           float* C, int64_t ldc, int64_t M, int N, int K, const uint8_t* rowflag, int period,
+                           DevRows dr, ActEpi ep) {
+  static bool attr_set = false;
+  const size_t lds = (size_t)2 * NP * (BM + BN) * (KC + 8) * 2;
+  if (lds > kMaxLds) {
+    launch_tile<NP, 128, 128, FUSE, F16>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep);
+    return;
+  }
+  if (!attr_set) {
+    if (lds > 64 * 1024)
+      (void)hipFuncSetAttribute((const void*)k_gemm_bf16_dq<NP, BM, BN, KC, PA, FUSE, F16, PERSIST>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    attr_set = true;
+  }
+  const int nT = (N + BN - 1) / BN;
+  const int64_t mtiles = (M + BM - 1) / BM;
+  static int per_cu = 0;  // resident blocks per CU (registers and LDS), for the persistent grid
+  if (PERSIST && per_cu == 0) {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, (const void*)k_gemm_bf16_dq<NP, BM, BN, KC, PA, FUSE, F16, PERSIST>,
+                                                     BM * BN / 64, lds) != hipSuccess || n <= 0)
+      n = 1;
+    per_cu = n;
+  }
+  const int64_t blocks = PERSIST ? std::min<int64_t>(mtiles, (int64_t)per_cu * device_cus()) : mtiles * nT;
+  k_gemm_bf16_dq<NP, BM, BN, KC, PA, FUSE, F16, PERSIST><<<(unsigned)blocks, BM * BN / 64, lds, s>>>(
+      A, lda, W[0], W[1], W[2], bias, C, ldc, M, N, K, rowflag, period, nT, dr, ep);
 }
 
 // PERSIST: one block per CU (the form's LDS and registers admit one) walking the row tiles
@@ -1548,6 +1593,8 @@ static void launch_np(hipStream_t s, int variant, const float* A, int64_t lda, c
       case 26: launch_tile_dq<NP, 128, 256, 16, 1, true, F16>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
       case 27: launch_tile_dw<NP, 256, 256, 16, 4, true, F16, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
       case 28: launch_tile_dw<NP, 256, 256, 16, 2, true, F16, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 29: launch_tile_dq<NP, 128, 256, 16, 1, true, F16, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 30: launch_tile_dq<NP, 128, 256, 16, 2, true, F16, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
       default: launch_tile<NP, 128, 128, true, F16>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
     }
     return;
@@ -1565,6 +1612,8 @@ static void launch_np(hipStream_t s, int variant, const float* A, int64_t lda, c
       case 26: launch_tile_dq<NP, 128, 256, 16, 1, false, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
       case 27: launch_tile_dw<NP, 256, 256, 16, 4, false, true, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
       case 28: launch_tile_dw<NP, 256, 256, 16, 2, false, true, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 29: launch_tile_dq<NP, 128, 256, 16, 1, false, true, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 30: launch_tile_dq<NP, 128, 256, 16, 2, false, true, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
       case 1: launch_tile<NP, 128, 256, false, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
       case 2: launch_tile<NP, 256, 128, false, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
       default: launch_tile<NP, 128, 128, false, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
@@ -1594,6 +1643,8 @@ static void launch_np(hipStream_t s, int variant, const float* A, int64_t lda, c
       case 26: launch_tile_dq<NP, 128, 256, 16, 1>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
       case 27: launch_tile_dw<NP, 256, 256, 16, 4, false, false, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
       case 28: launch_tile_dw<NP, 256, 256, 16, 2, false, false, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 29: launch_tile_dq<NP, 128, 256, 16, 1, false, false, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
+      case 30: launch_tile_dq<NP, 128, 256, 16, 2, false, false, true>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
       case 1: launch_tile<NP, 128, 256>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
       case 2: launch_tile<NP, 256, 128>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
       case 3: launch_tile<NP, 64, 256>(s, A, lda, W, bias, C, ldc, M, N, K, rowflag, period, dr, ep); break;
