@@ -234,7 +234,8 @@ struct fire_handle {
   // call of a shape times every candidate on the real operands and keeps the fastest.
   bool autotune = true;
   bool autotune_near = false;
-  int force_tile = -1;  // >= 0: every contraction uses this tile form where it applies (fire_autotune)  // reuse tuned tiles across neighbouring row buckets
+  int force_tile = -1;  // >= 0: every contraction uses this tile form where it applies (fire_autotune)
+  int n_cu = 0;          // compute units of the device (read once)  // reuse tuned tiles across neighbouring row buckets
   // silu of the tangent-row MLP layers fused into the contraction epilogue (FIRE_FUSED_ACT=0:
   // the separate element-wise pass instead)
   bool fuse_act = true;
@@ -516,8 +517,19 @@ void gemm(Ctx& c, const float* A, int64_t lda, int wi, int bi, float* C, int64_t
       hipEvent_t t0, t1;
       HIPCHK(hipEventCreate(&t0));
       HIPCHK(hipEventCreate(&t1));
+      if (h->n_cu == 0) {
+        int n = 0;
+        h->n_cu = hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, h->device) == hipSuccess && n > 0
+                      ? n
+                      : 256;
+      }
       for (int v = 0; v < nv; ++v) {
-        if ((ep.rowgrp && !gemm_bf16_fusable(v)) || (ns == 4 && !gemm_bf16_f16_ok(v)) || gemm_bf16_spills(v, ns)) {
+        int bm = 128, bn = 128;
+        gemm_bf16_tile(v, &bm, &bn);
+        // a persistent form pays off only with several row tiles per CU
+        const bool few_tiles = gemm_bf16_persistent(v) && M < (int64_t)2 * bm * h->n_cu;
+        if ((ep.rowgrp && !gemm_bf16_fusable(v)) || (ns == 4 && !gemm_bf16_f16_ok(v)) || gemm_bf16_spills(v, ns) ||
+            few_tiles) {
           ms[v] = 1e30f;
           continue;
         }

@@ -132,6 +132,7 @@ bool gemm_bf16_fusable(int variant);
 // variants that have an FP16-plane form (nsplit 4)
 bool gemm_bf16_f16_ok(int variant);
 bool gemm_bf16_spills(int variant, int nsplit);
+bool gemm_bf16_persistent(int variant);
 void gemm_bf16_tile(int variant, int* bm, int* bn);
 
 // pseudopotential tables on the device (radial grids in FP64)

@@ -1447,6 +1447,9 @@ bool gemm_bf16_fusable(int variant) {
 }
 
 int gemm_bf16_variants() { return (int)(sizeof(kBf16Tiles) / sizeof(kBf16Tiles[0])); }
+
+// forms whose blocks walk several row tiles (one block per CU or per resident slot)
+bool gemm_bf16_persistent(int variant) { return variant >= 27 && variant <= 30; }
 void gemm_bf16_tile(int variant, int* bm, int* bn) {
   const Bf16Tile t = kBf16Tiles[(variant >= 0 && variant < gemm_bf16_variants()) ? variant : 0];
   *bm = t.bm;
