@@ -17,6 +17,7 @@
 //               hi + mid + lo (x to ~2^-26) and accumulates the six products whose order is
 //               at most 2^-16 relative (x6, FP32-level products); NSPLIT = 1 is plain BF16.
 #include <algorithm>
+#include <atomic>
 #include <type_traits>
 
 #include "fire_common.hpp"
@@ -1502,18 +1503,19 @@ static void launch_tile_db(hipStream_t s, const float* A, int64_t lda, const uin
                                                                                ldc, M, N, K, rowflag, period, nT, dr, ep);
 }
 
-// compute units of the current device (cached per device)
+// compute units of the current device (cached per device; launches come from several host
+// threads, which all store the same value)
 static int device_cus() {
-  static int cus[64] = {0};
+  static std::atomic<int> cus[64];
   int dev = 0;
   (void)hipGetDevice(&dev);
   if (dev < 0 || dev >= 64) dev = 0;
-  if (cus[dev] == 0) {
-    int n = 0;
+  int n = cus[dev].load(std::memory_order_relaxed);
+  if (n == 0) {
     if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
-    cus[dev] = n;
+    cus[dev].store(n, std::memory_order_relaxed);
   }
-  return cus[dev];
+  return n;
 }
 
 // PERSIST: per CU as many blocks as its registers and LDS hold, each walking row tiles
@@ -1535,13 +1537,13 @@ static void launch_tile_dq(hipStream_t s, const float* A, int64_t lda, const uin
   }
   const int nT = (N + BN - 1) / BN;
   const int64_t mtiles = (M + BM - 1) / BM;
-  static int per_cu = 0;  // resident blocks per CU (registers and LDS), for the persistent grid
+  static std::atomic<int> resident{0};  // blocks per CU (registers and LDS), for the persistent grid
+  int per_cu = resident.load(std::memory_order_relaxed);
   if (PERSIST && per_cu == 0) {
-    int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, (const void*)k_gemm_bf16_dq<NP, BM, BN, KC, PA, FUSE, F16, PERSIST>,
-                                                     BM * BN / 64, lds) != hipSuccess || n <= 0)
-      n = 1;
-    per_cu = n;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_gemm_bf16_dq<NP, BM, BN, KC, PA, FUSE, F16, PERSIST>,
+                                                     BM * BN / 64, lds) != hipSuccess || per_cu <= 0)
+      per_cu = 1;
+    resident.store(per_cu, std::memory_order_relaxed);
   }
   const int64_t blocks = PERSIST ? std::min<int64_t>(mtiles, (int64_t)per_cu * device_cus()) : mtiles * nT;
   k_gemm_bf16_dq<NP, BM, BN, KC, PA, FUSE, F16, PERSIST><<<(unsigned)blocks, BM * BN / 64, lds, s>>>(
