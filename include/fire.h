@@ -194,8 +194,9 @@ int fire_state_stats(fire_handle* h, int32_t* n_walkers, int64_t* n_proposals, i
  * Q/V, attention readout.  3: three BF16 planes, six products accumulated down to 2^-16
  * relative (FP32-level); 2: a BF16 hi + lo pair, three products; 1: plain BF16; 4: two FP16
  * planes of operands scaled by powers of two (each row of A by its maximum, which the kernel
- * producing A leaves, each column of W at upload), three products, about 2^-22 relative per
- * product; where no row maxima exist (value path, sampler) class 4 runs as 3.  Default
+ * producing A leaves or a row-maximum pass takes, each column of W at upload), three products,
+ * about 2^-22 relative per product; the value path and the sampler's proposals take the row
+ * maxima from the rows themselves, so a moved electron's rows match a fresh evaluation.  Default
  * {4,4,4,2,2,2}.  Sets the first n classes from planes (n may be 0); current (optional,
  * int32 [6]) receives the plan. */
 int fire_precision_plan(fire_handle* h, const int32_t* planes, int32_t n, int32_t* current);

@@ -285,9 +285,9 @@ struct fire_handle {
     // proposal: positions, moved electron, uniform variate, affected sets and row lists
     DevBuf p_r, p_rk, p_u, aff, cu, cd, offu, offd, tail, list, rowof, klist;
     // moved-electron rows (ElecInit, messages, envelope)
-    DevBuf kT5, kZ2, kT5b, p_M5k, kcoef, p_envk;
+    DevBuf kT5, kZ2, kT5b, p_M5k, kcoef, p_envk, k_amx;
     // compact rows of the affected electrons
-    DevBuf pcnt, pptr, pnbr, p_hc, p_x, p_h, p_y, p_orb, p_jr, p_qv, p_av;
+    DevBuf pcnt, pptr, pnbr, p_hc, p_x, p_h, p_y, p_orb, p_jr, p_qv, p_av, p_amx;
     // determinant and Jastrow of the proposal
     DevBuf phin, sbuf, ldS, sgS, ccusp, cdelta, jsum, rin, ry1, ry2, att_out, cval, clap, cgrad, epot;
     DevBuf p_ld, p_sg, p_logpsi, p_sign, accept, wbscr;
@@ -306,7 +306,8 @@ struct fire_handle {
               &list,  &rowof, &klist, &kT5,     &kZ2,   &kT5b,   &p_M5k, &kcoef, &p_envk, &pcnt,  &pptr,
               &pnbr,  &p_hc,  &p_x,   &p_h,     &p_y,   &p_orb,  &p_jr,  &p_qv,  &p_av,   &phin,  &sbuf, &ccusp, &cdelta,
               &ldS,   &sgS,   &jsum,  &rin,     &ry1,   &ry2,    &att_out, &cval, &clap,  &cgrad, &epot,
-              &p_ld,  &p_sg,  &p_logpsi, &p_sign, &accept, &wbscr, &dstats, &doverflow, &dstep, &nacc};
+              &p_ld,  &p_sg,  &p_logpsi, &p_sign, &accept, &wbscr, &dstats, &doverflow, &dstep, &nacc,
+              &k_amx, &p_amx};
     }
     void release() {
       DevBuf* b[] = {&r,     &M5,    &hh,    &orb,     &env,   &A,      &Ainv,  &ld,    &sg,     &jr,    &av,
@@ -314,7 +315,8 @@ struct fire_handle {
                      &list,  &rowof, &klist, &kT5,     &kZ2,   &kT5b,   &p_M5k, &kcoef, &p_envk, &pcnt,  &pptr,
                      &pnbr,  &p_hc,  &p_x,   &p_h,     &p_y,   &p_orb,  &p_jr,  &p_qv,  &p_av,   &phin,  &sbuf, &ccusp, &cdelta,
                      &ldS,   &sgS,   &jsum,  &rin,     &ry1,   &ry2,    &att_out, &cval, &clap,  &cgrad, &epot,
-                     &p_ld,  &p_sg,  &p_logpsi, &p_sign, &accept, &wbscr, &dstats, &doverflow, &dstep, &nacc};
+                     &p_ld,  &p_sg,  &p_logpsi, &p_sign, &accept, &wbscr, &dstats, &doverflow, &dstep, &nacc,
+              &k_amx, &p_amx};
       if (graph) (void)hipGraphExecDestroy(graph);
       graph = nullptr;
       if (gstream) (void)hipStreamDestroy(gstream);
@@ -444,7 +446,7 @@ void gemm(Ctx& c, const float* A, int64_t lda, int wi, int bi, float* C, int64_t
   if (c.prec == FIRE_MIXED) {
     int np = h->planes[gemm_class(h, wi)];
     if (wi == h->iJV) np = (np == 4 || h->planes[4] == 4) ? 4 : std::max(np, h->planes[4]);
-    if (np == 4 && !(ep.amax && h->Pf16[wi])) np = 3;  // no row maxima of A: three BF16 planes
+    if (np == 4 && !(ep.amax && h->Pf16[wi])) np = 3;  // a caller without row maxima of A: three BF16 planes
     ns = np == 4 ? 4 : (np == 3 ? 6 : (np == 2 ? 3 : 1));
   }
   const uint16_t* W0 = h->Pbf[wi];
@@ -727,10 +729,11 @@ void run_embedding(Ctx& c, const float* r, int B, const Graph& gr, bool tan, Gro
   launch_elec_init_en(c.s, r, h->R, B, N, h->n_up, h->M, h->cfg.cutoff_1el, F, en_params(h), tan, h->T5.as<float>(),
                       nullptr, 0, ef, eok);
   const int64_t rows5 = (int64_t)G * NR;
-  // FP16 planes for the ElecInit MLP (tangent rows, class 0 set to 4): row maxima of each input
+  // FP16 planes for the ElecInit MLP (class 0 set to 4): row maxima of each input, taken from
+  // the rows themselves (so the sampler's rows of a moved electron get the same scales)
   ActEpi e0;
   const ActEpi* pe0 = nullptr;
-  if (tan && c.prec == FIRE_MIXED && h->planes[0] == 4) {
+  if (c.prec == FIRE_MIXED && h->planes[0] == 4) {
     h->AMX_E.ensure(sizeof(uint32_t) * (size_t)rows5);
     e0.amax = h->AMX_E.as<uint32_t>();
     pe0 = &e0;
@@ -788,10 +791,29 @@ void run_embedding(Ctx& c, const float* r, int B, const Graph& gr, bool tan, Gro
                               -1, 0, amx_b, &h->AMX_A);
     return;
   }
-  gemm(c, h->XA.as<float>(), F, m + 0, m + 1, h->XB.as<float>(), F, R, F, F, flag, period);
+  // value rows: FP16 planes (classes 1 and 2 set to 4) with the row maxima of each input
+  ActEpi em;
+  const bool f16m = c.prec == FIRE_MIXED && h->planes[1] == 4;
+  if (f16m || (c.prec == FIRE_MIXED && h->planes[2] == 4)) {
+    h->AMX_A.ensure(sizeof(uint32_t) * (size_t)R);
+    h->AMX_B.ensure(sizeof(uint32_t) * (size_t)R);
+  }
+  if (f16m) {
+    launch_rowmax(c.s, h->XA.as<float>(), F, R, F, h->AMX_A.as<uint32_t>());
+    em.amax = h->AMX_A.as<uint32_t>();
+  }
+  gemm(c, h->XA.as<float>(), F, m + 0, m + 1, h->XB.as<float>(), F, R, F, F, flag, period, DevRows(), &em);
   launch_act(c.s, h->XB.as<float>(), F, F, G, gd, ACT_SILU, 0.f);
-  gemm(c, h->XB.as<float>(), F, m + 2, m + 3, h->XA.as<float>(), F, R, F, F, flag, period);
+  if (f16m) {
+    launch_rowmax(c.s, h->XB.as<float>(), F, R, F, h->AMX_B.as<uint32_t>());
+    em.amax = h->AMX_B.as<uint32_t>();
+  }
+  gemm(c, h->XB.as<float>(), F, m + 2, m + 3, h->XA.as<float>(), F, R, F, F, flag, period, DevRows(), &em);
   launch_act(c.s, h->XA.as<float>(), F, F, G, gd, ACT_SILU, 0.f);
+  if (c.prec == FIRE_MIXED && h->planes[2] == 4) {  // the orbital contraction's row maxima
+    launch_rowmax(c.s, h->XA.as<float>(), F, R, F, h->AMX_A.as<uint32_t>());
+    h->amx_xa = h->AMX_A.as<uint32_t>();
+  }
 }
 
 struct DetOut {
@@ -839,7 +861,7 @@ void run_orbitals_det(Ctx& c, const float* r, int B, const Graph& gr, bool tan, 
   float* Ob = h->O.as<float>();
   const int wup = aug ? h->iORBA : o, wdn = aug ? h->iORBA + 2 : o + 2;
   ActEpi eu, ed;  // row maxima of the embedding rows for FP16 planes (tangent rows)
-  eu.amax = tan ? h->amx_xa : nullptr;
+  eu.amax = h->amx_xa;  // null when no FP16 planes are wanted
   ed.amax = eu.amax ? eu.amax + R_up_rows : nullptr;
   gemm(c, h->XA.as<float>(), F, wup, wup + 1, Ob, ldo, R_up_rows, NDA, F, flag, period, DevRows(), &eu);
   if (R > R_up_rows)
@@ -1144,6 +1166,8 @@ void state_init(Ctx& c, const float* r_dev, int B) {
   S.pptr.ensure(sizeof(int64_t) * (size_t)(T + 2));
   S.pnbr.ensure(sizeof(int32_t) * (size_t)T * std::max(N - 1, 1));
   S.p_hc.ensure(sizeof(float) * (size_t)T * F);
+  S.p_amx.ensure(sizeof(uint32_t) * (size_t)T);
+  S.k_amx.ensure(sizeof(uint32_t) * (size_t)B);
   S.p_x.ensure(sizeof(float) * (size_t)T * XW);
   S.p_y.ensure(sizeof(float) * (size_t)T * XW);
   S.p_h.ensure(sizeof(float) * (size_t)T * F);
@@ -1229,12 +1253,22 @@ void state_propose(Ctx& c, int k, int mode, const float* rk_in, float width, uin
   launch_mc_klist(c.s, B, k, S.klist.as<int2>());
   launch_elec_init_en(c.s, S.p_r.as<float>(), h->R, B, N, nu, h->M, h->cfg.cutoff_1el, F, en_params(h), false,
                       S.kT5.as<float>(), S.klist.as<int2>(), B);
-  gemm(cc, S.kT5.as<float>(), F, b + 11, -1, S.kZ2.as<float>(), 2 * F, B, 2 * F, F, nullptr, 0);
+  // FP16 planes of the classes set to 4 take the row maxima of their inputs, as the value path
+  // does, so the moved electron's rows match a fresh evaluation
+  const bool f16e = c.prec == FIRE_MIXED && h->planes[0] == 4;
+  const bool f16m = c.prec == FIRE_MIXED && h->planes[1] == 4;
+  const bool f16o = c.prec == FIRE_MIXED && h->planes[2] == 4;
+  ActEpi ek;
+  if (f16e) ek.amax = S.k_amx.as<uint32_t>();
+  if (f16e) launch_rowmax(c.s, S.kT5.as<float>(), F, B, F, S.k_amx.as<uint32_t>());
+  gemm(cc, S.kT5.as<float>(), F, b + 11, -1, S.kZ2.as<float>(), 2 * F, B, 2 * F, F, nullptr, 0, DevRows(), &ek);
   launch_glu(c.s, S.kZ2.as<float>(), B, F, false, S.kT5b.as<float>());
-  gemm(cc, S.kT5b.as<float>(), F, b + 12, b + 13, S.kT5.as<float>(), F, B, F, F, nullptr, 1);
+  if (f16e) launch_rowmax(c.s, S.kT5b.as<float>(), F, B, F, S.k_amx.as<uint32_t>());
+  gemm(cc, S.kT5b.as<float>(), F, b + 12, b + 13, S.kT5.as<float>(), F, B, F, F, nullptr, 1, DevRows(), &ek);
   const GroupDesc g1{1, 0, nullptr};
   launch_act(c.s, S.kT5.as<float>(), F, F, B, g1, ACT_SILU, 0.f);
-  gemm(cc, S.kT5.as<float>(), F, b + 14, b + 15, S.p_M5k.as<float>(), 5 * F, B, 5 * F, F, nullptr, 1);
+  if (f16e) launch_rowmax(c.s, S.kT5.as<float>(), F, B, F, S.k_amx.as<uint32_t>());
+  gemm(cc, S.kT5.as<float>(), F, b + 14, b + 15, S.p_M5k.as<float>(), 5 * F, B, 5 * F, F, nullptr, 1, DevRows(), &ek);
 
   // neighbour lists of the affected centres at the new positions, messages, update MLP
   const DevRows rT{dT, nullptr};
@@ -1247,15 +1281,26 @@ void state_propose(Ctx& c, int k, int mode, const float* rk_in, float width, uin
                   edge_params(h, h->iEEs), edge_params(h, h->iEEd), h->cut_learned, F, false, S.p_hc.as<float>(),
                   S.list.as<int2>(), (int)T, k, S.p_M5k.as<float>(), dT);
   const int m = h->iMLP;
-  gemm(cc, S.p_hc.as<float>(), F, m + 0, m + 1, S.p_x.as<float>(), F, T, F, F, nullptr, 1, rT);
+  ActEpi et;
+  if (f16m) et.amax = S.p_amx.as<uint32_t>();
+  if (f16m) launch_rowmax(c.s, S.p_hc.as<float>(), F, T, F, S.p_amx.as<uint32_t>(), dT);
+  gemm(cc, S.p_hc.as<float>(), F, m + 0, m + 1, S.p_x.as<float>(), F, T, F, F, nullptr, 1, rT, &et);
   launch_act(c.s, S.p_x.as<float>(), F, F, (int)T, g1, ACT_SILU, 0.f, dT);
-  gemm(cc, S.p_x.as<float>(), F, m + 2, m + 3, S.p_h.as<float>(), F, T, F, F, nullptr, 1, rT);
+  if (f16m) launch_rowmax(c.s, S.p_x.as<float>(), F, T, F, S.p_amx.as<uint32_t>(), dT);
+  gemm(cc, S.p_x.as<float>(), F, m + 2, m + 3, S.p_h.as<float>(), F, T, F, F, nullptr, 1, rT, &et);
   launch_act(c.s, S.p_h.as<float>(), F, F, (int)T, g1, ACT_SILU, 0.f, dT);
 
   // orbital rows of the affected electrons (up-spin rows first), envelope row of electron k
   const int o = h->iORB;
-  gemm(cc, S.p_h.as<float>(), F, o + 0, o + 1, S.p_orb.as<float>(), ND, T, ND, F, nullptr, 1, DevRows{dT_up, nullptr});
-  gemm(cc, S.p_h.as<float>(), F, o + 2, o + 3, S.p_orb.as<float>(), ND, T, ND, F, nullptr, 1, DevRows{dT_dn, dT_up});
+  ActEpi eo;
+  if (f16o) {
+    launch_rowmax(c.s, S.p_h.as<float>(), F, T, F, S.p_amx.as<uint32_t>(), dT);
+    eo.amax = S.p_amx.as<uint32_t>();  // the down-spin contraction offsets it with its rows
+  }
+  gemm(cc, S.p_h.as<float>(), F, o + 0, o + 1, S.p_orb.as<float>(), ND, T, ND, F, nullptr, 1, DevRows{dT_up, nullptr},
+       &eo);
+  gemm(cc, S.p_h.as<float>(), F, o + 2, o + 3, S.p_orb.as<float>(), ND, T, ND, F, nullptr, 1, DevRows{dT_dn, dT_up},
+       &eo);
   const int ME = h->M * h->Eiso;
   launch_env_coef(c.s, S.p_r.as<float>(), h->R, B, N, nu, h->M, D, h->Eiso, h->P[o + 4], h->P[o + 6], false,
                   S.kcoef.as<float>(), S.klist.as<int2>(), B);

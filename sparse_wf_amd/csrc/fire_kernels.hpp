@@ -51,7 +51,9 @@ void launch_elec_init_en(hipStream_t s, const float* r, const float* R, int B, i
                          float* ef = nullptr, uint8_t* eok = nullptr);
 void launch_glu(hipStream_t s, const float* Z, int G, int F, bool tangents, float* out);
 // dG (optional, device): groups actually in use (G is the launch capacity)
-void launch_rowmax(hipStream_t s, const float* A, int64_t lda, int64_t M, int K, uint32_t* amax);
+// dcount (optional, device): only the first *dcount of the M rows
+void launch_rowmax(hipStream_t s, const float* A, int64_t lda, int64_t M, int K, uint32_t* amax,
+                   const int32_t* dcount = nullptr);
 void launch_act(hipStream_t s, float* Y, int64_t ld, int ncols, int G, GroupDesc gd, int act, float act_scale,
                 const int32_t* dG = nullptr);
 // list mode: centres list[0..n_list) with CSR ptr/nbr over list positions; the message

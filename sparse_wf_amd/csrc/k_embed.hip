@@ -312,10 +312,10 @@ __global__ void k_act(float* __restrict__ Y, int64_t ld, int ncols, GroupDesc gd
 // amax[row] = bits of max_k |A[row][k]| (the row maxima the FP16-plane contractions scale by);
 // one wavefront per row, float4 loads (K % 4 == 0, lda % 4 == 0).
 __global__ void __launch_bounds__(256) k_rowmax(const float* __restrict__ A, int64_t lda, int64_t M, int K,
-                                                uint32_t* __restrict__ amax) {
+                                                uint32_t* __restrict__ amax, const int32_t* __restrict__ dcount) {
   const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  if (row >= M) return;
+  if (row >= M || (dcount && row >= *dcount)) return;
   float m = 0.f;
   for (int k = 4 * lane; k < K; k += 256) {
     const float4 v = *reinterpret_cast<const float4*>(A + row * lda + k);
@@ -326,8 +326,9 @@ __global__ void __launch_bounds__(256) k_rowmax(const float* __restrict__ A, int
   if (lane == 0) amax[row] = __float_as_uint(m);
 }
 
-void launch_rowmax(hipStream_t s, const float* A, int64_t lda, int64_t M, int K, uint32_t* amax) {
-  if (M) k_rowmax<<<(unsigned)((M + 3) / 4), 256, 0, s>>>(A, lda, M, K, amax);
+void launch_rowmax(hipStream_t s, const float* A, int64_t lda, int64_t M, int K, uint32_t* amax,
+                   const int32_t* dcount) {
+  if (M) k_rowmax<<<(unsigned)((M + 3) / 4), 256, 0, s>>>(A, lda, M, K, amax, dcount);
 }
 
 void launch_act(hipStream_t s, float* Y, int64_t ld, int ncols, int G, GroupDesc gd, int act, float act_scale,

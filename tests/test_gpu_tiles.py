@@ -8,7 +8,7 @@ the results agree bit for bit (the bounds below leave room for rounding); the ha
 table shows that each form did run.
 F = W = 256 gives K = 256 contractions with N = 256, 272 (joint Jastrow layer with attention
 values), 512 and the fused activation epilogues; the value path (``signed``) runs the
-contractions of the value rows, three BF16 planes where FP16 planes need row maxima.
+contractions of the value rows (their FP16 planes with row maxima taken from the rows).
 """
 from __future__ import annotations
 
